@@ -1,0 +1,268 @@
+"""ctypes access to the oracle -- TEST INFRASTRUCTURE ONLY.
+
+liboracle.so is the FP64 C restatement of PSI.voxels (psi_oracle.c);
+_ref/libpsiref.so, when it was built, is the reference's own C core compiled
+from its sources (Makefile target `ref`).  Only tests/, the smoke check and the
+CPU-baseline leg of bench.py import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REFLIB = os.path.join(HERE, "_ref", "libpsiref.so")
+
+FIELD_WIDTH = {"m": 1, "x": 3, "v": 3, "xx": 9, "xv": 9, "vv": 9}
+FIELD_ORDER = ["m", "x", "v", "xx", "xv", "vv"]
+
+
+def build(ref: bool = True) -> None:
+    targets = ["liboracle.so"] + (["ref"] if ref else [])
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+class _OMesh(C.Structure):
+    _fields_ = [("elemtype", C.c_int), ("npart", C.c_int64), ("nelem", C.c_int64),
+                ("pos", C.c_void_p), ("vel", C.c_void_p), ("mass", C.c_void_p), ("conn", C.c_void_p),
+                ("periodic", C.c_int), ("box", (C.c_double * 3) * 2)]
+
+
+class _OGrid(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("win", (C.c_double * 3) * 2), ("d", C.c_double * 3),
+                ("f", C.c_void_p * 6)]
+
+
+class OStats(C.Structure):
+    _fields_ = [("nfrag", C.c_int64), ("ntets", C.c_int64), ("nghosts", C.c_int64), ("nelem_used", C.c_int64),
+                ("maxverts", C.c_int), ("maxstack", C.c_int), ("overflow", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build(ref=False)
+        L = C.CDLL(LIB)
+        P = C.c_void_p
+        L.orc_voxels.argtypes = [P, P, C.c_double, C.c_int, C.c_int, P]
+        L.orc_voxels_range.argtypes = [P, P, C.c_double, C.c_int, C.c_int, C.c_int64, C.c_int64, P]
+        L.orc_voxels_omp.argtypes = [P, P, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, P]
+        L.orc_list_fragments.argtypes = [P, P, C.c_double, C.c_int, C.c_int64, C.c_int64, P, P, C.c_int64]
+        L.orc_list_fragments.restype = C.c_int64
+        L.orc_count_tets.argtypes = [P, P, C.c_double, C.c_int, P]
+        L.orc_voxelize_one_tet.argtypes = [P, P, P, P, C.c_double, C.c_int, P]
+        _lib = L
+    return _lib
+
+
+def block_connectivity(shape, periodic: bool) -> np.ndarray:
+    """Trilinear connectivity of a (nx,ny,nz) particle block, corner order
+    4i+2j+k (src/PSI.c:636-656)."""
+    nx, ny, nz = shape
+    ex, ey, ez = (nx, ny, nz) if periodic else (nx - 1, ny - 1, nz - 1)
+    i, j, k = np.meshgrid(np.arange(ex), np.arange(ey), np.arange(ez), indexing="ij")
+    conn = np.empty((ex, ey, ez, 8), dtype=np.int32)
+    for ii in range(2):
+        for jj in range(2):
+            for kk in range(2):
+                conn[..., 4 * ii + 2 * jj + kk] = (ny * nz * ((i + ii) % nx) + nz * ((j + jj) % ny)
+                                                   + ((k + kk) % nz))
+    return conn.reshape(-1, 8)
+
+
+class Problem:
+    """A mesh + grid description shared by the oracle and the reference build."""
+
+    def __init__(self, pos, vel, mass, conn, periodic_box=None, n=(32, 32, 32),
+                 bounds=((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)), fields=()):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        self.vel = np.ascontiguousarray(vel if vel is not None else np.zeros_like(self.pos),
+                                        dtype=np.float64).reshape(-1, 3)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32)
+        self.elemtype = self.conn.shape[1]
+        nelem = self.conn.shape[0]
+        self.mass = np.ascontiguousarray(mass if mass is not None else np.ones(nelem),
+                                         dtype=np.float64).reshape(-1)
+        self.periodic = periodic_box is not None
+        self.box = np.array(periodic_box if periodic_box is not None else ((0, 0, 0), (0, 0, 0)), dtype=np.float64)
+        self.n = tuple(int(v) for v in n)
+        self.bounds = np.array(bounds, dtype=np.float64)
+        self.d = tuple(float((self.bounds[1, a] - self.bounds[0, a]) / self.n[a]) for a in range(3))
+        self.fields = ["m"] + [f for f in fields if f != "m"]
+
+    @classmethod
+    def block(cls, pos, vel=None, mass=None, periodic_box=None, **kw):
+        shape = pos.shape[:3]
+        conn = block_connectivity(shape, periodic_box is not None)
+        if mass is not None:
+            mass = np.asarray(mass).reshape(-1)
+        return cls(pos.reshape(-1, 3), None if vel is None else vel.reshape(-1, 3), mass, conn,
+                   periodic_box=periodic_box, **kw)
+
+    def zero_fields(self):
+        out = {}
+        for f in self.fields:
+            w = FIELD_WIDTH[f]
+            shp = self.n if w == 1 else self.n + ((3,) if w == 3 else (3, 3))
+            out[f] = np.zeros(shp, dtype=np.float64)
+        return out
+
+    def _omesh(self):
+        m = _OMesh()
+        m.elemtype = self.elemtype
+        m.npart = self.pos.shape[0]
+        m.nelem = self.conn.shape[0]
+        m.pos = self.pos.ctypes.data
+        m.vel = self.vel.ctypes.data
+        m.mass = self.mass.ctypes.data
+        m.conn = self.conn.ctypes.data
+        m.periodic = int(self.periodic)
+        for s in range(2):
+            for a in range(3):
+                m.box[s][a] = self.box[s, a]
+        return m
+
+    def _ogrid(self, out):
+        g = _OGrid()
+        for a in range(3):
+            g.n[a] = self.n[a]
+            g.win[0][a] = self.bounds[0, a]
+            g.win[1][a] = self.bounds[1, a]
+            g.d[a] = self.d[a]
+        for i, f in enumerate(FIELD_ORDER):
+            g.f[i] = out[f].ctypes.data if f in out else None
+        return g
+
+
+def voxels(prob: Problem, reftol=1.0, maxlvl=0, xvfix=False, out=None, threads=0, e0=0, e1=None):
+    """Run the oracle.  threads=0: serial; threads>0 or -1: OpenMP."""
+    L = lib()
+    out = prob.zero_fields() if out is None else out
+    m, g, st = prob._omesh(), prob._ogrid(out), OStats()
+    e1 = prob.conn.shape[0] if e1 is None else e1
+    if threads:
+        rc = L.orc_voxels_omp(C.byref(m), C.byref(g), reftol, maxlvl, int(xvfix), int(threads), e0, e1,
+                              C.byref(st))
+    else:
+        rc = L.orc_voxels_range(C.byref(m), C.byref(g), reftol, maxlvl, int(xvfix), e0, e1, C.byref(st))
+    if rc != 0:
+        raise RuntimeError("oracle failed")
+    return out, st.as_dict()
+
+
+def list_fragments(prob: Problem, reftol=1.0, maxlvl=0, e0=0, e1=None, cap=1 << 22):
+    L = lib()
+    m, g = prob._omesh(), prob._ogrid(prob.zero_fields())
+    e1 = prob.conn.shape[0] if e1 is None else e1
+    gi = np.empty(cap, dtype=np.int64)
+    mm = np.empty(cap, dtype=np.float64)
+    n = L.orc_list_fragments(C.byref(m), C.byref(g), reftol, maxlvl, e0, e1, gi.ctypes.data, mm.ctypes.data, cap)
+    if n > cap:
+        return list_fragments(prob, reftol, maxlvl, e0, e1, cap=int(n))
+    return gi[:n].copy(), mm[:n].copy()
+
+
+def count_tets(prob: Problem, reftol=1.0, maxlvl=0):
+    L = lib()
+    m, g = prob._omesh(), prob._ogrid(prob.zero_fields())
+    counts = np.zeros(prob.conn.shape[0], dtype=np.int32)
+    L.orc_count_tets(C.byref(m), C.byref(g), reftol, maxlvl, counts.ctypes.data)
+    return counts
+
+
+# ---------------------------------------------------------------------------
+# the reference's own C core (oracle/_ref), driven through its psi_voxels()
+# ---------------------------------------------------------------------------
+
+class _RGrid(C.Structure):  # include/grid.h:46-53
+    _fields_ = [("type", C.c_int), ("dim", C.c_int), ("fields", C.c_void_p * 16),
+                ("window", (C.c_double * 3) * 2), ("n", C.c_int * 3), ("d", C.c_double * 3)]
+
+
+class _RMesh(C.Structure):  # include/mesh.h:42-52
+    _fields_ = [("elemtype", C.c_int), ("dim", C.c_int), ("pos", C.c_void_p), ("vel", C.c_void_p),
+                ("mass", C.c_void_p), ("npart", C.c_int), ("nelem", C.c_int), ("connectivity", C.c_void_p),
+                ("box", (C.c_double * 3) * 2), ("periodic", C.c_int)]
+
+
+_ref = None
+
+
+def have_ref() -> bool:
+    return os.path.exists(REFLIB)
+
+
+def reflib():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REFLIB)
+        L.psi_voxels.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.psi_voxels.restype = None
+        _ref = L
+    return _ref
+
+
+class _silence_stdout:
+    """The reference prints a progress line to stdout; keep it off fd 1."""
+
+    def __enter__(self):
+        import sys
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        self.null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self.null, 1)
+
+    def __exit__(self, *a):
+        C.CDLL(None).fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        os.close(self.null)
+
+
+def ref_voxels(prob: Problem, reftol=1.0, maxlvl=0, out=None, e0=0, e1=None):
+    """Run the reference psi_voxels() (density mode) on the same problem,
+    optionally on the element subset [e0, e1)."""
+    L = reflib()
+    out = prob.zero_fields() if out is None else out
+    conn = prob.conn
+    mass = prob.mass
+    if e0 != 0 or e1 is not None:
+        conn = np.ascontiguousarray(conn[e0:e1])
+        mass = np.ascontiguousarray(mass[e0:e1])
+    m = _RMesh()
+    m.elemtype = prob.elemtype
+    m.dim = 3
+    m.pos = prob.pos.ctypes.data
+    m.vel = prob.vel.ctypes.data
+    m.mass = mass.ctypes.data
+    m.npart = prob.pos.shape[0]
+    m.nelem = conn.shape[0]
+    m.connectivity = conn.ctypes.data
+    m.periodic = int(prob.periodic)
+    for s in range(2):
+        for a in range(3):
+            m.box[s][a] = prob.box[s, a]
+    g = _RGrid()
+    g.type = 0
+    g.dim = 3
+    for i, f in enumerate(FIELD_ORDER):
+        g.fields[i] = out[f].ctypes.data if f in out else None
+    for a in range(3):
+        g.window[0][a] = prob.bounds[0, a]
+        g.window[1][a] = prob.bounds[1, a]
+        g.n[a] = prob.n[a]
+        g.d[a] = prob.d[a]
+    with _silence_stdout():
+        L.psi_voxels(C.byref(g), C.byref(m), None, 0, reftol, maxlvl)
+    return out

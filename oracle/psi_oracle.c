@@ -1,0 +1,1046 @@
+/*
+ * psi_oracle.c -- TEST INFRASTRUCTURE ONLY (the oracle).
+ *
+ * FP64 CPU restatement of PSI's exact conservative voxelization of
+ * phase-space-sheet tetrahedra ("PSI.voxels", mode='density').  It exists to
+ * check the HIP product path and to time a CPU baseline; nothing in the
+ * product (pypsi_amd/) links or calls it.
+ *
+ * Each function names the reference lines it restates (paths relative to the
+ * reference repository root).  Arithmetic expressions on the fragment path are
+ * kept in the reference's evaluation order so that, compiled without FP
+ * contraction, results agree with the reference bit for bit; that is checked
+ * against a build of the reference's own C core (oracle/_ref) in tests/.
+ *
+ * Deliberate differences from the reference (all on inputs where the
+ * reference misbehaves):
+ *   - a leaf whose index box has zero extent on every axis (a point-like tet on
+ *     a grid node) is dropped instead of looping forever (src/geometry.c:865-893);
+ *   - a leaf whose lower index equals n on some axis is dropped instead of
+ *     writing one element past the grid (src/geometry.c:879);
+ *   - polyhedron vertex overflow is counted in stats->overflow instead of
+ *     corrupting memory (src/geometry.c:484-485,671-672).
+ *
+ * Extras the reference does not have: fragment counting (stats), an OpenMP
+ * element loop with private grids merged by the same online formula, and a
+ * "raw moment" deposit used by the adjoint checks (psi_oracle_adjoint.c).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "psi_oracle.h"
+
+#define OPOLY 64   /* vertex slots per polyhedron (include/geometry.h:120) */
+#define OSTACK 64  /* DFS stack depth (src/geometry.c:41-46) */
+
+typedef struct {
+    double x[3];
+    double q[3];
+    int16_t nb[3];
+} overt;
+
+typedef struct {
+    int nv;
+    int lo[3], hi[3];
+    overt v[OPOLY];
+} opoly;
+
+/* per-thread scratch: DFS stack and refinement buffer */
+typedef struct {
+    opoly stack[OSTACK];
+    int nstack;
+    /* refined tets: 4 pos + 4 vel + mass each */
+    double* tpos;
+    double* tvel;
+    double* tmass;
+    int ntet, cap;
+} oscratch;
+
+/* ------------------------------------------------------------------ */
+/* bounding boxes, periodic wrap, ghosts (src/cpsi.c:277-373)          */
+/* ------------------------------------------------------------------ */
+
+/* src/cpsi.c:277-291 */
+static void o_aabb(const double (*p)[3], int n, double lo[3], double hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = 1.0e30;
+        hi[a] = -1.0e30;
+    }
+    for (int v = 0; v < n; ++v)
+        for (int a = 0; a < 3; ++a) {
+            if (p[v][a] < lo[a]) lo[a] = p[v][a];
+            if (p[v][a] > hi[a]) hi[a] = p[v][a];
+        }
+}
+
+/* src/cpsi.c:309-341: wrap an element that spans more than half the periodic
+ * box, then reject it if it misses the grid window.  Returns 0 to skip. */
+static int o_wrap_and_test(double (*p)[3], int nv, double lo[3], double hi[3],
+                           const orc_grid* g, const orc_mesh* m) {
+    o_aabb((const double (*)[3])p, nv, lo, hi);
+    if (m->periodic) {
+        for (int a = 0; a < 3; ++a) {
+            double span = m->box[1][a] - m->box[0][a];
+            if (hi[a] - lo[a] > 0.5 * span) {
+                lo[a] = 1.0e30;
+                hi[a] = -1.0e30;
+                for (int v = 0; v < nv; ++v) {
+                    if (p[v][a] > m->box[0][a] + 0.5 * span) p[v][a] -= span;
+                    if (p[v][a] < lo[a]) lo[a] = p[v][a];
+                    if (p[v][a] > hi[a]) hi[a] = p[v][a];
+                }
+            }
+            if (lo[a] > g->win[1][a]) return 0;
+            if (hi[a] < g->win[0][a] && lo[a] + span > g->win[1][a]) return 0;
+        }
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            if (lo[a] > g->win[1][a]) return 0;
+            if (hi[a] < g->win[0][a]) return 0;
+        }
+    }
+    return 1;
+}
+
+/* src/cpsi.c:343-373: periodic copies of a tet shifted by +span along every
+ * nonempty subset of the axes where it pokes below the box; a copy is kept
+ * only if its shifted box still meets the window.  gp holds 8x4 vertices,
+ * gb holds 8x2 boxes; slot 0 is the tet itself.  Returns the count. */
+static int o_ghosts(double (*gp)[3], double (*gb)[3], const orc_grid* g, const orc_mesh* m) {
+    int num = 1;
+    if (!m->periodic) return num;
+    int flags = 0;
+    double span[3];
+    for (int a = 0; a < 3; ++a) {
+        span[a] = m->box[1][a] - m->box[0][a];
+        if (gb[0][a] < m->box[0][a]) flags |= 1 << a;
+    }
+    for (int p = 1; p < 8; ++p) {
+        if ((flags & p) != p) continue;
+        int ok = 1;
+        for (int a = 0; a < 3 && ok; ++a) {
+            double sh = span[a] * (double)(1 & (p >> a));
+            double blo = gb[0][a] + sh, bhi = gb[1][a] + sh;
+            gb[2 * num + 0][a] = blo;
+            gb[2 * num + 1][a] = bhi;
+            if (blo > g->win[1][a] || bhi < g->win[0][a]) {
+                ok = 0;
+                break;
+            }
+            for (int v = 0; v < 4; ++v) gp[4 * num + v][a] = gp[v][a] + sh;
+        }
+        if (ok) num++;
+    }
+    return num;
+}
+
+/* ------------------------------------------------------------------ */
+/* element -> tets (src/refine.c:39-368)                               */
+/* ------------------------------------------------------------------ */
+
+/* corners of the 6-tet split of a 3x3x3 lattice, indices 9i+3j+k
+ * (src/refine.c:39-41) */
+static const int O_TET27[6][4] = {{2, 0, 6, 18}, {8, 2, 6, 18}, {8, 20, 2, 18},
+                                  {8, 24, 20, 18}, {8, 6, 24, 18}, {8, 26, 20, 24}};
+
+static void o_tetbuf_push6(oscratch* s, const double (*p27)[3], const double (*v27)[3], double mass) {
+    if (s->ntet + 6 >= s->cap) {
+        s->cap = (int)(1.5 * s->cap) + 6;
+        s->tpos = (double*)realloc(s->tpos, (size_t)s->cap * 12 * sizeof(double));
+        s->tvel = (double*)realloc(s->tvel, (size_t)s->cap * 12 * sizeof(double));
+        s->tmass = (double*)realloc(s->tmass, (size_t)s->cap * sizeof(double));
+    }
+    for (int t = 0; t < 6; ++t) {
+        for (int v = 0; v < 4; ++v)
+            for (int a = 0; a < 3; ++a) {
+                s->tpos[12 * s->ntet + 3 * v + a] = p27[O_TET27[t][v]][a];
+                s->tvel[12 * s->ntet + 3 * v + a] = v27[O_TET27[t][v]][a];
+            }
+        s->tmass[s->ntet] = mass * (1.0 / 6);
+        s->ntet++;
+    }
+}
+
+/* src/refine.c:292-319: corners of a 2x2x2 element (index 4i+2j+k) to the
+ * 3x3x3 lattice by successive midpoints along x, then y, then z. */
+static void o_lin27(const double (*in)[3], double (*out)[3]) {
+    for (int a = 0; a < 3; ++a) {
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int k = 0; k < 2; ++k) out[9 * 2 * i + 3 * 2 * j + 2 * k][a] = in[4 * i + 2 * j + k][a];
+        for (int j = 0; j < 3; j += 2)
+            for (int k = 0; k < 3; k += 2) out[9 + 3 * j + k][a] = 0.5 * (out[3 * j + k][a] + out[18 + 3 * j + k][a]);
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; k += 2)
+                out[9 * i + 3 + k][a] = 0.5 * (out[9 * i + k][a] + out[9 * i + 6 + k][a]);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                out[9 * i + 3 * j + 1][a] = 0.5 * (out[9 * i + 3 * j][a] + out[9 * i + 3 * j + 2][a]);
+    }
+}
+
+/* src/refine.c:321-368: 3x3x3 quadratic nodes to the 5x5x5 lattice, using
+ * the quadratic Lagrange weights (3, 6, -1)/8 at the quarter points. */
+static void o_quad125(const double (*in)[3], double (*out)[3]) {
+#define I5(i, j, k) (25 * (i) + 5 * (j) + (k))
+    for (int a = 0; a < 3; ++a) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) out[I5(2 * i, 2 * j, 2 * k)][a] = in[9 * i + 3 * j + k][a];
+        for (int j = 0; j < 5; j += 2)
+            for (int k = 0; k < 5; k += 2) {
+                out[I5(1, j, k)][a] =
+                    0.125 * (3.0 * out[I5(0, j, k)][a] + 6.0 * out[I5(2, j, k)][a] - out[I5(4, j, k)][a]);
+                out[I5(3, j, k)][a] =
+                    0.125 * (3.0 * out[I5(4, j, k)][a] + 6.0 * out[I5(2, j, k)][a] - out[I5(0, j, k)][a]);
+            }
+        for (int i = 0; i < 5; ++i)
+            for (int k = 0; k < 5; k += 2) {
+                out[I5(i, 1, k)][a] =
+                    0.125 * (3.0 * out[I5(i, 0, k)][a] + 6.0 * out[I5(i, 2, k)][a] - out[I5(i, 4, k)][a]);
+                out[I5(i, 3, k)][a] =
+                    0.125 * (3.0 * out[I5(i, 4, k)][a] + 6.0 * out[I5(i, 2, k)][a] - out[I5(i, 0, k)][a]);
+            }
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j) {
+                out[I5(i, j, 1)][a] =
+                    0.125 * (3.0 * out[I5(i, j, 0)][a] + 6.0 * out[I5(i, j, 2)][a] - out[I5(i, j, 4)][a]);
+                out[I5(i, j, 3)][a] =
+                    0.125 * (3.0 * out[I5(i, j, 4)][a] + 6.0 * out[I5(i, j, 2)][a] - out[I5(i, j, 0)][a]);
+            }
+    }
+}
+
+/* squared distance between an edge midpoint and the lattice point between */
+static double o_mid_err2(const double (*p)[3], int e0, int em, int e1) {
+    double err2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        double mt = 0.5 * (p[e0][a] + p[e1][a]);
+        double ml = p[em][a];
+        err2 += (mt - ml) * (mt - ml);
+    }
+    return err2;
+}
+
+/* src/refine.c:223-289: adaptive trilinear refinement on the 27-point lattice.
+ * The 7 interior tet edges of the 6-tet split are tested; if any deviates by
+ * more than the tolerance the element is cut into 8 children of mass/8. */
+static void o_refine_lin(oscratch* s, const double (*p27)[3], const double (*v27)[3], double mass, int lvl,
+                         int maxlvl, double tol2) {
+    static const int E[7][3] = {{2, 4, 6}, {2, 10, 18}, {6, 12, 18}, {8, 13, 18},
+                                {8, 14, 20}, {8, 16, 24}, {20, 22, 24}};
+    if (lvl < maxlvl) {
+        for (int e = 0; e < 7; ++e) {
+            if (o_mid_err2(p27, E[e][0], E[e][1], E[e][2]) > tol2) {
+                double cp[8][3], cv[8][3], np[27][3], nv[27][3];
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j)
+                        for (int k = 0; k < 2; ++k) {
+                            for (int ii = 0; ii < 2; ++ii)
+                                for (int jj = 0; jj < 2; ++jj)
+                                    for (int kk = 0; kk < 2; ++kk) {
+                                        int src = 9 * (ii + i) + 3 * (jj + j) + (kk + k);
+                                        memcpy(cp[4 * ii + 2 * jj + kk], p27[src], sizeof(cp[0]));
+                                        memcpy(cv[4 * ii + 2 * jj + kk], v27[src], sizeof(cv[0]));
+                                    }
+                            o_lin27((const double (*)[3])cp, np);
+                            o_lin27((const double (*)[3])cv, nv);
+                            o_refine_lin(s, (const double (*)[3])np, (const double (*)[3])nv, 0.125 * mass, lvl + 1,
+                                         maxlvl, tol2);
+                        }
+                return;
+            }
+        }
+    }
+    o_tetbuf_push6(s, p27, v27, mass);
+}
+
+/* src/refine.c:152-221: adaptive triquadratic refinement (19 edges tested,
+ * children taken from the 5x5x5 quadratic interpolation). */
+static void o_refine_quad(oscratch* s, const double (*p27)[3], const double (*v27)[3], double mass, int lvl,
+                          int maxlvl, double tol2) {
+    static const int E[19][3] = {{0, 1, 2},     {0, 3, 6},     {0, 9, 18},    {2, 4, 6},     {2, 5, 8},
+                                 {2, 10, 18},   {2, 11, 20},   {6, 7, 8},     {6, 12, 18},   {6, 15, 24},
+                                 {8, 13, 18},   {8, 14, 20},   {8, 16, 24},   {8, 17, 26},   {18, 19, 20},
+                                 {18, 21, 24},  {20, 22, 24},  {20, 23, 26},  {24, 25, 26}};
+    if (lvl < maxlvl) {
+        for (int e = 0; e < 19; ++e) {
+            if (o_mid_err2(p27, E[e][0], E[e][1], E[e][2]) > tol2) {
+                double ip[125][3], iv[125][3], cp[27][3], cv[27][3];
+                o_quad125(p27, ip);
+                o_quad125(v27, iv);
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j)
+                        for (int k = 0; k < 2; ++k) {
+                            for (int ii = 0; ii < 3; ++ii)
+                                for (int jj = 0; jj < 3; ++jj)
+                                    for (int kk = 0; kk < 3; ++kk) {
+                                        int src = 25 * (ii + 2 * i) + 5 * (jj + 2 * j) + (kk + 2 * k);
+                                        memcpy(cp[9 * ii + 3 * jj + kk], ip[src], sizeof(cp[0]));
+                                        memcpy(cv[9 * ii + 3 * jj + kk], iv[src], sizeof(cv[0]));
+                                    }
+                            o_refine_quad(s, (const double (*)[3])cp, (const double (*)[3])cv, 0.125 * mass, lvl + 1,
+                                          maxlvl, tol2);
+                        }
+                return;
+            }
+        }
+    }
+    o_tetbuf_push6(s, p27, v27, mass);
+}
+
+/* src/refine.c:58-83 */
+static void o_refine(oscratch* s, const double (*p)[3], const double (*v)[3], double mass, int elemtype, int maxlvl,
+                     double tol2) {
+    s->ntet = 0;
+    if (elemtype == 4) {
+        if (s->cap < 1) {
+            s->cap = 6;
+            s->tpos = (double*)realloc(s->tpos, (size_t)s->cap * 12 * sizeof(double));
+            s->tvel = (double*)realloc(s->tvel, (size_t)s->cap * 12 * sizeof(double));
+            s->tmass = (double*)realloc(s->tmass, (size_t)s->cap * sizeof(double));
+        }
+        memcpy(s->tpos, p, 12 * sizeof(double));
+        memcpy(s->tvel, v, 12 * sizeof(double));
+        s->tmass[0] = mass;
+        s->ntet = 1;
+    } else if (elemtype == 8) {
+        double p27[27][3], v27[27][3];
+        o_lin27(p, p27);
+        o_lin27(v, v27);
+        o_refine_lin(s, (const double (*)[3])p27, (const double (*)[3])v27, mass, 0, maxlvl, tol2);
+    } else if (elemtype == 27) {
+        o_refine_quad(s, p, v, mass, 0, maxlvl, tol2);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* polyhedron clip / split / reduce (src/geometry.c:391-501,560-804)   */
+/* ------------------------------------------------------------------ */
+
+/* src/geometry.c:769-804: the tet as a vertex graph with three neighbours per
+ * vertex listed counter-clockwise seen from outside; q are mass coordinates. */
+static void o_init_tet(opoly* P, const double (*x)[3]) {
+    static const int16_t NB[4][3] = {{1, 3, 2}, {2, 3, 0}, {0, 3, 1}, {1, 2, 0}};
+    memset(P, 0, sizeof(*P));
+    P->nv = 4;
+    for (int v = 0; v < 4; ++v) {
+        for (int a = 0; a < 3; ++a) {
+            P->v[v].x[a] = x[v][a];
+            P->v[v].q[a] = (v == a + 1) ? 1.0 : 0.0;
+            P->v[v].nb[a] = NB[v][a];
+        }
+    }
+}
+
+/* Walk the face to the right of each freshly inserted vertex until the next
+ * fresh vertex and doubly link the two (src/geometry.c:459-482, 646-669). */
+static void o_link_new(overt* w, int onv, int tot) {
+    for (int vs = onv; vs < tot; ++vs) {
+        int vc = vs, vn = w[vs].nb[0];
+        do {
+            int k;
+            for (k = 0; k < 3; ++k)
+                if (w[vn].nb[k] == vc) break;
+            vc = vn;
+            vn = w[vc].nb[(k + 1) % 3];
+        } while (vc < onv);
+        w[vs].nb[2] = (int16_t)vc;
+        w[vc].nb[1] = (int16_t)vs;
+    }
+}
+
+/* src/geometry.c:391-501: keep the part of P with d + n.x >= 0 for each plane */
+static void o_clip(opoly* P, const double (*pn)[3], const double* pd, int np, orc_stats* st) {
+    if (P->nv <= 0) return;
+    for (int p = 0; p < np; ++p) {
+        int onv = P->nv;
+        double s[2 * OPOLY];
+        int gone[2 * OPOLY];
+        double smin = 1.0e30, smax = -1.0e30;
+        for (int v = 0; v < onv; ++v) {
+            s[v] = pd[p];
+            for (int a = 0; a < 3; ++a) s[v] += P->v[v].x[a] * pn[p][a];
+            if (s[v] < smin) smin = s[v];
+            if (s[v] > smax) smax = s[v];
+            gone[v] = s[v] < 0.0;
+        }
+        if (smin >= 0.0) continue;
+        if (smax <= 0.0) {
+            P->nv = 0;
+            return;
+        }
+        overt w[2 * OPOLY];
+        memcpy(w, P->v, (size_t)onv * sizeof(overt));
+        int tot = onv;
+        for (int vc = 0; vc < onv; ++vc) {
+            if (gone[vc]) continue;
+            for (int k = 0; k < 3; ++k) {
+                int vn = w[vc].nb[k];
+                if (!gone[vn]) continue;
+                if (tot >= 2 * OPOLY) {
+                    st->overflow++;
+                    continue;
+                }
+                double inv = 1.0 / (s[vc] - s[vn]);
+                for (int a = 0; a < 3; ++a) {
+                    w[tot].x[a] = inv * (w[vn].x[a] * s[vc] - w[vc].x[a] * s[vn]);
+                    w[tot].q[a] = inv * (w[vn].q[a] * s[vc] - w[vc].q[a] * s[vn]);
+                }
+                w[tot].nb[0] = (int16_t)vc;
+                w[vc].nb[k] = (int16_t)tot;
+                gone[tot] = 0;
+                tot++;
+            }
+        }
+        o_link_new(w, onv, tot);
+        int remap[2 * OPOLY], nk = 0;
+        for (int v = 0; v < tot; ++v) remap[v] = gone[v] ? -1 : nk++;
+        if (nk > OPOLY) {
+            st->overflow++;
+            nk = OPOLY;
+        }
+        for (int v = 0; v < tot; ++v) {
+            if (remap[v] < 0 || remap[v] >= OPOLY) continue;
+            P->v[remap[v]] = w[v];
+        }
+        P->nv = nk;
+        for (int v = 0; v < nk; ++v)
+            for (int k = 0; k < 3; ++k) P->v[v].nb[k] = (int16_t)remap[P->v[v].nb[k]];
+    }
+}
+
+/* src/geometry.c:560-690: cut P by the plane x[ax] = c.  Points with
+ * x[ax] <= c go to out[0], the rest to out[1]; every cut edge yields two
+ * coincident vertices, one per side, so the two pieces tile P exactly. */
+static void o_split(const opoly* P, opoly* out, double c, int ax, orc_stats* st) {
+    int onv = P->nv;
+    double s[2 * OPOLY];
+    int side[2 * OPOLY];
+    int nright = 0;
+    for (int v = 0; v < onv; ++v) {
+        s[v] = c - P->v[v].x[ax];
+        side[v] = s[v] < 0.0;
+        nright += side[v];
+    }
+    if (nright == 0) {
+        out[0] = *P;
+        out[1].nv = 0;
+        return;
+    }
+    if (nright == onv) {
+        out[1] = *P;
+        out[0].nv = 0;
+        return;
+    }
+    overt w[2 * OPOLY];
+    memcpy(w, P->v, (size_t)onv * sizeof(overt));
+    int tot = onv;
+    for (int vc = 0; vc < onv; ++vc) {
+        if (side[vc]) continue;
+        for (int k = 0; k < 3; ++k) {
+            int vn = w[vc].nb[k];
+            if (!side[vn]) continue;
+            if (tot + 2 > 2 * OPOLY) {
+                st->overflow++;
+                continue;
+            }
+            double inv = 1.0 / (s[vc] - s[vn]);
+            double nx[3], nq[3];
+            for (int a = 0; a < 3; ++a) {
+                nx[a] = inv * (w[vn].x[a] * s[vc] - w[vc].x[a] * s[vn]);
+                nq[a] = inv * (w[vn].q[a] * s[vc] - w[vc].q[a] * s[vn]);
+            }
+            /* copy that stays with the lower side */
+            memcpy(w[tot].x, nx, sizeof nx);
+            memcpy(w[tot].q, nq, sizeof nq);
+            w[tot].nb[0] = (int16_t)vc;
+            w[vc].nb[k] = (int16_t)tot;
+            side[tot] = 0;
+            tot++;
+            /* copy that goes with the upper side */
+            memcpy(w[tot].x, nx, sizeof nx);
+            memcpy(w[tot].q, nq, sizeof nq);
+            w[tot].nb[0] = (int16_t)vn;
+            side[tot] = 1;
+            int kk;
+            for (kk = 0; kk < 3; ++kk)
+                if (w[vn].nb[kk] == vc) break;
+            w[vn].nb[kk] = (int16_t)tot;
+            tot++;
+        }
+    }
+    o_link_new(w, onv, tot);
+    int remap[2 * OPOLY];
+    out[0].nv = 0;
+    out[1].nv = 0;
+    for (int v = 0; v < tot; ++v) {
+        opoly* o = &out[side[v]];
+        if (o->nv >= OPOLY) {
+            st->overflow++;
+            remap[v] = 0;
+            continue;
+        }
+        o->v[o->nv] = w[v];
+        remap[v] = o->nv++;
+    }
+    for (int h = 0; h < 2; ++h)
+        for (int v = 0; v < out[h].nv; ++v)
+            for (int k = 0; k < 3; ++k) out[h].v[v].nb[k] = (int16_t)remap[out[h].v[v].nb[k]];
+    if (out[0].nv > st->maxverts) st->maxverts = out[0].nv;
+    if (out[1].nv > st->maxverts) st->maxverts = out[1].nv;
+}
+
+/* src/geometry.c:692-767: mass-coordinate moments of P by fan-triangulating
+ * every face from its first vertex and summing the signed tets spanned with
+ * the origin.  m[0] = 6 vol_q (1 for the whole tet), m[1..3] first moments,
+ * m[4..9] second moments (xx, xy, xz, yy, yz, zz), all scaled alike. */
+static void o_reduce(const opoly* P, double* m, int order) {
+    static const int NMOM[3] = {1, 4, 10};
+    int nv = P->nv;
+    unsigned char mark[OPOLY][3];
+    memset(m, 0, (size_t)NMOM[order] * sizeof(double));
+    memset(mark, 0, sizeof mark);
+    for (int vs = 0; vs < nv; ++vs)
+        for (int ks = 0; ks < 3; ++ks) {
+            if (mark[vs][ks]) continue;
+            int vc = vs, kn = ks;
+            mark[vc][kn] = 1;
+            int vn = P->v[vc].nb[kn];
+            const double* q0 = P->v[vc].q;
+            int k;
+            for (k = 0; k < 3; ++k)
+                if (P->v[vn].nb[k] == vc) break;
+            vc = vn;
+            kn = (k + 1) % 3;
+            mark[vc][kn] = 1;
+            vn = P->v[vc].nb[kn];
+            while (vn != vs) {
+                const double* q1 = P->v[vn].q;
+                const double* q2 = P->v[vc].q;
+                double w = (-q2[0] * q1[1] * q0[2] + q1[0] * q2[1] * q0[2] + q2[0] * q0[1] * q1[2] -
+                            q0[0] * q2[1] * q1[2] - q1[0] * q0[1] * q2[2] + q0[0] * q1[1] * q2[2]);
+                m[0] += w;
+                if (order > 0) {
+                    m[1] += 0.25 * w * (q0[0] + q1[0] + q2[0]);
+                    m[2] += 0.25 * w * (q0[1] + q1[1] + q2[1]);
+                    m[3] += 0.25 * w * (q0[2] + q1[2] + q2[2]);
+                }
+                if (order > 1) {
+                    m[4] += 0.1 * w * (q0[0] * q0[0] + q1[0] * q1[0] + q2[0] * q2[0] + q1[0] * q2[0] + q0[0] * (q1[0] + q2[0]));
+                    m[5] += 0.05 * w *
+                            (q2[0] * q0[1] + q2[0] * q1[1] + 2 * q2[0] * q2[1] + q0[0] * (2 * q0[1] + q1[1] + q2[1]) +
+                             q1[0] * (q0[1] + 2 * q1[1] + q2[1]));
+                    m[6] += 0.05 * w *
+                            (q2[0] * q0[2] + q2[0] * q1[2] + 2 * q2[0] * q2[2] + q0[0] * (2 * q0[2] + q1[2] + q2[2]) +
+                             q1[0] * (q0[2] + 2 * q1[2] + q2[2]));
+                    m[7] += 0.1 * w * (q0[1] * q0[1] + q1[1] * q1[1] + q2[1] * q2[1] + q1[1] * q2[1] + q0[1] * (q1[1] + q2[1]));
+                    m[8] += 0.05 * w *
+                            (q2[1] * q0[2] + q2[1] * q1[2] + 2 * q2[1] * q2[2] + q0[1] * (2 * q0[2] + q1[2] + q2[2]) +
+                             q1[1] * (q0[2] + 2 * q1[2] + q2[2]));
+                    m[9] += 0.1 * w * (q0[2] * q0[2] + q1[2] * q1[2] + q2[2] * q2[2] + q1[2] * q2[2] + q0[2] * (q1[2] + q2[2]));
+                }
+                for (k = 0; k < 3; ++k)
+                    if (P->v[vn].nb[k] == vc) break;
+                vc = vn;
+                kn = (k + 1) % 3;
+                mark[vc][kn] = 1;
+                vn = P->v[vc].nb[kn];
+            }
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* one tet onto the grid (src/geometry.c:60-188, 806-896)              */
+/* ------------------------------------------------------------------ */
+
+static int o_polyorder(const orc_grid* g) {
+    if (g->f[ORC_XX] || g->f[ORC_XV] || g->f[ORC_VV]) return 2;
+    if (g->f[ORC_X] || g->f[ORC_V]) return 1;
+    return 0;
+}
+
+/* Deposit one fragment with the reference's online mass-weighted update
+ * (src/geometry.c:105-186).  m = q-moments of the fragment. */
+static void o_deposit(orc_grid* g, int64_t gi, const double* m, const double (*x)[3], const double (*u)[3],
+                      const double DX[3][3], const double DV[3][3], double mass, int xvfix) {
+    static const int QI[3][3] = {{4, 5, 6}, {5, 7, 8}, {6, 8, 9}};
+    double cm[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
+    double sxx[3][3], sxv[3][3], svv[3][3];
+    if (g->f[ORC_X])
+        for (int i = 0; i < 3; ++i) {
+            cm[i] = m[0] * x[0][i];
+            for (int j = 0; j < 3; ++j) cm[i] += m[j + 1] * DX[i][j];
+            cm[i] /= m[0];
+        }
+    if (g->f[ORC_V])
+        for (int i = 0; i < 3; ++i) {
+            cv[i] = m[0] * u[0][i];
+            for (int j = 0; j < 3; ++j) cv[i] += m[j + 1] * DV[i][j];
+            cv[i] /= m[0];
+        }
+    if (g->f[ORC_XX])
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                sxx[i][j] = m[0] * (x[0][i] - cm[i]) * (x[0][j] - cm[j]);
+                for (int a = 0; a < 3; ++a) {
+                    sxx[i][j] += DX[i][a] * m[1 + a] * (x[0][j] - cm[j]);
+                    sxx[i][j] += DX[j][a] * m[1 + a] * (x[0][i] - cm[i]);
+                }
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) sxx[i][j] += DX[i][a] * DX[j][b] * m[QI[a][b]];
+                sxx[i][j] /= m[0];
+            }
+    if (g->f[ORC_XV])
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                sxv[i][j] = m[0] * (x[0][i] - cm[i]) * (u[0][j] - cv[j]);
+                for (int a = 0; a < 3; ++a) {
+                    if (xvfix) {
+                        /* the true cross-covariance */
+                        sxv[i][j] += DX[i][a] * m[1 + a] * (u[0][j] - cv[j]);
+                        sxv[i][j] += DV[j][a] * m[1 + a] * (x[0][i] - cm[i]);
+                    } else {
+                        /* reference formula, mismatched factors kept
+                         * (src/geometry.c:139-140) */
+                        sxv[i][j] += DX[i][a] * m[1 + a] * (x[0][j] - cm[j]);
+                        sxv[i][j] += DV[j][a] * m[1 + a] * (u[0][i] - cv[i]);
+                    }
+                }
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) sxv[i][j] += DX[i][a] * DV[j][b] * m[QI[a][b]];
+                sxv[i][j] /= m[0];
+            }
+    if (g->f[ORC_VV])
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                svv[i][j] = m[0] * (u[0][i] - cv[i]) * (u[0][j] - cv[j]);
+                for (int a = 0; a < 3; ++a) {
+                    svv[i][j] += DV[i][a] * m[1 + a] * (u[0][j] - cv[j]);
+                    svv[i][j] += DV[j][a] * m[1 + a] * (u[0][i] - cv[i]);
+                }
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) svv[i][j] += DV[i][a] * DV[j][b] * m[QI[a][b]];
+                svv[i][j] /= m[0];
+            }
+    double* M = g->f[ORC_M];
+    double* X = g->f[ORC_X];
+    double* V = g->f[ORC_V];
+    double cw = mass * m[0] / (M[gi] + mass * m[0]);
+    double dw = M[gi] / (M[gi] + mass * m[0]);
+    if (g->f[ORC_XX]) {
+        double* T = g->f[ORC_XX] + 9 * gi;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                T[3 * i + j] = dw * T[3 * i + j] + cw * sxx[i][j] + cw * dw * (cm[i] - X[3 * gi + i]) * (cm[j] - X[3 * gi + j]);
+    }
+    if (g->f[ORC_XV]) {
+        double* T = g->f[ORC_XV] + 9 * gi;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                T[3 * i + j] = dw * T[3 * i + j] + cw * sxv[i][j] + cw * dw * (cm[i] - X[3 * gi + i]) * (cv[j] - V[3 * gi + j]);
+    }
+    if (g->f[ORC_VV]) {
+        double* T = g->f[ORC_VV] + 9 * gi;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                T[3 * i + j] = dw * T[3 * i + j] + cw * svv[i][j] + cw * dw * (cv[i] - V[3 * gi + i]) * (cv[j] - V[3 * gi + j]);
+    }
+    if (X)
+        for (int i = 0; i < 3; ++i) X[3 * gi + i] += cw * (cm[i] - X[3 * gi + i]);
+    if (V)
+        for (int i = 0; i < 3; ++i) V[3 * gi + i] += cw * (cv[i] - V[3 * gi + i]);
+    M[gi] += mass * m[0];
+}
+
+/* psi_voxelize_tet + psi_voxels_init + psi_voxels_next
+ * (src/geometry.c:60-188, 806-850, 852-896).  Calls fn(ctx, gridind, m) for
+ * every kept fragment if fn is given, otherwise deposits into g. */
+typedef void (*o_frag_fn)(void* ctx, int64_t gi, const int* ijk, const double* m, const opoly* P);
+
+static void o_voxelize_tet(oscratch* S, orc_grid* g, const double (*x)[3], const double (*u)[3], double mass,
+                           const double (*rb)[3], orc_stats* st, int xvfix, int order_override, o_frag_fn fn,
+                           void* ctx) {
+    double DX[3][3], DV[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            DX[i][j] = x[j + 1][i] - x[0][i];
+            DV[i][j] = u[j + 1][i] - u[0][i];
+        }
+    int order = order_override >= 0 ? order_override : o_polyorder(g);
+
+    opoly* P = &S->stack[0];
+    o_init_tet(P, x);
+    double pn[6][3], pd[6];
+    memset(pn, 0, sizeof pn);
+    memset(pd, 0, sizeof pd);
+    int np = 0;
+    for (int a = 0; a < 3; ++a) {
+        double cmin = rb[0][a], cmax = rb[1][a];
+        int clo = 0, chi = 0;
+        if (g->win[0][a] > cmin) {
+            cmin = g->win[0][a];
+            clo = 1;
+        }
+        if (g->win[1][a] < cmax) {
+            cmax = g->win[1][a];
+            chi = 1;
+        }
+        if (clo) {
+            pn[np][a] = 1.0;
+            pd[np] = -cmin;
+            np++;
+        }
+        if (chi) {
+            pn[np][a] = -1.0;
+            pd[np] = cmax;
+            np++;
+        }
+        P->lo[a] = (int)floor((cmin - g->win[0][a]) / g->d[a]);
+        P->hi[a] = (int)ceil((cmax - g->win[0][a]) / g->d[a]);
+        if (P->lo[a] < 0) P->lo[a] = 0;
+        if (P->hi[a] > g->n[a]) P->hi[a] = g->n[a];
+    }
+    o_clip(P, (const double (*)[3])pn, pd, np, st);
+    S->nstack = 1;
+
+    double m[10];
+    while (S->nstack > 0) {
+        opoly cur = S->stack[--S->nstack];
+        if (cur.nv <= 0) continue;
+        int dmax = 0, ax = 0;
+        for (int a = 0; a < 3; ++a) {
+            int sz = cur.hi[a] - cur.lo[a];
+            if (sz > dmax) {
+                dmax = sz;
+                ax = a;
+            }
+        }
+        if (dmax <= 1) {
+            o_reduce(&cur, m, order);
+            if (m[0] <= 0.0) continue;
+            if (cur.lo[0] >= g->n[0] || cur.lo[1] >= g->n[1] || cur.lo[2] >= g->n[2]) continue;
+            int64_t gi = (int64_t)g->n[1] * g->n[2] * cur.lo[0] + (int64_t)g->n[2] * cur.lo[1] + cur.lo[2];
+            st->nfrag++;
+            if (fn)
+                fn(ctx, gi, cur.lo, m, &cur);
+            else
+                o_deposit(g, gi, m, x, u, (const double (*)[3])DX, (const double (*)[3])DV, mass, xvfix);
+            continue;
+        }
+        if (S->nstack + 2 > OSTACK) {
+            st->overflow++;
+            continue;
+        }
+        double c = g->d[ax] * (cur.lo[ax] + dmax / 2) + g->win[0][ax];
+        opoly* kids = &S->stack[S->nstack];
+        o_split(&cur, kids, c, ax, st);
+        memcpy(kids[0].lo, cur.lo, sizeof cur.lo);
+        memcpy(kids[0].hi, cur.hi, sizeof cur.hi);
+        kids[0].hi[ax] -= dmax - dmax / 2;
+        memcpy(kids[1].lo, cur.lo, sizeof cur.lo);
+        memcpy(kids[1].hi, cur.hi, sizeof cur.hi);
+        kids[1].lo[ax] += dmax / 2;
+        S->nstack += 2;
+        if (S->nstack > st->maxstack) st->maxstack = S->nstack;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* element loop (src/cpsi.c:41-175, density branch :100-112)           */
+/* ------------------------------------------------------------------ */
+
+static void o_scratch_init(oscratch* s) { memset(s, 0, sizeof(*s)); }
+static void o_scratch_free(oscratch* s) {
+    free(s->tpos);
+    free(s->tvel);
+    free(s->tmass);
+}
+
+static double o_tol(const orc_grid* g, double reftol) {
+    double dxmin = 1.0e30;
+    for (int a = 0; a < 3; ++a)
+        if (g->d[a] < dxmin) dxmin = g->d[a];
+    return reftol * dxmin;
+}
+
+/* Process elements [e0, e1) into g.  Shared by the serial and OpenMP drivers
+ * and by the element-subset calls the tests make. */
+static void o_elements(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlvl, int64_t e0, int64_t e1,
+                       oscratch* S, orc_stats* st, int xvfix, int order_override, o_frag_fn fn, void* ctx) {
+    int vpe = mesh->elemtype;
+    double tol = o_tol(g, reftol);
+    double tol2 = tol * tol;
+    if (maxlvl > 6) maxlvl = 6; /* src/refine.c:89-92 */
+    double p[27][3], u[27][3], lo[3], hi[3];
+    double gp[32][3], gb[16][3];
+    for (int64_t e = e0; e < e1; ++e) {
+        double mass = mesh->mass[e];
+        for (int v = 0; v < vpe; ++v) {
+            int64_t id = mesh->conn[(int64_t)e * vpe + v];
+            for (int a = 0; a < 3; ++a) {
+                p[v][a] = mesh->pos[3 * id + a];
+                u[v][a] = mesh->vel[3 * id + a];
+            }
+        }
+        if (!o_wrap_and_test(p, vpe, lo, hi, g, mesh)) continue;
+        st->nelem_used++;
+        o_refine(S, (const double (*)[3])p, (const double (*)[3])u, mass, vpe, maxlvl, tol2);
+        for (int t = 0; t < S->ntet; ++t) {
+            memcpy(gp, &S->tpos[12 * t], 12 * sizeof(double));
+            o_aabb((const double (*)[3])gp, 4, gb[0], gb[1]);
+            int ng = o_ghosts(gp, gb, g, mesh);
+            st->ntets++;
+            st->nghosts += ng;
+            for (int h = 0; h < ng; ++h)
+                o_voxelize_tet(S, g, (const double (*)[3])&gp[4 * h], (const double (*)[3])&S->tvel[12 * t],
+                               S->tmass[t], (const double (*)[3])&gb[2 * h], st, xvfix, order_override, fn, ctx);
+        }
+    }
+}
+
+int orc_voxels(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlvl, int xvfix, orc_stats* st) {
+    orc_stats local;
+    memset(&local, 0, sizeof local);
+    oscratch* S = (oscratch*)malloc(sizeof(oscratch));
+    if (!S) return -1;
+    o_scratch_init(S);
+    o_elements(mesh, g, reftol, maxlvl, 0, mesh->nelem, S, &local, xvfix, -1, NULL, NULL);
+    o_scratch_free(S);
+    free(S);
+    if (st) *st = local;
+    return 0;
+}
+
+int orc_voxels_range(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlvl, int xvfix, int64_t e0,
+                     int64_t e1, orc_stats* st) {
+    orc_stats local;
+    memset(&local, 0, sizeof local);
+    oscratch* S = (oscratch*)malloc(sizeof(oscratch));
+    if (!S) return -1;
+    o_scratch_init(S);
+    if (e1 > mesh->nelem) e1 = mesh->nelem;
+    o_elements(mesh, g, reftol, maxlvl, e0, e1, S, &local, xvfix, -1, NULL, NULL);
+    o_scratch_free(S);
+    free(S);
+    if (st) *st = local;
+    return 0;
+}
+
+/* merge grid state B into A with the same online formula the deposit uses */
+static void o_merge_voxel(orc_grid* A, const orc_grid* B, int64_t gi) {
+    double w = B->f[ORC_M][gi];
+    if (w == 0.0) return;
+    double* M = A->f[ORC_M];
+    double cw = w / (M[gi] + w), dw = M[gi] / (M[gi] + w);
+    double cm[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
+    if (A->f[ORC_X])
+        for (int i = 0; i < 3; ++i) cm[i] = B->f[ORC_X][3 * gi + i];
+    if (A->f[ORC_V])
+        for (int i = 0; i < 3; ++i) cv[i] = B->f[ORC_V][3 * gi + i];
+    const double* X = A->f[ORC_X] ? A->f[ORC_X] + 3 * gi : NULL;
+    const double* V = A->f[ORC_V] ? A->f[ORC_V] + 3 * gi : NULL;
+    for (int f = ORC_XX; f <= ORC_VV; ++f) {
+        if (!A->f[f]) continue;
+        double* T = A->f[f] + 9 * gi;
+        const double* s = B->f[f] + 9 * gi;
+        const double* a = (f == ORC_VV) ? cv : cm;
+        const double* A0 = (f == ORC_VV) ? V : X;
+        const double* b = (f == ORC_XX) ? cm : cv;
+        const double* B0 = (f == ORC_XX) ? X : V;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                T[3 * i + j] = dw * T[3 * i + j] + cw * s[3 * i + j] + cw * dw * (a[i] - A0[i]) * (b[j] - B0[j]);
+    }
+    if (A->f[ORC_X])
+        for (int i = 0; i < 3; ++i) A->f[ORC_X][3 * gi + i] += cw * (cm[i] - A->f[ORC_X][3 * gi + i]);
+    if (A->f[ORC_V])
+        for (int i = 0; i < 3; ++i) A->f[ORC_V][3 * gi + i] += cw * (cv[i] - A->f[ORC_V][3 * gi + i]);
+    M[gi] += w;
+}
+
+/* OpenMP element loop: every thread deposits into a private zeroed grid with
+ * the online update; the private grids are then merged into g in thread
+ * order.  nthreads <= 0 uses the OpenMP default. */
+int orc_voxels_omp(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlvl, int xvfix, int nthreads, int64_t e0,
+                   int64_t e1, orc_stats* st) {
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    if (e1 > mesh->nelem) e1 = mesh->nelem;
+    int64_t nvox = (int64_t)g->n[0] * g->n[1] * g->n[2];
+    static const int FW[6] = {1, 3, 3, 9, 9, 9};
+    orc_grid* priv = (orc_grid*)calloc((size_t)nthreads, sizeof(orc_grid));
+    orc_stats* pst = (orc_stats*)calloc((size_t)nthreads, sizeof(orc_stats));
+    if (!priv || !pst) return -1;
+    for (int t = 0; t < nthreads; ++t) {
+        priv[t] = *g;
+        for (int f = 0; f < 6; ++f) {
+            priv[t].f[f] = NULL;
+            if (g->f[f]) {
+                priv[t].f[f] = (double*)calloc((size_t)(nvox * FW[f]), sizeof(double));
+                if (!priv[t].f[f]) return -1;
+            }
+        }
+    }
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+#ifdef _OPENMP
+        int tid = omp_get_thread_num();
+#else
+        int tid = 0;
+#endif
+        oscratch* S = (oscratch*)malloc(sizeof(oscratch));
+        o_scratch_init(S);
+        const int64_t chunk = 64;
+        int64_t nchunks = (e1 - e0 + chunk - 1) / chunk;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int64_t c = 0; c < nchunks; ++c) {
+            int64_t a = e0 + c * chunk, b = a + chunk < e1 ? a + chunk : e1;
+            o_elements(mesh, &priv[tid], reftol, maxlvl, a, b, S, &pst[tid], xvfix, -1, NULL, NULL);
+        }
+        o_scratch_free(S);
+        free(S);
+    }
+    orc_stats tot;
+    memset(&tot, 0, sizeof tot);
+    for (int t = 0; t < nthreads; ++t) {
+        for (int64_t gi = 0; gi < nvox; ++gi) o_merge_voxel(g, &priv[t], gi);
+        tot.nfrag += pst[t].nfrag;
+        tot.ntets += pst[t].ntets;
+        tot.nghosts += pst[t].nghosts;
+        tot.nelem_used += pst[t].nelem_used;
+        tot.overflow += pst[t].overflow;
+        if (pst[t].maxverts > tot.maxverts) tot.maxverts = pst[t].maxverts;
+        if (pst[t].maxstack > tot.maxstack) tot.maxstack = pst[t].maxstack;
+        for (int f = 0; f < 6; ++f) free(priv[t].f[f]);
+    }
+    free(priv);
+    free(pst);
+    if (st) *st = tot;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* fragment listing (used by tests to compare fragment sets)            */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t cap, n;
+    int64_t* gi;
+    double* m0;
+    double mass;
+} o_list_ctx;
+
+static void o_list_fn(void* vctx, int64_t gi, const int* ijk, const double* m, const opoly* P) {
+    (void)ijk;
+    (void)P;
+    o_list_ctx* c = (o_list_ctx*)vctx;
+    if (c->n < c->cap) {
+        c->gi[c->n] = gi;
+        c->m0[c->n] = c->mass * m[0];
+    }
+    c->n++;
+}
+
+/* Lists (voxel index, mass*m0) of every kept fragment of tets emitted by
+ * elements [e0,e1).  Returns the number of fragments (may exceed cap). */
+int64_t orc_list_fragments(const orc_mesh* mesh, const orc_grid* g, double reftol, int maxlvl, int64_t e0, int64_t e1,
+                           int64_t* gi, double* mm, int64_t cap) {
+    orc_stats st;
+    memset(&st, 0, sizeof st);
+    oscratch* S = (oscratch*)malloc(sizeof(oscratch));
+    o_scratch_init(S);
+    o_list_ctx ctx = {cap, 0, gi, mm, 1.0};
+    /* the per-tet mass is applied inside o_list_fn through ctx.mass; walk
+     * elements one tet at a time so the mass is known */
+    int vpe = mesh->elemtype;
+    double tol = o_tol(g, reftol), tol2 = tol * tol;
+    if (maxlvl > 6) maxlvl = 6;
+    double p[27][3], u[27][3], lo[3], hi[3], gp[32][3], gb[16][3];
+    if (e1 > mesh->nelem) e1 = mesh->nelem;
+    for (int64_t e = e0; e < e1; ++e) {
+        for (int v = 0; v < vpe; ++v) {
+            int64_t id = mesh->conn[(int64_t)e * vpe + v];
+            for (int a = 0; a < 3; ++a) {
+                p[v][a] = mesh->pos[3 * id + a];
+                u[v][a] = mesh->vel[3 * id + a];
+            }
+        }
+        if (!o_wrap_and_test(p, vpe, lo, hi, g, mesh)) continue;
+        o_refine(S, (const double (*)[3])p, (const double (*)[3])u, mesh->mass[e], vpe, maxlvl, tol2);
+        for (int t = 0; t < S->ntet; ++t) {
+            memcpy(gp, &S->tpos[12 * t], 12 * sizeof(double));
+            o_aabb((const double (*)[3])gp, 4, gb[0], gb[1]);
+            int ng = o_ghosts(gp, gb, g, mesh);
+            ctx.mass = S->tmass[t];
+            for (int h = 0; h < ng; ++h)
+                o_voxelize_tet(S, (orc_grid*)g, (const double (*)[3])&gp[4 * h], (const double (*)[3])&S->tvel[12 * t],
+                               S->tmass[t], (const double (*)[3])&gb[2 * h], &st, 0, 0, o_list_fn, &ctx);
+        }
+    }
+    o_scratch_free(S);
+    free(S);
+    return ctx.n;
+}
+
+/* Count the tets each element refines to (after wrap/reject; 0 if rejected). */
+int orc_count_tets(const orc_mesh* mesh, const orc_grid* g, double reftol, int maxlvl, int32_t* counts) {
+    oscratch* S = (oscratch*)malloc(sizeof(oscratch));
+    o_scratch_init(S);
+    int vpe = mesh->elemtype;
+    double tol = o_tol(g, reftol), tol2 = tol * tol;
+    if (maxlvl > 6) maxlvl = 6;
+    double p[27][3], u[27][3], lo[3], hi[3];
+    for (int64_t e = 0; e < mesh->nelem; ++e) {
+        for (int v = 0; v < vpe; ++v) {
+            int64_t id = mesh->conn[(int64_t)e * vpe + v];
+            for (int a = 0; a < 3; ++a) {
+                p[v][a] = mesh->pos[3 * id + a];
+                u[v][a] = mesh->vel[3 * id + a];
+            }
+        }
+        if (!o_wrap_and_test(p, vpe, lo, hi, g, mesh)) {
+            counts[e] = 0;
+            continue;
+        }
+        o_refine(S, (const double (*)[3])p, (const double (*)[3])u, mesh->mass[e], vpe, maxlvl, tol2);
+        counts[e] = S->ntet;
+    }
+    o_scratch_free(S);
+    free(S);
+    return 0;
+}
+
+/* Voxelize one explicit tet (4 pos, 4 vel, mass) into g, exactly as the
+ * element loop would after refinement (ghosts included). */
+int orc_voxelize_one_tet(const orc_mesh* mesh, orc_grid* g, const double* pos12, const double* vel12, double mass,
+                         int xvfix, orc_stats* st) {
+    orc_stats local;
+    memset(&local, 0, sizeof local);
+    oscratch* S = (oscratch*)malloc(sizeof(oscratch));
+    o_scratch_init(S);
+    double gp[32][3], gb[16][3];
+    memcpy(gp, pos12, 12 * sizeof(double));
+    o_aabb((const double (*)[3])gp, 4, gb[0], gb[1]);
+    int ng = o_ghosts(gp, gb, g, mesh);
+    for (int h = 0; h < ng; ++h)
+        o_voxelize_tet(S, g, (const double (*)[3])&gp[4 * h], (const double (*)[3])vel12, mass,
+                       (const double (*)[3])&gb[2 * h], &local, xvfix, -1, NULL, NULL);
+    o_scratch_free(S);
+    free(S);
+    if (st) *st = local;
+    return 0;
+}

@@ -1,0 +1,52 @@
+/*
+ * psi_oracle.h -- TEST INFRASTRUCTURE ONLY (the oracle).
+ * FP64 CPU restatement of PSI.voxels(mode='density'); see psi_oracle.c.
+ * Only tests/, the smoke check and the CPU-baseline leg of bench.py use it.
+ */
+#ifndef PSI_ORACLE_H
+#define PSI_ORACLE_H
+#include <stdint.h>
+
+/* field slots, same tags as the reference (include/grid.h:37-42) */
+enum { ORC_M = 0, ORC_X = 1, ORC_V = 2, ORC_XX = 3, ORC_XV = 4, ORC_VV = 5 };
+
+typedef struct {
+    int elemtype; /* 4 (tet), 8 (trilinear), 27 (triquadratic) */
+    int64_t npart, nelem;
+    const double* pos;   /* npart x 3 */
+    const double* vel;   /* npart x 3 */
+    const double* mass;  /* nelem */
+    const int32_t* conn; /* nelem x elemtype */
+    int periodic;
+    double box[2][3];
+} orc_mesh;
+
+typedef struct {
+    int n[3];
+    double win[2][3];
+    double d[3];
+    double* f[6]; /* m, x, v, xx, xv, vv; NULL when absent */
+} orc_grid;
+
+typedef struct {
+    int64_t nfrag;      /* kept (ghost, voxel) leaves, m0 > 0 */
+    int64_t ntets;      /* tets after refinement */
+    int64_t nghosts;    /* (tet, periodic copy) pairs voxelized */
+    int64_t nelem_used; /* elements that passed the window test */
+    int maxverts;       /* largest polyhedron produced by a split */
+    int maxstack;       /* deepest DFS stack */
+    int overflow;       /* vertex or stack overflows (0 on sane input) */
+} orc_stats;
+
+int orc_voxels(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlvl, int xvfix, orc_stats* st);
+int orc_voxels_range(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlvl, int xvfix, int64_t e0,
+                     int64_t e1, orc_stats* st);
+int orc_voxels_omp(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlvl, int xvfix, int nthreads, int64_t e0,
+                   int64_t e1, orc_stats* st);
+int64_t orc_list_fragments(const orc_mesh* mesh, const orc_grid* g, double reftol, int maxlvl, int64_t e0, int64_t e1,
+                           int64_t* gi, double* mm, int64_t cap);
+int orc_count_tets(const orc_mesh* mesh, const orc_grid* g, double reftol, int maxlvl, int32_t* counts);
+int orc_voxelize_one_tet(const orc_mesh* mesh, orc_grid* g, const double* pos12, const double* vel12, double mass,
+                         int xvfix, orc_stats* st);
+
+#endif
