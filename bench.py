@@ -1,0 +1,225 @@
+#!/usr/bin/env python3
+"""Benchmark: tet-voxel fragments/s of PSI.voxels on one MI355X.
+
+Workload (default): 128^3 particles on the unit periodic box, each lattice
+cube split into 6 tets (12,582,912 tets), Zel'dovich displacement with
+P(k) ~ k^-2 and rms 2.0 lattice cells (seed 0), voxelized onto a 256^3 grid
+with fields m, v, vv (mass, mean velocity, velocity dispersion tensor).
+
+A step = one voxelization of the whole mesh with inputs resident in HBM:
+zero the grid fields, convert to raw moments, expand elements to tets, the
+voxelization kernel, finalize to the reference layout.  `value` =
+N_frag / (mean step time), N_frag = fragments (copy, cell) with m0 > 0
+counted by the FP64 oracle for this exact input (cached in
+profiles/workload_counts.json; the GPU's own count is reported beside it).
+
+Prints ONE JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic FP32-equivalent flops per fragment of the reference algorithm
+# (canonical fragment: 8 vertices, one split, 12 fan triangles, transform and
+# update), by field set
+FLOPS_PER_FRAG = {"m": 337, "m,v": 550, "m,v,vv": 2287, "all": 3490}
+PEAK_FP32_TFLOPS = 157.3
+
+WORKLOADS = {
+    # name: (lattice n, rms cells, seed, grid n, fields)
+    "zeldovich-128^3-rms2.0-to-256^3-m,v,vv": (128, 2.0, 0, 256, ("v", "vv")),
+    "zeldovich-32^3-rms0.3-to-64^3-m": (32, 0.3, 0, 64, ()),
+    "zeldovich-64^3-rms1.0-to-128^3-m": (64, 1.0, 0, 128, ()),
+}
+DEFAULT = "zeldovich-128^3-rms2.0-to-256^3-m,v,vv"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def field_key(fields):
+    s = ",".join(["m"] + list(fields))
+    return s if s in FLOPS_PER_FRAG else "all"
+
+
+def load_counts():
+    p = os.path.join(ROOT, "profiles", "workload_counts.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return {}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return {}
+
+
+def build_inputs(wname):
+    import numpy as np
+    from pypsi_amd.synth import zeldovich_block
+    n, rms, seed, gn, fields = WORKLOADS[wname]
+    pos, vel = zeldovich_block(n, rms, seed)
+    # trilinear periodic connectivity, corner order 4i+2j+k (src/PSI.c:636-656)
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    conn = np.empty((n, n, n, 8), dtype=np.int32)
+    for ii in range(2):
+        for jj in range(2):
+            for kk in range(2):
+                conn[..., 4 * ii + 2 * jj + kk] = n * n * ((i + ii) % n) + n * ((j + jj) % n) + ((k + kk) % n)
+    conn = conn.reshape(-1, 8)
+    mass = np.ones(conn.shape[0])
+    return pos.reshape(-1, 3), vel.reshape(-1, 3), mass, conn, gn, fields
+
+
+def cpu_baseline(wname, pos, vel, mass, conn, gn, fields, budget_s=12.0):
+    """The reference's own C core (oracle/_ref) single-threaded on the first
+    elements of the same mesh, sized to about budget_s of CPU time; falls back
+    to the oracle's C restatement if the reference build is absent."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    kind = "reference" if O.have_ref() else "port"
+
+    def run(e1):
+        P = O.Problem(pos, vel, mass[:e1], conn[:e1], periodic_box=box, n=(gn,) * 3, bounds=box, fields=fields)
+        out = P.zero_fields()
+        t0 = time.perf_counter()
+        if kind == "reference":
+            O.ref_voxels(P, out=out)
+        else:
+            O.voxels(P, out=out)
+        dt = time.perf_counter() - t0
+        return P, dt
+
+    P, dt = run(512)
+    rate_e = 512 / max(dt, 1e-6)
+    e1 = int(min(conn.shape[0], max(1024, rate_e * budget_s)))
+    P, dt = run(e1)
+    # fragment count of the sample from the oracle (mass field only: the count
+    # does not depend on the field set)
+    Pm = O.Problem(pos, vel, mass[:e1], conn[:e1], periodic_box=box, n=(gn,) * 3, bounds=box)
+    _, st = O.voxels(Pm, threads=-1)
+    nfrag = st["nfrag"]
+    return {"value": nfrag / dt, "unit": "fragments/s", "cores": 1, "kind": kind,
+            "sample": f"first {e1} of {conn.shape[0]} elements ({6 * e1} tets, {nfrag} fragments) of the same "
+                      f"mesh onto the full {gn}^3 grid, fields m{''.join(',' + f for f in fields)}; "
+                      f"{dt:.1f} s single-threaded, -O3 x86-64-v3"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--kernel", default="auto", choices=["auto", "plain", "fast"])
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from pypsi_amd import device as D
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    wname = args.workload
+    log(f"[bench] building inputs for {wname}")
+    pos, vel, mass, conn, gn, fields = build_inputs(wname)
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    dev = torch.device("cuda:0")
+    tpos = torch.from_numpy(pos).to(dev)
+    tvel = torch.from_numpy(vel).to(dev)
+    tmass = torch.from_numpy(mass).to(dev)
+    tconn = torch.from_numpy(conn).to(dev)
+    tf = {"m": torch.zeros((gn,) * 3, dtype=torch.float64, device=dev)}
+    for f in fields:
+        tf[f] = torch.zeros((gn,) * 3 + ((3,) if len(f) == 1 else (3, 3)), dtype=torch.float64, device=dev)
+    mesh = D.make_mesh(tpos, tvel, tmass, tconn, periodic_box=box)
+    grid = D.make_grid((gn,) * 3, box, tf)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = D.make_opts(kernel=args.kernel, stream=stream)
+
+    def step():
+        for t in tf.values():
+            t.zero_()
+        return D.voxels(mesh, grid, opts)
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        st = step()
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = step()
+        kms.append(st["kernel_ms"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    gpu_nfrag = int(st["nfrag"])
+    mass_err = abs(1.0 - float(tf["m"].sum().item()) / float(mass.sum()))
+
+    counts = load_counts()
+    nfrag = counts.get(wname, {}).get("nfrag")
+    nfrag_src = "oracle"
+    if nfrag is None:
+        nfrag, nfrag_src = gpu_nfrag, "gpu (oracle count not cached)"
+    value = nfrag / (ms_step * 1e-3)
+
+    fk = field_key(fields)
+    kavg = sum(kms) / len(kms)
+    achieved = nfrag * FLOPS_PER_FRAG[fk] / (kavg * 1e-3) / 1e12
+    traffic = load_traffic().get(wname, {}).get(st["kernel"] == 1 and "plain" or "fast")
+    out = {
+        "metric": "tet-voxel fragments/s, 128^3 mesh\u2192256^3 grid; plain vs batched, fwd+adjoint",
+        "value": value,
+        "unit": "fragments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 clip, f64 accumulate" if st["kernel"] == 2 else "f64",
+        "data": "synthetic (seeded Zel'dovich lattice)",
+        "config": {"workload": wname, "kernel": ["auto", "plain", "fast"][st["kernel"]], "tets": conn.shape[0] * 6,
+                   "grid": gn, "fields": ["m"] + list(fields), "nfrag": nfrag, "nfrag_source": nfrag_src,
+                   "gpu_nfrag": gpu_nfrag, "mass_rel_err": mass_err, "kernel_ms_avg": kavg},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "flops_per_fragment": FLOPS_PER_FRAG[fk]},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline")
+        try:
+            out["cpu_baseline"] = cpu_baseline(wname, pos, vel, mass, conn, gn, fields)
+        except Exception as e:  # report, never hide
+            out["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

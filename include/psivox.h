@@ -1,0 +1,150 @@
+/*
+ * psivox.h -- C ABI of the MI355X voxelizer behind PSI.voxels().
+ *
+ * Plain C types and pointers only.  The Python module PSI (pypsi_amd/csrc/
+ * psimodule.c) calls these; so can any other host language through its FFI
+ * (ctypes / cffi stubs are shown in INTEGRATION.md).
+ *
+ * Reference interface replaced:
+ *   void psi_voxels(psi_grid* grid, psi_mesh* mesh, psi_rtree* rtree,
+ *                   psi_int mode, psi_real reftol, psi_int max_ref_lvl);
+ *     -- src/cpsi.c:41 (forward declared src/PSI.c:39, called src/PSI.c:1015
+ *        with rtree = NULL), density branch src/cpsi.c:100-112.
+ *   psi_grid (include/grid.h:46-53) -> psivox_grid
+ *   psi_mesh (include/mesh.h:42-52) -> psivox_mesh
+ *
+ * All entry points return 0 on success and a negative PSIVOX_E* code on
+ * failure; psivox_last_error() then gives a message (thread-local).
+ * Results are deposited into the grid's existing contents (the reference
+ * never clears the grid: src/cpsi.c:41-175, src/geometry.c:164-186).
+ */
+#ifndef PSIVOX_H
+#define PSIVOX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSIVOX_ABI_VERSION 1
+
+/* element types: vertices per element (include/mesh.h:26-28) */
+#define PSIVOX_SIMPLEX 4
+#define PSIVOX_LINEAR 8
+#define PSIVOX_QUADRATIC 27
+
+/* field slots, the reference's tags (include/grid.h:37-42) */
+#define PSIVOX_M 0
+#define PSIVOX_X 1
+#define PSIVOX_V 2
+#define PSIVOX_XX 3
+#define PSIVOX_XV 4
+#define PSIVOX_VV 5
+
+/* modes (include/psi.h:71-72) */
+#define PSIVOX_MODE_DENSITY 0
+#define PSIVOX_MODE_ANNIHILATION 1
+
+/* kernels */
+#define PSIVOX_KERNEL_AUTO 0  /* the fastest available (binned FP32 clip) */
+#define PSIVOX_KERNEL_PLAIN 1 /* one thread per tet, FP64 clip */
+#define PSIVOX_KERNEL_FAST 2  /* binned, FP32 voxel-local clip, FP64 deposit */
+
+/* error codes */
+#define PSIVOX_OK 0
+#define PSIVOX_E_ARG -1      /* invalid argument (message says which) */
+#define PSIVOX_E_NODEV -2    /* no usable GPU */
+#define PSIVOX_E_HIP -3      /* a HIP runtime call failed */
+#define PSIVOX_E_NOMEM -4    /* device allocation failed */
+#define PSIVOX_E_OVERFLOW -5 /* a polyhedron exceeded its vertex budget */
+#define PSIVOX_E_MODE -6     /* mode not supported by this build */
+#define PSIVOX_E_CANCELLED -7 /* the progress callback asked to stop */
+
+/* Mesh: same arrays as psi_mesh (include/mesh.h:42-52).  pos/vel are
+ * npart x 3 float64, mass has one entry per element, conn is
+ * nelem x elemtype int32 in the reference's corner order (4i+2j+k for
+ * trilinear elements, 9i+3j+k for triquadratic ones). vel may be NULL
+ * (zero velocities). */
+typedef struct {
+    int32_t elemtype;
+    int32_t periodic; /* 1: positions wrap in box (src/PSI.c:383-396) */
+    int64_t npart;
+    int64_t nelem;
+    const double* pos;
+    const double* vel;
+    const double* mass;
+    const int32_t* conn;
+    double box[2][3];
+    int32_t on_device; /* 1: the four pointers are device pointers */
+    int32_t reserved;
+} psivox_mesh;
+
+/* Grid: a Cartesian window split into n cells, psi_grid (include/grid.h:
+ * 46-53).  fields[k] point to C-ordered float64 arrays in the reference's
+ * layout (src/PSI.c:119-186): m (n0,n1,n2), x and v (n0,n1,n2,3), xx, xv, vv
+ * (n0,n1,n2,3,3); NULL for fields not requested.  m is required. */
+typedef struct {
+    int32_t n[3];
+    int32_t on_device; /* 1: fields are device pointers */
+    double window[2][3];
+    double d[3]; /* cell size; (window[1]-window[0])/n as src/PSI.c:111-113 */
+    double* fields[6];
+} psivox_grid;
+
+/* Called between element batches with the number of elements done; a nonzero
+ * return cancels the call (PSIVOX_E_CANCELLED), as the reference's per-element
+ * PyErr_CheckSignals() does (src/cpsi.c:160-163).  Fields then hold the
+ * batches deposited so far. */
+typedef int (*psivox_progress_fn)(void* ctx, int64_t elements_done, int64_t elements_total);
+
+typedef struct {
+    int32_t mode;             /* PSIVOX_MODE_DENSITY */
+    int32_t refine_max_lvl;   /* clamped to 6 (src/refine.c:89-92) */
+    double refine_tolerance;  /* in grid cells (src/cpsi.c:63-66) */
+    int32_t kernel;           /* PSIVOX_KERNEL_* */
+    int32_t xv_true_covariance; /* 0: reference xv formula (src/geometry.c:139-140) */
+    void* stream;             /* hipStream_t to run on; NULL = library stream */
+    psivox_progress_fn progress; /* may be NULL */
+    void* progress_ctx;
+    int64_t batch_elements;   /* elements per device batch; 0 = automatic */
+} psivox_opts;
+
+typedef struct {
+    int64_t nelem_used; /* elements inside the window after periodic wrap */
+    int64_t ntets;      /* tets after refinement */
+    int64_t nghosts;    /* (tet, periodic copy) pairs */
+    int64_t nfrag;      /* (copy, voxel) fragments deposited, m0 > 0 */
+    double kernel_ms;   /* device time of the voxelization kernels */
+    double total_ms;    /* wall time of the whole call */
+    int32_t kernel;     /* kernel that ran */
+    int32_t reserved;
+} psivox_stats;
+
+/* One-mesh voxelization, PSI.voxels(grid, mesh, mode, refine_tolerance,
+ * refine_max_lvl) -- src/PSI.c:974-1018 -> src/cpsi.c:41-175. */
+int psivox_voxels(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* opts, psivox_stats* stats);
+
+/* Many independent (mesh, grid) pairs in one device pass.  No reference
+ * counterpart: equivalent to calling psi_voxels (src/cpsi.c:41) once per
+ * pair.  stats may be NULL or point to nmesh entries. */
+int psivox_voxels_batched(int32_t nmesh, const psivox_mesh* meshes, psivox_grid* grids, const psivox_opts* opts,
+                          psivox_stats* stats);
+
+/* Adjoint of the mass deposit.  No reference counterpart (the reference has
+ * no derivatives).  Given dL/dm per cell (float64, shaped like grid m),
+ * accumulates dL/dpos (npart x 3 float64) of the mass field m produced by
+ * psivox_voxels on the same mesh and grid (refine_max_lvl 0).  Pointers follow
+ * mesh->on_device / grid->on_device. */
+int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,
+                        const psivox_opts* opts);
+
+const char* psivox_last_error(void);
+int psivox_abi_version(void);
+/* number of visible GPUs (0 if none); never throws */
+int psivox_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

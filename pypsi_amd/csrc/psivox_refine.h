@@ -1,0 +1,228 @@
+/*
+ * psivox_refine.h -- element -> tets on the device (and on the host for
+ * tests): periodic wrap of the element, then the reference's adaptive
+ * refinement (src/refine.c:58-368), written as an explicit per-level DFS
+ * so one thread handles one element without recursion.
+ *
+ * Tet record (TREC doubles): pos[4][3], vel[4][3], mass, valid.
+ */
+#ifndef PSIVOX_REFINE_H
+#define PSIVOX_REFINE_H
+
+#include "psivox_geom.h"
+
+namespace pvx {
+
+constexpr int TREC = 26;
+constexpr int MAXLVL = 6;
+
+/* corners of the 6-tet split of the 3x3x3 lattice, index 9i+3j+k
+ * (src/refine.c:39-41) */
+PVX_HD int tet27(int t, int v) {
+    const int T[6][4] = {{2, 0, 6, 18}, {8, 2, 6, 18}, {8, 20, 2, 18}, {8, 24, 20, 18}, {8, 6, 24, 18}, {8, 26, 20, 24}};
+    return T[t][v];
+}
+
+/* 2x2x2 corners (4i+2j+k) to the 3x3x3 lattice by midpoints along x, then y,
+ * then z (src/refine.c:292-319) */
+PVX_HD void lin27(const double (*in)[3], double (*out)[3]) {
+#pragma clang fp contract(off)
+    for (int a = 0; a < 3; ++a) {
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int k = 0; k < 2; ++k) out[18 * i + 6 * j + 2 * k][a] = in[4 * i + 2 * j + k][a];
+        for (int j = 0; j < 3; j += 2)
+            for (int k = 0; k < 3; k += 2) out[9 + 3 * j + k][a] = 0.5 * (out[3 * j + k][a] + out[18 + 3 * j + k][a]);
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; k += 2)
+                out[9 * i + 3 + k][a] = 0.5 * (out[9 * i + k][a] + out[9 * i + 6 + k][a]);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                out[9 * i + 3 * j + 1][a] = 0.5 * (out[9 * i + 3 * j][a] + out[9 * i + 3 * j + 2][a]);
+    }
+}
+
+/* 27-node quadratic element to the 5x5x5 lattice with the quarter-point
+ * Lagrange weights (3, 6, -1)/8, separably along x, y, z
+ * (src/refine.c:321-368) */
+PVX_HD void quad125(const double (*in)[3], double (*out)[3]) {
+#pragma clang fp contract(off)
+#define PVX_I5(i, j, k) (25 * (i) + 5 * (j) + (k))
+    for (int a = 0; a < 3; ++a) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) out[PVX_I5(2 * i, 2 * j, 2 * k)][a] = in[9 * i + 3 * j + k][a];
+        for (int j = 0; j < 5; j += 2)
+            for (int k = 0; k < 5; k += 2) {
+                out[PVX_I5(1, j, k)][a] = 0.125 * (3.0 * out[PVX_I5(0, j, k)][a] + 6.0 * out[PVX_I5(2, j, k)][a] -
+                                                   out[PVX_I5(4, j, k)][a]);
+                out[PVX_I5(3, j, k)][a] = 0.125 * (3.0 * out[PVX_I5(4, j, k)][a] + 6.0 * out[PVX_I5(2, j, k)][a] -
+                                                   out[PVX_I5(0, j, k)][a]);
+            }
+        for (int i = 0; i < 5; ++i)
+            for (int k = 0; k < 5; k += 2) {
+                out[PVX_I5(i, 1, k)][a] = 0.125 * (3.0 * out[PVX_I5(i, 0, k)][a] + 6.0 * out[PVX_I5(i, 2, k)][a] -
+                                                   out[PVX_I5(i, 4, k)][a]);
+                out[PVX_I5(i, 3, k)][a] = 0.125 * (3.0 * out[PVX_I5(i, 4, k)][a] + 6.0 * out[PVX_I5(i, 2, k)][a] -
+                                                   out[PVX_I5(i, 0, k)][a]);
+            }
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j) {
+                out[PVX_I5(i, j, 1)][a] = 0.125 * (3.0 * out[PVX_I5(i, j, 0)][a] + 6.0 * out[PVX_I5(i, j, 2)][a] -
+                                                   out[PVX_I5(i, j, 4)][a]);
+                out[PVX_I5(i, j, 3)][a] = 0.125 * (3.0 * out[PVX_I5(i, j, 4)][a] + 6.0 * out[PVX_I5(i, j, 2)][a] -
+                                                   out[PVX_I5(i, j, 0)][a]);
+            }
+    }
+#undef PVX_I5
+}
+
+/* does this lattice node need refining? (src/refine.c:245-254, 176-185) */
+PVX_HD bool needs_refine(const double (*p)[3], bool quadratic, double tol2) {
+#pragma clang fp contract(off)
+    const int EL[7][3] = {{2, 4, 6}, {2, 10, 18}, {6, 12, 18}, {8, 13, 18}, {8, 14, 20}, {8, 16, 24}, {20, 22, 24}};
+    const int EQ[19][3] = {{0, 1, 2},   {0, 3, 6},   {0, 9, 18},  {2, 4, 6},   {2, 5, 8},   {2, 10, 18}, {2, 11, 20},
+                           {6, 7, 8},   {6, 12, 18}, {6, 15, 24}, {8, 13, 18}, {8, 14, 20}, {8, 16, 24}, {8, 17, 26},
+                           {18, 19, 20}, {18, 21, 24}, {20, 22, 24}, {20, 23, 26}, {24, 25, 26}};
+    const int ne = quadratic ? 19 : 7;
+    for (int e = 0; e < ne; ++e) {
+        const int* E = quadratic ? EQ[e] : EL[e];
+        double err2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            double mt = 0.5 * (p[E[0]][a] + p[E[2]][a]);
+            double ml = p[E[1]][a];
+            err2 += (mt - ml) * (mt - ml);
+        }
+        if (err2 > tol2) return true;
+    }
+    return false;
+}
+
+/* child c (bits 4i+2j+k) of a lattice node: 27-point lattice of the child */
+PVX_HD void child_node(const double (*p)[3], const double (*u)[3], int c, bool quadratic, double (*cp)[3],
+                       double (*cu)[3], double (*tmp)[3]) {
+    const int ci = (c >> 2) & 1, cj = (c >> 1) & 1, ck = c & 1;
+    if (!quadratic) {
+        double c8p[8][3], c8u[8][3];
+        for (int ii = 0; ii < 2; ++ii)
+            for (int jj = 0; jj < 2; ++jj)
+                for (int kk = 0; kk < 2; ++kk) {
+                    int src = 9 * (ii + ci) + 3 * (jj + cj) + (kk + ck);
+                    for (int a = 0; a < 3; ++a) {
+                        c8p[4 * ii + 2 * jj + kk][a] = p[src][a];
+                        c8u[4 * ii + 2 * jj + kk][a] = u[src][a];
+                    }
+                }
+        lin27((const double (*)[3])c8p, cp);
+        lin27((const double (*)[3])c8u, cu);
+    } else {
+        quad125(p, tmp);
+        for (int ii = 0; ii < 3; ++ii)
+            for (int jj = 0; jj < 3; ++jj)
+                for (int kk = 0; kk < 3; ++kk)
+                    for (int a = 0; a < 3; ++a)
+                        cp[9 * ii + 3 * jj + kk][a] = tmp[25 * (ii + 2 * ci) + 5 * (jj + 2 * cj) + (kk + 2 * ck)][a];
+        quad125(u, tmp);
+        for (int ii = 0; ii < 3; ++ii)
+            for (int jj = 0; jj < 3; ++jj)
+                for (int kk = 0; kk < 3; ++kk)
+                    for (int a = 0; a < 3; ++a)
+                        cu[9 * ii + 3 * jj + kk][a] = tmp[25 * (ii + 2 * ci) + 5 * (jj + 2 * cj) + (kk + 2 * ck)][a];
+    }
+}
+
+PVX_HD void emit6(double* out, const double (*p)[3], const double (*u)[3], double mass) {
+    for (int t = 0; t < 6; ++t) {
+        double* r = out + TREC * t;
+        for (int v = 0; v < 4; ++v)
+            for (int a = 0; a < 3; ++a) {
+                r[3 * v + a] = p[tet27(t, v)][a];
+                r[12 + 3 * v + a] = u[tet27(t, v)][a];
+            }
+        r[24] = mass * (1.0 / 6);
+        r[25] = 1.0;
+    }
+}
+
+/* Per-level DFS state of one element's refinement.  Scratch: L+1 lattice
+ * nodes (27 pos + 27 vel each) and one 125-point temporary. */
+struct RefineScratch {
+    double p[MAXLVL + 1][27][3];
+    double u[MAXLVL + 1][27][3];
+    double tmp[125][3];
+    int child[MAXLVL + 1];
+};
+
+/*
+ * Expand element e into tets.  If out is null only counts.  Returns the tet
+ * count (0 if the element misses the window).  Corners in the mesh's
+ * connectivity order; vpe in {4, 8, 27}.
+ */
+PVX_HD int64_t expand_element(const double* pos, const double* vel, const double* mass, const int32_t* conn,
+                              int vpe, int64_t e, const GridInfo& g, const MeshBox& mb, int maxlvl, double tol2,
+                              RefineScratch& S, double* out) {
+    double P[27][3], U[27][3];
+    for (int v = 0; v < vpe; ++v) {
+        int64_t id = conn[e * vpe + v];
+        for (int a = 0; a < 3; ++a) {
+            P[v][a] = pos[3 * id + a];
+            U[v][a] = vel ? vel[3 * id + a] : 0.0;
+        }
+    }
+    if (!wrap_and_test(P, vpe, g, mb)) return 0;
+    const double m = mass[e];
+    if (vpe == 4) {
+        if (out) {
+            for (int v = 0; v < 4; ++v)
+                for (int a = 0; a < 3; ++a) {
+                    out[3 * v + a] = P[v][a];
+                    out[12 + 3 * v + a] = U[v][a];
+                }
+            out[24] = m;
+            out[25] = 1.0;
+        }
+        return 1;
+    }
+    const bool quad = (vpe == 27);
+    if (quad) {
+        for (int v = 0; v < 27; ++v)
+            for (int a = 0; a < 3; ++a) {
+                S.p[0][v][a] = P[v][a];
+                S.u[0][v][a] = U[v][a];
+            }
+    } else {
+        lin27((const double (*)[3])P, S.p[0]);
+        lin27((const double (*)[3])U, S.u[0]);
+    }
+    if (maxlvl > MAXLVL) maxlvl = MAXLVL;
+    int64_t count = 0;
+    int d = 0;
+    double dm = m;
+    /* visit node d */
+    for (;;) {
+        bool refine = d < maxlvl && needs_refine((const double (*)[3])S.p[d], quad, tol2);
+        if (refine) {
+            S.child[d] = 0;
+        } else {
+            if (out) emit6(out + TREC * count, (const double (*)[3])S.p[d], (const double (*)[3])S.u[d], dm);
+            count += 6;
+            /* ascend to the next unvisited sibling */
+            for (;;) {
+                if (d == 0) return count;
+                d--;
+                dm *= 8.0;
+                if (S.child[d] < 8) break;
+            }
+        }
+        /* descend into child S.child[d] */
+        child_node((const double (*)[3])S.p[d], (const double (*)[3])S.u[d], S.child[d], quad, S.p[d + 1],
+                   S.u[d + 1], S.tmp);
+        S.child[d]++;
+        d++;
+        dm *= 0.125;
+    }
+}
+
+}  // namespace pvx
+
+#endif

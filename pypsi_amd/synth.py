@@ -31,7 +31,7 @@ def _displacement(rng: np.random.Generator, n: int, rms_cells: float) -> np.ndar
     psi = np.empty((n, n, n, 3))
     for a, ka in enumerate((kx, ky, kz)):
         # psi_hat = i k delta_hat / k^2 (sign convention irrelevant for statistics)
-        psi[..., a] = np.fft.irfftn(1j * ka * delta / k2, s=(n, n, n))
+        psi[..., a] = np.fft.irfftn(1j * ka * delta / k2, s=(n, n, n), axes=(0, 1, 2))
     rms = math.sqrt(float(np.mean(np.sum(psi * psi, axis=-1))))
     if rms > 0:
         psi *= (rms_cells / n) / rms

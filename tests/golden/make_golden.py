@@ -1,0 +1,79 @@
+"""Write the golden vectors in this folder from the reference's own C core.
+
+Runs only where the reference checkout is present (it builds oracle/_ref from
+the reference sources with oracle/Makefile).  Each fixture holds its inputs
+(mesh arrays, grid, tolerance, level) and the reference's outputs, bit for bit.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+from pypsi_amd.synth import zeldovich_block  # noqa: E402
+
+BOX = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+ALL = ["x", "v", "xx", "xv", "vv"]
+
+
+def unit_elements(rng):
+    """Seeded versions of the tutorial's perturbed unit tet, trilinear and
+    triquadratic elements (tutorial/helpers.py:48-66, Tutorial.ipynb cell 3)."""
+    tet = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    lin = np.stack(np.meshgrid(*(np.array([0.0, 1.0]),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    quad = np.stack(np.meshgrid(*(np.array([0.0, 0.5, 1.0]),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    out = []
+    for name, p in (("tet", tet), ("trilinear", lin), ("triquadratic", quad)):
+        p = p - 0.5
+        p = p * (0.2 + 0.4 * rng.uniform(size=p.shape))
+        p = p + 0.5
+        out.append((name, p, np.arange(p.shape[0], dtype=np.int32).reshape(1, -1)))
+    return out
+
+
+def save(name, P, reftol, maxlvl, out):
+    arrays = dict(pos=P.pos, vel=P.vel, mass=P.mass, conn=P.conn, periodic=np.array(P.periodic), box=P.box,
+                  n=np.array(P.n), bounds=P.bounds, fields=np.array(P.fields[1:], dtype="U2"),
+                  reftol=np.array(reftol), maxlvl=np.array(maxlvl))
+    for f in ["m", "x", "v", "xx", "xv", "vv"]:
+        arrays["out_" + f] = out[f] if f in out else np.zeros(0)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+
+
+def main():
+    O.build(ref=True)
+    if not O.have_ref():
+        raise SystemExit("reference build missing")
+    rng = np.random.default_rng(2024)
+    # g1: single elements of each type, refined (tolerance 0.05 cells, level 3)
+    for name, p, conn in unit_elements(rng):
+        vel = rng.normal(size=p.shape)
+        P = O.Problem(p, vel, np.array([1.0]), conn, n=(8, 8, 8), bounds=BOX, fields=ALL)
+        save(f"g1_{name}", P, 0.05, 3, O.ref_voxels(P, reftol=0.05, maxlvl=3))
+    # g2: 4^3 periodic block -> 8^3, all fields, levels 0 and 2
+    pos, vel = zeldovich_block(4, 1.5, seed=11)
+    for lvl in (0, 2):
+        P = O.Problem.block(pos, vel, periodic_box=BOX, n=(8, 8, 8), bounds=BOX, fields=ALL)
+        save(f"g2_block4_L{lvl}", P, 0.1, lvl, O.ref_voxels(P, reftol=0.1, maxlvl=lvl))
+    # g3: 8^3 lattice, rms 0.3 cells, seed 0 -> 16^3, mass only
+    pos, vel = zeldovich_block(8, 0.3, seed=0)
+    P = O.Problem.block(pos, vel, periodic_box=BOX, n=(16, 16, 16), bounds=BOX)
+    save("g3_block8_m", P, 1.0, 0, O.ref_voxels(P))
+    # g4: open mesh onto a zoom window cutting through it
+    pos, vel = zeldovich_block(6, 2.0, seed=5)
+    P = O.Problem.block(pos, vel, n=(9, 7, 8), bounds=((0.15, 0.2, 0.1), (0.75, 0.8, 0.7)), fields=["v", "vv"])
+    save("g4_window_open", P, 1.0, 0, O.ref_voxels(P))
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,78 @@
+/*
+ * hostsim.cpp -- TEST HARNESS: the device algorithm of libpsivox (element
+ * expansion, FP64 sweep, raw deposit, finalize; pypsi_amd/csrc/*.h) compiled
+ * for the host with plain += deposits, so the CPU test suite can check the
+ * kernel logic against the oracle without a GPU.  Not used by the product.
+ */
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../pypsi_amd/csrc/psivox_geom.h"
+#include "../../pypsi_amd/csrc/psivox_moments.h"
+#include "../../pypsi_amd/csrc/psivox_plain.h"
+#include "../../pypsi_amd/csrc/psivox_refine.h"
+
+using namespace pvx;
+
+struct HostSink {
+    double* acc;
+    void add(int64_t i, double v) { acc[i] += v; }
+};
+
+extern "C" int hs_voxels_plain(int vpe, int64_t npart, int64_t nelem, const double* pos, const double* vel,
+                               const double* mass, const int32_t* conn, int periodic, const double* box6,
+                               const int* n, const double* win6, const double* d, double** fields, int xvfix,
+                               double reftol, int maxlvl, int64_t* out_stats) {
+    (void)npart;
+    GridInfo g;
+    for (int a = 0; a < 3; ++a) {
+        g.n[a] = n[a];
+        g.w0[a] = win6[a];
+        g.w1[a] = win6[3 + a];
+        g.d[a] = d[a];
+    }
+    g.flags = (fields[1] ? F_X : 0) | (fields[2] ? F_V : 0) | (fields[3] ? F_XX : 0) | (fields[4] ? F_XV : 0) |
+              (fields[5] ? F_VV : 0) | (xvfix ? F_XVFIX : 0);
+    set_layout(g);
+    MeshBox mb;
+    mb.periodic = periodic;
+    for (int a = 0; a < 3; ++a) {
+        mb.lo[a] = box6[a];
+        mb.hi[a] = box6[3 + a];
+    }
+    const int64_t nvox = (int64_t)n[0] * n[1] * n[2];
+    std::vector<double> acc((size_t)nvox * g.nacc, 0.0);
+    FieldPtrs F;
+    for (int f = 0; f < 6; ++f) F.f[f] = fields[f];
+    for (int64_t c = 0; c < nvox; ++c) cell_to_raw(g, F, acc.data(), c);
+    double dxmin = d[0] < d[1] ? d[0] : d[1];
+    dxmin = dxmin < d[2] ? dxmin : d[2];
+    double tol2 = (reftol * dxmin) * (reftol * dxmin);
+    if (vpe == 4) maxlvl = 0;
+    RefineScratch* RS = new RefineScratch;
+    PlainScratch* W = new PlainScratch;
+    HostSink sink{acc.data()};
+    std::vector<double> tets;
+    int64_t nfrag = 0, ncopies = 0, ntets = 0;
+    unsigned status = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        int64_t nt = expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, nullptr);
+        if (nt == 0) continue;
+        tets.resize((size_t)nt * TREC);
+        expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, tets.data());
+        for (int64_t t = 0; t < nt; ++t) {
+            nfrag += voxelize_tet64(sink, g, mb, &tets[(size_t)t * TREC], *W, status, &ncopies);
+        }
+        ntets += nt;
+    }
+    for (int64_t c = 0; c < nvox; ++c) cell_finalize(g, acc.data(), F, c);
+    delete RS;
+    delete W;
+    out_stats[0] = nfrag;
+    out_stats[1] = ncopies;
+    out_stats[2] = ntets;
+    out_stats[3] = status;
+    return 0;
+}
