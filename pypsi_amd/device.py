@@ -55,6 +55,8 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             raise RuntimeError(f"{LIB} is not built; run `python -m pypsi_amd.build`")
+        from pypsi_amd import preload_hip_runtime
+        preload_hip_runtime()
         L = C.CDLL(LIB)
         P = C.c_void_p
         L.psivox_voxels.argtypes = [P, P, P, P]

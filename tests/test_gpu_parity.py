@@ -1,16 +1,17 @@
 """GPU parity: PSI.voxels on the MI355X (through the C ABI) against the FP64
 oracle on the same seeded inputs.
 
-Tolerances (FP64 plain kernel): fragment count identical; per-cell mass
-within 1e-12 relative; means within 1e-9; covariances within 1e-7 of the
-cell's own diagonal scale on cells above 1e-9 of the peak mass; total mass
-conserved to 1e-12.  The FP32-clip fast kernel is held to the stated FP32 bar
+Tolerances (FP64 plain kernel), with the measures of tests/helpers.py:
+fragment count identical; on cells holding >= 1e-3 of the peak mass, mass
+within 1e-12 relative, means within 1e-10, covariances within 1e-7 of their
+diagonal scale; mass-weighted absolute errors over all cells within 1e-13 of
+the field's peak; total mass conserved to 1e-12.  The FP32-clip fast kernel is held to the stated FP32 bar
 in test_gpu_fast.py.
 """
 import numpy as np
 import pytest
 
-from helpers import ALL, BOX, field_errors
+from helpers import ALL, BOX, assert_close
 
 pytestmark = pytest.mark.gpu
 
@@ -39,13 +40,13 @@ def oracle_run(O, pos, vel, n, fields, bounds=BOX, periodic=True, reftol=1.0, ma
     return O.voxels(P, reftol=reftol, maxlvl=maxlvl)
 
 
-TOL = {"m": 1e-12, "x": 1e-9, "v": 1e-9, "xx": 1e-7, "xv": 1e-7, "vv": 1e-7}
+TOL = {"m": (1e-12, 1e-13), "x": (1e-10, 1e-13), "v": (1e-10, 1e-13), "xx": (1e-7, 1e-12),
+       "xv": (1e-7, 1e-12), "vv": (1e-7, 1e-12)}
 
 
 def check_close(grid, ref, tol=TOL):
-    errs = field_errors(grid.fields, ref, ref["m"])
-    for f, e in errs.items():
-        assert e <= tol[f], f"field {f}: error {e:.3e} > {tol[f]:.0e} ({errs})"
+    errs = assert_close(grid.fields, ref, tol)
+    print("errors (rel on heavy cells, mass-weighted abs):", {k: "%.1e/%.1e" % v for k, v in errs.items()})
     return errs
 
 
@@ -123,9 +124,7 @@ def test_device_pointer_path(oracle_mod):
     ref, ost = oracle_run(oracle_mod, pos, vel, (32,) * 3, ["v", "vv"])
     assert st["nfrag"] == ost["nfrag"]
     got = {k: v.cpu().numpy() for k, v in tf.items()}
-    errs = field_errors(got, ref, ref["m"])
-    for f, e in errs.items():
-        assert e <= TOL[f], (f, e)
+    assert_close(got, ref, TOL)
 
 
 def test_simplex_and_quadratic_elements(PSI, oracle_mod):
