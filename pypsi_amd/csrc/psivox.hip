@@ -197,6 +197,50 @@ __global__ __launch_bounds__(256) void k_vox_plain(const double* __restrict__ te
     block_count(nf, nc, counters);
 }
 
+/* exact in-window mass fraction of each tet (only when the window is not
+ * the whole periodic box) */
+__global__ __launch_bounds__(64) void k_targets(double* __restrict__ tets, int64_t ntet, GridInfo g, MeshBox mb) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntet) return;
+    double* r = tets + (int64_t)TREC * t;
+    if (r[25] == 0.0) return;
+    Poly64 pool[2];
+    r[26] = window_target(r, g, mb, pool);
+}
+
+/* one thread per tet record, FP32 column clip in LDS, FP64 deposit */
+template <int ORDER>
+__global__ __launch_bounds__(FBLOCK, 2) void k_vox_fast(const double* __restrict__ tets, int64_t ntet, GridInfo g,
+                                                     MeshBox mb, double* __restrict__ acc, unsigned* status,
+                                                     unsigned long long* counters) {
+    __shared__ float sq[3][NVF][FBLOCK];
+    __shared__ uint32_t snb[NVF][FBLOCK];
+    const int lane = threadIdx.x;
+    LPoly P{&sq[0][0][lane], &sq[1][0][lane], &sq[2][0][lane], &snb[0][lane], 0};
+    const int64_t t = (int64_t)blockIdx.x * FBLOCK + lane;
+    FastCounters cnt{0, 0};
+    if (t < ntet) {
+        const double* r = tets + (int64_t)TREC * t;
+        if (r[25] != 0.0) {
+            AtomicSink sink{acc};
+            if (!fast_tet<ORDER>(sink, g, mb, r, P, cnt)) atomicOr(status, ST_FALLBACK);
+        }
+    }
+    block_count(cnt.nfrag, cnt.ncopies, counters);
+}
+
+static int launch_fast(const double* tets, int64_t nt, const GridInfo& g, const MeshBox& mb, double* acc,
+                       unsigned* status, unsigned long long* counters, hipStream_t s) {
+    const unsigned nb = (unsigned)((nt + FBLOCK - 1) / FBLOCK);
+    if (g.order == 0)
+        k_vox_fast<0><<<nb, FBLOCK, 0, s>>>(tets, nt, g, mb, acc, status, counters);
+    else if (g.order == 1)
+        k_vox_fast<1><<<nb, FBLOCK, 0, s>>>(tets, nt, g, mb, acc, status, counters);
+    else
+        k_vox_fast<2><<<nb, FBLOCK, 0, s>>>(tets, nt, g, mb, acc, status, counters);
+    return (int)hipGetLastError();
+}
+
 /* ------------------------------------------------------------------ */
 /* host side                                                           */
 /* ------------------------------------------------------------------ */
@@ -295,15 +339,17 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     if (o->mode == PSIVOX_MODE_ANNIHILATION)
         return set_err(PSIVOX_E_MODE, "mode 'annihilation' is not supported by the GPU build");
     if (o->mode != PSIVOX_MODE_DENSITY) return set_err(PSIVOX_E_ARG, "invalid mode %d", o->mode);
-    int kernel = o->kernel == PSIVOX_KERNEL_AUTO ? PSIVOX_KERNEL_AUTO_CHOICE : o->kernel;
+    int kernel = o->kernel == PSIVOX_KERNEL_AUTO ? PSIVOX_KERNEL_FAST : o->kernel;
     if (kernel != PSIVOX_KERNEL_PLAIN && kernel != PSIVOX_KERNEL_FAST)
         return set_err(PSIVOX_E_ARG, "invalid kernel %d", o->kernel);
 
     MeshBox mb;
     mb.periodic = mesh->periodic ? 1 : 0;
+    mb.full_window = mb.periodic;
     for (int a = 0; a < 3; ++a) {
         mb.lo[a] = mesh->box[0][a];
         mb.hi[a] = mesh->box[1][a];
+        if (mb.lo[a] != g.w0[a] || mb.hi[a] != g.w1[a]) mb.full_window = 0;
     }
     const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
     const int vpe = mesh->elemtype;
@@ -367,6 +413,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     const int64_t ne = mesh->nelem;
     int64_t ntets_total = 0;
     double kms = 0.0;
+    int fell_back = 0;
 
     /* refinement counts and offsets (levels > 0) */
     int64_t* counts = nullptr;
@@ -418,6 +465,10 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 k_expand_fixed<<<grid_blocks(e1 - e0, 128), 128, 0, s>>>(md, g, mb, e0, e1 - e0, tets);
             }
             HIPCHK(hipGetLastError());
+            if (kernel == PSIVOX_KERNEL_FAST && !(mb.periodic && mb.full_window)) {
+                k_targets<<<grid_blocks(nt, 64), 64, 0, s>>>(tets, nt, g, mb);
+                HIPCHK(hipGetLastError());
+            }
             HIPCHK(hipEventRecord(W.ev0, s));
             if (kernel == PSIVOX_KERNEL_PLAIN) {
                 k_vox_plain<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, acc, status, counters);
@@ -432,6 +483,23 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
             HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
             kms += ms;
             ntets_total += nt;
+            if (kernel == PSIVOX_KERNEL_FAST) {
+                unsigned hs = 0;
+                HIPCHK(hipMemcpy(&hs, status, sizeof hs, hipMemcpyDeviceToHost));
+                if (hs & ST_FALLBACK) {
+                    /* a tet exceeded the FP32 path's vertex budget: redo the
+                     * whole call with the FP64 kernel from the untouched fields */
+                    kernel = PSIVOX_KERNEL_PLAIN;
+                    fell_back = 1;
+                    HIPCHK(hipMemsetAsync(counters, 0, 64, s));
+                    k_to_raw<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, fptrs, acc, nvox);
+                    HIPCHK(hipGetLastError());
+                    e0 = 0;
+                    kms = 0.0;
+                    ntets_total = 0;
+                    continue;
+                }
+            }
         }
         e0 = e1;
         if (o->progress && o->progress(o->progress_ctx, e0, ne)) {
@@ -465,6 +533,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
         st->ntets = ntets_total;
         st->kernel_ms = kms;
         st->kernel = kernel;
+        st->reserved = fell_back;
         st->total_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
         st->nelem_used = -1;

@@ -1,18 +1,766 @@
 /*
- * psivox_fast.h -- binned FP32 voxel-local voxelization (work in progress:
- * until it lands, the automatic kernel choice is the FP64 plain kernel and
- * an explicit request for the fast kernel is refused).
+ * psivox_fast.h -- the fast voxelization kernel: FP32 clipping in cell-local
+ * coordinates, FP64 deposit, one thread per tet (all its periodic copies).
+ *
+ * Per copy, for every (x, y) column of its index box the tet is clipped by
+ * the column's four planes into a polyhedron held in this lane's slice of
+ * LDS (vertex graph with 3 neighbours per vertex as in the reference,
+ * include/geometry.h:118-129, but storing only the FP32 mass coordinate q:
+ * positions are x(q) = x0 + DX q in grid units relative to the copy's index
+ * box).  Each cell of the column is then integrated without building its
+ * fragment: the column's faces are walked once (src/geometry.c:692-767),
+ * each face polygon clipped on the fly to the cell's z-slab and
+ * fan-triangulated, and the two cap faces closed through a point on each
+ * plane.  All moments are taken about the column's first vertex in q-space.
+ *
+ * Differences from the reference arithmetic (the method is the same):
+ *   - cuts and moments in FP32 on cell-local coordinates; a vertex within
+ *     1e-5 cells of a plane is classified with an FP64 evaluation of its
+ *     signed distance;
+ *   - each column is clipped from the tet rather than from its neighbour
+ *     slab, so shared faces of neighbouring columns agree only to FP32
+ *     rounding; the per-tet mass residual (FP64 exact target minus FP64
+ *     sum of the FP32 fragment masses) goes into the tet's largest fragment,
+ *     which conserves total mass to FP64 rounding;
+ *   - deposits are the same additive FP64 raw moments as the FP64 kernel.
  */
 #ifndef PSIVOX_FAST_H
 #define PSIVOX_FAST_H
 
 #include "psivox_geom.h"
 
-#define PSIVOX_KERNEL_AUTO_CHOICE PSIVOX_KERNEL_PLAIN
+#ifndef PVX_DBG
+#define PVX_DBG(...)
+#endif
 
-static inline int launch_fast(const double*, int64_t, const pvx::GridInfo&, const pvx::MeshBox&, double*, unsigned*,
-                              unsigned long long*, hipStream_t) {
-    return (int)hipErrorNotSupported;
+namespace pvx {
+
+constexpr int NVF = 18;   /* vertex slots of the column polyhedron */
+constexpr int FBLOCK = 64; /* lanes per workgroup (one wavefront) */
+
+/* A lane's polyhedron: q components and packed neighbours (3 x 8 bits) at
+ * stride FBLOCK so the 64 lanes of a wavefront touch consecutive words. */
+struct LPoly {
+    float* q0;
+    float* q1;
+    float* q2;
+    uint32_t* nb;
+    int nv;
+    PVX_HD float qa(int v, int a) const { return (a == 0 ? q0 : (a == 1 ? q1 : q2))[v * FBLOCK]; }
+    PVX_HD void setq(int v, float a, float b, float c) {
+        q0[v * FBLOCK] = a;
+        q1[v * FBLOCK] = b;
+        q2[v * FBLOCK] = c;
+    }
+    PVX_HD int nbr(int v, int k) const { return (int)((nb[v * FBLOCK] >> (8 * k)) & 0xffu); }
+    PVX_HD uint32_t nbw(int v) const { return nb[v * FBLOCK]; }
+    PVX_HD void setnbw(int v, uint32_t w) { nb[v * FBLOCK] = w; }
+    PVX_HD void setnbr(int v, int k, int val) {
+        uint32_t w = nb[v * FBLOCK];
+        w = (w & ~(0xffu << (8 * k))) | ((uint32_t)val << (8 * k));
+        nb[v * FBLOCK] = w;
+    }
+};
+
+PVX_HD uint32_t pack3(int a, int b, int c) { return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16); }
+
+/* The copy being processed: geometry in grid units relative to its index
+ * box corner in FP32; the FP64 tie-break re-reads the tet record. */
+struct CopyGeo {
+    float x0[3];
+    float DX[3][3]; /* DX[a][b] = x_{b+1,a} - x_{0,a} */
+    const double* rec;
+    double shift[3];
+    int ib0[3];
+};
+
+/* pick component ax of a 3-vector without dynamic indexing (keeps the
+ * values in registers) */
+template <typename T>
+PVX_HD T pick3(const T* v, int ax) {
+    return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]);
 }
+
+/* one axis of the copy's affine map, x_ax(q) = o + d0 q0 + d1 q1 + d2 q2,
+ * plus what the FP64 tie-break needs */
+struct AxisMap {
+    float o, d0, d1, d2;
+    int ax;
+    double shift, w0, dinv;
+    int ib0;
+};
+
+PVX_HD AxisMap axis_map(const CopyGeo& G, const GridInfo& g, int ax) {
+    AxisMap m;
+    m.ax = ax;
+    m.o = pick3(G.x0, ax);
+    m.d0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]);
+    m.d1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]);
+    m.d2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
+    m.shift = pick3(G.shift, ax);
+    m.w0 = pick3(g.w0, ax);
+    m.dinv = 1.0 / pick3(g.d, ax);
+    m.ib0 = pick3(G.ib0, ax);
+    return m;
+}
+
+/* FP64 c - x_ax(q) from the record: positions shifted by the copy's
+ * periodic offset, in grid units relative to the index box corner */
+PVX_HD double sdist64(const double* r, const AxisMap& m, float c, float q0, float q1, float q2) {
+    const double ra = m.ax == 0 ? r[0] : (m.ax == 1 ? r[1] : r[2]);
+    const double rb = m.ax == 0 ? r[3] : (m.ax == 1 ? r[4] : r[5]);
+    const double rc = m.ax == 0 ? r[6] : (m.ax == 1 ? r[7] : r[8]);
+    const double rd = m.ax == 0 ? r[9] : (m.ax == 1 ? r[10] : r[11]);
+    const double xa = (ra + m.shift) + (rb - ra) * (double)q0 + (rc - ra) * (double)q1 + (rd - ra) * (double)q2;
+    return (double)c - ((xa - m.w0) * m.dinv - (double)m.ib0);
+}
+
+/* signed distance c - x_ax(q) in cells; re-evaluated in FP64 from the
+ * record when the FP32 value is within 1e-5 cells of the plane */
+PVX_HD float sdist(const double* rec, const AxisMap& m, float c, float q0, float q1, float q2) {
+    float s = c - (m.o + m.d0 * q0 + m.d1 * q1 + m.d2 * q2);
+    if (fabsf(s) < 1e-5f) {
+        double sd = sdist64(rec, m, c, q0, q1, q2);
+        s = (sd < 0.0) ? fminf(-1e-30f, (float)sd) : fmaxf(0.0f, (float)sd);
+    }
+    return s;
+}
+
+/*
+ * Clip P in place to one side of the plane x_ax = c.  keep_upper: keep
+ * x > c (drop x <= c); otherwise keep x <= c.  New vertices are appended
+ * (the graph is the reference's, src/geometry.c:391-501), linked by the
+ * face walk, then everything is compacted.  Only registers and the lane's
+ * LDS slice are used.  Returns false on slot overflow or a broken graph.
+ */
+PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, float c, int ax, bool keep_upper) {
+    const int onv = P.nv;
+    if (onv <= 0) return true;
+    const AxisMap m = axis_map(G, g, ax);
+    uint32_t gone = 0;
+    for (int v = 0; v < onv; ++v) {
+        float t = sdist(G.rec, m, c, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
+        bool drop = keep_upper ? (t >= 0.0f) : (t < 0.0f);
+        if (drop) gone |= 1u << v;
+    }
+    if (gone == 0) return true;
+    if (gone == ((1u << onv) - 1u)) {
+        P.nv = 0;
+        return true;
+    }
+    int tot = onv;
+    for (int vc = 0; vc < onv; ++vc) {
+        if (gone & (1u << vc)) continue;
+        for (int k = 0; k < 3; ++k) {
+            int vn = P.nbr(vc, k);
+            if (vn >= onv) {
+                PVX_DBG("clip: bad nbr %d (onv %d)\n", vn, onv);
+                return false;
+            }
+            if (!(gone & (1u << vn))) continue;
+            if (tot >= NVF) {
+                PVX_DBG("clip: overflow onv %d\n", onv);
+                return false;
+            }
+            const float c0 = P.qa(vc, 0), c1 = P.qa(vc, 1), c2 = P.qa(vc, 2);
+            const float n0 = P.qa(vn, 0), n1 = P.qa(vn, 1), n2 = P.qa(vn, 2);
+            const float sc = sdist(G.rec, m, c, c0, c1, c2), sn = sdist(G.rec, m, c, n0, n1, n2);
+            const float inv = 1.0f / (sc - sn);
+            P.setq(tot, inv * (n0 * sc - c0 * sn), inv * (n1 * sc - c1 * sn), inv * (n2 * sc - c2 * sn));
+            P.setnbw(tot, pack3(vc, 0xff, 0xff));
+            P.setnbr(vc, k, tot);
+            tot++;
+        }
+    }
+    /* link each new vertex to the next one around its face */
+    for (int vs = onv; vs < tot; ++vs) {
+        int vcur = vs, vnext = P.nbr(vs, 0);
+        int guard = 0;
+        for (;;) {
+            if (vnext >= tot || ++guard > 2 * NVF) {
+                PVX_DBG("clip: walk lost vnext %d tot %d guard %d\n", vnext, tot, guard);
+                return false;
+            }
+            uint32_t w = P.nbw(vnext);
+            int k = 0;
+            while (k < 3 && (int)((w >> (8 * k)) & 0xffu) != vcur) k++;
+            if (k == 3) {
+                PVX_DBG("clip: walk no back link %d -> %d\n", vcur, vnext);
+                return false;
+            }
+            vcur = vnext;
+            if (vcur >= onv) break;
+            vnext = (int)((w >> (8 * ((k + 1) % 3))) & 0xffu);
+        }
+        P.setnbr(vs, 2, vcur);
+        P.setnbr(vcur, 1, vs);
+    }
+    /* compact: survivors keep their order (rank among survivors), new
+     * vertices follow; ranks come from the bit mask, no index table */
+    const uint32_t keep = ~gone & ((1u << onv) - 1u);
+    const int nk0 = __builtin_popcount(keep);
+    for (int v = 0; v < tot; ++v) {
+        if (v < onv && (gone & (1u << v))) continue;
+        uint32_t w = P.nbw(v);
+        uint32_t o = 0;
+        for (int k = 0; k < 3; ++k) {
+            int n = (int)((w >> (8 * k)) & 0xffu);
+            if (n >= tot) {
+                PVX_DBG("clip: compaction bad index %d (tot %d) at v %d\n", n, tot, v);
+                return false;
+            }
+            int r = n < onv ? __builtin_popcount(keep & ((1u << n) - 1u)) : nk0 + (n - onv);
+            o |= (uint32_t)r << (8 * k);
+        }
+        int d = v < onv ? __builtin_popcount(keep & ((1u << v) - 1u)) : nk0 + (v - onv);
+        if (d != v) P.setq(d, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
+        P.setnbw(d, o);
+    }
+    P.nv = nk0 + (tot - onv);
+    return true;
+}
+
+/* q-moments accumulator about an origin: m0 = 6 vol, m1 = sum d S / 4,
+ * M2 = sum d (AA^T + BB^T + CC^T + SS^T) / 20 with S = A+B+C, over signed
+ * tets (origin, A, B, C); same integrals as src/geometry.c:725-740 */
+struct Mom {
+    float m0, s1[3], s2[6];
+    PVX_HD void zero() {
+        m0 = 0.f;
+        for (int i = 0; i < 3; ++i) s1[i] = 0.f;
+        for (int i = 0; i < 6; ++i) s2[i] = 0.f;
+    }
+};
+
+template <int ORDER>
+PVX_HD void tri_acc(Mom& M, const float* A, const float* B, const float* C) {
+    float d = A[0] * (B[1] * C[2] - B[2] * C[1]) - A[1] * (B[0] * C[2] - B[2] * C[0]) + A[2] * (B[0] * C[1] - B[1] * C[0]);
+    M.m0 += d;
+    if (ORDER > 0) {
+        float S0 = A[0] + B[0] + C[0], S1 = A[1] + B[1] + C[1], S2 = A[2] + B[2] + C[2];
+        M.s1[0] += d * S0;
+        M.s1[1] += d * S1;
+        M.s1[2] += d * S2;
+        if (ORDER > 1) {
+            M.s2[0] += d * (A[0] * A[0] + B[0] * B[0] + C[0] * C[0] + S0 * S0);
+            M.s2[1] += d * (A[0] * A[1] + B[0] * B[1] + C[0] * C[1] + S0 * S1);
+            M.s2[2] += d * (A[0] * A[2] + B[0] * B[2] + C[0] * C[2] + S0 * S2);
+            M.s2[3] += d * (A[1] * A[1] + B[1] * B[1] + C[1] * C[1] + S1 * S1);
+            M.s2[4] += d * (A[1] * A[2] + B[1] * B[2] + C[1] * C[2] + S1 * S2);
+            M.s2[5] += d * (A[2] * A[2] + B[2] * B[2] + C[2] * C[2] + S2 * S2);
+        }
+    }
+}
+
+/*
+ * Moments of P intersected with the slab lo < x_ax <= hi (either side may be
+ * open), about the origin O (q of P's vertex 0).  Every face of P is walked
+ * once in the reference's order (src/geometry.c:708-746); its polygon
+ * clipped to the slab is fanned from its first kept point with the
+ * reference's orientation (triangle (first, next, current)); each clipped
+ * face leaves an edge exit -> entry on a slab plane, and the caps, which
+ * run those edges the other way, are fanned from a fixed point on each plane.
+ */
+template <int ORDER>
+PVX_HD void slab_moments(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, float lo, bool has_hi, float hi,
+                         Mom& M, float* O) {
+    M.zero();
+    const int nv = P.nv;
+    const float zx = pick3(G.x0, ax), zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
+                zd1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]),
+                zd2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
+    /* origin: the vertex nearest the slab, so the moments of the cell's
+     * piece are taken about a nearby point (small FP32 cancellation) */
+    {
+        const float zc = has_lo ? (has_hi ? 0.5f * (lo + hi) : lo + 0.5f) : (has_hi ? hi - 0.5f : 0.0f);
+        int vb = 0;
+        float db = 3.0e38f;
+        for (int v = 0; v < nv; ++v) {
+            const float z = zx + zd0 * P.qa(v, 0) + zd1 * P.qa(v, 1) + zd2 * P.qa(v, 2);
+            const float dd = fabsf(z - zc);
+            if (dd < db) {
+                db = dd;
+                vb = v;
+            }
+        }
+        O[0] = P.qa(vb, 0);
+        O[1] = P.qa(vb, 1);
+        O[2] = P.qa(vb, 2);
+    }
+    const float O0 = O[0], O1 = O[1], O2 = O[2];
+    uint64_t mark = 0;
+    float clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+    bool clo_set = false, chi_set = false;
+    for (int vs = 0; vs < nv; ++vs)
+        for (int ks = 0; ks < 3; ++ks) {
+            if (mark & (1ull << (3 * vs + ks))) continue;
+            int a = vs, ka = ks;
+            float a0 = P.qa(a, 0), a1 = P.qa(a, 1), a2 = P.qa(a, 2);
+            float za = zx + zd0 * a0 + zd1 * a1 + zd2 * a2;
+            a0 -= O0;
+            a1 -= O1;
+            a2 -= O2;
+            float e0[3] = {0, 0, 0}, ep[3] = {0, 0, 0}, xp[3] = {0, 0, 0};
+            int nem = 0;
+            int exit_plane = -1; /* pending exit on a plane: 0 lo, 1 hi */
+            int guard = 0;
+            for (;;) {
+                mark |= 1ull << (3 * a + ka);
+                const int b = P.nbr(a, ka);
+                float b0 = P.qa(b, 0), b1 = P.qa(b, 1), b2 = P.qa(b, 2);
+                const float zb = zx + zd0 * b0 + zd1 * b1 + zd2 * b2;
+                b0 -= O0;
+                b1 -= O1;
+                b2 -= O2;
+                float t0 = 0.f, t1 = 1.f;
+                int p0 = -1, p1 = -1;
+                bool empty = false;
+                const float dz = zb - za;
+                if (has_lo) { /* keep z > lo */
+                    if (za <= lo && zb <= lo) {
+                        empty = true;
+                    } else if (za <= lo) {
+                        t0 = (lo - za) / dz;
+                        p0 = 0;
+                    } else if (zb <= lo) {
+                        t1 = (lo - za) / dz;
+                        p1 = 0;
+                    }
+                }
+                if (!empty && has_hi) { /* keep z <= hi */
+                    if (za > hi && zb > hi) {
+                        empty = true;
+                    } else if (za > hi) {
+                        float t = (hi - za) / dz;
+                        if (t > t0) {
+                            t0 = t;
+                            p0 = 1;
+                        }
+                    } else if (zb > hi) {
+                        float t = (hi - za) / dz;
+                        if (t < t1) {
+                            t1 = t;
+                            p1 = 1;
+                        }
+                    }
+                }
+                /* a touching interval (t0 == t1, e.g. a on a plane) still emits
+                 * its point, as Sutherland-Hodgman emits an inside vertex */
+                if (!empty && t0 <= t1) {
+                    float pt[3];
+                    if (p0 < 0) {
+                        pt[0] = a0;
+                        pt[1] = a1;
+                        pt[2] = a2;
+                    } else {
+                        pt[0] = a0 + t0 * (b0 - a0);
+                        pt[1] = a1 + t0 * (b1 - a1);
+                        pt[2] = a2 + t0 * (b2 - a2);
+                        if (exit_plane == p0) { /* cap edge exit -> entry, run entry -> exit by the cap */
+                            if (p0 == 0) {
+                                if (!clo_set) {
+                                    clo[0] = xp[0];
+                                    clo[1] = xp[1];
+                                    clo[2] = xp[2];
+                                    clo_set = true;
+                                }
+                                tri_acc<ORDER>(M, clo, xp, pt);
+                            } else {
+                                if (!chi_set) {
+                                    chi[0] = xp[0];
+                                    chi[1] = xp[1];
+                                    chi[2] = xp[2];
+                                    chi_set = true;
+                                }
+                                tri_acc<ORDER>(M, chi, xp, pt);
+                            }
+                            exit_plane = -1;
+                        }
+                    }
+                    if (nem == 0) {
+                        e0[0] = pt[0];
+                        e0[1] = pt[1];
+                        e0[2] = pt[2];
+                    } else if (nem >= 2) {
+                        tri_acc<ORDER>(M, e0, pt, ep);
+                    }
+                    ep[0] = pt[0];
+                    ep[1] = pt[1];
+                    ep[2] = pt[2];
+                    nem++;
+                    if (p1 >= 0) {
+                        float pe[3] = {a0 + t1 * (b0 - a0), a1 + t1 * (b1 - a1), a2 + t1 * (b2 - a2)};
+                        if (nem >= 2) tri_acc<ORDER>(M, e0, pe, ep);
+                        ep[0] = pe[0];
+                        ep[1] = pe[1];
+                        ep[2] = pe[2];
+                        nem++;
+                        exit_plane = p1;
+                        xp[0] = pe[0];
+                        xp[1] = pe[1];
+                        xp[2] = pe[2];
+                    }
+                }
+                /* next edge of the face: from b, the neighbour after a */
+                const uint32_t wb = P.nbw(b);
+                int kb = 0;
+                while (kb < 2 && (int)((wb >> (8 * kb)) & 0xffu) != a) kb++;
+                a = b;
+                ka = (kb + 1) % 3;
+                a0 = b0;
+                a1 = b1;
+                a2 = b2;
+                za = zb;
+                if ((a == vs && ka == ks) || ++guard > 3 * NVF) break;
+            }
+            /* walk began outside: the last exit joins the first entry e0 */
+            if (exit_plane >= 0 && nem > 0) {
+                if (exit_plane == 0) {
+                    if (!clo_set) {
+                        clo[0] = xp[0];
+                        clo[1] = xp[1];
+                        clo[2] = xp[2];
+                        clo_set = true;
+                    }
+                    tri_acc<ORDER>(M, clo, xp, e0);
+                } else {
+                    if (!chi_set) {
+                        chi[0] = xp[0];
+                        chi[1] = xp[1];
+                        chi[2] = xp[2];
+                        chi_set = true;
+                    }
+                    tri_acc<ORDER>(M, chi, xp, e0);
+                }
+            }
+        }
+}
+
+/* index range of the x_ax-extent of a polyhedron inside [plo, phi), with
+ * the zero-extent rule of the FP64 path (one cell) */
+PVX_HD void lpoly_range(const LPoly& P, const CopyGeo& G, int ax, int ib0, int plo, int phi, int n, int& lo,
+                        int& hi) {
+    float zmin = 3.0e38f, zmax = -3.0e38f;
+    for (int v = 0; v < P.nv; ++v) {
+        float z = G.x0[ax] + G.DX[ax][0] * P.qa(v, 0) + G.DX[ax][1] * P.qa(v, 1) + G.DX[ax][2] * P.qa(v, 2);
+        zmin = fminf(zmin, z);
+        zmax = fmaxf(zmax, z);
+    }
+    /* local coordinates are relative to the copy's index box corner ib0 */
+    float fl = floorf(zmin), fh = ceilf(zmax);
+    lo = (fl < (float)(plo - ib0)) ? plo : ib0 + (int)fl;
+    hi = (fh > (float)(phi - ib0)) ? phi : ib0 + (int)fh;
+    if (lo < plo) lo = plo;
+    if (hi > phi) hi = phi;
+    if (hi <= lo) hi = lo + 1;
+    if (hi > n) hi = n;
+}
+
+/* y-extent of the tet restricted to the x-slab lo < X <= hi (local units):
+ * tet vertices inside plus edge crossings of the two planes */
+PVX_HD void slab_yrange(const float (*X)[3], bool has_lo, float lo, bool has_hi, float hi, float& ymin, float& ymax) {
+    ymin = 3.0e38f;
+    ymax = -3.0e38f;
+    /* the closed slab: a vertex on a plane bounds the piece next to it */
+    for (int v = 0; v < 4; ++v) {
+        bool in = (!has_lo || X[v][0] >= lo) && (!has_hi || X[v][0] <= hi);
+        if (in) {
+            ymin = fminf(ymin, X[v][1]);
+            ymax = fmaxf(ymax, X[v][1]);
+        }
+    }
+    for (int a = 0; a < 4; ++a)
+        for (int b = a + 1; b < 4; ++b) {
+            float xa = X[a][0], xb = X[b][0];
+            for (int pl = 0; pl < 2; ++pl) {
+                if (pl == 0 && !has_lo) continue;
+                if (pl == 1 && !has_hi) continue;
+                float c = pl == 0 ? lo : hi;
+                if ((xa - c) * (xb - c) < 0.0f) {
+                    float t = (c - xa) / (xb - xa);
+                    float y = X[a][1] + t * (X[b][1] - X[a][1]);
+                    ymin = fminf(ymin, y);
+                    ymax = fmaxf(ymax, y);
+                }
+            }
+        }
+}
+
+/* per-fragment deposit from FP32 q-moments about origin O (FP64 sums) */
+template <int ORDER, class Sink>
+PVX_HD void deposit_f(Sink& sink, const GridInfo& g, const CopyGeo& G, const int* ib0, int i, int j, int k,
+                      const Mom& Mm, const float* O, const double* v0, const float (*DV)[3], double mass,
+                      double wm0) {
+    /* weight: mass times the fragment's mass fraction wm0 (its m0, or m0
+     * plus the tet's residual for the fragment that absorbs it) */
+    const double w = mass * wm0;
+    const int64_t gi = ((int64_t)g.n[1] * g.n[2] * i + (int64_t)g.n[2] * j + k) * g.nacc;
+    sink.add(gi, w);
+    if (ORDER == 0) return;
+    const float inv = 1.0f / Mm.m0;
+    /* mean q (origin shifted back) and q covariance about the mean */
+    const float mu[3] = {0.25f * Mm.s1[0] * inv, 0.25f * Mm.s1[1] * inv, 0.25f * Mm.s1[2] * inv};
+    const float qb[3] = {O[0] + mu[0], O[1] + mu[1], O[2] + mu[2]};
+    float cov[3][3];
+    if (ORDER > 1) {
+        const float c20 = 0.05f * inv;
+        cov[0][0] = c20 * Mm.s2[0] - mu[0] * mu[0];
+        cov[0][1] = cov[1][0] = c20 * Mm.s2[1] - mu[0] * mu[1];
+        cov[0][2] = cov[2][0] = c20 * Mm.s2[2] - mu[0] * mu[2];
+        cov[1][1] = c20 * Mm.s2[3] - mu[1] * mu[1];
+        cov[1][2] = cov[2][1] = c20 * Mm.s2[4] - mu[1] * mu[2];
+        cov[2][2] = c20 * Mm.s2[5] - mu[2] * mu[2];
+    }
+    const int fl = g.flags;
+    const int idx[3] = {i, j, k};
+    double dx[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
+    float b_[3] = {0, 0, 0}; /* DV qbar */
+    if (fl & (F_X | F_XX | F_XV))
+        for (int a = 0; a < 3; ++a) {
+            /* centroid minus cell centre: the FP32 offset from the cell centre
+             * is accurate to FP32 of a cell size */
+            const float dq = G.DX[a][0] * qb[0] + G.DX[a][1] * qb[1] + G.DX[a][2] * qb[2];
+            const float cl = (G.x0[a] + dq) - ((float)(idx[a] - ib0[a]) + 0.5f);
+            dx[a] = (double)cl * g.d[a];
+        }
+    if (fl & (F_V | F_VV | F_XV))
+        for (int a = 0; a < 3; ++a) {
+            b_[a] = DV[a][0] * qb[0] + DV[a][1] * qb[1] + DV[a][2] * qb[2];
+            cv[a] = v0[a] + (double)b_[a];
+        }
+    if (fl & F_X)
+        for (int a = 0; a < 3; ++a) sink.add(gi + g.ox + a, w * dx[a]);
+    if (fl & F_V)
+        for (int a = 0; a < 3; ++a) sink.add(gi + g.ov + a, w * cv[a]);
+    if (ORDER < 2) return;
+    /* covariances of x and v inside the fragment: J cov J^T */
+    float DXa[3][3];
+    if (fl & (F_XX | F_XV))
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) DXa[a][b] = G.DX[a][b] * (float)g.d[a];
+    if (fl & F_XX) {
+        float T[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) T[a][b] = DXa[a][0] * cov[0][b] + DXa[a][1] * cov[1][b] + DXa[a][2] * cov[2][b];
+        int s = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b, ++s) {
+                float sxx = T[a][0] * DXa[b][0] + T[a][1] * DXa[b][1] + T[a][2] * DXa[b][2];
+                sink.add(gi + g.oxx + s, w * ((double)sxx + dx[a] * dx[b]));
+            }
+    }
+    if (fl & F_VV) {
+        float T[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) T[a][b] = DV[a][0] * cov[0][b] + DV[a][1] * cov[1][b] + DV[a][2] * cov[2][b];
+        int s = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b, ++s) {
+                float svv = T[a][0] * DV[b][0] + T[a][1] * DV[b][1] + T[a][2] * DV[b][2];
+                sink.add(gi + g.ovv + s, w * ((double)svv + cv[a] * cv[b]));
+            }
+    }
+    if (fl & F_XV) {
+        float T[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) T[a][b] = DXa[a][0] * cov[0][b] + DXa[a][1] * cov[1][b] + DXa[a][2] * cov[2][b];
+        const bool fix = (fl & F_XVFIX) != 0;
+        /* the reference formula (src/geometry.c:134-147) adds terms in the
+         * mean offsets from vertex 0, A = DX qbar and Bv = DV qbar, which are
+         * tet-sized: form them in FP64 from the record */
+        double A[3] = {0, 0, 0}, Bv[3] = {0, 0, 0};
+        if (!fix) {
+            const double* r = G.rec;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    A[a] += (r[3 * (b + 1) + a] - r[a]) * (double)qb[b];
+                    Bv[a] += (r[12 + 3 * (b + 1) + a] - r[12 + a]) * (double)qb[b];
+                }
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                double sxv = (double)(T[a][0] * DV[b][0] + T[a][1] * DV[b][1] + T[a][2] * DV[b][2]);
+                if (!fix) sxv += 2.0 * A[a] * Bv[b] - A[a] * A[b] - Bv[a] * Bv[b];
+                sink.add(gi + g.oxv + 3 * a + b, w * (sxv + dx[a] * cv[b]));
+            }
+    }
+}
+
+struct FastCounters {
+    int64_t nfrag, ncopies;
+};
+
+/* a fragment kept for a later deposit */
+struct FastFrag {
+    Mom M;
+    float O[3];
+    CopyGeo G;
+    int i, j, k;
+};
+
+/*
+ * All periodic copies of one tet record (pos[12], vel[12], mass, valid,
+ * target).  target = exact q-volume of the tet inside the window (1 when the
+ * window holds the whole tet), computed in FP64 by the expansion kernel.
+ * P is this lane's LDS polyhedron.  Returns false on a failure that needs the
+ * FP64 kernel (slot overflow or a broken graph).
+ */
+template <int ORDER, class Sink>
+PVX_HD bool fast_tet(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, LPoly& P, FastCounters& cnt) {
+    {
+        bool finite = true;
+        for (int t = 0; t < 12; ++t) finite = finite && (rec[t] - rec[t] == 0.0);
+        if (!finite) return false;
+    }
+    double u0[3];
+    float DV[3][3];
+    for (int a = 0; a < 3; ++a) {
+        u0[a] = rec[12 + a];
+        for (int b = 0; b < 3; ++b) DV[a][b] = (float)(rec[12 + 3 * (b + 1) + a] - rec[12 + a]);
+    }
+    const double mass = rec[24];
+    const double target = rec[26];
+    double lo[3], hi[3];
+    aabb((const double (*)[3])rec, 4, lo, hi);
+    uint32_t shp; /* 3 bits per copy */
+    const int ng = ghost_shifts_packed(lo, hi, g, mb, shp);
+    cnt.ncopies += ng;
+    double sum_m0 = 0.0;
+    FastFrag B;
+    B.M.zero();
+#pragma unroll 1
+    for (int h = 0; h < ng; ++h) {
+        CopyGeo G;
+        G.rec = rec;
+        int ib1[3];
+        unsigned wmask = 0; /* window planes: bit 2a lower, 2a+1 upper */
+        for (int a = 0; a < 3; ++a) {
+            const double s = mb.periodic ? (mb.hi[a] - mb.lo[a]) * (double)(1 & (shp >> (3 * h + a))) : 0.0;
+            G.shift[a] = s;
+            /* window planes and index box, as src/geometry.c:816-841 */
+            double cmin = lo[a] + s, cmax = hi[a] + s;
+            if (g.w0[a] > cmin) {
+                cmin = g.w0[a];
+                wmask |= 1u << (2 * a);
+            }
+            if (g.w1[a] < cmax) {
+                cmax = g.w1[a];
+                wmask |= 2u << (2 * a);
+            }
+            index_range(g, a, cmin, cmax, G.ib0[a], ib1[a]);
+            if (ib1[a] <= G.ib0[a]) ib1[a] = G.ib0[a] + 1;
+        }
+        if (G.ib0[0] >= g.n[0] || G.ib0[1] >= g.n[1] || G.ib0[2] >= g.n[2]) continue;
+        /* window planes in local units (used only when the copy crosses them) */
+        float wlo[3], whi[3];
+        for (int a = 0; a < 3; ++a) {
+            wlo[a] = (float)(-G.ib0[a]);
+            whi[a] = (float)((g.w1[a] - g.w0[a]) / g.d[a] - G.ib0[a]);
+        }
+        /* copy geometry in grid units relative to the index box corner */
+        float X[4][3];
+        for (int a = 0; a < 3; ++a) {
+            const double inv = 1.0 / g.d[a];
+            for (int v = 0; v < 4; ++v) X[v][a] = (float)((rec[3 * v + a] + G.shift[a] - g.w0[a]) * inv - G.ib0[a]);
+            G.x0[a] = X[0][a];
+            for (int b = 0; b < 3; ++b) G.DX[a][b] = (float)((rec[3 * (b + 1) + a] - rec[a]) * inv);
+        }
+#pragma unroll 1
+        for (int i = G.ib0[0]; i < ib1[0]; ++i) {
+            const int il = i - G.ib0[0];
+            const bool xlo = i > G.ib0[0], xhi = i < ib1[0] - 1;
+            /* slab planes for the y-range, window planes included */
+            const bool sxlo = xlo || (wmask & 1u), sxhi = xhi || (wmask & 2u);
+            const float cxlo = xlo ? (float)il : wlo[0], cxhi = xhi ? (float)(il + 1) : whi[0];
+            float ymin, ymax;
+            slab_yrange(X, sxlo, cxlo, sxhi, cxhi, ymin, ymax);
+            if (ymin > ymax) continue;
+            int j0 = G.ib0[1] + (int)floorf(ymin), j1 = G.ib0[1] + (int)ceilf(ymax);
+            if (j0 < G.ib0[1]) j0 = G.ib0[1];
+            if (j1 > ib1[1]) j1 = ib1[1];
+            if (j1 <= j0) j1 = j0 + 1;
+            if (j1 > g.n[1]) j1 = g.n[1];
+#pragma unroll 1
+            for (int j = j0; j < j1; ++j) {
+                const int jl = j - G.ib0[1];
+                /* the column: tet clipped by its x and y planes (and window) */
+                P.nv = 4;
+                P.setq(0, 0.f, 0.f, 0.f);
+                P.setq(1, 1.f, 0.f, 0.f);
+                P.setq(2, 0.f, 1.f, 0.f);
+                P.setq(3, 0.f, 0.f, 1.f);
+                P.setnbw(0, pack3(1, 3, 2));
+                P.setnbw(1, pack3(2, 3, 0));
+                P.setnbw(2, pack3(0, 3, 1));
+                P.setnbw(3, pack3(1, 2, 0));
+                /* the column's four planes: interior grid planes, or the
+                 * window planes on the first/last slab and column of a copy
+                 * that crosses the window (src/geometry.c:816-843) */
+                const bool yhi = j < j1 - 1, ylo = j > j0;
+                bool ok = true;
+                if (xhi)
+                    ok = clip_f(P, G, g, (float)(il + 1), 0, false);
+                else if (wmask & 2u)
+                    ok = clip_f(P, G, g, whi[0], 0, false);
+                if (ok && xlo)
+                    ok = clip_f(P, G, g, (float)il, 0, true);
+                else if (ok && (wmask & 1u))
+                    ok = clip_f(P, G, g, wlo[0], 0, true);
+                if (ok && yhi)
+                    ok = clip_f(P, G, g, (float)(jl + 1), 1, false);
+                else if (ok && j == ib1[1] - 1 && (wmask & 8u))
+                    ok = clip_f(P, G, g, whi[1], 1, false);
+                if (ok && ylo)
+                    ok = clip_f(P, G, g, (float)jl, 1, true);
+                else if (ok && j == G.ib0[1] && (wmask & 4u))
+                    ok = clip_f(P, G, g, wlo[1], 1, true);
+                if (!ok) return false;
+                if (P.nv <= 0) continue;
+                int k0, k1;
+                lpoly_range(P, G, 2, G.ib0[2], G.ib0[2], ib1[2], g.n[2], k0, k1);
+#pragma unroll 1
+                for (int k = k0; k < k1; ++k) {
+                    const int kl = k - G.ib0[2];
+                    Mom Mm;
+                    float O[3];
+                    /* cell planes, or the window planes at the ends */
+                    const bool zlo = k > k0 || (k == G.ib0[2] && (wmask & 16u));
+                    const bool zhi = k < k1 - 1 || (k == ib1[2] - 1 && (wmask & 32u));
+                    const float lo_c = k > k0 ? (float)kl : wlo[2];
+                    const float hi_c = k < k1 - 1 ? (float)(kl + 1) : whi[2];
+                    slab_moments<ORDER>(P, G, 2, zlo, lo_c, zhi, hi_c, Mm, O);
+                    if (!(Mm.m0 > 0.0f)) continue;
+                    cnt.nfrag++;
+                    sum_m0 += (double)Mm.m0;
+                    if (Mm.m0 > B.M.m0) { /* new largest: keep it, deposit the old one */
+                        FastFrag t = B;
+                        B.M = Mm;
+                        B.O[0] = O[0];
+                        B.O[1] = O[1];
+                        B.O[2] = O[2];
+                        B.G = G;
+                        B.i = i;
+                        B.j = j;
+                        B.k = k;
+                        if (!(t.M.m0 > 0.0f)) continue;
+                        deposit_f<ORDER>(sink, g, t.G, t.G.ib0, t.i, t.j, t.k, t.M, t.O, u0, (const float (*)[3])DV,
+                                         mass, (double)t.M.m0);
+                    } else {
+                        deposit_f<ORDER>(sink, g, G, G.ib0, i, j, k, Mm, O, u0, (const float (*)[3])DV, mass,
+                                         (double)Mm.m0);
+                    }
+                }
+            }
+        }
+    }
+    /* the largest fragment absorbs the tet's mass residual (exact target
+     * minus the FP64 sum of the FP32 fragment masses) in every moment */
+    if (B.M.m0 > 0.0f)
+        deposit_f<ORDER>(sink, g, B.G, B.G.ib0, B.i, B.j, B.k, B.M, B.O, u0, (const float (*)[3])DV, mass,
+                         (double)B.M.m0 + (target - sum_m0));
+    return true;
+}
+
+}  // namespace pvx
 
 #endif

@@ -42,7 +42,7 @@ constexpr int NV64 = 32;
 /* new vertices one cut may create (cap polygon size) */
 constexpr int NCUT = 16;
 
-enum { ST_OVERFLOW = 1, ST_BADGEOM = 2 };
+enum { ST_OVERFLOW = 1, ST_BADGEOM = 2, ST_FALLBACK = 4 };
 
 struct Poly64 {
     int nv;
@@ -378,6 +378,7 @@ PVX_HD void poly_range64(const Poly64* P, const GridInfo& g, int ax, int plo, in
 
 struct MeshBox {
     int periodic;
+    int full_window; /* the grid window is exactly the periodic box */
     double lo[3], hi[3];
 };
 
@@ -418,6 +419,30 @@ PVX_HD int wrap_and_test(double (*p)[3], int nv, const GridInfo& g, const MeshBo
         }
     }
     return 1;
+}
+
+/* ghost shift patterns packed 3 bits per copy (copy 0 = no shift); returns
+ * the count.  Same rules as ghost_shifts, without an indexed array. */
+PVX_HD int ghost_shifts_packed(const double* lo, const double* hi, const GridInfo& g, const MeshBox& mb,
+                               uint32_t& packed) {
+    packed = 0;
+    if (!mb.periodic) return 1;
+    int flags = 0, num = 1;
+    for (int a = 0; a < 3; ++a)
+        if (lo[a] < mb.lo[a]) flags |= 1 << a;
+    for (int p = 1; p < 8; ++p) {
+        if ((flags & p) != p) continue;
+        int ok = 1;
+        for (int a = 0; a < 3; ++a) {
+            double sh = (mb.hi[a] - mb.lo[a]) * (double)(1 & (p >> a));
+            if (lo[a] + sh > g.w1[a] || hi[a] + sh < g.w0[a]) {
+                ok = 0;
+                break;
+            }
+        }
+        if (ok) packed |= (uint32_t)p << (3 * num++);
+    }
+    return num;
 }
 
 /* ghost shifts of a tet with box [lo, hi]: bit a of the returned mask p's

@@ -4,7 +4,10 @@
  * refinement (src/refine.c:58-368), written as an explicit per-level DFS
  * so one thread handles one element without recursion.
  *
- * Tet record (TREC doubles): pos[4][3], vel[4][3], mass, valid.
+ * Tet record (TREC doubles): pos[4][3], vel[4][3], mass, valid, target, pad.
+ * target is the exact (FP64) fraction of the tet's mass coordinates inside
+ * the grid window summed over its periodic copies: 1 when the window holds
+ * every copy, or when the window is the whole periodic box.
  */
 #ifndef PSIVOX_REFINE_H
 #define PSIVOX_REFINE_H
@@ -13,7 +16,7 @@
 
 namespace pvx {
 
-constexpr int TREC = 26;
+constexpr int TREC = 28;
 constexpr int MAXLVL = 6;
 
 /* corners of the 6-tet split of the 3x3x3 lattice, index 9i+3j+k
@@ -141,7 +144,53 @@ PVX_HD void emit6(double* out, const double (*p)[3], const double (*u)[3], doubl
             }
         r[24] = mass * (1.0 / 6);
         r[25] = 1.0;
+        r[26] = 1.0;
+        r[27] = 0.0;
     }
+}
+
+/* FP64 q-volume fraction of the tet record inside the window, summed over
+ * its periodic copies (src/cpsi.c:343-373 copies, src/geometry.c:816-843
+ * window planes, src/geometry.c:692-767 volume) */
+PVX_HD double window_target(const double* rec, const GridInfo& g, const MeshBox& mb, Poly64* pool) {
+    double x0[4][3];
+    for (int v = 0; v < 4; ++v)
+        for (int a = 0; a < 3; ++a) x0[v][a] = rec[3 * v + a];
+    double lo[3], hi[3];
+    aabb((const double (*)[3])x0, 4, lo, hi);
+    int sh[8];
+    const int ng = ghost_shifts(lo, hi, g, mb, sh);
+    if (mb.periodic && mb.full_window) return 1.0;
+    double tot = 0.0;
+    for (int h = 0; h < ng; ++h) {
+        double x[4][3];
+        bool inside = true;
+        for (int a = 0; a < 3; ++a) {
+            double s = mb.periodic ? (mb.hi[a] - mb.lo[a]) * (double)(1 & (sh[h] >> a)) : 0.0;
+            for (int v = 0; v < 4; ++v) x[v][a] = x0[v][a] + s;
+            if (lo[a] + s < g.w0[a] || hi[a] + s > g.w1[a]) inside = false;
+        }
+        if (inside) {
+            tot += 1.0;
+            continue;
+        }
+        Poly64* P = &pool[0];
+        Poly64* T = &pool[1];
+        unsigned st = 0;
+        init_tet(P, (const double (*)[3])x);
+        for (int a = 0; a < 3; ++a) {
+            if (g.w0[a] > lo[a] + (mb.periodic ? (mb.hi[a] - mb.lo[a]) * (double)(1 & (sh[h] >> a)) : 0.0))
+                clip_lo64(P, T, g.w0[a], a, st);
+            if (g.w1[a] < hi[a] + (mb.periodic ? (mb.hi[a] - mb.lo[a]) * (double)(1 & (sh[h] >> a)) : 0.0))
+                clip_hi64(P, T, g.w1[a], a, st);
+        }
+        if (P->nv > 0) {
+            double m[10];
+            reduce64(P, m, 0);
+            tot += m[0];
+        }
+    }
+    return tot;
 }
 
 /* Per-level DFS state of one element's refinement.  Scratch: L+1 lattice
@@ -180,6 +229,8 @@ PVX_HD int64_t expand_element(const double* pos, const double* vel, const double
                 }
             out[24] = m;
             out[25] = 1.0;
+            out[26] = 1.0;
+            out[27] = 0.0;
         }
         return 1;
     }

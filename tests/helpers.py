@@ -4,7 +4,9 @@ Two error measures per field, both stated in the parity tests:
   rel  -- on "heavy" cells (mass >= 1e-3 of the peak cell mass):
             m:      |dm| / m
             x, v:   |dX| / (|X| + rms of the field over heavy cells)
-            2nd:    |dT_ij| / sqrt(|T_ii| |T_jj|)  (xv: sqrt(|xx_ii| |vv_jj|))
+            2nd:    |dT_ij| / sqrt(|T_ii| |T_jj|)  (xv: sqrt(|xx_ii| |vv_jj|) + |xv_ij|;
+                    the reference's xv formula, src/geometry.c:139-140, adds
+                    tet-sized terms, so xv is measured against its own size)
   wabs -- over all cells, mass-weighted and normalised by the field's peak:
             m: max |dm| / max m;  others: max |d(m T)| / max |m T|.
 Cells holding a sliver of a tet carry moments whose relative error grows as
@@ -50,6 +52,8 @@ def field_errors(a, b, mass_ref=None):
             else:
                 di = dj = np.abs(np.diagonal(B, axis1=-2, axis2=-1))
             scale = np.sqrt(di[..., :, None] * dj[..., None, :]) + 1e-300
+            if f == "xv":
+                scale = scale + np.abs(B)
             rel = float((d / scale)[heavy].max()) if heavy.any() else 0.0
             W = m[..., None, None]
             den = float(np.abs(W * B).max()) or 1.0

@@ -13,6 +13,7 @@
 #include "../../pypsi_amd/csrc/psivox_moments.h"
 #include "../../pypsi_amd/csrc/psivox_plain.h"
 #include "../../pypsi_amd/csrc/psivox_refine.h"
+#include "../../pypsi_amd/csrc/psivox_fast.h"
 
 using namespace pvx;
 
@@ -20,6 +21,81 @@ struct HostSink {
     double* acc;
     void add(int64_t i, double v) { acc[i] += v; }
 };
+
+static void fill_grid(GridInfo& g, MeshBox& mb, const int* n, const double* win6, const double* d, double** fields,
+                      int xvfix, int periodic, const double* box6) {
+    for (int a = 0; a < 3; ++a) {
+        g.n[a] = n[a];
+        g.w0[a] = win6[a];
+        g.w1[a] = win6[3 + a];
+        g.d[a] = d[a];
+    }
+    g.flags = (fields[1] ? F_X : 0) | (fields[2] ? F_V : 0) | (fields[3] ? F_XX : 0) | (fields[4] ? F_XV : 0) |
+              (fields[5] ? F_VV : 0) | (xvfix ? F_XVFIX : 0);
+    set_layout(g);
+    mb.periodic = periodic;
+    mb.full_window = periodic;
+    for (int a = 0; a < 3; ++a) {
+        mb.lo[a] = box6[a];
+        mb.hi[a] = box6[3 + a];
+        if (mb.lo[a] != g.w0[a] || mb.hi[a] != g.w1[a]) mb.full_window = 0;
+    }
+}
+
+/* the FP32 fast path on the host: kernel == 2 */
+extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const double* vel, const double* mass,
+                              const int32_t* conn, int periodic, const double* box6, const int* n, const double* win6,
+                              const double* d, double** fields, int xvfix, double reftol, int maxlvl,
+                              int64_t* out_stats) {
+    GridInfo g;
+    MeshBox mb;
+    fill_grid(g, mb, n, win6, d, fields, xvfix, periodic, box6);
+    const int64_t nvox = (int64_t)n[0] * n[1] * n[2];
+    std::vector<double> acc((size_t)nvox * g.nacc, 0.0);
+    FieldPtrs F;
+    for (int f = 0; f < 6; ++f) F.f[f] = fields[f];
+    for (int64_t c = 0; c < nvox; ++c) cell_to_raw(g, F, acc.data(), c);
+    double dxmin = d[0] < d[1] ? d[0] : d[1];
+    dxmin = dxmin < d[2] ? dxmin : d[2];
+    double tol2 = (reftol * dxmin) * (reftol * dxmin);
+    if (vpe == 4) maxlvl = 0;
+    RefineScratch* RS = new RefineScratch;
+    Poly64* pool = new Poly64[2];
+    std::vector<float> lq(3 * NVF * FBLOCK);
+    std::vector<uint32_t> lnb(NVF * FBLOCK);
+    LPoly P{&lq[0], &lq[NVF * FBLOCK], &lq[2 * NVF * FBLOCK], &lnb[0], 0};
+    HostSink sink{acc.data()};
+    std::vector<double> tets;
+    FastCounters cnt{0, 0};
+    int64_t ntets = 0, fails = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        int64_t nt = expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, nullptr);
+        if (nt == 0) continue;
+        tets.resize((size_t)nt * TREC);
+        expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, tets.data());
+        for (int64_t t = 0; t < nt; ++t) {
+            double* r = &tets[(size_t)t * TREC];
+            if (!(mb.periodic && mb.full_window)) r[26] = window_target(r, g, mb, pool);
+            bool ok;
+            if (g.order == 0)
+                ok = fast_tet<0>(sink, g, mb, r, P, cnt);
+            else if (g.order == 1)
+                ok = fast_tet<1>(sink, g, mb, r, P, cnt);
+            else
+                ok = fast_tet<2>(sink, g, mb, r, P, cnt);
+            if (!ok) fails++;
+        }
+        ntets += nt;
+    }
+    for (int64_t c = 0; c < nvox; ++c) cell_finalize(g, acc.data(), F, c);
+    delete RS;
+    delete[] pool;
+    out_stats[0] = cnt.nfrag;
+    out_stats[1] = cnt.ncopies;
+    out_stats[2] = ntets;
+    out_stats[3] = fails;
+    return 0;
+}
 
 extern "C" int hs_voxels_plain(int vpe, int64_t npart, int64_t nelem, const double* pos, const double* vel,
                                const double* mass, const int32_t* conn, int periodic, const double* box6,

@@ -1,0 +1,41 @@
+"""ctypes wrapper of tests/hostsim (the kernel algorithm built for the host)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "hostsim", "libhostsim.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "hostsim")], check=True)
+        _lib = C.CDLL(LIB)
+    return _lib
+
+
+def run(P, kernel="fast", reftol=1.0, maxlvl=0, xvfix=False, out=None):
+    """P: an oracle.Problem; returns (fields, stats[nfrag, ncopies, ntets, fails/status])."""
+    out = P.zero_fields() if out is None else out
+    order = ["m", "x", "v", "xx", "xv", "vv"]
+    fptr = (C.c_void_p * 6)(*[out[f].ctypes.data if f in out else None for f in order])
+    st = np.zeros(4, np.int64)
+    n = (C.c_int * 3)(*P.n)
+    win = (C.c_double * 6)(*P.bounds.reshape(-1))
+    d = (C.c_double * 3)(*P.d)
+    box = (C.c_double * 6)(*P.box.reshape(-1))
+    vp = lambda a: C.c_void_p(a.ctypes.data)
+    fn = lib().hs_voxels_fast if kernel == "fast" else lib().hs_voxels_plain
+    if kernel == "fast":
+        fn(C.c_int(P.elemtype), C.c_int64(P.conn.shape[0]), vp(P.pos), vp(P.vel), vp(P.mass), vp(P.conn),
+           C.c_int(int(P.periodic)), box, n, win, d, fptr, C.c_int(int(xvfix)), C.c_double(reftol), C.c_int(maxlvl),
+           vp(st))
+    else:
+        fn(C.c_int(P.elemtype), C.c_int64(P.pos.shape[0]), C.c_int64(P.conn.shape[0]), vp(P.pos), vp(P.vel),
+           vp(P.mass), vp(P.conn), C.c_int(int(P.periodic)), box, n, win, d, fptr, C.c_int(int(xvfix)),
+           C.c_double(reftol), C.c_int(maxlvl), vp(st))
+    return out, st

@@ -1,0 +1,95 @@
+"""The kernel algorithms of libpsivox, compiled for the host (tests/hostsim),
+against the oracle on the CPU: the FP64 sweep (bar: the GPU plain kernel's)
+and the FP32 cell-local path (bar: the GPU fast kernel's).  Catches logic
+errors before any GPU run; the GPU tests check the same bars on the device."""
+import numpy as np
+import pytest
+
+import hostsim_api as H
+from helpers import ALL, BOX, assert_close
+
+TOL64 = {"m": (1e-12, 1e-13), "x": (1e-10, 1e-13), "v": (1e-10, 1e-13), "xx": (1e-7, 1e-12), "xv": (1e-7, 1e-12),
+         "vv": (1e-7, 1e-12)}
+TOL32 = {"m": (1e-5, 1e-6), "x": (1e-5, 1e-6), "v": (1e-5, 1e-6), "xx": (1e-3, 1e-5), "xv": (1e-3, 1e-5),
+         "vv": (1e-3, 1e-5)}
+
+
+def zel(n, rms, seed):
+    from pypsi_amd.synth import zeldovich_block
+    return zeldovich_block(n, rms, seed)
+
+
+@pytest.mark.parametrize("kernel", ["plain", "fast"])
+@pytest.mark.parametrize("n,rms,gn,fields,lvl", [
+    (8, 1.0, 16, ALL, 0),
+    (8, 2.0, 16, ALL, 2),
+    (12, 3.0, 20, ["v", "vv"], 1),
+    (16, 0.3, 32, [], 0),
+])
+def test_block_meshes(oracle_mod, kernel, n, rms, gn, fields, lvl):
+    pos, vel = zel(n, rms, 1)
+    P = oracle_mod.Problem.block(pos, vel, periodic_box=BOX, n=(gn,) * 3, bounds=BOX, fields=fields)
+    ref, st = oracle_mod.voxels(P, reftol=0.1, maxlvl=lvl)
+    got, hs = H.run(P, kernel, reftol=0.1, maxlvl=lvl)
+    assert hs[3] == 0  # no failures / overflow
+    assert_close(got, ref, TOL64 if kernel == "plain" else TOL32, f"{kernel} {n}->{gn}")
+    assert abs(got["m"].sum() / P.mass.sum() - 1) < 1e-12
+    if kernel == "plain":
+        assert hs[0] == st["nfrag"]
+    else:
+        assert abs(hs[0] - st["nfrag"]) <= 1e-5 * st["nfrag"] + 2
+
+
+@pytest.mark.parametrize("kernel", ["plain", "fast"])
+def test_lattice_known_answer(oracle_mod, kernel):
+    """Vertices exactly on grid planes (every tet vertex on three planes)."""
+    for n in (4, 16):
+        g = (np.stack(np.meshgrid(*(np.arange(n),) * 3, indexing="ij"), -1) + 0.5) / n
+        P = oracle_mod.Problem.block(np.ascontiguousarray(g), None, periodic_box=BOX, n=(2 * n,) * 3)
+        got, hs = H.run(P, kernel)
+        np.testing.assert_allclose(got["m"], 0.125, rtol=1e-13 if kernel == "plain" else 1e-6, atol=0)
+
+
+@pytest.mark.parametrize("kernel", ["plain", "fast"])
+def test_windows(oracle_mod, kernel):
+    pos, vel = zel(10, 1.5, 4)
+    for per, bounds in ((False, ((0.2, 0.15, 0.3), (0.8, 0.7, 0.9))), (True, ((-0.1, 0.3, 0.55), (0.4, 0.8, 1.05)))):
+        P = oracle_mod.Problem.block(pos, vel, periodic_box=BOX if per else None, n=(10, 9, 11), bounds=bounds,
+                                     fields=["x", "v", "xx", "vv"])
+        ref, _ = oracle_mod.voxels(P)
+        got, hs = H.run(P, kernel)
+        assert_close(got, ref, TOL64 if kernel == "plain" else TOL32, f"{kernel} window periodic={per}")
+        assert abs(got["m"].sum() / ref["m"].sum() - 1) < 1e-12
+
+
+@pytest.mark.parametrize("kernel", ["plain", "fast"])
+def test_random_tets_and_quadratic(oracle_mod, kernel):
+    rng = np.random.default_rng(42)
+    pos = rng.uniform(0.05, 0.95, size=(40, 3))
+    vel = rng.normal(size=(40, 3))
+    conn = rng.integers(0, 40, size=(30, 4)).astype(np.int32)
+    mass = rng.uniform(0.5, 2.0, size=30)
+    P = oracle_mod.Problem(pos, vel, mass, conn, n=(12, 10, 14), bounds=BOX, fields=ALL)
+    ref, _ = oracle_mod.voxels(P)
+    got, _ = H.run(P, kernel)
+    assert_close(got, ref, TOL64 if kernel == "plain" else TOL32, f"{kernel} random tets")
+    g = np.stack(np.meshgrid(*(np.linspace(0, 1, 3),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    qpos = 0.5 + (g - 0.5) * (0.2 + 0.4 * rng.uniform(size=g.shape))
+    qconn = np.arange(27, dtype=np.int32).reshape(1, 27)
+    P = oracle_mod.Problem(qpos, None, None, qconn, n=(32,) * 3, bounds=BOX, fields=["x", "v", "xx"])
+    ref, _ = oracle_mod.voxels(P, reftol=0.05, maxlvl=3)
+    got, _ = H.run(P, kernel, reftol=0.05, maxlvl=3)
+    assert_close(got, ref, TOL64 if kernel == "plain" else TOL32, f"{kernel} quadratic")
+
+
+def test_merge_existing_contents(oracle_mod):
+    """Raw-moment conversion of existing fields reproduces the online merge."""
+    pos1, vel1 = zel(6, 1.0, 1)
+    pos2, vel2 = zel(6, 2.0, 2)
+    P1 = oracle_mod.Problem.block(pos1, vel1, periodic_box=BOX, n=(12,) * 3, bounds=BOX, fields=ALL)
+    P2 = oracle_mod.Problem.block(pos2, vel2, periodic_box=BOX, n=(12,) * 3, bounds=BOX, fields=ALL)
+    ref, _ = oracle_mod.voxels(P1)
+    ref, _ = oracle_mod.voxels(P2, out=ref)
+    got, _ = H.run(P1, "plain")
+    got, _ = H.run(P2, "plain", out=got)
+    assert_close(got, ref, TOL64, "merge")
