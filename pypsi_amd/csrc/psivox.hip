@@ -15,6 +15,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -208,36 +209,149 @@ __global__ __launch_bounds__(64) void k_targets(double* __restrict__ tets, int64
     r[26] = window_target(r, g, mb, pool);
 }
 
-/* one thread per tet record, FP32 column clip in LDS, FP64 deposit */
+/* column tasks per tet record (0 for invalid records); a tet the column
+ * path cannot take flags the FP64 fallback */
+__global__ __launch_bounds__(256) void k_col_count(const double* __restrict__ tets, int64_t ntet, GridInfo g,
+                                                   MeshBox mb, int64_t* __restrict__ cnt, unsigned* status,
+                                                   unsigned long long* counters) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int nc = 0;
+    if (t < ntet) {
+        const double* r = tets + (int64_t)TREC * t;
+        int64_t n = 0;
+        if (r[25] != 0.0) {
+            n = tet_columns(r, t, g, mb, nullptr, &nc);
+            if (n < 0) {
+                atomicOr(status, ST_FALLBACK);
+                n = 0;
+            }
+        }
+        cnt[t] = n;
+    }
+    block_count(0, nc, counters);
+}
+
+/* write the column tasks of tets [t0, t1) at colstart[t] - base */
+__global__ __launch_bounds__(256) void k_col_emit(const double* __restrict__ tets, int64_t t0, int64_t t1, GridInfo g,
+                                                  MeshBox mb, const int64_t* __restrict__ colstart, int64_t base,
+                                                  uint64_t* __restrict__ tasks) {
+    const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= t1) return;
+    const double* r = tets + (int64_t)TREC * t;
+    if (r[25] == 0.0 || colstart[t + 1] == colstart[t]) return;
+    tet_columns(r, t, g, mb, tasks + (colstart[t] - base), nullptr);
+}
+
+/* per-task outputs for the residual pass: FP64 fragment-mass sum, the
+ * largest fragment's key and (ORDER > 0) its moments and origin */
+struct TaskOut {
+    double* sum;
+    uint64_t* key;
+    float* mom; /* 16 floats per task: m0, s1[3], s2[6], O[3], unused */
+};
+
+/* one lane per column task: FP32 column clip in this lane's LDS slice,
+ * cell moments, FP64 deposit */
 template <int ORDER>
-__global__ __launch_bounds__(FBLOCK, 2) void k_vox_fast(const double* __restrict__ tets, int64_t ntet, GridInfo g,
-                                                     MeshBox mb, double* __restrict__ acc, unsigned* status,
-                                                     unsigned long long* counters) {
+__global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
+                                                        const uint64_t* __restrict__ tasks, int64_t ntask, GridInfo g,
+                                                        MeshBox mb, double* __restrict__ acc, TaskOut out,
+                                                        unsigned* status, unsigned long long* counters) {
     __shared__ float sq[3][NVF][FBLOCK];
     __shared__ uint32_t snb[NVF][FBLOCK];
     const int lane = threadIdx.x;
     LPoly P{&sq[0][0][lane], &sq[1][0][lane], &sq[2][0][lane], &snb[0][lane], 0};
-    const int64_t t = (int64_t)blockIdx.x * FBLOCK + lane;
-    FastCounters cnt{0, 0};
-    if (t < ntet) {
-        const double* r = tets + (int64_t)TREC * t;
-        if (r[25] != 0.0) {
-            AtomicSink sink{acc};
-            if (!fast_tet<ORDER>(sink, g, mb, r, P, cnt)) atomicOr(status, ST_FALLBACK);
+    const int64_t c = (int64_t)blockIdx.x * FBLOCK + lane;
+    int64_t nf = 0;
+    if (c < ntask) {
+        const uint64_t w = tasks[c];
+        const double* r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
+        AtomicSink sink{acc};
+        double sum;
+        BestFrag B;
+        if (!column_task<ORDER>(sink, g, mb, r, w, P, nf, sum, B)) atomicOr(status, ST_FALLBACK);
+        out.sum[c] = sum;
+        out.key[c] = B.kl >= 0 ? best_key(B.M.m0, B.kl) : 0ull;
+        if (ORDER > 0 && B.kl >= 0) {
+            float4* m = (float4*)(out.mom + 16 * c);
+            m[0] = make_float4(B.M.m0, B.M.s1[0], B.M.s1[1], B.M.s1[2]);
+            m[1] = make_float4(B.M.s2[0], B.M.s2[1], B.M.s2[2], B.M.s2[3]);
+            m[2] = make_float4(B.M.s2[4], B.M.s2[5], B.O[0], B.O[1]);
+            m[3] = make_float4(B.O[2], 0.f, 0.f, 0.f);
         }
     }
-    block_count(cnt.nfrag, cnt.ncopies, counters);
+    block_count(nf, 0, counters);
 }
 
-static int launch_fast(const double* tets, int64_t nt, const GridInfo& g, const MeshBox& mb, double* acc,
-                       unsigned* status, unsigned long long* counters, hipStream_t s) {
-    const unsigned nb = (unsigned)((nt + FBLOCK - 1) / FBLOCK);
+/* one thread per tet of [t0, t1): reduce its tasks' outputs and deposit the
+ * mass residual into its largest fragment */
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ tets, int64_t t0, int64_t t1,
+                                                  GridInfo g, MeshBox mb, const int64_t* __restrict__ colstart,
+                                                  int64_t base, const uint64_t* __restrict__ tasks, TaskOut out,
+                                                  double* __restrict__ acc) {
+    const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= t1) return;
+    const int64_t c0 = colstart[t] - base, c1 = colstart[t + 1] - base;
+    if (c1 <= c0) return;
+    double sum = 0.0;
+    uint64_t kbest = 0;
+    int64_t cbest = -1;
+    for (int64_t c = c0; c < c1; ++c) {
+        sum += out.sum[c];
+        const uint64_t k = out.key[c];
+        if (k > kbest) {
+            kbest = k;
+            cbest = c;
+        }
+    }
+    if (cbest < 0) return;
+    BestFrag B;
+    B.kl = (int)(kbest & 0xffffffffull);
+    const uint32_t mbits = (uint32_t)(kbest >> 32);
+    memcpy(&B.M.m0, &mbits, 4);
+    if (ORDER > 0) {
+        const float4* m = (const float4*)(out.mom + 16 * cbest);
+        const float4 a = m[0], b = m[1], c = m[2], d = m[3];
+        B.M.s1[0] = a.y;
+        B.M.s1[1] = a.z;
+        B.M.s1[2] = a.w;
+        B.M.s2[0] = b.x;
+        B.M.s2[1] = b.y;
+        B.M.s2[2] = b.z;
+        B.M.s2[3] = b.w;
+        B.M.s2[4] = c.x;
+        B.M.s2[5] = c.y;
+        B.O[0] = c.z;
+        B.O[1] = c.w;
+        B.O[2] = d.x;
+    }
+    AtomicSink sink{acc};
+    tet_residual<ORDER>(sink, g, mb, tets + (int64_t)TREC * t, tasks[cbest], B, sum);
+}
+
+static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask, const GridInfo& g, const MeshBox& mb,
+                       double* acc, TaskOut out, unsigned* status, unsigned long long* counters, hipStream_t s) {
+    const unsigned nb = (unsigned)((ntask + FBLOCK - 1) / FBLOCK);
     if (g.order == 0)
-        k_vox_fast<0><<<nb, FBLOCK, 0, s>>>(tets, nt, g, mb, acc, status, counters);
+        k_vox_cols<0><<<nb, FBLOCK, 0, s>>>(tets, tasks, ntask, g, mb, acc, out, status, counters);
     else if (g.order == 1)
-        k_vox_fast<1><<<nb, FBLOCK, 0, s>>>(tets, nt, g, mb, acc, status, counters);
+        k_vox_cols<1><<<nb, FBLOCK, 0, s>>>(tets, tasks, ntask, g, mb, acc, out, status, counters);
     else
-        k_vox_fast<2><<<nb, FBLOCK, 0, s>>>(tets, nt, g, mb, acc, status, counters);
+        k_vox_cols<2><<<nb, FBLOCK, 0, s>>>(tets, tasks, ntask, g, mb, acc, out, status, counters);
+    return (int)hipGetLastError();
+}
+
+static int launch_residual(const double* tets, int64_t t0, int64_t t1, const GridInfo& g, const MeshBox& mb,
+                           const int64_t* colstart, int64_t base, const uint64_t* tasks, TaskOut out, double* acc,
+                           hipStream_t s) {
+    const unsigned nb = (unsigned)((t1 - t0 + 255) / 256);
+    if (g.order == 0)
+        k_residual<0><<<nb, 256, 0, s>>>(tets, t0, t1, g, mb, colstart, base, tasks, out, acc);
+    else if (g.order == 1)
+        k_residual<1><<<nb, 256, 0, s>>>(tets, t0, t1, g, mb, colstart, base, tasks, out, acc);
+    else
+        k_residual<2><<<nb, 256, 0, s>>>(tets, t0, t1, g, mb, colstart, base, tasks, out, acc);
     return (int)hipGetLastError();
 }
 
@@ -276,7 +390,8 @@ struct DevBuf {
 };
 
 struct Workspace {
-    DevBuf pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters;
+    DevBuf pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
+        colsum, colkey, colmom;
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -469,23 +584,88 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 k_targets<<<grid_blocks(nt, 64), 64, 0, s>>>(tets, nt, g, mb);
                 HIPCHK(hipGetLastError());
             }
-            HIPCHK(hipEventRecord(W.ev0, s));
             if (kernel == PSIVOX_KERNEL_PLAIN) {
+                HIPCHK(hipEventRecord(W.ev0, s));
                 k_vox_plain<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, acc, status, counters);
                 HIPCHK(hipGetLastError());
             } else {
-                rc = launch_fast(tets, nt, g, mb, acc, status, counters, s);
-                if (rc) return set_err(PSIVOX_E_HIP, "fast kernel launch failed: %s", hipGetErrorString((hipError_t)rc));
+                /* column tasks: count, scan, emit */
+                if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
+                if ((rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
+                int64_t* colcnt = (int64_t*)W.colcnt.p;
+                int64_t* colstart = (int64_t*)W.colstart.p;
+                HIPCHK(hipMemsetAsync(colcnt + nt, 0, sizeof(int64_t), s));
+                k_col_count<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, colcnt, status, counters);
+                HIPCHK(hipGetLastError());
+                size_t tb = 0;
+                HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, colcnt, colstart, nt + 1, s));
+                if ((rc = W.scan_tmp.get(tb, dev))) return rc;
+                HIPCHK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp.p, tb, colcnt, colstart, nt + 1, s));
+                int64_t ntask = 0;
+                unsigned hs = 0;
+                HIPCHK(hipMemcpyAsync(&ntask, colstart + nt, sizeof ntask, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipMemcpyAsync(&hs, status, sizeof hs, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                if (!(hs & ST_FALLBACK) && ntask > 0) {
+                    /* tet sub-ranges whose tasks fit the task buffers */
+                    const int64_t cap = (int64_t)1 << 28;
+                    std::vector<int64_t> cuts{0, nt};
+                    std::vector<int64_t> hcs;
+                    if (ntask > cap) {
+                        hcs.resize(nt + 1);
+                        HIPCHK(hipMemcpy(hcs.data(), colstart, (nt + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+                        cuts.assign(1, 0);
+                        while (cuts.back() < nt) {
+                            const int64_t a0 = cuts.back();
+                            int64_t a1 = std::upper_bound(hcs.begin() + a0 + 1, hcs.end(), hcs[a0] + cap) - hcs.begin() - 1;
+                            if (a1 <= a0) a1 = a0 + 1; /* a single tet is below TASK_MAXPERTET < cap */
+                            cuts.push_back(a1);
+                        }
+                    }
+                    const int64_t tcap = std::min(ntask, cap);
+                    if ((rc = W.tasks.get((size_t)tcap * sizeof(uint64_t), dev))) return rc;
+                    if ((rc = W.colsum.get((size_t)tcap * sizeof(double), dev))) return rc;
+                    if ((rc = W.colkey.get((size_t)tcap * sizeof(uint64_t), dev))) return rc;
+                    if (g.order > 0 && (rc = W.colmom.get((size_t)tcap * 16 * sizeof(float), dev))) return rc;
+                    uint64_t* tasks = (uint64_t*)W.tasks.p;
+                    TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
+                    for (size_t q = 0; q + 1 < cuts.size(); ++q) {
+                        const int64_t a0 = cuts[q], a1 = cuts[q + 1];
+                        const int64_t base = hcs.empty() ? 0 : hcs[a0];
+                        const int64_t nsub = hcs.empty() ? ntask : hcs[a1] - hcs[a0];
+                        if (nsub == 0) continue;
+                        k_col_emit<<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, g, mb, colstart, base,
+                                                                             tasks);
+                        HIPCHK(hipGetLastError());
+                        HIPCHK(hipEventRecord(W.ev0, s));
+                        rc = launch_cols(tets, tasks, nsub, g, mb, acc, out, status, counters, s);
+                        if (rc)
+                            return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
+                                           hipGetErrorString((hipError_t)rc));
+                        HIPCHK(hipEventRecord(W.ev1, s));
+                        HIPCHK(hipEventSynchronize(W.ev1));
+                        float ms = 0.f;
+                        HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+                        kms += ms;
+                        rc = launch_residual(tets, a0, a1, g, mb, colstart, base, tasks, out, acc, s);
+                        if (rc)
+                            return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
+                                           hipGetErrorString((hipError_t)rc));
+                    }
+                }
             }
-            HIPCHK(hipEventRecord(W.ev1, s));
-            HIPCHK(hipEventSynchronize(W.ev1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
-            kms += ms;
+            if (kernel == PSIVOX_KERNEL_PLAIN) {
+                HIPCHK(hipEventRecord(W.ev1, s));
+                HIPCHK(hipEventSynchronize(W.ev1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+                kms += ms;
+            }
             ntets_total += nt;
             if (kernel == PSIVOX_KERNEL_FAST) {
                 unsigned hs = 0;
-                HIPCHK(hipMemcpy(&hs, status, sizeof hs, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpyAsync(&hs, status, sizeof hs, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
                 if (hs & ST_FALLBACK) {
                     /* a tet exceeded the FP32 path's vertex budget: redo the
                      * whole call with the FP64 kernel from the untouched fields */

@@ -28,6 +28,7 @@
 #define PSIVOX_FAST_H
 
 #include "psivox_geom.h"
+#include <string.h>
 
 #ifndef PVX_DBG
 #define PVX_DBG(...)
@@ -35,11 +36,20 @@
 
 namespace pvx {
 
-constexpr int NVF = 18;   /* vertex slots of the column polyhedron */
+constexpr int NVF = 16;   /* vertex slots of the column polyhedron (tet and 4 planes: <= 12) */
 constexpr int FBLOCK = 64; /* lanes per workgroup (one wavefront) */
 
-/* A lane's polyhedron: q components and packed neighbours (3 x 8 bits) at
- * stride FBLOCK so the 64 lanes of a wavefront touch consecutive words. */
+/* A lane's polyhedron: q components and one packed word per vertex at
+ * stride FBLOCK (the 64 lanes of a wavefront touch consecutive words).
+ * Word layout: neighbour k in bits [5k, 5k+5); the fan apex of the face to
+ * the left of the directed edge (v -> neighbour k) in bits [15+5k, 20+5k)
+ * (filled once per column by face_apexes).  31 marks an unset link. */
+constexpr uint32_t NBUNSET = 31u;
+
+PVX_HD int nbf(uint32_t w, int k) { return (int)((w >> (5 * k)) & 31u); }
+PVX_HD int apf(uint32_t w, int k) { return (int)((w >> (15 + 5 * k)) & 31u); }
+PVX_HD uint32_t pack3(int a, int b, int c) { return (uint32_t)a | ((uint32_t)b << 5) | ((uint32_t)c << 10); }
+
 struct LPoly {
     float* q0;
     float* q1;
@@ -52,17 +62,15 @@ struct LPoly {
         q1[v * FBLOCK] = b;
         q2[v * FBLOCK] = c;
     }
-    PVX_HD int nbr(int v, int k) const { return (int)((nb[v * FBLOCK] >> (8 * k)) & 0xffu); }
+    PVX_HD int nbr(int v, int k) const { return nbf(nb[v * FBLOCK], k); }
     PVX_HD uint32_t nbw(int v) const { return nb[v * FBLOCK]; }
     PVX_HD void setnbw(int v, uint32_t w) { nb[v * FBLOCK] = w; }
     PVX_HD void setnbr(int v, int k, int val) {
         uint32_t w = nb[v * FBLOCK];
-        w = (w & ~(0xffu << (8 * k))) | ((uint32_t)val << (8 * k));
+        w = (w & ~(31u << (5 * k))) | ((uint32_t)val << (5 * k));
         nb[v * FBLOCK] = w;
     }
 };
-
-PVX_HD uint32_t pack3(int a, int b, int c) { return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16); }
 
 /* The copy being processed: geometry in grid units relative to its index
  * box corner in FP32; the FP64 tie-break re-reads the tet record. */
@@ -167,7 +175,7 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, float c, int a
             const float sc = sdist(G.rec, m, c, c0, c1, c2), sn = sdist(G.rec, m, c, n0, n1, n2);
             const float inv = 1.0f / (sc - sn);
             P.setq(tot, inv * (n0 * sc - c0 * sn), inv * (n1 * sc - c1 * sn), inv * (n2 * sc - c2 * sn));
-            P.setnbw(tot, pack3(vc, 0xff, 0xff));
+            P.setnbw(tot, pack3(vc, NBUNSET, NBUNSET));
             P.setnbr(vc, k, tot);
             tot++;
         }
@@ -183,14 +191,14 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, float c, int a
             }
             uint32_t w = P.nbw(vnext);
             int k = 0;
-            while (k < 3 && (int)((w >> (8 * k)) & 0xffu) != vcur) k++;
+            while (k < 3 && nbf(w, k) != vcur) k++;
             if (k == 3) {
                 PVX_DBG("clip: walk no back link %d -> %d\n", vcur, vnext);
                 return false;
             }
             vcur = vnext;
             if (vcur >= onv) break;
-            vnext = (int)((w >> (8 * ((k + 1) % 3))) & 0xffu);
+            vnext = nbf(w, (k + 1) % 3);
         }
         P.setnbr(vs, 2, vcur);
         P.setnbr(vcur, 1, vs);
@@ -204,13 +212,13 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, float c, int a
         uint32_t w = P.nbw(v);
         uint32_t o = 0;
         for (int k = 0; k < 3; ++k) {
-            int n = (int)((w >> (8 * k)) & 0xffu);
+            int n = nbf(w, k);
             if (n >= tot) {
                 PVX_DBG("clip: compaction bad index %d (tot %d) at v %d\n", n, tot, v);
                 return false;
             }
             int r = n < onv ? __builtin_popcount(keep & ((1u << n) - 1u)) : nk0 + (n - onv);
-            o |= (uint32_t)r << (8 * k);
+            o |= (uint32_t)r << (5 * k);
         }
         int d = v < onv ? __builtin_popcount(keep & ((1u << v) - 1u)) : nk0 + (v - onv);
         if (d != v) P.setq(d, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
@@ -253,13 +261,155 @@ PVX_HD void tri_acc(Mom& M, const float* A, const float* B, const float* C) {
 }
 
 /*
+ * Fan apexes of P's faces: every face is walked once in the reference's
+ * order (src/geometry.c:708-746) and each of its directed edges records the
+ * walk's first vertex, so a face (v0, v1, ..., vm) is fanned into triangles
+ * (v0, vi, vi+1) exactly as the reference's reduce fans it.
+ */
+PVX_HD void face_apexes(LPoly& P) {
+    const int nv = P.nv;
+    for (int v = 0; v < nv; ++v) P.setnbw(v, P.nbw(v) & 0x7fffu);
+    uint64_t mark = 0;
+    for (int vs = 0; vs < nv; ++vs)
+        for (int ks = 0; ks < 3; ++ks) {
+            if (mark & (1ull << (3 * vs + ks))) continue;
+            int a = vs, ka = ks, guard = 0;
+            do {
+                mark |= 1ull << (3 * a + ka);
+                const uint32_t wa = P.nbw(a);
+                P.setnbw(a, wa | ((uint32_t)vs << (15 + 5 * ka)));
+                const int b = nbf(wa, ka);
+                const uint32_t wb = P.nbw(b);
+                int kb = 0;
+                while (kb < 2 && nbf(wb, kb) != a) kb++;
+                a = b;
+                ka = (kb + 1) % 3;
+            } while (!(a == vs && ka == ks) && ++guard <= 3 * NVF);
+        }
+}
+
+/* one polygon vertex stream clipped to the slab lo < z <= hi: feed points
+ * in traversal order, the clipped polygon is fanned from its first point
+ * and each exit -> entry edge on a slab plane adds the cap triangle */
+template <int ORDER>
+struct SlabClip {
+    Mom* M;
+    bool has_lo, has_hi;
+    float lo, hi;
+    float* clo;
+    float* chi;
+    bool* clo_set;
+    bool* chi_set;
+    float e0[3], ep[3], xp[3];
+    int nem, exit_plane;
+
+    PVX_HD void cap(int plane, const float* X, const float* N) {
+        if (plane == 0) {
+            if (!*clo_set) {
+                clo[0] = X[0];
+                clo[1] = X[1];
+                clo[2] = X[2];
+                *clo_set = true;
+            }
+            tri_acc<ORDER>(*M, clo, X, N);
+        } else {
+            if (!*chi_set) {
+                chi[0] = X[0];
+                chi[1] = X[1];
+                chi[2] = X[2];
+                *chi_set = true;
+            }
+            tri_acc<ORDER>(*M, chi, X, N);
+        }
+    }
+    PVX_HD void emit(const float* pt) {
+        if (nem == 0) {
+            e0[0] = pt[0];
+            e0[1] = pt[1];
+            e0[2] = pt[2];
+        } else if (nem >= 2) {
+            tri_acc<ORDER>(*M, e0, pt, ep);
+        }
+        ep[0] = pt[0];
+        ep[1] = pt[1];
+        ep[2] = pt[2];
+        nem++;
+    }
+    /* the edge a -> b (q relative to the origin, z their slab coordinates) */
+    PVX_HD void edge(const float* qa, float za, const float* qb, float zb) {
+        float t0 = 0.f, t1 = 1.f;
+        int p0 = -1, p1 = -1;
+        bool empty = false;
+        const float dz = zb - za;
+        if (has_lo) { /* keep z > lo */
+            if (za <= lo && zb <= lo) {
+                empty = true;
+            } else if (za <= lo) {
+                t0 = (lo - za) / dz;
+                p0 = 0;
+            } else if (zb <= lo) {
+                t1 = (lo - za) / dz;
+                p1 = 0;
+            }
+        }
+        if (!empty && has_hi) { /* keep z <= hi */
+            if (za > hi && zb > hi) {
+                empty = true;
+            } else if (za > hi) {
+                const float t = (hi - za) / dz;
+                if (t > t0) {
+                    t0 = t;
+                    p0 = 1;
+                }
+            } else if (zb > hi) {
+                const float t = (hi - za) / dz;
+                if (t < t1) {
+                    t1 = t;
+                    p1 = 1;
+                }
+            }
+        }
+        /* a touching interval (t0 == t1, e.g. a on a plane) still emits its
+         * point, as Sutherland-Hodgman emits an inside vertex */
+        if (empty || t0 > t1) return;
+        float pt[3];
+        if (p0 < 0) {
+            pt[0] = qa[0];
+            pt[1] = qa[1];
+            pt[2] = qa[2];
+        } else {
+            pt[0] = qa[0] + t0 * (qb[0] - qa[0]);
+            pt[1] = qa[1] + t0 * (qb[1] - qa[1]);
+            pt[2] = qa[2] + t0 * (qb[2] - qa[2]);
+            if (exit_plane == p0) { /* exit -> entry on this plane: cap runs it backwards */
+                cap(p0, xp, pt);
+                exit_plane = -1;
+            }
+        }
+        emit(pt);
+        if (p1 >= 0) {
+            const float pe[3] = {qa[0] + t1 * (qb[0] - qa[0]), qa[1] + t1 * (qb[1] - qa[1]),
+                                 qa[2] + t1 * (qb[2] - qa[2])};
+            emit(pe);
+            exit_plane = p1;
+            xp[0] = pe[0];
+            xp[1] = pe[1];
+            xp[2] = pe[2];
+        }
+    }
+    /* the stream began outside: its last exit joins its first entry */
+    PVX_HD void close() {
+        if (exit_plane >= 0 && nem > 0) cap(exit_plane, xp, e0);
+    }
+};
+
+/*
  * Moments of P intersected with the slab lo < x_ax <= hi (either side may be
- * open), about the origin O (q of P's vertex 0).  Every face of P is walked
- * once in the reference's order (src/geometry.c:708-746); its polygon
- * clipped to the slab is fanned from its first kept point with the
- * reference's orientation (triangle (first, next, current)); each clipped
- * face leaves an edge exit -> entry on a slab plane, and the caps, which
- * run those edges the other way, are fanned from a fixed point on each plane.
+ * open) about the origin O, the vertex nearest the slab (small FP32
+ * cancellation).  P's faces are covered by their fan triangles (apexes from
+ * face_apexes); each triangle is clipped to the slab on its own, its cap
+ * edges chain across the fan diagonals, so the result equals clipping whole
+ * faces.  The loop runs over directed edges: no per-face control flow.
  */
 template <int ORDER>
 PVX_HD void slab_moments(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, float lo, bool has_hi, float hi,
@@ -269,8 +419,6 @@ PVX_HD void slab_moments(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, 
     const float zx = pick3(G.x0, ax), zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
                 zd1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]),
                 zd2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
-    /* origin: the vertex nearest the slab, so the moments of the cell's
-     * piece are taken about a nearby point (small FP32 cancellation) */
     {
         const float zc = has_lo ? (has_hi ? 0.5f * (lo + hi) : lo + 0.5f) : (has_hi ? hi - 0.5f : 0.0f);
         int vb = 0;
@@ -287,153 +435,38 @@ PVX_HD void slab_moments(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, 
         O[1] = P.qa(vb, 1);
         O[2] = P.qa(vb, 2);
     }
-    const float O0 = O[0], O1 = O[1], O2 = O[2];
-    uint64_t mark = 0;
     float clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
     bool clo_set = false, chi_set = false;
-    for (int vs = 0; vs < nv; ++vs)
-        for (int ks = 0; ks < 3; ++ks) {
-            if (mark & (1ull << (3 * vs + ks))) continue;
-            int a = vs, ka = ks;
-            float a0 = P.qa(a, 0), a1 = P.qa(a, 1), a2 = P.qa(a, 2);
-            float za = zx + zd0 * a0 + zd1 * a1 + zd2 * a2;
-            a0 -= O0;
-            a1 -= O1;
-            a2 -= O2;
-            float e0[3] = {0, 0, 0}, ep[3] = {0, 0, 0}, xp[3] = {0, 0, 0};
-            int nem = 0;
-            int exit_plane = -1; /* pending exit on a plane: 0 lo, 1 hi */
-            int guard = 0;
-            for (;;) {
-                mark |= 1ull << (3 * a + ka);
-                const int b = P.nbr(a, ka);
-                float b0 = P.qa(b, 0), b1 = P.qa(b, 1), b2 = P.qa(b, 2);
-                const float zb = zx + zd0 * b0 + zd1 * b1 + zd2 * b2;
-                b0 -= O0;
-                b1 -= O1;
-                b2 -= O2;
-                float t0 = 0.f, t1 = 1.f;
-                int p0 = -1, p1 = -1;
-                bool empty = false;
-                const float dz = zb - za;
-                if (has_lo) { /* keep z > lo */
-                    if (za <= lo && zb <= lo) {
-                        empty = true;
-                    } else if (za <= lo) {
-                        t0 = (lo - za) / dz;
-                        p0 = 0;
-                    } else if (zb <= lo) {
-                        t1 = (lo - za) / dz;
-                        p1 = 0;
-                    }
-                }
-                if (!empty && has_hi) { /* keep z <= hi */
-                    if (za > hi && zb > hi) {
-                        empty = true;
-                    } else if (za > hi) {
-                        float t = (hi - za) / dz;
-                        if (t > t0) {
-                            t0 = t;
-                            p0 = 1;
-                        }
-                    } else if (zb > hi) {
-                        float t = (hi - za) / dz;
-                        if (t < t1) {
-                            t1 = t;
-                            p1 = 1;
-                        }
-                    }
-                }
-                /* a touching interval (t0 == t1, e.g. a on a plane) still emits
-                 * its point, as Sutherland-Hodgman emits an inside vertex */
-                if (!empty && t0 <= t1) {
-                    float pt[3];
-                    if (p0 < 0) {
-                        pt[0] = a0;
-                        pt[1] = a1;
-                        pt[2] = a2;
-                    } else {
-                        pt[0] = a0 + t0 * (b0 - a0);
-                        pt[1] = a1 + t0 * (b1 - a1);
-                        pt[2] = a2 + t0 * (b2 - a2);
-                        if (exit_plane == p0) { /* cap edge exit -> entry, run entry -> exit by the cap */
-                            if (p0 == 0) {
-                                if (!clo_set) {
-                                    clo[0] = xp[0];
-                                    clo[1] = xp[1];
-                                    clo[2] = xp[2];
-                                    clo_set = true;
-                                }
-                                tri_acc<ORDER>(M, clo, xp, pt);
-                            } else {
-                                if (!chi_set) {
-                                    chi[0] = xp[0];
-                                    chi[1] = xp[1];
-                                    chi[2] = xp[2];
-                                    chi_set = true;
-                                }
-                                tri_acc<ORDER>(M, chi, xp, pt);
-                            }
-                            exit_plane = -1;
-                        }
-                    }
-                    if (nem == 0) {
-                        e0[0] = pt[0];
-                        e0[1] = pt[1];
-                        e0[2] = pt[2];
-                    } else if (nem >= 2) {
-                        tri_acc<ORDER>(M, e0, pt, ep);
-                    }
-                    ep[0] = pt[0];
-                    ep[1] = pt[1];
-                    ep[2] = pt[2];
-                    nem++;
-                    if (p1 >= 0) {
-                        float pe[3] = {a0 + t1 * (b0 - a0), a1 + t1 * (b1 - a1), a2 + t1 * (b2 - a2)};
-                        if (nem >= 2) tri_acc<ORDER>(M, e0, pe, ep);
-                        ep[0] = pe[0];
-                        ep[1] = pe[1];
-                        ep[2] = pe[2];
-                        nem++;
-                        exit_plane = p1;
-                        xp[0] = pe[0];
-                        xp[1] = pe[1];
-                        xp[2] = pe[2];
-                    }
-                }
-                /* next edge of the face: from b, the neighbour after a */
-                const uint32_t wb = P.nbw(b);
-                int kb = 0;
-                while (kb < 2 && (int)((wb >> (8 * kb)) & 0xffu) != a) kb++;
-                a = b;
-                ka = (kb + 1) % 3;
-                a0 = b0;
-                a1 = b1;
-                a2 = b2;
-                za = zb;
-                if ((a == vs && ka == ks) || ++guard > 3 * NVF) break;
-            }
-            /* walk began outside: the last exit joins the first entry e0 */
-            if (exit_plane >= 0 && nem > 0) {
-                if (exit_plane == 0) {
-                    if (!clo_set) {
-                        clo[0] = xp[0];
-                        clo[1] = xp[1];
-                        clo[2] = xp[2];
-                        clo_set = true;
-                    }
-                    tri_acc<ORDER>(M, clo, xp, e0);
-                } else {
-                    if (!chi_set) {
-                        chi[0] = xp[0];
-                        chi[1] = xp[1];
-                        chi[2] = xp[2];
-                        chi_set = true;
-                    }
-                    tri_acc<ORDER>(M, chi, xp, e0);
-                }
-            }
+    for (int a = 0; a < nv; ++a) {
+        const uint32_t wa = P.nbw(a);
+        float qa[3] = {P.qa(a, 0) - O[0], P.qa(a, 1) - O[1], P.qa(a, 2) - O[2]};
+        const float za = zx + zd0 * P.qa(a, 0) + zd1 * P.qa(a, 1) + zd2 * P.qa(a, 2);
+        for (int k = 0; k < 3; ++k) {
+            const int b = nbf(wa, k), c = apf(wa, k);
+            if (c == a || c == b) continue;
+            const float qb[3] = {P.qa(b, 0) - O[0], P.qa(b, 1) - O[1], P.qa(b, 2) - O[2]};
+            const float qc[3] = {P.qa(c, 0) - O[0], P.qa(c, 1) - O[1], P.qa(c, 2) - O[2]};
+            const float zb = zx + zd0 * P.qa(b, 0) + zd1 * P.qa(b, 1) + zd2 * P.qa(b, 2);
+            const float zcv = zx + zd0 * P.qa(c, 0) + zd1 * P.qa(c, 1) + zd2 * P.qa(c, 2);
+            /* the fan triangle (c, a, b), traversed c -> a -> b */
+            SlabClip<ORDER> sc;
+            sc.M = &M;
+            sc.has_lo = has_lo;
+            sc.has_hi = has_hi;
+            sc.lo = lo;
+            sc.hi = hi;
+            sc.clo = clo;
+            sc.chi = chi;
+            sc.clo_set = &clo_set;
+            sc.chi_set = &chi_set;
+            sc.nem = 0;
+            sc.exit_plane = -1;
+            sc.edge(qc, zcv, qa, za);
+            sc.edge(qa, za, qb, zb);
+            sc.edge(qb, zb, qc, zcv);
+            sc.close();
         }
+    }
 }
 
 /* index range of the x_ax-extent of a polyhedron inside [plo, phi), with
@@ -590,175 +623,281 @@ struct FastCounters {
     int64_t nfrag, ncopies;
 };
 
-/* a fragment kept for a later deposit */
-struct FastFrag {
-    Mom M;
-    float O[3];
+/* ------------------------------------------------------------------ */
+/* column tasks                                                        */
+/* ------------------------------------------------------------------ */
+
+/* A copy of a tet prepared for column work: its FP32 cell-local geometry,
+ * index box, window-plane mask and window planes in local units. */
+struct CopySetup {
     CopyGeo G;
-    int i, j, k;
+    int ib1[3];
+    unsigned wmask; /* window planes crossed: bit 2a lower, 2a+1 upper */
+    float wlo[3], whi[3];
+    float X[4][3];
 };
 
-/*
- * All periodic copies of one tet record (pos[12], vel[12], mass, valid,
- * target).  target = exact q-volume of the tet inside the window (1 when the
- * window holds the whole tet), computed in FP64 by the expansion kernel.
- * P is this lane's LDS polyhedron.  Returns false on a failure that needs the
- * FP64 kernel (slot overflow or a broken graph).
- */
-template <int ORDER, class Sink>
-PVX_HD bool fast_tet(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, LPoly& P, FastCounters& cnt) {
-    {
-        bool finite = true;
-        for (int t = 0; t < 12; ++t) finite = finite && (rec[t] - rec[t] == 0.0);
-        if (!finite) return false;
-    }
-    double u0[3];
-    float DV[3][3];
+/* copy h of the tet record (shift pattern from ghost_shifts_packed);
+ * returns false when the copy's index box starts beyond the grid */
+PVX_HD bool copy_setup(const double* rec, const double* lo, const double* hi, uint32_t shp, int h, const GridInfo& g,
+                       const MeshBox& mb, CopySetup& C) {
+    C.G.rec = rec;
+    C.wmask = 0;
     for (int a = 0; a < 3; ++a) {
-        u0[a] = rec[12 + a];
-        for (int b = 0; b < 3; ++b) DV[a][b] = (float)(rec[12 + 3 * (b + 1) + a] - rec[12 + a]);
+        const double s = mb.periodic ? (mb.hi[a] - mb.lo[a]) * (double)(1 & (shp >> (3 * h + a))) : 0.0;
+        C.G.shift[a] = s;
+        /* window planes and index box, as src/geometry.c:816-841 */
+        double cmin = lo[a] + s, cmax = hi[a] + s;
+        if (g.w0[a] > cmin) {
+            cmin = g.w0[a];
+            C.wmask |= 1u << (2 * a);
+        }
+        if (g.w1[a] < cmax) {
+            cmax = g.w1[a];
+            C.wmask |= 2u << (2 * a);
+        }
+        index_range(g, a, cmin, cmax, C.G.ib0[a], C.ib1[a]);
+        if (C.ib1[a] <= C.G.ib0[a]) C.ib1[a] = C.G.ib0[a] + 1;
     }
-    const double mass = rec[24];
-    const double target = rec[26];
+    if (C.G.ib0[0] >= g.n[0] || C.G.ib0[1] >= g.n[1] || C.G.ib0[2] >= g.n[2]) return false;
+    for (int a = 0; a < 3; ++a) {
+        const double inv = 1.0 / g.d[a];
+        for (int v = 0; v < 4; ++v)
+            C.X[v][a] = (float)((rec[3 * v + a] + C.G.shift[a] - g.w0[a]) * inv - C.G.ib0[a]);
+        C.G.x0[a] = C.X[0][a];
+        for (int b = 0; b < 3; ++b) C.G.DX[a][b] = (float)((rec[3 * (b + 1) + a] - rec[a]) * inv);
+        C.wlo[a] = (float)(-C.G.ib0[a]);
+        C.whi[a] = (float)((g.w1[a] - g.w0[a]) / g.d[a] - C.G.ib0[a]);
+    }
+    return true;
+}
+
+/* y-column range of slab i of a prepared copy */
+PVX_HD bool slab_columns(const CopySetup& C, int i, const GridInfo& g, int& j0, int& j1) {
+    const int il = i - C.G.ib0[0];
+    const bool xlo = i > C.G.ib0[0], xhi = i < C.ib1[0] - 1;
+    const bool sxlo = xlo || (C.wmask & 1u), sxhi = xhi || (C.wmask & 2u);
+    const float cxlo = xlo ? (float)il : C.wlo[0], cxhi = xhi ? (float)(il + 1) : C.whi[0];
+    float ymin, ymax;
+    slab_yrange(C.X, sxlo, cxlo, sxhi, cxhi, ymin, ymax);
+    if (ymin > ymax) return false;
+    j0 = C.G.ib0[1] + (int)floorf(ymin);
+    j1 = C.G.ib0[1] + (int)ceilf(ymax);
+    if (j0 < C.G.ib0[1]) j0 = C.G.ib0[1];
+    if (j1 > C.ib1[1]) j1 = C.ib1[1];
+    if (j1 <= j0) j1 = j0 + 1;
+    if (j1 > g.n[1]) j1 = g.n[1];
+    return j1 > j0;
+}
+
+/* column task: tet index, copy, and the column's cell (i, j) relative to
+ * the copy's index box; 32 + 3 + 14 + 14 bits */
+PVX_HD uint64_t pack_task(int64_t t, int h, int il, int jl) {
+    return (uint64_t)t | ((uint64_t)h << 32) | ((uint64_t)il << 35) | ((uint64_t)jl << 49);
+}
+PVX_HD void unpack_task(uint64_t w, int64_t& t, int& h, int& il, int& jl) {
+    t = (int64_t)(w & 0xffffffffull);
+    h = (int)((w >> 32) & 7u);
+    il = (int)((w >> 35) & 0x3fffu);
+    jl = (int)((w >> 49) & 0x3fffu);
+}
+
+PVX_HD bool finite_rec(const double* rec) {
+    bool f = true;
+    for (int t = 0; t < 12; ++t) f = f && (rec[t] - rec[t] == 0.0);
+    return f;
+}
+
+/* limits of the task encoding: column offsets within a copy's index box,
+ * cells along z, and tasks per tet (the largest-fragment key) */
+constexpr int TASK_MAXIJ = 1 << 14;
+constexpr int TASK_MAXK = 1 << 10;
+constexpr int64_t TASK_MAXPERTET = 1 << 22;
+
+/* Enumerate the column tasks of one tet record; writes them to out (when
+ * given) and returns the count, or -1 when the tet cannot take the column
+ * path (non-finite vertices, or an index box beyond the task encoding): the
+ * caller then runs the FP64 kernel.  ncopies gets the number of copies. */
+PVX_HD int64_t tet_columns(const double* rec, int64_t t, const GridInfo& g, const MeshBox& mb, uint64_t* out,
+                           int* ncopies) {
+    if (ncopies) *ncopies = 0;
+    if (!finite_rec(rec)) return -1;
     double lo[3], hi[3];
     aabb((const double (*)[3])rec, 4, lo, hi);
-    uint32_t shp; /* 3 bits per copy */
+    uint32_t shp;
     const int ng = ghost_shifts_packed(lo, hi, g, mb, shp);
-    cnt.ncopies += ng;
-    double sum_m0 = 0.0;
-    FastFrag B;
-    B.M.zero();
-#pragma unroll 1
+    if (ncopies) *ncopies = ng;
+    int64_t n = 0;
     for (int h = 0; h < ng; ++h) {
-        CopyGeo G;
-        G.rec = rec;
-        int ib1[3];
-        unsigned wmask = 0; /* window planes: bit 2a lower, 2a+1 upper */
-        for (int a = 0; a < 3; ++a) {
-            const double s = mb.periodic ? (mb.hi[a] - mb.lo[a]) * (double)(1 & (shp >> (3 * h + a))) : 0.0;
-            G.shift[a] = s;
-            /* window planes and index box, as src/geometry.c:816-841 */
-            double cmin = lo[a] + s, cmax = hi[a] + s;
-            if (g.w0[a] > cmin) {
-                cmin = g.w0[a];
-                wmask |= 1u << (2 * a);
-            }
-            if (g.w1[a] < cmax) {
-                cmax = g.w1[a];
-                wmask |= 2u << (2 * a);
-            }
-            index_range(g, a, cmin, cmax, G.ib0[a], ib1[a]);
-            if (ib1[a] <= G.ib0[a]) ib1[a] = G.ib0[a] + 1;
-        }
-        if (G.ib0[0] >= g.n[0] || G.ib0[1] >= g.n[1] || G.ib0[2] >= g.n[2]) continue;
-        /* window planes in local units (used only when the copy crosses them) */
-        float wlo[3], whi[3];
-        for (int a = 0; a < 3; ++a) {
-            wlo[a] = (float)(-G.ib0[a]);
-            whi[a] = (float)((g.w1[a] - g.w0[a]) / g.d[a] - G.ib0[a]);
-        }
-        /* copy geometry in grid units relative to the index box corner */
-        float X[4][3];
-        for (int a = 0; a < 3; ++a) {
-            const double inv = 1.0 / g.d[a];
-            for (int v = 0; v < 4; ++v) X[v][a] = (float)((rec[3 * v + a] + G.shift[a] - g.w0[a]) * inv - G.ib0[a]);
-            G.x0[a] = X[0][a];
-            for (int b = 0; b < 3; ++b) G.DX[a][b] = (float)((rec[3 * (b + 1) + a] - rec[a]) * inv);
-        }
-#pragma unroll 1
-        for (int i = G.ib0[0]; i < ib1[0]; ++i) {
-            const int il = i - G.ib0[0];
-            const bool xlo = i > G.ib0[0], xhi = i < ib1[0] - 1;
-            /* slab planes for the y-range, window planes included */
-            const bool sxlo = xlo || (wmask & 1u), sxhi = xhi || (wmask & 2u);
-            const float cxlo = xlo ? (float)il : wlo[0], cxhi = xhi ? (float)(il + 1) : whi[0];
-            float ymin, ymax;
-            slab_yrange(X, sxlo, cxlo, sxhi, cxhi, ymin, ymax);
-            if (ymin > ymax) continue;
-            int j0 = G.ib0[1] + (int)floorf(ymin), j1 = G.ib0[1] + (int)ceilf(ymax);
-            if (j0 < G.ib0[1]) j0 = G.ib0[1];
-            if (j1 > ib1[1]) j1 = ib1[1];
-            if (j1 <= j0) j1 = j0 + 1;
-            if (j1 > g.n[1]) j1 = g.n[1];
-#pragma unroll 1
+        CopySetup C;
+        if (!copy_setup(rec, lo, hi, shp, h, g, mb, C)) continue;
+        if (C.ib1[0] - C.G.ib0[0] >= TASK_MAXIJ || C.ib1[1] - C.G.ib0[1] >= TASK_MAXIJ ||
+            C.ib1[2] - C.G.ib0[2] >= TASK_MAXK)
+            return -1;
+        for (int i = C.G.ib0[0]; i < C.ib1[0]; ++i) {
+            int j0, j1;
+            if (!slab_columns(C, i, g, j0, j1)) continue;
             for (int j = j0; j < j1; ++j) {
-                const int jl = j - G.ib0[1];
-                /* the column: tet clipped by its x and y planes (and window) */
-                P.nv = 4;
-                P.setq(0, 0.f, 0.f, 0.f);
-                P.setq(1, 1.f, 0.f, 0.f);
-                P.setq(2, 0.f, 1.f, 0.f);
-                P.setq(3, 0.f, 0.f, 1.f);
-                P.setnbw(0, pack3(1, 3, 2));
-                P.setnbw(1, pack3(2, 3, 0));
-                P.setnbw(2, pack3(0, 3, 1));
-                P.setnbw(3, pack3(1, 2, 0));
-                /* the column's four planes: interior grid planes, or the
-                 * window planes on the first/last slab and column of a copy
-                 * that crosses the window (src/geometry.c:816-843) */
-                const bool yhi = j < j1 - 1, ylo = j > j0;
-                bool ok = true;
-                if (xhi)
-                    ok = clip_f(P, G, g, (float)(il + 1), 0, false);
-                else if (wmask & 2u)
-                    ok = clip_f(P, G, g, whi[0], 0, false);
-                if (ok && xlo)
-                    ok = clip_f(P, G, g, (float)il, 0, true);
-                else if (ok && (wmask & 1u))
-                    ok = clip_f(P, G, g, wlo[0], 0, true);
-                if (ok && yhi)
-                    ok = clip_f(P, G, g, (float)(jl + 1), 1, false);
-                else if (ok && j == ib1[1] - 1 && (wmask & 8u))
-                    ok = clip_f(P, G, g, whi[1], 1, false);
-                if (ok && ylo)
-                    ok = clip_f(P, G, g, (float)jl, 1, true);
-                else if (ok && j == G.ib0[1] && (wmask & 4u))
-                    ok = clip_f(P, G, g, wlo[1], 1, true);
-                if (!ok) return false;
-                if (P.nv <= 0) continue;
-                int k0, k1;
-                lpoly_range(P, G, 2, G.ib0[2], G.ib0[2], ib1[2], g.n[2], k0, k1);
-#pragma unroll 1
-                for (int k = k0; k < k1; ++k) {
-                    const int kl = k - G.ib0[2];
-                    Mom Mm;
-                    float O[3];
-                    /* cell planes, or the window planes at the ends */
-                    const bool zlo = k > k0 || (k == G.ib0[2] && (wmask & 16u));
-                    const bool zhi = k < k1 - 1 || (k == ib1[2] - 1 && (wmask & 32u));
-                    const float lo_c = k > k0 ? (float)kl : wlo[2];
-                    const float hi_c = k < k1 - 1 ? (float)(kl + 1) : whi[2];
-                    slab_moments<ORDER>(P, G, 2, zlo, lo_c, zhi, hi_c, Mm, O);
-                    if (!(Mm.m0 > 0.0f)) continue;
-                    cnt.nfrag++;
-                    sum_m0 += (double)Mm.m0;
-                    if (Mm.m0 > B.M.m0) { /* new largest: keep it, deposit the old one */
-                        FastFrag t = B;
-                        B.M = Mm;
-                        B.O[0] = O[0];
-                        B.O[1] = O[1];
-                        B.O[2] = O[2];
-                        B.G = G;
-                        B.i = i;
-                        B.j = j;
-                        B.k = k;
-                        if (!(t.M.m0 > 0.0f)) continue;
-                        deposit_f<ORDER>(sink, g, t.G, t.G.ib0, t.i, t.j, t.k, t.M, t.O, u0, (const float (*)[3])DV,
-                                         mass, (double)t.M.m0);
-                    } else {
-                        deposit_f<ORDER>(sink, g, G, G.ib0, i, j, k, Mm, O, u0, (const float (*)[3])DV, mass,
-                                         (double)Mm.m0);
-                    }
-                }
+                if (out) out[n] = pack_task(t, h, i - C.G.ib0[0], j - C.G.ib0[1]);
+                n++;
             }
         }
     }
-    /* the largest fragment absorbs the tet's mass residual (exact target
-     * minus the FP64 sum of the FP32 fragment masses) in every moment */
-    if (B.M.m0 > 0.0f)
-        deposit_f<ORDER>(sink, g, B.G, B.G.ib0, B.i, B.j, B.k, B.M, B.O, u0, (const float (*)[3])DV, mass,
-                         (double)B.M.m0 + (target - sum_m0));
+    return n < TASK_MAXPERTET ? n : -1;
+}
+
+/* Build the column polyhedron of task (copy C, cell i, j) in P: the tet
+ * clipped by the column's four planes -- interior grid planes or, on the
+ * copy's first/last slab or column, the window planes it crosses
+ * (src/geometry.c:816-843).  False on a vertex-budget overflow. */
+PVX_HD bool build_column(LPoly& P, const CopySetup& C, const GridInfo& g, int i, int j, int j0, int j1) {
+    const CopyGeo& G = C.G;
+    const int il = i - G.ib0[0], jl = j - G.ib0[1];
+    const bool xlo = i > G.ib0[0], xhi = i < C.ib1[0] - 1, ylo = j > j0, yhi = j < j1 - 1;
+    P.nv = 4;
+    P.setq(0, 0.f, 0.f, 0.f);
+    P.setq(1, 1.f, 0.f, 0.f);
+    P.setq(2, 0.f, 1.f, 0.f);
+    P.setq(3, 0.f, 0.f, 1.f);
+    P.setnbw(0, pack3(1, 3, 2));
+    P.setnbw(1, pack3(2, 3, 0));
+    P.setnbw(2, pack3(0, 3, 1));
+    P.setnbw(3, pack3(1, 2, 0));
+    bool ok = true;
+    if (xhi)
+        ok = clip_f(P, G, g, (float)(il + 1), 0, false);
+    else if (C.wmask & 2u)
+        ok = clip_f(P, G, g, C.whi[0], 0, false);
+    if (ok && xlo)
+        ok = clip_f(P, G, g, (float)il, 0, true);
+    else if (ok && (C.wmask & 1u))
+        ok = clip_f(P, G, g, C.wlo[0], 0, true);
+    if (ok && yhi)
+        ok = clip_f(P, G, g, (float)(jl + 1), 1, false);
+    else if (ok && j == C.ib1[1] - 1 && (C.wmask & 8u))
+        ok = clip_f(P, G, g, C.whi[1], 1, false);
+    if (ok && ylo)
+        ok = clip_f(P, G, g, (float)jl, 1, true);
+    else if (ok && j == G.ib0[1] && (C.wmask & 4u))
+        ok = clip_f(P, G, g, C.wlo[1], 1, true);
+    return ok;
+}
+
+/* cell k of a built column: moments (about O) of its z-slab piece */
+template <int ORDER>
+PVX_HD void column_cell(const LPoly& P, const CopySetup& C, int k, int k0, int k1, Mom& Mm, float* O) {
+    const int kl = k - C.G.ib0[2];
+    const bool zlo = k > k0 || (k == C.G.ib0[2] && (C.wmask & 16u));
+    const bool zhi = k < k1 - 1 || (k == C.ib1[2] - 1 && (C.wmask & 32u));
+    const float lo_c = k > k0 ? (float)kl : C.wlo[2];
+    const float hi_c = k < k1 - 1 ? (float)(kl + 1) : C.whi[2];
+    slab_moments<ORDER>(P, C.G, 2, zlo, lo_c, zhi, hi_c, Mm, O);
+}
+
+/* per-tet velocity data of a record */
+struct TetVel {
+    double u0[3];
+    float DV[3][3];
+};
+
+PVX_HD void tet_vel(const double* rec, TetVel& V) {
+    for (int a = 0; a < 3; ++a) {
+        V.u0[a] = rec[12 + a];
+        for (int b = 0; b < 3; ++b) V.DV[a][b] = (float)(rec[12 + 3 * (b + 1) + a] - rec[12 + a]);
+    }
+}
+
+/* the largest fragment of a column task: its moments, origin and cell */
+struct BestFrag {
+    Mom M;
+    float O[3];
+    int kl; /* z cell relative to the copy's index box; -1: none */
+};
+
+/* key of a task's largest fragment for the per-tet max: the positive FP32
+ * m0 orders as its bit pattern */
+PVX_HD uint64_t best_key(float m0, int kl) {
+    uint32_t mb;
+    memcpy(&mb, &m0, 4);
+    return ((uint64_t)mb << 32) | (uint64_t)(uint32_t)kl;
+}
+
+/*
+ * One column task: build the column, integrate and deposit its cells.
+ * Returns false on overflow (the caller falls back to the FP64 kernel).
+ * sum_m0 gets the FP64 sum of the task's fragment masses and B its largest
+ * fragment, for the tet's residual pass.
+ */
+template <int ORDER, class Sink>
+PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task, LPoly& P,
+                        int64_t& nfrag, double& sum_m0, BestFrag& B) {
+    int64_t t;
+    int h, il, jl;
+    unpack_task(task, t, h, il, jl);
+    sum_m0 = 0.0;
+    B.kl = -1;
+    B.M.m0 = 0.0f;
+    double lo[3], hi[3];
+    aabb((const double (*)[3])rec, 4, lo, hi);
+    uint32_t shp;
+    ghost_shifts_packed(lo, hi, g, mb, shp);
+    CopySetup C;
+    if (!copy_setup(rec, lo, hi, shp, h, g, mb, C)) return true;
+    const int i = C.G.ib0[0] + il, j = C.G.ib0[1] + jl;
+    int j0, j1;
+    if (!slab_columns(C, i, g, j0, j1)) return true;
+    if (!build_column(P, C, g, i, j, j0, j1)) return false;
+    if (P.nv <= 0) return true;
+    face_apexes(P);
+    int k0, k1;
+    lpoly_range(P, C.G, 2, C.G.ib0[2], C.G.ib0[2], C.ib1[2], g.n[2], k0, k1);
+    TetVel V;
+    if (ORDER > 0) tet_vel(rec, V);
+    const double mass = rec[24];
+#pragma unroll 1
+    for (int k = k0; k < k1; ++k) {
+        Mom Mm;
+        float O[3];
+        column_cell<ORDER>(P, C, k, k0, k1, Mm, O);
+        if (!(Mm.m0 > 0.0f)) continue;
+        deposit_f<ORDER>(sink, g, C.G, C.G.ib0, i, j, k, Mm, O, V.u0, (const float (*)[3])V.DV, mass, (double)Mm.m0);
+        nfrag++;
+        sum_m0 += (double)Mm.m0;
+        if (Mm.m0 > B.M.m0) {
+            B.M = Mm;
+            B.O[0] = O[0];
+            B.O[1] = O[1];
+            B.O[2] = O[2];
+            B.kl = k - C.G.ib0[2];
+        }
+    }
     return true;
+}
+
+/*
+ * The residual of one tet: its exact in-window mass fraction (record slot
+ * 26) minus the FP64 sum of its FP32 fragment masses goes into its largest
+ * fragment (task word `task`, fragment B), deposited with that fragment's
+ * own means and covariances so every moment stays consistent.
+ */
+template <int ORDER, class Sink>
+PVX_HD void tet_residual(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task,
+                         const BestFrag& B, double sum_m0) {
+    if (B.kl < 0 || !(B.M.m0 > 0.0f)) return;
+    const double r = rec[26] - sum_m0;
+    int64_t t;
+    int h, il, jl;
+    unpack_task(task, t, h, il, jl);
+    double lo[3], hi[3];
+    aabb((const double (*)[3])rec, 4, lo, hi);
+    uint32_t shp;
+    ghost_shifts_packed(lo, hi, g, mb, shp);
+    CopySetup C;
+    if (!copy_setup(rec, lo, hi, shp, h, g, mb, C)) return;
+    TetVel V;
+    if (ORDER > 0) tet_vel(rec, V);
+    deposit_f<ORDER>(sink, g, C.G, C.G.ib0, C.G.ib0[0] + il, C.G.ib0[1] + jl, C.G.ib0[2] + B.kl, B.M, B.O, V.u0,
+                     (const float (*)[3])V.DV, rec[24], r);
 }
 
 }  // namespace pvx

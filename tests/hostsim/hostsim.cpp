@@ -66,8 +66,8 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
     LPoly P{&lq[0], &lq[NVF * FBLOCK], &lq[2 * NVF * FBLOCK], &lnb[0], 0};
     HostSink sink{acc.data()};
     std::vector<double> tets;
-    FastCounters cnt{0, 0};
-    int64_t ntets = 0, fails = 0;
+    std::vector<uint64_t> tasks;
+    int64_t ntets = 0, fails = 0, nfrag = 0, ncopies = 0, ntasks = 0;
     for (int64_t e = 0; e < nelem; ++e) {
         int64_t nt = expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, nullptr);
         if (nt == 0) continue;
@@ -75,25 +75,57 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
         expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, tets.data());
         for (int64_t t = 0; t < nt; ++t) {
             double* r = &tets[(size_t)t * TREC];
+            if (r[25] == 0.0) continue;
             if (!(mb.periodic && mb.full_window)) r[26] = window_target(r, g, mb, pool);
-            bool ok;
+            int nc = 0;
+            const int64_t ncol = tet_columns(r, t, g, mb, nullptr, &nc);
+            ncopies += nc;
+            if (ncol < 0) {
+                fails++;
+                continue;
+            }
+            tasks.resize((size_t)ncol);
+            tet_columns(r, t, g, mb, tasks.data(), nullptr);
+            ntasks += ncol;
+            double sum = 0.0;
+            BestFrag best;
+            best.kl = -1;
+            best.M.m0 = 0.0f;
+            uint64_t best_task = 0;
+            for (int64_t c = 0; c < ncol; ++c) {
+                double s1;
+                BestFrag b1;
+                bool ok;
+                if (g.order == 0)
+                    ok = column_task<0>(sink, g, mb, r, tasks[c], P, nfrag, s1, b1);
+                else if (g.order == 1)
+                    ok = column_task<1>(sink, g, mb, r, tasks[c], P, nfrag, s1, b1);
+                else
+                    ok = column_task<2>(sink, g, mb, r, tasks[c], P, nfrag, s1, b1);
+                if (!ok) fails++;
+                sum += s1;
+                if (b1.kl >= 0 && best_key(b1.M.m0, b1.kl) > best_key(best.M.m0, best.kl)) {
+                    best = b1;
+                    best_task = tasks[c];
+                }
+            }
             if (g.order == 0)
-                ok = fast_tet<0>(sink, g, mb, r, P, cnt);
+                tet_residual<0>(sink, g, mb, r, best_task, best, sum);
             else if (g.order == 1)
-                ok = fast_tet<1>(sink, g, mb, r, P, cnt);
+                tet_residual<1>(sink, g, mb, r, best_task, best, sum);
             else
-                ok = fast_tet<2>(sink, g, mb, r, P, cnt);
-            if (!ok) fails++;
+                tet_residual<2>(sink, g, mb, r, best_task, best, sum);
         }
         ntets += nt;
     }
     for (int64_t c = 0; c < nvox; ++c) cell_finalize(g, acc.data(), F, c);
     delete RS;
     delete[] pool;
-    out_stats[0] = cnt.nfrag;
-    out_stats[1] = cnt.ncopies;
+    out_stats[0] = nfrag;
+    out_stats[1] = ncopies;
     out_stats[2] = ntets;
     out_stats[3] = fails;
+    out_stats[4] = ntasks;
     return 0;
 }
 

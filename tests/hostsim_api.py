@@ -19,11 +19,11 @@ def lib():
 
 
 def run(P, kernel="fast", reftol=1.0, maxlvl=0, xvfix=False, out=None):
-    """P: an oracle.Problem; returns (fields, stats[nfrag, ncopies, ntets, fails/status])."""
+    """P: an oracle.Problem; returns (fields, stats[nfrag, ncopies, ntets, fails/status, column tasks])."""
     out = P.zero_fields() if out is None else out
     order = ["m", "x", "v", "xx", "xv", "vv"]
     fptr = (C.c_void_p * 6)(*[out[f].ctypes.data if f in out else None for f in order])
-    st = np.zeros(4, np.int64)
+    st = np.zeros(5, np.int64)
     n = (C.c_int * 3)(*P.n)
     win = (C.c_double * 6)(*P.bounds.reshape(-1))
     d = (C.c_double * 3)(*P.d)
