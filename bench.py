@@ -191,7 +191,11 @@ def main():
     fk = field_key(fields)
     kavg = sum(kms) / len(kms)
     achieved = nfrag * FLOPS_PER_FRAG[fk] / (kavg * 1e-3) / 1e12
-    traffic = load_traffic().get(wname, {}).get(st["kernel"] == 1 and "plain" or "fast")
+    # HBM bytes per launch of the voxelization kernel from rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel's reading
+    # (pypsi_amd/profiling.py); None until measured for this workload
+    tr = load_traffic().get(wname, {}).get("plain" if st["kernel"] == 1 else "fast") or {}
+    traffic = tr.get("bytes")
     out = {
         "metric": "tet-voxel fragments/s, 128^3 mesh\u2192256^3 grid; plain vs batched, fwd+adjoint",
         "value": value,
