@@ -132,10 +132,13 @@ int psivox_voxels_batched(int32_t nmesh, const psivox_mesh* meshes, psivox_grid*
                           psivox_stats* stats);
 
 /* Adjoint of the mass deposit.  No reference counterpart (the reference has
- * no derivatives).  Given dL/dm per cell (float64, shaped like grid m),
- * accumulates dL/dpos (npart x 3 float64) of the mass field m produced by
- * psivox_voxels on the same mesh and grid (refine_max_lvl 0).  Pointers follow
- * mesh->on_device / grid->on_device. */
+ * no derivatives; the forward it differentiates is psi_voxels,
+ * src/cpsi.c:41, density branch :100-112).  Given dL/dm per cell (float64,
+ * shaped like grid m), ADDS dL/dpos (npart x 3 float64) of the mass field m
+ * that psivox_voxels produces on the same mesh and grid into dl_dpos
+ * (refine_max_lvl 0, or simplex meshes).  dl_dm is a device pointer when
+ * grid->on_device, dl_dpos when mesh->on_device; grid->fields is not used.
+ * Errors: PSIVOX_E_MODE for refine_max_lvl > 0 on 8/27-node elements. */
 int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,
                         const psivox_opts* opts);
 

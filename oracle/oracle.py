@@ -62,6 +62,8 @@ def lib():
         L.orc_list_fragments.restype = C.c_int64
         L.orc_count_tets.argtypes = [P, P, C.c_double, C.c_int, P]
         L.orc_voxelize_one_tet.argtypes = [P, P, P, P, C.c_double, C.c_int, P]
+        L.orc_mass_adjoint.argtypes = [P, P, P, P]
+        L.orc_fd_mass_loss.argtypes = [P, P, P, P, C.c_int, C.c_int64, C.c_int, C.c_double, P]
         _lib = L
     return _lib
 
@@ -171,6 +173,49 @@ def list_fragments(prob: Problem, reftol=1.0, maxlvl=0, e0=0, e1=None, cap=1 << 
     if n > cap:
         return list_fragments(prob, reftol, maxlvl, e0, e1, cap=int(n))
     return gi[:n].copy(), mm[:n].copy()
+
+
+def mass_adjoint(prob: Problem, w):
+    """FP64 dL/dpos (npart, 3) for w = dL/dm on the grid (refine level 0)."""
+    L = lib()
+    w = np.ascontiguousarray(w, dtype=np.float64).reshape(prob.n)
+    m, g = prob._omesh(), prob._ogrid({})
+    out = np.zeros((prob.pos.shape[0], 3))
+    L.orc_mass_adjoint(C.byref(m), C.byref(g), w.ctypes.data, out.ctypes.data)
+    return out
+
+
+def particle_elements(prob: Problem):
+    """CSR map particle -> elements that use it: (start, elems)."""
+    conn = prob.conn.astype(np.int64)
+    pid = conn.reshape(-1)
+    eid = np.repeat(np.arange(conn.shape[0], dtype=np.int64), conn.shape[1])
+    order = np.lexsort((eid, pid))
+    pid, eid = pid[order], eid[order]
+    keep = np.ones(pid.size, bool)
+    keep[1:] = (pid[1:] != pid[:-1]) | (eid[1:] != eid[:-1])
+    pid, eid = pid[keep], eid[keep]
+    start = np.searchsorted(pid, np.arange(prob.pos.shape[0] + 1))
+    return start, np.ascontiguousarray(eid)
+
+
+def fd_mass_loss_grad(prob: Problem, r, particles, axes, h=1e-6, csr=None):
+    """FP64 central differences of L = sum_C (m_C/V - rho_t)^2 with respect to
+    pos[p, a] for each (p, a) pair; r = m/V - rho_t of the unperturbed mesh."""
+    L = lib()
+    r = np.ascontiguousarray(r, dtype=np.float64).reshape(prob.n)
+    start, elems = csr if csr is not None else particle_elements(prob)
+    m, g = prob._omesh(), prob._ogrid({})
+    out = np.zeros(len(particles))
+    val = C.c_double()
+    for i, (p, a) in enumerate(zip(particles, axes)):
+        s0, s1 = int(start[p]), int(start[p + 1])
+        rc = L.orc_fd_mass_loss(C.byref(m), C.byref(g), r.ctypes.data, elems[s0:].ctypes.data, s1 - s0, int(p),
+                                int(a), float(h), C.byref(val))
+        if rc != 0:
+            raise RuntimeError("oracle finite difference failed")
+        out[i] = val.value
+    return out
 
 
 def count_tets(prob: Problem, reftol=1.0, maxlvl=0):

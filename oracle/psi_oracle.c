@@ -1044,3 +1044,33 @@ int orc_voxelize_one_tet(const orc_mesh* mesh, orc_grid* g, const double* pos12,
     if (st) *st = local;
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* entry points for psi_adjoint.c (same semantics as above)             */
+/* ------------------------------------------------------------------ */
+
+int orc_internal_wrap(double (*p)[3], int nv, double lo[3], double hi[3], const orc_grid* g, const orc_mesh* m) {
+    return o_wrap_and_test(p, nv, lo, hi, g, m);
+}
+
+int orc_internal_ghosts(double (*gp)[3], double (*gb)[3], const orc_grid* g, const orc_mesh* m) {
+    return o_ghosts(gp, gb, g, m);
+}
+
+/* (voxel index, mass * m0) of the fragments of one tet copy x with bounding
+ * box rb; returns the count (may exceed cap) */
+int64_t orc_internal_tet_fragments(const orc_mesh* mesh, const orc_grid* g, const double (*x)[3], const double (*rb)[3],
+                                   double mass, int64_t* gi, double* mm, int64_t cap) {
+    (void)mesh;
+    orc_stats st;
+    memset(&st, 0, sizeof st);
+    oscratch* S = (oscratch*)malloc(sizeof(oscratch));
+    if (!S) return -1;
+    o_scratch_init(S);
+    o_list_ctx ctx = {cap, 0, gi, mm, mass};
+    const double u[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    o_voxelize_tet(S, (orc_grid*)g, x, u, mass, rb, &st, 0, 0, o_list_fn, &ctx);
+    o_scratch_free(S);
+    free(S);
+    return ctx.n;
+}

@@ -49,4 +49,13 @@ int orc_count_tets(const orc_mesh* mesh, const orc_grid* g, double reftol, int m
 int orc_voxelize_one_tet(const orc_mesh* mesh, orc_grid* g, const double* pos12, const double* vel12, double mass,
                          int xvfix, orc_stats* st);
 
+/* mass adjoint (psi_adjoint.c), refine_max_lvl = 0: dpos = dL/dpos
+ * (npart x 3, overwritten) from w = dL/dm per cell */
+int orc_mass_adjoint(const orc_mesh* mesh, const orc_grid* g, const double* w, double* dpos);
+/* central difference (L(x+h) - L(x-h)) / 2h of L = sum (m/V - rho_t)^2 for
+ * particle p, axis a; r = m/V - rho_t of the unperturbed mesh; elems are
+ * the elements using p */
+int orc_fd_mass_loss(orc_mesh* mesh, const orc_grid* g, const double* r, const int64_t* elems, int nel, int64_t p,
+                     int a, double h, double* out);
+
 #endif

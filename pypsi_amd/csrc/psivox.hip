@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/psivox.h"
+#include "psivox_adjoint.h"
 #include "psivox_fast.h"
 #include "psivox_geom.h"
 #include "psivox_plain.h"
@@ -356,6 +357,70 @@ static int launch_residual(const double* tets, int64_t t0, int64_t t1, const Gri
 }
 
 /* ------------------------------------------------------------------ */
+/* mass adjoint                                                        */
+/* ------------------------------------------------------------------ */
+
+/* adjoint tasks per tet record (0 for invalid records) */
+__global__ __launch_bounds__(256) void k_adj_count(const double* __restrict__ tets, int64_t ntet, GridInfo g,
+                                                   MeshBox mb, int64_t* __restrict__ cnt, unsigned* status) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntet) return;
+    const double* r = tets + (int64_t)TREC * t;
+    int64_t n = 0;
+    if (r[25] != 0.0) {
+        n = adj_tasks(r, t, g, mb, nullptr);
+        if (n < 0) {
+            atomicOr(status, ST_BADGEOM);
+            n = 0;
+        }
+    }
+    cnt[t] = n;
+}
+
+__global__ __launch_bounds__(256) void k_adj_emit(const double* __restrict__ tets, int64_t t0, int64_t t1, GridInfo g,
+                                                  MeshBox mb, const int64_t* __restrict__ start, int64_t base,
+                                                  uint64_t* __restrict__ tasks) {
+    const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= t1) return;
+    const double* r = tets + (int64_t)TREC * t;
+    if (r[25] == 0.0 || start[t + 1] == start[t]) return;
+    adj_tasks(r, t, g, mb, tasks + (start[t] - base));
+}
+
+struct AdjArgs {
+    const double* w; /* dL/dm */
+    MeshDev md;
+    int64_t e0;      /* first element of the batch */
+    int per;         /* tets per element */
+    double* dpos;    /* dL/dpos, npart x 3 */
+};
+
+/* one lane per (tet, copy, axis, plane): FP64 section integrals scattered
+ * to the tet's particles */
+__global__ __launch_bounds__(FBLOCK) void k_adj(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
+                                                int64_t ntask, GridInfo g, MeshBox mb, AdjArgs A) {
+    __shared__ double ru[8][FBLOCK];
+    __shared__ double rv[8][FBLOCK];
+    const int lane = threadIdx.x;
+    const int64_t c = (int64_t)blockIdx.x * FBLOCK + lane;
+    if (c >= ntask) return;
+    const uint64_t w = tasks[c];
+    int64_t t;
+    int h, ax, p;
+    unpack_adj(w, t, h, ax, p);
+    double o[4];
+    Poly2<FBLOCK> R{&ru[0][lane], &rv[0][lane]};
+    adj_plane_task<FBLOCK>(g, mb, tets + (int64_t)TREC * t, w, A.w, R, o);
+    const int64_t e = A.e0 + t / A.per;
+    const int tl = (int)(t % A.per);
+    for (int v = 0; v < 4; ++v) {
+        if (o[v] == 0.0) continue;
+        const int64_t id = A.md.conn[e * A.md.vpe + tet_node(A.md.vpe, tl, v)];
+        unsafeAtomicAdd(A.dpos + 3 * id + ax, o[v]);
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* host side                                                           */
 /* ------------------------------------------------------------------ */
 
@@ -391,7 +456,7 @@ struct DevBuf {
 
 struct Workspace {
     DevBuf pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
-        colsum, colkey, colmom;
+        colsum, colkey, colmom, ptets, adj_w, adj_g;
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -442,6 +507,95 @@ int upload(DevBuf& b, const void* src, size_t bytes, int dev, hipStream_t s, con
 
 inline unsigned grid_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+/* the mesh arrays as device pointers (uploaded when they are on the host) */
+int mesh_to_device(Workspace& W, const psivox_mesh* mesh, int dev, hipStream_t s, MeshDev& md) {
+    int rc;
+    md = MeshDev{};
+    md.vpe = mesh->elemtype;
+    if (mesh->nelem <= 0) return 0;
+    if (mesh->on_device) {
+        md.pos = mesh->pos;
+        md.vel = mesh->vel;
+        md.mass = mesh->mass;
+        md.conn = mesh->conn;
+        return 0;
+    }
+    const void* p;
+    if ((rc = upload(W.pos, mesh->pos, (size_t)mesh->npart * 3 * sizeof(double), dev, s, &p))) return rc;
+    md.pos = (const double*)p;
+    if (mesh->vel) {
+        if ((rc = upload(W.vel, mesh->vel, (size_t)mesh->npart * 3 * sizeof(double), dev, s, &p))) return rc;
+        md.vel = (const double*)p;
+    }
+    if ((rc = upload(W.mass, mesh->mass, (size_t)mesh->nelem * sizeof(double), dev, s, &p))) return rc;
+    md.mass = (const double*)p;
+    if ((rc = upload(W.conn, mesh->conn, (size_t)mesh->nelem * md.vpe * sizeof(int32_t), dev, s, &p))) return rc;
+    md.conn = (const int32_t*)p;
+    return 0;
+}
+
+/*
+ * Column tasks of tet records [0, nt): count, scan, and for each chunk of
+ * tets whose tasks fit the task buffer emit them and call
+ * fn(a0, a1, colstart, base, ntask_chunk).  Returns PSIVOX_E_* or 0; *fell
+ * is set (and nothing run) when a tet cannot take the column path.
+ */
+template <class Fn>
+int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const GridInfo& g, const MeshBox& mb,
+                    unsigned* status, unsigned long long* counters, hipStream_t s, int dev, bool* fell, Fn fn) {
+    int rc;
+    *fell = false;
+    if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
+    if ((rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
+    int64_t* colcnt = (int64_t*)W.colcnt.p;
+    int64_t* colstart = (int64_t*)W.colstart.p;
+    HIPCHK(hipMemsetAsync(colcnt + nt, 0, sizeof(int64_t), s));
+    k_col_count<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, colcnt, status, counters);
+    HIPCHK(hipGetLastError());
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, colcnt, colstart, nt + 1, s));
+    if ((rc = W.scan_tmp.get(tb, dev))) return rc;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp.p, tb, colcnt, colstart, nt + 1, s));
+    int64_t ntask = 0;
+    unsigned hs = 0;
+    HIPCHK(hipMemcpyAsync(&ntask, colstart + nt, sizeof ntask, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hs, status, sizeof hs, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hs & ST_FALLBACK) {
+        *fell = true;
+        return 0;
+    }
+    if (ntask == 0) return 0;
+    /* tet sub-ranges whose tasks fit the task buffers */
+    const int64_t cap = (int64_t)1 << 28;
+    std::vector<int64_t> cuts{0, nt};
+    std::vector<int64_t> hcs;
+    if (ntask > cap) {
+        hcs.resize(nt + 1);
+        HIPCHK(hipMemcpy(hcs.data(), colstart, (nt + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        cuts.assign(1, 0);
+        while (cuts.back() < nt) {
+            const int64_t a0 = cuts.back();
+            int64_t a1 = std::upper_bound(hcs.begin() + a0 + 1, hcs.end(), hcs[a0] + cap) - hcs.begin() - 1;
+            if (a1 <= a0) a1 = a0 + 1; /* a single tet has fewer than TASK_MAXPERTET < cap tasks */
+            cuts.push_back(a1);
+        }
+    }
+    const int64_t tcap = std::min(ntask, cap);
+    if ((rc = W.tasks.get((size_t)tcap * sizeof(uint64_t), dev))) return rc;
+    uint64_t* tasks = (uint64_t*)W.tasks.p;
+    for (size_t q = 0; q + 1 < cuts.size(); ++q) {
+        const int64_t a0 = cuts[q], a1 = cuts[q + 1];
+        const int64_t base = hcs.empty() ? 0 : hcs[a0];
+        const int64_t nsub = hcs.empty() ? ntask : hcs[a1] - hcs[a0];
+        if (nsub == 0) continue;
+        k_col_emit<<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, g, mb, colstart, base, tasks);
+        HIPCHK(hipGetLastError());
+        if ((rc = fn(a0, a1, (const int64_t*)colstart, base, (const uint64_t*)tasks, nsub))) return rc;
+    }
+    return 0;
+}
+
 /* One (mesh, grid) voxelization on stream s with workspace W. */
 int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* o, psivox_stats* st,
             hipStream_t s, int dev) {
@@ -471,27 +625,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
 
     /* mesh arrays on the device */
     MeshDev md{};
-    md.vpe = vpe;
-    if (mesh->nelem > 0) {
-        if (mesh->on_device) {
-            md.pos = mesh->pos;
-            md.vel = mesh->vel;
-            md.mass = mesh->mass;
-            md.conn = mesh->conn;
-        } else {
-            const void* p;
-            if ((rc = upload(W.pos, mesh->pos, (size_t)mesh->npart * 3 * sizeof(double), dev, s, &p))) return rc;
-            md.pos = (const double*)p;
-            if (mesh->vel) {
-                if ((rc = upload(W.vel, mesh->vel, (size_t)mesh->npart * 3 * sizeof(double), dev, s, &p))) return rc;
-                md.vel = (const double*)p;
-            }
-            if ((rc = upload(W.mass, mesh->mass, (size_t)mesh->nelem * sizeof(double), dev, s, &p))) return rc;
-            md.mass = (const double*)p;
-            if ((rc = upload(W.conn, mesh->conn, (size_t)mesh->nelem * vpe * sizeof(int32_t), dev, s, &p))) return rc;
-            md.conn = (const int32_t*)p;
-        }
-    }
+    if ((rc = mesh_to_device(W, mesh, dev, s, md))) return rc;
 
     /* grid fields on the device */
     double* fd[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -589,70 +723,35 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 k_vox_plain<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, acc, status, counters);
                 HIPCHK(hipGetLastError());
             } else {
-                /* column tasks: count, scan, emit */
-                if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
-                if ((rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
-                int64_t* colcnt = (int64_t*)W.colcnt.p;
-                int64_t* colstart = (int64_t*)W.colstart.p;
-                HIPCHK(hipMemsetAsync(colcnt + nt, 0, sizeof(int64_t), s));
-                k_col_count<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, colcnt, status, counters);
-                HIPCHK(hipGetLastError());
-                size_t tb = 0;
-                HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, colcnt, colstart, nt + 1, s));
-                if ((rc = W.scan_tmp.get(tb, dev))) return rc;
-                HIPCHK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp.p, tb, colcnt, colstart, nt + 1, s));
-                int64_t ntask = 0;
-                unsigned hs = 0;
-                HIPCHK(hipMemcpyAsync(&ntask, colstart + nt, sizeof ntask, hipMemcpyDeviceToHost, s));
-                HIPCHK(hipMemcpyAsync(&hs, status, sizeof hs, hipMemcpyDeviceToHost, s));
-                HIPCHK(hipStreamSynchronize(s));
-                if (!(hs & ST_FALLBACK) && ntask > 0) {
-                    /* tet sub-ranges whose tasks fit the task buffers */
-                    const int64_t cap = (int64_t)1 << 28;
-                    std::vector<int64_t> cuts{0, nt};
-                    std::vector<int64_t> hcs;
-                    if (ntask > cap) {
-                        hcs.resize(nt + 1);
-                        HIPCHK(hipMemcpy(hcs.data(), colstart, (nt + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
-                        cuts.assign(1, 0);
-                        while (cuts.back() < nt) {
-                            const int64_t a0 = cuts.back();
-                            int64_t a1 = std::upper_bound(hcs.begin() + a0 + 1, hcs.end(), hcs[a0] + cap) - hcs.begin() - 1;
-                            if (a1 <= a0) a1 = a0 + 1; /* a single tet is below TASK_MAXPERTET < cap */
-                            cuts.push_back(a1);
-                        }
-                    }
-                    const int64_t tcap = std::min(ntask, cap);
-                    if ((rc = W.tasks.get((size_t)tcap * sizeof(uint64_t), dev))) return rc;
-                    if ((rc = W.colsum.get((size_t)tcap * sizeof(double), dev))) return rc;
-                    if ((rc = W.colkey.get((size_t)tcap * sizeof(uint64_t), dev))) return rc;
-                    if (g.order > 0 && (rc = W.colmom.get((size_t)tcap * 16 * sizeof(float), dev))) return rc;
-                    uint64_t* tasks = (uint64_t*)W.tasks.p;
-                    TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
-                    for (size_t q = 0; q + 1 < cuts.size(); ++q) {
-                        const int64_t a0 = cuts[q], a1 = cuts[q + 1];
-                        const int64_t base = hcs.empty() ? 0 : hcs[a0];
-                        const int64_t nsub = hcs.empty() ? ntask : hcs[a1] - hcs[a0];
-                        if (nsub == 0) continue;
-                        k_col_emit<<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, g, mb, colstart, base,
-                                                                             tasks);
-                        HIPCHK(hipGetLastError());
+                /* column tasks, then the per-tet residual pass */
+                bool fell = false;
+                rc = for_task_chunks(
+                    W, tets, nt, g, mb, status, counters, s, dev, &fell,
+                    [&](int64_t a0, int64_t a1, const int64_t* colstart, int64_t base, const uint64_t* tasks,
+                        int64_t nsub) -> int {
+                        int rc2;
+                        if ((rc2 = W.colsum.get((size_t)nsub * sizeof(double), dev))) return rc2;
+                        if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
+                        if (g.order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev))) return rc2;
+                        TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
                         HIPCHK(hipEventRecord(W.ev0, s));
-                        rc = launch_cols(tets, tasks, nsub, g, mb, acc, out, status, counters, s);
-                        if (rc)
+                        rc2 = launch_cols(tets, tasks, nsub, g, mb, acc, out, status, counters, s);
+                        if (rc2)
                             return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
-                                           hipGetErrorString((hipError_t)rc));
+                                           hipGetErrorString((hipError_t)rc2));
                         HIPCHK(hipEventRecord(W.ev1, s));
                         HIPCHK(hipEventSynchronize(W.ev1));
                         float ms = 0.f;
                         HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
                         kms += ms;
-                        rc = launch_residual(tets, a0, a1, g, mb, colstart, base, tasks, out, acc, s);
-                        if (rc)
+                        rc2 = launch_residual(tets, a0, a1, g, mb, colstart, base, tasks, out, acc, s);
+                        if (rc2)
                             return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
-                                           hipGetErrorString((hipError_t)rc));
-                    }
-                }
+                                           hipGetErrorString((hipError_t)rc2));
+                        return 0;
+                    });
+                if (rc) return rc;
+                (void)fell; /* the status check below restarts with the FP64 kernel */
             }
             if (kernel == PSIVOX_KERNEL_PLAIN) {
                 HIPCHK(hipEventRecord(W.ev1, s));
@@ -721,6 +820,114 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     return 0;
 }
 
+/* dL/dpos += adjoint of the mass deposit (refine level 0) */
+int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,
+                const psivox_opts* o, hipStream_t s, int dev) {
+    int rc = check_mesh(mesh);
+    if (rc) return rc;
+    if (!dl_dm || !dl_dpos) return set_err(PSIVOX_E_ARG, "dl_dm and dl_dpos are required");
+    if (o->refine_max_lvl > 0 && mesh->elemtype != 4)
+        return set_err(PSIVOX_E_MODE, "the mass adjoint supports refine_max_lvl = 0 only");
+    GridInfo g;
+    for (int a = 0; a < 3; ++a) {
+        if (grid->n[a] <= 0) return set_err(PSIVOX_E_ARG, "grid n[%d] = %d must be positive", a, grid->n[a]);
+        if (!(grid->d[a] > 0.0)) return set_err(PSIVOX_E_ARG, "grid d[%d] must be positive", a);
+        g.n[a] = grid->n[a];
+        g.w0[a] = grid->window[0][a];
+        g.w1[a] = grid->window[1][a];
+        g.d[a] = grid->d[a];
+    }
+    g.flags = 0;
+    set_layout(g);
+    MeshBox mb;
+    mb.periodic = mesh->periodic ? 1 : 0;
+    mb.full_window = mb.periodic;
+    for (int a = 0; a < 3; ++a) {
+        mb.lo[a] = mesh->box[0][a];
+        mb.hi[a] = mesh->box[1][a];
+        if (mb.lo[a] != g.w0[a] || mb.hi[a] != g.w1[a]) mb.full_window = 0;
+    }
+    const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int vpe = mesh->elemtype;
+    const int per = vpe == 4 ? 1 : 6;
+    MeshDev md{};
+    if ((rc = mesh_to_device(W, mesh, dev, s, md))) return rc;
+    const double* w = dl_dm;
+    if (!grid->on_device) {
+        const void* p;
+        if ((rc = upload(W.adj_w, dl_dm, (size_t)nvox * sizeof(double), dev, s, &p))) return rc;
+        w = (const double*)p;
+    }
+    double* dpos = dl_dpos;
+    const size_t gbytes = (size_t)mesh->npart * 3 * sizeof(double);
+    if (!mesh->on_device && mesh->npart > 0) {
+        const void* p;
+        if ((rc = upload(W.adj_g, dl_dpos, gbytes, dev, s, &p))) return rc;
+        dpos = (double*)p;
+    }
+    if ((rc = W.counters.get(64, dev))) return rc;
+    unsigned long long* counters = (unsigned long long*)W.counters.p;
+    unsigned* status = (unsigned*)(counters + 4);
+    HIPCHK(hipMemsetAsync(counters, 0, 64, s));
+    const int64_t ne = mesh->nelem;
+    const int64_t batch = o->batch_elements > 0 ? o->batch_elements : (int64_t)1 << 22;
+    for (int64_t e0 = 0; e0 < ne; e0 += batch) {
+        const int64_t e1 = std::min(ne, e0 + batch);
+        const int64_t nt = (e1 - e0) * per;
+        if ((rc = W.tets.get((size_t)nt * TREC * sizeof(double), dev))) return rc;
+        double* tets = (double*)W.tets.p;
+        k_expand_fixed<<<grid_blocks(e1 - e0, 128), 128, 0, s>>>(md, g, mb, e0, e1 - e0, tets);
+        HIPCHK(hipGetLastError());
+        /* tasks: count, scan, emit (chunks of tets whose tasks fit) */
+        if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
+        if ((rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
+        int64_t* cnt = (int64_t*)W.colcnt.p;
+        int64_t* start = (int64_t*)W.colstart.p;
+        HIPCHK(hipMemsetAsync(cnt + nt, 0, sizeof(int64_t), s));
+        k_adj_count<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, cnt, status);
+        HIPCHK(hipGetLastError());
+        size_t tb = 0;
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, start, nt + 1, s));
+        if ((rc = W.scan_tmp.get(tb, dev))) return rc;
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp.p, tb, cnt, start, nt + 1, s));
+        std::vector<int64_t> hs((size_t)nt + 1);
+        HIPCHK(hipMemcpyAsync(hs.data(), start, (nt + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const int64_t cap = (int64_t)1 << 28;
+        if ((rc = W.tasks.get((size_t)std::min(hs[nt], cap) * sizeof(uint64_t) + 8, dev))) return rc;
+        uint64_t* tasks = (uint64_t*)W.tasks.p;
+        AdjArgs A;
+        A.w = w;
+        A.md = md;
+        A.e0 = e0;
+        A.per = per;
+        A.dpos = dpos;
+        for (int64_t a0 = 0; a0 < nt;) {
+            int64_t a1 = std::upper_bound(hs.begin() + a0 + 1, hs.end(), hs[a0] + cap) - hs.begin() - 1;
+            if (a1 <= a0) a1 = a0 + 1;
+            const int64_t nsub = hs[a1] - hs[a0];
+            if (nsub > 0) {
+                k_adj_emit<<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, g, mb, start, hs[a0], tasks);
+                HIPCHK(hipGetLastError());
+                k_adj<<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, tasks, nsub, g, mb, A);
+                HIPCHK(hipGetLastError());
+            }
+            a0 = a1;
+        }
+        if (o->progress && o->progress(o->progress_ctx, e1, ne)) {
+            (void)hipStreamSynchronize(s);
+            return set_err(PSIVOX_E_CANCELLED, "cancelled after %lld of %lld elements", (long long)e1, (long long)ne);
+        }
+    }
+    if (!mesh->on_device && mesh->npart > 0)
+        HIPCHK(hipMemcpyAsync(dl_dpos, dpos, gbytes, hipMemcpyDeviceToHost, s));
+    unsigned hst = 0;
+    HIPCHK(hipMemcpyAsync(&hst, status, sizeof hst, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hst & ST_BADGEOM) return set_err(PSIVOX_E_OVERFLOW, "mass adjoint: a tet has non-finite vertices");
+    return 0;
+}
+
 int prepare(Workspace& W, const psivox_opts* o, hipStream_t* s, int* dev) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(PSIVOX_E_NODEV, "no GPU visible");
@@ -784,12 +991,16 @@ int psivox_voxels_batched(int32_t nmesh, const psivox_mesh* meshes, psivox_grid*
 
 int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,
                         const psivox_opts* opts) {
-    (void)mesh;
-    (void)grid;
-    (void)dl_dm;
-    (void)dl_dpos;
-    (void)opts;
-    return set_err(PSIVOX_E_MODE, "adjoint not built yet");
+    if (!mesh || !grid) return set_err(PSIVOX_E_ARG, "mesh and grid are required");
+    psivox_opts o{};
+    o.refine_tolerance = 1.0;
+    if (opts) o = *opts;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    hipStream_t s;
+    int dev;
+    int rc = prepare(g_ws, &o, &s, &dev);
+    if (rc) return rc;
+    return run_adjoint(g_ws, mesh, grid, dl_dm, dl_dpos, &o, s, dev);
 }
 
 }  // extern "C"

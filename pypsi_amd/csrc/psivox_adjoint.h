@@ -1,0 +1,314 @@
+/*
+ * psivox_adjoint.h -- the mass adjoint: dL/dpos of the mass field that
+ * PSI.voxels(mode='density') deposits (the reference has no derivatives; the
+ * forward is psi_voxels, src/cpsi.c:41-175, density branch :100-112).
+ *
+ * A tet T puts mass_T * vol(T n C) / vol(T) into cell C (the fragment's m0,
+ * src/geometry.c:725-728).  Moving tet vertex v along axis a moves only the
+ * faces of T n C that lie on the cell's a-planes, with normal speed
+ * lambda_v (v's barycentric coordinate) there, so for w_C = dL/dm_C
+ *
+ *   dL/dx_{v,a} = mass_T / vol(T) * sum over grid planes x_a = c crossing T,
+ *                 over the cells (j, k) of the plane:
+ *                 (w_above - w_below) * int_{T n plane n cell(j,k)} lambda_v dA
+ *
+ * A cell outside the grid window has w = 0, so window planes enter like any
+ * other.  The formula is exact: in the tet's mass coordinates q the tet is
+ * the fixed unit simplex and only the cell planes x_a(q) = c move, the mass
+ * fraction is 6 vol_q(T n C), its derivative is 6 times the signed integral
+ * of lambda_v over the fragment's a-plane faces divided by |grad_q x_a|,
+ * and dA_q / |grad_q x_a| = dA_x / (6 vol(T)).
+ *
+ * One task = (tet, periodic copy, axis, plane).  The section of the tet by
+ * the plane is a triangle or a quadrilateral taken in FP64 straight from the
+ * tet record; it is swept row by row through the plane's cells, each piece
+ * clipped to its cell and integrated (area, first moment) on the fly in
+ * cell-local coordinates; lambda comes from an FP64 inverse of the tet's edge
+ * matrix.  Everything is FP64: the per-plane terms are large and cancel in
+ * the sum, so their rounding has to be far below the gradient's.
+ */
+#ifndef PSIVOX_ADJOINT_H
+#define PSIVOX_ADJOINT_H
+
+#include "psivox_geom.h"
+#include "psivox_refine.h"
+
+namespace pvx {
+
+/* adjoint task word: tet (32 bits), copy (3), axis (2), plane (17) */
+constexpr int ADJ_MAXPLANE = 1 << 17;
+
+PVX_HD uint64_t pack_adj(int64_t t, int h, int ax, int p) {
+    return (uint64_t)t | ((uint64_t)h << 32) | ((uint64_t)ax << 35) | ((uint64_t)p << 37);
+}
+PVX_HD void unpack_adj(uint64_t w, int64_t& t, int& h, int& ax, int& p) {
+    t = (int64_t)(w & 0xffffffffull);
+    h = (int)((w >> 32) & 7u);
+    ax = (int)((w >> 35) & 3u);
+    p = (int)((w >> 37) & 0x1ffffu);
+}
+
+/* planes p (index 0..n, plane p between cells p-1 and p) crossing the copy
+ * with offset s along axis a: p0..p1 (empty when p1 < p0) */
+PVX_HD void adj_planes(const GridInfo& g, int a, double lo, double hi, double s, int& p0, int& p1) {
+    const double w0 = pick3(g.w0, a), d = pick3(g.d, a);
+    const int n = pick3(g.n, a);
+    const double fl = ceil((lo + s - w0) / d), fh = floor((hi + s - w0) / d);
+    p0 = fl < 0.0 ? 0 : (fl > (double)n ? n + 1 : (int)fl);
+    p1 = fh > (double)n ? n : (fh < 0.0 ? -1 : (int)fh);
+}
+
+/* the copy offsets of a tet (periodic ghosts, src/cpsi.c:343-373) */
+PVX_HD int adj_copies(const double* rec, const GridInfo& g, const MeshBox& mb, double* lo, double* hi, uint32_t& shp) {
+    aabb((const double (*)[3])rec, 4, lo, hi);
+    return ghost_shifts_packed(lo, hi, g, mb, shp);
+}
+
+PVX_HD double copy_shift(const MeshBox& mb, uint32_t shp, int h, int a) {
+    return mb.periodic ? (pick3(mb.hi, a) - pick3(mb.lo, a)) * (double)(1 & (shp >> (3 * h + a))) : 0.0;
+}
+
+/* Enumerate the adjoint tasks of one tet record (writes them when out is
+ * given); returns the count, -1 for a record the adjoint cannot take. */
+PVX_HD int64_t adj_tasks(const double* rec, int64_t t, const GridInfo& g, const MeshBox& mb, uint64_t* out) {
+    bool fin = true;
+    for (int q = 0; q < 12; ++q) fin = fin && (rec[q] - rec[q] == 0.0);
+    if (!fin) return -1;
+    double lo[3], hi[3];
+    uint32_t shp;
+    const int ng = adj_copies(rec, g, mb, lo, hi, shp);
+    int64_t n = 0;
+    for (int h = 0; h < ng; ++h)
+        for (int a = 0; a < 3; ++a) {
+            int p0, p1;
+            adj_planes(g, a, pick3(lo, a), pick3(hi, a), copy_shift(mb, shp, h, a), p0, p1);
+            for (int p = p0; p <= p1; ++p) {
+                if (out) out[n] = pack_adj(t, h, a, p);
+                n++;
+            }
+        }
+    return n;
+}
+
+/* the tet's FP64 inverse edge matrix: lambda_{1..3} = B (x - x0) */
+struct TetInv64 {
+    double x0[3], B[3][3], vol;
+};
+
+PVX_HD bool tet_inv64(const double* rec, TetInv64& T) {
+    double D[3][3];
+    for (int a = 0; a < 3; ++a) {
+        T.x0[a] = rec[a];
+        for (int b = 0; b < 3; ++b) D[a][b] = rec[3 * (b + 1) + a] - rec[a];
+    }
+    const double c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1], c01 = D[1][2] * D[2][0] - D[1][0] * D[2][2],
+                 c02 = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+    const double det = D[0][0] * c00 + D[0][1] * c01 + D[0][2] * c02;
+    T.vol = fabs(det) / 6.0;
+    if (!(T.vol > 0.0)) return false;
+    const double id = 1.0 / det;
+    T.B[0][0] = c00 * id;
+    T.B[1][0] = c01 * id;
+    T.B[2][0] = c02 * id;
+    T.B[0][1] = (D[0][2] * D[2][1] - D[0][1] * D[2][2]) * id;
+    T.B[1][1] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * id;
+    T.B[2][1] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) * id;
+    T.B[0][2] = (D[0][1] * D[1][2] - D[0][2] * D[1][1]) * id;
+    T.B[1][2] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * id;
+    T.B[2][2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
+    return true;
+}
+
+/* a lane's row polygon (<= 8 vertex slots) in LDS (or host memory), stride FB */
+template <int FB>
+struct Poly2 {
+    double* u;
+    double* v;
+    PVX_HD void set(int i, double a, double b) {
+        u[i * FB] = a;
+        v[i * FB] = b;
+    }
+};
+
+/* integrates emitted polygon points: twice the area and twice the first
+ * moments (shoelace), coordinates relative to the cell corner */
+struct Shoelace {
+    double a2 = 0.0, mu = 0.0, mv = 0.0;
+    double fu = 0.0, fv = 0.0, pu = 0.0, pv = 0.0;
+    int n = 0;
+    PVX_HD void emit(double u, double v) {
+        if (n == 0) {
+            fu = u;
+            fv = v;
+        } else {
+            const double cr = pu * v - u * pv;
+            a2 += cr;
+            mu += (pu + u) * cr;
+            mv += (pv + v) * cr;
+        }
+        pu = u;
+        pv = v;
+        n++;
+    }
+    PVX_HD void close() {
+        if (n > 2) emit(fu, fv);
+    }
+};
+
+/* points of the edge P -> Q inside lo <= x_k <= hi, in order: P if inside,
+ * then the crossings of the two lines met along the open edge */
+template <class Emit>
+PVX_HD void slab_edge(double pu, double pv, double qu, double qv, int k, double lo, double hi, Emit&& emit) {
+    const double pk = k == 0 ? pu : pv, qk = k == 0 ? qu : qv;
+    if (pk >= lo && pk <= hi) emit(pu, pv);
+    double t0 = 2.0, t1 = 2.0;
+    if ((pk < lo && qk > lo) || (pk > lo && qk < lo)) t0 = (lo - pk) / (qk - pk);
+    if ((pk < hi && qk > hi) || (pk > hi && qk < hi)) t1 = (hi - pk) / (qk - pk);
+    const double ta = t0 < t1 ? t0 : t1, tb = t0 < t1 ? t1 : t0;
+    if (ta <= 1.0) {
+        const double cu = pu + ta * (qu - pu), cv = pv + ta * (qv - pv);
+        emit(k == 0 ? (ta == t0 ? lo : hi) : cu, k == 0 ? cv : (ta == t0 ? lo : hi));
+    }
+    if (tb <= 1.0) {
+        const double cu = pu + tb * (qu - pu), cv = pv + tb * (qv - pv);
+        emit(k == 0 ? (tb == t0 ? lo : hi) : cu, k == 0 ? cv : (tb == t0 ? lo : hi));
+    }
+}
+
+/*
+ * One adjoint task.  w: dL/dm (grid layout n0 x n1 x n2); R: a lane's row
+ * polygon buffer.  out[v] gets dL/dx_{v, axis} of this tet for the task's
+ * plane.  Tets of zero volume contribute nothing.
+ */
+template <int FB>
+PVX_HD void adj_plane_task(const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task,
+                           const double* __restrict__ w, Poly2<FB> R, double* out) {
+    int64_t t;
+    int h, a, p;
+    unpack_adj(task, t, h, a, p);
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    TetInv64 T;
+    if (!tet_inv64(rec, T)) return;
+    double lo[3], hi[3];
+    uint32_t shp;
+    adj_copies(rec, g, mb, lo, hi, shp);
+    const int b = a == 2 ? 0 : a + 1, c = a == 0 ? 2 : a - 1;
+    const double sa = copy_shift(mb, shp, h, a), sb = copy_shift(mb, shp, h, b), sc = copy_shift(mb, shp, h, c);
+    const int na = pick3(g.n, a), nb = pick3(g.n, b), nc = pick3(g.n, c);
+    const double w0b = pick3(g.w0, b), w0c = pick3(g.w0, c), db = pick3(g.d, b), dc = pick3(g.d, c);
+    /* the plane in the record's frame */
+    const double xp = (p == na ? pick3(g.w1, a) : pick3(g.w0, a) + pick3(g.d, a) * (double)p) - sa;
+    /* section: crossing points of the edges between the two sides, in
+     * cyclic order (1-3 split: the three edges at the lone vertex; 2-2
+     * split: b1a1, b1a2, b2a2, b2a1) */
+    int below[4], above[4], nbl = 0, nab = 0;
+    for (int v = 0; v < 4; ++v) {
+        if (rec[3 * v + a] < xp)
+            below[nbl++] = v;
+        else
+            above[nab++] = v;
+    }
+    if (nbl == 0 || nab == 0) return;
+    int eu[4], ev[4], ne;
+    if (nbl == 1 || nab == 1) {
+        const int lone = nbl == 1 ? below[0] : above[0];
+        const int* oth = nbl == 1 ? above : below;
+        for (int q = 0; q < 3; ++q) {
+            eu[q] = lone;
+            ev[q] = oth[q];
+        }
+        ne = 3;
+    } else {
+        eu[0] = below[0], ev[0] = above[0];
+        eu[1] = below[0], ev[1] = above[1];
+        eu[2] = below[1], ev[2] = above[1];
+        eu[3] = below[1], ev[3] = above[0];
+        ne = 4;
+    }
+    double su[4], sv[4];
+    double umin = 1e300, umax = -1e300;
+    for (int q = 0; q < 4; ++q) {
+        if (q >= ne) break;
+        const double* X = rec + 3 * eu[q];
+        const double* Y = rec + 3 * ev[q];
+        const double f = (xp - X[a]) / (Y[a] - X[a]);
+        /* cell-local in-plane coordinates of the copy */
+        su[q] = (X[b] + f * (Y[b] - X[b]) + sb - w0b) / db;
+        sv[q] = (X[c] + f * (Y[c] - X[c]) + sc - w0c) / dc;
+        umin = fmin(umin, su[q]);
+        umax = fmax(umax, su[q]);
+    }
+    int j0 = (int)floor(umin), j1 = (int)floor(umax);
+    if (j0 < 0) j0 = 0;
+    if (j1 > nb - 1) j1 = nb - 1;
+    const int64_t Sa = a == 0 ? (int64_t)g.n[1] * g.n[2] : (a == 1 ? (int64_t)g.n[2] : 1);
+    const int64_t Sb = b == 0 ? (int64_t)g.n[1] * g.n[2] : (b == 1 ? (int64_t)g.n[2] : 1);
+    const int64_t Sc = c == 0 ? (int64_t)g.n[1] * g.n[2] : (c == 1 ? (int64_t)g.n[2] : 1);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int j = j0; j <= j1; ++j) {
+        /* the row u in [j, j+1], stored relative to the row */
+        int m = 0;
+        double vmin = 1e300, vmax = -1e300;
+        auto put = [&](double uu, double vv) {
+            if (m < 8) R.set(m, uu - (double)j, vv);
+            m++;
+            vmin = fmin(vmin, vv);
+            vmax = fmax(vmax, vv);
+        };
+        for (int q = 0; q < 4; ++q) {
+            if (q >= ne) break;
+            const int r = q + 1 == ne ? 0 : q + 1;
+            slab_edge(su[q], sv[q], su[r], sv[r], 0, (double)j, (double)(j + 1), put);
+        }
+        if (m < 3 || m > 8) continue; /* a quad cut by a slab has at most 6 */
+        int k0 = (int)floor(vmin), k1 = (int)floor(vmax);
+        if (k0 < 0) k0 = 0;
+        if (k1 > nc - 1) k1 = nc - 1;
+#pragma unroll 1
+        for (int k = k0; k <= k1; ++k) {
+            const int64_t gi = (int64_t)j * Sb + (int64_t)k * Sc;
+            const double wa = p < na ? w[gi + (int64_t)p * Sa] : 0.0;
+            const double wb = p > 0 ? w[gi + (int64_t)(p - 1) * Sa] : 0.0;
+            const double J = wa - wb;
+            if (J == 0.0) continue;
+            Shoelace S;
+            auto emit = [&](double uu, double vv) { S.emit(uu, vv - (double)k); };
+            for (int q = 0; q < m; ++q) {
+                const int r = q + 1 == m ? 0 : q + 1;
+                slab_edge(R.u[q * FB], R.v[q * FB], R.u[r * FB], R.v[r * FB], 1, (double)k, (double)(k + 1), emit);
+            }
+            S.close();
+            if (S.a2 == 0.0) continue;
+            const double cu = S.mu / (3.0 * S.a2), cv = S.mv / (3.0 * S.a2);
+            double X[3];
+            X[a] = xp;
+            X[b] = w0b + db * ((double)j + cu) - sb;
+            X[c] = w0c + dc * ((double)k + cv) - sc;
+            double l[3];
+            for (int r = 0; r < 3; ++r)
+                l[r] = T.B[r][0] * (X[0] - T.x0[0]) + T.B[r][1] * (X[1] - T.x0[1]) + T.B[r][2] * (X[2] - T.x0[2]);
+            const double f = J * 0.5 * fabs(S.a2) * db * dc;
+            acc[0] += f * (1.0 - l[0] - l[1] - l[2]);
+            acc[1] += f * l[0];
+            acc[2] += f * l[1];
+            acc[3] += f * l[2];
+        }
+    }
+    const double s = rec[24] / T.vol;
+    for (int v = 0; v < 4; ++v) out[v] = acc[v] * s;
+}
+
+/* element-local node of vertex v of tet t at refinement level 0
+ * (the 6-tet table on the 3x3x3 lattice, src/refine.c:39-41) */
+PVX_HD int tet_node(int vpe, int t, int v) {
+    if (vpe == 4) return v;
+    const int L = tet27(t, v);
+    if (vpe == 27) return L;
+    const int i = L / 9, j = (L / 3) % 3, k = L % 3;
+    return 4 * (i / 2) + 2 * (j / 2) + (k / 2);
+}
+
+}  // namespace pvx
+
+#endif

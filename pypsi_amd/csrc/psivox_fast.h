@@ -82,13 +82,6 @@ struct CopyGeo {
     int ib0[3];
 };
 
-/* pick component ax of a 3-vector without dynamic indexing (keeps the
- * values in registers) */
-template <typename T>
-PVX_HD T pick3(const T* v, int ax) {
-    return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]);
-}
-
 /* one axis of the copy's affine map, x_ax(q) = o + d0 q0 + d1 q1 + d2 q2,
  * plus what the FP64 tie-break needs */
 struct AxisMap {

@@ -42,6 +42,13 @@ constexpr int NV64 = 32;
 /* new vertices one cut may create (cap polygon size) */
 constexpr int NCUT = 16;
 
+/* pick component ax of a 3-vector without dynamic indexing (keeps the
+ * values in registers) */
+template <typename T>
+PVX_HD T pick3(const T* v, int ax) {
+    return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]);
+}
+
 enum { ST_OVERFLOW = 1, ST_BADGEOM = 2, ST_FALLBACK = 4 };
 
 struct Poly64 {

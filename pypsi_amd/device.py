@@ -95,6 +95,8 @@ def make_mesh(pos, vel, mass, conn, periodic_box=None) -> Mesh:
             for a in range(3):
                 m.box[s][a] = float(periodic_box[s][a])
     m.on_device = 1
+    # the descriptor carries raw device pointers: it keeps the tensors alive
+    m._keep = (pos, vel, mass, conn)
     return m
 
 
@@ -108,6 +110,7 @@ def make_grid(n, bounds, fields: dict) -> Grid:
     for i, f in enumerate(FIELD_ORDER):
         g.fields[i] = fields[f].data_ptr() if f in fields else None
     g.on_device = 1
+    g._keep = dict(fields)
     return g
 
 
@@ -126,3 +129,10 @@ def voxels(mesh: Mesh, grid: Grid, opts: Opts) -> dict:
     st = Stats()
     check(lib().psivox_voxels(C.byref(mesh), C.byref(grid), C.byref(opts), C.byref(st)))
     return st.as_dict()
+
+
+def mass_adjoint(mesh: Mesh, grid: Grid, dl_dm, dl_dpos, opts: Opts) -> None:
+    """dl_dpos += dL/dpos for dL/dm = dl_dm (device tensors: grid-shaped
+    float64, and (npart, 3) float64)."""
+    check(lib().psivox_mass_adjoint(C.byref(mesh), C.byref(grid), C.c_void_p(dl_dm.data_ptr()),
+                                    C.c_void_p(dl_dpos.data_ptr()), C.byref(opts)))

@@ -14,6 +14,7 @@
 #include "../../pypsi_amd/csrc/psivox_plain.h"
 #include "../../pypsi_amd/csrc/psivox_refine.h"
 #include "../../pypsi_amd/csrc/psivox_fast.h"
+#include "../../pypsi_amd/csrc/psivox_adjoint.h"
 
 using namespace pvx;
 
@@ -182,5 +183,56 @@ extern "C" int hs_voxels_plain(int vpe, int64_t npart, int64_t nelem, const doub
     out_stats[1] = ncopies;
     out_stats[2] = ntets;
     out_stats[3] = status;
+    return 0;
+}
+
+/* the mass adjoint of the GPU kernels on the host (refine level 0): dpos
+ * (npart x 3) overwritten with dL/dpos for w = dL/dm */
+extern "C" int hs_mass_adjoint(int vpe, int64_t npart, int64_t nelem, const double* pos, const double* mass,
+                               const int32_t* conn, int periodic, const double* box6, const int* n, const double* win6,
+                               const double* d, const double* w, double* dpos, int64_t* out_stats) {
+    double* fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    GridInfo g;
+    MeshBox mb;
+    fill_grid(g, mb, n, win6, d, fields, 0, periodic, box6);
+    for (int64_t q = 0; q < 3 * npart; ++q) dpos[q] = 0.0;
+    std::vector<double> zv(3 * (size_t)(npart > 0 ? npart : 1), 0.0);
+    RefineScratch* RS = new RefineScratch;
+    std::vector<double> ru(8), rv(8);
+    Poly2<1> R{ru.data(), rv.data()};
+    std::vector<double> tets;
+    std::vector<uint64_t> tasks;
+    int64_t ntasks = 0, fails = 0;
+    const int per = vpe == 4 ? 1 : 6;
+    for (int64_t e = 0; e < nelem; ++e) {
+        int64_t nt = expand_element(pos, zv.data(), mass, conn, vpe, e, g, mb, 0, 0.0, *RS, nullptr);
+        if (nt == 0) continue;
+        tets.resize((size_t)nt * TREC);
+        expand_element(pos, zv.data(), mass, conn, vpe, e, g, mb, 0, 0.0, *RS, tets.data());
+        for (int64_t t = 0; t < nt; ++t) {
+            const double* r = &tets[(size_t)t * TREC];
+            if (r[25] == 0.0) continue;
+            const int64_t nk = adj_tasks(r, t, g, mb, nullptr);
+            if (nk < 0) {
+                fails++;
+                continue;
+            }
+            tasks.resize((size_t)nk);
+            adj_tasks(r, t, g, mb, tasks.data());
+            ntasks += nk;
+            for (int64_t c = 0; c < nk; ++c) {
+                double o4[4];
+                adj_plane_task<1>(g, mb, r, tasks[c], w, R, o4);
+                const int ax = (int)((tasks[c] >> 35) & 3u);
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t id = conn[e * vpe + tet_node(vpe, (int)(t % per), v)];
+                    dpos[3 * id + ax] += o4[v];
+                }
+            }
+        }
+    }
+    delete RS;
+    out_stats[0] = ntasks;
+    out_stats[1] = fails;
     return 0;
 }

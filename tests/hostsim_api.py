@@ -39,3 +39,18 @@ def run(P, kernel="fast", reftol=1.0, maxlvl=0, xvfix=False, out=None):
            vp(P.mass), vp(P.conn), C.c_int(int(P.periodic)), box, n, win, d, fptr, C.c_int(int(xvfix)),
            C.c_double(reftol), C.c_int(maxlvl), vp(st))
     return out, st
+
+
+def mass_adjoint(P, w):
+    """dL/dpos (npart, 3) of the column-kernel adjoint for w = dL/dm."""
+    w = np.ascontiguousarray(w, dtype=np.float64).reshape(P.n)
+    out = np.zeros((P.pos.shape[0], 3))
+    st = np.zeros(2, np.int64)
+    n = (C.c_int * 3)(*P.n)
+    win = (C.c_double * 6)(*P.bounds.reshape(-1))
+    d = (C.c_double * 3)(*P.d)
+    box = (C.c_double * 6)(*P.box.reshape(-1))
+    vp = lambda a: C.c_void_p(a.ctypes.data)
+    lib().hs_mass_adjoint(C.c_int(P.elemtype), C.c_int64(P.pos.shape[0]), C.c_int64(P.conn.shape[0]), vp(P.pos),
+                          vp(P.mass), vp(P.conn), C.c_int(int(P.periodic)), box, n, win, d, vp(w), vp(out), vp(st))
+    return out, st

@@ -1,0 +1,160 @@
+"""GPU mass adjoint (psivox_mass_adjoint through the C ABI, and the
+torch.autograd wrapper) against the FP64 oracle adjoint and FP64 central
+finite differences.
+
+Tolerances: the adjoint kernel (FP64 throughout), fed the same dL/dm as the
+oracle, within 1e-9 of the largest gradient component; the 64^3 -> 128^3 workload (rms 1.0 cells, loss
+sum (rho - rho_t)^2 with rho_t from seed 1) on 1000 seeded (vertex, axis)
+pairs: |g - g_fd| / max(|g_fd|, 1e-3 rms(g_fd)) <= 1e-4 against FP64
+central differences with h = 1e-6 box units, for the adjoint fed the
+oracle's FP64 dL/dm and for the end-to-end autograd path with the FP64
+forward (1e-3 with the FP32-clip forward, whose cell masses carry FP32
+error into dL/dm).
+"""
+import numpy as np
+import pytest
+
+from adjoint_cases import BOX, cases, loss_weights, sample
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_adjoint(P, w):
+    import torch
+    from pypsi_amd import device as D
+    dev = torch.device("cuda:0")
+    pos = torch.from_numpy(P.pos).to(dev)
+    mass = torch.from_numpy(P.mass).to(dev)
+    conn = torch.from_numpy(P.conn).to(dev)
+    mesh = D.make_mesh(pos, None, mass, conn, P.box if P.periodic else None)
+    grid = D.make_grid(P.n, P.bounds, {"m": torch.empty(0, dtype=torch.float64, device=dev)})
+    tw = torch.from_numpy(np.ascontiguousarray(w)).to(dev)
+    out = torch.zeros_like(pos)
+    D.mass_adjoint(mesh, grid, tw, out, D.make_opts(stream=torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("idx", range(5))
+def test_adjoint_vs_oracle(oracle_mod, idx):
+    label, P = cases(oracle_mod)[idx]
+    print("case: adjoint", label, flush=True)
+    r, w = loss_weights(oracle_mod, P)
+    go = oracle_mod.mass_adjoint(P, w)
+    gg = gpu_adjoint(P, w)
+    err = np.abs(gg - go).max() / np.abs(go).max()
+    print(label, "gpu adjoint vs oracle: %.2e" % err)
+    assert err < 1e-9
+
+
+def test_adjoint_accumulates(oracle_mod):
+    """dl_dpos is added to, not overwritten."""
+    import torch
+    from pypsi_amd import device as D
+    label, P = cases(oracle_mod)[0]
+    print("case: accumulate", flush=True)
+    r, w = loss_weights(oracle_mod, P)
+    dev = torch.device("cuda:0")
+    # the descriptors hold raw device pointers: keep every tensor alive
+    pos = torch.from_numpy(P.pos).to(dev)
+    mass = torch.from_numpy(P.mass).to(dev)
+    conn = torch.from_numpy(P.conn).to(dev)
+    mesh = D.make_mesh(pos, None, mass, conn, P.box)
+    grid = D.make_grid(P.n, P.bounds, {})
+    tw = torch.from_numpy(np.ascontiguousarray(w)).to(dev)
+    out = torch.ones_like(pos)
+    opts = D.make_opts(stream=torch.cuda.current_stream().cuda_stream)
+    D.mass_adjoint(mesh, grid, tw, out, opts)
+    D.mass_adjoint(mesh, grid, tw, out, opts)
+    torch.cuda.synchronize()
+    go = oracle_mod.mass_adjoint(P, w)
+    assert np.abs(out.cpu().numpy() - 1.0 - 2.0 * go).max() <= 1e-9 * np.abs(go).max()
+
+
+def test_adjoint_rejects_refinement(oracle_mod):
+    import torch
+    from pypsi_amd import device as D
+    label, P = cases(oracle_mod)[0]
+    dev = torch.device("cuda:0")
+    pos = torch.from_numpy(P.pos).to(dev)
+    mass = torch.from_numpy(P.mass).to(dev)
+    conn = torch.from_numpy(P.conn).to(dev)
+    mesh = D.make_mesh(pos, None, mass, conn, P.box)
+    grid = D.make_grid(P.n, P.bounds, {})
+    tw = torch.zeros(P.n, dtype=torch.float64, device=dev)
+    with pytest.raises(RuntimeError, match="refine_max_lvl"):
+        D.mass_adjoint(mesh, grid, tw, torch.zeros_like(pos), D.make_opts(refine_max_lvl=1))
+
+
+def _c4(oracle_mod):
+    from oracle import block_connectivity
+    from pypsi_amd.synth import zeldovich_block
+    pos, _ = zeldovich_block(64, 1.0, seed=0)
+    post, _ = zeldovich_block(64, 1.0, seed=1)
+    P = oracle_mod.Problem.block(pos, None, periodic_box=BOX, n=(128,) * 3, bounds=BOX)
+    Pt = oracle_mod.Problem.block(post, None, periodic_box=BOX, n=(128,) * 3, bounds=BOX)
+    V = float(np.prod(P.d))
+    rho = oracle_mod.voxels(P, threads=-1)[0]["m"] / V
+    rhot = oracle_mod.voxels(Pt, threads=-1)[0]["m"] / V
+    rng = np.random.default_rng(1000)
+    parts = rng.choice(P.pos.shape[0], size=1000, replace=False)
+    axes = rng.integers(0, 3, size=1000)
+    return P, rho, rhot, parts, axes
+
+
+def _rel_err(g, fd):
+    den = np.maximum(np.abs(fd), 1e-3 * np.sqrt(np.mean(fd ** 2)))
+    return np.abs(g - fd) / den
+
+
+def test_c4_adjoint_vs_fd_and_oracle(oracle_mod):
+    """64^3 -> 128^3: adjoint kernel vs FP64 finite differences and vs the
+    FP64 oracle adjoint on 1000 seeded (vertex, axis) pairs."""
+    print("case: c4 adjoint vs fd", flush=True)
+    P, rho, rhot, parts, axes = _c4(oracle_mod)
+    V = float(np.prod(P.d))
+    r = rho - rhot
+    w = 2.0 * r / V
+    gg = gpu_adjoint(P, w)[parts, axes]
+    fd = oracle_mod.fd_mass_loss_grad(P, r, parts, axes, h=1e-6)
+    e_fd = _rel_err(gg, fd)
+    # the oracle adjoint on the elements that use the sampled particles
+    start, elems = oracle_mod.particle_elements(P)
+    sel = np.unique(np.concatenate([elems[start[p]:start[p + 1]] for p in parts]))
+    Ps = oracle_mod.Problem(P.pos, None, P.mass[sel], P.conn[sel], periodic_box=BOX, n=P.n, bounds=BOX)
+    go = oracle_mod.mass_adjoint(Ps, w)[parts, axes]
+    e_or = _rel_err(gg, go)
+    e_ofd = _rel_err(go, fd)
+    print("max rel err: gpu-fd %.2e, gpu-oracle %.2e, oracle-fd %.2e" % (e_fd.max(), e_or.max(), e_ofd.max()))
+    assert e_ofd.max() <= 1e-4
+    assert e_fd.max() <= 1e-4
+    assert e_or.max() <= 1e-6
+
+
+@pytest.mark.parametrize("kernel,tol", [("plain", 1e-4), ("fast", 1e-3)])
+def test_c4_autograd_vs_fd(oracle_mod, kernel, tol):
+    """The torch.autograd path end to end (GPU forward, loss in torch,
+    backward through the adjoint kernel).  With the FP64 forward the bar is
+    the adjoint's, 1e-4; with the FP32-clip forward dL/dm itself carries the
+    forward's FP32 error (cell masses within 1e-5 relative), bar 1e-3."""
+    import torch
+    from pypsi_amd.autograd import voxel_density
+    print("case: c4 autograd", kernel, flush=True)
+    P, rho, rhot, parts, axes = _c4(oracle_mod)
+    dev = torch.device("cuda:0")
+    pos = torch.from_numpy(P.pos).to(dev).requires_grad_(True)
+    conn = torch.from_numpy(P.conn).to(dev)
+    trho_t = torch.from_numpy(rhot).to(dev)
+    rho_g = voxel_density(pos, conn, (128,) * 3, BOX, periodic_box=BOX, kernel=kernel)
+    loss = ((rho_g - trho_t) ** 2).sum()
+    loss.backward()
+    g = pos.grad.cpu().numpy()[parts, axes]
+    fd = oracle_mod.fd_mass_loss_grad(P, rho - rhot, parts, axes, h=1e-6)
+    e = _rel_err(g, fd)
+    print("autograd (%s forward) vs fd: max %.2e" % (kernel, e.max()))
+    assert e.max() <= tol
+    # FP32 positions: the gradient comes back in the input dtype
+    p32 = torch.from_numpy(P.pos.astype(np.float32)).to(dev).requires_grad_(True)
+    rho32 = voxel_density(p32, conn, (128,) * 3, BOX, periodic_box=BOX, kernel=kernel)
+    ((rho32 - trho_t) ** 2).sum().backward()
+    assert p32.grad.dtype == torch.float32 and torch.isfinite(p32.grad).all()
