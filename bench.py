@@ -68,9 +68,14 @@ def load_traffic():
 
 
 def build_inputs(wname):
+    n, rms, seed, gn, fields = WORKLOADS[wname]
+    return build_inputs_seed(wname, seed)
+
+
+def build_inputs_seed(wname, seed):
     import numpy as np
     from pypsi_amd.synth import zeldovich_block
-    n, rms, seed, gn, fields = WORKLOADS[wname]
+    n, rms, _, gn, fields = WORKLOADS[wname]
     pos, vel = zeldovich_block(n, rms, seed)
     # trilinear periodic connectivity, corner order 4i+2j+k (src/PSI.c:636-656)
     i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
@@ -120,6 +125,77 @@ def cpu_baseline(wname, pos, vel, mass, conn, gn, fields, budget_s=12.0):
                       f"{dt:.1f} s single-threaded, -O3 x86-64-v3"}
 
 
+def plain_leg(mesh, grid, tf, stream, nfrag, steps=2, warmup=1):
+    """The FP64 one-thread-per-tet kernel on the same inputs (the plain GPU
+    method the FP32-clip column kernel is measured against)."""
+    import torch
+    from pypsi_amd import device as D
+    opts = D.make_opts(kernel="plain", stream=stream)
+
+    def step():
+        for t in tf.values():
+            t.zero_()
+        return D.voxels(mesh, grid, opts)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kms = []
+    for _ in range(steps):
+        kms.append(step()["kernel_ms"])
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    return {"value": nfrag / (ms * 1e-3), "unit": "fragments/s", "ms_per_step": ms, "kernel_ms_avg": sum(kms) / steps,
+            "steps": steps, "dtype": "f64"}
+
+
+def fwd_adjoint_leg(steps=5, warmup=2):
+    """64^3 particles, rms 1.0 cells -> 128^3, mass: forward (FP32-clip
+    kernel) and the mass adjoint of L = sum (rho - rho_t)^2, rho_t from the
+    same construction with seed 1.  Reports t(fwd + adj) / t(fwd)."""
+    import torch
+    from pypsi_amd import device as D
+    pos, vel, mass, conn, gn, _ = build_inputs("zeldovich-64^3-rms1.0-to-128^3-m")
+    post, _, _, _, _, _ = build_inputs_seed("zeldovich-64^3-rms1.0-to-128^3-m", 1)
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    dev = torch.device("cuda:0")
+    tconn = torch.from_numpy(conn).to(dev)
+    tmass = torch.from_numpy(mass).to(dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = D.make_opts(kernel="fast", stream=stream)
+    V = (1.0 / gn) ** 3
+    m = torch.zeros((gn,) * 3, dtype=torch.float64, device=dev)
+    grid = D.make_grid((gn,) * 3, box, {"m": m})
+    # target density
+    tpos_t = torch.from_numpy(post).to(dev)
+    D.voxels(D.make_mesh(tpos_t, None, tmass, tconn, box), grid, opts)
+    rho_t = m / V
+    tpos = torch.from_numpy(pos).to(dev)
+    mesh = D.make_mesh(tpos, None, tmass, tconn, box)
+    g = torch.zeros_like(tpos)
+    tf, ta = [], []
+    for it in range(warmup + steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.zero_()
+        D.voxels(mesh, grid, opts)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        w = (2.0 / V) * (m / V - rho_t)
+        g.zero_()
+        D.mass_adjoint(mesh, grid, w, g, opts)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if it >= warmup:
+            tf.append(t1 - t0)
+            ta.append(t2 - t1)
+    f, a = 1e3 * sum(tf) / steps, 1e3 * sum(ta) / steps
+    return {"workload": "zeldovich-64^3-rms1.0-to-128^3-m, loss sum (rho - rho_t)^2, rho_t seed 1",
+            "fwd_ms": f, "adjoint_ms": a, "ratio_fwd_plus_adj_over_fwd": (f + a) / f, "steps": steps,
+            "adjoint_dtype": "f64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +204,7 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "plain", "fast"])
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the plain-kernel and forward+adjoint legs")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -216,6 +293,11 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "flops_per_fragment": FLOPS_PER_FRAG[fk]},
     }
+    if not args.no_extra:
+        log("[bench] plain FP64 kernel on the same workload")
+        out["plain"] = plain_leg(mesh, grid, tf, stream, nfrag)
+        log("[bench] forward + adjoint, 64^3 -> 128^3")
+        out["fwd_adjoint"] = fwd_adjoint_leg()
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         try:

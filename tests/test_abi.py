@@ -1,0 +1,79 @@
+"""The C ABI and the PSI module surface on the CPU: libpsivox.so exports every
+function include/psivox.h declares, answers argument errors before touching a
+GPU, and the PSI module validates arguments with the reference's exceptions
+and messages (src/PSI.c:58-279, 367-722, 974-1018)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..")
+HEADER = os.path.join(ROOT, "include", "psivox.h")
+LIB = os.path.join(ROOT, "pypsi_amd", "libpsivox.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(psivox_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared()
+    for n in ("psivox_voxels", "psivox_voxels_batched", "psivox_mass_adjoint", "psivox_last_error",
+              "psivox_abi_version", "psivox_device_count"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from pypsi_amd import preload_hip_runtime
+    preload_hip_runtime()
+    lib = C.CDLL(LIB)
+    for n in declared():
+        assert hasattr(lib, n), n
+    from pypsi_amd.device import EXPORTS
+    assert sorted(EXPORTS) == declared()
+
+
+def test_abi_version_and_argument_errors():
+    from pypsi_amd import device as D
+    L = D.lib()
+    assert L.psivox_abi_version() == 1
+    # NULL descriptors are rejected before any device work
+    assert L.psivox_voxels(None, None, None, None) == -1
+    assert b"required" in L.psivox_last_error()
+    assert L.psivox_mass_adjoint(None, None, None, None, None) == -1
+    assert L.psivox_voxels_batched(-1, None, None, None, None) == -1
+
+
+def test_struct_layouts_match_header():
+    """ctypes layouts used by the bindings (sizes of the C structs)."""
+    from pypsi_amd import device as D
+    assert C.sizeof(D.Mesh) == 4 + 4 + 8 + 8 + 4 * 8 + 48 + 4 + 4
+    assert C.sizeof(D.Grid) == 12 + 4 + 48 + 24 + 48
+    assert C.sizeof(D.Stats) == 4 * 8 + 2 * 8 + 4 + 4
+
+
+def test_module_argument_errors():
+    from pypsi_amd import PSI
+    with pytest.raises(ValueError, match="Invalid grid type"):
+        PSI.Grid(type="polar", n=(4, 4, 4))
+    with pytest.raises(ValueError, match="requires you to also provide field 'x'"):
+        PSI.Grid(n=(4, 4, 4), fields=["xx"])
+    with pytest.raises(ValueError, match="Unrecognized field identifier"):
+        PSI.Grid(n=(4, 4, 4), fields=["q"])
+    g = PSI.Grid(n=(4, 5, 6), bounds=((0, 0, 0), (1, 2, 3)), fields=["x", "v", "xx", "vx", "vv"])
+    assert set(g.fields) == {"m", "x", "v", "xx", "xv", "vv"}
+    assert g.fields["xv"].shape == (4, 5, 6, 3, 3) and g.fields["m"].dtype == np.float64
+    assert np.allclose(g.d, (0.25, 0.4, 0.5))
+    with pytest.raises(ValueError, match="Invalid mesh loader"):
+        PSI.Mesh(loader="nope")
+    with pytest.raises(TypeError, match="pos must be"):
+        PSI.Mesh(loader="block", pos=np.zeros((2, 2, 2, 3), dtype=np.float32))
+    m = PSI.Mesh(loader="block", pos=np.zeros((3, 3, 3, 3)), periodic_box=((0, 0, 0), (1, 1, 1)))
+    assert m.connectivity.shape == (27, 8) and m.mass.shape == (3, 3, 3)  # (ne, ne, ne) as src/PSI.c:530-545
+    with pytest.raises(ValueError, match="Invalid mode for PSI.voxels"):
+        PSI.voxels(grid=g, mesh=m, mode="nonsense")
+    with pytest.raises(NotImplementedError):
+        PSI.voxels(grid=g, mesh=m, mode="annihilation")
