@@ -150,6 +150,72 @@ def plain_leg(mesh, grid, tf, stream, nfrag, steps=2, warmup=1):
             "steps": steps, "dtype": "f64"}
 
 
+UNEVEN = "zeldovich-uneven64-16^3..96^3-rms0.2..3.0-m"
+
+
+def batched_leg(steps=3, warmup=1):
+    """64 meshes (seeds 0..63): lattice n log-uniform in 16..96, rms uniform
+    in 0.2..3.0 cells, each onto its own grid 2^round(log2(8n/3)) clipped to
+    32..256, mass only.  One psivox_voxels_batched call per step, beside a
+    loop of single-mesh calls on the same device-resident inputs."""
+    import numpy as np
+    import torch
+    from pypsi_amd import device as D
+    from pypsi_amd.synth import uneven_batch_specs, zeldovich_block
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    dev = torch.device("cuda:0")
+    meshes, grids, fields = [], [], []
+    ntet = 0
+    for n, rms, gn, seed in uneven_batch_specs(64):
+        pos, _ = zeldovich_block(n, rms, seed)
+        i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        conn = np.empty((n, n, n, 8), dtype=np.int32)
+        for ii in range(2):
+            for jj in range(2):
+                for kk in range(2):
+                    conn[..., 4 * ii + 2 * jj + kk] = n * n * ((i + ii) % n) + n * ((j + jj) % n) + ((k + kk) % n)
+        conn = conn.reshape(-1, 8)
+        tp = torch.from_numpy(pos.reshape(-1, 3)).to(dev)
+        tc = torch.from_numpy(conn).to(dev)
+        tm = torch.ones(conn.shape[0], dtype=torch.float64, device=dev)
+        m = torch.zeros((gn,) * 3, dtype=torch.float64, device=dev)
+        meshes.append(D.make_mesh(tp, None, tm, tc, box))
+        grids.append(D.make_grid((gn,) * 3, box, {"m": m}))
+        fields.append(m)
+        ntet += 6 * conn.shape[0]
+    stream = torch.cuda.current_stream().cuda_stream
+    opts = D.make_opts(kernel="fast", stream=stream)
+
+    def run(batched):
+        for m in fields:
+            m.zero_()
+        if batched:
+            return D.voxels_batched(meshes, grids, opts)
+        return [D.voxels(mm, g, opts) for mm, g in zip(meshes, grids)]
+
+    res = {}
+    for label, batched in (("batched", True), ("loop", False)):
+        for _ in range(warmup):
+            run(batched)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sts = run(batched)
+        torch.cuda.synchronize()
+        res[label] = 1e3 * (time.perf_counter() - t0) / steps
+        res[label + "_gpu_nfrag"] = int(sum(st["nfrag"] for st in sts))
+    mass_err = max(abs(float(m.sum().item()) / float(meshes[q].nelem) - 1.0) for q, m in enumerate(fields))
+    nfrag = load_counts().get(UNEVEN, {}).get("nfrag")
+    src = "oracle"
+    if nfrag is None:
+        nfrag, src = res["batched_gpu_nfrag"], "gpu (oracle count not cached)"
+    return {"workload": UNEVEN, "meshes": 64, "tets": ntet, "nfrag": nfrag, "nfrag_source": src,
+            "value": nfrag / (res["batched"] * 1e-3), "unit": "fragments/s", "ms_per_step": res["batched"],
+            "loop_ms_per_step": res["loop"], "loop_value": nfrag / (res["loop"] * 1e-3),
+            "gpu_nfrag": res["batched_gpu_nfrag"], "max_mass_rel_err": mass_err, "steps": steps}
+
+
 def fwd_adjoint_leg(steps=5, warmup=2):
     """64^3 particles, rms 1.0 cells -> 128^3, mass: forward (FP32-clip
     kernel) and the mass adjoint of L = sum (rho - rho_t)^2, rho_t from the
@@ -204,7 +270,8 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "plain", "fast"])
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the plain-kernel and forward+adjoint legs")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the plain-kernel, forward+adjoint and batched legs")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -298,6 +365,8 @@ def main():
         out["plain"] = plain_leg(mesh, grid, tf, stream, nfrag)
         log("[bench] forward + adjoint, 64^3 -> 128^3")
         out["fwd_adjoint"] = fwd_adjoint_leg()
+        log("[bench] batched uneven meshes")
+        out["batched_uneven"] = batched_leg()
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         try:

@@ -16,6 +16,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -97,7 +98,10 @@ struct MeshDev {
 
 /* refine_max_lvl == 0: fixed slots (6 per element, 1 for simplices);
  * elements outside the window leave invalid records */
-__global__ void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, int64_t ne, double* __restrict__ tets) {
+/* tag: written to every record's slot 27 (the mesh's index in a batched
+ * call, 0 otherwise) */
+__global__ void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, int64_t ne, double* __restrict__ tets,
+                               double tag = 0.0) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ne) return;
     const int per = (m.vpe == 4) ? 1 : 6;
@@ -112,7 +116,10 @@ __global__ void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, in
     }
     double* out = tets + (int64_t)TREC * per * t;
     if (!wrap_and_test(P, m.vpe, g, mb)) {
-        for (int s = 0; s < per; ++s) out[TREC * s + 25] = 0.0;
+        for (int s = 0; s < per; ++s) {
+            out[TREC * s + 25] = 0.0;
+            out[TREC * s + 27] = tag;
+        }
         return;
     }
     const double mass = m.mass[e];
@@ -124,6 +131,8 @@ __global__ void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, in
             }
         out[24] = mass;
         out[25] = 1.0;
+        out[26] = 1.0;
+        out[27] = tag;
         return;
     }
     if (m.vpe == 8) {
@@ -134,6 +143,7 @@ __global__ void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, in
     } else {
         emit6(out, (const double (*)[3])P, (const double (*)[3])U, mass);
     }
+    for (int s = 0; s < 6; ++s) out[TREC * s + 27] = tag;
 }
 
 /* refine_max_lvl > 0: count pass and emit pass of the per-element DFS */
@@ -210,18 +220,72 @@ __global__ __launch_bounds__(64) void k_targets(double* __restrict__ tets, int64
     r[26] = window_target(r, g, mb, pool);
 }
 
+/* dst[i] = src[idx[i]] */
+__global__ void k_gather(const int64_t* __restrict__ src, const int64_t* __restrict__ idx, int n,
+                         int64_t* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+/* One entry per mesh of a batched call: its grid, box and accumulator.
+ * Tet records carry their entry's index in slot 27. */
+struct BatchGeo {
+    GridInfo g;
+    MeshBox mb;
+    double* acc;
+};
+
+/* f(g, mb, acc, id) with the geometry of record r's mesh.  Batched: a
+ * waterfall over the distinct meshes among the wave's active lanes, each
+ * table entry read at a wave-uniform index (scalar loads). */
+template <bool BATCH, class F>
+__device__ __forceinline__ void with_mesh(const double* r, const GridInfo& g0, const MeshBox& mb0, double* acc0,
+                                          const BatchGeo* table, F&& f) {
+    if (!BATCH) {
+        f(g0, mb0, acc0, 0);
+        return;
+    }
+    const int my = (int)r[27];
+    for (;;) {
+        const int id = __builtin_amdgcn_readfirstlane(my);
+        if (my == id) {
+            const BatchGeo* e = table + id;
+            f(e->g, e->mb, e->acc, id);
+            break;
+        }
+    }
+}
+
+/* per-mesh counter add, aggregated over the wave when its lanes share a
+ * mesh; every lane of the wave calls it (id < 0: nothing) */
+__device__ void mesh_count(unsigned long long* mcnt, int id, int64_t v, int slot) {
+    const int id0 = __builtin_amdgcn_readfirstlane(id);
+    if (__all(id == id0)) {
+        unsigned long long sum = (unsigned long long)v;
+        for (int o = warpSize / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if ((threadIdx.x & (warpSize - 1)) == 0 && id0 >= 0 && sum) atomicAdd(mcnt + 2 * id0 + slot, sum);
+    } else if (id >= 0 && v) {
+        atomicAdd(mcnt + 2 * id + slot, (unsigned long long)v);
+    }
+}
+
 /* column tasks per tet record (0 for invalid records); a tet the column
  * path cannot take flags the FP64 fallback */
-__global__ __launch_bounds__(256) void k_col_count(const double* __restrict__ tets, int64_t ntet, GridInfo g,
-                                                   MeshBox mb, int64_t* __restrict__ cnt, unsigned* status,
-                                                   unsigned long long* counters) {
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_col_count(const double* __restrict__ tets, int64_t ntet, GridInfo g0,
+                                                   MeshBox mb0, const BatchGeo* table, int64_t* __restrict__ cnt,
+                                                   unsigned* status, unsigned long long* counters,
+                                                   unsigned long long* mcnt) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int nc = 0;
+    int nc = 0, id = -1;
     if (t < ntet) {
         const double* r = tets + (int64_t)TREC * t;
         int64_t n = 0;
         if (r[25] != 0.0) {
-            n = tet_columns(r, t, g, mb, nullptr, &nc);
+            with_mesh<BATCH>(r, g0, mb0, nullptr, table, [&](const GridInfo& g, const MeshBox& mb, double*, int m) {
+                n = tet_columns(r, t, g, mb, nullptr, &nc);
+                id = m;
+            });
             if (n < 0) {
                 atomicOr(status, ST_FALLBACK);
                 n = 0;
@@ -230,17 +294,22 @@ __global__ __launch_bounds__(256) void k_col_count(const double* __restrict__ te
         cnt[t] = n;
     }
     block_count(0, nc, counters);
+    if (BATCH) mesh_count(mcnt, id, nc, 1);
 }
 
 /* write the column tasks of tets [t0, t1) at colstart[t] - base */
-__global__ __launch_bounds__(256) void k_col_emit(const double* __restrict__ tets, int64_t t0, int64_t t1, GridInfo g,
-                                                  MeshBox mb, const int64_t* __restrict__ colstart, int64_t base,
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_col_emit(const double* __restrict__ tets, int64_t t0, int64_t t1, GridInfo g0,
+                                                  MeshBox mb0, const BatchGeo* table,
+                                                  const int64_t* __restrict__ colstart, int64_t base,
                                                   uint64_t* __restrict__ tasks) {
     const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= t1) return;
     const double* r = tets + (int64_t)TREC * t;
     if (r[25] == 0.0 || colstart[t + 1] == colstart[t]) return;
-    tet_columns(r, t, g, mb, tasks + (colstart[t] - base), nullptr);
+    with_mesh<BATCH>(r, g0, mb0, nullptr, table, [&](const GridInfo& g, const MeshBox& mb, double*, int) {
+        tet_columns(r, t, g, mb, tasks + (colstart[t] - base), nullptr);
+    });
 }
 
 /* per-task outputs for the residual pass: FP64 fragment-mass sum, the
@@ -252,7 +321,9 @@ struct TaskOut {
 };
 
 /* one lane per column task: FP32 column clip in this lane's LDS slice,
- * cell moments, FP64 deposit */
+ * cell moments, FP64 deposit.  One mesh per launch: its geometry stays in
+ * kernel arguments (a batched call launches it once per mesh over that
+ * mesh's contiguous task range). */
 template <int ORDER>
 __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
                                                         const uint64_t* __restrict__ tasks, int64_t ntask, GridInfo g,
@@ -284,36 +355,68 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
     block_count(nf, 0, counters);
 }
 
-/* one thread per tet of [t0, t1): reduce its tasks' outputs and deposit the
- * mass residual into its largest fragment */
-template <int ORDER>
-__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ tets, int64_t t0, int64_t t1,
-                                                  GridInfo g, MeshBox mb, const int64_t* __restrict__ colstart,
-                                                  int64_t base, const uint64_t* __restrict__ tasks, TaskOut out,
-                                                  double* __restrict__ acc) {
+/* one thread per tet of [t0, t1): how its residual is deposited.  A small
+ * residual goes whole to the largest fragment: tetf[t] = the residual and
+ * tetb[t] = that task; otherwise it is spread: tetf[t] = (target - sum) /
+ * sum and tetb[t] = -1 (sums: the tasks' FP64 fragment-mass sums) */
+__global__ __launch_bounds__(256) void k_resid_factor(const double* __restrict__ tets, int64_t t0, int64_t t1,
+                                                      const int64_t* __restrict__ colstart, int64_t base,
+                                                      const double* __restrict__ tsum,
+                                                      const uint64_t* __restrict__ tkey, double* __restrict__ tetf,
+                                                      int64_t* __restrict__ tetb) {
     const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= t1) return;
     const int64_t c0 = colstart[t] - base, c1 = colstart[t + 1] - base;
-    if (c1 <= c0) return;
     double sum = 0.0;
-    uint64_t kbest = 0;
-    int64_t cbest = -1;
+    uint64_t kb = 0;
+    int64_t cb = -1;
     for (int64_t c = c0; c < c1; ++c) {
-        sum += out.sum[c];
-        const uint64_t k = out.key[c];
-        if (k > kbest) {
-            kbest = k;
-            cbest = c;
+        sum += tsum[c];
+        const uint64_t k = tkey[c];
+        if (k > kb) {
+            kb = k;
+            cb = c;
         }
     }
-    if (cbest < 0) return;
+    const double* r = tets + (int64_t)TREC * t;
+    const double resid = r[26] - sum;
+    float m0b = 0.f;
+    const uint32_t mbits = (uint32_t)(kb >> 32);
+    memcpy(&m0b, &mbits, 4);
+    if (cb >= 0 && fabs(resid) <= RESID_WHOLE * (double)m0b) {
+        tetf[t] = resid;
+        tetb[t] = cb + base;
+    } else {
+        tetf[t] = residual_factor(r, sum);
+        tetb[t] = -1;
+    }
+}
+
+/* one thread per task: its share of its tet's residual, deposited into the
+ * task's largest fragment */
+template <int ORDER, bool BATCH>
+__global__ __launch_bounds__(256) void k_resid_dep(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
+                                                   int64_t ntask, int64_t base, GridInfo g0, MeshBox mb0,
+                                                   double* __restrict__ acc0, const BatchGeo* table, TaskOut out,
+                                                   const double* __restrict__ tetf,
+                                                   const int64_t* __restrict__ tetb) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ntask) return;
+    const uint64_t kb = out.key[c];
+    if (kb == 0) return;
+    const uint64_t w = tasks[c];
+    const int64_t t = (int64_t)(w & 0xffffffffull);
+    const int64_t whole = tetb[t];
+    if (whole >= 0 && whole != c + base) return;
+    const double weight = whole >= 0 ? tetf[t] : tetf[t] * out.sum[c];
+    if (weight == 0.0) return;
     BestFrag B;
-    B.kl = (int)(kbest & 0xffffffffull);
-    const uint32_t mbits = (uint32_t)(kbest >> 32);
+    B.kl = (int)(kb & 0xffffffffull);
+    const uint32_t mbits = (uint32_t)(kb >> 32);
     memcpy(&B.M.m0, &mbits, 4);
     if (ORDER > 0) {
-        const float4* m = (const float4*)(out.mom + 16 * cbest);
-        const float4 a = m[0], b = m[1], c = m[2], d = m[3];
+        const float4* m = (const float4*)(out.mom + 16 * c);
+        const float4 a = m[0], b = m[1], cc = m[2], d = m[3];
         B.M.s1[0] = a.y;
         B.M.s1[1] = a.z;
         B.M.s1[2] = a.w;
@@ -321,16 +424,29 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ tet
         B.M.s2[1] = b.y;
         B.M.s2[2] = b.z;
         B.M.s2[3] = b.w;
-        B.M.s2[4] = c.x;
-        B.M.s2[5] = c.y;
-        B.O[0] = c.z;
-        B.O[1] = c.w;
+        B.M.s2[4] = cc.x;
+        B.M.s2[5] = cc.y;
+        B.O[0] = cc.z;
+        B.O[1] = cc.w;
         B.O[2] = d.x;
     }
-    AtomicSink sink{acc};
-    tet_residual<ORDER>(sink, g, mb, tets + (int64_t)TREC * t, tasks[cbest], B, sum);
+    const double* r = tets + (int64_t)TREC * t;
+    with_mesh<BATCH>(r, g0, mb0, acc0, table, [&](const GridInfo& g, const MeshBox& mb, double* acc, int) {
+        AtomicSink sink{acc};
+        task_residual<ORDER>(sink, g, mb, r, w, B, weight);
+    });
 }
 
+/* the geometry of one column pass: one mesh, or a table of meshes */
+struct PassGeo {
+    GridInfo g;
+    MeshBox mb;
+    double* acc;
+    const BatchGeo* table; /* non-null: batched */
+    unsigned long long* mcnt;
+};
+
+/* the column kernel over tasks [0, ntask) of one mesh */
 static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask, const GridInfo& g, const MeshBox& mb,
                        double* acc, TaskOut out, unsigned* status, unsigned long long* counters, hipStream_t s) {
     const unsigned nb = (unsigned)((ntask + FBLOCK - 1) / FBLOCK);
@@ -343,16 +459,32 @@ static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask,
     return (int)hipGetLastError();
 }
 
-static int launch_residual(const double* tets, int64_t t0, int64_t t1, const GridInfo& g, const MeshBox& mb,
-                           const int64_t* colstart, int64_t base, const uint64_t* tasks, TaskOut out, double* acc,
-                           hipStream_t s) {
-    const unsigned nb = (unsigned)((t1 - t0 + 255) / 256);
-    if (g.order == 0)
-        k_residual<0><<<nb, 256, 0, s>>>(tets, t0, t1, g, mb, colstart, base, tasks, out, acc);
-    else if (g.order == 1)
-        k_residual<1><<<nb, 256, 0, s>>>(tets, t0, t1, g, mb, colstart, base, tasks, out, acc);
+template <int ORDER>
+static void launch_resid_dep_o(const double* tets, const uint64_t* tasks, int64_t ntask, int64_t base,
+                               const PassGeo& G, TaskOut out, const double* tetf, const int64_t* tetb,
+                               hipStream_t s) {
+    const unsigned nb = (unsigned)((ntask + 255) / 256);
+    if (G.table)
+        k_resid_dep<ORDER, true><<<nb, 256, 0, s>>>(tets, tasks, ntask, base, G.g, G.mb, G.acc, G.table, out, tetf,
+                                                    tetb);
     else
-        k_residual<2><<<nb, 256, 0, s>>>(tets, t0, t1, g, mb, colstart, base, tasks, out, acc);
+        k_resid_dep<ORDER, false><<<nb, 256, 0, s>>>(tets, tasks, ntask, base, G.g, G.mb, G.acc, nullptr, out, tetf,
+                                                     tetb);
+}
+
+/* the residual pass over the tets [t0, t1) and their ntask tasks; tetf,
+ * tetb: one double and one int64 per tet of the batch */
+static int launch_residual(const double* tets, int64_t t0, int64_t t1, const PassGeo& G, const int64_t* colstart,
+                           int64_t base, const uint64_t* tasks, int64_t ntask, TaskOut out, double* tetf,
+                           int64_t* tetb, hipStream_t s) {
+    k_resid_factor<<<(unsigned)((t1 - t0 + 255) / 256), 256, 0, s>>>(tets, t0, t1, colstart, base, out.sum, out.key,
+                                                                     tetf, tetb);
+    if (G.g.order == 0)
+        launch_resid_dep_o<0>(tets, tasks, ntask, base, G, out, tetf, tetb, s);
+    else if (G.g.order == 1)
+        launch_resid_dep_o<1>(tets, tasks, ntask, base, G, out, tetf, tetb, s);
+    else
+        launch_resid_dep_o<2>(tets, tasks, ntask, base, G, out, tetf, tetb, s);
     return (int)hipGetLastError();
 }
 
@@ -456,7 +588,7 @@ struct DevBuf {
 
 struct Workspace {
     DevBuf pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
-        colsum, colkey, colmom, ptets, adj_w, adj_g;
+        colsum, colkey, colmom, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt, bidx, bgat;
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -541,8 +673,8 @@ int mesh_to_device(Workspace& W, const psivox_mesh* mesh, int dev, hipStream_t s
  * is set (and nothing run) when a tet cannot take the column path.
  */
 template <class Fn>
-int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const GridInfo& g, const MeshBox& mb,
-                    unsigned* status, unsigned long long* counters, hipStream_t s, int dev, bool* fell, Fn fn) {
+int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo& G, unsigned* status,
+                    unsigned long long* counters, hipStream_t s, int dev, bool* fell, Fn fn) {
     int rc;
     *fell = false;
     if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
@@ -550,7 +682,12 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const GridInfo
     int64_t* colcnt = (int64_t*)W.colcnt.p;
     int64_t* colstart = (int64_t*)W.colstart.p;
     HIPCHK(hipMemsetAsync(colcnt + nt, 0, sizeof(int64_t), s));
-    k_col_count<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, colcnt, status, counters);
+    if (G.table)
+        k_col_count<true><<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, colcnt, status, counters,
+                                                               G.mcnt);
+    else
+        k_col_count<false><<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, G.g, G.mb, nullptr, colcnt, status,
+                                                                counters, nullptr);
     HIPCHK(hipGetLastError());
     size_t tb = 0;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, colcnt, colstart, nt + 1, s));
@@ -589,7 +726,12 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const GridInfo
         const int64_t base = hcs.empty() ? 0 : hcs[a0];
         const int64_t nsub = hcs.empty() ? ntask : hcs[a1] - hcs[a0];
         if (nsub == 0) continue;
-        k_col_emit<<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, g, mb, colstart, base, tasks);
+        if (G.table)
+            k_col_emit<true><<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, G.g, G.mb, G.table, colstart,
+                                                                       base, tasks);
+        else
+            k_col_emit<false><<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, G.g, G.mb, nullptr, colstart,
+                                                                        base, tasks);
         HIPCHK(hipGetLastError());
         if ((rc = fn(a0, a1, (const int64_t*)colstart, base, (const uint64_t*)tasks, nsub))) return rc;
     }
@@ -725,8 +867,9 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
             } else {
                 /* column tasks, then the per-tet residual pass */
                 bool fell = false;
+                const PassGeo G{g, mb, acc, nullptr, nullptr};
                 rc = for_task_chunks(
-                    W, tets, nt, g, mb, status, counters, s, dev, &fell,
+                    W, tets, nt, G, status, counters, s, dev, &fell,
                     [&](int64_t a0, int64_t a1, const int64_t* colstart, int64_t base, const uint64_t* tasks,
                         int64_t nsub) -> int {
                         int rc2;
@@ -744,7 +887,9 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                         float ms = 0.f;
                         HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
                         kms += ms;
-                        rc2 = launch_residual(tets, a0, a1, g, mb, colstart, base, tasks, out, acc, s);
+                        if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
+                        rc2 = launch_residual(tets, a0, a1, G, colstart, base, tasks, nsub, out,
+                                              (double*)W.colcnt.p, (int64_t*)W.tetb.p, s);
                         if (rc2)
                             return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
                                            hipGetErrorString((hipError_t)rc2));
@@ -928,6 +1073,255 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
     return 0;
 }
 
+/*
+ * Many (mesh, grid) pairs in one device pass.  Meshes are grouped up to the
+ * tet-buffer capacity; per group: raw conversion and expansion per mesh
+ * (records tagged with the mesh's index), one count / scan / emit over all
+ * its tets, the column kernel launched per mesh on that mesh's contiguous
+ * task range (its geometry in kernel arguments), one residual pass, and
+ * finalize per mesh.  The pass needs one field set for the whole batch,
+ * refinement level 0 and the column kernel; other batches, and a group in
+ * which a tet needs the FP64 kernel, run mesh by mesh.
+ */
+int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid* grids, const psivox_opts* o,
+                psivox_stats* stats, hipStream_t s, int dev) {
+    auto t_start = std::chrono::steady_clock::now();
+    int rc;
+    const int kernel = o->kernel == PSIVOX_KERNEL_AUTO ? PSIVOX_KERNEL_FAST : o->kernel;
+    std::vector<GridInfo> G(nmesh);
+    std::vector<MeshBox> MB(nmesh);
+    bool same = kernel == PSIVOX_KERNEL_FAST && o->mode == PSIVOX_MODE_DENSITY;
+    for (int i = 0; i < nmesh; ++i) {
+        if ((rc = check_mesh(&meshes[i]))) return rc;
+        if ((rc = fill_grid_info(&grids[i], o->xv_true_covariance, G[i]))) return rc;
+        MB[i].periodic = meshes[i].periodic ? 1 : 0;
+        MB[i].full_window = MB[i].periodic;
+        for (int a = 0; a < 3; ++a) {
+            MB[i].lo[a] = meshes[i].box[0][a];
+            MB[i].hi[a] = meshes[i].box[1][a];
+            if (MB[i].lo[a] != G[i].w0[a] || MB[i].hi[a] != G[i].w1[a]) MB[i].full_window = 0;
+        }
+        if (G[i].flags != G[0].flags) same = false;
+        if (o->refine_max_lvl > 0 && meshes[i].elemtype != 4) same = false;
+    }
+    auto one = [&](int i) -> int {
+        return run_one(W, &meshes[i], &grids[i], o, stats ? &stats[i] : nullptr, s, dev);
+    };
+    if (!same) {
+        for (int i = 0; i < nmesh; ++i)
+            if ((rc = one(i))) return rc;
+        return 0;
+    }
+    const int64_t max_tets = (int64_t)6 << 22;
+    auto per_of = [&](int i) -> int64_t { return meshes[i].elemtype == 4 ? 1 : 6; };
+    int64_t elems_total = 0, elems_done = 0;
+    for (int i = 0; i < nmesh; ++i) elems_total += meshes[i].nelem;
+    int i0 = 0;
+    while (i0 < nmesh) {
+        /* the group [i0, i1) */
+        int i1 = i0;
+        int64_t nt = 0;
+        while (i1 < nmesh && (i1 == i0 || nt + meshes[i1].nelem * per_of(i1) <= max_tets)) {
+            nt += meshes[i1].nelem * per_of(i1);
+            i1++;
+        }
+        if (i1 == i0 + 1 || nt == 0) {
+            for (int i = i0; i < i1; ++i)
+                if ((rc = one(i))) return rc;
+            for (int i = i0; i < i1; ++i) elems_done += meshes[i].nelem;
+            i0 = i1;
+            continue;
+        }
+        const int nm = i1 - i0;
+        /* device layout of the group: mesh arrays, fields, accumulators */
+        std::vector<MeshDev> md(nm);
+        std::vector<int64_t> T(nm + 1, 0), acc_off(nm + 1, 0), fld_off(nm + 1, 0);
+        size_t mesh_bytes = 0;
+        auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        for (int q = 0; q < nm; ++q) {
+            const psivox_mesh& m = meshes[i0 + q];
+            const GridInfo& g = G[i0 + q];
+            const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+            T[q + 1] = T[q] + m.nelem * per_of(i0 + q);
+            acc_off[q + 1] = acc_off[q] + nvox * g.nacc;
+            int64_t fw = 0;
+            for (int f = 0; f < 6; ++f)
+                if (grids[i0 + q].fields[f] && !grids[i0 + q].on_device) fw += FIELD_WIDTH[f];
+            fld_off[q + 1] = fld_off[q] + nvox * fw;
+            if (!m.on_device)
+                mesh_bytes += al(m.npart * 24) * (m.vel ? 2 : 1) + al(m.nelem * 8) + al(m.nelem * m.elemtype * 4);
+        }
+        if ((rc = W.bmesh.get(mesh_bytes, dev))) return rc;
+        if ((rc = W.bfield.get((size_t)fld_off[nm] * sizeof(double), dev))) return rc;
+        if ((rc = W.bacc.get((size_t)acc_off[nm] * sizeof(double), dev))) return rc;
+        if ((rc = W.tets.get((size_t)nt * TREC * sizeof(double), dev))) return rc;
+        if ((rc = W.bmcnt.get((size_t)(2 * nm + 8) * sizeof(unsigned long long), dev))) return rc;
+        if ((rc = W.counters.get(64, dev))) return rc;
+        unsigned long long* counters = (unsigned long long*)W.counters.p;
+        unsigned* status = (unsigned*)(counters + 4);
+        unsigned long long* mcnt = (unsigned long long*)W.bmcnt.p;
+        HIPCHK(hipMemsetAsync(counters, 0, 64, s));
+        HIPCHK(hipMemsetAsync(mcnt, 0, (size_t)(2 * nm) * sizeof(unsigned long long), s));
+        double* tets = (double*)W.tets.p;
+        double* bacc = (double*)W.bacc.p;
+        std::vector<FieldPtrs> fp(nm);
+        std::vector<std::array<double*, 6>> fdev(nm);
+        char* mp = (char*)W.bmesh.p;
+        for (int q = 0; q < nm; ++q) {
+            const int i = i0 + q;
+            const psivox_mesh& m = meshes[i];
+            md[q] = MeshDev{};
+            md[q].vpe = m.elemtype;
+            if (m.on_device) {
+                md[q].pos = m.pos;
+                md[q].vel = m.vel;
+                md[q].mass = m.mass;
+                md[q].conn = m.conn;
+            } else if (m.nelem > 0) {
+                HIPCHK(hipMemcpyAsync(mp, m.pos, m.npart * 24, hipMemcpyHostToDevice, s));
+                md[q].pos = (const double*)mp;
+                mp += al(m.npart * 24);
+                if (m.vel) {
+                    HIPCHK(hipMemcpyAsync(mp, m.vel, m.npart * 24, hipMemcpyHostToDevice, s));
+                    md[q].vel = (const double*)mp;
+                    mp += al(m.npart * 24);
+                }
+                HIPCHK(hipMemcpyAsync(mp, m.mass, m.nelem * 8, hipMemcpyHostToDevice, s));
+                md[q].mass = (const double*)mp;
+                mp += al(m.nelem * 8);
+                HIPCHK(hipMemcpyAsync(mp, m.conn, m.nelem * m.elemtype * 4, hipMemcpyHostToDevice, s));
+                md[q].conn = (const int32_t*)mp;
+                mp += al(m.nelem * m.elemtype * 4);
+            }
+            const GridInfo& g = G[i];
+            const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+            double* fbase = (double*)W.bfield.p + fld_off[q];
+            for (int f = 0; f < 6; ++f) {
+                fdev[q][f] = nullptr;
+                if (!grids[i].fields[f]) continue;
+                if (grids[i].on_device) {
+                    fdev[q][f] = grids[i].fields[f];
+                } else {
+                    fdev[q][f] = fbase;
+                    HIPCHK(hipMemcpyAsync(fbase, grids[i].fields[f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double),
+                                          hipMemcpyHostToDevice, s));
+                    fbase += nvox * FIELD_WIDTH[f];
+                }
+            }
+            for (int f = 0; f < 6; ++f) fp[q].f[f] = fdev[q][f];
+            k_to_raw<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, fp[q], bacc + acc_off[q], nvox);
+            HIPCHK(hipGetLastError());
+            if (m.nelem > 0) {
+                k_expand_fixed<<<grid_blocks(m.nelem, 128), 128, 0, s>>>(md[q], g, MB[i], 0, m.nelem,
+                                                                        tets + TREC * T[q], (double)q);
+                HIPCHK(hipGetLastError());
+                if (!(MB[i].periodic && MB[i].full_window)) {
+                    k_targets<<<grid_blocks(T[q + 1] - T[q], 64), 64, 0, s>>>(tets + TREC * T[q], T[q + 1] - T[q], g,
+                                                                             MB[i]);
+                    HIPCHK(hipGetLastError());
+                }
+            }
+        }
+        std::vector<BatchGeo> hb(nm);
+        for (int q = 0; q < nm; ++q) hb[q] = BatchGeo{G[i0 + q], MB[i0 + q], bacc + acc_off[q]};
+        if ((rc = W.btable.get(nm * sizeof(BatchGeo), dev))) return rc;
+        HIPCHK(hipMemcpyAsync(W.btable.p, hb.data(), nm * sizeof(BatchGeo), hipMemcpyHostToDevice, s));
+        if ((rc = W.bidx.get((nm + 1) * sizeof(int64_t), dev))) return rc;
+        if ((rc = W.bgat.get((nm + 1) * sizeof(int64_t), dev))) return rc;
+        const PassGeo PG{G[i0], MB[i0], nullptr, (const BatchGeo*)W.btable.p, mcnt};
+        double kms = 0.0;
+        bool fell = false;
+        rc = for_task_chunks(
+            W, tets, nt, PG, status, counters, s, dev, &fell,
+            [&](int64_t a0, int64_t a1, const int64_t* colstart, int64_t base, const uint64_t* tasks,
+                int64_t nsub) -> int {
+                int rc2;
+                /* task ranges of the meshes in this chunk of tets */
+                std::vector<int64_t> bnd(nm + 1), hv(nm + 1);
+                for (int q = 0; q <= nm; ++q) bnd[q] = std::min(std::max(T[q], a0), a1);
+                HIPCHK(hipMemcpyAsync(W.bidx.p, bnd.data(), (nm + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+                k_gather<<<grid_blocks(nm + 1, 64), 64, 0, s>>>(colstart, (const int64_t*)W.bidx.p, nm + 1,
+                                                                (int64_t*)W.bgat.p);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(hv.data(), W.bgat.p, (nm + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                if ((rc2 = W.colsum.get((size_t)nsub * sizeof(double), dev))) return rc2;
+                if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
+                if (G[i0].order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev))) return rc2;
+                TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
+                HIPCHK(hipEventRecord(W.ev0, s));
+                for (int q = 0; q < nm; ++q) {
+                    const int64_t c0 = hv[q] - base, c1 = hv[q + 1] - base;
+                    if (c1 <= c0) continue;
+                    TaskOut oq{out.sum + c0, out.key + c0, out.mom ? out.mom + 16 * c0 : nullptr};
+                    rc2 = launch_cols(tets, tasks + c0, c1 - c0, G[i0 + q], MB[i0 + q], bacc + acc_off[q], oq, status,
+                                      mcnt + 2 * q, s);
+                    if (rc2)
+                        return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
+                                       hipGetErrorString((hipError_t)rc2));
+                }
+                HIPCHK(hipEventRecord(W.ev1, s));
+                if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
+                rc2 = launch_residual(tets, a0, a1, PG, colstart, base, tasks, nsub, out, (double*)W.colcnt.p,
+                                      (int64_t*)W.tetb.p, s);
+                if (rc2)
+                    return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
+                                   hipGetErrorString((hipError_t)rc2));
+                HIPCHK(hipEventSynchronize(W.ev1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+                kms += ms;
+                return 0;
+            });
+        if (rc) return rc;
+        unsigned hs = 0;
+        HIPCHK(hipMemcpyAsync(&hs, status, sizeof hs, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (fell || (hs & ST_FALLBACK)) {
+            /* the fields are untouched until finalize: redo the group mesh by
+             * mesh (each falls back to the FP64 kernel where it must) */
+            for (int i = i0; i < i1; ++i)
+                if ((rc = one(i))) return rc;
+        } else {
+            std::vector<unsigned long long> hm(2 * nm);
+            for (int q = 0; q < nm; ++q) {
+                const int i = i0 + q;
+                const GridInfo& g = G[i];
+                const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+                k_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, bacc + acc_off[q], fp[q], nvox);
+                HIPCHK(hipGetLastError());
+                for (int f = 0; f < 6; ++f)
+                    if (grids[i].fields[f] && !grids[i].on_device)
+                        HIPCHK(hipMemcpyAsync(grids[i].fields[f], fdev[q][f],
+                                              (size_t)nvox * FIELD_WIDTH[f] * sizeof(double), hipMemcpyDeviceToHost, s));
+            }
+            HIPCHK(hipMemcpyAsync(hm.data(), mcnt, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (stats) {
+                const double wall =
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+                for (int q = 0; q < nm; ++q) {
+                    psivox_stats& st = stats[i0 + q];
+                    st.nfrag = (int64_t)hm[2 * q];
+                    st.nghosts = (int64_t)hm[2 * q + 1];
+                    st.ntets = T[q + 1] - T[q];
+                    st.kernel_ms = kms;
+                    st.total_ms = wall;
+                    st.kernel = PSIVOX_KERNEL_FAST;
+                    st.reserved = 0;
+                    st.nelem_used = -1;
+                }
+            }
+        }
+        for (int i = i0; i < i1; ++i) elems_done += meshes[i].nelem;
+        if (o->progress && o->progress(o->progress_ctx, elems_done, elems_total))
+            return set_err(PSIVOX_E_CANCELLED, "cancelled after %lld of %lld elements", (long long)elems_done,
+                           (long long)elems_total);
+        i0 = i1;
+    }
+    return 0;
+}
+
 int prepare(Workspace& W, const psivox_opts* o, hipStream_t* s, int* dev) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(PSIVOX_E_NODEV, "no GPU visible");
@@ -982,11 +1376,16 @@ int psivox_voxels(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts*
 int psivox_voxels_batched(int32_t nmesh, const psivox_mesh* meshes, psivox_grid* grids, const psivox_opts* opts,
                           psivox_stats* stats) {
     if (nmesh < 0 || (nmesh > 0 && (!meshes || !grids))) return set_err(PSIVOX_E_ARG, "bad batch arguments");
-    for (int i = 0; i < nmesh; ++i) {
-        int rc = psivox_voxels(&meshes[i], &grids[i], opts, stats ? &stats[i] : nullptr);
-        if (rc) return rc;
-    }
-    return 0;
+    if (nmesh == 0) return 0;
+    psivox_opts o{};
+    o.refine_tolerance = 1.0;
+    if (opts) o = *opts;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    hipStream_t s;
+    int dev;
+    int rc = prepare(g_ws, &o, &s, &dev);
+    if (rc) return rc;
+    return run_batched(g_ws, nmesh, meshes, grids, &o, stats, s, dev);
 }
 
 int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,

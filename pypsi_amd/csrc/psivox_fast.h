@@ -20,8 +20,8 @@
  *   - each column is clipped from the tet rather than from its neighbour
  *     slab, so shared faces of neighbouring columns agree only to FP32
  *     rounding; the per-tet mass residual (FP64 exact target minus FP64
- *     sum of the FP32 fragment masses) goes into the tet's largest fragment,
- *     which conserves total mass to FP64 rounding;
+ *     sum of the FP32 fragment masses) is spread over the largest fragment
+ *     of each of its columns, which conserves total mass to FP64 rounding;
  *   - deposits are the same additive FP64 raw moments as the FP64 kernel.
  */
 #ifndef PSIVOX_FAST_H
@@ -76,6 +76,7 @@ struct LPoly {
  * box corner in FP32; the FP64 tie-break re-reads the tet record. */
 struct CopyGeo {
     float x0[3];
+    double x0d[3]; /* x0 in FP64: plane offsets c - x0 are formed in FP64 */
     float DX[3][3]; /* DX[a][b] = x_{b+1,a} - x_{0,a} */
     const double* rec;
     double shift[3];
@@ -86,6 +87,7 @@ struct CopyGeo {
  * plus what the FP64 tie-break needs */
 struct AxisMap {
     float o, d0, d1, d2;
+    double od;
     int ax;
     double shift, w0, dinv;
     int ib0;
@@ -95,6 +97,7 @@ PVX_HD AxisMap axis_map(const CopyGeo& G, const GridInfo& g, int ax) {
     AxisMap m;
     m.ax = ax;
     m.o = pick3(G.x0, ax);
+    m.od = pick3(G.x0d, ax);
     m.d0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]);
     m.d1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]);
     m.d2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
@@ -107,19 +110,21 @@ PVX_HD AxisMap axis_map(const CopyGeo& G, const GridInfo& g, int ax) {
 
 /* FP64 c - x_ax(q) from the record: positions shifted by the copy's
  * periodic offset, in grid units relative to the index box corner */
-PVX_HD double sdist64(const double* r, const AxisMap& m, float c, float q0, float q1, float q2) {
+PVX_HD double sdist64(const double* r, const AxisMap& m, double c, float q0, float q1, float q2) {
     const double ra = m.ax == 0 ? r[0] : (m.ax == 1 ? r[1] : r[2]);
     const double rb = m.ax == 0 ? r[3] : (m.ax == 1 ? r[4] : r[5]);
     const double rc = m.ax == 0 ? r[6] : (m.ax == 1 ? r[7] : r[8]);
     const double rd = m.ax == 0 ? r[9] : (m.ax == 1 ? r[10] : r[11]);
     const double xa = (ra + m.shift) + (rb - ra) * (double)q0 + (rc - ra) * (double)q1 + (rd - ra) * (double)q2;
-    return (double)c - ((xa - m.w0) * m.dinv - (double)m.ib0);
+    return c - ((xa - m.w0) * m.dinv - (double)m.ib0);
 }
 
-/* signed distance c - x_ax(q) in cells; re-evaluated in FP64 from the
- * record when the FP32 value is within 1e-5 cells of the plane */
-PVX_HD float sdist(const double* rec, const AxisMap& m, float c, float q0, float q1, float q2) {
-    float s = c - (m.o + m.d0 * q0 + m.d1 * q1 + m.d2 * q2);
+/* signed distance c - x_ax(q) in cells as off - DX_ax q with off = c - x0
+ * formed in FP64 (the plane crosses the tet, so |off| is at most the tet's
+ * extent and its rounding scales with the tet, not with the cell index);
+ * re-evaluated in FP64 from the record within 1e-5 cells of the plane */
+PVX_HD float sdist(const double* rec, const AxisMap& m, double c, float off, float q0, float q1, float q2) {
+    float s = off - (m.d0 * q0 + m.d1 * q1 + m.d2 * q2);
     if (fabsf(s) < 1e-5f) {
         double sd = sdist64(rec, m, c, q0, q1, q2);
         s = (sd < 0.0) ? fminf(-1e-30f, (float)sd) : fmaxf(0.0f, (float)sd);
@@ -134,13 +139,14 @@ PVX_HD float sdist(const double* rec, const AxisMap& m, float c, float q0, float
  * face walk, then everything is compacted.  Only registers and the lane's
  * LDS slice are used.  Returns false on slot overflow or a broken graph.
  */
-PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, float c, int ax, bool keep_upper) {
+PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int ax, bool keep_upper) {
     const int onv = P.nv;
     if (onv <= 0) return true;
     const AxisMap m = axis_map(G, g, ax);
+    const float off = (float)(c - m.od);
     uint32_t gone = 0;
     for (int v = 0; v < onv; ++v) {
-        float t = sdist(G.rec, m, c, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
+        float t = sdist(G.rec, m, c, off, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
         bool drop = keep_upper ? (t >= 0.0f) : (t < 0.0f);
         if (drop) gone |= 1u << v;
     }
@@ -165,7 +171,7 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, float c, int a
             }
             const float c0 = P.qa(vc, 0), c1 = P.qa(vc, 1), c2 = P.qa(vc, 2);
             const float n0 = P.qa(vn, 0), n1 = P.qa(vn, 1), n2 = P.qa(vn, 2);
-            const float sc = sdist(G.rec, m, c, c0, c1, c2), sn = sdist(G.rec, m, c, n0, n1, n2);
+            const float sc = sdist(G.rec, m, c, off, c0, c1, c2), sn = sdist(G.rec, m, c, off, n0, n1, n2);
             const float inv = 1.0f / (sc - sn);
             P.setq(tot, inv * (n0 * sc - c0 * sn), inv * (n1 * sc - c1 * sn), inv * (n2 * sc - c2 * sn));
             P.setnbw(tot, pack3(vc, NBUNSET, NBUNSET));
@@ -405,15 +411,19 @@ struct SlabClip {
  * faces.  The loop runs over directed edges: no per-face control flow.
  */
 template <int ORDER>
-PVX_HD void slab_moments(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, float lo, bool has_hi, float hi,
-                         Mom& M, float* O) {
+PVX_HD void slab_moments(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi,
+                         double hi_c, Mom& M, float* O) {
     M.zero();
     const int nv = P.nv;
-    const float zx = pick3(G.x0, ax), zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
+    /* slab coordinates relative to vertex 0: the plane offsets are formed
+     * in FP64 (see sdist) */
+    const double x0d = pick3(G.x0d, ax);
+    const float lo = (float)(lo_c - x0d), hi = (float)(hi_c - x0d);
+    const float zx = 0.0f, zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
                 zd1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]),
                 zd2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
     {
-        const float zc = has_lo ? (has_hi ? 0.5f * (lo + hi) : lo + 0.5f) : (has_hi ? hi - 0.5f : 0.0f);
+        const float zc = has_lo ? (has_hi ? 0.5f * (lo + hi) : lo + 0.5f) : (has_hi ? hi - 0.5f : (float)(-x0d));
         int vb = 0;
         float db = 3.0e38f;
         for (int v = 0; v < nv; ++v) {
@@ -546,7 +556,7 @@ PVX_HD void deposit_f(Sink& sink, const GridInfo& g, const CopyGeo& G, const int
             /* centroid minus cell centre: the FP32 offset from the cell centre
              * is accurate to FP32 of a cell size */
             const float dq = G.DX[a][0] * qb[0] + G.DX[a][1] * qb[1] + G.DX[a][2] * qb[2];
-            const float cl = (G.x0[a] + dq) - ((float)(idx[a] - ib0[a]) + 0.5f);
+            const float cl = (float)(G.x0d[a] - ((double)(idx[a] - ib0[a]) + 0.5)) + dq;
             dx[a] = (double)cl * g.d[a];
         }
     if (fl & (F_V | F_VV | F_XV))
@@ -626,7 +636,7 @@ struct CopySetup {
     CopyGeo G;
     int ib1[3];
     unsigned wmask; /* window planes crossed: bit 2a lower, 2a+1 upper */
-    float wlo[3], whi[3];
+    double wlo[3], whi[3]; /* window planes, local units */
     float X[4][3];
 };
 
@@ -658,9 +668,10 @@ PVX_HD bool copy_setup(const double* rec, const double* lo, const double* hi, ui
         for (int v = 0; v < 4; ++v)
             C.X[v][a] = (float)((rec[3 * v + a] + C.G.shift[a] - g.w0[a]) * inv - C.G.ib0[a]);
         C.G.x0[a] = C.X[0][a];
+        C.G.x0d[a] = (rec[a] + C.G.shift[a] - g.w0[a]) * inv - C.G.ib0[a];
         for (int b = 0; b < 3; ++b) C.G.DX[a][b] = (float)((rec[3 * (b + 1) + a] - rec[a]) * inv);
-        C.wlo[a] = (float)(-C.G.ib0[a]);
-        C.whi[a] = (float)((g.w1[a] - g.w0[a]) / g.d[a] - C.G.ib0[a]);
+        C.wlo[a] = (double)(-C.G.ib0[a]);
+        C.whi[a] = (g.w1[a] - g.w0[a]) / g.d[a] - C.G.ib0[a];
     }
     return true;
 }
@@ -670,7 +681,7 @@ PVX_HD bool slab_columns(const CopySetup& C, int i, const GridInfo& g, int& j0, 
     const int il = i - C.G.ib0[0];
     const bool xlo = i > C.G.ib0[0], xhi = i < C.ib1[0] - 1;
     const bool sxlo = xlo || (C.wmask & 1u), sxhi = xhi || (C.wmask & 2u);
-    const float cxlo = xlo ? (float)il : C.wlo[0], cxhi = xhi ? (float)(il + 1) : C.whi[0];
+    const float cxlo = xlo ? (float)il : (float)C.wlo[0], cxhi = xhi ? (float)(il + 1) : (float)C.whi[0];
     float ymin, ymax;
     slab_yrange(C.X, sxlo, cxlo, sxhi, cxhi, ymin, ymax);
     if (ymin > ymax) return false;
@@ -758,19 +769,19 @@ PVX_HD bool build_column(LPoly& P, const CopySetup& C, const GridInfo& g, int i,
     P.setnbw(3, pack3(1, 2, 0));
     bool ok = true;
     if (xhi)
-        ok = clip_f(P, G, g, (float)(il + 1), 0, false);
+        ok = clip_f(P, G, g, (double)(il + 1), 0, false);
     else if (C.wmask & 2u)
         ok = clip_f(P, G, g, C.whi[0], 0, false);
     if (ok && xlo)
-        ok = clip_f(P, G, g, (float)il, 0, true);
+        ok = clip_f(P, G, g, (double)il, 0, true);
     else if (ok && (C.wmask & 1u))
         ok = clip_f(P, G, g, C.wlo[0], 0, true);
     if (ok && yhi)
-        ok = clip_f(P, G, g, (float)(jl + 1), 1, false);
+        ok = clip_f(P, G, g, (double)(jl + 1), 1, false);
     else if (ok && j == C.ib1[1] - 1 && (C.wmask & 8u))
         ok = clip_f(P, G, g, C.whi[1], 1, false);
     if (ok && ylo)
-        ok = clip_f(P, G, g, (float)jl, 1, true);
+        ok = clip_f(P, G, g, (double)jl, 1, true);
     else if (ok && j == G.ib0[1] && (C.wmask & 4u))
         ok = clip_f(P, G, g, C.wlo[1], 1, true);
     return ok;
@@ -782,8 +793,8 @@ PVX_HD void column_cell(const LPoly& P, const CopySetup& C, int k, int k0, int k
     const int kl = k - C.G.ib0[2];
     const bool zlo = k > k0 || (k == C.G.ib0[2] && (C.wmask & 16u));
     const bool zhi = k < k1 - 1 || (k == C.ib1[2] - 1 && (C.wmask & 32u));
-    const float lo_c = k > k0 ? (float)kl : C.wlo[2];
-    const float hi_c = k < k1 - 1 ? (float)(kl + 1) : C.whi[2];
+    const double lo_c = k > k0 ? (double)kl : C.wlo[2];
+    const double hi_c = k < k1 - 1 ? (double)(kl + 1) : C.whi[2];
     slab_moments<ORDER>(P, C.G, 2, zlo, lo_c, zhi, hi_c, Mm, O);
 }
 
@@ -868,16 +879,20 @@ PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const 
 }
 
 /*
- * The residual of one tet: its exact in-window mass fraction (record slot
- * 26) minus the FP64 sum of its FP32 fragment masses goes into its largest
- * fragment (task word `task`, fragment B), deposited with that fragment's
- * own means and covariances so every moment stays consistent.
+ * The residual of one tet, r = (exact in-window mass fraction, record slot
+ * 26) - (FP64 sum of its FP32 fragment masses), is spread over its column
+ * tasks: task c's largest fragment B takes r * sum_c / sum (sum_c: the
+ * task's fragment-mass sum), deposited with that fragment's own means and
+ * covariances so every moment stays consistent.  Spread this way each
+ * correction is about the size of the fragments' own FP32 error (a single
+ * receiving fragment would take the whole tet's).  This deposits one
+ * task's share: weight = r * sum_c / sum (or r for the one receiver of a
+ * tet with a small residual, RESID_WHOLE).
  */
 template <int ORDER, class Sink>
-PVX_HD void tet_residual(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task,
-                         const BestFrag& B, double sum_m0) {
-    if (B.kl < 0 || !(B.M.m0 > 0.0f)) return;
-    const double r = rec[26] - sum_m0;
+PVX_HD void task_residual(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task,
+                          const BestFrag& B, double weight) {
+    if (B.kl < 0 || !(B.M.m0 > 0.0f) || weight == 0.0) return;
     int64_t t;
     int h, il, jl;
     unpack_task(task, t, h, il, jl);
@@ -890,8 +905,16 @@ PVX_HD void tet_residual(Sink& sink, const GridInfo& g, const MeshBox& mb, const
     TetVel V;
     if (ORDER > 0) tet_vel(rec, V);
     deposit_f<ORDER>(sink, g, C.G, C.G.ib0, C.G.ib0[0] + il, C.G.ib0[1] + jl, C.G.ib0[2] + B.kl, B.M, B.O, V.u0,
-                     (const float (*)[3])V.DV, rec[24], r);
+                     (const float (*)[3])V.DV, rec[24], weight);
 }
+
+/* the per-tet factor r / sum of the residual spread */
+PVX_HD double residual_factor(const double* rec, double sum) { return sum > 0.0 ? (rec[26] - sum) / sum : 0.0; }
+
+/* A tet whose residual is at most RESID_WHOLE times its largest fragment's
+ * mass gives the whole residual to that fragment (one deposit; its
+ * relative change stays below RESID_WHOLE); larger residuals are spread. */
+constexpr double RESID_WHOLE = 2e-6;
 
 }  // namespace pvx
 

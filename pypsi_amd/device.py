@@ -131,6 +131,19 @@ def voxels(mesh: Mesh, grid: Grid, opts: Opts) -> dict:
     return st.as_dict()
 
 
+def voxels_batched(meshes, grids, opts: Opts) -> list:
+    """Many (mesh, grid) pairs in one device pass (psivox_voxels_batched);
+    returns one stats dict per pair."""
+    n = len(meshes)
+    if len(grids) != n:
+        raise ValueError("meshes and grids must have the same length")
+    ma = (Mesh * n)(*meshes)
+    ga = (Grid * n)(*grids)
+    sa = (Stats * n)()
+    check(lib().psivox_voxels_batched(n, ma, ga, C.byref(opts), sa))
+    return [st.as_dict() for st in sa]
+
+
 def mass_adjoint(mesh: Mesh, grid: Grid, dl_dm, dl_dpos, opts: Opts) -> None:
     """dl_dpos += dL/dpos for dL/dm = dl_dm (device tensors: grid-shaped
     float64, and (npart, 3) float64)."""

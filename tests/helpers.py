@@ -19,6 +19,14 @@ BOX = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
 ALL = ["x", "v", "xx", "xv", "vv"]
 HEAVY = 1e-3
 
+# The FP32-clip column kernel against the FP64 oracle, as (rel, wabs):
+# cell masses and means within 2e-5 relative on heavy cells (a heavy cell can
+# hold a single sliver fragment of one tet, whose FP32 q-coordinates carry
+# ~1e-8 of the tet's mass), mass-weighted errors within 1e-6 of the peak;
+# second moments within 1e-3 and 3e-5.
+TOL32 = {"m": (2e-5, 1e-6), "x": (2e-5, 1e-6), "v": (2e-5, 1e-6), "xx": (1e-3, 3e-5), "xv": (1e-3, 3e-5),
+         "vv": (1e-3, 3e-5)}
+
 
 def field_errors(a, b, mass_ref=None):
     """Errors of a against the reference b; returns {field: (rel, wabs)}."""

@@ -4,6 +4,7 @@
  * for the host with plain += deposits, so the CPU test suite can check the
  * kernel logic against the oracle without a GPU.  Not used by the product.
  */
+#include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -88,34 +89,42 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
             tasks.resize((size_t)ncol);
             tet_columns(r, t, g, mb, tasks.data(), nullptr);
             ntasks += ncol;
+            std::vector<double> tsum((size_t)ncol);
+            std::vector<BestFrag> tbest((size_t)ncol);
             double sum = 0.0;
-            BestFrag best;
-            best.kl = -1;
-            best.M.m0 = 0.0f;
-            uint64_t best_task = 0;
             for (int64_t c = 0; c < ncol; ++c) {
-                double s1;
-                BestFrag b1;
                 bool ok;
                 if (g.order == 0)
-                    ok = column_task<0>(sink, g, mb, r, tasks[c], P, nfrag, s1, b1);
+                    ok = column_task<0>(sink, g, mb, r, tasks[c], P, nfrag, tsum[c], tbest[c]);
                 else if (g.order == 1)
-                    ok = column_task<1>(sink, g, mb, r, tasks[c], P, nfrag, s1, b1);
+                    ok = column_task<1>(sink, g, mb, r, tasks[c], P, nfrag, tsum[c], tbest[c]);
                 else
-                    ok = column_task<2>(sink, g, mb, r, tasks[c], P, nfrag, s1, b1);
+                    ok = column_task<2>(sink, g, mb, r, tasks[c], P, nfrag, tsum[c], tbest[c]);
                 if (!ok) fails++;
-                sum += s1;
-                if (b1.kl >= 0 && best_key(b1.M.m0, b1.kl) > best_key(best.M.m0, best.kl)) {
-                    best = b1;
-                    best_task = tasks[c];
+                sum += tsum[c];
+            }
+            /* the largest fragment (first task on ties, as the GPU's key max) */
+            int64_t cb = -1;
+            uint64_t kb = 0;
+            for (int64_t c = 0; c < ncol; ++c) {
+                const uint64_t k = tbest[c].kl >= 0 ? best_key(tbest[c].M.m0, tbest[c].kl) : 0ull;
+                if (k > kb) {
+                    kb = k;
+                    cb = c;
                 }
             }
-            if (g.order == 0)
-                tet_residual<0>(sink, g, mb, r, best_task, best, sum);
-            else if (g.order == 1)
-                tet_residual<1>(sink, g, mb, r, best_task, best, sum);
-            else
-                tet_residual<2>(sink, g, mb, r, best_task, best, sum);
+            const double resid = r[26] - sum;
+            const bool whole = cb >= 0 && fabs(resid) <= RESID_WHOLE * (double)tbest[cb].M.m0;
+            const double f = residual_factor(r, sum);
+            for (int64_t c = 0; c < ncol; ++c) {
+                const double wgt = whole ? (c == cb ? resid : 0.0) : f * tsum[c];
+                if (g.order == 0)
+                    task_residual<0>(sink, g, mb, r, tasks[c], tbest[c], wgt);
+                else if (g.order == 1)
+                    task_residual<1>(sink, g, mb, r, tasks[c], tbest[c], wgt);
+                else
+                    task_residual<2>(sink, g, mb, r, tasks[c], tbest[c], wgt);
+            }
         }
         ntets += nt;
     }

@@ -2,7 +2,7 @@
 the FP64 oracle.
 
 Tolerances (FP32 clipping; measures of tests/helpers.py): on cells holding
->= 1e-3 of the peak mass, mass within 1e-5 relative, means within 1e-5,
+>= 1e-3 of the peak mass, mass within 2e-5 relative, means within 2e-5,
 covariances within 1e-3 of their diagonal scale; mass-weighted absolute
 errors within 1e-6 (m, x, v) and 3e-5 (second moments) of the field's peak;
 total mass conserved to 1e-12 relative; fragment count within 1e-5 relative
@@ -12,12 +12,10 @@ drops m0 <= 0, so slivers at the FP32 noise floor may differ).
 import numpy as np
 import pytest
 
-from helpers import ALL, BOX, assert_close
+from helpers import ALL, BOX, assert_close, TOL32
 
 pytestmark = pytest.mark.gpu
 
-TOL32 = {"m": (1e-5, 1e-6), "x": (1e-5, 1e-6), "v": (1e-5, 1e-6), "xx": (1e-3, 3e-5), "xv": (1e-3, 3e-5),
-         "vv": (1e-3, 3e-5)}
 
 
 @pytest.fixture(scope="module")

@@ -6,12 +6,10 @@ import numpy as np
 import pytest
 
 import hostsim_api as H
-from helpers import ALL, BOX, assert_close
+from helpers import ALL, BOX, assert_close, TOL32
 
 TOL64 = {"m": (1e-12, 1e-13), "x": (1e-10, 1e-13), "v": (1e-10, 1e-13), "xx": (1e-7, 1e-12), "xv": (1e-7, 1e-12),
          "vv": (1e-7, 1e-12)}
-TOL32 = {"m": (1e-5, 1e-6), "x": (1e-5, 1e-6), "v": (1e-5, 1e-6), "xx": (1e-3, 3e-5), "xv": (1e-3, 3e-5),
-         "vv": (1e-3, 3e-5)}
 
 
 def zel(n, rms, seed):
