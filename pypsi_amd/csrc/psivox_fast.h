@@ -1,27 +1,29 @@
 /*
- * psivox_fast.h -- the fast voxelization kernel: FP32 clipping in cell-local
- * coordinates, FP64 deposit, one thread per tet (all its periodic copies).
+ * psivox_fast.h -- the fast voxelization path: FP32 clipping in cell-local
+ * coordinates, FP64 deposit, one lane per column task (a tet's periodic copy
+ * restricted to one (x, y) column of its index box).
  *
- * Per copy, for every (x, y) column of its index box the tet is clipped by
- * the column's four planes into a polyhedron held in this lane's slice of
- * LDS (vertex graph with 3 neighbours per vertex as in the reference,
- * include/geometry.h:118-129, but storing only the FP32 mass coordinate q:
- * positions are x(q) = x0 + DX q in grid units relative to the copy's index
- * box).  Each cell of the column is then integrated without building its
- * fragment: the column's faces are walked once (src/geometry.c:692-767),
- * each face polygon clipped on the fly to the cell's z-slab and
- * fan-triangulated, and the two cap faces closed through a point on each
- * plane.  All moments are taken about the column's first vertex in q-space.
+ * A task clips the tet by the column's four planes into a polyhedron held in
+ * the lane's slice of LDS (vertex graph with 3 neighbours per vertex as in
+ * the reference, include/geometry.h:118-129, but storing only the FP32 mass
+ * coordinate q: positions are x(q) = x0 + DX q in grid units relative to the
+ * copy's index box).  Each cell of the column is then integrated without
+ * building its fragment: the column's faces are fanned once as the
+ * reference's reduce fans them (src/geometry.c:692-767), and every fan
+ * triangle's piece inside the cell's z-slab is integrated by a fixed set of
+ * weighted terms (no branches on the geometry, so the lanes of a wavefront
+ * stay together).
  *
  * Differences from the reference arithmetic (the method is the same):
- *   - cuts and moments in FP32 on cell-local coordinates; a vertex within
- *     1e-5 cells of a plane is classified with an FP64 evaluation of its
- *     signed distance;
+ *   - cuts and moments in FP32 on cell-local coordinates, plane offsets
+ *     formed in FP64; a vertex within 1e-5 cells of a plane is classified
+ *     with an FP64 evaluation of its signed distance;
  *   - each column is clipped from the tet rather than from its neighbour
  *     slab, so shared faces of neighbouring columns agree only to FP32
  *     rounding; the per-tet mass residual (FP64 exact target minus FP64
- *     sum of the FP32 fragment masses) is spread over the largest fragment
- *     of each of its columns, which conserves total mass to FP64 rounding;
+ *     sum of the FP32 fragment masses) goes to its largest fragment when
+ *     small, otherwise it is spread over the largest fragment of each of
+ *     its columns; total mass is conserved to FP64 rounding;
  *   - deposits are the same additive FP64 raw moments as the FP64 kernel.
  */
 #ifndef PSIVOX_FAST_H
@@ -287,187 +289,197 @@ PVX_HD void face_apexes(LPoly& P) {
         }
 }
 
-/* one polygon vertex stream clipped to the slab lo < z <= hi: feed points
- * in traversal order, the clipped polygon is fanned from its first point
- * and each exit -> entry edge on a slab plane adds the cap triangle */
+/* tri_acc with the signed volume scaled by w (0 or 1: a term that is
+ * absent contributes nothing without a branch) */
 template <int ORDER>
-struct SlabClip {
-    Mom* M;
-    bool has_lo, has_hi;
-    float lo, hi;
-    float* clo;
-    float* chi;
-    bool* clo_set;
-    bool* chi_set;
-    float e0[3], ep[3], xp[3];
-    int nem, exit_plane;
+PVX_HD void tri_acc_w(Mom& M, const float* A, const float* B, const float* C, float w) {
+    float d = A[0] * (B[1] * C[2] - B[2] * C[1]) - A[1] * (B[0] * C[2] - B[2] * C[0]) + A[2] * (B[0] * C[1] - B[1] * C[0]);
+    d *= w;
+    M.m0 += d;
+    if (ORDER > 0) {
+        const float S0 = A[0] + B[0] + C[0], S1 = A[1] + B[1] + C[1], S2 = A[2] + B[2] + C[2];
+        M.s1[0] += d * S0;
+        M.s1[1] += d * S1;
+        M.s1[2] += d * S2;
+        if (ORDER > 1) {
+            M.s2[0] += d * (A[0] * A[0] + B[0] * B[0] + C[0] * C[0] + S0 * S0);
+            M.s2[1] += d * (A[0] * A[1] + B[0] * B[1] + C[0] * C[1] + S0 * S1);
+            M.s2[2] += d * (A[0] * A[2] + B[0] * B[2] + C[0] * C[2] + S0 * S2);
+            M.s2[3] += d * (A[1] * A[1] + B[1] * B[1] + C[1] * C[1] + S1 * S1);
+            M.s2[4] += d * (A[1] * A[2] + B[1] * B[2] + C[1] * C[2] + S1 * S2);
+            M.s2[5] += d * (A[2] * A[2] + B[2] * B[2] + C[2] * C[2] + S2 * S2);
+        }
+    }
+}
 
-    PVX_HD void cap(int plane, const float* X, const float* N) {
-        if (plane == 0) {
-            if (!*clo_set) {
-                clo[0] = X[0];
-                clo[1] = X[1];
-                clo[2] = X[2];
-                *clo_set = true;
+/* a point of P on the plane z = p (z = zd . q): the crossing of the first
+ * edge found between z <= p and z > p; false when no edge crosses */
+PVX_HD bool plane_point(const LPoly& P, float zd0, float zd1, float zd2, float p, float* out) {
+    bool found = false;
+    for (int v = 0; v < P.nv && !found; ++v) {
+        const float q0 = P.qa(v, 0), q1 = P.qa(v, 1), q2 = P.qa(v, 2);
+        const float zv = zd0 * q0 + zd1 * q1 + zd2 * q2;
+        const uint32_t w = P.nbw(v);
+        for (int k = 0; k < 3 && !found; ++k) {
+            const int b = nbf(w, k);
+            const float r0 = P.qa(b, 0), r1 = P.qa(b, 1), r2 = P.qa(b, 2);
+            const float zb = zd0 * r0 + zd1 * r1 + zd2 * r2;
+            if (zv <= p && zb > p) {
+                const float t = (p - zv) / (zb - zv);
+                out[0] = q0 + t * (r0 - q0);
+                out[1] = q1 + t * (r1 - q1);
+                out[2] = q2 + t * (r2 - q2);
+                found = true;
             }
-            tri_acc<ORDER>(*M, clo, X, N);
-        } else {
-            if (!*chi_set) {
-                chi[0] = X[0];
-                chi[1] = X[1];
-                chi[2] = X[2];
-                *chi_set = true;
-            }
-            tri_acc<ORDER>(*M, chi, X, N);
         }
     }
-    PVX_HD void emit(const float* pt) {
-        if (nem == 0) {
-            e0[0] = pt[0];
-            e0[1] = pt[1];
-            e0[2] = pt[2];
-        } else if (nem >= 2) {
-            tri_acc<ORDER>(*M, e0, pt, ep);
-        }
-        ep[0] = pt[0];
-        ep[1] = pt[1];
-        ep[2] = pt[2];
-        nem++;
+    return found;
+}
+
+/* exit and entry points of the triangle (c, a, b), traversed c -> a -> b,
+ * on the plane z = p; inside(z) is the kept side.  X: where the boundary
+ * leaves the kept side, N: where it re-enters; valid when both exist.
+ * Invalid segments come back as (c, c). */
+template <bool UPPER_KEPT_IS_LOWER>
+PVX_HD bool tri_segment(const float* qc, const float* qa, const float* qb, float zc, float za, float zb, float p,
+                        float* X, float* N) {
+    /* kept: z > p (lower plane of a slab) or z <= p (upper plane) */
+    const bool ic = UPPER_KEPT_IS_LOWER ? (zc <= p) : (zc > p);
+    const bool ia = UPPER_KEPT_IS_LOWER ? (za <= p) : (za > p);
+    const bool ib = UPPER_KEPT_IS_LOWER ? (zb <= p) : (zb > p);
+    const bool valid = (ic != ia) || (ia != ib);
+    /* crossing parameters of the three edges (only used where they cross) */
+    const float tca = (za != zc) ? (p - zc) / (za - zc) : 0.f;
+    const float tab = (zb != za) ? (p - za) / (zb - za) : 0.f;
+    const float tbc = (zc != zb) ? (p - zb) / (zc - zb) : 0.f;
+    float Pca[3], Pab[3], Pbc[3];
+    for (int i = 0; i < 3; ++i) {
+        Pca[i] = qc[i] + tca * (qa[i] - qc[i]);
+        Pab[i] = qa[i] + tab * (qb[i] - qa[i]);
+        Pbc[i] = qb[i] + tbc * (qc[i] - qb[i]);
     }
-    /* the edge a -> b (q relative to the origin, z their slab coordinates) */
-    PVX_HD void edge(const float* qa, float za, const float* qb, float zb) {
-        float t0 = 0.f, t1 = 1.f;
-        int p0 = -1, p1 = -1;
-        bool empty = false;
-        const float dz = zb - za;
-        if (has_lo) { /* keep z > lo */
-            if (za <= lo && zb <= lo) {
-                empty = true;
-            } else if (za <= lo) {
-                t0 = (lo - za) / dz;
-                p0 = 0;
-            } else if (zb <= lo) {
-                t1 = (lo - za) / dz;
-                p1 = 0;
-            }
-        }
-        if (!empty && has_hi) { /* keep z <= hi */
-            if (za > hi && zb > hi) {
-                empty = true;
-            } else if (za > hi) {
-                const float t = (hi - za) / dz;
-                if (t > t0) {
-                    t0 = t;
-                    p0 = 1;
-                }
-            } else if (zb > hi) {
-                const float t = (hi - za) / dz;
-                if (t < t1) {
-                    t1 = t;
-                    p1 = 1;
-                }
-            }
-        }
-        /* a touching interval (t0 == t1, e.g. a on a plane) still emits its
-         * point, as Sutherland-Hodgman emits an inside vertex */
-        if (empty || t0 > t1) return;
-        float pt[3];
-        if (p0 < 0) {
-            pt[0] = qa[0];
-            pt[1] = qa[1];
-            pt[2] = qa[2];
-        } else {
-            pt[0] = qa[0] + t0 * (qb[0] - qa[0]);
-            pt[1] = qa[1] + t0 * (qb[1] - qa[1]);
-            pt[2] = qa[2] + t0 * (qb[2] - qa[2]);
-            if (exit_plane == p0) { /* exit -> entry on this plane: cap runs it backwards */
-                cap(p0, xp, pt);
-                exit_plane = -1;
-            }
-        }
-        emit(pt);
-        if (p1 >= 0) {
-            const float pe[3] = {qa[0] + t1 * (qb[0] - qa[0]), qa[1] + t1 * (qb[1] - qa[1]),
-                                 qa[2] + t1 * (qb[2] - qa[2])};
-            emit(pe);
-            exit_plane = p1;
-            xp[0] = pe[0];
-            xp[1] = pe[1];
-            xp[2] = pe[2];
-        }
+    const bool xca = ic && !ia, xab = ia && !ib;
+    const bool nca = !ic && ia, nab = !ia && ib;
+    for (int i = 0; i < 3; ++i) {
+        const float x = xca ? Pca[i] : (xab ? Pab[i] : Pbc[i]);
+        const float n = nca ? Pca[i] : (nab ? Pab[i] : Pbc[i]);
+        X[i] = valid ? x : qc[i];
+        N[i] = valid ? n : qc[i];
     }
-    /* the stream began outside: its last exit joins its first entry */
-    PVX_HD void close() {
-        if (exit_plane >= 0 && nem > 0) cap(exit_plane, xp, e0);
+    return valid;
+}
+
+/* the piece of the edge u -> v inside lo < z <= hi as parameters t0 < t1
+ * (valid) of u + t (v - u) */
+PVX_HD bool edge_piece(float zu, float zv, float lo, float hi, float& t0, float& t1) {
+    const float dz = zv - zu;
+    if (dz != 0.f) {
+        const float inv = 1.0f / dz;
+        const float tl = (lo - zu) * inv, th = (hi - zu) * inv;
+        t0 = fmaxf(0.f, fminf(tl, th));
+        t1 = fminf(1.f, fmaxf(tl, th));
+    } else {
+        const bool in = zu > lo && zu <= hi;
+        t0 = in ? 0.f : 1.f;
+        t1 = in ? 1.f : 0.f;
     }
-};
+    return t0 < t1;
+}
 
 /*
  * Moments of P intersected with the slab lo < x_ax <= hi (either side may be
- * open) about the origin O, the vertex nearest the slab (small FP32
- * cancellation).  P's faces are covered by their fan triangles (apexes from
- * face_apexes); each triangle is clipped to the slab on its own, its cap
- * edges chain across the fan diagonals, so the result equals clipping whole
- * faces.  The loop runs over directed edges: no per-face control flow.
+ * open) about an origin O on the lower plane (on the upper one when there
+ * is no lower plane), with no branches on the geometry.  For each fan
+ * triangle T = (c, a, b) of P's faces (apexes from face_apexes) the cone
+ * over T n slab from O is fanned from G, the mean of the endpoints of T's
+ * edge pieces inside the slab (a point of T n slab, so every term is of
+ * the fragment's size): one term per boundary piece -- the three clipped
+ * edges and T's segments on the two planes.  The cap on the upper plane
+ * is fanned from a point of that plane through the same segments; the cap
+ * on the lower plane adds nothing (O lies on it).  Six fixed terms per
+ * triangle, each weighted 0 or 1.
  */
 template <int ORDER>
-PVX_HD void slab_moments(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi,
-                         double hi_c, Mom& M, float* O) {
+PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi,
+                           double hi_c, Mom& M, float* O) {
     M.zero();
     const int nv = P.nv;
-    /* slab coordinates relative to vertex 0: the plane offsets are formed
-     * in FP64 (see sdist) */
     const double x0d = pick3(G.x0d, ax);
-    const float lo = (float)(lo_c - x0d), hi = (float)(hi_c - x0d);
-    const float zx = 0.0f, zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
+    const float lo = has_lo ? (float)(lo_c - x0d) : -3.0e38f, hi = has_hi ? (float)(hi_c - x0d) : 3.0e38f;
+    const float zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
                 zd1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]),
                 zd2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
-    {
-        const float zc = has_lo ? (has_hi ? 0.5f * (lo + hi) : lo + 0.5f) : (has_hi ? hi - 0.5f : (float)(-x0d));
-        int vb = 0;
-        float db = 3.0e38f;
-        for (int v = 0; v < nv; ++v) {
-            const float z = zx + zd0 * P.qa(v, 0) + zd1 * P.qa(v, 1) + zd2 * P.qa(v, 2);
-            const float dd = fabsf(z - zc);
-            if (dd < db) {
-                db = dd;
-                vb = v;
-            }
-        }
-        O[0] = P.qa(vb, 0);
-        O[1] = P.qa(vb, 1);
-        O[2] = P.qa(vb, 2);
+    /* the origin on a slab plane, the upper cap's anchor */
+    O[0] = P.qa(0, 0);
+    O[1] = P.qa(0, 1);
+    O[2] = P.qa(0, 2);
+    float Gh[3] = {O[0], O[1], O[2]};
+    if (has_lo) {
+        plane_point(P, zd0, zd1, zd2, lo, O);
+        if (has_hi && !plane_point(P, zd0, zd1, zd2, hi, Gh)) Gh[0] = O[0], Gh[1] = O[1], Gh[2] = O[2];
+    } else if (has_hi) {
+        plane_point(P, zd0, zd1, zd2, hi, O);
     }
-    float clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
-    bool clo_set = false, chi_set = false;
+    const float gh[3] = {Gh[0] - O[0], Gh[1] - O[1], Gh[2] - O[2]};
+    const bool cap_hi = has_lo && has_hi;
     for (int a = 0; a < nv; ++a) {
         const uint32_t wa = P.nbw(a);
-        float qa[3] = {P.qa(a, 0) - O[0], P.qa(a, 1) - O[1], P.qa(a, 2) - O[2]};
-        const float za = zx + zd0 * P.qa(a, 0) + zd1 * P.qa(a, 1) + zd2 * P.qa(a, 2);
+        const float pa0 = P.qa(a, 0), pa1 = P.qa(a, 1), pa2 = P.qa(a, 2);
+        const float qa[3] = {pa0 - O[0], pa1 - O[1], pa2 - O[2]};
+        const float za = zd0 * pa0 + zd1 * pa1 + zd2 * pa2;
         for (int k = 0; k < 3; ++k) {
             const int b = nbf(wa, k), c = apf(wa, k);
             if (c == a || c == b) continue;
-            const float qb[3] = {P.qa(b, 0) - O[0], P.qa(b, 1) - O[1], P.qa(b, 2) - O[2]};
-            const float qc[3] = {P.qa(c, 0) - O[0], P.qa(c, 1) - O[1], P.qa(c, 2) - O[2]};
-            const float zb = zx + zd0 * P.qa(b, 0) + zd1 * P.qa(b, 1) + zd2 * P.qa(b, 2);
-            const float zcv = zx + zd0 * P.qa(c, 0) + zd1 * P.qa(c, 1) + zd2 * P.qa(c, 2);
-            /* the fan triangle (c, a, b), traversed c -> a -> b */
-            SlabClip<ORDER> sc;
-            sc.M = &M;
-            sc.has_lo = has_lo;
-            sc.has_hi = has_hi;
-            sc.lo = lo;
-            sc.hi = hi;
-            sc.clo = clo;
-            sc.chi = chi;
-            sc.clo_set = &clo_set;
-            sc.chi_set = &chi_set;
-            sc.nem = 0;
-            sc.exit_plane = -1;
-            sc.edge(qc, zcv, qa, za);
-            sc.edge(qa, za, qb, zb);
-            sc.edge(qb, zb, qc, zcv);
-            sc.close();
+            const float pb0 = P.qa(b, 0), pb1 = P.qa(b, 1), pb2 = P.qa(b, 2);
+            const float pc0 = P.qa(c, 0), pc1 = P.qa(c, 1), pc2 = P.qa(c, 2);
+            const float qb[3] = {pb0 - O[0], pb1 - O[1], pb2 - O[2]};
+            const float qc[3] = {pc0 - O[0], pc1 - O[1], pc2 - O[2]};
+            const float zb = zd0 * pb0 + zd1 * pb1 + zd2 * pb2;
+            const float zc = zd0 * pc0 + zd1 * pc1 + zd2 * pc2;
+            /* the edge pieces inside the slab, in traversal order */
+            float s0, s1, u0, u1, v0, v1;
+            const bool eca = edge_piece(zc, za, lo, hi, s0, s1);
+            const bool eab = edge_piece(za, zb, lo, hi, u0, u1);
+            const bool ebc = edge_piece(zb, zc, lo, hi, v0, v1);
+            float A1[3], B1[3], A2[3], B2[3], A3[3], B3[3], Gp[3];
+            float nG = 0.f;
+            for (int i = 0; i < 3; ++i) {
+                A1[i] = qc[i] + s0 * (qa[i] - qc[i]);
+                B1[i] = qc[i] + s1 * (qa[i] - qc[i]);
+                A2[i] = qa[i] + u0 * (qb[i] - qa[i]);
+                B2[i] = qa[i] + u1 * (qb[i] - qa[i]);
+                A3[i] = qb[i] + v0 * (qc[i] - qb[i]);
+                B3[i] = qb[i] + v1 * (qc[i] - qb[i]);
+                Gp[i] = (eca ? A1[i] + B1[i] : 0.f) + (eab ? A2[i] + B2[i] : 0.f) + (ebc ? A3[i] + B3[i] : 0.f);
+            }
+            nG = (eca ? 2.f : 0.f) + (eab ? 2.f : 0.f) + (ebc ? 2.f : 0.f);
+            if (nG == 0.f) {
+                /* no edge inside: the slab may still cross the triangle's
+                 * interior; any point of the plane works (its segments
+                 * are the whole boundary) */
+                Gp[0] = qc[0];
+                Gp[1] = qc[1];
+                Gp[2] = qc[2];
+            } else {
+                const float inv = 1.0f / nG;
+                Gp[0] *= inv;
+                Gp[1] *= inv;
+                Gp[2] *= inv;
+            }
+            tri_acc_w<ORDER>(M, Gp, B1, A1, eca ? 1.f : 0.f);
+            tri_acc_w<ORDER>(M, Gp, B2, A2, eab ? 1.f : 0.f);
+            tri_acc_w<ORDER>(M, Gp, B3, A3, ebc ? 1.f : 0.f);
+            if (has_lo) {
+                float X[3], N[3];
+                const bool v = tri_segment<false>(qc, qa, qb, zc, za, zb, lo, X, N);
+                tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
+            }
+            if (has_hi) {
+                float X[3], N[3];
+                const bool v = tri_segment<true>(qc, qa, qb, zc, za, zb, hi, X, N);
+                tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
+                if (cap_hi) tri_acc_w<ORDER>(M, gh, X, N, v ? 1.f : 0.f);
+            }
         }
     }
 }
@@ -795,7 +807,7 @@ PVX_HD void column_cell(const LPoly& P, const CopySetup& C, int k, int k0, int k
     const bool zhi = k < k1 - 1 || (k == C.ib1[2] - 1 && (C.wmask & 32u));
     const double lo_c = k > k0 ? (double)kl : C.wlo[2];
     const double hi_c = k < k1 - 1 ? (double)(kl + 1) : C.whi[2];
-    slab_moments<ORDER>(P, C.G, 2, zlo, lo_c, zhi, hi_c, Mm, O);
+    slab_moments_u<ORDER>(P, C.G, 2, zlo, lo_c, zhi, hi_c, Mm, O);
 }
 
 /* per-tet velocity data of a record */
