@@ -422,14 +422,32 @@ PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo
     }
     const float gh[3] = {Gh[0] - O[0], Gh[1] - O[1], Gh[2] - O[2]};
     const bool cap_hi = has_lo && has_hi;
+    /* the fan triangles (bit 3a + k: directed edge a -> neighbour k) that
+     * reach into the slab: z_max > lo and z_min <= hi (a triangle lying on
+     * the upper plane belongs to the piece, one on the lower plane not) */
+    uint64_t mask = 0;
     for (int a = 0; a < nv; ++a) {
         const uint32_t wa = P.nbw(a);
-        const float pa0 = P.qa(a, 0), pa1 = P.qa(a, 1), pa2 = P.qa(a, 2);
-        const float qa[3] = {pa0 - O[0], pa1 - O[1], pa2 - O[2]};
-        const float za = zd0 * pa0 + zd1 * pa1 + zd2 * pa2;
+        const float za = zd0 * P.qa(a, 0) + zd1 * P.qa(a, 1) + zd2 * P.qa(a, 2);
         for (int k = 0; k < 3; ++k) {
             const int b = nbf(wa, k), c = apf(wa, k);
             if (c == a || c == b) continue;
+            const float zb = zd0 * P.qa(b, 0) + zd1 * P.qa(b, 1) + zd2 * P.qa(b, 2);
+            const float zc = zd0 * P.qa(c, 0) + zd1 * P.qa(c, 1) + zd2 * P.qa(c, 2);
+            const float zmax = fmaxf(za, fmaxf(zb, zc)), zmin = fminf(za, fminf(zb, zc));
+            if (zmax > lo && zmin <= hi) mask |= 1ull << (3 * a + k);
+        }
+    }
+    while (mask) {
+        const int e = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int a = e / 3, k = e - 3 * a;
+        {
+            const uint32_t wa = P.nbw(a);
+            const float pa0 = P.qa(a, 0), pa1 = P.qa(a, 1), pa2 = P.qa(a, 2);
+            const float qa[3] = {pa0 - O[0], pa1 - O[1], pa2 - O[2]};
+            const float za = zd0 * pa0 + zd1 * pa1 + zd2 * pa2;
+            const int b = nbf(wa, k), c = apf(wa, k);
             const float pb0 = P.qa(b, 0), pb1 = P.qa(b, 1), pb2 = P.qa(b, 2);
             const float pc0 = P.qa(c, 0), pc1 = P.qa(c, 1), pc2 = P.qa(c, 2);
             const float qb[3] = {pb0 - O[0], pb1 - O[1], pb2 - O[2]};
