@@ -152,6 +152,90 @@ def plain_leg(mesh, grid, tf, stream, nfrag, steps=2, warmup=1):
 
 UNEVEN = "zeldovich-uneven64-16^3..96^3-rms0.2..3.0-m"
 
+# refined workload: lattice n, rms cells, seed, grid n, fields, refine level,
+# refine tolerance (cells), infall clumps (seeded centres, radius 10 cells)
+REFINED = {"clumps8-128^3-rms1.5-L2-tol0.1-to-512^3-m,v,vv": (128, 1.5, 0, 512, ("v", "vv"), 2, 0.1, 8)}
+
+
+def build_refined(wname):
+    """Zel'dovich lattice plus seeded spherical infall clumps; velocities are
+    the total displacements (wrapped to the nearest image)."""
+    import numpy as np
+    from pypsi_amd.synth import add_infall_clumps, zeldovich_block
+    n, rms, seed, gn, fields, lvl, tol, nclumps = REFINED[wname]
+    pos0, vel0 = zeldovich_block(n, rms, seed)
+    pos = add_infall_clumps(pos0, n, nclumps=nclumps, seed=100)
+    d = pos - pos0
+    d -= np.round(d)
+    vel = np.ascontiguousarray(vel0 + d)
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    conn = np.empty((n, n, n, 8), dtype=np.int32)
+    for ii in range(2):
+        for jj in range(2):
+            for kk in range(2):
+                conn[..., 4 * ii + 2 * jj + kk] = n * n * ((i + ii) % n) + n * ((j + jj) % n) + ((k + kk) % n)
+    conn = conn.reshape(-1, 8)
+    return pos.reshape(-1, 3), vel.reshape(-1, 3), np.ones(conn.shape[0]), conn, gn, fields, lvl, tol
+
+
+def refined_leg(wname, steps=2, warmup=1, kernel="fast"):
+    """The refined workload on the GPU: on-device level-2 refinement then
+    the column kernel, fields m, v, vv on a 512^3 FP64 grid."""
+    import torch
+    from pypsi_amd import device as D
+    pos, vel, mass, conn, gn, fields, lvl, tol = build_refined(wname)
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    dev = torch.device("cuda:0")
+    tpos, tvel = torch.from_numpy(pos).to(dev), torch.from_numpy(vel).to(dev)
+    tmass, tconn = torch.from_numpy(mass).to(dev), torch.from_numpy(conn).to(dev)
+    tf = {"m": torch.zeros((gn,) * 3, dtype=torch.float64, device=dev)}
+    for f in fields:
+        tf[f] = torch.zeros((gn,) * 3 + ((3,) if len(f) == 1 else (3, 3)), dtype=torch.float64, device=dev)
+    mesh = D.make_mesh(tpos, tvel, tmass, tconn, periodic_box=box)
+    grid = D.make_grid((gn,) * 3, box, tf)
+    opts = D.make_opts(kernel=kernel, refine_max_lvl=lvl, refine_tolerance=tol,
+                       stream=torch.cuda.current_stream().cuda_stream)
+
+    def step():
+        for t in tf.values():
+            t.zero_()
+        return D.voxels(mesh, grid, opts)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kms = []
+    for _ in range(steps):
+        st = step()
+        kms.append(st["kernel_ms"])
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    mass_err = abs(1.0 - float(tf["m"].sum().item()) / float(mass.sum()))
+    rec = load_counts().get(wname, {})
+    nfrag = rec.get("nfrag")
+    src = rec.get("source", "oracle")
+    if nfrag is None:
+        nfrag, src = int(st["nfrag"]), "gpu (count not cached)"
+    return {"workload": wname, "value": nfrag / (ms * 1e-3), "unit": "fragments/s", "ms_per_step": ms,
+            "kernel_ms_avg": sum(kms) / steps, "tets": int(st["ntets"]), "nfrag": nfrag, "nfrag_source": src,
+            "gpu_nfrag": int(st["nfrag"]), "mass_rel_err": mass_err, "kernel": ["auto", "plain", "fast"][st["kernel"]],
+            "steps": steps}
+
+
+def count_fragments(wname):
+    """Fragment count of a refined workload with the FP64 kernel (it keeps
+    the reference's fragments one for one), stored beside the oracle counts."""
+    t0 = time.perf_counter()
+    r = refined_leg(wname, steps=1, warmup=0, kernel="plain")
+    p = os.path.join(ROOT, "profiles", "workload_counts.json")
+    data = load_counts()
+    data[wname] = {"nfrag": r["gpu_nfrag"], "ntets": r["tets"], "source": "FP64 GPU kernel (k_vox_plain)",
+                   "seconds": time.perf_counter() - t0}
+    with open(p, "w") as f:
+        json.dump(data, f, indent=1)
+    print(json.dumps(data[wname]), flush=True)
+
 
 def batched_leg(steps=3, warmup=1):
     """64 meshes (seeds 0..63): lattice n log-uniform in 16..96, rms uniform
@@ -270,8 +354,10 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "plain", "fast"])
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--count-fragments", choices=sorted(REFINED),
+                    help="count a refined workload's fragments with the FP64 kernel into profiles/ and exit")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the plain-kernel, forward+adjoint and batched legs")
+                    help="skip the plain-kernel, forward+adjoint, batched and refined legs")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -286,6 +372,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.count_fragments:
+        count_fragments(args.count_fragments)
+        return
 
     wname = args.workload
     log(f"[bench] building inputs for {wname}")
@@ -367,6 +456,8 @@ def main():
         out["fwd_adjoint"] = fwd_adjoint_leg()
         log("[bench] batched uneven meshes")
         out["batched_uneven"] = batched_leg()
+        log("[bench] refined clumps, level 2, 512^3")
+        out["refined"] = refined_leg(next(iter(REFINED)))
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         try:

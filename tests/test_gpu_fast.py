@@ -134,3 +134,27 @@ def test_fast_c1_count_and_mass(PSI, oracle_mod):
     ref = counts["zeldovich-32^3-rms0.3-to-64^3-m"]["nfrag"]
     assert abs(st["nfrag"] - ref) <= 1e-5 * ref, (st["nfrag"], ref)
     assert abs(tf["m"].sum().item() / conn.shape[0] - 1) < 1e-12
+
+
+def test_refined_clumps_against_oracle(PSI, oracle_mod):
+    """Shell crossing with level-2 refinement (the refined workload's
+    construction at 32^3 -> 128^3): lattice rms 1.5 cells plus 8 seeded
+    infall clumps, tolerance 0.1 cells, fields v, vv."""
+    from oracle import block_connectivity
+    from pypsi_amd.synth import add_infall_clumps, zeldovich_block
+    print("case: refined clumps 32^3 L2 -> 128^3", flush=True)
+    n = 32
+    pos0, vel0 = zeldovich_block(n, 1.5, 0)
+    pos = add_infall_clumps(pos0, n, nclumps=8, seed=100, radius_cells=4.0)
+    d = pos - pos0
+    d -= np.round(d)
+    vel = vel0 + d
+    grid = PSI.Grid(n=(128,) * 3, bounds=BOX, fields=["v", "vv"])
+    mesh = PSI.Mesh(loader="block", pos=pos, vel=vel, periodic_box=BOX)
+    PSI.voxels(grid=grid, mesh=mesh, refine_tolerance=0.1, refine_max_lvl=2, kernel="fast")
+    P = oracle_mod.Problem.block(pos, vel, periodic_box=BOX, n=(128,) * 3, bounds=BOX, fields=["v", "vv"])
+    ref, st = oracle_mod.voxels(P, reftol=0.1, maxlvl=2, threads=-1)
+    print("tets", st["ntets"], "fragments", st["nfrag"])
+    errs = assert_close(grid.fields, ref, TOL32, "refined clumps")
+    print({k: "%.1e/%.1e" % v for k, v in errs.items()})
+    assert abs(grid.fields["m"].sum() / mesh.mass.sum() - 1) < 1e-12
