@@ -391,13 +391,14 @@ PVX_HD bool edge_piece(float zu, float zv, float lo, float hi, float& t0, float&
  * open) about an origin O on the lower plane (on the upper one when there
  * is no lower plane), with no branches on the geometry.  For each fan
  * triangle T = (c, a, b) of P's faces (apexes from face_apexes) the cone
- * over T n slab from O is fanned from G, the mean of the endpoints of T's
- * edge pieces inside the slab (a point of T n slab, so every term is of
- * the fragment's size): one term per boundary piece -- the three clipped
- * edges and T's segments on the two planes.  The cap on the upper plane
- * is fanned from a point of that plane through the same segments; the cap
- * on the lower plane adds nothing (O lies on it).  Six fixed terms per
- * triangle, each weighted 0 or 1.
+ * over T n slab from O is fanned from G, the start of the piece of edge
+ * a -> b inside the slab (else of another edge piece: always a vertex of
+ * T n slab, so every term is of the fragment's size): one term per
+ * boundary piece -- the clipped edges c -> a and b -> c (a -> b passes
+ * through G and adds nothing) and T's segments on the two planes.  The
+ * cap on the upper plane is fanned from a point of that plane through the
+ * same segments; the cap on the lower plane adds nothing (O lies on it).
+ * Five fixed terms per triangle, each weighted 0 or 1.
  */
 template <int ORDER>
 PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi,
@@ -459,33 +460,20 @@ PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo
             const bool eca = edge_piece(zc, za, lo, hi, s0, s1);
             const bool eab = edge_piece(za, zb, lo, hi, u0, u1);
             const bool ebc = edge_piece(zb, zc, lo, hi, v0, v1);
-            float A1[3], B1[3], A2[3], B2[3], A3[3], B3[3], Gp[3];
-            float nG = 0.f;
+            float A1[3], B1[3], A2[3], A3[3], B3[3], Gp[3];
             for (int i = 0; i < 3; ++i) {
                 A1[i] = qc[i] + s0 * (qa[i] - qc[i]);
                 B1[i] = qc[i] + s1 * (qa[i] - qc[i]);
                 A2[i] = qa[i] + u0 * (qb[i] - qa[i]);
-                B2[i] = qa[i] + u1 * (qb[i] - qa[i]);
                 A3[i] = qb[i] + v0 * (qc[i] - qb[i]);
                 B3[i] = qb[i] + v1 * (qc[i] - qb[i]);
-                Gp[i] = (eca ? A1[i] + B1[i] : 0.f) + (eab ? A2[i] + B2[i] : 0.f) + (ebc ? A3[i] + B3[i] : 0.f);
+                /* fan point: where the piece of a -> b starts (so that piece
+                 * adds nothing and is not evaluated), else the start of
+                 * another edge piece; every choice is a vertex of T n slab */
+                Gp[i] = eab ? A2[i] : (eca ? A1[i] : (ebc ? A3[i] : qc[i]));
             }
-            nG = (eca ? 2.f : 0.f) + (eab ? 2.f : 0.f) + (ebc ? 2.f : 0.f);
-            if (nG == 0.f) {
-                /* no edge inside: the slab may still cross the triangle's
-                 * interior; any point of the plane works (its segments
-                 * are the whole boundary) */
-                Gp[0] = qc[0];
-                Gp[1] = qc[1];
-                Gp[2] = qc[2];
-            } else {
-                const float inv = 1.0f / nG;
-                Gp[0] *= inv;
-                Gp[1] *= inv;
-                Gp[2] *= inv;
-            }
+            (void)u1;
             tri_acc_w<ORDER>(M, Gp, B1, A1, eca ? 1.f : 0.f);
-            tri_acc_w<ORDER>(M, Gp, B2, A2, eab ? 1.f : 0.f);
             tri_acc_w<ORDER>(M, Gp, B3, A3, ebc ? 1.f : 0.f);
             if (has_lo) {
                 float X[3], N[3];
