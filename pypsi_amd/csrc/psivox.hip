@@ -329,30 +329,29 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
                                                         const uint64_t* __restrict__ tasks, int64_t ntask, GridInfo g,
                                                         MeshBox mb, double* __restrict__ acc, TaskOut out,
                                                         unsigned* status, unsigned long long* counters) {
-    __shared__ float sq[3][NVF][FBLOCK];
-    __shared__ uint32_t snb[NVF][FBLOCK];
+    __shared__ float sq[3][NVF][FSTRIDE];
+    __shared__ uint32_t snb[NVF][FSTRIDE];
     const int lane = threadIdx.x;
     LPoly P{&sq[0][0][lane], &sq[1][0][lane], &sq[2][0][lane], &snb[0][lane], 0};
     const int64_t c = (int64_t)blockIdx.x * FBLOCK + lane;
-    int64_t nf = 0;
+    TaskTally T;
+    T.reset();
     if (c < ntask) {
         const uint64_t w = tasks[c];
         const double* r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
         AtomicSink sink{acc};
-        double sum;
-        BestFrag B;
-        if (!column_task<ORDER>(sink, g, mb, r, w, P, nf, sum, B)) atomicOr(status, ST_FALLBACK);
-        out.sum[c] = sum;
-        out.key[c] = B.kl >= 0 ? best_key(B.M.m0, B.kl) : 0ull;
-        if (ORDER > 0 && B.kl >= 0) {
+        if (!column_task<ORDER>(sink, g, mb, r, w, P, T)) atomicOr(status, ST_FALLBACK);
+        out.sum[c] = T.sum_m0;
+        out.key[c] = T.B.kl >= 0 ? best_key(T.B.M.m0, T.B.kl) : 0ull;
+        if (ORDER > 0 && T.B.kl >= 0) {
             float4* m = (float4*)(out.mom + 16 * c);
-            m[0] = make_float4(B.M.m0, B.M.s1[0], B.M.s1[1], B.M.s1[2]);
-            m[1] = make_float4(B.M.s2[0], B.M.s2[1], B.M.s2[2], B.M.s2[3]);
-            m[2] = make_float4(B.M.s2[4], B.M.s2[5], B.O[0], B.O[1]);
-            m[3] = make_float4(B.O[2], 0.f, 0.f, 0.f);
+            m[0] = make_float4(T.B.M.m0, T.B.M.s1[0], T.B.M.s1[1], T.B.M.s1[2]);
+            m[1] = make_float4(T.B.M.s2[0], T.B.M.s2[1], T.B.M.s2[2], T.B.M.s2[3]);
+            m[2] = make_float4(T.B.M.s2[4], T.B.M.s2[5], T.B.O[0], T.B.O[1]);
+            m[3] = make_float4(T.B.O[2], 0.f, 0.f, 0.f);
         }
     }
-    block_count(nf, 0, counters);
+    block_count(T.nfrag, 0, counters);
 }
 
 /* one thread per tet of [t0, t1): how its residual is deposited.  A small

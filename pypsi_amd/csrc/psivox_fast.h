@@ -40,9 +40,12 @@ namespace pvx {
 
 constexpr int NVF = 16;   /* vertex slots of the column polyhedron (tet and 4 planes: <= 12) */
 constexpr int FBLOCK = 64; /* lanes per workgroup (one wavefront) */
+/* words between a lane's consecutive vertex slots (each lane reads only its
+ * own slot, so the 64 lanes of a wavefront touch 64 consecutive words) */
+constexpr int FSTRIDE = FBLOCK;
 
 /* A lane's polyhedron: q components and one packed word per vertex at
- * stride FBLOCK (the 64 lanes of a wavefront touch consecutive words).
+ * stride FSTRIDE (the 64 lanes of a wavefront touch consecutive words).
  * Word layout: neighbour k in bits [5k, 5k+5); the fan apex of the face to
  * the left of the directed edge (v -> neighbour k) in bits [15+5k, 20+5k)
  * (filled once per column by face_apexes).  31 marks an unset link. */
@@ -58,19 +61,19 @@ struct LPoly {
     float* q2;
     uint32_t* nb;
     int nv;
-    PVX_HD float qa(int v, int a) const { return (a == 0 ? q0 : (a == 1 ? q1 : q2))[v * FBLOCK]; }
+    PVX_HD float qa(int v, int a) const { return (a == 0 ? q0 : (a == 1 ? q1 : q2))[v * FSTRIDE]; }
     PVX_HD void setq(int v, float a, float b, float c) {
-        q0[v * FBLOCK] = a;
-        q1[v * FBLOCK] = b;
-        q2[v * FBLOCK] = c;
+        q0[v * FSTRIDE] = a;
+        q1[v * FSTRIDE] = b;
+        q2[v * FSTRIDE] = c;
     }
-    PVX_HD int nbr(int v, int k) const { return nbf(nb[v * FBLOCK], k); }
-    PVX_HD uint32_t nbw(int v) const { return nb[v * FBLOCK]; }
-    PVX_HD void setnbw(int v, uint32_t w) { nb[v * FBLOCK] = w; }
+    PVX_HD int nbr(int v, int k) const { return nbf(nb[v * FSTRIDE], k); }
+    PVX_HD uint32_t nbw(int v) const { return nb[v * FSTRIDE]; }
+    PVX_HD void setnbw(int v, uint32_t w) { nb[v * FSTRIDE] = w; }
     PVX_HD void setnbr(int v, int k, int val) {
-        uint32_t w = nb[v * FBLOCK];
+        uint32_t w = nb[v * FSTRIDE];
         w = (w & ~(31u << (5 * k))) | ((uint32_t)val << (5 * k));
-        nb[v * FBLOCK] = w;
+        nb[v * FSTRIDE] = w;
     }
 };
 
@@ -386,48 +389,51 @@ PVX_HD bool edge_piece(float zu, float zv, float lo, float hi, float& t0, float&
     return t0 < t1;
 }
 
+/* One cell's slab of a column: the planes as offsets from the tet's vertex
+ * 0 along the sweep axis, the moment origin O (on the lower plane, on the
+ * upper one when there is no lower plane), the upper cap's anchor
+ * relative to O, and the fan triangles reaching into the slab. */
+struct SlabCell {
+    float lo, hi; /* -3e38 / +3e38 when the side is open */
+    float O[3], gh[3];
+    float zd[3]; /* x_ax(q) - x_ax(q = 0) = zd . q */
+    bool has_lo, has_hi;
+    uint64_t mask; /* bit 3a + k: the fan triangle of directed edge a -> k */
+};
+
 /*
- * Moments of P intersected with the slab lo < x_ax <= hi (either side may be
- * open) about an origin O on the lower plane (on the upper one when there
- * is no lower plane), with no branches on the geometry.  For each fan
- * triangle T = (c, a, b) of P's faces (apexes from face_apexes) the cone
- * over T n slab from O is fanned from G, the start of the piece of edge
- * a -> b inside the slab (else of another edge piece: always a vertex of
- * T n slab, so every term is of the fragment's size): one term per
- * boundary piece -- the clipped edges c -> a and b -> c (a -> b passes
- * through G and adds nothing) and T's segments on the two planes.  The
- * cap on the upper plane is fanned from a point of that plane through the
- * same segments; the cap on the lower plane adds nothing (O lies on it).
- * Five fixed terms per triangle, each weighted 0 or 1.
+ * The slab lo < x_ax <= hi of P (either side may be open): its planes, the
+ * origin and cap anchor on them, and the mask of fan triangles reaching
+ * into it (z_max > lo and z_min <= hi: a triangle lying on the upper plane
+ * belongs to the piece, one on the lower plane not).
  */
-template <int ORDER>
-PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi,
-                           double hi_c, Mom& M, float* O) {
-    M.zero();
-    const int nv = P.nv;
+PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi, double hi_c,
+                       SlabCell& S) {
     const double x0d = pick3(G.x0d, ax);
-    const float lo = has_lo ? (float)(lo_c - x0d) : -3.0e38f, hi = has_hi ? (float)(hi_c - x0d) : 3.0e38f;
+    S.has_lo = has_lo;
+    S.has_hi = has_hi;
+    S.lo = has_lo ? (float)(lo_c - x0d) : -3.0e38f;
+    S.hi = has_hi ? (float)(hi_c - x0d) : 3.0e38f;
     const float zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
                 zd1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]),
                 zd2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
-    /* the origin on a slab plane, the upper cap's anchor */
-    O[0] = P.qa(0, 0);
-    O[1] = P.qa(0, 1);
-    O[2] = P.qa(0, 2);
+    S.zd[0] = zd0;
+    S.zd[1] = zd1;
+    S.zd[2] = zd2;
+    float O[3] = {P.qa(0, 0), P.qa(0, 1), P.qa(0, 2)};
     float Gh[3] = {O[0], O[1], O[2]};
     if (has_lo) {
-        plane_point(P, zd0, zd1, zd2, lo, O);
-        if (has_hi && !plane_point(P, zd0, zd1, zd2, hi, Gh)) Gh[0] = O[0], Gh[1] = O[1], Gh[2] = O[2];
+        plane_point(P, zd0, zd1, zd2, S.lo, O);
+        if (has_hi && !plane_point(P, zd0, zd1, zd2, S.hi, Gh)) Gh[0] = O[0], Gh[1] = O[1], Gh[2] = O[2];
     } else if (has_hi) {
-        plane_point(P, zd0, zd1, zd2, hi, O);
+        plane_point(P, zd0, zd1, zd2, S.hi, O);
     }
-    const float gh[3] = {Gh[0] - O[0], Gh[1] - O[1], Gh[2] - O[2]};
-    const bool cap_hi = has_lo && has_hi;
-    /* the fan triangles (bit 3a + k: directed edge a -> neighbour k) that
-     * reach into the slab: z_max > lo and z_min <= hi (a triangle lying on
-     * the upper plane belongs to the piece, one on the lower plane not) */
+    for (int i = 0; i < 3; ++i) {
+        S.O[i] = O[i];
+        S.gh[i] = Gh[i] - O[i];
+    }
     uint64_t mask = 0;
-    for (int a = 0; a < nv; ++a) {
+    for (int a = 0; a < P.nv; ++a) {
         const uint32_t wa = P.nbw(a);
         const float za = zd0 * P.qa(a, 0) + zd1 * P.qa(a, 1) + zd2 * P.qa(a, 2);
         for (int k = 0; k < 3; ++k) {
@@ -436,57 +442,86 @@ PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo
             const float zb = zd0 * P.qa(b, 0) + zd1 * P.qa(b, 1) + zd2 * P.qa(b, 2);
             const float zc = zd0 * P.qa(c, 0) + zd1 * P.qa(c, 1) + zd2 * P.qa(c, 2);
             const float zmax = fmaxf(za, fmaxf(zb, zc)), zmin = fminf(za, fminf(zb, zc));
-            if (zmax > lo && zmin <= hi) mask |= 1ull << (3 * a + k);
+            if (zmax > S.lo && zmin <= S.hi) mask |= 1ull << (3 * a + k);
         }
     }
+    S.mask = mask;
+}
+
+/*
+ * The moments (about S.O) of one fan triangle T = (c, a, b) of P (directed
+ * edge e = 3a + k) cut to the slab, added to M, with no branches on the
+ * geometry.  The cone over T n slab from O is fanned from G, the start of
+ * the piece of edge a -> b inside the slab (else of another edge piece:
+ * always a vertex of T n slab, so every term is of the fragment's size):
+ * one term per boundary piece -- the clipped edges c -> a and b -> c
+ * (a -> b passes through G and adds nothing) and T's segments on the two
+ * planes.  The cap on the upper plane is fanned from the anchor through the
+ * same segments; the cap on the lower plane adds nothing (O lies on it).
+ * Five fixed terms, each weighted 0 or 1.
+ */
+template <int ORDER>
+PVX_HD void pair_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
+    const int a = e / 3, k = e - 3 * a;
+    const float* O = S.O;
+    const float zd0 = S.zd[0], zd1 = S.zd[1], zd2 = S.zd[2];
+    const float lo = S.lo, hi = S.hi;
+    const uint32_t wa = P.nbw(a);
+    const float pa0 = P.qa(a, 0), pa1 = P.qa(a, 1), pa2 = P.qa(a, 2);
+    const float qa[3] = {pa0 - O[0], pa1 - O[1], pa2 - O[2]};
+    const float za = zd0 * pa0 + zd1 * pa1 + zd2 * pa2;
+    const int b = nbf(wa, k), c = apf(wa, k);
+    const float pb0 = P.qa(b, 0), pb1 = P.qa(b, 1), pb2 = P.qa(b, 2);
+    const float pc0 = P.qa(c, 0), pc1 = P.qa(c, 1), pc2 = P.qa(c, 2);
+    const float qb[3] = {pb0 - O[0], pb1 - O[1], pb2 - O[2]};
+    const float qc[3] = {pc0 - O[0], pc1 - O[1], pc2 - O[2]};
+    const float zb = zd0 * pb0 + zd1 * pb1 + zd2 * pb2;
+    const float zc = zd0 * pc0 + zd1 * pc1 + zd2 * pc2;
+    /* the edge pieces inside the slab, in traversal order */
+    float s0, s1, u0, u1, v0, v1;
+    const bool eca = edge_piece(zc, za, lo, hi, s0, s1);
+    const bool eab = edge_piece(za, zb, lo, hi, u0, u1);
+    const bool ebc = edge_piece(zb, zc, lo, hi, v0, v1);
+    (void)u1;
+    float A1[3], B1[3], A3[3], B3[3], Gp[3];
+    for (int i = 0; i < 3; ++i) {
+        A1[i] = qc[i] + s0 * (qa[i] - qc[i]);
+        B1[i] = qc[i] + s1 * (qa[i] - qc[i]);
+        const float A2 = qa[i] + u0 * (qb[i] - qa[i]);
+        A3[i] = qb[i] + v0 * (qc[i] - qb[i]);
+        B3[i] = qb[i] + v1 * (qc[i] - qb[i]);
+        Gp[i] = eab ? A2 : (eca ? A1[i] : (ebc ? A3[i] : qc[i]));
+    }
+    tri_acc_w<ORDER>(M, Gp, B1, A1, eca ? 1.f : 0.f);
+    tri_acc_w<ORDER>(M, Gp, B3, A3, ebc ? 1.f : 0.f);
+    if (S.has_lo) {
+        float X[3], N[3];
+        const bool v = tri_segment<false>(qc, qa, qb, zc, za, zb, lo, X, N);
+        tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
+    }
+    if (S.has_hi) {
+        float X[3], N[3];
+        const bool v = tri_segment<true>(qc, qa, qb, zc, za, zb, hi, X, N);
+        tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
+        if (S.has_lo) tri_acc_w<ORDER>(M, S.gh, X, N, v ? 1.f : 0.f);
+    }
+}
+
+/* moments of P's piece in the slab, one lane doing every fan triangle */
+template <int ORDER>
+PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi,
+                           double hi_c, Mom& M, float* O) {
+    M.zero();
+    SlabCell S;
+    slab_setup(P, G, ax, has_lo, lo_c, has_hi, hi_c, S);
+    O[0] = S.O[0];
+    O[1] = S.O[1];
+    O[2] = S.O[2];
+    uint64_t mask = S.mask;
     while (mask) {
         const int e = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const int a = e / 3, k = e - 3 * a;
-        {
-            const uint32_t wa = P.nbw(a);
-            const float pa0 = P.qa(a, 0), pa1 = P.qa(a, 1), pa2 = P.qa(a, 2);
-            const float qa[3] = {pa0 - O[0], pa1 - O[1], pa2 - O[2]};
-            const float za = zd0 * pa0 + zd1 * pa1 + zd2 * pa2;
-            const int b = nbf(wa, k), c = apf(wa, k);
-            const float pb0 = P.qa(b, 0), pb1 = P.qa(b, 1), pb2 = P.qa(b, 2);
-            const float pc0 = P.qa(c, 0), pc1 = P.qa(c, 1), pc2 = P.qa(c, 2);
-            const float qb[3] = {pb0 - O[0], pb1 - O[1], pb2 - O[2]};
-            const float qc[3] = {pc0 - O[0], pc1 - O[1], pc2 - O[2]};
-            const float zb = zd0 * pb0 + zd1 * pb1 + zd2 * pb2;
-            const float zc = zd0 * pc0 + zd1 * pc1 + zd2 * pc2;
-            /* the edge pieces inside the slab, in traversal order */
-            float s0, s1, u0, u1, v0, v1;
-            const bool eca = edge_piece(zc, za, lo, hi, s0, s1);
-            const bool eab = edge_piece(za, zb, lo, hi, u0, u1);
-            const bool ebc = edge_piece(zb, zc, lo, hi, v0, v1);
-            float A1[3], B1[3], A2[3], A3[3], B3[3], Gp[3];
-            for (int i = 0; i < 3; ++i) {
-                A1[i] = qc[i] + s0 * (qa[i] - qc[i]);
-                B1[i] = qc[i] + s1 * (qa[i] - qc[i]);
-                A2[i] = qa[i] + u0 * (qb[i] - qa[i]);
-                A3[i] = qb[i] + v0 * (qc[i] - qb[i]);
-                B3[i] = qb[i] + v1 * (qc[i] - qb[i]);
-                /* fan point: where the piece of a -> b starts (so that piece
-                 * adds nothing and is not evaluated), else the start of
-                 * another edge piece; every choice is a vertex of T n slab */
-                Gp[i] = eab ? A2[i] : (eca ? A1[i] : (ebc ? A3[i] : qc[i]));
-            }
-            (void)u1;
-            tri_acc_w<ORDER>(M, Gp, B1, A1, eca ? 1.f : 0.f);
-            tri_acc_w<ORDER>(M, Gp, B3, A3, ebc ? 1.f : 0.f);
-            if (has_lo) {
-                float X[3], N[3];
-                const bool v = tri_segment<false>(qc, qa, qb, zc, za, zb, lo, X, N);
-                tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
-            }
-            if (has_hi) {
-                float X[3], N[3];
-                const bool v = tri_segment<true>(qc, qa, qb, zc, za, zb, hi, X, N);
-                tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
-                if (cap_hi) tri_acc_w<ORDER>(M, gh, X, N, v ? 1.f : 0.f);
-            }
-        }
+        pair_terms<ORDER>(P, e, S, M);
     }
 }
 
@@ -805,15 +840,46 @@ PVX_HD bool build_column(LPoly& P, const CopySetup& C, const GridInfo& g, int i,
     return ok;
 }
 
-/* cell k of a built column: moments (about O) of its z-slab piece */
-template <int ORDER>
-PVX_HD void column_cell(const LPoly& P, const CopySetup& C, int k, int k0, int k1, Mom& Mm, float* O) {
+/* a column task after its column is built: the copy, the column's cell
+ * (i, j) and its z-cell range [k0, k1) */
+struct ColumnState {
+    CopySetup C;
+    int i, j, k0, k1;
+};
+
+/* the planes of cell k of a built column: interior planes between the
+ * column's cells, window planes where the copy crosses them, open ends
+ * elsewhere (src/geometry.c:816-843) */
+PVX_HD void cell_planes(const ColumnState& S, int k, bool& has_lo, double& lo_c, bool& has_hi, double& hi_c) {
+    const CopySetup& C = S.C;
     const int kl = k - C.G.ib0[2];
-    const bool zlo = k > k0 || (k == C.G.ib0[2] && (C.wmask & 16u));
-    const bool zhi = k < k1 - 1 || (k == C.ib1[2] - 1 && (C.wmask & 32u));
-    const double lo_c = k > k0 ? (double)kl : C.wlo[2];
-    const double hi_c = k < k1 - 1 ? (double)(kl + 1) : C.whi[2];
-    slab_moments_u<ORDER>(P, C.G, 2, zlo, lo_c, zhi, hi_c, Mm, O);
+    has_lo = k > S.k0 || (k == C.G.ib0[2] && (C.wmask & 16u));
+    has_hi = k < S.k1 - 1 || (k == C.ib1[2] - 1 && (C.wmask & 32u));
+    lo_c = k > S.k0 ? (double)kl : C.wlo[2];
+    hi_c = k < S.k1 - 1 ? (double)(kl + 1) : C.whi[2];
+}
+
+/* Build the column of a task: 1 with cells [k0, k1), 0 when the column is
+ * empty, -1 on a vertex-budget overflow (the FP64 kernel takes over). */
+PVX_HD int column_begin(const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task, LPoly& P,
+                        ColumnState& S) {
+    int64_t t;
+    int h, il, jl;
+    unpack_task(task, t, h, il, jl);
+    double lo[3], hi[3];
+    aabb((const double (*)[3])rec, 4, lo, hi);
+    uint32_t shp;
+    ghost_shifts_packed(lo, hi, g, mb, shp);
+    if (!copy_setup(rec, lo, hi, shp, h, g, mb, S.C)) return 0;
+    S.i = S.C.G.ib0[0] + il;
+    S.j = S.C.G.ib0[1] + jl;
+    int j0, j1;
+    if (!slab_columns(S.C, S.i, g, j0, j1)) return 0;
+    if (!build_column(P, S.C, g, S.i, S.j, j0, j1)) return -1;
+    if (P.nv <= 0) return 0;
+    face_apexes(P);
+    lpoly_range(P, S.C.G, 2, S.C.G.ib0[2], S.C.G.ib0[2], S.C.ib1[2], g.n[2], S.k0, S.k1);
+    return S.k1 > S.k0 ? 1 : 0;
 }
 
 /* per-tet velocity data of a record */
@@ -844,54 +910,62 @@ PVX_HD uint64_t best_key(float m0, int kl) {
     return ((uint64_t)mb << 32) | (uint64_t)(uint32_t)kl;
 }
 
+/* A task's record of its cells: fragment count, FP64 mass sum and its
+ * largest fragment (for the tet's residual pass). */
+struct TaskTally {
+    int64_t nfrag;
+    double sum_m0;
+    BestFrag B;
+    PVX_HD void reset() {
+        nfrag = 0;
+        sum_m0 = 0.0;
+        B.kl = -1;
+        B.M.m0 = 0.0f;
+    }
+};
+
+/* deposit cell k of a task from its slab moments Mm about O */
+template <int ORDER, class Sink>
+PVX_HD void cell_finish(Sink& sink, const GridInfo& g, const ColumnState& S, const double* rec, const TetVel& V, int k,
+                        const Mom& Mm, const float* O, TaskTally& T) {
+    if (!(Mm.m0 > 0.0f)) return;
+    deposit_f<ORDER>(sink, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0, (const float (*)[3])V.DV, rec[24],
+                     (double)Mm.m0);
+    T.nfrag++;
+    T.sum_m0 += (double)Mm.m0;
+    if (Mm.m0 > T.B.M.m0) {
+        T.B.M = Mm;
+        T.B.O[0] = O[0];
+        T.B.O[1] = O[1];
+        T.B.O[2] = O[2];
+        T.B.kl = k - S.C.G.ib0[2];
+    }
+}
+
 /*
- * One column task: build the column, integrate and deposit its cells.
- * Returns false on overflow (the caller falls back to the FP64 kernel).
- * sum_m0 gets the FP64 sum of the task's fragment masses and B its largest
- * fragment, for the tet's residual pass.
+ * One column task done by one lane: build the column, integrate and
+ * deposit its cells.  Returns false on overflow (the caller falls back to
+ * the FP64 kernel).  T gets the task's tally for the residual pass.
  */
 template <int ORDER, class Sink>
 PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task, LPoly& P,
-                        int64_t& nfrag, double& sum_m0, BestFrag& B) {
-    int64_t t;
-    int h, il, jl;
-    unpack_task(task, t, h, il, jl);
-    sum_m0 = 0.0;
-    B.kl = -1;
-    B.M.m0 = 0.0f;
-    double lo[3], hi[3];
-    aabb((const double (*)[3])rec, 4, lo, hi);
-    uint32_t shp;
-    ghost_shifts_packed(lo, hi, g, mb, shp);
-    CopySetup C;
-    if (!copy_setup(rec, lo, hi, shp, h, g, mb, C)) return true;
-    const int i = C.G.ib0[0] + il, j = C.G.ib0[1] + jl;
-    int j0, j1;
-    if (!slab_columns(C, i, g, j0, j1)) return true;
-    if (!build_column(P, C, g, i, j, j0, j1)) return false;
-    if (P.nv <= 0) return true;
-    face_apexes(P);
-    int k0, k1;
-    lpoly_range(P, C.G, 2, C.G.ib0[2], C.G.ib0[2], C.ib1[2], g.n[2], k0, k1);
+                        TaskTally& T) {
+    T.reset();
+    ColumnState S;
+    const int st = column_begin(g, mb, rec, task, P, S);
+    if (st < 0) return false;
+    if (st == 0) return true;
     TetVel V;
     if (ORDER > 0) tet_vel(rec, V);
-    const double mass = rec[24];
 #pragma unroll 1
-    for (int k = k0; k < k1; ++k) {
+    for (int k = S.k0; k < S.k1; ++k) {
+        bool hl, hh;
+        double lc, hc;
+        cell_planes(S, k, hl, lc, hh, hc);
         Mom Mm;
         float O[3];
-        column_cell<ORDER>(P, C, k, k0, k1, Mm, O);
-        if (!(Mm.m0 > 0.0f)) continue;
-        deposit_f<ORDER>(sink, g, C.G, C.G.ib0, i, j, k, Mm, O, V.u0, (const float (*)[3])V.DV, mass, (double)Mm.m0);
-        nfrag++;
-        sum_m0 += (double)Mm.m0;
-        if (Mm.m0 > B.M.m0) {
-            B.M = Mm;
-            B.O[0] = O[0];
-            B.O[1] = O[1];
-            B.O[2] = O[2];
-            B.kl = k - C.G.ib0[2];
-        }
+        slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O);
+        cell_finish<ORDER>(sink, g, S, rec, V, k, Mm, O, T);
     }
     return true;
 }

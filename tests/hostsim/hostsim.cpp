@@ -63,9 +63,9 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
     if (vpe == 4) maxlvl = 0;
     RefineScratch* RS = new RefineScratch;
     Poly64* pool = new Poly64[2];
-    std::vector<float> lq(3 * NVF * FBLOCK);
-    std::vector<uint32_t> lnb(NVF * FBLOCK);
-    LPoly P{&lq[0], &lq[NVF * FBLOCK], &lq[2 * NVF * FBLOCK], &lnb[0], 0};
+    std::vector<float> lq(3 * NVF * FSTRIDE);
+    std::vector<uint32_t> lnb(NVF * FSTRIDE);
+    LPoly P{&lq[0], &lq[NVF * FSTRIDE], &lq[2 * NVF * FSTRIDE], &lnb[0], 0};
     HostSink sink{acc.data()};
     std::vector<double> tets;
     std::vector<uint64_t> tasks;
@@ -94,13 +94,17 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
             double sum = 0.0;
             for (int64_t c = 0; c < ncol; ++c) {
                 bool ok;
+                TaskTally T;
                 if (g.order == 0)
-                    ok = column_task<0>(sink, g, mb, r, tasks[c], P, nfrag, tsum[c], tbest[c]);
+                    ok = column_task<0>(sink, g, mb, r, tasks[c], P, T);
                 else if (g.order == 1)
-                    ok = column_task<1>(sink, g, mb, r, tasks[c], P, nfrag, tsum[c], tbest[c]);
+                    ok = column_task<1>(sink, g, mb, r, tasks[c], P, T);
                 else
-                    ok = column_task<2>(sink, g, mb, r, tasks[c], P, nfrag, tsum[c], tbest[c]);
+                    ok = column_task<2>(sink, g, mb, r, tasks[c], P, T);
                 if (!ok) fails++;
+                nfrag += T.nfrag;
+                tsum[c] = T.sum_m0;
+                tbest[c] = T.B;
                 sum += tsum[c];
             }
             /* the largest fragment (first task on ties, as the GPU's key max) */
