@@ -320,31 +320,100 @@ struct TaskOut {
     float* mom; /* 16 floats per task: m0, s1[3], s2[6], O[3], unused */
 };
 
-/* one lane per column task: FP32 column clip in this lane's LDS slice,
- * cell moments, FP64 deposit.  One mesh per launch: its geometry stays in
- * kernel arguments (a batched call launches it once per mesh over that
- * mesh's contiguous task range). */
+/* Tasks per ordering window and key classes.  A lane's work grows with its
+ * column's cell count, and a wavefront runs as long as its longest lane; the
+ * column kernel takes its tasks through perm, which orders each window of
+ * ORDW consecutive tasks by cell count, largest first (windows keep
+ * neighbouring tets together for the deposit's cache locality). */
+constexpr int ORDW = 1024;
+constexpr int ORDK = 32;
+static_assert(ORDW % 256 == 0, "whole threads per window");
+
+/* perm[w0 .. w0 + ORDW) of each window: the window's task indices, largest
+ * cell count first (order within a class is arbitrary).  Also clears the
+ * column kernel's wavefront counter. */
+__global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
+                                                   int64_t ntask, GridInfo g, MeshBox mb, uint32_t* __restrict__ perm,
+                                                   unsigned* __restrict__ next) {
+    __shared__ int cnt[ORDK];
+    const int64_t w0 = (int64_t)blockIdx.x * ORDW;
+    const int nw = (int)(ntask - w0 < ORDW ? ntask - w0 : ORDW);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next = 0u;
+    if (threadIdx.x < ORDK) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int key[ORDW / 256];
+#pragma unroll
+    for (int q = 0; q < ORDW / 256; ++q) {
+        const int e = (int)threadIdx.x + 256 * q;
+        key[q] = -1;
+        if (e >= nw) continue;
+        const uint64_t w = tasks[w0 + e];
+        int64_t t;
+        int h, il, jl;
+        unpack_task(w, t, h, il, jl);
+        const double* r = tets + (int64_t)TREC * t;
+        double lo[3], hi[3];
+        aabb((const double (*)[3])r, 4, lo, hi);
+        uint32_t shp;
+        ghost_shifts_packed(lo, hi, g, mb, shp);
+        CopySetup C;
+        int nc = 0;
+        if (copy_setup(r, lo, hi, shp, h, g, mb, C)) nc = column_cells(C, il, jl);
+        key[q] = ORDK - 1 - (nc < ORDK - 1 ? nc : ORDK - 1);
+        atomicAdd(&cnt[key[q]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int sum = 0;
+        for (int k = 0; k < ORDK; ++k) {
+            const int c = cnt[k];
+            cnt[k] = sum;
+            sum += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < ORDW / 256; ++q) {
+        if (key[q] < 0) continue;
+        const int pos = atomicAdd(&cnt[key[q]], 1);
+        perm[w0 + pos] = (uint32_t)(w0 + threadIdx.x + 256 * q);
+    }
+}
+
+/* One lane per column task: FP32 column clip in this lane's LDS slice,
+ * cell moments, FP64 deposit.  A workgroup takes the next wavefront of
+ * tasks (perm order) from the counter when it starts: the hardware places
+ * workgroups on XCDs and CUs in a fixed rotation, and with wavefronts of
+ * unequal cost a fixed mapping would load some CUs far more than others.
+ * One mesh per launch: its geometry stays in kernel arguments (a batched
+ * call launches it once per mesh over that mesh's contiguous task range). */
 template <int ORDER>
 __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
-                                                        const uint64_t* __restrict__ tasks, int64_t ntask, GridInfo g,
-                                                        MeshBox mb, double* __restrict__ acc, TaskOut out,
-                                                        unsigned* status, unsigned long long* counters) {
+                                                        const uint64_t* __restrict__ tasks,
+                                                        const uint32_t* __restrict__ perm, unsigned* __restrict__ next,
+                                                        int64_t ntask, GridInfo g, MeshBox mb,
+                                                        double* __restrict__ acc, TaskOut out, unsigned* status,
+                                                        unsigned long long* counters) {
     __shared__ float sq[3][NVF][FSTRIDE];
     __shared__ uint32_t snb[NVF][FSTRIDE];
+    __shared__ unsigned swave;
     const int lane = threadIdx.x;
     LPoly P{&sq[0][0][lane], &sq[1][0][lane], &sq[2][0][lane], &snb[0][lane], 0};
-    const int64_t c = (int64_t)blockIdx.x * FBLOCK + lane;
+    if (lane == 0) swave = atomicAdd(next, 1u);
+    __syncthreads();
+    const int64_t c = (int64_t)swave * FBLOCK + lane;
     TaskTally T;
     T.reset();
     if (c < ntask) {
-        const uint64_t w = tasks[c];
+        const int64_t p = perm[c];
+        const uint64_t w = tasks[p];
         const double* r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
         AtomicSink sink{acc};
         if (!column_task<ORDER>(sink, g, mb, r, w, P, T)) atomicOr(status, ST_FALLBACK);
-        out.sum[c] = T.sum_m0;
-        out.key[c] = T.B.kl >= 0 ? best_key(T.B.M.m0, T.B.kl) : 0ull;
+        out.sum[p] = T.sum_m0;
+        out.key[p] = T.B.kl >= 0 ? best_key(T.B.M.m0, T.B.kl) : 0ull;
         if (ORDER > 0 && T.B.kl >= 0) {
-            float4* m = (float4*)(out.mom + 16 * c);
+            float4* m = (float4*)(out.mom + 16 * p);
             m[0] = make_float4(T.B.M.m0, T.B.M.s1[0], T.B.M.s1[1], T.B.M.s1[2]);
             m[1] = make_float4(T.B.M.s2[0], T.B.M.s2[1], T.B.M.s2[2], T.B.M.s2[3]);
             m[2] = make_float4(T.B.M.s2[4], T.B.M.s2[5], T.B.O[0], T.B.O[1]);
@@ -445,16 +514,19 @@ struct PassGeo {
     unsigned long long* mcnt;
 };
 
-/* the column kernel over tasks [0, ntask) of one mesh */
+/* the column kernel over tasks [0, ntask) of one mesh (perm: ntask
+ * entries of scratch for the task order; next: one counter) */
 static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask, const GridInfo& g, const MeshBox& mb,
-                       double* acc, TaskOut out, unsigned* status, unsigned long long* counters, hipStream_t s) {
+                       double* acc, TaskOut out, uint32_t* perm, unsigned* next, unsigned* status,
+                       unsigned long long* counters, hipStream_t s) {
+    k_col_order<<<(unsigned)((ntask + ORDW - 1) / ORDW), 256, 0, s>>>(tets, tasks, ntask, g, mb, perm, next);
     const unsigned nb = (unsigned)((ntask + FBLOCK - 1) / FBLOCK);
     if (g.order == 0)
-        k_vox_cols<0><<<nb, FBLOCK, 0, s>>>(tets, tasks, ntask, g, mb, acc, out, status, counters);
+        k_vox_cols<0><<<nb, FBLOCK, 0, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
     else if (g.order == 1)
-        k_vox_cols<1><<<nb, FBLOCK, 0, s>>>(tets, tasks, ntask, g, mb, acc, out, status, counters);
+        k_vox_cols<1><<<nb, FBLOCK, 0, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
     else
-        k_vox_cols<2><<<nb, FBLOCK, 0, s>>>(tets, tasks, ntask, g, mb, acc, out, status, counters);
+        k_vox_cols<2><<<nb, FBLOCK, 0, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
     return (int)hipGetLastError();
 }
 
@@ -587,7 +659,7 @@ struct DevBuf {
 
 struct Workspace {
     DevBuf pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
-        colsum, colkey, colmom, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt, bidx, bgat;
+        colsum, colkey, colmom, perm, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt, bidx, bgat;
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -671,6 +743,10 @@ int mesh_to_device(Workspace& W, const psivox_mesh* mesh, int dev, hipStream_t s
  * fn(a0, a1, colstart, base, ntask_chunk).  Returns PSIVOX_E_* or 0; *fell
  * is set (and nothing run) when a tet cannot take the column path.
  */
+/* the column kernel's wavefront counter: the word after a chunk's nsub
+ * task-order entries */
+static unsigned* perm_counter(const DevBuf& b, int64_t nsub) { return (unsigned*)b.p + nsub; }
+
 template <class Fn>
 int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo& G, unsigned* status,
                     unsigned long long* counters, hipStream_t s, int dev, bool* fell, Fn fn) {
@@ -875,9 +951,11 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                         if ((rc2 = W.colsum.get((size_t)nsub * sizeof(double), dev))) return rc2;
                         if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
                         if (g.order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev))) return rc2;
+                        if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
                         TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
                         HIPCHK(hipEventRecord(W.ev0, s));
-                        rc2 = launch_cols(tets, tasks, nsub, g, mb, acc, out, status, counters, s);
+                        rc2 = launch_cols(tets, tasks, nsub, g, mb, acc, out, (uint32_t*)W.perm.p,
+                                          perm_counter(W.perm, nsub), status, counters, s);
                         if (rc2)
                             return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
                                            hipGetErrorString((hipError_t)rc2));
@@ -1247,14 +1325,15 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                 if ((rc2 = W.colsum.get((size_t)nsub * sizeof(double), dev))) return rc2;
                 if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
                 if (G[i0].order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev))) return rc2;
+                if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
                 TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
                 HIPCHK(hipEventRecord(W.ev0, s));
                 for (int q = 0; q < nm; ++q) {
                     const int64_t c0 = hv[q] - base, c1 = hv[q + 1] - base;
                     if (c1 <= c0) continue;
                     TaskOut oq{out.sum + c0, out.key + c0, out.mom ? out.mom + 16 * c0 : nullptr};
-                    rc2 = launch_cols(tets, tasks + c0, c1 - c0, G[i0 + q], MB[i0 + q], bacc + acc_off[q], oq, status,
-                                      mcnt + 2 * q, s);
+                    rc2 = launch_cols(tets, tasks + c0, c1 - c0, G[i0 + q], MB[i0 + q], bacc + acc_off[q], oq,
+                                      (uint32_t*)W.perm.p + c0, perm_counter(W.perm, nsub), status, mcnt + 2 * q, s);
                     if (rc2)
                         return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
                                        hipGetErrorString((hipError_t)rc2));

@@ -803,6 +803,94 @@ PVX_HD int64_t tet_columns(const double* rec, int64_t t, const GridInfo& g, cons
     return n < TASK_MAXPERTET ? n : -1;
 }
 
+/*
+ * z-extent of the tet X (cell-local vertices) inside the column xl <= x <=
+ * xh, yl <= y <= yh, without building it: a linear function over the
+ * convex intersection takes its extremes at vertices of it -- tet vertices
+ * inside the column, tet edges crossing a side plane of the column, and
+ * the column's vertical edges where they pass through the tet (the interval
+ * on which all four barycentric coordinates stay non-negative).  False when
+ * the intersection is empty.
+ */
+PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float yh, float& zmin, float& zmax) {
+    float lo = 3.0e38f, hi = -3.0e38f;
+    auto take = [&](float x, float y, float z) {
+        if (x >= xl && x <= xh && y >= yl && y <= yh) {
+            lo = fminf(lo, z);
+            hi = fmaxf(hi, z);
+        }
+    };
+    for (int v = 0; v < 4; ++v) take(X[v][0], X[v][1], X[v][2]);
+    for (int u = 0; u < 3; ++u)
+        for (int v = u + 1; v < 4; ++v)
+            for (int s = 0; s < 4; ++s) {
+                const int ax = s >> 1;
+                const float c = s == 0 ? xl : (s == 1 ? xh : (s == 2 ? yl : yh));
+                const float du = X[u][ax] - c, dv = X[v][ax] - c;
+                if ((du < 0.f) == (dv < 0.f)) continue;
+                const float t = du / (du - dv);
+                const float px = ax == 0 ? c : X[u][0] + t * (X[v][0] - X[u][0]);
+                const float py = ax == 1 ? c : X[u][1] + t * (X[v][1] - X[u][1]);
+                take(px, py, X[u][2] + t * (X[v][2] - X[u][2]));
+            }
+    /* barycentrics l_1..3 = Inv (p - X0), l_0 = 1 - sum; Inv = adj(D) / det */
+    float D[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) D[a][b] = X[b + 1][a] - X[0][a];
+    const float c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1], c01 = D[0][2] * D[2][1] - D[0][1] * D[2][2],
+                c02 = D[0][1] * D[1][2] - D[0][2] * D[1][1];
+    const float det = D[0][0] * c00 + D[1][0] * c01 + D[2][0] * c02;
+    if (det != 0.f) {
+        const float id = 1.0f / det;
+        float inv[3][3];
+        inv[0][0] = c00 * id;
+        inv[0][1] = c01 * id;
+        inv[0][2] = c02 * id;
+        inv[1][0] = (D[1][2] * D[2][0] - D[1][0] * D[2][2]) * id;
+        inv[1][1] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * id;
+        inv[1][2] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * id;
+        inv[2][0] = (D[1][0] * D[2][1] - D[1][1] * D[2][0]) * id;
+        inv[2][1] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) * id;
+        inv[2][2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
+        const float b1 = inv[0][2], b2 = inv[1][2], b3 = inv[2][2], b0 = -(b1 + b2 + b3);
+        for (int q = 0; q < 4; ++q) {
+            const float rx = ((q & 1) ? xh : xl) - X[0][0], ry = ((q & 2) ? yh : yl) - X[0][1], rz = -X[0][2];
+            const float a1 = inv[0][0] * rx + inv[0][1] * ry + b1 * rz, a2 = inv[1][0] * rx + inv[1][1] * ry + b2 * rz,
+                        a3 = inv[2][0] * rx + inv[2][1] * ry + b3 * rz, a0 = 1.f - (a1 + a2 + a3);
+            const float av[4] = {a0, a1, a2, a3}, bv[4] = {b0, b1, b2, b3};
+            float zl = -3.0e38f, zh = 3.0e38f;
+            for (int i = 0; i < 4; ++i) {
+                if (bv[i] > 0.f)
+                    zl = fmaxf(zl, -av[i] / bv[i]);
+                else if (bv[i] < 0.f)
+                    zh = fminf(zh, -av[i] / bv[i]);
+                else if (av[i] < 0.f)
+                    zl = 3.0e38f;
+            }
+            if (zl <= zh) {
+                lo = fminf(lo, zl);
+                hi = fmaxf(hi, zh);
+            }
+        }
+    }
+    zmin = lo;
+    zmax = hi;
+    return lo <= hi;
+}
+
+/* Number of z cells of a column task (copy C, column (il, jl) relative to
+ * the copy's index box), from column_zext: the sort key that groups tasks of
+ * equal work into the same wavefront. */
+PVX_HD int column_cells(const CopySetup& C, int il, int jl) {
+    float zmin, zmax;
+    if (!column_zext(C.X, (float)il, (float)(il + 1), (float)jl, (float)(jl + 1), zmin, zmax)) return 0;
+    const int nk = C.ib1[2] - C.G.ib0[2];
+    int k0 = (int)floorf(zmin), k1 = (int)ceilf(zmax);
+    if (k0 < 0) k0 = 0;
+    if (k1 > nk) k1 = nk;
+    return k1 > k0 ? k1 - k0 : 1;
+}
+
 /* Build the column polyhedron of task (copy C, cell i, j) in P: the tet
  * clipped by the column's four planes -- interior grid planes or, on the
  * copy's first/last slab or column, the window planes it crosses
