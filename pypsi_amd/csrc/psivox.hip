@@ -321,30 +321,32 @@ struct TaskOut {
 };
 
 /* Tasks per ordering window and key classes.  A lane's work grows with its
- * column's cell count, and a wavefront runs as long as its longest lane; the
- * column kernel takes its tasks through perm, which orders each window of
- * ORDW consecutive tasks by cell count, largest first (windows keep
- * neighbouring tets together for the deposit's cache locality). */
+ * column's cell count and vertex count, and a wavefront runs as long as its
+ * longest lane; the column kernel takes its tasks through perm, which orders
+ * each window of ORDW consecutive tasks by (cells, vertices), largest first
+ * (windows keep neighbouring tets together for the deposit's cache
+ * locality).  Both counts are clamped to 15: 256 classes. */
 constexpr int ORDW = 1024;
-constexpr int ORDK = 32;
+constexpr int ORDK = 256;
 static_assert(ORDW % 256 == 0, "whole threads per window");
 
 /* perm[w0 .. w0 + ORDW) of each window: the window's task indices, largest
- * cell count first (order within a class is arbitrary).  Also clears the
- * column kernel's wavefront counter. */
+ * class first (order within a class is arbitrary).  Also clears the column
+ * kernel's wavefront counter.  256 threads: one class each in the scan. */
 __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
                                                    int64_t ntask, GridInfo g, MeshBox mb, uint32_t* __restrict__ perm,
                                                    unsigned* __restrict__ next) {
     __shared__ int cnt[ORDK];
     const int64_t w0 = (int64_t)blockIdx.x * ORDW;
     const int nw = (int)(ntask - w0 < ORDW ? ntask - w0 : ORDW);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *next = 0u;
-    if (threadIdx.x < ORDK) cnt[threadIdx.x] = 0;
+    const int tid = (int)threadIdx.x;
+    if (blockIdx.x == 0 && tid == 0) *next = 0u;
+    cnt[tid] = 0;
     __syncthreads();
     int key[ORDW / 256];
 #pragma unroll
     for (int q = 0; q < ORDW / 256; ++q) {
-        const int e = (int)threadIdx.x + 256 * q;
+        const int e = tid + 256 * q;
         key[q] = -1;
         if (e >= nw) continue;
         const uint64_t w = tasks[w0 + e];
@@ -357,34 +359,42 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
         uint32_t shp;
         ghost_shifts_packed(lo, hi, g, mb, shp);
         CopySetup C;
-        int nc = 0;
-        if (copy_setup(r, lo, hi, shp, h, g, mb, C)) nc = column_cells(C, il, jl);
-        key[q] = ORDK - 1 - (nc < ORDK - 1 ? nc : ORDK - 1);
+        int nc = 0, nv = 0;
+        if (copy_setup(r, lo, hi, shp, h, g, mb, C)) nc = column_cells(C, il, jl, &nv);
+        nc = nc < 15 ? nc : 15;
+        nv = nv < 15 ? nv : 15;
+        key[q] = ORDK - 1 - (16 * nc + nv);
         atomicAdd(&cnt[key[q]], 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int sum = 0;
-        for (int k = 0; k < ORDK; ++k) {
-            const int c = cnt[k];
-            cnt[k] = sum;
-            sum += c;
-        }
+    /* exclusive scan of the 256 class counts (Hillis-Steele) */
+    int v = cnt[tid];
+    for (int d = 1; d < ORDK; d <<= 1) {
+        __syncthreads();
+        const int u = tid >= d ? cnt[tid - d] : 0;
+        __syncthreads();
+        v += u;
+        cnt[tid] = v;
     }
+    __syncthreads();
+    const int ex = tid > 0 ? cnt[tid - 1] : 0;
+    __syncthreads();
+    cnt[tid] = ex;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < ORDW / 256; ++q) {
         if (key[q] < 0) continue;
         const int pos = atomicAdd(&cnt[key[q]], 1);
-        perm[w0 + pos] = (uint32_t)(w0 + threadIdx.x + 256 * q);
+        perm[w0 + pos] = (uint32_t)(w0 + tid + 256 * q);
     }
 }
 
 /* One lane per column task: FP32 column clip in this lane's LDS slice,
- * cell moments, FP64 deposit.  A workgroup takes the next wavefront of
- * tasks (perm order) from the counter when it starts: the hardware places
- * workgroups on XCDs and CUs in a fixed rotation, and with wavefronts of
- * unequal cost a fixed mapping would load some CUs far more than others.
+ * cell moments, FP64 deposit.  The grid is as many workgroups as the chip
+ * holds at once; each takes wavefronts of tasks (perm order) from the
+ * counter until it runs past the end.  The hardware places workgroups on
+ * XCDs and CUs in a fixed rotation, and the XCDs do not run at quite the
+ * same speed: with a fixed mapping the slowest XCD would set the time.
  * One mesh per launch: its geometry stays in kernel arguments (a batched
  * call launches it once per mesh over that mesh's contiguous task range). */
 template <int ORDER>
@@ -399,28 +409,34 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
     __shared__ unsigned swave;
     const int lane = threadIdx.x;
     LPoly P{&sq[0][0][lane], &sq[1][0][lane], &sq[2][0][lane], &snb[0][lane], 0};
-    if (lane == 0) swave = atomicAdd(next, 1u);
-    __syncthreads();
-    const int64_t c = (int64_t)swave * FBLOCK + lane;
-    TaskTally T;
-    T.reset();
-    if (c < ntask) {
-        const int64_t p = perm[c];
-        const uint64_t w = tasks[p];
-        const double* r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
-        AtomicSink sink{acc};
-        if (!column_task<ORDER>(sink, g, mb, r, w, P, T)) atomicOr(status, ST_FALLBACK);
-        out.sum[p] = T.sum_m0;
-        out.key[p] = T.B.kl >= 0 ? best_key(T.B.M.m0, T.B.kl) : 0ull;
-        if (ORDER > 0 && T.B.kl >= 0) {
-            float4* m = (float4*)(out.mom + 16 * p);
-            m[0] = make_float4(T.B.M.m0, T.B.M.s1[0], T.B.M.s1[1], T.B.M.s1[2]);
-            m[1] = make_float4(T.B.M.s2[0], T.B.M.s2[1], T.B.M.s2[2], T.B.M.s2[3]);
-            m[2] = make_float4(T.B.M.s2[4], T.B.M.s2[5], T.B.O[0], T.B.O[1]);
-            m[3] = make_float4(T.B.O[2], 0.f, 0.f, 0.f);
+    AtomicSink sink{acc};
+    int64_t nfrag = 0;
+    for (;;) {
+        if (lane == 0) swave = atomicAdd(next, 1u);
+        __syncthreads();
+        const int64_t c0 = (int64_t)swave * FBLOCK;
+        __syncthreads(); /* everyone has read swave before lane 0 writes it again */
+        if (c0 >= ntask) break;
+        const int64_t c = c0 + lane;
+        if (c < ntask) {
+            TaskTally T;
+            const int64_t p = perm[c];
+            const uint64_t w = tasks[p];
+            const double* r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
+            if (!column_task<ORDER>(sink, g, mb, r, w, P, T)) atomicOr(status, ST_FALLBACK);
+            nfrag += T.nfrag;
+            out.sum[p] = T.sum_m0;
+            out.key[p] = T.B.kl >= 0 ? best_key(T.B.M.m0, T.B.kl) : 0ull;
+            if (ORDER > 0 && T.B.kl >= 0) {
+                float4* m = (float4*)(out.mom + 16 * p);
+                m[0] = make_float4(T.B.M.m0, T.B.M.s1[0], T.B.M.s1[1], T.B.M.s1[2]);
+                m[1] = make_float4(T.B.M.s2[0], T.B.M.s2[1], T.B.M.s2[2], T.B.M.s2[3]);
+                m[2] = make_float4(T.B.M.s2[4], T.B.M.s2[5], T.B.O[0], T.B.O[1]);
+                m[3] = make_float4(T.B.O[2], 0.f, 0.f, 0.f);
+            }
         }
     }
-    block_count(T.nfrag, 0, counters);
+    block_count(nfrag, 0, counters);
 }
 
 /* one thread per tet of [t0, t1): how its residual is deposited.  A small
@@ -514,13 +530,35 @@ struct PassGeo {
     unsigned long long* mcnt;
 };
 
+/* workgroups of the column kernel the current device holds at once */
+static int64_t resident_blocks(int order) {
+    static thread_local int cached_dev = -1;
+    static thread_local int64_t cached[3] = {0, 0, 0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1 << 20;
+    if (dev != cached_dev) {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+            ncu = 256;
+        int per[3] = {0, 0, 0};
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_vox_cols<0>, FBLOCK, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_vox_cols<1>, FBLOCK, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[2], k_vox_cols<2>, FBLOCK, 0);
+        for (int o = 0; o < 3; ++o) cached[o] = (int64_t)ncu * (per[o] > 0 ? per[o] : 8);
+        cached_dev = dev;
+    }
+    return cached[order < 0 ? 0 : (order > 2 ? 2 : order)];
+}
+
 /* the column kernel over tasks [0, ntask) of one mesh (perm: ntask
  * entries of scratch for the task order; next: one counter) */
 static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask, const GridInfo& g, const MeshBox& mb,
                        double* acc, TaskOut out, uint32_t* perm, unsigned* next, unsigned* status,
                        unsigned long long* counters, hipStream_t s) {
     k_col_order<<<(unsigned)((ntask + ORDW - 1) / ORDW), 256, 0, s>>>(tets, tasks, ntask, g, mb, perm, next);
-    const unsigned nb = (unsigned)((ntask + FBLOCK - 1) / FBLOCK);
+    const int64_t nwave = (ntask + FBLOCK - 1) / FBLOCK;
+    const int64_t hold = resident_blocks(g.order);
+    const unsigned nb = (unsigned)(nwave < hold ? nwave : hold);
     if (g.order == 0)
         k_vox_cols<0><<<nb, FBLOCK, 0, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
     else if (g.order == 1)

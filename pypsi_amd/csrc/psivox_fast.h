@@ -810,14 +810,18 @@ PVX_HD int64_t tet_columns(const double* rec, int64_t t, const GridInfo& g, cons
  * inside the column, tet edges crossing a side plane of the column, and
  * the column's vertical edges where they pass through the tet (the interval
  * on which all four barycentric coordinates stay non-negative).  False when
- * the intersection is empty.
+ * the intersection is empty.  nvert (optional) gets the number of those
+ * vertices (a point on several planes counts more than once).
  */
-PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float yh, float& zmin, float& zmax) {
+PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float yh, float& zmin, float& zmax,
+                        int* nvert = nullptr) {
     float lo = 3.0e38f, hi = -3.0e38f;
+    int nvx = 0;
     auto take = [&](float x, float y, float z) {
         if (x >= xl && x <= xh && y >= yl && y <= yh) {
             lo = fminf(lo, z);
             hi = fmaxf(hi, z);
+            nvx++;
         }
     };
     for (int v = 0; v < 4; ++v) take(X[v][0], X[v][1], X[v][2]);
@@ -870,20 +874,23 @@ PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float
             if (zl <= zh) {
                 lo = fminf(lo, zl);
                 hi = fmaxf(hi, zh);
+                nvx += 2;
             }
         }
     }
     zmin = lo;
     zmax = hi;
+    if (nvert) *nvert = nvx;
     return lo <= hi;
 }
 
 /* Number of z cells of a column task (copy C, column (il, jl) relative to
- * the copy's index box), from column_zext: the sort key that groups tasks of
- * equal work into the same wavefront. */
-PVX_HD int column_cells(const CopySetup& C, int il, int jl) {
+ * the copy's index box) and (nvert) the column polyhedron's vertex count,
+ * from column_zext: the sort key that groups tasks of equal work into the
+ * same wavefront. */
+PVX_HD int column_cells(const CopySetup& C, int il, int jl, int* nvert = nullptr) {
     float zmin, zmax;
-    if (!column_zext(C.X, (float)il, (float)(il + 1), (float)jl, (float)(jl + 1), zmin, zmax)) return 0;
+    if (!column_zext(C.X, (float)il, (float)(il + 1), (float)jl, (float)(jl + 1), zmin, zmax, nvert)) return 0;
     const int nk = C.ib1[2] - C.G.ib0[2];
     int k0 = (int)floorf(zmin), k1 = (int)ceilf(zmax);
     if (k0 < 0) k0 = 0;
