@@ -404,11 +404,10 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
                                                         int64_t ntask, GridInfo g, MeshBox mb,
                                                         double* __restrict__ acc, TaskOut out, unsigned* status,
                                                         unsigned long long* counters) {
-    __shared__ float sq[3][NVF][FSTRIDE];
-    __shared__ uint32_t snb[NVF][FSTRIDE];
+    __shared__ __align__(16) uint32_t sv[NVF * VSTRIDE];
     __shared__ unsigned swave;
     const int lane = threadIdx.x;
-    LPoly P{&sq[0][0][lane], &sq[1][0][lane], &sq[2][0][lane], &snb[0][lane], 0};
+    LPoly P{(uint32_t*)__builtin_assume_aligned(&sv[4 * lane], 16), 0};
     AtomicSink sink{acc};
     int64_t nfrag = 0;
     for (;;) {

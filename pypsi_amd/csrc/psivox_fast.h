@@ -40,40 +40,49 @@ namespace pvx {
 
 constexpr int NVF = 16;   /* vertex slots of the column polyhedron (tet and 4 planes: <= 12) */
 constexpr int FBLOCK = 64; /* lanes per workgroup (one wavefront) */
-/* words between a lane's consecutive vertex slots (each lane reads only its
- * own slot, so the 64 lanes of a wavefront touch 64 consecutive words) */
-constexpr int FSTRIDE = FBLOCK;
+/* words between a lane's consecutive vertex slots: a vertex is 4 words
+ * (q0, q1, q2 as FP32 bit patterns, then its link word) and the 64 lanes'
+ * copies of one slot are consecutive, so one 16-byte read fetches a vertex
+ * and 8 consecutive lanes cover all 32 banks */
+constexpr int VSTRIDE = 4 * FBLOCK;
 
-/* A lane's polyhedron: q components and one packed word per vertex at
- * stride FSTRIDE (the 64 lanes of a wavefront touch consecutive words).
- * Word layout: neighbour k in bits [5k, 5k+5); the fan apex of the face to
- * the left of the directed edge (v -> neighbour k) in bits [15+5k, 20+5k)
- * (filled once per column by face_apexes).  31 marks an unset link. */
+/* Link word layout: neighbour k in bits [5k, 5k+5); the fan apex of the
+ * face to the left of the directed edge (v -> neighbour k) in bits
+ * [15+5k, 20+5k) (filled once per column by face_apexes).  31 marks an
+ * unset link. */
 constexpr uint32_t NBUNSET = 31u;
 
 PVX_HD int nbf(uint32_t w, int k) { return (int)((w >> (5 * k)) & 31u); }
 PVX_HD int apf(uint32_t w, int k) { return (int)((w >> (15 + 5 * k)) & 31u); }
 PVX_HD uint32_t pack3(int a, int b, int c) { return (uint32_t)a | ((uint32_t)b << 5) | ((uint32_t)c << 10); }
+PVX_HD float u2f(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+PVX_HD uint32_t f2u(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
 
+/* A lane's polyhedron: vertex v at w[v * VSTRIDE .. + 4) */
 struct LPoly {
-    float* q0;
-    float* q1;
-    float* q2;
-    uint32_t* nb;
+    uint32_t* w;
     int nv;
-    PVX_HD float qa(int v, int a) const { return (a == 0 ? q0 : (a == 1 ? q1 : q2))[v * FSTRIDE]; }
+    PVX_HD float qa(int v, int a) const { return u2f(w[v * VSTRIDE + a]); }
     PVX_HD void setq(int v, float a, float b, float c) {
-        q0[v * FSTRIDE] = a;
-        q1[v * FSTRIDE] = b;
-        q2[v * FSTRIDE] = c;
+        w[v * VSTRIDE] = f2u(a);
+        w[v * VSTRIDE + 1] = f2u(b);
+        w[v * VSTRIDE + 2] = f2u(c);
     }
-    PVX_HD int nbr(int v, int k) const { return nbf(nb[v * FSTRIDE], k); }
-    PVX_HD uint32_t nbw(int v) const { return nb[v * FSTRIDE]; }
-    PVX_HD void setnbw(int v, uint32_t w) { nb[v * FSTRIDE] = w; }
+    PVX_HD int nbr(int v, int k) const { return nbf(w[v * VSTRIDE + 3], k); }
+    PVX_HD uint32_t nbw(int v) const { return w[v * VSTRIDE + 3]; }
+    PVX_HD void setnbw(int v, uint32_t x) { w[v * VSTRIDE + 3] = x; }
     PVX_HD void setnbr(int v, int k, int val) {
-        uint32_t w = nb[v * FSTRIDE];
-        w = (w & ~(31u << (5 * k))) | ((uint32_t)val << (5 * k));
-        nb[v * FSTRIDE] = w;
+        uint32_t x = w[v * VSTRIDE + 3];
+        x = (x & ~(31u << (5 * k))) | ((uint32_t)val << (5 * k));
+        w[v * VSTRIDE + 3] = x;
     }
 };
 

@@ -63,9 +63,8 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
     if (vpe == 4) maxlvl = 0;
     RefineScratch* RS = new RefineScratch;
     Poly64* pool = new Poly64[2];
-    std::vector<float> lq(3 * NVF * FSTRIDE);
-    std::vector<uint32_t> lnb(NVF * FSTRIDE);
-    LPoly P{&lq[0], &lq[NVF * FSTRIDE], &lq[2 * NVF * FSTRIDE], &lnb[0], 0};
+    std::vector<uint32_t> lv(NVF * VSTRIDE);
+    LPoly P{&lv[0], 0};
     HostSink sink{acc.data()};
     std::vector<double> tets;
     std::vector<uint64_t> tasks;
