@@ -66,6 +66,37 @@ PVX_HD uint32_t f2u(float f) {
     return u;
 }
 
+/* 1 / x in FP32 with the hardware reciprocal on the GPU (about 1 ulp, where
+ * a correctly rounded division costs some ten instructions; the FP32 path's
+ * tolerance is far coarser), an exact division on the host */
+PVX_HD float frcp(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
+/* Keeps a value in a register from here on.  Values derived from the tet
+ * record are otherwise re-read from memory where the compiler likes (the
+ * record is read-only), also inside the cell loop, and vector memory
+ * operations retire in order: such a load waits for every deposit atomic
+ * issued before it. */
+PVX_HD void pin(float& x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    __asm__ volatile("" : "+v"(x));
+#else
+    (void)x;
+#endif
+}
+PVX_HD void pin(double& x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    __asm__ volatile("" : "+v"(x));
+#else
+    (void)x;
+#endif
+}
+
 /* A lane's polyhedron: vertex v at w[v * VSTRIDE .. + 4) */
 struct LPoly {
     uint32_t* w;
@@ -186,7 +217,7 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int 
             const float c0 = P.qa(vc, 0), c1 = P.qa(vc, 1), c2 = P.qa(vc, 2);
             const float n0 = P.qa(vn, 0), n1 = P.qa(vn, 1), n2 = P.qa(vn, 2);
             const float sc = sdist(G.rec, m, c, off, c0, c1, c2), sn = sdist(G.rec, m, c, off, n0, n1, n2);
-            const float inv = 1.0f / (sc - sn);
+            const float inv = frcp(sc - sn);
             P.setq(tot, inv * (n0 * sc - c0 * sn), inv * (n1 * sc - c1 * sn), inv * (n2 * sc - c2 * sn));
             P.setnbw(tot, pack3(vc, NBUNSET, NBUNSET));
             P.setnbr(vc, k, tot);
@@ -337,7 +368,7 @@ PVX_HD bool plane_point(const LPoly& P, float zd0, float zd1, float zd2, float p
             const float r0 = P.qa(b, 0), r1 = P.qa(b, 1), r2 = P.qa(b, 2);
             const float zb = zd0 * r0 + zd1 * r1 + zd2 * r2;
             if (zv <= p && zb > p) {
-                const float t = (p - zv) / (zb - zv);
+                const float t = (p - zv) * frcp(zb - zv);
                 out[0] = q0 + t * (r0 - q0);
                 out[1] = q1 + t * (r1 - q1);
                 out[2] = q2 + t * (r2 - q2);
@@ -361,9 +392,9 @@ PVX_HD bool tri_segment(const float* qc, const float* qa, const float* qb, float
     const bool ib = UPPER_KEPT_IS_LOWER ? (zb <= p) : (zb > p);
     const bool valid = (ic != ia) || (ia != ib);
     /* crossing parameters of the three edges (only used where they cross) */
-    const float tca = (za != zc) ? (p - zc) / (za - zc) : 0.f;
-    const float tab = (zb != za) ? (p - za) / (zb - za) : 0.f;
-    const float tbc = (zc != zb) ? (p - zb) / (zc - zb) : 0.f;
+    const float tca = (za != zc) ? (p - zc) * frcp(za - zc) : 0.f;
+    const float tab = (zb != za) ? (p - za) * frcp(zb - za) : 0.f;
+    const float tbc = (zc != zb) ? (p - zb) * frcp(zc - zb) : 0.f;
     float Pca[3], Pab[3], Pbc[3];
     for (int i = 0; i < 3; ++i) {
         Pca[i] = qc[i] + tca * (qa[i] - qc[i]);
@@ -386,7 +417,7 @@ PVX_HD bool tri_segment(const float* qc, const float* qa, const float* qb, float
 PVX_HD bool edge_piece(float zu, float zv, float lo, float hi, float& t0, float& t1) {
     const float dz = zv - zu;
     if (dz != 0.f) {
-        const float inv = 1.0f / dz;
+        const float inv = frcp(dz);
         const float tl = (lo - zu) * inv, th = (hi - zu) * inv;
         t0 = fmaxf(0.f, fminf(tl, th));
         t1 = fminf(1.f, fmaxf(tl, th));
@@ -595,7 +626,7 @@ PVX_HD void deposit_f(Sink& sink, const GridInfo& g, const CopyGeo& G, const int
     const int64_t gi = ((int64_t)g.n[1] * g.n[2] * i + (int64_t)g.n[2] * j + k) * g.nacc;
     sink.add(gi, w);
     if (ORDER == 0) return;
-    const float inv = 1.0f / Mm.m0;
+    const float inv = frcp(Mm.m0);
     /* mean q (origin shifted back) and q covariance about the mean */
     const float mu[3] = {0.25f * Mm.s1[0] * inv, 0.25f * Mm.s1[1] * inv, 0.25f * Mm.s1[2] * inv};
     const float qb[3] = {O[0] + mu[0], O[1] + mu[1], O[2] + mu[2]};
@@ -986,10 +1017,11 @@ PVX_HD int column_begin(const GridInfo& g, const MeshBox& mb, const double* rec,
     return S.k1 > S.k0 ? 1 : 0;
 }
 
-/* per-tet velocity data of a record */
+/* per-tet mass and velocity data of a record */
 struct TetVel {
     double u0[3];
     float DV[3][3];
+    double mass;
 };
 
 PVX_HD void tet_vel(const double* rec, TetVel& V) {
@@ -998,6 +1030,7 @@ PVX_HD void tet_vel(const double* rec, TetVel& V) {
         for (int b = 0; b < 3; ++b) V.DV[a][b] = (float)(rec[12 + 3 * (b + 1) + a] - rec[12 + a]);
     }
 }
+
 
 /* the largest fragment of a column task: its moments, origin and cell */
 struct BestFrag {
@@ -1033,7 +1066,7 @@ template <int ORDER, class Sink>
 PVX_HD void cell_finish(Sink& sink, const GridInfo& g, const ColumnState& S, const double* rec, const TetVel& V, int k,
                         const Mom& Mm, const float* O, TaskTally& T) {
     if (!(Mm.m0 > 0.0f)) return;
-    deposit_f<ORDER>(sink, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0, (const float (*)[3])V.DV, rec[24],
+    deposit_f<ORDER>(sink, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0, (const float (*)[3])V.DV, V.mass,
                      (double)Mm.m0);
     T.nfrag++;
     T.sum_m0 += (double)Mm.m0;
@@ -1061,6 +1094,8 @@ PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const 
     if (st == 0) return true;
     TetVel V;
     if (ORDER > 0) tet_vel(rec, V);
+    V.mass = rec[24];
+    pin(V.mass);
 #pragma unroll 1
     for (int k = S.k0; k < S.k1; ++k) {
         bool hl, hh;
