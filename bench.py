@@ -362,7 +362,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              "(the path shards by tets, one mesh slice and one grid copy per GPU, summed at the end)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -2,8 +2,7 @@
  * psivox.h -- C ABI of the MI355X voxelizer behind PSI.voxels().
  *
  * Plain C types and pointers only.  The Python module PSI (pypsi_amd/csrc/
- * psimodule.c) calls these; so can any other host language through its FFI
- * (ctypes / cffi stubs are shown in INTEGRATION.md).
+ * psimodule.c) calls these; so can any other host language through its FFI.
  *
  * Reference interface replaced:
  *   void psi_voxels(psi_grid* grid, psi_mesh* mesh, psi_rtree* rtree,
@@ -47,9 +46,9 @@ extern "C" {
 #define PSIVOX_MODE_ANNIHILATION 1
 
 /* kernels */
-#define PSIVOX_KERNEL_AUTO 0  /* the fastest available (binned FP32 clip) */
+#define PSIVOX_KERNEL_AUTO 0  /* the fastest available (PSIVOX_KERNEL_FAST) */
 #define PSIVOX_KERNEL_PLAIN 1 /* one thread per tet, FP64 clip */
-#define PSIVOX_KERNEL_FAST 2  /* binned, FP32 voxel-local clip, FP64 deposit */
+#define PSIVOX_KERNEL_FAST 2  /* one lane per (tet, column) task, FP32 cell-local clip, FP64 deposit */
 
 /* error codes */
 #define PSIVOX_OK 0
@@ -115,7 +114,9 @@ typedef struct {
     int64_t ntets;      /* tets after refinement */
     int64_t nghosts;    /* (tet, periodic copy) pairs */
     int64_t nfrag;      /* (copy, voxel) fragments deposited, m0 > 0 */
-    double kernel_ms;   /* device time of the voxelization kernels */
+    double kernel_ms;   /* device time of the voxelization kernel: the FP64 kernel, or the
+                           column kernel (task setup, ordering and residual kernels not
+                           included); a batched call reports its whole column pass */
     double total_ms;    /* wall time of the whole call */
     int32_t kernel;     /* kernel that ran */
     int32_t reserved;

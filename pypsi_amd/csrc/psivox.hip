@@ -550,11 +550,13 @@ static int64_t resident_blocks(int order) {
 }
 
 /* the column kernel over tasks [0, ntask) of one mesh (perm: ntask
- * entries of scratch for the task order; next: one counter) */
+ * entries of scratch for the task order; next: one counter); mark, when
+ * given, is recorded between the ordering and the column kernel */
 static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask, const GridInfo& g, const MeshBox& mb,
                        double* acc, TaskOut out, uint32_t* perm, unsigned* next, unsigned* status,
-                       unsigned long long* counters, hipStream_t s) {
+                       unsigned long long* counters, hipStream_t s, hipEvent_t mark = nullptr) {
     k_col_order<<<(unsigned)((ntask + ORDW - 1) / ORDW), 256, 0, s>>>(tets, tasks, ntask, g, mb, perm, next);
+    if (mark) (void)hipEventRecord(mark, s);
     const int64_t nwave = (ntask + FBLOCK - 1) / FBLOCK;
     const int64_t hold = resident_blocks(g.order);
     const unsigned nb = (unsigned)(nwave < hold ? nwave : hold);
@@ -696,7 +698,7 @@ struct DevBuf {
 
 struct Workspace {
     DevBuf pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
-        colsum, colkey, colmom, perm, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt, bidx, bgat;
+        colsum, colkey, colmom, perm, cut, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt, bidx, bgat;
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -780,6 +782,28 @@ int mesh_to_device(Workspace& W, const psivox_mesh* mesh, int dev, hipStream_t s
  * fn(a0, a1, colstart, base, ntask_chunk).  Returns PSIVOX_E_* or 0; *fell
  * is set (and nothing run) when a tet cannot take the column path.
  */
+/* The end a1 of the task chunk that starts at tet a0: the largest a1 <= nt
+ * with colstart[a1] - colstart[a0] <= cap (a single tet has fewer than
+ * TASK_MAXPERTET < cap tasks, so a1 > a0).  out = {a1, colstart[a0],
+ * colstart[a1]}.  One thread: a binary search over the nondecreasing
+ * colstart. */
+__global__ void k_chunk_cut(const int64_t* __restrict__ colstart, int64_t nt, int64_t a0, int64_t cap,
+                            int64_t* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int64_t lim = colstart[a0] + cap;
+    int64_t lo = a0 + 1, hi = nt;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo + 1) / 2;
+        if (colstart[mid] <= lim)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    out[0] = lo;
+    out[1] = colstart[a0];
+    out[2] = colstart[lo];
+}
+
 /* the column kernel's wavefront counter: the word after a chunk's nsub
  * task-order entries */
 static unsigned* perm_counter(const DevBuf& b, int64_t nsub) { return (unsigned*)b.p + nsub; }
@@ -815,28 +839,27 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
         return 0;
     }
     if (ntask == 0) return 0;
-    /* tet sub-ranges whose tasks fit the task buffers */
+    /* tet sub-ranges whose tasks fit the task buffers, cut on the device
+     * (colstart stays there: for a large batch it is hundreds of MB) */
     const int64_t cap = (int64_t)1 << 28;
-    std::vector<int64_t> cuts{0, nt};
-    std::vector<int64_t> hcs;
-    if (ntask > cap) {
-        hcs.resize(nt + 1);
-        HIPCHK(hipMemcpy(hcs.data(), colstart, (nt + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
-        cuts.assign(1, 0);
-        while (cuts.back() < nt) {
-            const int64_t a0 = cuts.back();
-            int64_t a1 = std::upper_bound(hcs.begin() + a0 + 1, hcs.end(), hcs[a0] + cap) - hcs.begin() - 1;
-            if (a1 <= a0) a1 = a0 + 1; /* a single tet has fewer than TASK_MAXPERTET < cap tasks */
-            cuts.push_back(a1);
-        }
-    }
     const int64_t tcap = std::min(ntask, cap);
     if ((rc = W.tasks.get((size_t)tcap * sizeof(uint64_t), dev))) return rc;
+    if ((rc = W.cut.get(4 * sizeof(int64_t), dev))) return rc;
     uint64_t* tasks = (uint64_t*)W.tasks.p;
-    for (size_t q = 0; q + 1 < cuts.size(); ++q) {
-        const int64_t a0 = cuts[q], a1 = cuts[q + 1];
-        const int64_t base = hcs.empty() ? 0 : hcs[a0];
-        const int64_t nsub = hcs.empty() ? ntask : hcs[a1] - hcs[a0];
+    int64_t a1 = 0;
+    for (int64_t a0 = 0; a0 < nt; a0 = a1) {
+        int64_t base = 0, nsub = ntask;
+        a1 = nt;
+        if (ntask > cap) {
+            int64_t hc[3];
+            k_chunk_cut<<<1, 64, 0, s>>>(colstart, nt, a0, cap, (int64_t*)W.cut.p);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hc, W.cut.p, sizeof hc, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            a1 = hc[0];
+            base = hc[1];
+            nsub = hc[2] - hc[1];
+        }
         if (nsub == 0) continue;
         if (G.table)
             k_col_emit<true><<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, G.g, G.mb, G.table, colstart,
@@ -990,9 +1013,8 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                         if (g.order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev))) return rc2;
                         if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
                         TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
-                        HIPCHK(hipEventRecord(W.ev0, s));
                         rc2 = launch_cols(tets, tasks, nsub, g, mb, acc, out, (uint32_t*)W.perm.p,
-                                          perm_counter(W.perm, nsub), status, counters, s);
+                                          perm_counter(W.perm, nsub), status, counters, s, W.ev0);
                         if (rc2)
                             return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
                                            hipGetErrorString((hipError_t)rc2));
