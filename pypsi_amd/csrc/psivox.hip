@@ -397,6 +397,18 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
  * same speed: with a fixed mapping the slowest XCD would set the time.
  * One mesh per launch: its geometry stays in kernel arguments (a batched
  * call launches it once per mesh over that mesh's contiguous task range). */
+/* A fragment's deposit staged in LDS: add(i, v) puts v at row[i - base]
+ * (deposit_f adds each of the cell's nacc slots exactly once). */
+struct StageSink {
+    double* row;
+    int64_t base;
+    __device__ void add(int64_t i, double v) { row[i - base] = v; }
+};
+
+/* bytes of the column kernel's deposit stage: FBLOCK fragments of nacc
+ * values, then their FBLOCK cell offsets */
+static size_t stage_bytes(const GridInfo& g) { return (size_t)FBLOCK * (g.nacc + 1) * sizeof(double); }
+
 template <int ORDER>
 __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
                                                         const uint64_t* __restrict__ tasks,
@@ -406,9 +418,16 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
                                                         unsigned long long* counters) {
     __shared__ __align__(16) uint32_t sv[NVF * VSTRIDE];
     __shared__ unsigned swave;
+    /* the deposit stage: the wavefront's fragments of one cell step, in
+     * rank order, nacc values each, then their accumulator offsets */
+    extern __shared__ double stage[];
+    const int na = g.nacc;
+    int64_t* sgi = (int64_t*)(stage + FBLOCK * na);
     const int lane = threadIdx.x;
     LPoly P{(uint32_t*)__builtin_assume_aligned(&sv[4 * lane], 16), 0};
-    AtomicSink sink{acc};
+    /* a lane's first staged value in a round of the transposed deposit, as
+     * (fragment, slot), and the step to its next */
+    const int f0 = lane / na, j0 = lane - f0 * na, fstep = FBLOCK / na, jstep = FBLOCK - fstep * na;
     int64_t nfrag = 0;
     for (;;) {
         if (lane == 0) swave = atomicAdd(next, 1u);
@@ -417,12 +436,69 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
         __syncthreads(); /* everyone has read swave before lane 0 writes it again */
         if (c0 >= ntask) break;
         const int64_t c = c0 + lane;
+        TaskTally T;
+        T.reset();
+        ColumnState S;
+        TetVel V;
+        int64_t p = 0;
+        int ncell = 0;
         if (c < ntask) {
-            TaskTally T;
-            const int64_t p = perm[c];
+            p = perm[c];
             const uint64_t w = tasks[p];
             const double* r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
-            if (!column_task<ORDER>(sink, g, mb, r, w, P, T)) atomicOr(status, ST_FALLBACK);
+            const int st = column_begin(g, mb, r, w, P, S);
+            if (st < 0) atomicOr(status, ST_FALLBACK);
+            if (st > 0) {
+                ncell = S.k1 - S.k0;
+                if (ORDER > 0) tet_vel(r, V);
+                V.mass = r[24];
+                pin(V.mass);
+            }
+        }
+        int nmax = ncell;
+        for (int o = FBLOCK / 2; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+        /* cell steps in lockstep: every lane takes part in each step's
+         * deposit, which issues the wavefront's fragments slot by slot so
+         * that neighbouring lanes add into the same cell's consecutive
+         * slots (the memory-side atomic units then take several lanes per
+         * request; one lane per cell would give one request per lane) */
+        for (int it = 0; it < nmax; ++it) {
+            Mom Mm;
+            float O[3];
+            int k = 0;
+            bool has = false;
+            if (it < ncell) {
+                k = S.k0 + it;
+                bool hl, hh;
+                double lc, hc;
+                cell_planes(S, k, hl, lc, hh, hc);
+                slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O);
+                has = Mm.m0 > 0.0f;
+            }
+            const uint64_t mask = __ballot(has);
+            if (mask == 0ull) continue;
+            if (has) {
+                const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+                const int64_t gi = ((int64_t)g.n[1] * g.n[2] * S.i + (int64_t)g.n[2] * S.j + k) * na;
+                StageSink ss{stage + rank * na, gi};
+                cell_finish<ORDER>(ss, g, S, V, k, Mm, O, T);
+                sgi[rank] = gi;
+            }
+            __syncthreads();
+            const int np = __popcll(mask) * na;
+            int f = f0, j = j0;
+            for (int q = lane; q < np; q += FBLOCK) {
+                unsafeAtomicAdd(acc + sgi[f] + j, stage[q]);
+                f += fstep;
+                j += jstep;
+                if (j >= na) {
+                    j -= na;
+                    ++f;
+                }
+            }
+            __syncthreads(); /* the stage is read before the next step writes it */
+        }
+        if (c < ntask) {
             nfrag += T.nfrag;
             out.sum[p] = T.sum_m0;
             out.key[p] = T.B.kl >= 0 ? best_key(T.B.M.m0, T.B.kl) : 0ull;
@@ -529,24 +605,26 @@ struct PassGeo {
     unsigned long long* mcnt;
 };
 
-/* workgroups of the column kernel the current device holds at once */
-static int64_t resident_blocks(int order) {
+/* workgroups of the column kernel the current device holds at once, with
+ * sb bytes of dynamic LDS */
+static int64_t resident_blocks(int order, size_t sb) {
     static thread_local int cached_dev = -1;
-    static thread_local int64_t cached[3] = {0, 0, 0};
+    static thread_local int ncu = 0;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1 << 20;
     if (dev != cached_dev) {
-        int ncu = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
             ncu = 256;
-        int per[3] = {0, 0, 0};
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_vox_cols<0>, FBLOCK, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_vox_cols<1>, FBLOCK, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[2], k_vox_cols<2>, FBLOCK, 0);
-        for (int o = 0; o < 3; ++o) cached[o] = (int64_t)ncu * (per[o] > 0 ? per[o] : 8);
         cached_dev = dev;
     }
-    return cached[order < 0 ? 0 : (order > 2 ? 2 : order)];
+    int per = 0;
+    if (order == 0)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vox_cols<0>, FBLOCK, sb);
+    else if (order == 1)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vox_cols<1>, FBLOCK, sb);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vox_cols<2>, FBLOCK, sb);
+    return (int64_t)ncu * (per > 0 ? per : 8);
 }
 
 /* the column kernel over tasks [0, ntask) of one mesh (perm: ntask
@@ -558,14 +636,15 @@ static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask,
     k_col_order<<<(unsigned)((ntask + ORDW - 1) / ORDW), 256, 0, s>>>(tets, tasks, ntask, g, mb, perm, next);
     if (mark) (void)hipEventRecord(mark, s);
     const int64_t nwave = (ntask + FBLOCK - 1) / FBLOCK;
-    const int64_t hold = resident_blocks(g.order);
+    const size_t sb = stage_bytes(g);
+    const int64_t hold = resident_blocks(g.order, sb);
     const unsigned nb = (unsigned)(nwave < hold ? nwave : hold);
     if (g.order == 0)
-        k_vox_cols<0><<<nb, FBLOCK, 0, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
+        k_vox_cols<0><<<nb, FBLOCK, sb, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
     else if (g.order == 1)
-        k_vox_cols<1><<<nb, FBLOCK, 0, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
+        k_vox_cols<1><<<nb, FBLOCK, sb, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
     else
-        k_vox_cols<2><<<nb, FBLOCK, 0, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
+        k_vox_cols<2><<<nb, FBLOCK, sb, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
     return (int)hipGetLastError();
 }
 

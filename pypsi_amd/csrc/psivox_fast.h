@@ -1063,8 +1063,8 @@ struct TaskTally {
 
 /* deposit cell k of a task from its slab moments Mm about O */
 template <int ORDER, class Sink>
-PVX_HD void cell_finish(Sink& sink, const GridInfo& g, const ColumnState& S, const double* rec, const TetVel& V, int k,
-                        const Mom& Mm, const float* O, TaskTally& T) {
+PVX_HD void cell_finish(Sink& sink, const GridInfo& g, const ColumnState& S, const TetVel& V, int k, const Mom& Mm,
+                        const float* O, TaskTally& T) {
     if (!(Mm.m0 > 0.0f)) return;
     deposit_f<ORDER>(sink, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0, (const float (*)[3])V.DV, V.mass,
                      (double)Mm.m0);
@@ -1104,7 +1104,7 @@ PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const 
         Mom Mm;
         float O[3];
         slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O);
-        cell_finish<ORDER>(sink, g, S, rec, V, k, Mm, O, T);
+        cell_finish<ORDER>(sink, g, S, V, k, Mm, O, T);
     }
     return true;
 }
