@@ -551,49 +551,84 @@ __global__ __launch_bounds__(256) void k_resid_factor(const double* __restrict__
     }
 }
 
+/* A deposit staged in LDS when its base is not known beforehand: the
+ * first add (deposit_f's mass slot, offset 0) sets it. */
+struct RowSink {
+    double* row;
+    int64_t base;
+    __device__ void add(int64_t i, double v) {
+        if (base < 0) base = i;
+        row[i - base] = v;
+    }
+};
+
 /* one thread per task: its share of its tet's residual, deposited into the
- * task's largest fragment */
+ * task's largest fragment.  Each lane stages its deposit (nacc values) in
+ * LDS; the wavefront then issues the staged rows slot by slot, so that
+ * neighbouring lanes add into the same cell's consecutive slots. */
 template <int ORDER, bool BATCH>
 __global__ __launch_bounds__(256) void k_resid_dep(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
                                                    int64_t ntask, int64_t base, GridInfo g0, MeshBox mb0,
                                                    double* __restrict__ acc0, const BatchGeo* table, TaskOut out,
                                                    const double* __restrict__ tetf,
                                                    const int64_t* __restrict__ tetb) {
+    extern __shared__ double rstage[]; /* per wavefront: FBLOCK rows of nacc values, FBLOCK targets */
+    const int na = g0.nacc;
+    const int lane = threadIdx.x & (FBLOCK - 1), wv = threadIdx.x / FBLOCK;
+    double* rows = rstage + (size_t)wv * FBLOCK * (na + 1);
+    double** dst = (double**)(rows + FBLOCK * na);
+    double* target = nullptr;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ntask) return;
-    const uint64_t kb = out.key[c];
-    if (kb == 0) return;
-    const uint64_t w = tasks[c];
-    const int64_t t = (int64_t)(w & 0xffffffffull);
-    const int64_t whole = tetb[t];
-    if (whole >= 0 && whole != c + base) return;
-    const double weight = whole >= 0 ? tetf[t] : tetf[t] * out.sum[c];
-    if (weight == 0.0) return;
-    BestFrag B;
-    B.kl = (int)(kb & 0xffffffffull);
-    const uint32_t mbits = (uint32_t)(kb >> 32);
-    memcpy(&B.M.m0, &mbits, 4);
-    if (ORDER > 0) {
-        const float4* m = (const float4*)(out.mom + 16 * c);
-        const float4 a = m[0], b = m[1], cc = m[2], d = m[3];
-        B.M.s1[0] = a.y;
-        B.M.s1[1] = a.z;
-        B.M.s1[2] = a.w;
-        B.M.s2[0] = b.x;
-        B.M.s2[1] = b.y;
-        B.M.s2[2] = b.z;
-        B.M.s2[3] = b.w;
-        B.M.s2[4] = cc.x;
-        B.M.s2[5] = cc.y;
-        B.O[0] = cc.z;
-        B.O[1] = cc.w;
-        B.O[2] = d.x;
+    const uint64_t kb = c < ntask ? out.key[c] : 0ull;
+    if (kb != 0ull) {
+        const uint64_t w = tasks[c];
+        const int64_t t = (int64_t)(w & 0xffffffffull);
+        const int64_t whole = tetb[t];
+        const double weight = whole >= 0 ? tetf[t] : tetf[t] * out.sum[c];
+        if ((whole < 0 || whole == c + base) && weight != 0.0) {
+            BestFrag B;
+            B.kl = (int)(kb & 0xffffffffull);
+            const uint32_t mbits = (uint32_t)(kb >> 32);
+            memcpy(&B.M.m0, &mbits, 4);
+            if (ORDER > 0) {
+                const float4* m = (const float4*)(out.mom + 16 * c);
+                const float4 a = m[0], b = m[1], cc = m[2], d = m[3];
+                B.M.s1[0] = a.y;
+                B.M.s1[1] = a.z;
+                B.M.s1[2] = a.w;
+                B.M.s2[0] = b.x;
+                B.M.s2[1] = b.y;
+                B.M.s2[2] = b.z;
+                B.M.s2[3] = b.w;
+                B.M.s2[4] = cc.x;
+                B.M.s2[5] = cc.y;
+                B.O[0] = cc.z;
+                B.O[1] = cc.w;
+                B.O[2] = d.x;
+            }
+            const double* r = tets + (int64_t)TREC * t;
+            with_mesh<BATCH>(r, g0, mb0, acc0, table, [&](const GridInfo& g, const MeshBox& mb, double* acc, int) {
+                RowSink rs{rows + lane * na, -1};
+                task_residual<ORDER>(rs, g, mb, r, w, B, weight);
+                if (rs.base >= 0) target = acc + rs.base;
+            });
+        }
     }
-    const double* r = tets + (int64_t)TREC * t;
-    with_mesh<BATCH>(r, g0, mb0, acc0, table, [&](const GridInfo& g, const MeshBox& mb, double* acc, int) {
-        AtomicSink sink{acc};
-        task_residual<ORDER>(sink, g, mb, r, w, B, weight);
-    });
+    dst[lane] = target;
+    __syncthreads();
+    /* value q = lane + FBLOCK s of the wavefront's rows: row q / na, slot q % na */
+    const int fstep = FBLOCK / na, jstep = FBLOCK - fstep * na;
+    int f = lane / na, j = lane - (lane / na) * na;
+    for (int q = lane; q < FBLOCK * na; q += FBLOCK) {
+        double* d = dst[f];
+        if (d) unsafeAtomicAdd(d + j, rows[q]);
+        f += fstep;
+        j += jstep;
+        if (j >= na) {
+            j -= na;
+            ++f;
+        }
+    }
 }
 
 /* the geometry of one column pass: one mesh, or a table of meshes */
@@ -653,12 +688,13 @@ static void launch_resid_dep_o(const double* tets, const uint64_t* tasks, int64_
                                const PassGeo& G, TaskOut out, const double* tetf, const int64_t* tetb,
                                hipStream_t s) {
     const unsigned nb = (unsigned)((ntask + 255) / 256);
+    const size_t sb = (size_t)(256 / FBLOCK) * stage_bytes(G.g);
     if (G.table)
-        k_resid_dep<ORDER, true><<<nb, 256, 0, s>>>(tets, tasks, ntask, base, G.g, G.mb, G.acc, G.table, out, tetf,
-                                                    tetb);
-    else
-        k_resid_dep<ORDER, false><<<nb, 256, 0, s>>>(tets, tasks, ntask, base, G.g, G.mb, G.acc, nullptr, out, tetf,
+        k_resid_dep<ORDER, true><<<nb, 256, sb, s>>>(tets, tasks, ntask, base, G.g, G.mb, G.acc, G.table, out, tetf,
                                                      tetb);
+    else
+        k_resid_dep<ORDER, false><<<nb, 256, sb, s>>>(tets, tasks, ntask, base, G.g, G.mb, G.acc, nullptr, out, tetf,
+                                                      tetb);
 }
 
 /* the residual pass over the tets [t0, t1) and their ntask tasks; tetf,
