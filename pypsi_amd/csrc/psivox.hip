@@ -405,9 +405,13 @@ struct StageSink {
     __device__ void add(int64_t i, double v) { row[i - base] = v; }
 };
 
-/* bytes of the column kernel's deposit stage: FBLOCK fragments of nacc
- * values, then their FBLOCK cell offsets */
-static size_t stage_bytes(const GridInfo& g) { return (size_t)FBLOCK * (g.nacc + 1) * sizeof(double); }
+/* bytes of a deposit stage of nf fragments: nacc values each, then their
+ * cell offsets */
+static size_t stage_bytes(const GridInfo& g, int nf) { return (size_t)nf * (g.nacc + 1) * sizeof(double); }
+
+/* fragments the column kernel stages at once: half a wavefront, so that the
+ * stage (5.6 KB for m, v, vv) costs no occupancy next to the polyhedra */
+constexpr int SFRAG = FBLOCK / 2;
 
 template <int ORDER>
 __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
@@ -418,11 +422,11 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
                                                         unsigned long long* counters) {
     __shared__ __align__(16) uint32_t sv[NVF * VSTRIDE];
     __shared__ unsigned swave;
-    /* the deposit stage: the wavefront's fragments of one cell step, in
-     * rank order, nacc values each, then their accumulator offsets */
+    /* the deposit stage: up to SFRAG fragments of one cell step, in rank
+     * order, nacc values each, then their accumulator offsets */
     extern __shared__ double stage[];
     const int na = g.nacc;
-    int64_t* sgi = (int64_t*)(stage + FBLOCK * na);
+    int64_t* sgi = (int64_t*)(stage + SFRAG * na);
     const int lane = threadIdx.x;
     LPoly P{(uint32_t*)__builtin_assume_aligned(&sv[4 * lane], 16), 0};
     /* a lane's first staged value in a round of the transposed deposit, as
@@ -477,26 +481,30 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
             }
             const uint64_t mask = __ballot(has);
             if (mask == 0ull) continue;
-            if (has) {
-                const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-                const int64_t gi = ((int64_t)g.n[1] * g.n[2] * S.i + (int64_t)g.n[2] * S.j + k) * na;
-                StageSink ss{stage + rank * na, gi};
-                cell_finish<ORDER>(ss, g, S, V, k, Mm, O, T);
-                sgi[rank] = gi;
-            }
-            __syncthreads();
-            const int np = __popcll(mask) * na;
-            int f = f0, j = j0;
-            for (int q = lane; q < np; q += FBLOCK) {
-                unsafeAtomicAdd(acc + sgi[f] + j, stage[q]);
-                f += fstep;
-                j += jstep;
-                if (j >= na) {
-                    j -= na;
-                    ++f;
+            const int nf = __popcll(mask);
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            /* the step's fragments in groups of SFRAG: stage, then issue */
+            for (int r0 = 0; r0 < nf; r0 += SFRAG) {
+                if (has && rank >= r0 && rank < r0 + SFRAG) {
+                    const int64_t gi = ((int64_t)g.n[1] * g.n[2] * S.i + (int64_t)g.n[2] * S.j + k) * na;
+                    StageSink ss{stage + (rank - r0) * na, gi};
+                    cell_finish<ORDER>(ss, g, S, V, k, Mm, O, T);
+                    sgi[rank - r0] = gi;
                 }
+                __syncthreads();
+                const int np = (nf - r0 < SFRAG ? nf - r0 : SFRAG) * na;
+                int f = f0, j = j0;
+                for (int q = lane; q < np; q += FBLOCK) {
+                    unsafeAtomicAdd(acc + sgi[f] + j, stage[q]);
+                    f += fstep;
+                    j += jstep;
+                    if (j >= na) {
+                        j -= na;
+                        ++f;
+                    }
+                }
+                __syncthreads(); /* the stage is read before it is written again */
             }
-            __syncthreads(); /* the stage is read before the next step writes it */
         }
         if (c < ntask) {
             nfrag += T.nfrag;
@@ -671,7 +679,7 @@ static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask,
     k_col_order<<<(unsigned)((ntask + ORDW - 1) / ORDW), 256, 0, s>>>(tets, tasks, ntask, g, mb, perm, next);
     if (mark) (void)hipEventRecord(mark, s);
     const int64_t nwave = (ntask + FBLOCK - 1) / FBLOCK;
-    const size_t sb = stage_bytes(g);
+    const size_t sb = stage_bytes(g, SFRAG);
     const int64_t hold = resident_blocks(g.order, sb);
     const unsigned nb = (unsigned)(nwave < hold ? nwave : hold);
     if (g.order == 0)
@@ -688,7 +696,7 @@ static void launch_resid_dep_o(const double* tets, const uint64_t* tasks, int64_
                                const PassGeo& G, TaskOut out, const double* tetf, const int64_t* tetb,
                                hipStream_t s) {
     const unsigned nb = (unsigned)((ntask + 255) / 256);
-    const size_t sb = (size_t)(256 / FBLOCK) * stage_bytes(G.g);
+    const size_t sb = (size_t)(256 / FBLOCK) * stage_bytes(G.g, FBLOCK);
     if (G.table)
         k_resid_dep<ORDER, true><<<nb, 256, sb, s>>>(tets, tasks, ntask, base, G.g, G.mb, G.acc, G.table, out, tetf,
                                                      tetb);
