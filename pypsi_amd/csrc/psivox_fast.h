@@ -100,7 +100,10 @@ PVX_HD void pin(double& x) {
 /* A lane's polyhedron: vertex v at w[v * VSTRIDE .. + 4) */
 struct LPoly {
     uint32_t* w;
-    int nv;
+    uint32_t live; /* bit v: slot v holds a vertex */
+    int nv;        /* vertices */
+    uint64_t ord;  /* their slots in order, 4 bits each (rank r in bits [4r, 4r+4)) */
+    PVX_HD int at(int r) const { return (int)((ord >> (4 * r)) & 15u); }
     PVX_HD float qa(int v, int a) const { return u2f(w[v * VSTRIDE + a]); }
     PVX_HD void setq(int v, float a, float b, float c) {
         w[v * VSTRIDE] = f2u(a);
@@ -179,61 +182,82 @@ PVX_HD float sdist(const double* rec, const AxisMap& m, double c, float off, flo
 
 /*
  * Clip P in place to one side of the plane x_ax = c.  keep_upper: keep
- * x > c (drop x <= c); otherwise keep x <= c.  New vertices are appended
- * (the graph is the reference's, src/geometry.c:391-501), linked by the
- * face walk, then everything is compacted.  Only registers and the lane's
- * LDS slice are used.  Returns false on slot overflow or a broken graph.
+ * x > c (drop x <= c); otherwise keep x <= c.  New vertices go into free
+ * slots (the graph is the reference's, src/geometry.c:391-501) and are
+ * linked by the face walk; the slots of dropped vertices are freed, nothing
+ * is renumbered.  Only registers and the lane's LDS slice are used.  Returns
+ * false on slot overflow or a broken graph.
  */
 PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int ax, bool keep_upper) {
-    const int onv = P.nv;
-    if (onv <= 0) return true;
+    const uint32_t live = P.live;
+    if (live == 0u) return true;
     const AxisMap m = axis_map(G, g, ax);
     const float off = (float)(c - m.od);
     uint32_t gone = 0;
-    for (int v = 0; v < onv; ++v) {
-        float t = sdist(G.rec, m, c, off, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
-        bool drop = keep_upper ? (t >= 0.0f) : (t < 0.0f);
+    for (uint32_t lm = live; lm; lm &= lm - 1u) {
+        const int v = __builtin_ctz(lm);
+        const float t = sdist(G.rec, m, c, off, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
+        const bool drop = keep_upper ? (t >= 0.0f) : (t < 0.0f);
         if (drop) gone |= 1u << v;
     }
-    if (gone == 0) return true;
-    if (gone == ((1u << onv) - 1u)) {
+    if (gone == 0u) return true;
+    if (gone == live) {
+        P.live = 0u;
         P.nv = 0;
         return true;
     }
-    int tot = onv;
-    for (int vc = 0; vc < onv; ++vc) {
-        if (gone & (1u << vc)) continue;
+    /* free slots, those above the last survivor first: vertices then keep
+     * the order a compacted graph would give them (survivors in their order,
+     * new vertices after), and so the faces keep their fan apexes */
+    /* the order: survivors in their order, then the new vertices as they
+     * are made (the order a compacted graph has; the faces' fan apexes and
+     * the moment sums follow it) */
+    uint64_t nord = 0;
+    int nk = 0;
+    for (int r = 0; r < P.nv; ++r) {
+        const int v = P.at(r);
+        if (!((gone >> v) & 1u)) nord |= (uint64_t)v << (4 * nk++);
+    }
+    int nt = nk;
+    uint32_t avail = ~live & ((1u << NVF) - 1u), fresh = 0u;
+    for (int r = 0; r < nk; ++r) {
+        const int vc = (int)((nord >> (4 * r)) & 15u);
         for (int k = 0; k < 3; ++k) {
-            int vn = P.nbr(vc, k);
-            if (vn >= onv) {
-                PVX_DBG("clip: bad nbr %d (onv %d)\n", vn, onv);
+            const int vn = P.nbr(vc, k);
+            if (vn >= NVF || !((live >> vn) & 1u)) {
+                PVX_DBG("clip: bad nbr %d of %d\n", vn, vc);
                 return false;
             }
-            if (!(gone & (1u << vn))) continue;
-            if (tot >= NVF) {
-                PVX_DBG("clip: overflow onv %d\n", onv);
+            if (!((gone >> vn) & 1u)) continue;
+            if (avail == 0u) {
+                PVX_DBG("clip: overflow\n");
                 return false;
             }
+            const int vx = __builtin_ctz(avail);
+            avail &= avail - 1u;
             const float c0 = P.qa(vc, 0), c1 = P.qa(vc, 1), c2 = P.qa(vc, 2);
             const float n0 = P.qa(vn, 0), n1 = P.qa(vn, 1), n2 = P.qa(vn, 2);
             const float sc = sdist(G.rec, m, c, off, c0, c1, c2), sn = sdist(G.rec, m, c, off, n0, n1, n2);
             const float inv = frcp(sc - sn);
-            P.setq(tot, inv * (n0 * sc - c0 * sn), inv * (n1 * sc - c1 * sn), inv * (n2 * sc - c2 * sn));
-            P.setnbw(tot, pack3(vc, NBUNSET, NBUNSET));
-            P.setnbr(vc, k, tot);
-            tot++;
+            P.setq(vx, inv * (n0 * sc - c0 * sn), inv * (n1 * sc - c1 * sn), inv * (n2 * sc - c2 * sn));
+            P.setnbw(vx, pack3(vc, NBUNSET, NBUNSET));
+            P.setnbr(vc, k, vx);
+            fresh |= 1u << vx;
+            nord |= (uint64_t)vx << (4 * nt++);
         }
     }
+    const uint32_t after = (live & ~gone) | fresh;
     /* link each new vertex to the next one around its face */
-    for (int vs = onv; vs < tot; ++vs) {
+    for (uint32_t fm = fresh; fm; fm &= fm - 1u) {
+        const int vs = __builtin_ctz(fm);
         int vcur = vs, vnext = P.nbr(vs, 0);
         int guard = 0;
         for (;;) {
-            if (vnext >= tot || ++guard > 2 * NVF) {
-                PVX_DBG("clip: walk lost vnext %d tot %d guard %d\n", vnext, tot, guard);
+            if (vnext >= NVF || !((after >> vnext) & 1u) || ++guard > 2 * NVF) {
+                PVX_DBG("clip: walk lost vnext %d guard %d\n", vnext, guard);
                 return false;
             }
-            uint32_t w = P.nbw(vnext);
+            const uint32_t w = P.nbw(vnext);
             int k = 0;
             while (k < 3 && nbf(w, k) != vcur) k++;
             if (k == 3) {
@@ -241,34 +265,15 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int 
                 return false;
             }
             vcur = vnext;
-            if (vcur >= onv) break;
+            if ((fresh >> vcur) & 1u) break;
             vnext = nbf(w, (k + 1) % 3);
         }
         P.setnbr(vs, 2, vcur);
         P.setnbr(vcur, 1, vs);
     }
-    /* compact: survivors keep their order (rank among survivors), new
-     * vertices follow; ranks come from the bit mask, no index table */
-    const uint32_t keep = ~gone & ((1u << onv) - 1u);
-    const int nk0 = __builtin_popcount(keep);
-    for (int v = 0; v < tot; ++v) {
-        if (v < onv && (gone & (1u << v))) continue;
-        uint32_t w = P.nbw(v);
-        uint32_t o = 0;
-        for (int k = 0; k < 3; ++k) {
-            int n = nbf(w, k);
-            if (n >= tot) {
-                PVX_DBG("clip: compaction bad index %d (tot %d) at v %d\n", n, tot, v);
-                return false;
-            }
-            int r = n < onv ? __builtin_popcount(keep & ((1u << n) - 1u)) : nk0 + (n - onv);
-            o |= (uint32_t)r << (5 * k);
-        }
-        int d = v < onv ? __builtin_popcount(keep & ((1u << v) - 1u)) : nk0 + (v - onv);
-        if (d != v) P.setq(d, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
-        P.setnbw(d, o);
-    }
-    P.nv = nk0 + (tot - onv);
+    P.live = after;
+    P.nv = nt;
+    P.ord = nord;
     return true;
 }
 
@@ -311,10 +316,14 @@ PVX_HD void tri_acc(Mom& M, const float* A, const float* B, const float* C) {
  * (v0, vi, vi+1) exactly as the reference's reduce fans it.
  */
 PVX_HD void face_apexes(LPoly& P) {
-    const int nv = P.nv;
-    for (int v = 0; v < nv; ++v) P.setnbw(v, P.nbw(v) & 0x7fffu);
+    const uint32_t live = P.live;
+    for (uint32_t lm = live; lm; lm &= lm - 1u) {
+        const int v = __builtin_ctz(lm);
+        P.setnbw(v, P.nbw(v) & 0x7fffu);
+    }
     uint64_t mark = 0;
-    for (int vs = 0; vs < nv; ++vs)
+    for (int r = 0; r < P.nv; ++r) {
+        const int vs = P.at(r);
         for (int ks = 0; ks < 3; ++ks) {
             if (mark & (1ull << (3 * vs + ks))) continue;
             int a = vs, ka = ks, guard = 0;
@@ -330,6 +339,7 @@ PVX_HD void face_apexes(LPoly& P) {
                 ka = (kb + 1) % 3;
             } while (!(a == vs && ka == ks) && ++guard <= 3 * NVF);
         }
+    }
 }
 
 /* tri_acc with the signed volume scaled by w (0 or 1: a term that is
@@ -359,7 +369,8 @@ PVX_HD void tri_acc_w(Mom& M, const float* A, const float* B, const float* C, fl
  * edge found between z <= p and z > p; false when no edge crosses */
 PVX_HD bool plane_point(const LPoly& P, float zd0, float zd1, float zd2, float p, float* out) {
     bool found = false;
-    for (int v = 0; v < P.nv && !found; ++v) {
+    for (int r = 0; r < P.nv && !found; ++r) {
+        const int v = P.at(r);
         const float q0 = P.qa(v, 0), q1 = P.qa(v, 1), q2 = P.qa(v, 2);
         const float zv = zd0 * q0 + zd1 * q1 + zd2 * q2;
         const uint32_t w = P.nbw(v);
@@ -438,7 +449,7 @@ struct SlabCell {
     float O[3], gh[3];
     float zd[3]; /* x_ax(q) - x_ax(q = 0) = zd . q */
     bool has_lo, has_hi;
-    uint64_t mask; /* bit 3a + k: the fan triangle of directed edge a -> k */
+    uint64_t mask; /* bit 3r + k: the fan triangle of directed edge a -> k, a the vertex of rank r */
 };
 
 /*
@@ -460,7 +471,8 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
     S.zd[0] = zd0;
     S.zd[1] = zd1;
     S.zd[2] = zd2;
-    float O[3] = {P.qa(0, 0), P.qa(0, 1), P.qa(0, 2)};
+    const int v0 = P.at(0);
+    float O[3] = {P.qa(v0, 0), P.qa(v0, 1), P.qa(v0, 2)};
     float Gh[3] = {O[0], O[1], O[2]};
     if (has_lo) {
         plane_point(P, zd0, zd1, zd2, S.lo, O);
@@ -473,7 +485,8 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
         S.gh[i] = Gh[i] - O[i];
     }
     uint64_t mask = 0;
-    for (int a = 0; a < P.nv; ++a) {
+    for (int ra = 0; ra < P.nv; ++ra) {
+        const int a = P.at(ra);
         const uint32_t wa = P.nbw(a);
         const float za = zd0 * P.qa(a, 0) + zd1 * P.qa(a, 1) + zd2 * P.qa(a, 2);
         for (int k = 0; k < 3; ++k) {
@@ -482,7 +495,7 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
             const float zb = zd0 * P.qa(b, 0) + zd1 * P.qa(b, 1) + zd2 * P.qa(b, 2);
             const float zc = zd0 * P.qa(c, 0) + zd1 * P.qa(c, 1) + zd2 * P.qa(c, 2);
             const float zmax = fmaxf(za, fmaxf(zb, zc)), zmin = fminf(za, fminf(zb, zc));
-            if (zmax > S.lo && zmin <= S.hi) mask |= 1ull << (3 * a + k);
+            if (zmax > S.lo && zmin <= S.hi) mask |= 1ull << (3 * ra + k);
         }
     }
     S.mask = mask;
@@ -502,7 +515,7 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
  */
 template <int ORDER>
 PVX_HD void pair_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
-    const int a = e / 3, k = e - 3 * a;
+    const int ra = e / 3, k = e - 3 * ra, a = P.at(ra);
     const float* O = S.O;
     const float zd0 = S.zd[0], zd1 = S.zd[1], zd2 = S.zd[2];
     const float lo = S.lo, hi = S.hi;
@@ -570,7 +583,8 @@ PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo
 PVX_HD void lpoly_range(const LPoly& P, const CopyGeo& G, int ax, int ib0, int plo, int phi, int n, int& lo,
                         int& hi) {
     float zmin = 3.0e38f, zmax = -3.0e38f;
-    for (int v = 0; v < P.nv; ++v) {
+    for (uint32_t lm = P.live; lm; lm &= lm - 1u) {
+        const int v = __builtin_ctz(lm);
         float z = G.x0[ax] + G.DX[ax][0] * P.qa(v, 0) + G.DX[ax][1] * P.qa(v, 1) + G.DX[ax][2] * P.qa(v, 2);
         zmin = fminf(zmin, z);
         zmax = fmaxf(zmax, z);
@@ -946,7 +960,9 @@ PVX_HD bool build_column(LPoly& P, const CopySetup& C, const GridInfo& g, int i,
     const CopyGeo& G = C.G;
     const int il = i - G.ib0[0], jl = j - G.ib0[1];
     const bool xlo = i > G.ib0[0], xhi = i < C.ib1[0] - 1, ylo = j > j0, yhi = j < j1 - 1;
+    P.live = 0xfu;
     P.nv = 4;
+    P.ord = 0x3210ull;
     P.setq(0, 0.f, 0.f, 0.f);
     P.setq(1, 1.f, 0.f, 0.f);
     P.setq(2, 0.f, 1.f, 0.f);
@@ -1011,7 +1027,7 @@ PVX_HD int column_begin(const GridInfo& g, const MeshBox& mb, const double* rec,
     int j0, j1;
     if (!slab_columns(S.C, S.i, g, j0, j1)) return 0;
     if (!build_column(P, S.C, g, S.i, S.j, j0, j1)) return -1;
-    if (P.nv <= 0) return 0;
+    if (P.live == 0u) return 0;
     face_apexes(P);
     lpoly_range(P, S.C.G, 2, S.C.G.ib0[2], S.C.G.ib0[2], S.C.ib1[2], g.n[2], S.k0, S.k1);
     return S.k1 > S.k0 ? 1 : 0;
