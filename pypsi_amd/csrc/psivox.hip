@@ -440,8 +440,12 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
         __syncthreads(); /* everyone has read swave before lane 0 writes it again */
         if (c0 >= ntask) break;
         const int64_t c = c0 + lane;
-        TaskTally T;
-        T.reset();
+        /* the task's tally: fragments, FP64 mass sum and its largest
+         * fragment (moments written to out.mom as they improve) */
+        int tfrag = 0;
+        double tsum = 0.0;
+        float bm0 = 0.0f;
+        int bkl = -1;
         ColumnState S;
         TetVel V;
         int64_t p = 0;
@@ -488,7 +492,8 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
                 if (has && rank >= r0 && rank < r0 + SFRAG) {
                     const int64_t gi = ((int64_t)g.n[1] * g.n[2] * S.i + (int64_t)g.n[2] * S.j + k) * na;
                     StageSink ss{stage + (rank - r0) * na, gi};
-                    cell_finish<ORDER>(ss, g, S, V, k, Mm, O, T);
+                    deposit_f<ORDER>(ss, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0, (const float (*)[3])V.DV,
+                                     V.mass, (double)Mm.m0);
                     sgi[rank - r0] = gi;
                 }
                 __syncthreads();
@@ -505,18 +510,26 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
                 }
                 __syncthreads(); /* the stage is read before it is written again */
             }
+            if (has) {
+                tfrag++;
+                tsum += (double)Mm.m0;
+                if (Mm.m0 > bm0) {
+                    bm0 = Mm.m0;
+                    bkl = k - S.C.G.ib0[2];
+                    if (ORDER > 0) {
+                        float4* m = (float4*)(out.mom + 16 * p);
+                        m[0] = make_float4(Mm.m0, Mm.s1[0], Mm.s1[1], Mm.s1[2]);
+                        m[1] = make_float4(Mm.s2[0], Mm.s2[1], Mm.s2[2], Mm.s2[3]);
+                        m[2] = make_float4(Mm.s2[4], Mm.s2[5], O[0], O[1]);
+                        m[3] = make_float4(O[2], 0.f, 0.f, 0.f);
+                    }
+                }
+            }
         }
         if (c < ntask) {
-            nfrag += T.nfrag;
-            out.sum[p] = T.sum_m0;
-            out.key[p] = T.B.kl >= 0 ? best_key(T.B.M.m0, T.B.kl) : 0ull;
-            if (ORDER > 0 && T.B.kl >= 0) {
-                float4* m = (float4*)(out.mom + 16 * p);
-                m[0] = make_float4(T.B.M.m0, T.B.M.s1[0], T.B.M.s1[1], T.B.M.s1[2]);
-                m[1] = make_float4(T.B.M.s2[0], T.B.M.s2[1], T.B.M.s2[2], T.B.M.s2[3]);
-                m[2] = make_float4(T.B.M.s2[4], T.B.M.s2[5], T.B.O[0], T.B.O[1]);
-                m[3] = make_float4(T.B.O[2], 0.f, 0.f, 0.f);
-            }
+            nfrag += tfrag;
+            out.sum[p] = tsum;
+            out.key[p] = bkl >= 0 ? best_key(bm0, bkl) : 0ull;
         }
     }
     block_count(nfrag, 0, counters);
