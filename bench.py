@@ -125,6 +125,49 @@ def cpu_baseline(wname, pos, vel, mass, conn, gn, fields, budget_s=12.0):
                       f"{dt:.1f} s single-threaded, -O3 x86-64-v3"}
 
 
+def cpu_all_cores(pos, vel, mass, conn, gn, fields, budget_s=10.0):
+    """The oracle's C restatement with OpenMP over elements on all host
+    cores the process may use (per-thread private grids summed at the end),
+    on the first elements of the same mesh, about budget_s of wall time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+
+    def run(e1):
+        P = O.Problem(pos, vel, mass[:e1], conn[:e1], periodic_box=box, n=(gn,) * 3, bounds=box, fields=fields)
+        t0 = time.perf_counter()
+        _, st = O.voxels(P, threads=nth)
+        return st, time.perf_counter() - t0
+
+    st, dt = run(min(conn.shape[0], 4096 * nth))
+    e1 = int(min(conn.shape[0], max(4096, 4096 * nth * budget_s / max(dt, 1e-3))))
+    st, dt = run(e1)
+    return {"value": st["nfrag"] / dt, "unit": "fragments/s", "cores": nth, "kind": "port",
+            "sample": f"first {e1} of {conn.shape[0]} elements ({st['nfrag']} fragments) onto the full {gn}^3 grid, "
+                      f"fields m{''.join(',' + f for f in fields)}; {dt:.1f} s on {nth} threads, private grids summed"}
+
+
+def end_to_end_leg(pos, vel, mass, conn, gn, fields, nfrag, steps=2, warmup=1):
+    """PSI.voxels on NumPy arrays (the drop-in call): uploads of the mesh and
+    the fields' contents, the device work, the download of the fields."""
+    import numpy as np
+    from pypsi_amd import PSI
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    n = round(conn.shape[0] ** (1.0 / 3.0))
+    mesh = PSI.Mesh(loader="block", pos=pos.reshape(n, n, n, 3), vel=vel.reshape(n, n, n, 3), periodic_box=box)
+    ts = []
+    for i in range(warmup + steps):
+        grid = PSI.Grid(type="cart", n=(gn,) * 3, bounds=box, fields=list(fields))
+        t0 = time.perf_counter()
+        PSI.voxels(grid=grid, mesh=mesh)
+        if i >= warmup:
+            ts.append(time.perf_counter() - t0)
+    ms = 1e3 * sum(ts) / len(ts)
+    return {"value": nfrag / (ms * 1e-3), "unit": "fragments/s", "ms_per_step": ms, "steps": steps,
+            "what": "PSI.voxels(grid, mesh) on host NumPy arrays: PCIe in and out included"}
+
+
 def plain_leg(mesh, grid, tf, stream, nfrag, steps=2, warmup=1):
     """The FP64 one-thread-per-tet kernel on the same inputs (the plain GPU
     method the FP32-clip column kernel is measured against)."""
@@ -458,12 +501,19 @@ def main():
         out["batched_uneven"] = batched_leg()
         log("[bench] refined clumps, level 2, 512^3")
         out["refined"] = refined_leg(next(iter(REFINED)))
+        log("[bench] end to end from host arrays")
+        out["end_to_end"] = end_to_end_leg(pos, vel, mass, conn, gn, fields, nfrag)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         try:
             out["cpu_baseline"] = cpu_baseline(wname, pos, vel, mass, conn, gn, fields)
         except Exception as e:  # report, never hide
             out["cpu_baseline"] = {"error": repr(e)}
+        log("[bench] cpu, all host cores")
+        try:
+            out["cpu_baseline_all_cores"] = cpu_all_cores(pos, vel, mass, conn, gn, fields)
+        except Exception as e:
+            out["cpu_baseline_all_cores"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 
