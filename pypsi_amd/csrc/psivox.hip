@@ -413,7 +413,11 @@ static size_t stage_bytes(const GridInfo& g, int nf) { return (size_t)nf * (g.na
  * stage (5.6 KB for m, v, vv) costs no occupancy next to the polyhedra */
 constexpr int SFRAG = FBLOCK / 2;
 
-template <int ORDER>
+/* field flags of grids without position moments (x, xx, xv): the column
+ * kernel built for them keeps no position data through its cell loop */
+constexpr int FMASK_NOX = ~(F_X | F_XX | F_XV);
+
+template <int ORDER, int FMASK>
 __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
                                                         const uint64_t* __restrict__ tasks,
                                                         const uint32_t* __restrict__ perm, unsigned* __restrict__ next,
@@ -492,8 +496,8 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
                 if (has && rank >= r0 && rank < r0 + SFRAG) {
                     const int64_t gi = ((int64_t)g.n[1] * g.n[2] * S.i + (int64_t)g.n[2] * S.j + k) * na;
                     StageSink ss{stage + (rank - r0) * na, gi};
-                    deposit_f<ORDER>(ss, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0, (const float (*)[3])V.DV,
-                                     V.mass, (double)Mm.m0);
+                    deposit_f<ORDER, FMASK>(ss, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0,
+                                            (const float (*)[3])V.DV, V.mass, (double)Mm.m0);
                     sgi[rank - r0] = gi;
                 }
                 __syncthreads();
@@ -661,9 +665,12 @@ struct PassGeo {
     unsigned long long* mcnt;
 };
 
-/* workgroups of the column kernel the current device holds at once, with
- * sb bytes of dynamic LDS */
-static int64_t resident_blocks(int order, size_t sb) {
+typedef void (*KColsFn)(const double*, const uint64_t*, const uint32_t*, unsigned*, int64_t, GridInfo, MeshBox,
+                        double*, TaskOut, unsigned*, unsigned long long*);
+
+/* workgroups of a column kernel the current device holds at once, with sb
+ * bytes of dynamic LDS */
+static int64_t resident_blocks(KColsFn kern, size_t sb) {
     static thread_local int cached_dev = -1;
     static thread_local int ncu = 0;
     int dev = 0;
@@ -674,13 +681,16 @@ static int64_t resident_blocks(int order, size_t sb) {
         cached_dev = dev;
     }
     int per = 0;
-    if (order == 0)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vox_cols<0>, FBLOCK, sb);
-    else if (order == 1)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vox_cols<1>, FBLOCK, sb);
-    else
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vox_cols<2>, FBLOCK, sb);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, FBLOCK, sb);
     return (int64_t)ncu * (per > 0 ? per : 8);
+}
+
+/* the column kernel built for a grid's moment order and field set */
+static KColsFn cols_kernel(const GridInfo& g) {
+    const bool nox = !(g.flags & (F_X | F_XX | F_XV));
+    if (g.order == 0) return k_vox_cols<0, -1>;
+    if (g.order == 1) return nox ? k_vox_cols<1, FMASK_NOX> : k_vox_cols<1, -1>;
+    return nox ? k_vox_cols<2, FMASK_NOX> : k_vox_cols<2, -1>;
 }
 
 /* the column kernel over tasks [0, ntask) of one mesh (perm: ntask
@@ -693,14 +703,11 @@ static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask,
     if (mark) (void)hipEventRecord(mark, s);
     const int64_t nwave = (ntask + FBLOCK - 1) / FBLOCK;
     const size_t sb = stage_bytes(g, SFRAG);
-    const int64_t hold = resident_blocks(g.order, sb);
+    const KColsFn kern = cols_kernel(g);
+    const int64_t hold = resident_blocks(kern, sb);
     const unsigned nb = (unsigned)(nwave < hold ? nwave : hold);
-    if (g.order == 0)
-        k_vox_cols<0><<<nb, FBLOCK, sb, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
-    else if (g.order == 1)
-        k_vox_cols<1><<<nb, FBLOCK, sb, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
-    else
-        k_vox_cols<2><<<nb, FBLOCK, sb, s>>>(tets, tasks, perm, next, ntask, g, mb, acc, out, status, counters);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(FBLOCK), sb, s, tets, tasks, perm, next, ntask, g, mb, acc, out, status,
+                       counters);
     return (int)hipGetLastError();
 }
 

@@ -629,8 +629,9 @@ PVX_HD void slab_yrange(const float (*X)[3], bool has_lo, float lo, bool has_hi,
         }
 }
 
-/* per-fragment deposit from FP32 q-moments about origin O (FP64 sums) */
-template <int ORDER, class Sink>
+/* per-fragment deposit from FP32 q-moments about origin O (FP64 sums);
+ * FMASK: the field flags a caller can have (the others compile away) */
+template <int ORDER, int FMASK = -1, class Sink>
 PVX_HD void deposit_f(Sink& sink, const GridInfo& g, const CopyGeo& G, const int* ib0, int i, int j, int k,
                       const Mom& Mm, const float* O, const double* v0, const float (*DV)[3], double mass,
                       double wm0) {
@@ -654,7 +655,7 @@ PVX_HD void deposit_f(Sink& sink, const GridInfo& g, const CopyGeo& G, const int
         cov[1][2] = cov[2][1] = c20 * Mm.s2[4] - mu[1] * mu[2];
         cov[2][2] = c20 * Mm.s2[5] - mu[2] * mu[2];
     }
-    const int fl = g.flags;
+    const int fl = g.flags & FMASK;
     const int idx[3] = {i, j, k};
     double dx[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
     float b_[3] = {0, 0, 0}; /* DV qbar */
