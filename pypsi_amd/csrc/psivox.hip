@@ -100,22 +100,24 @@ struct MeshDev {
  * elements outside the window leave invalid records */
 /* tag: written to every record's slot 27 (the mesh's index in a batched
  * call, 0 otherwise) */
-__global__ void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, int64_t ne, double* __restrict__ tets,
+__global__ __launch_bounds__(128) void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, int64_t ne, double* __restrict__ tets,
                                double tag = 0.0) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ne) return;
     const int per = (m.vpe == 4) ? 1 : 6;
-    double P[27][3], U[27][3];
     const int64_t e = e0 + t;
-    for (int v = 0; v < m.vpe; ++v) {
-        int64_t id = m.conn[e * m.vpe + v];
+    double* out = tets + (int64_t)TREC * per * t;
+    /* simplex and linear elements: corners only, all in registers */
+    double P[8][3], U[8][3];
+    const int nv = m.vpe == 4 ? 4 : 8;
+    for (int v = 0; v < 8; ++v) {
+        const int64_t id = m.conn[e * m.vpe + (v < nv ? v : 0)];
         for (int a = 0; a < 3; ++a) {
             P[v][a] = m.pos[3 * id + a];
             U[v][a] = m.vel ? m.vel[3 * id + a] : 0.0;
         }
     }
-    double* out = tets + (int64_t)TREC * per * t;
-    if (!wrap_and_test(P, m.vpe, g, mb)) {
+    if (!(m.vpe == 4 ? wrap_and_test(P, 4, g, mb) : wrap_and_test(P, 8, g, mb))) {
         for (int s = 0; s < per; ++s) {
             out[TREC * s + 25] = 0.0;
             out[TREC * s + 27] = tag;
@@ -135,15 +137,44 @@ __global__ void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, in
         out[27] = tag;
         return;
     }
-    if (m.vpe == 8) {
-        double p27[27][3], u27[27][3];
-        lin27((const double (*)[3])P, p27);
-        lin27((const double (*)[3])U, u27);
-        emit6(out, (const double (*)[3])p27, (const double (*)[3])u27, mass);
-    } else {
-        emit6(out, (const double (*)[3])P, (const double (*)[3])U, mass);
-    }
+    emit6_corners(out, (const double (*)[3])P, (const double (*)[3])U, mass);
     for (int s = 0; s < 6; ++s) out[TREC * s + 27] = tag;
+}
+
+/* the same for triquadratic elements: the 27 nodes are the lattice */
+__global__ __launch_bounds__(128) void k_expand_fixed27(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, int64_t ne,
+                                 double* __restrict__ tets, double tag = 0.0) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne) return;
+    const int64_t e = e0 + t;
+    double* out = tets + (int64_t)TREC * 6 * t;
+    double P[27][3], U[27][3];
+    for (int v = 0; v < 27; ++v) {
+        const int64_t id = m.conn[e * 27 + v];
+        for (int a = 0; a < 3; ++a) {
+            P[v][a] = m.pos[3 * id + a];
+            U[v][a] = m.vel ? m.vel[3 * id + a] : 0.0;
+        }
+    }
+    if (!wrap_and_test(P, 27, g, mb)) {
+        for (int s = 0; s < 6; ++s) {
+            out[TREC * s + 25] = 0.0;
+            out[TREC * s + 27] = tag;
+        }
+        return;
+    }
+    emit6(out, (const double (*)[3])P, (const double (*)[3])U, m.mass[e]);
+    for (int s = 0; s < 6; ++s) out[TREC * s + 27] = tag;
+}
+
+/* fixed expansion of elements [e0, e0 + ne) */
+static void launch_expand_fixed(const MeshDev& md, const GridInfo& g, const MeshBox& mb, int64_t e0, int64_t ne,
+                                double* tets, double tag, hipStream_t s) {
+    const unsigned nb = (unsigned)((ne + 127) / 128);
+    if (md.vpe == 27)
+        k_expand_fixed27<<<nb, 128, 0, s>>>(md, g, mb, e0, ne, tets, tag);
+    else
+        k_expand_fixed<<<nb, 128, 0, s>>>(md, g, mb, e0, ne, tets, tag);
 }
 
 /* refine_max_lvl > 0: count pass and emit pass of the per-element DFS */
@@ -1131,7 +1162,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 k_expand_emit<<<grid_blocks(e1 - e0, 128), 128, 0, s>>>(md, g, mb, e0, e1, maxlvl, tol2,
                                                                         (const int64_t*)W.offs.p, hoffs[e0], tets);
             } else {
-                k_expand_fixed<<<grid_blocks(e1 - e0, 128), 128, 0, s>>>(md, g, mb, e0, e1 - e0, tets);
+                launch_expand_fixed(md, g, mb, e0, e1 - e0, tets, 0.0, s);
             }
             HIPCHK(hipGetLastError());
             if (kernel == PSIVOX_KERNEL_FAST && !(mb.periodic && mb.full_window)) {
@@ -1300,7 +1331,7 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
         const int64_t nt = (e1 - e0) * per;
         if ((rc = W.tets.get((size_t)nt * TREC * sizeof(double), dev))) return rc;
         double* tets = (double*)W.tets.p;
-        k_expand_fixed<<<grid_blocks(e1 - e0, 128), 128, 0, s>>>(md, g, mb, e0, e1 - e0, tets);
+        launch_expand_fixed(md, g, mb, e0, e1 - e0, tets, 0.0, s);
         HIPCHK(hipGetLastError());
         /* tasks: count, scan, emit (chunks of tets whose tasks fit) */
         if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
@@ -1491,8 +1522,7 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
             k_to_raw<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, fp[q], bacc + acc_off[q], nvox);
             HIPCHK(hipGetLastError());
             if (m.nelem > 0) {
-                k_expand_fixed<<<grid_blocks(m.nelem, 128), 128, 0, s>>>(md[q], g, MB[i], 0, m.nelem,
-                                                                        tets + TREC * T[q], (double)q);
+                launch_expand_fixed(md[q], g, MB[i], 0, m.nelem, tets + TREC * T[q], (double)q, s);
                 HIPCHK(hipGetLastError());
                 if (!(MB[i].periodic && MB[i].full_window)) {
                     k_targets<<<grid_blocks(T[q + 1] - T[q], 64), 64, 0, s>>>(tets + TREC * T[q], T[q + 1] - T[q], g,

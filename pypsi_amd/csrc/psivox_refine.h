@@ -149,6 +149,27 @@ PVX_HD void emit6(double* out, const double (*p)[3], const double (*u)[3], doubl
     }
 }
 
+/* emit6 straight from the 8 corners (4i+2j+k) of a linear element: the six
+ * tets use only the 3x3x3 lattice's corners, which lin27 copies unchanged,
+ * so this writes the same records without the 27-point arrays */
+PVX_HD void emit6_corners(double* out, const double (*p)[3], const double (*u)[3], double mass) {
+    for (int t = 0; t < 6; ++t) {
+        double* r = out + TREC * t;
+        for (int v = 0; v < 4; ++v) {
+            /* lattice point l = 9i + 3j + k with i, j, k in {0, 2} */
+            const int l = tet27(t, v), c = 4 * ((l / 9) / 2) + 2 * (((l / 3) % 3) / 2) + (l % 3) / 2;
+            for (int a = 0; a < 3; ++a) {
+                r[3 * v + a] = p[c][a];
+                r[12 + 3 * v + a] = u[c][a];
+            }
+        }
+        r[24] = mass * (1.0 / 6);
+        r[25] = 1.0;
+        r[26] = 1.0;
+        r[27] = 0.0;
+    }
+}
+
 /* FP64 q-volume fraction of the tet record inside the window, summed over
  * its periodic copies (src/cpsi.c:343-373 copies, src/geometry.c:816-843
  * window planes, src/geometry.c:692-767 volume) */
