@@ -201,42 +201,48 @@ PVX_HD void adj_plane_task(const GridInfo& g, const MeshBox& mb, const double* r
     /* section: crossing points of the edges between the two sides, in
      * cyclic order (1-3 split: the three edges at the lone vertex; 2-2
      * split: b1a1, b1a2, b2a2, b2a1) */
-    int below[4], above[4], nbl = 0, nab = 0;
-    for (int v = 0; v < 4; ++v) {
-        if (rec[3 * v + a] < xp)
-            below[nbl++] = v;
-        else
-            above[nab++] = v;
-    }
+    /* vertices below the plane as a bit mask; the k-th set (clear) bit is
+     * the k-th vertex below (above) in index order */
+    unsigned bm = 0;
+    for (int v = 0; v < 4; ++v) bm |= (rec[3 * v + a] < xp ? 1u : 0u) << v;
+    const int nbl = __builtin_popcount(bm), nab = 4 - nbl;
     if (nbl == 0 || nab == 0) return;
+    const unsigned am = ~bm & 15u;
+    const int b0 = __builtin_ctz(bm), b1 = __builtin_ctz(bm & (bm - 1u) | 16u);
+    const int a0 = __builtin_ctz(am), a1 = __builtin_ctz(am & (am - 1u) | 16u);
     int eu[4], ev[4], ne;
     if (nbl == 1 || nab == 1) {
-        const int lone = nbl == 1 ? below[0] : above[0];
-        const int* oth = nbl == 1 ? above : below;
-        for (int q = 0; q < 3; ++q) {
-            eu[q] = lone;
-            ev[q] = oth[q];
-        }
+        const int lone = nbl == 1 ? b0 : a0;
+        /* the other three vertices in index order */
+        const unsigned om = (nbl == 1 ? am : bm) & 15u, om1 = om & (om - 1u), om2 = om1 & (om1 - 1u);
+        const int o0 = __builtin_ctz(om), o1 = __builtin_ctz(om1), o2 = __builtin_ctz(om2);
+        eu[0] = eu[1] = eu[2] = eu[3] = lone;
+        ev[0] = o0;
+        ev[1] = o1;
+        ev[2] = o2;
+        ev[3] = o2;
         ne = 3;
     } else {
-        eu[0] = below[0], ev[0] = above[0];
-        eu[1] = below[0], ev[1] = above[1];
-        eu[2] = below[1], ev[2] = above[1];
-        eu[3] = below[1], ev[3] = above[0];
+        eu[0] = b0, ev[0] = a0;
+        eu[1] = b0, ev[1] = a1;
+        eu[2] = b1, ev[2] = a1;
+        eu[3] = b1, ev[3] = a0;
         ne = 4;
     }
     double su[4], sv[4];
     double umin = 1e300, umax = -1e300;
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
-        if (q >= ne) break;
         const double* X = rec + 3 * eu[q];
         const double* Y = rec + 3 * ev[q];
         const double f = (xp - X[a]) / (Y[a] - X[a]);
         /* cell-local in-plane coordinates of the copy */
         su[q] = (X[b] + f * (Y[b] - X[b]) + sb - w0b) / db;
         sv[q] = (X[c] + f * (Y[c] - X[c]) + sc - w0c) / dc;
-        umin = fmin(umin, su[q]);
-        umax = fmax(umax, su[q]);
+        if (q < ne) {
+            umin = fmin(umin, su[q]);
+            umax = fmax(umax, su[q]);
+        }
     }
     int j0 = (int)floor(umin), j1 = (int)floor(umax);
     if (j0 < 0) j0 = 0;
@@ -256,10 +262,13 @@ PVX_HD void adj_plane_task(const GridInfo& g, const MeshBox& mb, const double* r
             vmin = fmin(vmin, vv);
             vmax = fmax(vmax, vv);
         };
+#pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (q >= ne) break;
-            const int r = q + 1 == ne ? 0 : q + 1;
-            slab_edge(su[q], sv[q], su[r], sv[r], 0, (double)j, (double)(j + 1), put);
+            /* the next crossing point, wrapping to the first after the last */
+            const bool wrap = q + 1 == ne;
+            const double nu = wrap ? su[0] : su[q < 3 ? q + 1 : 0], nv = wrap ? sv[0] : sv[q < 3 ? q + 1 : 0];
+            slab_edge(su[q], sv[q], nu, nv, 0, (double)j, (double)(j + 1), put);
         }
         if (m < 3 || m > 8) continue; /* a quad cut by a slab has at most 6 */
         int k0 = (int)floor(vmin), k1 = (int)floor(vmax);
@@ -281,13 +290,14 @@ PVX_HD void adj_plane_task(const GridInfo& g, const MeshBox& mb, const double* r
             S.close();
             if (S.a2 == 0.0) continue;
             const double cu = S.mu / (3.0 * S.a2), cv = S.mv / (3.0 * S.a2);
-            double X[3];
-            X[a] = xp;
-            X[b] = w0b + db * ((double)j + cu) - sb;
-            X[c] = w0c + dc * ((double)k + cv) - sc;
+            /* the section's centroid in the record's frame (axes a, b, c
+             * are a cyclic permutation of x, y, z) */
+            const double xb = w0b + db * ((double)j + cu) - sb, xc = w0c + dc * ((double)k + cv) - sc;
+            const double X0 = a == 0 ? xp : (b == 0 ? xb : xc), X1 = a == 1 ? xp : (b == 1 ? xb : xc),
+                         X2 = a == 2 ? xp : (b == 2 ? xb : xc);
             double l[3];
             for (int r = 0; r < 3; ++r)
-                l[r] = T.B[r][0] * (X[0] - T.x0[0]) + T.B[r][1] * (X[1] - T.x0[1]) + T.B[r][2] * (X[2] - T.x0[2]);
+                l[r] = T.B[r][0] * (X0 - T.x0[0]) + T.B[r][1] * (X1 - T.x0[1]) + T.B[r][2] * (X2 - T.x0[2]);
             const double f = J * 0.5 * fabs(S.a2) * db * dc;
             acc[0] += f * (1.0 - l[0] - l[1] - l[2]);
             acc[1] += f * l[0];
