@@ -151,7 +151,7 @@ PVX_HD AxisMap axis_map(const CopyGeo& G, const GridInfo& g, int ax) {
     m.d2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
     m.shift = pick3(G.shift, ax);
     m.w0 = pick3(g.w0, ax);
-    m.dinv = 1.0 / pick3(g.d, ax);
+    m.dinv = pick3(g.dinv, ax);
     m.ib0 = pick3(G.ib0, ax);
     return m;
 }
@@ -772,14 +772,14 @@ PVX_HD bool copy_setup(const double* rec, const double* lo, const double* hi, ui
     }
     if (C.G.ib0[0] >= g.n[0] || C.G.ib0[1] >= g.n[1] || C.G.ib0[2] >= g.n[2]) return false;
     for (int a = 0; a < 3; ++a) {
-        const double inv = 1.0 / g.d[a];
+        const double inv = g.dinv[a];
         for (int v = 0; v < 4; ++v)
             C.X[v][a] = (float)((rec[3 * v + a] + C.G.shift[a] - g.w0[a]) * inv - C.G.ib0[a]);
         C.G.x0[a] = C.X[0][a];
         C.G.x0d[a] = (rec[a] + C.G.shift[a] - g.w0[a]) * inv - C.G.ib0[a];
         for (int b = 0; b < 3; ++b) C.G.DX[a][b] = (float)((rec[3 * (b + 1) + a] - rec[a]) * inv);
         C.wlo[a] = (double)(-C.G.ib0[a]);
-        C.whi[a] = (g.w1[a] - g.w0[a]) / g.d[a] - C.G.ib0[a];
+        C.whi[a] = g.span[a] - C.G.ib0[a];
     }
     return true;
 }

@@ -315,6 +315,9 @@ struct GridInfo {
     int n[3];
     int flags;
     double w0[3], w1[3], d[3];
+    /* 1 / d and the window's extent in cells, (w1 - w0) / d, formed once
+     * (set_layout) with the same FP64 divisions a kernel would make */
+    double dinv[3], span[3];
     /* raw accumulator layout per cell */
     int nacc, ox, ov, oxx, oxv, ovv;
     int order;
@@ -322,6 +325,10 @@ struct GridInfo {
 
 /* raw accumulator offsets and moment order from g.flags */
 PVX_HD void set_layout(GridInfo& g) {
+    for (int a = 0; a < 3; ++a) {
+        g.dinv[a] = 1.0 / g.d[a];
+        g.span[a] = (g.w1[a] - g.w0[a]) / g.d[a];
+    }
     int o = 1;
     g.ox = g.ov = g.oxx = g.oxv = g.ovv = 0;
     if (g.flags & F_X) {
