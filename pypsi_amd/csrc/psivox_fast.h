@@ -396,16 +396,16 @@ PVX_HD bool plane_point(const LPoly& P, float zd0, float zd1, float zd2, float p
  * Invalid segments come back as (c, c). */
 template <bool UPPER_KEPT_IS_LOWER>
 PVX_HD bool tri_segment(const float* qc, const float* qa, const float* qb, float zc, float za, float zb, float p,
-                        float* X, float* N) {
+                        float rca, float rab, float rbc, float* X, float* N) {
     /* kept: z > p (lower plane of a slab) or z <= p (upper plane) */
     const bool ic = UPPER_KEPT_IS_LOWER ? (zc <= p) : (zc > p);
     const bool ia = UPPER_KEPT_IS_LOWER ? (za <= p) : (za > p);
     const bool ib = UPPER_KEPT_IS_LOWER ? (zb <= p) : (zb > p);
     const bool valid = (ic != ia) || (ia != ib);
     /* crossing parameters of the three edges (only used where they cross) */
-    const float tca = (za != zc) ? (p - zc) * frcp(za - zc) : 0.f;
-    const float tab = (zb != za) ? (p - za) * frcp(zb - za) : 0.f;
-    const float tbc = (zc != zb) ? (p - zb) * frcp(zc - zb) : 0.f;
+    const float tca = (za != zc) ? (p - zc) * rca : 0.f;
+    const float tab = (zb != za) ? (p - za) * rab : 0.f;
+    const float tbc = (zc != zb) ? (p - zb) * rbc : 0.f;
     float Pca[3], Pab[3], Pbc[3];
     for (int i = 0; i < 3; ++i) {
         Pca[i] = qc[i] + tca * (qa[i] - qc[i]);
@@ -424,11 +424,10 @@ PVX_HD bool tri_segment(const float* qc, const float* qa, const float* qb, float
 }
 
 /* the piece of the edge u -> v inside lo < z <= hi as parameters t0 < t1
- * (valid) of u + t (v - u) */
-PVX_HD bool edge_piece(float zu, float zv, float lo, float hi, float& t0, float& t1) {
+ * (valid) of u + t (v - u); inv = 1 / (zv - zu) (read only when zv != zu) */
+PVX_HD bool edge_piece(float zu, float zv, float inv, float lo, float hi, float& t0, float& t1) {
     const float dz = zv - zu;
     if (dz != 0.f) {
-        const float inv = frcp(dz);
         const float tl = (lo - zu) * inv, th = (hi - zu) * inv;
         t0 = fmaxf(0.f, fminf(tl, th));
         t1 = fminf(1.f, fmaxf(tl, th));
@@ -530,11 +529,15 @@ PVX_HD void pair_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
     const float qc[3] = {pc0 - O[0], pc1 - O[1], pc2 - O[2]};
     const float zb = zd0 * pb0 + zd1 * pb1 + zd2 * pb2;
     const float zc = zd0 * pc0 + zd1 * pc1 + zd2 * pc2;
+    /* the edges' reciprocal z-extents, shared by the edge pieces and the
+     * plane segments */
+    const float rca = za != zc ? frcp(za - zc) : 0.f, rab = zb != za ? frcp(zb - za) : 0.f,
+                rbc = zc != zb ? frcp(zc - zb) : 0.f;
     /* the edge pieces inside the slab, in traversal order */
     float s0, s1, u0, u1, v0, v1;
-    const bool eca = edge_piece(zc, za, lo, hi, s0, s1);
-    const bool eab = edge_piece(za, zb, lo, hi, u0, u1);
-    const bool ebc = edge_piece(zb, zc, lo, hi, v0, v1);
+    const bool eca = edge_piece(zc, za, rca, lo, hi, s0, s1);
+    const bool eab = edge_piece(za, zb, rab, lo, hi, u0, u1);
+    const bool ebc = edge_piece(zb, zc, rbc, lo, hi, v0, v1);
     (void)u1;
     float A1[3], B1[3], A3[3], B3[3], Gp[3];
     for (int i = 0; i < 3; ++i) {
@@ -549,12 +552,12 @@ PVX_HD void pair_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
     tri_acc_w<ORDER>(M, Gp, B3, A3, ebc ? 1.f : 0.f);
     if (S.has_lo) {
         float X[3], N[3];
-        const bool v = tri_segment<false>(qc, qa, qb, zc, za, zb, lo, X, N);
+        const bool v = tri_segment<false>(qc, qa, qb, zc, za, zb, lo, rca, rab, rbc, X, N);
         tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
     }
     if (S.has_hi) {
         float X[3], N[3];
-        const bool v = tri_segment<true>(qc, qa, qb, zc, za, zb, hi, X, N);
+        const bool v = tri_segment<true>(qc, qa, qb, zc, za, zb, hi, rca, rab, rbc, X, N);
         tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
         if (S.has_lo) tri_acc_w<ORDER>(M, S.gh, X, N, v ? 1.f : 0.f);
     }
