@@ -1632,10 +1632,26 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
     return 0;
 }
 
+/* the x-slab step table in the current device's constant memory, once per
+ * device (every thread's calls see it once the first upload returned) */
+int upload_tables(int dev) {
+    static std::mutex mu;
+    static bool done[256];
+    if (dev < 0 || dev >= 256) return set_err(PSIVOX_E_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (done[dev]) return 0;
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(slab_steps), host_slab_steps(), sizeof(SlabStep) * 81, 0,
+                             hipMemcpyHostToDevice));
+    done[dev] = true;
+    return 0;
+}
+
 int prepare(Workspace& W, const psivox_opts* o, hipStream_t* s, int* dev) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(PSIVOX_E_NODEV, "no GPU visible");
     HIPCHK(hipGetDevice(dev));
+    int rc = upload_tables(*dev);
+    if (rc) return rc;
     if (o && o->stream) {
         *s = (hipStream_t)o->stream;
     } else {

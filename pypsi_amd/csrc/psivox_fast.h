@@ -188,30 +188,16 @@ PVX_HD float sdist(const double* rec, const AxisMap& m, double c, float off, flo
  * is renumbered.  Only registers and the lane's LDS slice are used.  Returns
  * false on slot overflow or a broken graph.
  */
-PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int ax, bool keep_upper) {
+/* The graph part of a clip: drop the vertices in gone (a non-empty proper
+ * subset of P's), put a new vertex into a free slot on each edge from a
+ * survivor vc to a dropped vn (place(vx, vc, vn) sets its q), link the new
+ * vertices around their faces, and keep the order: survivors in their order,
+ * then the new vertices as they are made (the order a compacted graph has;
+ * the faces' fan apexes and the moment sums follow it).  False on slot
+ * overflow or a broken graph. */
+template <class Place>
+PVX_HD bool clip_graph(LPoly& P, uint32_t gone, Place&& place) {
     const uint32_t live = P.live;
-    if (live == 0u) return true;
-    const AxisMap m = axis_map(G, g, ax);
-    const float off = (float)(c - m.od);
-    uint32_t gone = 0;
-    for (uint32_t lm = live; lm; lm &= lm - 1u) {
-        const int v = __builtin_ctz(lm);
-        const float t = sdist(G.rec, m, c, off, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
-        const bool drop = keep_upper ? (t >= 0.0f) : (t < 0.0f);
-        if (drop) gone |= 1u << v;
-    }
-    if (gone == 0u) return true;
-    if (gone == live) {
-        P.live = 0u;
-        P.nv = 0;
-        return true;
-    }
-    /* free slots, those above the last survivor first: vertices then keep
-     * the order a compacted graph would give them (survivors in their order,
-     * new vertices after), and so the faces keep their fan apexes */
-    /* the order: survivors in their order, then the new vertices as they
-     * are made (the order a compacted graph has; the faces' fan apexes and
-     * the moment sums follow it) */
     uint64_t nord = 0;
     int nk = 0;
     for (int r = 0; r < P.nv; ++r) {
@@ -235,11 +221,7 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int 
             }
             const int vx = __builtin_ctz(avail);
             avail &= avail - 1u;
-            const float c0 = P.qa(vc, 0), c1 = P.qa(vc, 1), c2 = P.qa(vc, 2);
-            const float n0 = P.qa(vn, 0), n1 = P.qa(vn, 1), n2 = P.qa(vn, 2);
-            const float sc = sdist(G.rec, m, c, off, c0, c1, c2), sn = sdist(G.rec, m, c, off, n0, n1, n2);
-            const float inv = frcp(sc - sn);
-            P.setq(vx, inv * (n0 * sc - c0 * sn), inv * (n1 * sc - c1 * sn), inv * (n2 * sc - c2 * sn));
+            place(vx, vc, vn);
             P.setnbw(vx, pack3(vc, NBUNSET, NBUNSET));
             P.setnbr(vc, k, vx);
             fresh |= 1u << vx;
@@ -275,6 +257,40 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int 
     P.nv = nt;
     P.ord = nord;
     return true;
+}
+
+/* the point of the edge vc -> vn on the plane, from the two vertices'
+ * signed distances sc, sn (of opposite signs) */
+PVX_HD void cut_point(LPoly& P, int vx, int vc, int vn, float sc, float sn) {
+    const float c0 = P.qa(vc, 0), c1 = P.qa(vc, 1), c2 = P.qa(vc, 2);
+    const float n0 = P.qa(vn, 0), n1 = P.qa(vn, 1), n2 = P.qa(vn, 2);
+    const float inv = frcp(sc - sn);
+    P.setq(vx, inv * (n0 * sc - c0 * sn), inv * (n1 * sc - c1 * sn), inv * (n2 * sc - c2 * sn));
+}
+
+PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int ax, bool keep_upper) {
+    const uint32_t live = P.live;
+    if (live == 0u) return true;
+    const AxisMap m = axis_map(G, g, ax);
+    const float off = (float)(c - m.od);
+    uint32_t gone = 0;
+    for (uint32_t lm = live; lm; lm &= lm - 1u) {
+        const int v = __builtin_ctz(lm);
+        const float t = sdist(G.rec, m, c, off, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
+        const bool drop = keep_upper ? (t >= 0.0f) : (t < 0.0f);
+        if (drop) gone |= 1u << v;
+    }
+    if (gone == 0u) return true;
+    if (gone == live) {
+        P.live = 0u;
+        P.nv = 0;
+        return true;
+    }
+    return clip_graph(P, gone, [&](int vx, int vc, int vn) {
+        const float sc = sdist(G.rec, m, c, off, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
+        const float sn = sdist(G.rec, m, c, off, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
+        cut_point(P, vx, vc, vn, sc, sn);
+    });
 }
 
 /* q-moments accumulator about an origin: m0 = 6 vol, m1 = sum d S / 4,
@@ -956,14 +972,28 @@ PVX_HD int column_cells(const CopySetup& C, int il, int jl, int* nvert = nullptr
     return k1 > k0 ? k1 - k0 : 1;
 }
 
-/* Build the column polyhedron of task (copy C, cell i, j) in P: the tet
- * clipped by the column's four planes -- interior grid planes or, on the
- * copy's first/last slab or column, the window planes it crosses
- * (src/geometry.c:816-843).  False on a vertex-budget overflow. */
-PVX_HD bool build_column(LPoly& P, const CopySetup& C, const GridInfo& g, int i, int j, int j0, int j1) {
-    const CopyGeo& G = C.G;
-    const int il = i - G.ib0[0], jl = j - G.ib0[1];
-    const bool xlo = i > G.ib0[0], xhi = i < C.ib1[0] - 1, ylo = j > j0, yhi = j < j1 - 1;
+/*
+ * The x-slab step of a column, tabulated.  Clipping the tet by its upper x
+ * plane and then its lower one gives a graph that depends only on where
+ * each vertex lies: state 0 dropped by the lower plane, 1 kept, 2 dropped by
+ * the upper plane; 81 patterns, pattern = sum of state_v 3^v.  Each entry is
+ * the result of clip_graph on the tet for that pattern: per output vertex
+ * (rank order) its slot, its link word, and its source (kind 0: tet vertex
+ * vc; kind 1: cut of the upper plane on the edge from tet vertex vc to tet
+ * vertex vn; kind 2: cut of the lower plane on the edge from slot vc to tet
+ * vertex vn).  The device evaluates the cuts with the generic clip's own
+ * formula in the generic clip's sequence, so the result is the same
+ * polyhedron.
+ */
+struct SlabStep {
+    uint32_t link[8];
+    uint8_t slot[8];
+    uint8_t src[8]; /* kind in bits 6-7, vc in bits 2-5, vn in bits 0-1 */
+    uint8_t nv;
+    uint8_t pad[15];
+};
+
+PVX_HD void tet_start(LPoly& P) {
     P.live = 0xfu;
     P.nv = 4;
     P.ord = 0x3210ull;
@@ -975,15 +1005,144 @@ PVX_HD bool build_column(LPoly& P, const CopySetup& C, const GridInfo& g, int i,
     P.setnbw(1, pack3(2, 3, 0));
     P.setnbw(2, pack3(0, 3, 1));
     P.setnbw(3, pack3(1, 2, 0));
+}
+
+/* one table entry, from the graph steps alone (no coordinates) */
+inline bool make_slab_step(int pattern, SlabStep& E) {
+    static uint32_t mem[NVF * VSTRIDE];
+    LPoly P{mem, 0};
+    tet_start(P);
+    uint8_t src[NVF] = {0};
+    for (int v = 0; v < 4; ++v) src[v] = (uint8_t)(v << 2);
+    int st[4];
+    for (int v = 0, x = pattern; v < 4; ++v, x /= 3) st[v] = x % 3;
+    E = SlabStep{};
+    uint32_t gone = 0;
+    for (int v = 0; v < 4; ++v) gone |= (st[v] == 2 ? 1u : 0u) << v;
+    if (gone == P.live) return true;
+    if (gone && !clip_graph(P, gone, [&](int vx, int vc, int vn) {
+            src[vx] = (uint8_t)(64 | (vc << 2) | vn);
+        }))
+        return false;
+    gone = 0;
+    for (int v = 0; v < 4; ++v) gone |= (st[v] == 0 && ((P.live >> v) & 1u) ? 1u : 0u) << v;
+    if (gone == P.live) return true;
+    if (gone && !clip_graph(P, gone, [&](int vx, int vc, int vn) {
+            src[vx] = (uint8_t)(128 | (vc << 2) | vn);
+        }))
+        return false;
+    if (P.nv > 8) return false;
+    E.nv = (uint8_t)P.nv;
+    for (int r = 0; r < P.nv; ++r) {
+        const int sl = P.at(r);
+        E.slot[r] = (uint8_t)sl;
+        E.src[r] = src[sl];
+        E.link[r] = P.nbw(sl) & 0x7fffu;
+    }
+    return true;
+}
+
+/* the 81 entries, built once on the host */
+inline const SlabStep* host_slab_steps() {
+    static SlabStep T[81];
+    static bool made = false;
+    if (!made) {
+        for (int p = 0; p < 81; ++p) make_slab_step(p, T[p]);
+        made = true;
+    }
+    return T;
+}
+
+#ifdef __HIPCC__
+__constant__ SlabStep slab_steps[81];
+#endif
+
+PVX_HD const SlabStep& slab_step(int p) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return slab_steps[p];
+#else
+    return host_slab_steps()[p];
+#endif
+}
+
+/* P = the tet clipped by the upper x plane chi (when has_hi) and then the
+ * lower one clo (when has_lo), from the table.  False when the table does
+ * not apply (a cut of the upper plane on the dropped side of the lower one:
+ * a window slab thinner than FP32 resolves); P then holds the tet. */
+PVX_HD bool slab_from_table(LPoly& P, const CopyGeo& G, const GridInfo& g, bool has_hi, double chi, bool has_lo,
+                            double clo) {
+    tet_start(P);
+    const AxisMap m = axis_map(G, g, 0);
+    const float offh = (float)(chi - m.od), offl = (float)(clo - m.od);
+    float th[4], tl[4];
+    int pat = 0;
+    for (int v = 3; v >= 0; --v) {
+        const float q0 = v == 1 ? 1.f : 0.f, q1 = v == 2 ? 1.f : 0.f, q2 = v == 3 ? 1.f : 0.f;
+        th[v] = has_hi ? sdist(G.rec, m, chi, offh, q0, q1, q2) : 1.f;
+        tl[v] = has_lo ? sdist(G.rec, m, clo, offl, q0, q1, q2) : -1.f;
+        pat = 3 * pat + (th[v] < 0.0f ? 2 : (tl[v] >= 0.0f ? 0 : 1));
+    }
+    const SlabStep& E = slab_step(pat);
+    if (E.nv == 0) {
+        P.live = 0u;
+        P.nv = 0;
+        return true;
+    }
+    const int nv = E.nv;
+    /* upper cuts (between tet vertices), as the generic clip makes them */
     bool ok = true;
-    if (xhi)
-        ok = clip_f(P, G, g, (double)(il + 1), 0, false);
-    else if (C.wmask & 2u)
-        ok = clip_f(P, G, g, C.whi[0], 0, false);
-    if (ok && xlo)
-        ok = clip_f(P, G, g, (double)il, 0, true);
-    else if (ok && (C.wmask & 1u))
-        ok = clip_f(P, G, g, C.wlo[0], 0, true);
+    for (int r = 0; r < 8; ++r) {
+        if (r >= nv || (E.src[r] >> 6) != 1) continue;
+        const int vc = (E.src[r] >> 2) & 15, vn = E.src[r] & 3, vx = E.slot[r];
+        const float sc = vc == 0 ? th[0] : (vc == 1 ? th[1] : (vc == 2 ? th[2] : th[3]));
+        const float sn = vn == 0 ? th[0] : (vn == 1 ? th[1] : (vn == 2 ? th[2] : th[3]));
+        cut_point(P, vx, vc, vn, sc, sn);
+        if (has_lo && sdist(G.rec, m, clo, offl, P.qa(vx, 0), P.qa(vx, 1), P.qa(vx, 2)) >= 0.0f) ok = false;
+    }
+    if (!ok) {
+        tet_start(P);
+        return false;
+    }
+    /* lower cuts (from a survivor to a tet vertex) */
+    for (int r = 0; r < 8; ++r) {
+        if (r >= nv || (E.src[r] >> 6) != 2) continue;
+        const int vc = (E.src[r] >> 2) & 15, vn = E.src[r] & 3, vx = E.slot[r];
+        const float sc = sdist(G.rec, m, clo, offl, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
+        const float sn = vn == 0 ? tl[0] : (vn == 1 ? tl[1] : (vn == 2 ? tl[2] : tl[3]));
+        cut_point(P, vx, vc, vn, sc, sn);
+    }
+    uint32_t live = 0;
+    uint64_t ord = 0;
+    for (int r = 0; r < 8; ++r) {
+        if (r >= nv) continue;
+        const int sl = E.slot[r];
+        P.setnbw(sl, E.link[r]);
+        live |= 1u << sl;
+        ord |= (uint64_t)sl << (4 * r);
+    }
+    P.live = live;
+    P.nv = nv;
+    P.ord = ord;
+    return true;
+}
+
+/* Build the column polyhedron of task (copy C, cell i, j) in P: the tet
+ * clipped by the column's four planes -- interior grid planes or, on the
+ * copy's first/last slab or column, the window planes it crosses
+ * (src/geometry.c:816-843).  False on a vertex-budget overflow. */
+PVX_HD bool build_column(LPoly& P, const CopySetup& C, const GridInfo& g, int i, int j, int j0, int j1) {
+    const CopyGeo& G = C.G;
+    const int il = i - G.ib0[0], jl = j - G.ib0[1];
+    const bool xlo = i > G.ib0[0], xhi = i < C.ib1[0] - 1, ylo = j > j0, yhi = j < j1 - 1;
+    /* the x-slab: from the table, or (when it does not apply) by the two
+     * generic clips */
+    const bool has_hi = xhi || (C.wmask & 2u), has_lo = xlo || (C.wmask & 1u);
+    const double chi = xhi ? (double)(il + 1) : C.whi[0], clo = xlo ? (double)il : C.wlo[0];
+    bool ok = true;
+    if (!slab_from_table(P, G, g, has_hi, chi, has_lo, clo)) {
+        if (has_hi) ok = clip_f(P, G, g, chi, 0, false);
+        if (ok && has_lo) ok = clip_f(P, G, g, clo, 0, true);
+    }
     if (ok && yhi)
         ok = clip_f(P, G, g, (double)(jl + 1), 1, false);
     else if (ok && j == C.ib1[1] - 1 && (C.wmask & 8u))
