@@ -168,6 +168,64 @@ def end_to_end_leg(pos, vel, mass, conn, gn, fields, nfrag, steps=2, warmup=1):
             "what": "PSI.voxels(grid, mesh) on host NumPy arrays: PCIe in and out included"}
 
 
+def snapshot_leg(steps=3, warmup=1):
+    """The notebook's Example 2 call pattern (tutorial/Tutorial.ipynb:160-186)
+    on a written Gadget-2 snapshot: PSI.Mesh(loader='gadget2') of a 32^3
+    Zel'dovich lattice (rms 2.0 cells, seed 0) in a 40-unit box, then
+    PSI.voxels onto 128^3 with v and vv at refinement level 2, tolerance 0.1,
+    from host arrays.  Also the loader alone on a 128^3 file (2,097,152
+    particles, 58.7 MB)."""
+    import tempfile
+
+    import numpy as np
+    import torch
+    from pypsi_amd import PSI, snapshot
+    from pypsi_amd import device as D
+    out = {"workload": "gadget2-32^3-rms2.0-box40-to-128^3-m,v,vv-L2-tol0.1"}
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "snap32")
+        snapshot.zeldovich_snapshot(fn, 32, 2.0, seed=0, box=40.0, mass1=14.96)
+        t0 = time.perf_counter()
+        mesh = PSI.Mesh(loader="gadget2", filename=fn)
+        out["load_ms"] = 1e3 * (time.perf_counter() - t0)
+        # the GPU's own fragment count for this input (the FP32 kernel's is within 1e-6 of the FP64 count)
+        dev = torch.device("cuda:0")
+        tf = {"m": torch.zeros((128,) * 3, dtype=torch.float64, device=dev),
+              "v": torch.zeros((128,) * 3 + (3,), dtype=torch.float64, device=dev),
+              "vv": torch.zeros((128,) * 3 + (3, 3), dtype=torch.float64, device=dev)}
+        dm = D.make_mesh(*(torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                           for a in (mesh.pos, mesh.vel, mesh.mass, mesh.connectivity)), periodic_box=mesh.wrapbox)
+        st = D.voxels(dm, D.make_grid((128,) * 3, mesh.wrapbox, tf),
+                      D.make_opts(refine_tolerance=0.1, refine_max_lvl=2,
+                                   stream=torch.cuda.current_stream().cuda_stream))
+        nfrag = int(st["nfrag"])
+        ts = []
+        for i in range(warmup + steps):
+            grid = PSI.Grid(n=(128,) * 3, bounds=mesh.wrapbox, fields=["v", "vv"])
+            t0 = time.perf_counter()
+            PSI.voxels(grid=grid, mesh=mesh, refine_max_lvl=2, refine_tolerance=0.1)
+            if i >= warmup:
+                ts.append(time.perf_counter() - t0)
+        ms = 1e3 * sum(ts) / len(ts)
+        out.update({"value": nfrag / (ms * 1e-3), "unit": "fragments/s", "ms_per_step": ms, "steps": steps,
+                    "gpu_nfrag": nfrag, "tets": int(st["ntets"]),
+                    "mass_rel_err": abs(1.0 - float(grid.fields["m"].sum()) / float(mesh.mass.sum())),
+                    "what": "PSI.voxels(grid, mesh) on the loaded host arrays: PCIe in and out included"})
+        del mesh
+        fn = os.path.join(d, "snap128")
+        snapshot.zeldovich_snapshot(fn, 128, 2.0, seed=0, box=40.0, mass1=14.96)
+        nbytes = os.path.getsize(fn)
+        lt = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            big = PSI.Mesh(loader="gadget2", filename=fn)
+            lt.append(time.perf_counter() - t0)
+            del big
+        out["load_128^3"] = {"ms": 1e3 * min(lt), "file_bytes": nbytes, "MB_per_s": nbytes / min(lt) / 1e6,
+                             "what": "PSI.Mesh(loader='gadget2'), particle ids in random order in the file"}
+    return out
+
+
 def plain_leg(mesh, grid, tf, stream, nfrag, steps=2, warmup=1):
     """The FP64 one-thread-per-tet kernel on the same inputs (the plain GPU
     method the FP32-clip column kernel is measured against)."""
@@ -400,7 +458,8 @@ def main():
     ap.add_argument("--count-fragments", choices=sorted(REFINED),
                     help="count a refined workload's fragments with the FP64 kernel into profiles/ and exit")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the plain-kernel, forward+adjoint, batched and refined legs")
+                    help="skip the plain-kernel, forward+adjoint, batched, refined, end-to-end and snapshot legs")
+    ap.add_argument("--only-snapshot", action="store_true", help="run the snapshot leg alone and print it")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -417,6 +476,9 @@ def main():
         sys.exit(1)
     if args.count_fragments:
         count_fragments(args.count_fragments)
+        return
+    if args.only_snapshot:
+        print(json.dumps({"snapshot": snapshot_leg()}), flush=True)
         return
 
     wname = args.workload
@@ -503,6 +565,8 @@ def main():
         out["refined"] = refined_leg(next(iter(REFINED)))
         log("[bench] end to end from host arrays")
         out["end_to_end"] = end_to_end_leg(pos, vel, mass, conn, gn, fields, nfrag)
+        log("[bench] gadget2 snapshot, Example 2 settings")
+        out["snapshot"] = snapshot_leg()
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         try:

@@ -311,3 +311,31 @@ def ref_voxels(prob: Problem, reftol=1.0, maxlvl=0, out=None, e0=0, e1=None):
     with _silence_stdout():
         L.psi_voxels(C.byref(g), C.byref(m), None, 0, reftol, maxlvl)
     return out
+
+
+def ref_load_snapshot(filename: str, loader: str = "gadget2"):
+    """The reference's own snapshot loader (src/mesh.c:35-302, peek then
+    load, as src/PSI.c:665-702) on a file: (pos, vel, mass, connectivity,
+    box) as it leaves them, or None when it reports failure."""
+    L = reflib()
+    for fn in (L.peek_gadget2, L.load_gadget2, L.load_gevolution):
+        fn.argtypes = [C.c_void_p, C.c_char_p]
+        fn.restype = C.c_int
+    m = _RMesh()
+    name = filename.encode()
+    with _silence_stdout():
+        if not L.peek_gadget2(C.byref(m), name):
+            return None
+    n = m.npart
+    pos = np.empty((n, 3))
+    vel = np.empty((n, 3))
+    mass = np.empty(n)
+    conn = np.empty((m.nelem, m.elemtype), dtype=np.int32)
+    m.pos, m.vel, m.mass, m.connectivity = (pos.ctypes.data, vel.ctypes.data, mass.ctypes.data,
+                                            conn.ctypes.data)
+    load = L.load_gevolution if loader == "gevolution" else L.load_gadget2
+    with _silence_stdout():
+        if not load(C.byref(m), name):
+            return None
+    box = np.array([[m.box[s][a] for a in range(3)] for s in range(2)])
+    return pos, vel, mass, conn, box

@@ -11,15 +11,18 @@
  *
  * Differences: voxels() also takes kernel='auto'|'plain'|'fast' and
  * xv_true_covariance=False; non-contiguous input arrays are read correctly
- * (the reference reads their raw buffer); mode='annihilation', VDF(),
- * RStarTree and the gadget2/gevolution loaders are not part of this build and
+ * (the reference reads their raw buffer); the gadget2/gevolution loaders
+ * reject malformed particle ids and non-cubic particle counts;
+ * mode='annihilation', VDF() and RStarTree are not part of this build and
  * raise NotImplementedError.
  */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
 #define NPY_NO_DEPRECATED_API NPY_1_7_API_VERSION
 #include <numpy/arrayobject.h>
+#include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "structmember.h"
@@ -232,6 +235,214 @@ static int is_f64(PyObject* o) { return PyArray_Check(o) && PyArray_TYPE((PyArra
 
 static PyObject* zeros_like_shape(int nd, npy_intp* dims) { return PyArray_ZEROS(nd, dims, NPY_FLOAT64, 0); }
 
+/* trilinear lattice connectivity: element (i, j, k) of an ne lattice (C
+ * order) takes particle ((i+ii) mod ns0, (j+jj) mod ns1, (k+kk) mod ns2) of
+ * the ns lattice as corner 4ii+2jj+kk (src/PSI.c:636-656, src/mesh.c:137-150) */
+static void lattice_connectivity(int32_t* cd, const npy_intp* ns, const npy_intp* ne) {
+    int32_t* row = cd;
+    for (npy_intp i = 0; i < ne[0]; ++i) {
+        const npy_intp ia[2] = {i * ns[1], (i + 1 == ns[0] ? 0 : i + 1) * ns[1]};
+        for (npy_intp j = 0; j < ne[1]; ++j) {
+            const npy_intp jb[2] = {j, j + 1 == ns[1] ? 0 : j + 1};
+            for (npy_intp k = 0; k < ne[2]; ++k, row += 8) {
+                const npy_intp kc[2] = {k, k + 1 == ns[2] ? 0 : k + 1};
+                for (int c = 0; c < 8; ++c)
+                    row[c] = (int32_t)((ia[c >> 2] + jb[(c >> 1) & 1]) * ns[2] + kc[c & 1]);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Snapshot loaders: Gadget-2 format-1 files, particle type 1          */
+/* (src/mesh.c:35-302, src/PSI.c:659-707)                              */
+/* ------------------------------------------------------------------ */
+
+/* The header block is 256 bytes; the loaders use npart[1] (int32 at byte
+ * 4), mass[1] (double at byte 32) and BoxSize (double at byte 128). */
+enum { HDR_BYTES = 256, HDR_NPART1 = 4, HDR_MASS1 = 32, HDR_BOX = 128 };
+
+/* one Fortran record holding exactly nbytes: both 4-byte length words must
+ * say nbytes (the block-size checks of src/mesh.c:101-121) */
+static int read_record(FILE* f, void* dst, size_t nbytes) {
+    uint32_t head, tail;
+    if (fread(&head, 4, 1, f) != 1 || (size_t)head != nbytes) return 0;
+    if (nbytes && fread(dst, 1, nbytes, f) != nbytes) return 0;
+    return fread(&tail, 4, 1, f) == 1 && (size_t)tail == nbytes;
+}
+
+/* gevolution writes particles in tiles of 4x4x4, tile (ti, tj, tk) at index
+ * ti + T tj + T^2 tk.  Inside a tile, particle f sits at 2 G(f / 8) + G(f % 8)
+ * with G(n) = n ^ (n >> 1), the 3-bit reflected Gray code, read as
+ * (i, j, k) = (bit 2, bit 1, bit 0): the order of the tables at
+ * src/mesh.c:163-164. */
+static void tile_position(int f, int* c) {
+    const int go = (f >> 3) ^ (f >> 4), gi = (f & 7) ^ ((f & 7) >> 1);
+    for (int a = 0; a < 3; ++a) c[a] = 2 * ((go >> (2 - a)) & 1) + ((gi >> (2 - a)) & 1);
+}
+
+/* element p of a gevolution mesh is the lattice cube whose corner 0 is
+ * particle p; corner 4ii+2jj+kk is the particle (ii, jj, kk) further on,
+ * across tile edges and the periodic box (src/mesh.c:168-209, 296-299) */
+static void tiled_connectivity(int32_t* cd, int64_t T) {
+    /* the in-tile cell of each slot and the slot of each cell, once */
+    int pos64[64][3], slot64[4][4][4];
+    for (int f = 0; f < 64; ++f) {
+        tile_position(f, pos64[f]);
+        slot64[pos64[f][0]][pos64[f][1]][pos64[f][2]] = f;
+    }
+    int32_t* row = cd;
+    for (int64_t tk = 0; tk < T; ++tk)
+        for (int64_t tj = 0; tj < T; ++tj)
+            for (int64_t ti = 0; ti < T; ++ti) {
+                /* per axis: this tile's coordinate and its +1 neighbour's */
+                const int64_t t[3] = {ti, tj, tk};
+                int64_t base[3][2];
+                for (int a = 0; a < 3; ++a) {
+                    base[a][0] = t[a];
+                    base[a][1] = t[a] + 1 == T ? 0 : t[a] + 1;
+                }
+                for (int f = 0; f < 64; ++f, row += 8) {
+                    const int* c = pos64[f];
+                    for (int corner = 0; corner < 8; ++corner) {
+                        const int s0 = c[0] + (corner >> 2), s1 = c[1] + ((corner >> 1) & 1), s2 = c[2] + (corner & 1);
+                        const int64_t u0 = base[0][s0 >> 2], u1 = base[1][s1 >> 2], u2 = base[2][s2 >> 2];
+                        row[corner] = (int32_t)((u0 + T * u1 + T * T * u2) * 64 + slot64[s0 & 3][s1 & 3][s2 & 3]);
+                    }
+                }
+            }
+}
+
+/* loader='gadget2' or 'gevolution': positions and velocities (float32),
+ * particle ids (int32 for gadget2, int64 for gevolution), no mass block.
+ * Particle id i goes to row i; every element gets mass[1] / npart; the
+ * mesh is periodic on [0, BoxSize]^3 (src/mesh.c:62-155, 214-302).
+ * Errors as src/PSI.c:665-702: ValueError "Bad file name" when the file
+ * does not open, "Could not read" when its blocks are not as described.
+ * Unlike the reference, ids that are not a permutation of 0..npart-1 and a
+ * particle count that is not a cube (of a multiple of 4 for gevolution's
+ * tiles) are errors too; the reference writes out of bounds or leaves
+ * connectivity rows unset there. */
+static int load_snapshot(Mesh* self, const char* loader, const char* fn) {
+    const int gev = strcmp(loader, "gevolution") == 0;
+    PySys_WriteStdout(gev ? "Using the Gevolution loader.\n" : "Using the Gadget2 loader.\n");
+    if (!fn) {
+        PyErr_SetString(PyExc_ValueError, "Bad file name: None");
+        return -1;
+    }
+    FILE* f = fopen(fn, "rb");
+    if (!f) {
+        PySys_WriteStdout("Failed to open %.900s.\n", fn);
+        PyErr_Format(PyExc_ValueError, "Bad file name: %s", fn);
+        return -1;
+    }
+    PySys_WriteStdout("Loading %.900s...\n", fn);
+    const char* why = NULL;
+    unsigned char hdr[HDR_BYTES];
+    uint32_t mark;
+    int32_t np = 0;
+    double m1 = 0.0, boxsz = 0.0;
+    if (fread(&mark, 4, 1, f) != 1 || fread(hdr, 1, HDR_BYTES, f) != HDR_BYTES || fread(&mark, 4, 1, f) != 1)
+        why = "short header";
+    else {
+        memcpy(&np, hdr + HDR_NPART1, 4);
+        memcpy(&m1, hdr + HDR_MASS1, 8);
+        memcpy(&boxsz, hdr + HDR_BOX, 8);
+        if (np < 0) why = "negative particle count";
+    }
+    const int64_t n = np;
+    int64_t side = (int64_t)floor(cbrt((double)n) + 0.5);
+    if (!why && (side * side * side != n || (gev && side % 4 != 0)))
+        why = gev ? "particle count is not a cube of a multiple of 4" : "particle count is not a cube";
+    const size_t idb = gev ? 8 : 4;
+    float *tp = NULL, *tv = NULL;
+    void* tid = NULL;
+    if (!why) {
+        tp = (float*)malloc(12 * (size_t)n + 4);
+        tv = (float*)malloc(12 * (size_t)n + 4);
+        tid = malloc(idb * (size_t)n + 8);
+        if (!tp || !tv || !tid) why = "out of host memory";
+    }
+    if (!why) {
+        int ok;
+        Py_BEGIN_ALLOW_THREADS;
+        ok = read_record(f, tp, 12 * (size_t)n) && read_record(f, tv, 12 * (size_t)n) &&
+             read_record(f, tid, idb * (size_t)n);
+        Py_END_ALLOW_THREADS;
+        if (!ok) why = "block sizes do not match the header";
+    }
+    fclose(f);
+    PyObject *pos = NULL, *vel = NULL, *mass = NULL, *conn = NULL;
+    if (!why) {
+        npy_intp d2[2] = {(npy_intp)n, 3}, dc[2] = {(npy_intp)n, 8};
+        pos = PyArray_SimpleNew(2, d2, NPY_FLOAT64);
+        vel = PyArray_SimpleNew(2, d2, NPY_FLOAT64);
+        mass = PyArray_SimpleNew(1, d2, NPY_FLOAT64);
+        conn = PyArray_SimpleNew(2, dc, NPY_INT32);
+        if (!pos || !vel || !mass || !conn) why = "out of host memory";
+    }
+    if (!why) {
+        double* pd = (double*)PyArray_DATA((PyArrayObject*)pos);
+        double* vd = (double*)PyArray_DATA((PyArrayObject*)vel);
+        double* md = (double*)PyArray_DATA((PyArrayObject*)mass);
+        int32_t* cd = (int32_t*)PyArray_DATA((PyArrayObject*)conn);
+        unsigned char* seen = (unsigned char*)calloc((size_t)n + 1, 1);
+        if (!seen) why = "out of host memory";
+        Py_BEGIN_ALLOW_THREADS;
+        for (int64_t p = 0; !why && p < n; ++p) {
+            const int64_t id = gev ? ((const int64_t*)tid)[p] : (int64_t)((const int32_t*)tid)[p];
+            if (id < 0 || id >= n || seen[id]) {
+                why = "particle ids are not a permutation of 0..npart-1";
+                break;
+            }
+            seen[id] = 1;
+            for (int a = 0; a < 3; ++a) {
+                pd[3 * id + a] = (double)tp[3 * p + a];
+                vd[3 * id + a] = (double)tv[3 * p + a];
+            }
+            md[id] = m1 / (double)n;
+        }
+        if (!why) {
+            if (gev)
+                tiled_connectivity(cd, side / 4);
+            else {
+                const npy_intp ns[3] = {(npy_intp)side, (npy_intp)side, (npy_intp)side};
+                lattice_connectivity(cd, ns, ns);
+            }
+        }
+        Py_END_ALLOW_THREADS;
+        free(seen);
+    }
+    free(tp);
+    free(tv);
+    free(tid);
+    if (why) {
+        Py_XDECREF(pos);
+        Py_XDECREF(vel);
+        Py_XDECREF(mass);
+        Py_XDECREF(conn);
+        if (!PyErr_Occurred()) {
+            PySys_WriteStdout("%.900s: %s\n", fn, why);
+            PyErr_Format(PyExc_ValueError, "Could not read %s", fn);
+        }
+        return -1;
+    }
+    Py_XSETREF(self->pos, pos);
+    Py_XSETREF(self->vel, vel);
+    Py_XSETREF(self->mass, mass);
+    Py_XSETREF(self->connectivity, conn);
+    self->npart = n;
+    self->nelem = n;
+    self->elemtype = 8;
+    self->periodic = 1;
+    for (int a = 0; a < 3; ++a) {
+        self->box[0][a] = 0.0;
+        self->box[1][a] = boxsz;
+    }
+    PySys_WriteStdout("...done.\n");
+    return 0;
+}
+
 static int Mesh_init(Mesh* self, PyObject* args, PyObject* kwds) {
     char *cloader, *cfile = NULL;
     PyObject *posar = NULL, *velar = NULL, *massar = NULL, *connar = NULL, *box = NULL;
@@ -410,23 +621,10 @@ static int Mesh_init(Mesh* self, PyObject* args, PyObject* kwds) {
         npy_intp dc[2] = {(npy_intp)self->nelem, 8};
         PyObject* c = PyArray_SimpleNew(2, dc, NPY_INT32);
         if (!c) return -1;
-        int32_t* cd = (int32_t*)PyArray_DATA((PyArrayObject*)c);
-        for (npy_intp i = 0; i < ne[0]; ++i)
-            for (npy_intp j = 0; j < ne[1]; ++j)
-                for (npy_intp k = 0; k < ne[2]; ++k) {
-                    int64_t el = ((int64_t)i * ne[1] + j) * ne[2] + k;
-                    for (int ii = 0; ii < 2; ++ii)
-                        for (int jj = 0; jj < 2; ++jj)
-                            for (int kk = 0; kk < 2; ++kk)
-                                cd[8 * el + 4 * ii + 2 * jj + kk] =
-                                    (int32_t)(ns[1] * ns[2] * ((i + ii) % ns[0]) + ns[2] * ((j + jj) % ns[1]) +
-                                              ((k + kk) % ns[2]));
-                }
+        lattice_connectivity((int32_t*)PyArray_DATA((PyArrayObject*)c), ns, ne);
         Py_XSETREF(self->connectivity, c);
     } else if (strcmp(cloader, "gadget2") == 0 || strcmp(cloader, "gevolution") == 0) {
-        PyErr_Format(PyExc_NotImplementedError, "loader '%s' is not part of this build; load the particles with "
-                     "your own reader and use loader='array' or 'block'", cloader);
-        return -1;
+        if (load_snapshot(self, cloader, cfile) < 0) return -1;
     } else {
         PyErr_SetString(PyExc_ValueError, "Invalid mesh loader.");
         return -1;
