@@ -226,6 +226,46 @@ def snapshot_leg(steps=3, warmup=1):
     return out
 
 
+def vdf_leg(pos, vel, npts=400):
+    """PSI.RStarTree over the 128^3 mesh (host arrays; the element boxes are
+    built on the GPU), then PSI.VDF at random points of [0.1, 0.9]^3 with that
+    tree, refinement level 0; the oracle's C restatement times a few of the
+    same points (it scans every element's box per query)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pypsi_amd import PSI
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    n = round(pos.shape[0] ** (1.0 / 3.0))
+    mesh = PSI.Mesh(loader="block", pos=pos.reshape(n, n, n, 3), vel=vel.reshape(n, n, n, 3), periodic_box=box)
+    t0 = time.perf_counter()
+    tree = PSI.RStarTree(mesh)
+    build_ms = 1e3 * (time.perf_counter() - t0)
+    pts = np.random.default_rng(0).uniform(0.1, 0.9, size=(npts, 3))
+    for x in pts[:5]:
+        PSI.VDF(mesh, tuple(x), tree=tree)
+    streams = 0
+    t0 = time.perf_counter()
+    for x in pts:
+        rho, _ = PSI.VDF(mesh, tuple(x), tree=tree)
+        streams += len(rho)
+    us = 1e6 * (time.perf_counter() - t0) / npts
+    out = {"workload": f"zeldovich-{n}^3-rms2.0, VDF at {npts} random points, refine level 0",
+           "tree_build_ms": build_ms, "us_per_point": us, "points_per_s": 1e6 / us,
+           "streams_per_point": streams / npts,
+           "what": "PSI.VDF(mesh, x, tree) from host: box scan, candidate refinement and barycentric tests on the GPU"}
+    try:
+        import oracle as O
+        P = O.Problem.block(pos.reshape(n, n, n, 3), vel.reshape(n, n, n, 3), periodic_box=box, n=(8,) * 3)
+        W = O.tree_window(P)
+        t0 = time.perf_counter()
+        for x in pts[:5]:
+            O.sample_vdf(P, x, window=W)
+        out["cpu_oracle_us_per_point"] = 1e6 * (time.perf_counter() - t0) / 5
+    except Exception as e:  # report, never hide
+        out["cpu_oracle_error"] = repr(e)
+    return out
+
+
 def plain_leg(mesh, grid, tf, stream, nfrag, steps=2, warmup=1):
     """The FP64 one-thread-per-tet kernel on the same inputs (the plain GPU
     method the FP32-clip column kernel is measured against)."""
@@ -459,7 +499,7 @@ def main():
                     help="count a refined workload's fragments with the FP64 kernel into profiles/ and exit")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the plain-kernel, forward+adjoint, batched, refined, end-to-end and snapshot legs")
-    ap.add_argument("--only-snapshot", action="store_true", help="run the snapshot leg alone and print it")
+    ap.add_argument("--only", choices=["snapshot", "vdf"], help="run one extra leg alone and print it")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -477,13 +517,16 @@ def main():
     if args.count_fragments:
         count_fragments(args.count_fragments)
         return
-    if args.only_snapshot:
+    if args.only == "snapshot":
         print(json.dumps({"snapshot": snapshot_leg()}), flush=True)
         return
 
     wname = args.workload
     log(f"[bench] building inputs for {wname}")
     pos, vel, mass, conn, gn, fields = build_inputs(wname)
+    if args.only == "vdf":
+        print(json.dumps({"vdf": vdf_leg(pos, vel)}), flush=True)
+        return
     box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
     dev = torch.device("cuda:0")
     tpos = torch.from_numpy(pos).to(dev)
@@ -567,6 +610,8 @@ def main():
         out["end_to_end"] = end_to_end_leg(pos, vel, mass, conn, gn, fields, nfrag)
         log("[bench] gadget2 snapshot, Example 2 settings")
         out["snapshot"] = snapshot_leg()
+        log("[bench] RStarTree + VDF point queries")
+        out["vdf"] = vdf_leg(pos, vel)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         try:

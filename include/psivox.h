@@ -143,6 +143,41 @@ int psivox_voxels_batched(int32_t nmesh, const psivox_mesh* meshes, psivox_grid*
 int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,
                         const psivox_opts* opts);
 
+/* ---- point queries: RStarTree and PSI.VDF ---- */
+
+/* An index of the mesh's element boxes on the GPU.  Replaces the R*-tree
+ * (psi_rtree, include/rtree.h:40-71) that psi_rtree_from_mesh builds
+ * (src/rtree.c:48-73): every element's vertex box after the periodic wrap of
+ * psi_aabb_periodic (src/cpsi.c:309-341), elements that test outside the
+ * window left out.  The index keeps a device copy of a host mesh (a
+ * snapshot: later changes to the host arrays are not seen); a device mesh's
+ * arrays must outlive the index.  Build, query and sample on the device that
+ * was current at build time; stream may be NULL (library stream). */
+typedef struct psivox_elem_index psivox_elem_index;
+
+int psivox_elem_index_build(const psivox_mesh* mesh, const double window[2][3], void* stream,
+                            psivox_elem_index** out);
+void psivox_elem_index_free(psivox_elem_index* idx);
+
+/* The elements whose box meets the closed box q -- RStarTree.query
+ * (src/PSI.c:817-861; psi_rtree_query_init/next, src/rtree.c:75-123) -- in
+ * ascending element order (the reference returns them in its tree's order).
+ * Writes min(*n, cap) indices to the host array out and the count to *n. */
+int psivox_elem_index_query(psivox_elem_index* idx, const double q[2][3], int32_t* out, int64_t cap, int64_t* n,
+                            void* stream);
+
+/* The multistream sample at a point -- psi_sample_vdf (src/cpsi.c:177-275),
+ * PSI.VDF (src/PSI.c:1022-1058).  Each element whose box holds the point is
+ * refined on its raw vertex positions as psi_voxels refines (tolerance in
+ * position units, src/cpsi.c:197; level clamped to 6); every tet holding the
+ * point (barycentric coordinates >= 0, nonzero volume; src/geometry.c:335-389)
+ * is one stream: rho = tet mass / tet volume, vel = barycentric interpolation
+ * of its vertex velocities.  Streams in ascending element order, tets in
+ * refinement order.  Writes min(*n, cap) streams (rho[i]; vel[3i..3i+3)) to
+ * host arrays and the count to *n. */
+int psivox_sample_vdf(psivox_elem_index* idx, const double point[3], double refine_tolerance,
+                      int32_t refine_max_lvl, double* rho, double* vel, int64_t cap, int64_t* n, void* stream);
+
 const char* psivox_last_error(void);
 int psivox_abi_version(void);
 /* number of visible GPUs (0 if none); never throws */

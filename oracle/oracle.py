@@ -64,6 +64,10 @@ def lib():
         L.orc_voxelize_one_tet.argtypes = [P, P, P, P, C.c_double, C.c_int, P]
         L.orc_mass_adjoint.argtypes = [P, P, P, P]
         L.orc_fd_mass_loss.argtypes = [P, P, P, P, C.c_int, C.c_int64, C.c_int, C.c_double, P]
+        L.orc_elem_query.argtypes = [P, P, P, P, C.c_int64]
+        L.orc_elem_query.restype = C.c_int64
+        L.orc_sample_vdf.argtypes = [P, P, P, C.c_double, C.c_int, P, P, C.c_int64]
+        L.orc_sample_vdf.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -339,3 +343,123 @@ def ref_load_snapshot(filename: str, loader: str = "gadget2"):
             return None
     box = np.array([[m.box[s][a] for a in range(3)] for s in range(2)])
     return pos, vel, mass, conn, box
+
+
+# ---------------------------------------------------------------------------
+# point queries: element boxes and the multistream sample (PSI.VDF)
+# ---------------------------------------------------------------------------
+
+def _box2(w):
+    W = ((C.c_double * 3) * 2)()
+    for s in range(2):
+        for a in range(3):
+            W[s][a] = float(w[s][a])
+    return W
+
+
+def tree_window(prob: Problem):
+    """The window RStarTree(mesh) builds its tree with (src/PSI.c:875-886):
+    the periodic box, or +-1e30 for a non-periodic mesh."""
+    if prob.periodic:
+        return prob.box.copy()
+    return np.array([[-1.0e30] * 3, [1.0e30] * 3])
+
+
+def vdf_window(prob: Problem):
+    """The window psi_sample_vdf builds its internal tree with
+    (src/cpsi.c:203-208): the mesh's box, all zeros for a non-periodic mesh."""
+    return prob.box.copy()
+
+
+def elem_query(prob: Problem, window, q):
+    """Elements whose wrapped box meets the closed box q, ascending."""
+    L = lib()
+    m = prob._omesh()
+    W, Q = _box2(window), _box2(q)
+    n = L.orc_elem_query(C.byref(m), C.byref(W), C.byref(Q), None, 0)
+    out = np.empty(n, dtype=np.int32)
+    L.orc_elem_query(C.byref(m), C.byref(W), C.byref(Q), out.ctypes.data, n)
+    return out
+
+
+def sample_vdf(prob: Problem, x, reftol=1.0, maxlvl=0, window=None):
+    """(rho, vel) streams at x, ascending element order."""
+    L = lib()
+    m = prob._omesh()
+    W = _box2(vdf_window(prob) if window is None else window)
+    X = (C.c_double * 3)(*[float(v) for v in x])
+    n = L.orc_sample_vdf(C.byref(m), C.byref(W), X, reftol, maxlvl, None, None, 0)
+    rho = np.empty(n)
+    vel = np.empty((n, 3))
+    L.orc_sample_vdf(C.byref(m), C.byref(W), X, reftol, maxlvl, rho.ctypes.data, vel.ctypes.data, n)
+    return rho, vel
+
+
+class _RVec(C.Structure):  # psi_rvec, include/psi.h:84-87
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
+def _rmesh(prob: Problem):
+    m = _RMesh()
+    m.elemtype = prob.elemtype
+    m.dim = 3
+    m.pos = prob.pos.ctypes.data
+    m.vel = prob.vel.ctypes.data
+    m.mass = prob.mass.ctypes.data
+    m.npart = prob.pos.shape[0]
+    m.nelem = prob.conn.shape[0]
+    m.connectivity = prob.conn.ctypes.data
+    m.periodic = int(prob.periodic)
+    for s in range(2):
+        for a in range(3):
+            m.box[s][a] = prob.box[s, a]
+    return m
+
+
+def ref_sample_vdf(prob: Problem, x, reftol=1.0, maxlvl=0):
+    """The reference's psi_sample_vdf (src/cpsi.c:177-275) with its internal
+    R*-tree: (rho, vel) in its tree's element order."""
+    L = reflib()
+    P = C.c_void_p
+    L.psi_sample_vdf.argtypes = [_RVec, P, P, P, P, P, C.c_double, C.c_int]
+    L.psi_sample_vdf.restype = None
+    m = _rmesh(prob)
+    rp, vp = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    n = C.c_int(0)
+    with _silence_stdout():
+        L.psi_sample_vdf(_RVec(*[float(v) for v in x]), C.byref(m), None, C.byref(rp), C.byref(vp), C.byref(n),
+                         reftol, maxlvl)
+    k = n.value
+    rho = np.array([rp[i] for i in range(k)])
+    vel = np.array([vp[i] for i in range(3 * k)]).reshape(k, 3)
+    libc = C.CDLL(None)
+    libc.free.argtypes = [P]
+    libc.free(C.cast(rp, P))
+    libc.free(C.cast(vp, P))
+    return rho, vel
+
+
+def ref_tree_query(prob: Problem, window, q):
+    """The reference's R*-tree (src/rtree.c) built from the mesh with window
+    and queried with the closed box q: element indices in its order."""
+    L = reflib()
+    P = C.c_void_p
+    L.psi_rtree_from_mesh.argtypes = [P, P, P]
+    L.psi_rtree_query_init.argtypes = [P, P, P]
+    L.psi_rtree_query_next.argtypes = [P, P]
+    L.psi_rtree_query_next.restype = C.c_int
+    L.psi_rtree_destroy.argtypes = [P]
+    m = _rmesh(prob)
+    tree = (C.c_char * 256)()  # psi_rtree, include/rtree.h:53-60 (32 bytes)
+    qry = (C.c_char * 16384)()  # psi_rtree_query: a 1024-entry stack and two pointers
+    W = (_RVec * 2)(_RVec(*window[0]), _RVec(*window[1]))
+    Q = (_RVec * 2)(_RVec(*q[0]), _RVec(*q[1]))
+    with _silence_stdout():
+        L.psi_rtree_from_mesh(tree, C.byref(m), W)
+    out = []
+    e = C.c_int(0)
+    L.psi_rtree_query_init(qry, tree, Q)
+    while L.psi_rtree_query_next(qry, C.byref(e)):
+        out.append(e.value)
+    L.psi_rtree_destroy(tree)
+    return np.array(out, dtype=np.int32)

@@ -1074,3 +1074,107 @@ int64_t orc_internal_tet_fragments(const orc_mesh* mesh, const orc_grid* g, cons
     free(S);
     return ctx.n;
 }
+
+/* ------------------------------------------------------------------ */
+/* point queries: element boxes (src/rtree.c:48-123) and the           */
+/* multistream sample (src/cpsi.c:177-275)                             */
+/* ------------------------------------------------------------------ */
+
+/* element e's box after the periodic wrap of psi_aabb_periodic
+ * (src/cpsi.c:309-341) on a copy of its vertices; 0 when the window test
+ * leaves it out of the tree (src/rtree.c:62-70) */
+static int o_elem_box(const orc_mesh* m, const double win[2][3], int64_t e, double lo[3], double hi[3]) {
+    double p[27][3];
+    for (int v = 0; v < m->elemtype; ++v) {
+        int64_t id = m->conn[e * m->elemtype + v];
+        for (int a = 0; a < 3; ++a) p[v][a] = m->pos[3 * id + a];
+    }
+    orc_grid g;
+    memset(&g, 0, sizeof g);
+    for (int s = 0; s < 2; ++s)
+        for (int a = 0; a < 3; ++a) g.win[s][a] = win[s][a];
+    return o_wrap_and_test(p, m->elemtype, lo, hi, &g, m);
+}
+
+/* closed boxes meet (src/rtree.c:106-110) */
+static int o_meets(const double lo[3], const double hi[3], const double q[2][3]) {
+    for (int a = 0; a < 3; ++a)
+        if (q[1][a] < lo[a] || q[0][a] > hi[a]) return 0;
+    return 1;
+}
+
+int64_t orc_elem_query(const orc_mesh* m, const double win[2][3], const double q[2][3], int32_t* out, int64_t cap) {
+    int64_t n = 0;
+    double lo[3], hi[3];
+    for (int64_t e = 0; e < m->nelem; ++e) {
+        if (!o_elem_box(m, win, e, lo, hi) || !o_meets(lo, hi, q)) continue;
+        if (n < cap) out[n] = (int32_t)e;
+        n++;
+    }
+    return n;
+}
+
+/* barycentric coordinates of x in tet p by Cramer's rule on the edges from
+ * vertex 0 (b0 = 1 - b1 - b2 - b3) and the volume |det| / 6: the quantities
+ * of psi_barycentric (src/geometry.c:335-389) */
+static double o_det3(const double* a, const double* b, const double* c) {
+    return a[0] * (b[1] * c[2] - b[2] * c[1]) - b[0] * (a[1] * c[2] - a[2] * c[1]) +
+           c[0] * (a[1] * b[2] - a[2] * b[1]);
+}
+
+static double o_bary(const double* p, const double* x, double* b) {
+    double e1[3], e2[3], e3[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+        e1[a] = p[3 + a] - p[a];
+        e2[a] = p[6 + a] - p[a];
+        e3[a] = p[9 + a] - p[a];
+        d[a] = x[a] - p[a];
+    }
+    const double D = o_det3(e1, e2, e3);
+    b[1] = o_det3(d, e2, e3) / D;
+    b[2] = o_det3(e1, d, e3) / D;
+    b[3] = o_det3(e1, e2, d) / D;
+    b[0] = 1.0 - b[1] - b[2] - b[3];
+    return fabs(D) / 6.0;
+}
+
+/* src/cpsi.c:177-275: every element whose box holds x (ascending index),
+ * refined on its raw vertices with tolerance reftol in position units
+ * (src/cpsi.c:197), and every tet holding x with nonzero volume gives
+ * rho = mass / volume and the barycentric velocity */
+int64_t orc_sample_vdf(const orc_mesh* m, const double win[2][3], const double* x, double reftol, int maxlvl,
+                       double* rho, double* vel, int64_t cap) {
+    oscratch S;
+    o_scratch_init(&S);
+    if (maxlvl > 6) maxlvl = 6;
+    const double q[2][3] = {{x[0], x[1], x[2]}, {x[0], x[1], x[2]}};
+    double lo[3], hi[3], p[27][3], u[27][3];
+    int64_t n = 0;
+    for (int64_t e = 0; e < m->nelem; ++e) {
+        if (!o_elem_box(m, win, e, lo, hi) || !o_meets(lo, hi, q)) continue;
+        for (int v = 0; v < m->elemtype; ++v) {
+            int64_t id = m->conn[e * m->elemtype + v];
+            for (int a = 0; a < 3; ++a) {
+                p[v][a] = m->pos[3 * id + a];
+                u[v][a] = m->vel ? m->vel[3 * id + a] : 0.0;
+            }
+        }
+        o_refine(&S, (const double (*)[3])p, (const double (*)[3])u, m->mass[e], m->elemtype, maxlvl,
+                 reftol * reftol);
+        for (int t = 0; t < S.ntet; ++t) {
+            double b[4];
+            const double vol = o_bary(&S.tpos[12 * t], x, b);
+            if (vol <= 0.0) continue;
+            if (b[0] < 0.0 || b[1] < 0.0 || b[2] < 0.0 || b[3] < 0.0) continue;
+            if (n < cap) {
+                rho[n] = S.tmass[t] / vol;
+                for (int a = 0; a < 3; ++a)
+                    vel[3 * n + a] = b[0] * S.tvel[12 * t + a] + b[1] * S.tvel[12 * t + 3 + a] +
+                                     b[2] * S.tvel[12 * t + 6 + a] + b[3] * S.tvel[12 * t + 9 + a];
+            }
+            n++;
+        }
+    }
+    o_scratch_free(&S);
+    return n;
+}

@@ -49,6 +49,15 @@ int orc_count_tets(const orc_mesh* mesh, const orc_grid* g, double reftol, int m
 int orc_voxelize_one_tet(const orc_mesh* mesh, orc_grid* g, const double* pos12, const double* vel12, double mass,
                          int xvfix, orc_stats* st);
 
+/* point queries: elements whose wrapped box (window test as the R*-tree
+ * build) meets the closed box q, ascending; returns the count, writes up to
+ * cap indices */
+int64_t orc_elem_query(const orc_mesh* m, const double win[2][3], const double q[2][3], int32_t* out, int64_t cap);
+/* the multistream sample at x (PSI.VDF); returns the stream count, writes
+ * up to cap (rho, vel[3]) */
+int64_t orc_sample_vdf(const orc_mesh* m, const double win[2][3], const double* x, double reftol, int maxlvl,
+                       double* rho, double* vel, int64_t cap);
+
 /* mass adjoint (psi_adjoint.c), refine_max_lvl = 0: dpos = dL/dpos
  * (npart x 3, overwritten) from w = dL/dm per cell */
 int orc_mass_adjoint(const orc_mesh* mesh, const orc_grid* g, const double* w, double* dpos);

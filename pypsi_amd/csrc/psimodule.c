@@ -13,8 +13,10 @@
  * xv_true_covariance=False; non-contiguous input arrays are read correctly
  * (the reference reads their raw buffer); the gadget2/gevolution loaders
  * reject malformed particle ids and non-cubic particle counts;
- * mode='annihilation', VDF() and RStarTree are not part of this build and
- * raise NotImplementedError.
+ * RStarTree.query() returns its elements in ascending order (the reference:
+ * in its tree's order) and VDF() its streams in ascending element order;
+ * mode='annihilation' is not part of this build and raises
+ * NotImplementedError.
  */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
@@ -818,12 +820,232 @@ done:
     return ret;
 }
 
-static PyObject* PSI_VDF(PyObject* self, PyObject* args, PyObject* kwds) {
-    (void)self;
+/* ------------------------------------------------------------------ */
+/* RStarTree (src/PSI.c:808-935): the mesh's element boxes, indexed on */
+/* the GPU (psivox_elem_index_*)                                       */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    PyObject_HEAD psivox_elem_index* idx;
+    PyObject* mesh; /* the Mesh it was built from */
+} RStarTree;
+
+/* an element index of mesh on the GPU with the given window */
+static int mesh_index(Mesh* mesh, const double win[2][3], psivox_elem_index** out, const char* who) {
+    *out = NULL;
+    if (mesh->pos == Py_None) {
+        PyErr_Format(PyExc_ValueError, "%s: the mesh holds no particles", who);
+        return -1;
+    }
+    if (psivox_device_count() <= 0) {
+        PyErr_Format(PyExc_RuntimeError, "%s: no GPU visible to the MI355X voxelizer", who);
+        return -1;
+    }
+    int ret = -1;
+    PyObject* pos = contiguous(mesh->pos);
+    PyObject* vel = contiguous(mesh->vel);
+    PyObject* mass = contiguous(mesh->mass);
+    PyObject* conn = contiguous(mesh->connectivity);
+    if (pos && vel && mass && conn) {
+        psivox_mesh cm;
+        memset(&cm, 0, sizeof cm);
+        cm.elemtype = mesh->elemtype;
+        cm.periodic = mesh->periodic;
+        cm.npart = mesh->npart;
+        cm.nelem = mesh->nelem;
+        cm.pos = (const double*)PyArray_DATA((PyArrayObject*)pos);
+        cm.vel = (const double*)PyArray_DATA((PyArrayObject*)vel);
+        cm.mass = (const double*)PyArray_DATA((PyArrayObject*)mass);
+        cm.conn = (const int32_t*)PyArray_DATA((PyArrayObject*)conn);
+        memcpy(cm.box, mesh->box, sizeof cm.box);
+        int rc;
+        Py_BEGIN_ALLOW_THREADS;
+        rc = psivox_elem_index_build(&cm, win, NULL, out);
+        Py_END_ALLOW_THREADS;
+        if (rc != 0)
+            PyErr_Format(rc == PSIVOX_E_NOMEM ? PyExc_MemoryError : PyExc_RuntimeError, "%s: %s", who,
+                         psivox_last_error());
+        else
+            ret = 0;
+    }
+    Py_XDECREF(pos);
+    Py_XDECREF(vel);
+    Py_XDECREF(mass);
+    Py_XDECREF(conn);
+    return ret;
+}
+
+static int RStarTree_init(RStarTree* self, PyObject* args, PyObject* kwds) {
+    Mesh* mesh = NULL;
+    int init_cap = 1024; /* accepted and unused, as in the reference (src/PSI.c:889) */
+    static char* kwlist[] = {"mesh", "initial_capacity", NULL};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "O!|i", kwlist, &MeshType, &mesh, &init_cap)) return -1;
+    (void)init_cap;
+    /* the periodic box, or an unbounded window (src/PSI.c:875-886) */
+    double win[2][3];
+    for (int a = 0; a < 3; ++a) {
+        win[0][a] = mesh->periodic ? mesh->box[0][a] : -1.0e30;
+        win[1][a] = mesh->periodic ? mesh->box[1][a] : 1.0e30;
+    }
+    psivox_elem_index* idx;
+    if (mesh_index(mesh, win, &idx, "PSI.RStarTree()") < 0) return -1;
+    psivox_elem_index_free(self->idx);
+    self->idx = idx;
+    Py_INCREF(mesh);
+    Py_XSETREF(self->mesh, (PyObject*)mesh);
+    return 0;
+}
+
+static PyObject* RStarTree_new(PyTypeObject* type, PyObject* args, PyObject* kwds) {
     (void)args;
     (void)kwds;
-    PyErr_SetString(PyExc_NotImplementedError, "PSI.VDF() is not part of this build");
-    return NULL;
+    RStarTree* self = (RStarTree*)type->tp_alloc(type, 0);
+    if (self) {
+        self->idx = NULL;
+        self->mesh = NULL;
+    }
+    return (PyObject*)self;
+}
+
+static void RStarTree_dealloc(RStarTree* self) {
+    psivox_elem_index_free(self->idx);
+    Py_XDECREF(self->mesh);
+    Py_TYPE(self)->tp_free((PyObject*)self);
+}
+
+/* query(box): the elements whose box meets the closed box, as an int32
+ * array in ascending order (src/PSI.c:817-861 returns the same elements in
+ * its tree's order) */
+static PyObject* RStarTree_query(RStarTree* self, PyObject* args, PyObject* kwds) {
+    PyObject* box;
+    double q[2][3];
+    static char* kwlist[] = {"box", NULL};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "O", kwlist, &box)) return NULL;
+    if (!PyArg_ParseTuple(box, "(ddd)(ddd)", &q[0][0], &q[0][1], &q[0][2], &q[1][0], &q[1][1], &q[1][2])) {
+        PyErr_SetString(PyExc_TypeError, "Query box must be a tuple containing two tuples of 3 values each");
+        return NULL;
+    }
+    if (!self->idx) {
+        PyErr_SetString(PyExc_RuntimeError, "RStarTree was not initialised");
+        return NULL;
+    }
+    int64_t cap = 1024, n = 0;
+    for (;;) {
+        npy_intp d = (npy_intp)cap;
+        PyObject* out = PyArray_SimpleNew(1, &d, NPY_INT32);
+        if (!out) return NULL;
+        int rc;
+        int32_t* od = (int32_t*)PyArray_DATA((PyArrayObject*)out);
+        Py_BEGIN_ALLOW_THREADS;
+        rc = psivox_elem_index_query(self->idx, (const double(*)[3])q, od, cap, &n, NULL);
+        Py_END_ALLOW_THREADS;
+        if (rc != 0) {
+            Py_DECREF(out);
+            PyErr_Format(PyExc_RuntimeError, "RStarTree.query(): %s", psivox_last_error());
+            return NULL;
+        }
+        if (n <= cap) {
+            PyArray_Dims nd = {&d, 1};
+            d = (npy_intp)n;
+            PyObject* r = PyArray_Resize((PyArrayObject*)out, &nd, 0, NPY_CORDER);
+            if (!r) {
+                Py_DECREF(out);
+                return NULL;
+            }
+            Py_DECREF(r);
+            return out;
+        }
+        Py_DECREF(out);
+        cap = n;
+    }
+}
+
+static PyMethodDef RStarTree_methods[] = {
+    {"query", (PyCFunction)(void (*)(void))RStarTree_query, METH_VARARGS | METH_KEYWORDS, "Query the RStarTree."},
+    {NULL, NULL, 0, NULL}};
+
+static PyTypeObject RStarTreeType = {
+    PyVarObject_HEAD_INIT(NULL, 0).tp_name = "PSI.RStarTree",
+    .tp_basicsize = sizeof(RStarTree),
+    .tp_dealloc = (destructor)RStarTree_dealloc,
+    .tp_flags = Py_TPFLAGS_DEFAULT | Py_TPFLAGS_BASETYPE,
+    .tp_doc = "Element bounding boxes of a Mesh, indexed on the GPU",
+    .tp_methods = RStarTree_methods,
+    .tp_init = (initproc)RStarTree_init,
+    .tp_new = RStarTree_new,
+};
+
+/* VDF(mesh, sample_pos, tree=None, refine_tolerance=1.0, refine_max_lvl=0)
+ * -> (rho, vel): the streams at sample_pos (src/PSI.c:1022-1058,
+ * src/cpsi.c:177-275).  Without a tree, the elements are indexed with the
+ * mesh's box as the window, as psi_sample_vdf does (src/cpsi.c:203-208;
+ * for a non-periodic mesh that box is all zeros, so only elements whose box
+ * holds the origin take part there too). */
+static PyObject* PSI_VDF(PyObject* self, PyObject* args, PyObject* kwds) {
+    (void)self;
+    Mesh* mesh = NULL;
+    RStarTree* tree = NULL;
+    double x[3], reftol = 1.0;
+    int maxlvl = 0;
+    static char* kwlist[] = {"mesh", "sample_pos", "tree", "refine_tolerance", "refine_max_lvl", NULL};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "O!(ddd)|O!di", kwlist, &MeshType, &mesh, &x[0], &x[1], &x[2],
+                                     &RStarTreeType, &tree, &reftol, &maxlvl))
+        return NULL;
+    psivox_elem_index* idx = NULL;
+    int own = 0;
+    if (tree) {
+        if (!tree->idx) {
+            PyErr_SetString(PyExc_RuntimeError, "RStarTree was not initialised");
+            return NULL;
+        }
+        if (tree->mesh != (PyObject*)mesh) {
+            PyErr_SetString(PyExc_ValueError, "PSI.VDF(): tree was built from a different mesh");
+            return NULL;
+        }
+        idx = tree->idx;
+    } else {
+        if (mesh_index(mesh, (const double(*)[3])mesh->box, &idx, "PSI.VDF()") < 0) return NULL;
+        own = 1;
+    }
+    PyObject *rho = NULL, *vel = NULL, *ret = NULL;
+    int64_t cap = 256, n = 0;
+    for (;;) {
+        npy_intp d[2] = {(npy_intp)cap, 3};
+        rho = PyArray_SimpleNew(1, d, NPY_FLOAT64);
+        vel = PyArray_SimpleNew(2, d, NPY_FLOAT64);
+        if (!rho || !vel) goto done;
+        int rc;
+        double* rd = (double*)PyArray_DATA((PyArrayObject*)rho);
+        double* vd = (double*)PyArray_DATA((PyArrayObject*)vel);
+        Py_BEGIN_ALLOW_THREADS;
+        rc = psivox_sample_vdf(idx, x, reftol, maxlvl, rd, vd, cap, &n, NULL);
+        Py_END_ALLOW_THREADS;
+        if (rc != 0) {
+            PyErr_Format(rc == PSIVOX_E_NOMEM ? PyExc_MemoryError : PyExc_RuntimeError, "PSI.VDF(): %s",
+                         psivox_last_error());
+            goto done;
+        }
+        if (n <= cap) break;
+        Py_CLEAR(rho);
+        Py_CLEAR(vel);
+        cap = n;
+    }
+    {
+        npy_intp d[2] = {(npy_intp)n, 3};
+        PyArray_Dims d1 = {d, 1}, d2 = {d, 2};
+        PyObject* r1 = PyArray_Resize((PyArrayObject*)rho, &d1, 0, NPY_CORDER);
+        if (!r1) goto done;
+        Py_DECREF(r1);
+        PyObject* r2 = PyArray_Resize((PyArrayObject*)vel, &d2, 0, NPY_CORDER);
+        if (!r2) goto done;
+        Py_DECREF(r2);
+    }
+    ret = Py_BuildValue("OO", rho, vel);
+done:
+    Py_XDECREF(rho);
+    Py_XDECREF(vel);
+    if (own) psivox_elem_index_free(idx);
+    return ret;
 }
 
 static PyMethodDef module_methods[] = {
@@ -844,5 +1066,8 @@ PyMODINIT_FUNC PyInit_PSI(void) {
     if (PyType_Ready(&GridType) < 0) return NULL;
     Py_INCREF(&GridType);
     PyModule_AddObject(m, "Grid", (PyObject*)&GridType);
+    if (PyType_Ready(&RStarTreeType) < 0) return NULL;
+    Py_INCREF(&RStarTreeType);
+    PyModule_AddObject(m, "RStarTree", (PyObject*)&RStarTreeType);
     return m;
 }

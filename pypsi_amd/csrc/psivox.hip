@@ -22,6 +22,8 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <new>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -32,6 +34,7 @@
 #include "psivox_plain.h"
 #include "psivox_refine.h"
 #include "psivox_moments.h"
+#include "psivox_query.h"
 
 using namespace pvx;
 
@@ -193,6 +196,68 @@ __global__ void k_expand_emit(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, int
     RefineScratch S;
     expand_element(m.pos, m.vel, m.mass, m.conn, m.vpe, e, g, mb, maxlvl, tol2, S,
                    tets + (int64_t)TREC * (offsets[e] - base));
+}
+
+/* ------------------------------------------------------------------ */
+/* element boxes and point queries (RStarTree, PSI.VDF)                */
+/* ------------------------------------------------------------------ */
+
+/* element e's box: its vertex box after the periodic wrap of
+ * psi_aabb_periodic (src/cpsi.c:309-341); valid[e] = 0 when that test
+ * rejects it for the window (src/rtree.c:62-70 leaves it out) */
+__global__ void k_elem_boxes(MeshDev m, GridInfo g, MeshBox mb, int64_t ne, double* __restrict__ box,
+                             int8_t* __restrict__ valid) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    double P[27][3], lo[3], hi[3];
+    for (int v = 0; v < m.vpe; ++v) {
+        const int64_t id = m.conn[e * m.vpe + v];
+        for (int a = 0; a < 3; ++a) P[v][a] = m.pos[3 * id + a];
+    }
+    valid[e] = (int8_t)wrap_and_test(P, m.vpe, g, mb);
+    aabb((const double (*)[3])P, m.vpe, lo, hi);
+    for (int a = 0; a < 3; ++a) {
+        box[6 * e + a] = lo[a];
+        box[6 * e + 3 + a] = hi[a];
+    }
+}
+
+/* flags[e] = element e is in the index and its box meets q */
+__global__ void k_box_meets(const double* __restrict__ box, const int8_t* __restrict__ valid, int64_t ne, QueryBox q,
+                            uint8_t* __restrict__ flags) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    flags[e] = (uint8_t)(valid[e] && boxes_meet(box + 6 * e, box + 6 * e + 3, q));
+}
+
+/* tets of the candidate elements, refined as PSI.voxels refines them but on
+ * the raw vertex positions (src/cpsi.c:217-224: no periodic wrap); g's
+ * window is unbounded so every candidate expands */
+__global__ void k_vdf_count(MeshDev m, GridInfo g, MeshBox mb, const int32_t* __restrict__ cand, int64_t nc,
+                            int maxlvl, double tol2, int64_t* __restrict__ counts) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    RefineScratch S;
+    counts[c] = expand_element(m.pos, m.vel, m.mass, m.conn, m.vpe, cand[c], g, mb, maxlvl, tol2, S, nullptr);
+}
+
+__global__ void k_vdf_emit(MeshDev m, GridInfo g, MeshBox mb, const int32_t* __restrict__ cand, int64_t nc,
+                           int maxlvl, double tol2, const int64_t* __restrict__ offs, double* __restrict__ tets) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    RefineScratch S;
+    expand_element(m.pos, m.vel, m.mass, m.conn, m.vpe, cand[c], g, mb, maxlvl, tol2, S, tets + TREC * offs[c]);
+}
+
+/* per tet: out[5t] = 1 when x lies in it, then rho and v */
+__global__ void k_vdf_sample(const double* __restrict__ tets, int64_t nt, QueryBox x, double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double rho = 0.0, v[3] = {0.0, 0.0, 0.0};
+    const int in = sample_tet(tets + TREC * t, x.lo, &rho, v);
+    out[5 * t] = in ? 1.0 : 0.0;
+    out[5 * t + 1] = rho;
+    for (int a = 0; a < 3; ++a) out[5 * t + 2 + a] = v[a];
 }
 
 /* per-block reduction of two counters into global ones */
@@ -1670,6 +1735,167 @@ int prepare(Workspace& W, const psivox_opts* o, hipStream_t* s, int* dev) {
 
 }  // namespace
 
+/* The element-box index behind psivox_elem_index_* and psivox_sample_vdf:
+ * the mesh on the device (copies of a host mesh, or the caller's device
+ * arrays), every element's wrapped box and whether it passed the window
+ * test, and grow-only buffers for queries. */
+struct psivox_elem_index {
+    int dev = -1;
+    int64_t nelem = 0;
+    MeshDev md{};
+    DevBuf pos, vel, mass, conn, box, valid, flags, cand, nsel, tmp, counts, offs, tets, out;
+};
+
+namespace {
+
+int index_stream(const void* stream, hipStream_t* s, int* dev) {
+    psivox_opts o{};
+    o.stream = (void*)stream;
+    return prepare(g_ws, &o, s, dev);
+}
+
+/* the elements whose box meets q into idx->cand (ascending); *nc = count */
+int index_select(psivox_elem_index* X, const QueryBox& q, hipStream_t s, int64_t* nc) {
+    int rc;
+    const int64_t ne = X->nelem;
+    *nc = 0;
+    if (ne == 0) return 0;
+    if ((rc = X->flags.get((size_t)ne, X->dev)) || (rc = X->cand.get((size_t)ne * sizeof(int32_t), X->dev)) ||
+        (rc = X->nsel.get(sizeof(int64_t), X->dev)))
+        return rc;
+    k_box_meets<<<grid_blocks(ne, 256), 256, 0, s>>>((const double*)X->box.p, (const int8_t*)X->valid.p, ne, q,
+                                                     (uint8_t*)X->flags.p);
+    HIPCHK(hipGetLastError());
+    hipcub::CountingInputIterator<int32_t> ids(0);
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, tb, ids, (const uint8_t*)X->flags.p, (int32_t*)X->cand.p,
+                                         (int64_t*)X->nsel.p, ne, s));
+    if ((rc = X->tmp.get(tb, X->dev))) return rc;
+    HIPCHK(hipcub::DeviceSelect::Flagged(X->tmp.p, tb, ids, (const uint8_t*)X->flags.p, (int32_t*)X->cand.p,
+                                         (int64_t*)X->nsel.p, ne, s));
+    HIPCHK(hipMemcpyAsync(nc, X->nsel.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int index_build(const psivox_mesh* mesh, const double window[2][3], hipStream_t s, int dev,
+                psivox_elem_index* X) {
+    int rc = check_mesh(mesh);
+    if (rc) return rc;
+    if (mesh->nelem > INT32_MAX) return set_err(PSIVOX_E_ARG, "the element index takes at most 2^31-1 elements");
+    X->dev = dev;
+    X->nelem = mesh->nelem;
+    X->md = MeshDev{};
+    X->md.vpe = mesh->elemtype;
+    if (mesh->nelem == 0) return 0;
+    if (mesh->on_device) {
+        X->md.pos = mesh->pos;
+        X->md.vel = mesh->vel;
+        X->md.mass = mesh->mass;
+        X->md.conn = mesh->conn;
+    } else {
+        const void* p;
+        if ((rc = upload(X->pos, mesh->pos, (size_t)mesh->npart * 3 * sizeof(double), dev, s, &p))) return rc;
+        X->md.pos = (const double*)p;
+        if (mesh->vel) {
+            if ((rc = upload(X->vel, mesh->vel, (size_t)mesh->npart * 3 * sizeof(double), dev, s, &p))) return rc;
+            X->md.vel = (const double*)p;
+        }
+        if ((rc = upload(X->mass, mesh->mass, (size_t)mesh->nelem * sizeof(double), dev, s, &p))) return rc;
+        X->md.mass = (const double*)p;
+        if ((rc = upload(X->conn, mesh->conn, (size_t)mesh->nelem * mesh->elemtype * sizeof(int32_t), dev, s, &p)))
+            return rc;
+        X->md.conn = (const int32_t*)p;
+    }
+    GridInfo g;
+    memset(&g, 0, sizeof g);
+    MeshBox mb;
+    mb.periodic = mesh->periodic ? 1 : 0;
+    mb.full_window = 0;
+    for (int a = 0; a < 3; ++a) {
+        g.w0[a] = window[0][a];
+        g.w1[a] = window[1][a];
+        mb.lo[a] = mesh->box[0][a];
+        mb.hi[a] = mesh->box[1][a];
+    }
+    if ((rc = X->box.get((size_t)mesh->nelem * 6 * sizeof(double), dev)) ||
+        (rc = X->valid.get((size_t)mesh->nelem, dev)))
+        return rc;
+    k_elem_boxes<<<grid_blocks(mesh->nelem, 128), 128, 0, s>>>(X->md, g, mb, mesh->nelem, (double*)X->box.p,
+                                                               (int8_t*)X->valid.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+/* the multistream sample at x (psi_sample_vdf, src/cpsi.c:177-275): streams
+ * in ascending element order, tets in refinement order */
+int sample_vdf(psivox_elem_index* X, const double* x, double reftol, int maxlvl, double* rho, double* vel,
+               int64_t cap, int64_t* n, hipStream_t s) {
+    int rc;
+    *n = 0;
+    QueryBox q;
+    for (int a = 0; a < 3; ++a) q.lo[a] = q.hi[a] = x[a];
+    int64_t nc = 0;
+    if ((rc = index_select(X, q, s, &nc))) return rc;
+    if (nc == 0) return 0;
+    if (maxlvl > MAXLVL) maxlvl = MAXLVL; /* src/refine.c:89-92 */
+    if (maxlvl < 0) maxlvl = 0;
+    GridInfo g; /* unbounded window, no wrap: every candidate expands as it lies */
+    memset(&g, 0, sizeof g);
+    MeshBox mb;
+    mb.periodic = 0;
+    mb.full_window = 0;
+    for (int a = 0; a < 3; ++a) {
+        g.w0[a] = -HUGE_VAL;
+        g.w1[a] = HUGE_VAL;
+        mb.lo[a] = mb.hi[a] = 0.0;
+    }
+    const double tol2 = reftol * reftol; /* position units here (src/cpsi.c:197) */
+    if ((rc = X->counts.get((size_t)nc * sizeof(int64_t), X->dev)) ||
+        (rc = X->offs.get((size_t)nc * sizeof(int64_t), X->dev)))
+        return rc;
+    k_vdf_count<<<grid_blocks(nc, 64), 64, 0, s>>>(X->md, g, mb, (const int32_t*)X->cand.p, nc, maxlvl, tol2,
+                                                   (int64_t*)X->counts.p);
+    HIPCHK(hipGetLastError());
+    std::vector<int64_t> cnt((size_t)nc), off((size_t)nc);
+    HIPCHK(hipMemcpyAsync(cnt.data(), X->counts.p, (size_t)nc * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int64_t nt = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+        off[(size_t)c] = nt;
+        nt += cnt[(size_t)c];
+    }
+    if (nt == 0) return 0;
+    if ((rc = X->tets.get((size_t)nt * TREC * sizeof(double), X->dev)) ||
+        (rc = X->out.get((size_t)nt * 5 * sizeof(double), X->dev)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(X->offs.p, off.data(), (size_t)nc * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    k_vdf_emit<<<grid_blocks(nc, 64), 64, 0, s>>>(X->md, g, mb, (const int32_t*)X->cand.p, nc, maxlvl, tol2,
+                                                  (const int64_t*)X->offs.p, (double*)X->tets.p);
+    HIPCHK(hipGetLastError());
+    k_vdf_sample<<<grid_blocks(nt, 256), 256, 0, s>>>((const double*)X->tets.p, nt, q, (double*)X->out.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> res((size_t)nt * 5);
+    HIPCHK(hipMemcpyAsync(res.data(), X->out.p, res.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int64_t k = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        const double* r = &res[(size_t)t * 5];
+        if (r[0] == 0.0) continue;
+        if (k < cap) {
+            if (rho) rho[k] = r[1];
+            if (vel)
+                for (int a = 0; a < 3; ++a) vel[3 * k + a] = r[2 + a];
+        }
+        k++;
+    }
+    *n = k;
+    return 0;
+}
+
+}  // namespace
+
 /* ------------------------------------------------------------------ */
 /* C ABI                                                               */
 /* ------------------------------------------------------------------ */
@@ -1726,6 +1952,77 @@ int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const 
     int rc = prepare(g_ws, &o, &s, &dev);
     if (rc) return rc;
     return run_adjoint(g_ws, mesh, grid, dl_dm, dl_dpos, &o, s, dev);
+}
+
+int psivox_elem_index_build(const psivox_mesh* mesh, const double window[2][3], void* stream,
+                            psivox_elem_index** out) {
+    if (!out || !window) return set_err(PSIVOX_E_ARG, "window and out are required");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    hipStream_t s;
+    int dev;
+    int rc = index_stream(stream, &s, &dev);
+    if (rc) return rc;
+    psivox_elem_index* X = new (std::nothrow) psivox_elem_index;
+    if (!X) return set_err(PSIVOX_E_NOMEM, "out of host memory");
+    if ((rc = index_build(mesh, window, s, dev, X))) {
+        psivox_elem_index_free(X);
+        return rc;
+    }
+    *out = X;
+    return 0;
+}
+
+void psivox_elem_index_free(psivox_elem_index* idx) {
+    if (!idx) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (idx->dev >= 0) (void)hipSetDevice(idx->dev);
+    DevBuf* bufs[] = {&idx->pos, &idx->vel, &idx->mass, &idx->conn, &idx->box, &idx->valid, &idx->flags,
+                      &idx->cand, &idx->nsel, &idx->tmp, &idx->counts, &idx->offs, &idx->tets, &idx->out};
+    for (DevBuf* b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    (void)hipSetDevice(cur);
+    delete idx;
+}
+
+int psivox_elem_index_query(psivox_elem_index* idx, const double box[2][3], int32_t* out, int64_t cap, int64_t* n,
+                            void* stream) {
+    if (!idx || !box || !n || cap < 0 || (cap > 0 && !out)) return set_err(PSIVOX_E_ARG, "bad query arguments");
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    hipStream_t s;
+    int dev;
+    int rc = index_stream(stream, &s, &dev);
+    if (rc) return rc;
+    if (dev != idx->dev && idx->nelem > 0)
+        return set_err(PSIVOX_E_ARG, "the index lives on device %d, the current device is %d", idx->dev, dev);
+    QueryBox q;
+    for (int a = 0; a < 3; ++a) {
+        q.lo[a] = box[0][a];
+        q.hi[a] = box[1][a];
+    }
+    int64_t nc = 0;
+    if ((rc = index_select(idx, q, s, &nc))) return rc;
+    const int64_t k = nc < cap ? nc : cap;
+    if (k > 0) {
+        HIPCHK(hipMemcpyAsync(out, idx->cand.p, (size_t)k * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    *n = nc;
+    return 0;
+}
+
+int psivox_sample_vdf(psivox_elem_index* idx, const double point[3], double refine_tolerance,
+                      int32_t refine_max_lvl, double* rho, double* vel, int64_t cap, int64_t* n, void* stream) {
+    if (!idx || !point || !n || cap < 0) return set_err(PSIVOX_E_ARG, "bad sample arguments");
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    hipStream_t s;
+    int dev;
+    int rc = index_stream(stream, &s, &dev);
+    if (rc) return rc;
+    if (dev != idx->dev && idx->nelem > 0)
+        return set_err(PSIVOX_E_ARG, "the index lives on device %d, the current device is %d", idx->dev, dev);
+    return sample_vdf(idx, point, refine_tolerance, refine_max_lvl, rho, vel, cap, n, s);
 }
 
 }  // extern "C"

@@ -65,12 +65,18 @@ def lib():
         L.psivox_last_error.restype = C.c_char_p
         L.psivox_device_count.restype = C.c_int
         L.psivox_abi_version.restype = C.c_int
+        L.psivox_elem_index_build.argtypes = [P, P, P, P]
+        L.psivox_elem_index_free.argtypes = [P]
+        L.psivox_elem_index_free.restype = None
+        L.psivox_elem_index_query.argtypes = [P, P, P, C.c_int64, P, P]
+        L.psivox_sample_vdf.argtypes = [P, P, C.c_double, C.c_int32, P, P, C.c_int64, P, P]
         _lib = L
     return _lib
 
 
 EXPORTS = ["psivox_voxels", "psivox_voxels_batched", "psivox_mass_adjoint", "psivox_last_error",
-           "psivox_abi_version", "psivox_device_count"]
+           "psivox_abi_version", "psivox_device_count", "psivox_elem_index_build", "psivox_elem_index_free",
+           "psivox_elem_index_query", "psivox_sample_vdf"]
 
 
 def check(rc: int) -> None:
@@ -149,3 +155,49 @@ def mass_adjoint(mesh: Mesh, grid: Grid, dl_dm, dl_dpos, opts: Opts) -> None:
     float64, and (npart, 3) float64)."""
     check(lib().psivox_mass_adjoint(C.byref(mesh), C.byref(grid), C.c_void_p(dl_dm.data_ptr()),
                                     C.c_void_p(dl_dpos.data_ptr()), C.byref(opts)))
+
+
+class ElemIndex:
+    """The element-box index of a mesh on the GPU (psivox_elem_index_*): the
+    device form of PSI.RStarTree.  window: ((lo), (hi)) for the build's
+    window test (the periodic box, or an unbounded window)."""
+
+    def __init__(self, mesh: Mesh, window, stream=None):
+        W = ((C.c_double * 3) * 2)(*[(C.c_double * 3)(*map(float, w)) for w in window])
+        h = C.c_void_p()
+        check(lib().psivox_elem_index_build(C.byref(mesh), C.byref(W), stream, C.byref(h)))
+        self._h = h
+        self._mesh = mesh  # a device mesh must outlive the index
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().psivox_elem_index_free(h)
+            self._h = None
+
+    def query(self, box, stream=None):
+        """int32 element indices whose box meets the closed box, ascending."""
+        import numpy as np
+        Q = ((C.c_double * 3) * 2)(*[(C.c_double * 3)(*map(float, b)) for b in box])
+        n = C.c_int64(0)
+        cap = 1024
+        while True:
+            out = np.empty(cap, dtype=np.int32)
+            check(lib().psivox_elem_index_query(self._h, C.byref(Q), out.ctypes.data, cap, C.byref(n), stream))
+            if n.value <= cap:
+                return out[:n.value].copy()
+            cap = n.value
+
+    def sample(self, x, refine_tolerance=1.0, refine_max_lvl=0, stream=None):
+        """(rho, vel) streams at x, as PSI.VDF."""
+        import numpy as np
+        X = (C.c_double * 3)(*map(float, x))
+        n = C.c_int64(0)
+        cap = 256
+        while True:
+            rho, vel = np.empty(cap), np.empty((cap, 3))
+            check(lib().psivox_sample_vdf(self._h, X, refine_tolerance, refine_max_lvl, rho.ctypes.data,
+                                          vel.ctypes.data, cap, C.byref(n), stream))
+            if n.value <= cap:
+                return rho[:n.value].copy(), vel[:n.value].copy()
+            cap = n.value

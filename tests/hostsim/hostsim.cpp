@@ -16,6 +16,7 @@
 #include "../../pypsi_amd/csrc/psivox_refine.h"
 #include "../../pypsi_amd/csrc/psivox_fast.h"
 #include "../../pypsi_amd/csrc/psivox_adjoint.h"
+#include "../../pypsi_amd/csrc/psivox_query.h"
 
 using namespace pvx;
 
@@ -247,4 +248,62 @@ extern "C" int hs_mass_adjoint(int vpe, int64_t npart, int64_t nelem, const doub
     out_stats[0] = ntasks;
     out_stats[1] = fails;
     return 0;
+}
+
+/* the device query path on the host: element boxes as k_elem_boxes, the
+ * box test of k_box_meets, candidates expanded as k_vdf_count/emit and
+ * sampled as k_vdf_sample; returns the stream count, writes up to cap */
+extern "C" int64_t hs_sample_vdf(int vpe, int64_t nelem, const double* pos, const double* vel, const double* mass,
+                                 const int32_t* conn, int periodic, const double* box6, const double* win6,
+                                 const double* x, double reftol, int maxlvl, double* rho, double* velout,
+                                 int64_t cap) {
+    GridInfo g;
+    memset(&g, 0, sizeof g);
+    MeshBox mb;
+    mb.periodic = periodic;
+    mb.full_window = 0;
+    for (int a = 0; a < 3; ++a) {
+        g.w0[a] = win6[a];
+        g.w1[a] = win6[3 + a];
+        mb.lo[a] = box6[a];
+        mb.hi[a] = box6[3 + a];
+    }
+    QueryBox q;
+    for (int a = 0; a < 3; ++a) q.lo[a] = q.hi[a] = x[a];
+    GridInfo gu;
+    memset(&gu, 0, sizeof gu);
+    MeshBox mu;
+    mu.periodic = 0;
+    mu.full_window = 0;
+    for (int a = 0; a < 3; ++a) {
+        gu.w0[a] = -HUGE_VAL;
+        gu.w1[a] = HUGE_VAL;
+        mu.lo[a] = mu.hi[a] = 0.0;
+    }
+    if (maxlvl > MAXLVL) maxlvl = MAXLVL;
+    RefineScratch* RS = new RefineScratch;
+    std::vector<double> tets;
+    int64_t n = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        double P[27][3], lo[3], hi[3];
+        for (int v = 0; v < vpe; ++v)
+            for (int a = 0; a < 3; ++a) P[v][a] = pos[3 * (int64_t)conn[e * vpe + v] + a];
+        const int ok = wrap_and_test(P, vpe, g, mb);
+        aabb((const double (*)[3])P, vpe, lo, hi);
+        if (!ok || !boxes_meet(lo, hi, q)) continue;
+        const int64_t nt = expand_element(pos, vel, mass, conn, vpe, e, gu, mu, maxlvl, reftol * reftol, *RS, nullptr);
+        tets.resize((size_t)nt * TREC);
+        expand_element(pos, vel, mass, conn, vpe, e, gu, mu, maxlvl, reftol * reftol, *RS, tets.data());
+        for (int64_t t = 0; t < nt; ++t) {
+            double r, v[3];
+            if (!sample_tet(&tets[(size_t)t * TREC], x, &r, v)) continue;
+            if (n < cap) {
+                rho[n] = r;
+                for (int a = 0; a < 3; ++a) velout[3 * n + a] = v[a];
+            }
+            n++;
+        }
+    }
+    delete RS;
+    return n;
 }

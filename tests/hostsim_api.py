@@ -54,3 +54,20 @@ def mass_adjoint(P, w):
     lib().hs_mass_adjoint(C.c_int(P.elemtype), C.c_int64(P.pos.shape[0]), C.c_int64(P.conn.shape[0]), vp(P.pos),
                           vp(P.mass), vp(P.conn), C.c_int(int(P.periodic)), box, n, win, d, vp(w), vp(out), vp(st))
     return out, st
+
+
+def sample_vdf(P, x, reftol=1.0, maxlvl=0, window=None):
+    """(rho, vel) of the device VDF path run on the host; window defaults to
+    the mesh's box (as PSI.VDF without a tree)."""
+    L = lib()
+    L.hs_sample_vdf.restype = C.c_int64
+    win = (C.c_double * 6)(*(P.box if window is None else np.asarray(window, dtype=np.float64)).reshape(-1))
+    box = (C.c_double * 6)(*P.box.reshape(-1))
+    X = (C.c_double * 3)(*[float(v) for v in x])
+    vp = lambda a: C.c_void_p(a.ctypes.data)
+    args = (C.c_int(P.elemtype), C.c_int64(P.conn.shape[0]), vp(P.pos), vp(P.vel), vp(P.mass), vp(P.conn),
+            C.c_int(int(P.periodic)), box, win, X, C.c_double(reftol), C.c_int(maxlvl))
+    n = L.hs_sample_vdf(*args, None, None, C.c_int64(0))
+    rho, vel = np.empty(n), np.empty((n, 3))
+    L.hs_sample_vdf(*args, vp(rho), vp(vel), C.c_int64(n))
+    return rho, vel

@@ -16,13 +16,14 @@ LIB = os.path.join(ROOT, "pypsi_amd", "libpsivox.so")
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(psivox_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(psivox_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_entry_points():
     names = declared()
     for n in ("psivox_voxels", "psivox_voxels_batched", "psivox_mass_adjoint", "psivox_last_error",
-              "psivox_abi_version", "psivox_device_count"):
+              "psivox_abi_version", "psivox_device_count", "psivox_elem_index_build", "psivox_elem_index_free",
+              "psivox_elem_index_query", "psivox_sample_vdf"):
         assert n in names
 
 
