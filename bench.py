@@ -266,6 +266,60 @@ def vdf_leg(pos, vel, npts=400):
     return out
 
 
+def annihilation_leg(steps=3):
+    """mode='annihilation' (the integral of rho^2 per cell from every pair of
+    overlapping tets) from device-resident inputs: Zel'dovich 16^3 (rms 1.0
+    cells) onto 32^3, with the reference's own C core (oracle/_ref) timed on
+    the same input single-threaded, and 32^3 (rms 0.3) onto 64^3."""
+    import numpy as np
+    import torch
+    from pypsi_amd import device as D
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    dev = torch.device("cuda:0")
+    out = {}
+    for n, rms, gn in ((16, 1.0, 32), (32, 0.3, 64)):
+        from pypsi_amd.synth import zeldovich_block
+        pos, vel = zeldovich_block(n, rms, 0)
+        i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        cn = np.empty((n, n, n, 8), dtype=np.int32)
+        for c in range(8):
+            cn[..., c] = n * n * ((i + (c >> 2)) % n) + n * ((j + ((c >> 1) & 1)) % n) + ((k + (c & 1)) % n)
+        cn = cn.reshape(-1, 8)
+        mass = np.ones(cn.shape[0])
+        t = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (pos.reshape(-1, 3), vel.reshape(-1, 3), mass, cn)]
+        m = torch.zeros((gn,) * 3, dtype=torch.float64, device=dev)
+        mesh = D.make_mesh(*t, periodic_box=box)
+        grid = D.make_grid((gn,) * 3, box, {"m": m})
+        opts = D.make_opts(mode="annihilation", stream=torch.cuda.current_stream().cuda_stream)
+        st = D.voxels(mesh, grid, opts)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(steps):
+            m.zero_()
+            t0 = time.perf_counter()
+            st = D.voxels(mesh, grid, opts)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ms = 1e3 * sum(ts) / len(ts)
+        leg = {"ms": ms, "tet_pairs": st["nghosts"], "tet_pairs_per_s": st["nghosts"] / (ms * 1e-3),
+               "fragments": st["nfrag"], "pair_kernel_ms": st["kernel_ms"]}
+        if n == 16:
+            try:
+                import oracle as O
+                if O.have_ref():
+                    P = O.Problem.block(pos, vel, periodic_box=box, n=(gn,) * 3)
+                    t0 = time.perf_counter()
+                    O.ref_voxels(P, mode=1)
+                    dt = time.perf_counter() - t0
+                    leg["cpu_reference_ms"] = 1e3 * dt
+                    leg["cpu_reference_cores"] = 1
+            except Exception as e:  # report, never hide
+                leg["cpu_reference_error"] = repr(e)
+        out[f"zeldovich-{n}^3-rms{rms}-to-{gn}^3"] = leg
+    return out
+
+
 def plain_leg(mesh, grid, tf, stream, nfrag, steps=2, warmup=1):
     """The FP64 one-thread-per-tet kernel on the same inputs (the plain GPU
     method the FP32-clip column kernel is measured against)."""
@@ -499,7 +553,7 @@ def main():
                     help="count a refined workload's fragments with the FP64 kernel into profiles/ and exit")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the plain-kernel, forward+adjoint, batched, refined, end-to-end and snapshot legs")
-    ap.add_argument("--only", choices=["snapshot", "vdf"], help="run one extra leg alone and print it")
+    ap.add_argument("--only", choices=["snapshot", "vdf", "annihilation"], help="run one extra leg alone and print it")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -519,6 +573,9 @@ def main():
         return
     if args.only == "snapshot":
         print(json.dumps({"snapshot": snapshot_leg()}), flush=True)
+        return
+    if args.only == "annihilation":
+        print(json.dumps({"annihilation": annihilation_leg()}), flush=True)
         return
 
     wname = args.workload
@@ -612,6 +669,8 @@ def main():
         out["snapshot"] = snapshot_leg()
         log("[bench] RStarTree + VDF point queries")
         out["vdf"] = vdf_leg(pos, vel)
+        log("[bench] annihilation mode")
+        out["annihilation"] = annihilation_leg()
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         try:

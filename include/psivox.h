@@ -41,7 +41,10 @@ extern "C" {
 #define PSIVOX_XV 4
 #define PSIVOX_VV 5
 
-/* modes (include/psi.h:71-72) */
+/* modes (include/psi.h:71-72).  PSIVOX_MODE_ANNIHILATION adds the integral
+ * of rho^2 over each cell into the mass field (src/cpsi.c:115-158): every
+ * ordered pair of tets whose boxes overlap adds rho0 rho1 vol(overlap ∩
+ * cell), FP64, whatever the kernel option; the other fields are untouched. */
 #define PSIVOX_MODE_DENSITY 0
 #define PSIVOX_MODE_ANNIHILATION 1
 
@@ -112,7 +115,7 @@ typedef struct {
 typedef struct {
     int64_t nelem_used; /* elements inside the window after periodic wrap */
     int64_t ntets;      /* tets after refinement */
-    int64_t nghosts;    /* (tet, periodic copy) pairs */
+    int64_t nghosts;    /* (tet, periodic copy) pairs; annihilation: tet pairs voxelized */
     int64_t nfrag;      /* (copy, voxel) fragments deposited, m0 > 0 */
     double kernel_ms;   /* device time of the voxelization kernel: the FP64 kernel, or the
                            column kernel (task setup, ordering and residual kernels not

@@ -64,6 +64,7 @@ def lib():
         L.orc_voxelize_one_tet.argtypes = [P, P, P, P, C.c_double, C.c_int, P]
         L.orc_mass_adjoint.argtypes = [P, P, P, P]
         L.orc_fd_mass_loss.argtypes = [P, P, P, P, C.c_int, C.c_int64, C.c_int, C.c_double, P]
+        L.orc_annihilation.argtypes = [P, P, C.c_double, C.c_int, P]
         L.orc_elem_query.argtypes = [P, P, P, P, C.c_int64]
         L.orc_elem_query.restype = C.c_int64
         L.orc_sample_vdf.argtypes = [P, P, P, C.c_double, C.c_int, P, P, C.c_int64]
@@ -279,9 +280,20 @@ class _silence_stdout:
         os.close(self.null)
 
 
-def ref_voxels(prob: Problem, reftol=1.0, maxlvl=0, out=None, e0=0, e1=None):
-    """Run the reference psi_voxels() (density mode) on the same problem,
-    optionally on the element subset [e0, e1)."""
+def annihilation(prob: Problem, reftol=1.0, maxlvl=0, out=None):
+    """mode='annihilation': the pairwise overlap rate added into out['m']."""
+    L = lib()
+    out = prob.zero_fields() if out is None else out
+    m, g, st = prob._omesh(), prob._ogrid(out), OStats()
+    if L.orc_annihilation(C.byref(m), C.byref(g), reftol, maxlvl, C.byref(st)) != 0:
+        raise RuntimeError("oracle failed")
+    return out, st.as_dict()
+
+
+def ref_voxels(prob: Problem, reftol=1.0, maxlvl=0, out=None, e0=0, e1=None, mode=0):
+    """Run the reference psi_voxels() (density mode, or mode=1 for
+    annihilation) on the same problem, optionally on the element subset
+    [e0, e1)."""
     L = reflib()
     out = prob.zero_fields() if out is None else out
     conn = prob.conn
@@ -313,7 +325,7 @@ def ref_voxels(prob: Problem, reftol=1.0, maxlvl=0, out=None, e0=0, e1=None):
         g.n[a] = prob.n[a]
         g.d[a] = prob.d[a]
     with _silence_stdout():
-        L.psi_voxels(C.byref(g), C.byref(m), None, 0, reftol, maxlvl)
+        L.psi_voxels(C.byref(g), C.byref(m), None, mode, reftol, maxlvl)
     return out
 
 

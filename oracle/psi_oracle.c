@@ -1178,3 +1178,198 @@ int64_t orc_sample_vdf(const orc_mesh* m, const double win[2][3], const double* 
     o_scratch_free(&S);
     return n;
 }
+
+/* ------------------------------------------------------------------ */
+/* mode='annihilation' (src/cpsi.c:66-77, 115-158; src/geometry.c:     */
+/* 190-252, 503-558)                                                   */
+/* ------------------------------------------------------------------ */
+
+/* src/geometry.c:190-217: a tet with negative signed volume gets vertices
+ * 0 and 1 swapped; returns the (now positive) volume */
+static double o_orient(double (*p)[3]) {
+    double e1[3], e2[3], e3[3];
+    for (int a = 0; a < 3; ++a) {
+        e1[a] = p[1][a] - p[0][a];
+        e2[a] = p[2][a] - p[0][a];
+        e3[a] = p[3][a] - p[0][a];
+    }
+    double vol = o_det3(e1, e2, e3) / 6.0;
+    if (vol < 0.0) {
+        for (int a = 0; a < 3; ++a) {
+            double t = p[0][a];
+            p[0][a] = p[1][a];
+            p[1][a] = t;
+        }
+        vol = -vol;
+    }
+    return vol;
+}
+
+/* src/geometry.c:503-558: the four face planes of a positively oriented
+ * tet, normals (not normalised) pointing inward, d = -n . face centroid:
+ * face f is opposite vertex f, n = (c - a) x (b - a) over its vertices */
+static void o_tet_faces(const double (*p)[3], double (*n)[3], double* d) {
+    static const int F[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+    for (int f = 0; f < 4; ++f) {
+        const double* A = p[F[f][0]];
+        const double* B = p[F[f][1]];
+        const double* Cc = p[F[f][2]];
+        double u[3], w[3], c[3];
+        for (int a = 0; a < 3; ++a) {
+            u[a] = Cc[a] - A[a];
+            w[a] = B[a] - A[a];
+            c[a] = (A[a] + B[a] + Cc[a]) / 3.0;
+        }
+        n[f][0] = u[1] * w[2] - u[2] * w[1];
+        n[f][1] = u[2] * w[0] - u[0] * w[2];
+        n[f][2] = u[0] * w[1] - u[1] * w[0];
+        d[f] = -(n[f][0] * c[0] + n[f][1] * c[1] + n[f][2] * c[2]);
+    }
+}
+
+/* src/geometry.c:223-252: tet0 clipped by tet1's faces, then voxelized in
+ * the box mb (the two tets' common box) with m0 only; each leaf adds
+ * rho0 rho1 vol0 m0 to the mass field */
+static void o_annihilate_pair(oscratch* S, orc_grid* g, const double* x0, double mass0, const double* x1,
+                              double mass1, const double (*mb)[3], orc_stats* st) {
+    double a[4][3], b[4][3];
+    memcpy(a, x0, sizeof a);
+    memcpy(b, x1, sizeof b);
+    const double vol0 = o_orient(a), vol1 = o_orient(b);
+    const double w = mass0 / vol0 * (mass1 / vol1) * vol0;
+    opoly* P = &S->stack[0];
+    o_init_tet(P, (const double (*)[3])a);
+    double fn[4][3], fd[4];
+    o_tet_faces((const double (*)[3])b, fn, fd);
+    o_clip(P, (const double (*)[3])fn, fd, 4, st);
+    /* window planes and index box from mb (src/geometry.c:806-850) */
+    double pn[6][3], pd[6];
+    memset(pn, 0, sizeof pn);
+    memset(pd, 0, sizeof pd);
+    int np = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        double cmin = mb[0][ax], cmax = mb[1][ax];
+        int clo = 0, chi = 0;
+        if (g->win[0][ax] > cmin) {
+            cmin = g->win[0][ax];
+            clo = 1;
+        }
+        if (g->win[1][ax] < cmax) {
+            cmax = g->win[1][ax];
+            chi = 1;
+        }
+        if (clo) {
+            pn[np][ax] = 1.0;
+            pd[np] = -cmin;
+            np++;
+        }
+        if (chi) {
+            pn[np][ax] = -1.0;
+            pd[np] = cmax;
+            np++;
+        }
+        P->lo[ax] = (int)floor((cmin - g->win[0][ax]) / g->d[ax]);
+        P->hi[ax] = (int)ceil((cmax - g->win[0][ax]) / g->d[ax]);
+        if (P->lo[ax] < 0) P->lo[ax] = 0;
+        if (P->hi[ax] > g->n[ax]) P->hi[ax] = g->n[ax];
+    }
+    o_clip(P, (const double (*)[3])pn, pd, np, st);
+    S->nstack = 1;
+    double m[10];
+    while (S->nstack > 0) {
+        opoly cur = S->stack[--S->nstack];
+        if (cur.nv <= 0) continue;
+        int dmax = 0, ax = 0;
+        for (int k = 0; k < 3; ++k) {
+            int sz = cur.hi[k] - cur.lo[k];
+            if (sz > dmax) {
+                dmax = sz;
+                ax = k;
+            }
+        }
+        if (dmax <= 1) {
+            o_reduce(&cur, m, 0);
+            if (m[0] <= 0.0) continue;
+            if (cur.lo[0] >= g->n[0] || cur.lo[1] >= g->n[1] || cur.lo[2] >= g->n[2]) continue;
+            int64_t gi = (int64_t)g->n[1] * g->n[2] * cur.lo[0] + (int64_t)g->n[2] * cur.lo[1] + cur.lo[2];
+            st->nfrag++;
+            g->f[ORC_M][gi] += w * m[0];
+            continue;
+        }
+        if (S->nstack + 2 > OSTACK) {
+            st->overflow++;
+            continue;
+        }
+        double c = g->d[ax] * (cur.lo[ax] + dmax / 2) + g->win[0][ax];
+        opoly* kids = &S->stack[S->nstack];
+        o_split(&cur, kids, c, ax, st);
+        memcpy(kids[0].lo, cur.lo, sizeof cur.lo);
+        memcpy(kids[0].hi, cur.hi, sizeof cur.hi);
+        kids[0].hi[ax] -= dmax - dmax / 2;
+        memcpy(kids[1].lo, cur.lo, sizeof cur.lo);
+        memcpy(kids[1].hi, cur.hi, sizeof cur.hi);
+        kids[1].lo[ax] += dmax / 2;
+        S->nstack += 2;
+        if (S->nstack > st->maxstack) st->maxstack = S->nstack;
+    }
+}
+
+/* src/cpsi.c:115-158: for every element e inside the window (wrapped,
+ * refined), every element e1 of the tree (wrapped box meets e's box;
+ * refined from its raw vertices) and every pair of their tets whose boxes
+ * overlap with positive volume, the pair's annihilation rate.  Adds into
+ * the mass field only. */
+int orc_annihilation(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlvl, orc_stats* st) {
+    memset(st, 0, sizeof(*st));
+    const int vpe = mesh->elemtype;
+    const double tol = o_tol(g, reftol);
+    if (maxlvl > 6) maxlvl = 6;
+    oscratch S0, S1;
+    o_scratch_init(&S0);
+    o_scratch_init(&S1);
+    double p[27][3], u[27][3], lo[3], hi[3], lo1[3], hi1[3];
+    for (int64_t e = 0; e < mesh->nelem; ++e) {
+        for (int v = 0; v < vpe; ++v) {
+            int64_t id = mesh->conn[e * vpe + v];
+            for (int a = 0; a < 3; ++a) {
+                p[v][a] = mesh->pos[3 * id + a];
+                u[v][a] = mesh->vel ? mesh->vel[3 * id + a] : 0.0;
+            }
+        }
+        if (!o_wrap_and_test(p, vpe, lo, hi, g, mesh)) continue;
+        st->nelem_used++;
+        o_refine(&S0, (const double (*)[3])p, (const double (*)[3])u, mesh->mass[e], vpe, maxlvl, tol * tol);
+        const double q[2][3] = {{lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+        for (int64_t e1 = 0; e1 < mesh->nelem; ++e1) {
+            if (!o_elem_box(mesh, (const double (*)[3])g->win, e1, lo1, hi1) || !o_meets(lo1, hi1, q)) continue;
+            for (int v = 0; v < vpe; ++v) {
+                int64_t id = mesh->conn[e1 * vpe + v];
+                for (int a = 0; a < 3; ++a) {
+                    p[v][a] = mesh->pos[3 * id + a];
+                    u[v][a] = mesh->vel ? mesh->vel[3 * id + a] : 0.0;
+                }
+            }
+            o_refine(&S1, (const double (*)[3])p, (const double (*)[3])u, mesh->mass[e1], vpe, maxlvl, tol * tol);
+            for (int t = 0; t < S0.ntet; ++t) {
+                double b0[2][3], b1[2][3], mb[2][3];
+                o_aabb((const double (*)[3])&S0.tpos[12 * t], 4, b0[0], b0[1]);
+                for (int t1 = 0; t1 < S1.ntet; ++t1) {
+                    o_aabb((const double (*)[3])&S1.tpos[12 * t1], 4, b1[0], b1[1]);
+                    int ok = 1;
+                    for (int a = 0; a < 3; ++a) {
+                        mb[0][a] = fmax(b0[0][a], b1[0][a]);
+                        mb[1][a] = fmin(b0[1][a], b1[1][a]);
+                        ok &= mb[1][a] > mb[0][a];
+                    }
+                    if (!ok) continue;
+                    st->ntets++;
+                    o_annihilate_pair(&S0, g, &S0.tpos[12 * t], S0.tmass[t], &S1.tpos[12 * t1], S1.tmass[t1],
+                                      (const double (*)[3])mb, st);
+                }
+            }
+        }
+    }
+    o_scratch_free(&S0);
+    o_scratch_free(&S1);
+    return 0;
+}

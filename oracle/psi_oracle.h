@@ -58,6 +58,10 @@ int64_t orc_elem_query(const orc_mesh* m, const double win[2][3], const double q
 int64_t orc_sample_vdf(const orc_mesh* m, const double win[2][3], const double* x, double reftol, int maxlvl,
                        double* rho, double* vel, int64_t cap);
 
+/* mode='annihilation': adds the pairwise tet-overlap rate rho0 rho1 vol into
+ * the mass field (stats: nfrag = leaves deposited, ntets = tet pairs) */
+int orc_annihilation(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlvl, orc_stats* st);
+
 /* mass adjoint (psi_adjoint.c), refine_max_lvl = 0: dpos = dL/dpos
  * (npart x 3, overwritten) from w = dL/dm per cell */
 int orc_mass_adjoint(const orc_mesh* mesh, const orc_grid* g, const double* w, double* dpos);

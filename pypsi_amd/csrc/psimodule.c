@@ -14,9 +14,7 @@
  * (the reference reads their raw buffer); the gadget2/gevolution loaders
  * reject malformed particle ids and non-cubic particle counts;
  * RStarTree.query() returns its elements in ascending order (the reference:
- * in its tree's order) and VDF() its streams in ascending element order;
- * mode='annihilation' is not part of this build and raises
- * NotImplementedError.
+ * in its tree's order) and VDF() its streams in ascending element order.
  */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
@@ -726,10 +724,9 @@ static PyObject* PSI_voxels(PyObject* self, PyObject* args, PyObject* kwds) {
     int mode;
     if (strcmp(modestr, "density") == 0)
         mode = PSIVOX_MODE_DENSITY;
-    else if (strcmp(modestr, "annihilation") == 0) {
-        PyErr_SetString(PyExc_NotImplementedError, "mode 'annihilation' is not part of this build");
-        return NULL;
-    } else {
+    else if (strcmp(modestr, "annihilation") == 0)
+        mode = PSIVOX_MODE_ANNIHILATION;
+    else {
         PyErr_Format(PyExc_ValueError, "Invalid mode for PSI.voxels(): %s", modestr);
         return NULL;
     }

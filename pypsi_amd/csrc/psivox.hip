@@ -35,6 +35,7 @@
 #include "psivox_refine.h"
 #include "psivox_moments.h"
 #include "psivox_query.h"
+#include "psivox_annihilate.h"
 
 using namespace pvx;
 
@@ -303,6 +304,103 @@ __global__ __launch_bounds__(256) void k_vox_plain(const double* __restrict__ te
         }
     }
     block_count(nf, nc, counters);
+}
+
+/* ------------------------------------------------------------------ */
+/* mode='annihilation'                                                 */
+/* ------------------------------------------------------------------ */
+
+/* sort keys of the broad phase: the box's low x, +inf for elements outside
+ * the window (they sort last and end every sweep) */
+__global__ void k_sweep_keys(const double* __restrict__ box, const int8_t* __restrict__ valid, int64_t ne,
+                             double* __restrict__ key, int32_t* __restrict__ id) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    key[e] = valid[e] ? box[6 * e] : HUGE_VAL;
+    id[e] = (int32_t)e;
+}
+
+/* Broad phase by sort and sweep along x: element id[i] meets every later
+ * id[j] whose low x is at most its high x and whose y, z intervals meet its
+ * own (closed, as the reference's tree query, src/rtree.c:106-110).  Each
+ * such pair is two ordered pairs; every element also pairs with itself.
+ * emit == nullptr: count into cnt[i]; else write the pairs at off[i]. */
+__global__ void k_sweep_pairs(const double* __restrict__ key, const int32_t* __restrict__ id,
+                              const double* __restrict__ box, int64_t ne, int64_t* __restrict__ cnt,
+                              const int64_t* __restrict__ off, int2* __restrict__ emit) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne) return;
+    if (!(key[i] < HUGE_VAL)) {
+        if (!emit) cnt[i] = 0;
+        return;
+    }
+    const int32_t e = id[i];
+    const double* b = box + 6 * (int64_t)e;
+    const double hx = b[3];
+    int64_t n = 0, o = emit ? off[i] : 0;
+    if (emit) emit[o++] = make_int2(e, e);
+    n++;
+    for (int64_t j = i + 1; j < ne && key[j] <= hx; ++j) {
+        const int32_t f = id[j];
+        const double* c = box + 6 * (int64_t)f;
+        if (c[4] < b[1] || c[1] > b[4] || c[5] < b[2] || c[2] > b[5]) continue;
+        if (emit) {
+            emit[o++] = make_int2(e, f);
+            emit[o++] = make_int2(f, e);
+        }
+        n += 2;
+    }
+    if (!emit) cnt[i] = n;
+}
+
+/* Tet pairs of element pair p = (e, e1): tets [offA[e], offA[e+1]) of the
+ * wrapped expansion against [offB[e1], offB[e1+1]) of the raw one, kept
+ * when their boxes overlap with positive volume (src/cpsi.c:138-152).
+ * emit == nullptr: count; else write (tet a, tet b) at toff[p] - base. */
+__global__ void k_tet_pairs(const int2* __restrict__ pairs, int64_t p0, int64_t p1, const int64_t* __restrict__ offA,
+                            const int64_t* __restrict__ offB, const double* __restrict__ ta,
+                            const double* __restrict__ tb, int64_t* __restrict__ cnt,
+                            const int64_t* __restrict__ toff, int64_t base, uint2* __restrict__ emit) {
+    const int64_t p = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= p1) return;
+    const int2 pr = pairs[p];
+    int64_t n = 0, o = emit ? toff[p] - base : 0;
+    for (int64_t t = offA[pr.x]; t < offA[pr.x + 1]; ++t)
+        for (int64_t u = offB[pr.y]; u < offB[pr.y + 1]; ++u) {
+            double lo[3], hi[3];
+            if (!tet_box_overlap(ta + TREC * t, tb + TREC * u, lo, hi)) continue;
+            if (emit) emit[o++] = make_uint2((unsigned)t, (unsigned)u);
+            n++;
+        }
+    if (!emit) cnt[p] = n;
+}
+
+struct MassSink {
+    double* m;
+    __device__ void add(int64_t i, double v) { unsafeAtomicAdd(m + i, v); }
+};
+
+/* one thread per tet pair, FP64: the pair's overlap swept through the cells */
+__global__ __launch_bounds__(128) void k_annihilate(const double* __restrict__ ta, const double* __restrict__ tb,
+                                                    const uint2* __restrict__ tasks, int64_t n, GridInfo g,
+                                                    double* __restrict__ m, unsigned* status,
+                                                    unsigned long long* counters) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nf = 0;
+    if (t < n) {
+        const uint2 k = tasks[t];
+        const double* ra = ta + (int64_t)TREC * k.x;
+        const double* rb = tb + (int64_t)TREC * k.y;
+        double lo[3], hi[3];
+        if (tet_box_overlap(ra, rb, lo, hi)) {
+            PlainScratch W;
+            MassSink sink{m};
+            unsigned st = 0;
+            nf = annihilate_pair64(sink, g, ra, rb, lo, hi, W, st);
+            if (st) atomicOr(status, st);
+        }
+    }
+    block_count(nf, 0, counters);
 }
 
 /* exact in-window mass fraction of each tet (only when the window is not
@@ -937,7 +1035,9 @@ struct DevBuf {
 
 struct Workspace {
     DevBuf pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
-        colsum, colkey, colmom, perm, cut, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt, bidx, bgat;
+        colsum, colkey, colmom, perm, cut, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt, bidx, bgat,
+        an_box, an_valid, an_key, an_key2, an_id, an_id2, an_pcnt, an_poff, an_pairs, an_cntA, an_offA, an_cntB,
+        an_offB, an_ta, an_tb, an_tcnt, an_toff, an_tasks;
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1113,6 +1213,208 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
 }
 
 /* One (mesh, grid) voxelization on stream s with workspace W. */
+/* exclusive sum of cnt[0..n) into off[0..n] (cnt[n] is set to 0 first);
+ * returns off[n] on the host */
+int exclusive_sum(Workspace& W, int64_t* cnt, int64_t* off, int64_t n, hipStream_t s, int dev, int64_t* total) {
+    HIPCHK(hipMemsetAsync(cnt + n, 0, sizeof(int64_t), s));
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, off, n + 1, s));
+    int rc = W.scan_tmp.get(tb, dev);
+    if (rc) return rc;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp.p, tb, cnt, off, n + 1, s));
+    HIPCHK(hipMemcpyAsync(total, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+/* all tets of the elements, expanded with (g, mb): counts, offsets
+ * (offs[e] .. offs[e+1]) and records */
+int expand_all(Workspace& W, DevBuf& cntb, DevBuf& offb, DevBuf& tetb, const MeshDev& md, const GridInfo& g,
+               const MeshBox& mb, int64_t ne, int maxlvl, double tol2, hipStream_t s, int dev, int64_t* nt) {
+    int rc;
+    if ((rc = cntb.get((size_t)(ne + 1) * sizeof(int64_t), dev)) || (rc = offb.get((size_t)(ne + 1) * sizeof(int64_t), dev)))
+        return rc;
+    int64_t* cnt = (int64_t*)cntb.p;
+    int64_t* off = (int64_t*)offb.p;
+    k_expand_count<<<grid_blocks(ne, 128), 128, 0, s>>>(md, g, mb, ne, maxlvl, tol2, cnt);
+    HIPCHK(hipGetLastError());
+    if ((rc = exclusive_sum(W, cnt, off, ne, s, dev, nt))) return rc;
+    if (*nt >= ((int64_t)1 << 32)) return set_err(PSIVOX_E_ARG, "annihilation: more than 2^32 tets");
+    if ((rc = tetb.get((size_t)(*nt > 0 ? *nt : 1) * TREC * sizeof(double), dev))) return rc;
+    k_expand_emit<<<grid_blocks(ne, 128), 128, 0, s>>>(md, g, mb, 0, ne, maxlvl, tol2, off, 0, (double*)tetb.p);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+/*
+ * mode='annihilation' (src/cpsi.c:66-77, 115-158): for every element e in
+ * the window and every element e1 whose wrapped box meets e's, every pair of
+ * their tets (e's wrapped, e1's raw, as the reference refines them) whose
+ * boxes overlap adds rho0 rho1 vol(overlap ∩ cell) into the mass field.
+ * The other fields are untouched.
+ */
+int run_annihilation(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const GridInfo& g,
+                     const psivox_opts* o, psivox_stats* st, hipStream_t s, int dev) {
+    auto t_start = std::chrono::steady_clock::now();
+    int rc;
+    MeshBox mb;
+    mb.periodic = mesh->periodic ? 1 : 0;
+    mb.full_window = 0;
+    for (int a = 0; a < 3; ++a) {
+        mb.lo[a] = mesh->box[0][a];
+        mb.hi[a] = mesh->box[1][a];
+    }
+    GridInfo gu = g; /* unbounded window, no wrap: the raw expansion */
+    MeshBox mu;
+    mu.periodic = 0;
+    mu.full_window = 0;
+    for (int a = 0; a < 3; ++a) {
+        gu.w0[a] = -HUGE_VAL;
+        gu.w1[a] = HUGE_VAL;
+        mu.lo[a] = mu.hi[a] = 0.0;
+    }
+    const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int64_t ne = mesh->nelem;
+    int maxlvl = o->refine_max_lvl < 0 ? 0 : (o->refine_max_lvl > MAXLVL ? MAXLVL : o->refine_max_lvl);
+    if (mesh->elemtype == 4) maxlvl = 0;
+    const double dxmin = std::min(g.d[0], std::min(g.d[1], g.d[2]));
+    const double tol = o->refine_tolerance * dxmin;
+    const double tol2 = tol * tol;
+    if (ne > INT32_MAX) return set_err(PSIVOX_E_ARG, "annihilation: more than 2^31-1 elements");
+
+    MeshDev md{};
+    if ((rc = mesh_to_device(W, mesh, dev, s, md))) return rc;
+    double* mf;
+    if (grid->on_device) {
+        mf = grid->fields[PSIVOX_M];
+    } else {
+        const void* p;
+        if ((rc = upload(W.fields[0], grid->fields[PSIVOX_M], (size_t)nvox * sizeof(double), dev, s, &p))) return rc;
+        mf = (double*)p;
+    }
+    if ((rc = W.counters.get(64, dev))) return rc;
+    unsigned long long* counters = (unsigned long long*)W.counters.p;
+    unsigned* status = (unsigned*)(counters + 4);
+    HIPCHK(hipMemsetAsync(counters, 0, 64, s));
+    int64_t nused = 0, ntask_total = 0;
+    double kms = 0.0;
+    if (ne > 0) {
+        /* element boxes and the broad phase */
+        if ((rc = W.an_box.get((size_t)ne * 6 * sizeof(double), dev)) || (rc = W.an_valid.get((size_t)ne, dev)) ||
+            (rc = W.an_key.get((size_t)ne * sizeof(double), dev)) ||
+            (rc = W.an_key2.get((size_t)ne * sizeof(double), dev)) ||
+            (rc = W.an_id.get((size_t)ne * sizeof(int32_t), dev)) ||
+            (rc = W.an_id2.get((size_t)ne * sizeof(int32_t), dev)) ||
+            (rc = W.an_pcnt.get((size_t)(ne + 1) * sizeof(int64_t), dev)) ||
+            (rc = W.an_poff.get((size_t)(ne + 1) * sizeof(int64_t), dev)))
+            return rc;
+        double* box = (double*)W.an_box.p;
+        k_elem_boxes<<<grid_blocks(ne, 128), 128, 0, s>>>(md, g, mb, ne, box, (int8_t*)W.an_valid.p);
+        HIPCHK(hipGetLastError());
+        k_sweep_keys<<<grid_blocks(ne, 256), 256, 0, s>>>(box, (const int8_t*)W.an_valid.p, ne, (double*)W.an_key.p,
+                                                          (int32_t*)W.an_id.p);
+        HIPCHK(hipGetLastError());
+        size_t tb = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const double*)W.an_key.p, (double*)W.an_key2.p,
+                                                  (const int32_t*)W.an_id.p, (int32_t*)W.an_id2.p, (int)ne, 0, 64, s));
+        if ((rc = W.scan_tmp.get(tb, dev))) return rc;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(W.scan_tmp.p, tb, (const double*)W.an_key.p, (double*)W.an_key2.p,
+                                                  (const int32_t*)W.an_id.p, (int32_t*)W.an_id2.p, (int)ne, 0, 64, s));
+        const double* key = (const double*)W.an_key2.p;
+        const int32_t* id = (const int32_t*)W.an_id2.p;
+        int64_t* pcnt = (int64_t*)W.an_pcnt.p;
+        int64_t* poff = (int64_t*)W.an_poff.p;
+        k_sweep_pairs<<<grid_blocks(ne, 128), 128, 0, s>>>(key, id, box, ne, pcnt, nullptr, nullptr);
+        HIPCHK(hipGetLastError());
+        int64_t npairs = 0;
+        if ((rc = exclusive_sum(W, pcnt, poff, ne, s, dev, &npairs))) return rc;
+        if ((rc = W.an_pairs.get((size_t)(npairs > 0 ? npairs : 1) * sizeof(int2), dev))) return rc;
+        int2* pairs = (int2*)W.an_pairs.p;
+        k_sweep_pairs<<<grid_blocks(ne, 128), 128, 0, s>>>(key, id, box, ne, nullptr, poff, pairs);
+        HIPCHK(hipGetLastError());
+
+        /* the two expansions: wrapped and window-tested (the outer element),
+         * raw (the tree's element) */
+        int64_t nA = 0, nB = 0;
+        if ((rc = expand_all(W, W.an_cntA, W.an_offA, W.an_ta, md, g, mb, ne, maxlvl, tol2, s, dev, &nA))) return rc;
+        if ((rc = expand_all(W, W.an_cntB, W.an_offB, W.an_tb, md, gu, mu, ne, maxlvl, tol2, s, dev, &nB))) return rc;
+        const int64_t* offA = (const int64_t*)W.an_offA.p;
+        const int64_t* offB = (const int64_t*)W.an_offB.p;
+        const double* ta = (const double*)W.an_ta.p;
+        const double* tbp = (const double*)W.an_tb.p;
+
+        /* tet-pair tasks, in chunks of at most cap */
+        if ((rc = W.an_tcnt.get((size_t)(npairs + 1) * sizeof(int64_t), dev)) ||
+            (rc = W.an_toff.get((size_t)(npairs + 1) * sizeof(int64_t), dev)))
+            return rc;
+        int64_t* tcnt = (int64_t*)W.an_tcnt.p;
+        int64_t* toff = (int64_t*)W.an_toff.p;
+        if (npairs > 0) {
+            k_tet_pairs<<<grid_blocks(npairs, 128), 128, 0, s>>>(pairs, 0, npairs, offA, offB, ta, tbp, tcnt, nullptr,
+                                                                 0, nullptr);
+            HIPCHK(hipGetLastError());
+        }
+        if ((rc = exclusive_sum(W, tcnt, toff, npairs, s, dev, &ntask_total))) return rc;
+        const int64_t cap = std::max<int64_t>(1 << 20, std::min<int64_t>(ntask_total, (int64_t)1 << 26));
+        if ((rc = W.an_tasks.get((size_t)cap * sizeof(uint2), dev)) || (rc = W.cut.get(4 * sizeof(int64_t), dev)))
+            return rc;
+        uint2* tasks = (uint2*)W.an_tasks.p;
+        int64_t p0 = 0;
+        while (p0 < npairs) {
+            /* the largest p1 with toff[p1] - toff[p0] <= cap (one element pair has at most
+             * (6 * 8^6)^2 tasks; a chunk always takes at least one pair) */
+            int64_t hcut[3];
+            k_chunk_cut<<<1, 1, 0, s>>>(toff, npairs, p0, cap, (int64_t*)W.cut.p);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hcut, W.cut.p, sizeof hcut, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            const int64_t p1 = hcut[0], nchunk = hcut[2] - hcut[1];
+            if (nchunk > cap) return set_err(PSIVOX_E_ARG, "annihilation: one element pair exceeds the task buffer");
+            if (nchunk > 0) {
+                k_tet_pairs<<<grid_blocks(p1 - p0, 128), 128, 0, s>>>(pairs, p0, p1, offA, offB, ta, tbp, nullptr,
+                                                                      toff, hcut[1], tasks);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipEventRecord(W.ev0, s));
+                k_annihilate<<<grid_blocks(nchunk, 128), 128, 0, s>>>(ta, tbp, tasks, nchunk, g, mf, status, counters);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipEventRecord(W.ev1, s));
+                HIPCHK(hipEventSynchronize(W.ev1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+                kms += ms;
+            }
+            p0 = p1;
+        }
+        /* elements inside the window */
+        std::vector<int8_t> hv((size_t)ne);
+        HIPCHK(hipMemcpyAsync(hv.data(), W.an_valid.p, (size_t)ne, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int8_t v : hv) nused += v != 0;
+        if (st) st->ntets = nA;
+    }
+    unsigned long long hcnt[4] = {0, 0, 0, 0};
+    unsigned hstatus = 0;
+    HIPCHK(hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hstatus, status, sizeof hstatus, hipMemcpyDeviceToHost, s));
+    if (!grid->on_device)
+        HIPCHK(hipMemcpyAsync(grid->fields[PSIVOX_M], mf, (size_t)nvox * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hstatus & ST_OVERFLOW)
+        return set_err(PSIVOX_E_OVERFLOW, "a polyhedron exceeded its vertex budget (degenerate input?)");
+    if (hstatus & ST_BADGEOM) return set_err(PSIVOX_E_OVERFLOW, "inconsistent polyhedron graph (degenerate input?)");
+    if (st) {
+        st->nelem_used = nused;
+        if (ne == 0) st->ntets = 0;
+        st->nghosts = ntask_total; /* tet pairs */
+        st->nfrag = (int64_t)hcnt[0];
+        st->kernel_ms = kms;
+        st->kernel = PSIVOX_KERNEL_PLAIN;
+        st->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
+    return 0;
+}
+
 int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* o, psivox_stats* st,
             hipStream_t s, int dev) {
     auto t_start = std::chrono::steady_clock::now();
@@ -1121,8 +1423,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     GridInfo g;
     rc = fill_grid_info(grid, o->xv_true_covariance, g);
     if (rc) return rc;
-    if (o->mode == PSIVOX_MODE_ANNIHILATION)
-        return set_err(PSIVOX_E_MODE, "mode 'annihilation' is not supported by the GPU build");
+    if (o->mode == PSIVOX_MODE_ANNIHILATION) return run_annihilation(W, mesh, grid, g, o, st, s, dev);
     if (o->mode != PSIVOX_MODE_DENSITY) return set_err(PSIVOX_E_ARG, "invalid mode %d", o->mode);
     int kernel = o->kernel == PSIVOX_KERNEL_AUTO ? PSIVOX_KERNEL_FAST : o->kernel;
     if (kernel != PSIVOX_KERNEL_PLAIN && kernel != PSIVOX_KERNEL_FAST)

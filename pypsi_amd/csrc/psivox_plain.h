@@ -111,6 +111,85 @@ struct PlainScratch {
 };
 
 /*
+ * The window planes and index box of a polyhedron whose bounding box is
+ * [lo, hi] (src/geometry.c:816-843): P is clipped to the window where the
+ * box crosses it; [ib0, ib1) are its cells.  False when nothing is left.
+ * (voxelize_copy64 below does the same steps inline.)
+ */
+PVX_HD bool window_prologue64(Poly64*& P, Poly64*& tmp, const GridInfo& g, const double* lo, const double* hi,
+                              int* ib0, int* ib1, unsigned& status) {
+    for (int a = 0; a < 3; ++a) {
+        double cmin = lo[a], cmax = hi[a];
+        bool clo = false, chi = false;
+        if (g.w0[a] > cmin) {
+            cmin = g.w0[a];
+            clo = true;
+        }
+        if (g.w1[a] < cmax) {
+            cmax = g.w1[a];
+            chi = true;
+        }
+        if (clo) clip_lo64(P, tmp, cmin, a, status);
+        if (chi) clip_hi64(P, tmp, cmax, a, status);
+        index_range(g, a, cmin, cmax, ib0[a], ib1[a]);
+        /* zero extent: one cell, as the reference's leaf rule */
+        if (ib1[a] <= ib0[a]) ib1[a] = ib0[a] + 1;
+        if (ib0[a] >= g.n[a]) return false;
+    }
+    return P->nv > 0;
+}
+
+/*
+ * Sweep P through its cells [ib0, ib1): x slabs, then y columns, then z
+ * cells, each cut off the remainder in turn; leaf(i, j, k, V) gets every
+ * non-empty cell piece and returns 1 when it deposited a fragment.  The
+ * pieces are the reference's DFS leaves (src/geometry.c:852-896).  Returns
+ * the fragment count.
+ */
+template <class Leaf>
+PVX_HD int64_t sweep_cells64(Poly64* P, Poly64* S, Poly64* C, Poly64* V, const GridInfo& g, const int* ib0,
+                             const int* ib1, unsigned& status, Leaf&& leaf) {
+    int64_t nfrag = 0;
+    for (int i = ib0[0]; i < ib1[0]; ++i) {
+        if (i < ib1[0] - 1) {
+            split64(P, S, g.d[0] * (i + 1) + g.w0[0], 0, status);
+        } else {
+            swap_ptr(P, S);
+            P->nv = 0;
+        }
+        if (S->nv > 0) {
+            int j0, j1;
+            poly_range64(S, g, 1, ib0[1], ib1[1], j0, j1);
+            for (int j = j0; j < j1; ++j) {
+                if (j < j1 - 1) {
+                    split64(S, C, g.d[1] * (j + 1) + g.w0[1], 1, status);
+                } else {
+                    swap_ptr(S, C);
+                    S->nv = 0;
+                }
+                if (C->nv > 0) {
+                    int k0, k1;
+                    poly_range64(C, g, 2, ib0[2], ib1[2], k0, k1);
+                    for (int k = k0; k < k1; ++k) {
+                        if (k < k1 - 1) {
+                            split64(C, V, g.d[2] * (k + 1) + g.w0[2], 2, status);
+                        } else {
+                            swap_ptr(C, V);
+                            C->nv = 0;
+                        }
+                        if (V->nv > 0) nfrag += leaf(i, j, k, V);
+                        if (C->nv <= 0) break;
+                    }
+                }
+                if (S->nv <= 0) break;
+            }
+        }
+        if (P->nv <= 0) break;
+    }
+    return nfrag;
+}
+
+/*
  * Voxelize one periodic copy: x = its 4 (shifted) positions, u velocities,
  * [lo, hi] its bounding box.  Returns the number of fragments deposited.
  */

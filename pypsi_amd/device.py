@@ -120,9 +120,10 @@ def make_grid(n, bounds, fields: dict) -> Grid:
     return g
 
 
-def make_opts(kernel="auto", refine_tolerance=1.0, refine_max_lvl=0, stream=None, xv_true_covariance=False) -> Opts:
+def make_opts(kernel="auto", refine_tolerance=1.0, refine_max_lvl=0, stream=None, xv_true_covariance=False,
+              mode="density") -> Opts:
     o = Opts()
-    o.mode = 0
+    o.mode = {"density": 0, "annihilation": 1}[mode]
     o.refine_tolerance = float(refine_tolerance)
     o.refine_max_lvl = int(refine_max_lvl)
     o.kernel = KERNELS[kernel]

@@ -17,6 +17,7 @@
 #include "../../pypsi_amd/csrc/psivox_fast.h"
 #include "../../pypsi_amd/csrc/psivox_adjoint.h"
 #include "../../pypsi_amd/csrc/psivox_query.h"
+#include "../../pypsi_amd/csrc/psivox_annihilate.h"
 
 using namespace pvx;
 
@@ -306,4 +307,80 @@ extern "C" int64_t hs_sample_vdf(int vpe, int64_t nelem, const double* pos, cons
     }
     delete RS;
     return n;
+}
+
+/* annihilation mode with the device's pair code (annihilate_pair64) and
+ * expansion (expand_element): element boxes as k_elem_boxes, every ordered
+ * pair of elements whose boxes meet, tets of the first wrapped and of the
+ * second raw, tet pairs whose boxes overlap.  Adds into m; returns the
+ * fragment count, out_pairs = tet pairs. */
+extern "C" int64_t hs_annihilation(int vpe, int64_t nelem, const double* pos, const double* mass,
+                                   const int32_t* conn, int periodic, const double* box6, const int* n,
+                                   const double* win6, const double* d, double* m, double reftol, int maxlvl,
+                                   int64_t* out_pairs) {
+    GridInfo g;
+    MeshBox mb;
+    double* fields[6] = {m, nullptr, nullptr, nullptr, nullptr, nullptr};
+    fill_grid(g, mb, n, win6, d, fields, 0, periodic, box6);
+    mb.full_window = 0;
+    GridInfo gu = g;
+    MeshBox mu;
+    mu.periodic = 0;
+    mu.full_window = 0;
+    for (int a = 0; a < 3; ++a) {
+        gu.w0[a] = -HUGE_VAL;
+        gu.w1[a] = HUGE_VAL;
+        mu.lo[a] = mu.hi[a] = 0.0;
+    }
+    double dxmin = d[0] < d[1] ? d[0] : d[1];
+    dxmin = dxmin < d[2] ? dxmin : d[2];
+    const double tol2 = (reftol * dxmin) * (reftol * dxmin);
+    if (vpe == 4) maxlvl = 0;
+    std::vector<double> blo((size_t)nelem * 3), bhi((size_t)nelem * 3);
+    std::vector<int> valid((size_t)nelem);
+    for (int64_t e = 0; e < nelem; ++e) {
+        double P[27][3];
+        for (int v = 0; v < vpe; ++v)
+            for (int a = 0; a < 3; ++a) P[v][a] = pos[3 * (int64_t)conn[e * vpe + v] + a];
+        valid[(size_t)e] = wrap_and_test(P, vpe, g, mb);
+        aabb((const double (*)[3])P, vpe, &blo[(size_t)e * 3], &bhi[(size_t)e * 3]);
+    }
+    RefineScratch* RS = new RefineScratch;
+    PlainScratch* W = new PlainScratch;
+    struct MSink {
+        double* m;
+        void add(int64_t i, double v) { m[i] += v; }
+    } sink{m};
+    std::vector<double> ta, tb;
+    int64_t nfrag = 0, npairs = 0;
+    unsigned status = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        if (!valid[(size_t)e]) continue;
+        const int64_t na = expand_element(pos, nullptr, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, nullptr);
+        ta.resize((size_t)na * TREC);
+        expand_element(pos, nullptr, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, ta.data());
+        QueryBox q;
+        for (int a = 0; a < 3; ++a) {
+            q.lo[a] = blo[(size_t)e * 3 + a];
+            q.hi[a] = bhi[(size_t)e * 3 + a];
+        }
+        for (int64_t e1 = 0; e1 < nelem; ++e1) {
+            if (!valid[(size_t)e1] || !boxes_meet(&blo[(size_t)e1 * 3], &bhi[(size_t)e1 * 3], q)) continue;
+            const int64_t nb = expand_element(pos, nullptr, mass, conn, vpe, e1, gu, mu, maxlvl, tol2, *RS, nullptr);
+            tb.resize((size_t)nb * TREC);
+            expand_element(pos, nullptr, mass, conn, vpe, e1, gu, mu, maxlvl, tol2, *RS, tb.data());
+            for (int64_t t = 0; t < na; ++t)
+                for (int64_t t1 = 0; t1 < nb; ++t1) {
+                    double lo[3], hi[3];
+                    if (!tet_box_overlap(&ta[(size_t)t * TREC], &tb[(size_t)t1 * TREC], lo, hi)) continue;
+                    npairs++;
+                    nfrag += annihilate_pair64(sink, g, &ta[(size_t)t * TREC], &tb[(size_t)t1 * TREC], lo, hi, *W,
+                                               status);
+                }
+        }
+    }
+    delete RS;
+    delete W;
+    *out_pairs = npairs;
+    return status ? -(int64_t)status : nfrag;
 }

@@ -71,3 +71,21 @@ def sample_vdf(P, x, reftol=1.0, maxlvl=0, window=None):
     rho, vel = np.empty(n), np.empty((n, 3))
     L.hs_sample_vdf(*args, vp(rho), vp(vel), C.c_int64(n))
     return rho, vel
+
+
+def annihilation(P, reftol=1.0, maxlvl=0, out=None):
+    """mode='annihilation' with the device's pair code on the host: (fields,
+    fragments, tet pairs)."""
+    L = lib()
+    L.hs_annihilation.restype = C.c_int64
+    out = P.zero_fields() if out is None else out
+    n = (C.c_int * 3)(*P.n)
+    win = (C.c_double * 6)(*P.bounds.reshape(-1))
+    d = (C.c_double * 3)(*P.d)
+    box = (C.c_double * 6)(*P.box.reshape(-1))
+    vp = lambda a: C.c_void_p(a.ctypes.data)
+    pairs = C.c_int64(0)
+    nf = L.hs_annihilation(C.c_int(P.elemtype), C.c_int64(P.conn.shape[0]), vp(P.pos), vp(P.mass), vp(P.conn),
+                           C.c_int(int(P.periodic)), box, n, win, d, vp(out["m"]), C.c_double(reftol),
+                           C.c_int(maxlvl), C.byref(pairs))
+    return out, nf, pairs.value

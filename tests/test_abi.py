@@ -76,5 +76,3 @@ def test_module_argument_errors():
     assert m.connectivity.shape == (27, 8) and m.mass.shape == (3, 3, 3)  # (ne, ne, ne) as src/PSI.c:530-545
     with pytest.raises(ValueError, match="Invalid mode for PSI.voxels"):
         PSI.voxels(grid=g, mesh=m, mode="nonsense")
-    with pytest.raises(NotImplementedError):
-        PSI.voxels(grid=g, mesh=m, mode="annihilation")
