@@ -46,6 +46,11 @@ def test_abi_version_and_argument_errors():
     assert b"required" in L.psivox_last_error()
     assert L.psivox_mass_adjoint(None, None, None, None, None) == -1
     assert L.psivox_voxels_batched(-1, None, None, None, None) == -1
+    # the point-query entry points likewise
+    assert L.psivox_elem_index_build(None, None, None, None) == -1
+    assert L.psivox_elem_index_query(None, None, None, 0, None, None) == -1
+    assert L.psivox_sample_vdf(None, None, 1.0, 0, None, None, 0, None, None) == -1
+    L.psivox_elem_index_free(None)
 
 
 def test_struct_layouts_match_header():
@@ -76,3 +81,7 @@ def test_module_argument_errors():
     assert m.connectivity.shape == (27, 8) and m.mass.shape == (3, 3, 3)  # (ne, ne, ne) as src/PSI.c:530-545
     with pytest.raises(ValueError, match="Invalid mode for PSI.voxels"):
         PSI.voxels(grid=g, mesh=m, mode="nonsense")
+    with pytest.raises(TypeError):
+        PSI.VDF(m, (0.5, 0.5))
+    with pytest.raises(TypeError):
+        PSI.RStarTree(mesh=3)
