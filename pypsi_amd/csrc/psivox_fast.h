@@ -332,27 +332,29 @@ PVX_HD void tri_acc(Mom& M, const float* A, const float* B, const float* C) {
  * (v0, vi, vi+1) exactly as the reference's reduce fans it.
  */
 PVX_HD void face_apexes(LPoly& P) {
-    const uint32_t live = P.live;
-    for (uint32_t lm = live; lm; lm &= lm - 1u) {
-        const int v = __builtin_ctz(lm);
-        P.setnbw(v, P.nbw(v) & 0x7fffu);
-    }
     uint64_t mark = 0;
     for (int r = 0; r < P.nv; ++r) {
         const int vs = P.at(r);
         for (int ks = 0; ks < 3; ++ks) {
             if (mark & (1ull << (3 * vs + ks))) continue;
             int a = vs, ka = ks, guard = 0;
+            /* a's link word as it is in LDS: read once per step, for the
+             * next vertex, and carried (nothing writes it in between) */
+            uint32_t wa = P.nbw(a);
             do {
                 mark |= 1ull << (3 * a + ka);
-                const uint32_t wa = P.nbw(a);
-                P.setnbw(a, wa | ((uint32_t)vs << (15 + 5 * ka)));
+                /* every directed edge lies on one face: its apex field is
+                 * written exactly once, so stale bits need no clearing */
+                const int sh = 15 + 5 * ka;
+                wa = (wa & ~(31u << sh)) | ((uint32_t)vs << sh);
+                P.setnbw(a, wa);
                 const int b = nbf(wa, ka);
                 const uint32_t wb = P.nbw(b);
                 int kb = 0;
                 while (kb < 2 && nbf(wb, kb) != a) kb++;
                 a = b;
                 ka = (kb + 1) % 3;
+                wa = wb;
             } while (!(a == vs && ka == ks) && ++guard <= 3 * NVF);
         }
     }
