@@ -2,7 +2,8 @@
  * psi_oracle.c -- TEST INFRASTRUCTURE ONLY (the oracle).
  *
  * FP64 CPU restatement of PSI's exact conservative voxelization of
- * phase-space-sheet tetrahedra ("PSI.voxels", mode='density').  It exists to
+ * phase-space-sheet tetrahedra ("PSI.voxels", mode='density'), plus the
+ * point queries and mode='annihilation' (listed below).  It exists to
  * check the HIP product path and to time a CPU baseline; nothing in the
  * product (pypsi_amd/) links or calls it.
  *
@@ -24,6 +25,20 @@
  * Extras the reference does not have: fragment counting (stats), an OpenMP
  * element loop with private grids merged by the same online formula, and a
  * "raw moment" deposit used by the adjoint checks (psi_oracle_adjoint.c).
+ *
+ * Beyond mode='density' (end of this file), each pinned against the
+ * reference's own C core in tests/ (to a tolerance: these restate the
+ * quantities, not every expression):
+ *   - orc_elem_query, orc_sample_vdf: the R*-tree query and PSI.VDF's
+ *     multistream sample (src/rtree.c:48-123, src/cpsi.c:177-275); Cramer's
+ *     rule for the barycentric coordinates instead of the reference's face
+ *     planes (src/geometry.c:335-389); element sets equal, stream values
+ *     within 1e-11;
+ *   - orc_annihilation: mode='annihilation' (src/cpsi.c:115-158,
+ *     src/geometry.c:190-252, 503-558); face planes as cross products over
+ *     the face's vertices, brute-force pairs in element order; within 1e-8
+ *     relative on cells holding >= 1e-3 of the peak (caustic cells are
+ *     ill-conditioned, 2.4e-9 measured).
  */
 #include <math.h>
 #include <stdint.h>
