@@ -1,4 +1,7 @@
-"""Write the golden vectors in this folder from the reference's own C core.
+"""Write the golden vectors in this folder from the reference's own C core:
+g*: PSI.voxels (density); ann_*: mode='annihilation'; vdf_*: PSI.VDF's
+streams (in the reference's order, per point); tree_*: the R*-tree query
+(element indices sorted).
 
 Runs only where the reference checkout is present (it builds oracle/_ref from
 the reference sources with oracle/Makefile).  Each fixture holds its inputs
@@ -70,9 +73,45 @@ def main():
     pos, vel = zeldovich_block(6, 2.0, seed=5)
     P = O.Problem.block(pos, vel, n=(9, 7, 8), bounds=((0.15, 0.2, 0.1), (0.75, 0.8, 0.7)), fields=["v", "vv"])
     save("g4_window_open", P, 1.0, 0, O.ref_voxels(P))
+    modes()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+def modes():
+    """Fixtures of the reference's other entry points: mode='annihilation'
+    (ann_*), PSI.VDF (vdf_*) and the R*-tree query (tree_*)."""
+    # annihilation: 5^3 periodic block, rms 1.5 cells -> 10^3, mass field, levels 0 and 1
+    pos, vel = zeldovich_block(5, 1.5, seed=11)
+    for lvl in (0, 1):
+        P = O.Problem.block(pos, vel, periodic_box=BOX, n=(10, 10, 10), bounds=BOX)
+        out = O.ref_voxels(P, reftol=0.2, maxlvl=lvl, mode=1)
+        np.savez_compressed(os.path.join(HERE, f"ann_block5_L{lvl}.npz"), pos=pos, vel=vel, n=np.array(P.n),
+                            reftol=np.array(0.2), maxlvl=np.array(lvl), out_m=out["m"])
+    # VDF and tree queries: 8^3 periodic block, rms 2.0 cells
+    pos, vel = zeldovich_block(8, 2.0, seed=5)
+    P = O.Problem.block(pos, vel, periodic_box=BOX, n=(8, 8, 8), bounds=BOX)
+    rng = np.random.default_rng(77)
+    pts = rng.uniform(0.0, 1.0, size=(40, 3))
+    cnt, rho, vv = [], [], []
+    for x in pts:
+        r, v = O.ref_sample_vdf(P, x, reftol=0.02, maxlvl=1)
+        cnt.append(len(r))
+        rho.append(r)
+        vv.append(v.reshape(-1, 3))
+    np.savez_compressed(os.path.join(HERE, "vdf_block8.npz"), pos=pos, vel=vel, points=pts, reftol=np.array(0.02),
+                        maxlvl=np.array(1), count=np.array(cnt), rho=np.concatenate(rho),
+                        vel_out=np.concatenate(vv))
+    W = O.tree_window(P)
+    boxes = []
+    for _ in range(20):
+        c = rng.uniform(-0.1, 1.1, 3)
+        h = rng.uniform(0.0, 0.2, 3)
+        boxes.append(np.array([c - h, c + h]))
+    hits = [np.sort(O.ref_tree_query(P, W, q)) for q in boxes]
+    np.savez_compressed(os.path.join(HERE, "tree_block8.npz"), pos=pos, vel=vel, boxes=np.array(boxes),
+                        count=np.array([len(h) for h in hits]), elems=np.concatenate(hits).astype(np.int32))
 
 
 if __name__ == "__main__":

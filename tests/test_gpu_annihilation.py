@@ -88,3 +88,17 @@ def test_lattice_known_answer(PSI):
     grid = PSI.Grid(n=(G,) * 3, bounds=BOX)
     PSI.voxels(grid=grid, mesh=PSI.Mesh(loader="block", pos=g, periodic_box=BOX), mode="annihilation")
     np.testing.assert_allclose(grid.fields["m"][3:G - 1, 3:G - 1, 3:G - 1], float(N ** 3) ** 2 / G ** 3, rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", __import__("mode_fixtures").ann_cases())
+def test_reference_golden(PSI, name):
+    """Against the reference's own output (tests/golden, written by its C core)."""
+    import mode_fixtures as MF
+    z = MF.load(name)
+    print(f"case: annihilation golden {name}", flush=True)
+    grid = PSI.Grid(n=tuple(int(v) for v in z["n"]), bounds=BOX)
+    mesh = PSI.Mesh(loader="block", pos=z["pos"], vel=z["vel"], periodic_box=BOX)
+    PSI.voxels(grid=grid, mesh=mesh, mode="annihilation", refine_tolerance=float(z["reftol"]),
+               refine_max_lvl=int(z["maxlvl"]))
+    rel, ab = MF.check_annihilation(grid.fields["m"], z["out_m"], name)
+    print(f"{name}: heavy rel {rel:.1e}, abs {ab:.1e} of peak", flush=True)

@@ -153,3 +153,23 @@ def test_vdf_known_answers(PSI):
         if len(rho):
             np.testing.assert_allclose(vel, np.tile(A @ x + b, (len(rho), 1)), atol=1e-13)
     assert odd == 30 and multi > 0
+
+
+def test_reference_golden(PSI):
+    """VDF streams and tree queries against the reference's own outputs
+    (tests/golden, written by its C core)."""
+    import mode_fixtures as MF
+    print("case: VDF / tree golden", flush=True)
+    z = MF.load("vdf_block8.npz")
+    mesh = PSI.Mesh(loader="block", pos=z["pos"], vel=z["vel"], periodic_box=BOX)
+    tree = PSI.RStarTree(mesh)
+    for x, e in zip(z["points"], MF.vdf_expected(z)):
+        got = PSI.VDF(mesh, tuple(x), tree=tree, refine_tolerance=float(z["reftol"]), refine_max_lvl=int(z["maxlvl"]))
+        MF.check_streams(got, e, f"x={x}")
+        MF.check_streams(PSI.VDF(mesh, tuple(x), refine_tolerance=float(z["reftol"]),
+                                 refine_max_lvl=int(z["maxlvl"])), e, f"x={x} no tree")
+    t = MF.load("tree_block8.npz")
+    tmesh = PSI.Mesh(loader="block", pos=t["pos"], vel=t["vel"], periodic_box=BOX)
+    ttree = PSI.RStarTree(tmesh)
+    for q, e in zip(t["boxes"], MF.tree_expected(t)):
+        np.testing.assert_array_equal(ttree.query((tuple(q[0]), tuple(q[1]))), e)

@@ -113,7 +113,7 @@ GOLDEN = os.path.join(HERE, "golden")
 def golden_cases():
     if not os.path.isdir(GOLDEN):
         return []
-    return sorted(f for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+    return sorted(f for f in os.listdir(GOLDEN) if f.startswith("g") and f.endswith(".npz"))
 
 
 @pytest.mark.parametrize("name", golden_cases())
