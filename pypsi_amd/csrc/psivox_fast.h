@@ -383,29 +383,31 @@ PVX_HD void tri_acc_w(Mom& M, const float* A, const float* B, const float* C, fl
     }
 }
 
-/* a point of P on the plane z = p (z = zd . q): the crossing of the first
- * edge found between z <= p and z > p; false when no edge crosses */
-PVX_HD bool plane_point(const LPoly& P, float zd0, float zd1, float zd2, float p, float* out) {
-    bool found = false;
-    for (int r = 0; r < P.nv && !found; ++r) {
+
+/* A point of P on the plane z = p (z = zd . q): on the first edge, in rank
+ * order then neighbour order, from a vertex at or below p to one above it;
+ * le has bit v set when z(v) <= p, so only that edge's vertices are read.
+ * False when no edge crosses (P lies on one side). */
+PVX_HD bool plane_point_le(const LPoly& P, float zd0, float zd1, float zd2, float p, uint32_t le, float* out) {
+    for (int r = 0; r < P.nv; ++r) {
         const int v = P.at(r);
-        const float q0 = P.qa(v, 0), q1 = P.qa(v, 1), q2 = P.qa(v, 2);
-        const float zv = zd0 * q0 + zd1 * q1 + zd2 * q2;
+        if (!((le >> v) & 1u)) continue;
         const uint32_t w = P.nbw(v);
-        for (int k = 0; k < 3 && !found; ++k) {
+        for (int k = 0; k < 3; ++k) {
             const int b = nbf(w, k);
+            if ((le >> b) & 1u) continue;
+            const float q0 = P.qa(v, 0), q1 = P.qa(v, 1), q2 = P.qa(v, 2);
+            const float zv = zd0 * q0 + zd1 * q1 + zd2 * q2;
             const float r0 = P.qa(b, 0), r1 = P.qa(b, 1), r2 = P.qa(b, 2);
             const float zb = zd0 * r0 + zd1 * r1 + zd2 * r2;
-            if (zv <= p && zb > p) {
-                const float t = (p - zv) * frcp(zb - zv);
-                out[0] = q0 + t * (r0 - q0);
-                out[1] = q1 + t * (r1 - q1);
-                out[2] = q2 + t * (r2 - q2);
-                found = true;
-            }
+            const float t = (p - zv) * frcp(zb - zv);
+            out[0] = q0 + t * (r0 - q0);
+            out[1] = q1 + t * (r1 - q1);
+            out[2] = q2 + t * (r2 - q2);
+            return true;
         }
     }
-    return found;
+    return false;
 }
 
 /* exit and entry points of the triangle (c, a, b), traversed c -> a -> b,
@@ -488,31 +490,39 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
     S.zd[0] = zd0;
     S.zd[1] = zd1;
     S.zd[2] = zd2;
+    /* each vertex's side of the two planes, once: bit v of above is
+     * z(v) > lo, of below z(v) <= hi */
+    uint32_t above = 0u, below = 0u;
+    for (uint32_t lm = P.live; lm; lm &= lm - 1u) {
+        const int v = __builtin_ctz(lm);
+        const float z = zd0 * P.qa(v, 0) + zd1 * P.qa(v, 1) + zd2 * P.qa(v, 2);
+        above |= (z > S.lo ? 1u : 0u) << v;
+        below |= (z <= S.hi ? 1u : 0u) << v;
+    }
     const int v0 = P.at(0);
     float O[3] = {P.qa(v0, 0), P.qa(v0, 1), P.qa(v0, 2)};
     float Gh[3] = {O[0], O[1], O[2]};
     if (has_lo) {
-        plane_point(P, zd0, zd1, zd2, S.lo, O);
-        if (has_hi && !plane_point(P, zd0, zd1, zd2, S.hi, Gh)) Gh[0] = O[0], Gh[1] = O[1], Gh[2] = O[2];
+        plane_point_le(P, zd0, zd1, zd2, S.lo, ~above, O);
+        if (has_hi && !plane_point_le(P, zd0, zd1, zd2, S.hi, below, Gh)) Gh[0] = O[0], Gh[1] = O[1], Gh[2] = O[2];
     } else if (has_hi) {
-        plane_point(P, zd0, zd1, zd2, S.hi, O);
+        plane_point_le(P, zd0, zd1, zd2, S.hi, below, O);
     }
     for (int i = 0; i < 3; ++i) {
         S.O[i] = O[i];
         S.gh[i] = Gh[i] - O[i];
     }
+    /* a fan triangle reaches into the slab when its highest vertex is above
+     * lo and its lowest at or below hi: one of its vertices each */
     uint64_t mask = 0;
     for (int ra = 0; ra < P.nv; ++ra) {
         const int a = P.at(ra);
         const uint32_t wa = P.nbw(a);
-        const float za = zd0 * P.qa(a, 0) + zd1 * P.qa(a, 1) + zd2 * P.qa(a, 2);
         for (int k = 0; k < 3; ++k) {
             const int b = nbf(wa, k), c = apf(wa, k);
             if (c == a || c == b) continue;
-            const float zb = zd0 * P.qa(b, 0) + zd1 * P.qa(b, 1) + zd2 * P.qa(b, 2);
-            const float zc = zd0 * P.qa(c, 0) + zd1 * P.qa(c, 1) + zd2 * P.qa(c, 2);
-            const float zmax = fmaxf(za, fmaxf(zb, zc)), zmin = fminf(za, fminf(zb, zc));
-            if (zmax > S.lo && zmin <= S.hi) mask |= 1ull << (3 * ra + k);
+            const uint32_t tri = (1u << a) | (1u << b) | (1u << c);
+            if ((above & tri) && (below & tri)) mask |= 1ull << (3 * ra + k);
         }
     }
     S.mask = mask;
