@@ -468,7 +468,9 @@ struct SlabCell {
     float O[3], gh[3];
     float zd[3]; /* x_ax(q) - x_ax(q = 0) = zd . q */
     bool has_lo, has_hi;
-    uint64_t mask; /* bit 3r + k: the fan triangle of directed edge a -> k, a the vertex of rank r */
+    uint64_t mask;   /* bit 3r + k: the fan triangle of directed edge a -> k, a the vertex of rank r,
+                        reaching into the slab and cut by one of its planes */
+    uint64_t inside; /* the same for fan triangles wholly inside the slab */
 };
 
 /*
@@ -513,8 +515,10 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
         S.gh[i] = Gh[i] - O[i];
     }
     /* a fan triangle reaches into the slab when its highest vertex is above
-     * lo and its lowest at or below hi: one of its vertices each */
-    uint64_t mask = 0;
+     * lo and its lowest at or below hi: one of its vertices each; it lies
+     * wholly inside when all three are */
+    const uint32_t both = above & below;
+    uint64_t mask = 0, inside = 0;
     for (int ra = 0; ra < P.nv; ++ra) {
         const int a = P.at(ra);
         const uint32_t wa = P.nbw(a);
@@ -522,10 +526,15 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
             const int b = nbf(wa, k), c = apf(wa, k);
             if (c == a || c == b) continue;
             const uint32_t tri = (1u << a) | (1u << b) | (1u << c);
-            if ((above & tri) && (below & tri)) mask |= 1ull << (3 * ra + k);
+            const uint64_t bit = 1ull << (3 * ra + k);
+            if ((both & tri) == tri)
+                inside |= bit;
+            else if ((above & tri) && (below & tri))
+                mask |= bit;
         }
     }
     S.mask = mask;
+    S.inside = inside;
 }
 
 /*
@@ -591,6 +600,22 @@ PVX_HD void pair_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
     }
 }
 
+/* The moments (about S.O) of a fan triangle T = (c, a, b) wholly inside the
+ * slab: the cone over T from O, the one term of pair_terms that does not
+ * vanish there (its edge pieces are the whole edges, it has no segment on
+ * either plane). */
+template <int ORDER>
+PVX_HD void inside_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
+    const int ra = e / 3, k = e - 3 * ra, a = P.at(ra);
+    const uint32_t wa = P.nbw(a);
+    const int b = nbf(wa, k), c = apf(wa, k);
+    const float* O = S.O;
+    const float qa[3] = {P.qa(a, 0) - O[0], P.qa(a, 1) - O[1], P.qa(a, 2) - O[2]};
+    const float qb[3] = {P.qa(b, 0) - O[0], P.qa(b, 1) - O[1], P.qa(b, 2) - O[2]};
+    const float qc[3] = {P.qa(c, 0) - O[0], P.qa(c, 1) - O[1], P.qa(c, 2) - O[2]};
+    tri_acc<ORDER>(M, qa, qc, qb);
+}
+
 /* moments of P's piece in the slab, one lane doing every fan triangle */
 template <int ORDER>
 PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi,
@@ -601,6 +626,12 @@ PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo
     O[0] = S.O[0];
     O[1] = S.O[1];
     O[2] = S.O[2];
+    uint64_t in = S.inside;
+    while (in) {
+        const int e = __builtin_ctzll(in);
+        in &= in - 1;
+        inside_terms<ORDER>(P, e, S, M);
+    }
     uint64_t mask = S.mask;
     while (mask) {
         const int e = __builtin_ctzll(mask);
