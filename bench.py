@@ -553,7 +553,8 @@ def main():
                     help="count a refined workload's fragments with the FP64 kernel into profiles/ and exit")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the plain-kernel, forward+adjoint, batched, refined, end-to-end and snapshot legs")
-    ap.add_argument("--only", choices=["snapshot", "vdf", "annihilation"], help="run one extra leg alone and print it")
+    ap.add_argument("--only", choices=["snapshot", "vdf", "annihilation", "end_to_end"],
+                    help="run one extra leg alone and print it")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -583,6 +584,10 @@ def main():
     pos, vel, mass, conn, gn, fields = build_inputs(wname)
     if args.only == "vdf":
         print(json.dumps({"vdf": vdf_leg(pos, vel)}), flush=True)
+        return
+    if args.only == "end_to_end":
+        nfrag = load_counts().get(wname, {}).get("nfrag")
+        print(json.dumps({"end_to_end": end_to_end_leg(pos, vel, mass, conn, gn, fields, nfrag)}), flush=True)
         return
     box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
     dev = torch.device("cuda:0")

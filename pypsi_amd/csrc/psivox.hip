@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <new>
 #include <cmath>
 #include <string>
@@ -1078,10 +1079,108 @@ int check_mesh(const psivox_mesh* m) {
     return 0;
 }
 
+/* Host <-> device copies of large caller arrays (pageable memory) go
+ * through two pinned buffers in chunks: a multi-threaded host memcpy fills
+ * (or drains) one while the DMA engine moves the other.  Pageable copies
+ * run at 29 GB/s up and 17 GB/s down on the MI355X host; pinned DMA at 57.
+ * Callers hold g_ws_mutex. */
+constexpr size_t STAGE_CHUNK = (size_t)64 << 20;
+constexpr size_t STAGE_MIN = (size_t)16 << 20; /* smaller copies go direct */
+
+struct Staging {
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+};
+Staging g_stage;
+
+void par_memcpy(void* dst, const void* src, size_t n) {
+    unsigned nth = std::thread::hardware_concurrency();
+    nth = nth < 1 ? 1 : (nth > 8 ? 8 : nth);
+    if (n < ((size_t)4 << 20) || nth == 1) {
+        memcpy(dst, src, n);
+        return;
+    }
+    std::vector<std::thread> ts;
+    const size_t ch = (n + nth - 1) / nth;
+    for (unsigned t = 1; t < nth; ++t) {
+        const size_t o = t * ch;
+        if (o >= n) break;
+        const size_t m = std::min(ch, n - o);
+        ts.emplace_back([=] { memcpy((char*)dst + o, (const char*)src + o, m); });
+    }
+    memcpy(dst, src, std::min(ch, n));
+    for (auto& t : ts) t.join();
+}
+
+int stage_ready() {
+    for (int b = 0; b < 2; ++b) {
+        if (!g_stage.buf[b]) HIPCHK(hipHostMalloc(&g_stage.buf[b], STAGE_CHUNK, hipHostMallocDefault));
+        if (!g_stage.ev[b]) HIPCHK(hipEventCreateWithFlags(&g_stage.ev[b], hipEventDisableTiming));
+    }
+    return 0;
+}
+
+/* wait until the DMA that last used stage buffer b has finished */
+int stage_wait(int b) {
+    if (g_stage.pending[b]) HIPCHK(hipEventSynchronize(g_stage.ev[b]));
+    g_stage.pending[b] = false;
+    return 0;
+}
+
+int copy_h2d(void* d, const void* h, size_t bytes, hipStream_t s) {
+    if (bytes < STAGE_MIN) {
+        HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+        return 0;
+    }
+    int rc = stage_ready();
+    if (rc) return rc;
+    for (size_t off = 0, i = 0; off < bytes; off += STAGE_CHUNK, ++i) {
+        const int b = (int)(i & 1);
+        const size_t m = std::min(STAGE_CHUNK, bytes - off);
+        if ((rc = stage_wait(b))) return rc;
+        par_memcpy(g_stage.buf[b], (const char*)h + off, m);
+        HIPCHK(hipMemcpyAsync((char*)d + off, g_stage.buf[b], m, hipMemcpyHostToDevice, s));
+        HIPCHK(hipEventRecord(g_stage.ev[b], s));
+        g_stage.pending[b] = true;
+    }
+    return 0;
+}
+
+/* returns when the host copy is complete (work queued on s before it
+ * included) */
+int copy_d2h(void* h, const void* d, size_t bytes, hipStream_t s) {
+    if (bytes < STAGE_MIN) {
+        HIPCHK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    }
+    int rc = stage_ready();
+    if (rc) return rc;
+    const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    for (size_t i = 0; i <= nch; ++i) {
+        if (i < nch) {
+            const int b = (int)(i & 1);
+            const size_t off = i * STAGE_CHUNK, m = std::min(STAGE_CHUNK, bytes - off);
+            if ((rc = stage_wait(b))) return rc;
+            HIPCHK(hipMemcpyAsync(g_stage.buf[b], (const char*)d + off, m, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipEventRecord(g_stage.ev[b], s));
+            g_stage.pending[b] = true;
+        }
+        if (i >= 1) {
+            const int b = (int)((i - 1) & 1);
+            const size_t off = (i - 1) * STAGE_CHUNK, m = std::min(STAGE_CHUNK, bytes - off);
+            if ((rc = stage_wait(b))) return rc;
+            par_memcpy((char*)h + off, g_stage.buf[b], m);
+        }
+    }
+    return 0;
+}
+
 int upload(DevBuf& b, const void* src, size_t bytes, int dev, hipStream_t s, const void** out) {
     int rc = b.get(bytes, dev);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+    if ((rc = copy_h2d(b.p, src, bytes, s))) return rc;
     *out = b.p;
     return 0;
 }
@@ -1396,8 +1495,7 @@ int run_annihilation(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, c
     unsigned hstatus = 0;
     HIPCHK(hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&hstatus, status, sizeof hstatus, hipMemcpyDeviceToHost, s));
-    if (!grid->on_device)
-        HIPCHK(hipMemcpyAsync(grid->fields[PSIVOX_M], mf, (size_t)nvox * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (!grid->on_device && (rc = copy_d2h(grid->fields[PSIVOX_M], mf, (size_t)nvox * sizeof(double), s))) return rc;
     HIPCHK(hipStreamSynchronize(s));
     if (hstatus & ST_OVERFLOW)
         return set_err(PSIVOX_E_OVERFLOW, "a polyhedron exceeded its vertex budget (degenerate input?)");
@@ -1617,9 +1715,9 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     k_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, acc, fptrs, nvox);
     HIPCHK(hipGetLastError());
     for (int f = 0; f < 6; ++f)
-        if (grid->fields[f] && !grid->on_device)
-            HIPCHK(hipMemcpyAsync(grid->fields[f], fd[f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double),
-                                  hipMemcpyDeviceToHost, s));
+        if (grid->fields[f] && !grid->on_device &&
+            (rc = copy_d2h(grid->fields[f], fd[f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double), s)))
+            return rc;
     unsigned long long hc[8];
     HIPCHK(hipMemcpyAsync(hc, counters, 64, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1854,18 +1952,18 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                 md[q].mass = m.mass;
                 md[q].conn = m.conn;
             } else if (m.nelem > 0) {
-                HIPCHK(hipMemcpyAsync(mp, m.pos, m.npart * 24, hipMemcpyHostToDevice, s));
+                if ((rc = copy_h2d(mp, m.pos, m.npart * 24, s))) return rc;
                 md[q].pos = (const double*)mp;
                 mp += al(m.npart * 24);
                 if (m.vel) {
-                    HIPCHK(hipMemcpyAsync(mp, m.vel, m.npart * 24, hipMemcpyHostToDevice, s));
+                    if ((rc = copy_h2d(mp, m.vel, m.npart * 24, s))) return rc;
                     md[q].vel = (const double*)mp;
                     mp += al(m.npart * 24);
                 }
-                HIPCHK(hipMemcpyAsync(mp, m.mass, m.nelem * 8, hipMemcpyHostToDevice, s));
+                if ((rc = copy_h2d(mp, m.mass, m.nelem * 8, s))) return rc;
                 md[q].mass = (const double*)mp;
                 mp += al(m.nelem * 8);
-                HIPCHK(hipMemcpyAsync(mp, m.conn, m.nelem * m.elemtype * 4, hipMemcpyHostToDevice, s));
+                if ((rc = copy_h2d(mp, m.conn, m.nelem * m.elemtype * 4, s))) return rc;
                 md[q].conn = (const int32_t*)mp;
                 mp += al(m.nelem * m.elemtype * 4);
             }
@@ -1879,8 +1977,8 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                     fdev[q][f] = grids[i].fields[f];
                 } else {
                     fdev[q][f] = fbase;
-                    HIPCHK(hipMemcpyAsync(fbase, grids[i].fields[f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double),
-                                          hipMemcpyHostToDevice, s));
+                    if ((rc = copy_h2d(fbase, grids[i].fields[f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double), s)))
+                        return rc;
                     fbase += nvox * FIELD_WIDTH[f];
                 }
             }
@@ -1967,9 +2065,10 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                 k_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, bacc + acc_off[q], fp[q], nvox);
                 HIPCHK(hipGetLastError());
                 for (int f = 0; f < 6; ++f)
-                    if (grids[i].fields[f] && !grids[i].on_device)
-                        HIPCHK(hipMemcpyAsync(grids[i].fields[f], fdev[q][f],
-                                              (size_t)nvox * FIELD_WIDTH[f] * sizeof(double), hipMemcpyDeviceToHost, s));
+                    if (grids[i].fields[f] && !grids[i].on_device &&
+                        (rc = copy_d2h(grids[i].fields[f], fdev[q][f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double),
+                                       s)))
+                        return rc;
             }
             HIPCHK(hipMemcpyAsync(hm.data(), mcnt, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
