@@ -101,23 +101,31 @@ PVX_HD bool needs_refine(const double (*p)[3], bool quadratic, double tol2) {
     return false;
 }
 
-/* child c (bits 4i+2j+k) of a lattice node: 27-point lattice of the child */
+/* the 8 corners (4i+2j+k) of child c (bits 4i+2j+k) of a linear node:
+ * points of the node's 27-point lattice */
+PVX_HD void child_corners(const double (*p)[3], const double (*u)[3], int c, double (*c8p)[3], double (*c8u)[3]) {
+    const int ci = (c >> 2) & 1, cj = (c >> 1) & 1, ck = c & 1;
+    for (int ii = 0; ii < 2; ++ii)
+        for (int jj = 0; jj < 2; ++jj)
+            for (int kk = 0; kk < 2; ++kk) {
+                int src = 9 * (ii + ci) + 3 * (jj + cj) + (kk + ck);
+                for (int a = 0; a < 3; ++a) {
+                    c8p[4 * ii + 2 * jj + kk][a] = p[src][a];
+                    c8u[4 * ii + 2 * jj + kk][a] = u[src][a];
+                }
+            }
+}
+
+/* child c (bits 4i+2j+k) of a lattice node: 27-point lattice of the child
+ * (positions; velocities too when vel) */
 PVX_HD void child_node(const double (*p)[3], const double (*u)[3], int c, bool quadratic, double (*cp)[3],
-                       double (*cu)[3], double (*tmp)[3]) {
+                       double (*cu)[3], double (*tmp)[3], bool vel = true) {
     const int ci = (c >> 2) & 1, cj = (c >> 1) & 1, ck = c & 1;
     if (!quadratic) {
         double c8p[8][3], c8u[8][3];
-        for (int ii = 0; ii < 2; ++ii)
-            for (int jj = 0; jj < 2; ++jj)
-                for (int kk = 0; kk < 2; ++kk) {
-                    int src = 9 * (ii + ci) + 3 * (jj + cj) + (kk + ck);
-                    for (int a = 0; a < 3; ++a) {
-                        c8p[4 * ii + 2 * jj + kk][a] = p[src][a];
-                        c8u[4 * ii + 2 * jj + kk][a] = u[src][a];
-                    }
-                }
+        child_corners(p, u, c, c8p, c8u);
         lin27((const double (*)[3])c8p, cp);
-        lin27((const double (*)[3])c8u, cu);
+        if (vel) lin27((const double (*)[3])c8u, cu);
     } else {
         quad125(p, tmp);
         for (int ii = 0; ii < 3; ++ii)
@@ -125,6 +133,7 @@ PVX_HD void child_node(const double (*p)[3], const double (*u)[3], int c, bool q
                 for (int kk = 0; kk < 3; ++kk)
                     for (int a = 0; a < 3; ++a)
                         cp[9 * ii + 3 * jj + kk][a] = tmp[25 * (ii + 2 * ci) + 5 * (jj + 2 * cj) + (kk + 2 * ck)][a];
+        if (!vel) return;
         quad125(u, tmp);
         for (int ii = 0; ii < 3; ++ii)
             for (int jj = 0; jj < 3; ++jj)
@@ -256,6 +265,9 @@ PVX_HD int64_t expand_element(const double* pos, const double* vel, const double
         return 1;
     }
     const bool quad = (vpe == 27);
+    /* counting needs neither velocities nor the lattices of the last
+     * level's nodes (their tets are counted, not placed) */
+    const bool counting = out == nullptr;
     if (quad) {
         for (int v = 0; v < 27; ++v)
             for (int a = 0; a < 3; ++a) {
@@ -264,7 +276,7 @@ PVX_HD int64_t expand_element(const double* pos, const double* vel, const double
             }
     } else {
         lin27((const double (*)[3])P, S.p[0]);
-        lin27((const double (*)[3])U, S.u[0]);
+        if (!counting) lin27((const double (*)[3])U, S.u[0]);
     }
     if (maxlvl > MAXLVL) maxlvl = MAXLVL;
     int64_t count = 0;
@@ -273,11 +285,27 @@ PVX_HD int64_t expand_element(const double* pos, const double* vel, const double
     /* visit node d */
     for (;;) {
         bool refine = d < maxlvl && needs_refine((const double (*)[3])S.p[d], quad, tol2);
-        if (refine) {
+        /* a refined node one level above the last: its 8 children are
+         * leaves, counted without their lattices, and (linear elements)
+         * emitted from their corners, which are points of this node's
+         * lattice (lin27 copies corners unchanged: the same records) */
+        const bool last = refine && d == maxlvl - 1 && (counting || !quad);
+        if (refine && !last) {
             S.child[d] = 0;
         } else {
-            if (out) emit6(out + TREC * count, (const double (*)[3])S.p[d], (const double (*)[3])S.u[d], dm);
-            count += 6;
+            if (out) {
+                if (last) {
+                    for (int c = 0; c < 8; ++c) {
+                        double c8p[8][3], c8u[8][3];
+                        child_corners((const double (*)[3])S.p[d], (const double (*)[3])S.u[d], c, c8p, c8u);
+                        emit6_corners(out + TREC * (count + 6 * c), (const double (*)[3])c8p,
+                                      (const double (*)[3])c8u, dm * 0.125);
+                    }
+                } else {
+                    emit6(out + TREC * count, (const double (*)[3])S.p[d], (const double (*)[3])S.u[d], dm);
+                }
+            }
+            count += last ? 48 : 6;
             /* ascend to the next unvisited sibling */
             for (;;) {
                 if (d == 0) return count;
@@ -288,7 +316,7 @@ PVX_HD int64_t expand_element(const double* pos, const double* vel, const double
         }
         /* descend into child S.child[d] */
         child_node((const double (*)[3])S.p[d], (const double (*)[3])S.u[d], S.child[d], quad, S.p[d + 1],
-                   S.u[d + 1], S.tmp);
+                   S.u[d + 1], S.tmp, !counting);
         S.child[d]++;
         d++;
         dm *= 0.125;
