@@ -93,6 +93,13 @@ __global__ void k_finalize(GridInfo g, const double* __restrict__ acc, FieldPtrs
     cell_finalize(g, acc, f, c);
 }
 
+/* the end-of-call merge: new deposits in acc plus the fields' contents */
+__global__ void k_merge_finalize(GridInfo g, double* __restrict__ acc, FieldPtrs f, int64_t nvox) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nvox) return;
+    cell_merge_finalize(g, acc, f, c);
+}
+
 struct MeshDev {
     const double* pos;
     const double* vel;
@@ -1042,6 +1049,9 @@ struct Workspace {
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t side = nullptr; /* the field upload beside the kernels */
+    int side_dev = -1;
+    hipEvent_t side_done = nullptr;
 };
 
 std::mutex g_ws_mutex;
@@ -1093,6 +1103,7 @@ struct Staging {
     bool pending[2] = {false, false};
 };
 Staging g_stage;
+Staging g_stage_side; /* for the field upload that runs beside the kernels */
 
 void par_memcpy(void* dst, const void* src, size_t n) {
     unsigned nth = std::thread::hardware_concurrency();
@@ -1113,36 +1124,36 @@ void par_memcpy(void* dst, const void* src, size_t n) {
     for (auto& t : ts) t.join();
 }
 
-int stage_ready() {
+int stage_ready(Staging& S) {
     for (int b = 0; b < 2; ++b) {
-        if (!g_stage.buf[b]) HIPCHK(hipHostMalloc(&g_stage.buf[b], STAGE_CHUNK, hipHostMallocDefault));
-        if (!g_stage.ev[b]) HIPCHK(hipEventCreateWithFlags(&g_stage.ev[b], hipEventDisableTiming));
+        if (!S.buf[b]) HIPCHK(hipHostMalloc(&S.buf[b], STAGE_CHUNK, hipHostMallocDefault));
+        if (!S.ev[b]) HIPCHK(hipEventCreateWithFlags(&S.ev[b], hipEventDisableTiming));
     }
     return 0;
 }
 
 /* wait until the DMA that last used stage buffer b has finished */
-int stage_wait(int b) {
-    if (g_stage.pending[b]) HIPCHK(hipEventSynchronize(g_stage.ev[b]));
-    g_stage.pending[b] = false;
+int stage_wait(Staging& S, int b) {
+    if (S.pending[b]) HIPCHK(hipEventSynchronize(S.ev[b]));
+    S.pending[b] = false;
     return 0;
 }
 
-int copy_h2d(void* d, const void* h, size_t bytes, hipStream_t s) {
+int copy_h2d(void* d, const void* h, size_t bytes, hipStream_t s, Staging& S = g_stage) {
     if (bytes < STAGE_MIN) {
         HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
         return 0;
     }
-    int rc = stage_ready();
+    int rc = stage_ready(S);
     if (rc) return rc;
     for (size_t off = 0, i = 0; off < bytes; off += STAGE_CHUNK, ++i) {
         const int b = (int)(i & 1);
         const size_t m = std::min(STAGE_CHUNK, bytes - off);
-        if ((rc = stage_wait(b))) return rc;
-        par_memcpy(g_stage.buf[b], (const char*)h + off, m);
-        HIPCHK(hipMemcpyAsync((char*)d + off, g_stage.buf[b], m, hipMemcpyHostToDevice, s));
-        HIPCHK(hipEventRecord(g_stage.ev[b], s));
-        g_stage.pending[b] = true;
+        if ((rc = stage_wait(S, b))) return rc;
+        par_memcpy(S.buf[b], (const char*)h + off, m);
+        HIPCHK(hipMemcpyAsync((char*)d + off, S.buf[b], m, hipMemcpyHostToDevice, s));
+        HIPCHK(hipEventRecord(S.ev[b], s));
+        S.pending[b] = true;
     }
     return 0;
 }
@@ -1155,23 +1166,24 @@ int copy_d2h(void* h, const void* d, size_t bytes, hipStream_t s) {
         HIPCHK(hipStreamSynchronize(s));
         return 0;
     }
-    int rc = stage_ready();
+    Staging& S = g_stage;
+    int rc = stage_ready(S);
     if (rc) return rc;
     const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
     for (size_t i = 0; i <= nch; ++i) {
         if (i < nch) {
             const int b = (int)(i & 1);
             const size_t off = i * STAGE_CHUNK, m = std::min(STAGE_CHUNK, bytes - off);
-            if ((rc = stage_wait(b))) return rc;
-            HIPCHK(hipMemcpyAsync(g_stage.buf[b], (const char*)d + off, m, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipEventRecord(g_stage.ev[b], s));
-            g_stage.pending[b] = true;
+            if ((rc = stage_wait(S, b))) return rc;
+            HIPCHK(hipMemcpyAsync(S.buf[b], (const char*)d + off, m, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipEventRecord(S.ev[b], s));
+            S.pending[b] = true;
         }
         if (i >= 1) {
             const int b = (int)((i - 1) & 1);
             const size_t off = (i - 1) * STAGE_CHUNK, m = std::min(STAGE_CHUNK, bytes - off);
-            if ((rc = stage_wait(b))) return rc;
-            par_memcpy((char*)h + off, g_stage.buf[b], m);
+            if ((rc = stage_wait(S, b))) return rc;
+            par_memcpy((char*)h + off, S.buf[b], m);
         }
     }
     return 0;
@@ -1184,6 +1196,60 @@ int upload(DevBuf& b, const void* src, size_t bytes, int dev, hipStream_t s, con
     *out = b.p;
     return 0;
 }
+
+/* A host-to-device upload of the grid's fields run by a thread of its own
+ * on the workspace's side stream, so that the PCIe transfer overlaps the
+ * deposit (which starts from a zeroed accumulator; k_merge_finalize adds
+ * the fields once W.side_done has passed).  The destructor joins, so every
+ * return path of the caller waits for the thread. */
+struct SideUpload {
+    std::thread t;
+    int rc = 0;
+    std::string err;
+    hipStream_t used = nullptr;
+    ~SideUpload() {
+        if (t.joinable()) t.join();
+        /* an early return of the caller leaves no copy in flight into the
+         * workspace's field buffers */
+        if (used) (void)hipStreamSynchronize(used);
+    }
+    int start(Workspace& W, int dev, double* const* fd, const psivox_grid* grid, int64_t nvox) {
+        if (!W.side || W.side_dev != dev) {
+            HIPCHK(hipStreamCreateWithFlags(&W.side, hipStreamNonBlocking));
+            W.side_dev = dev;
+        }
+        if (!W.side_done) HIPCHK(hipEventCreateWithFlags(&W.side_done, hipEventDisableTiming));
+        double* d[6];
+        for (int f = 0; f < 6; ++f) d[f] = fd[f];
+        used = W.side;
+        t = std::thread([this, &W, dev, d, grid, nvox] {
+            rc = side_body(W, dev, d, grid, nvox);
+            if (rc) err = g_err;
+        });
+        return 0;
+    }
+    static int side_body(Workspace& W, int dev, double* const* d, const psivox_grid* grid, int64_t nvox) {
+        HIPCHK(hipSetDevice(dev));
+        for (int f = 0; f < 6; ++f) {
+            if (!grid->fields[f]) continue;
+            const int rc = copy_h2d(d[f], grid->fields[f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double), W.side,
+                                    g_stage_side);
+            if (rc) return rc;
+        }
+        HIPCHK(hipEventRecord(W.side_done, W.side));
+        return 0;
+    }
+    /* wait for the thread; s then waits for the copies it queued */
+    int finish(Workspace& W, hipStream_t s) {
+        if (t.joinable()) t.join();
+        if (rc) {
+            g_err = err;
+            return rc;
+        }
+        HIPCHK(hipStreamWaitEvent(s, W.side_done, 0));
+        return 0;
+    }
+};
 
 inline unsigned grid_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -1542,16 +1608,25 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     MeshDev md{};
     if ((rc = mesh_to_device(W, mesh, dev, s, md))) return rc;
 
-    /* grid fields on the device */
+    /* grid fields on the device.  Large host fields go up beside the
+     * kernels (SideUpload): the deposit starts from a zeroed accumulator
+     * and the fields' contents are merged in at the end. */
+    size_t field_bytes = 0;
+    for (int f = 0; f < 6; ++f)
+        if (grid->fields[f]) field_bytes += (size_t)nvox * FIELD_WIDTH[f] * sizeof(double);
+    const bool side_up = !grid->on_device && field_bytes >= STAGE_MIN;
     double* fd[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     for (int f = 0; f < 6; ++f) {
         if (!grid->fields[f]) continue;
+        const size_t bytes = (size_t)nvox * FIELD_WIDTH[f] * sizeof(double);
         if (grid->on_device) {
             fd[f] = grid->fields[f];
+        } else if (side_up) {
+            if ((rc = W.fields[f].get(bytes, dev))) return rc;
+            fd[f] = (double*)W.fields[f].p;
         } else {
             const void* p;
-            if ((rc = upload(W.fields[f], grid->fields[f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double), dev, s, &p)))
-                return rc;
+            if ((rc = upload(W.fields[f], grid->fields[f], bytes, dev, s, &p))) return rc;
             fd[f] = (double*)p;
         }
     }
@@ -1564,8 +1639,32 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     unsigned* status = (unsigned*)(counters + 4);
     HIPCHK(hipMemsetAsync(counters, 0, 64, s));
 
-    k_to_raw<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, fptrs, acc, nvox);
-    HIPCHK(hipGetLastError());
+    SideUpload side;
+    /* the accumulator's starting state: the fields' contents, or zero when
+     * they are merged at the end */
+    auto start_acc = [&]() -> int {
+        if (side_up) {
+            HIPCHK(hipMemsetAsync(acc, 0, (size_t)nvox * g.nacc * sizeof(double), s));
+        } else {
+            k_to_raw<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, fptrs, acc, nvox);
+            HIPCHK(hipGetLastError());
+        }
+        return 0;
+    };
+    /* raw moments -> the fields (after the side upload, when there is one) */
+    auto finish_acc = [&]() -> int {
+        if (side_up) {
+            int r = side.finish(W, s);
+            if (r) return r;
+            k_merge_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, acc, fptrs, nvox);
+        } else {
+            k_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, acc, fptrs, nvox);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    };
+    if (side_up && (rc = side.start(W, dev, fd, grid, nvox))) return rc;
+    if ((rc = start_acc())) return rc;
 
     int maxlvl = o->refine_max_lvl < 0 ? 0 : (o->refine_max_lvl > MAXLVL ? MAXLVL : o->refine_max_lvl);
     if (vpe == 4) maxlvl = 0;
@@ -1690,8 +1789,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                     kernel = PSIVOX_KERNEL_PLAIN;
                     fell_back = 1;
                     HIPCHK(hipMemsetAsync(counters, 0, 64, s));
-                    k_to_raw<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, fptrs, acc, nvox);
-                    HIPCHK(hipGetLastError());
+                    if ((rc = start_acc())) return rc;
                     e0 = 0;
                     kms = 0.0;
                     ntets_total = 0;
@@ -1702,7 +1800,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
         e0 = e1;
         if (o->progress && o->progress(o->progress_ctx, e0, ne)) {
             /* leave what was deposited so far, as the reference does on Ctrl-C */
-            k_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, acc, fptrs, nvox);
+            if ((rc = finish_acc())) return rc;
             for (int f = 0; f < 6; ++f)
                 if (grid->fields[f] && !grid->on_device)
                     (void)hipMemcpyAsync(grid->fields[f], fd[f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double),
@@ -1712,8 +1810,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
         }
     }
 
-    k_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, acc, fptrs, nvox);
-    HIPCHK(hipGetLastError());
+    if ((rc = finish_acc())) return rc;
     for (int f = 0; f < 6; ++f)
         if (grid->fields[f] && !grid->on_device &&
             (rc = copy_d2h(grid->fields[f], fd[f], (size_t)nvox * FIELD_WIDTH[f] * sizeof(double), s)))

@@ -30,41 +30,48 @@ PVX_HD void cell_centre(const GridInfo& g, int64_t c, double* cc) {
     cc[2] = g.w0[2] + g.d[2] * (k + 0.5);
 }
 
-PVX_HD void cell_to_raw(const GridInfo& g, const FieldPtrs& F, double* acc, int64_t c) {
+/* the raw moments of cell c's present contents into acc (ADD: added to
+ * what acc holds for the cell) */
+template <bool ADD>
+PVX_HD void cell_raw(const GridInfo& g, const FieldPtrs& F, double* acc, int64_t c) {
     double* A = acc + c * g.nacc;
+    auto put = [A](int t, double x) { A[t] = ADD ? A[t] + x : x; };
     const double M = F.f[0][c];
     if (M == 0.0) {
-        for (int t = 0; t < g.nacc; ++t) A[t] = 0.0;
+        if (!ADD)
+            for (int t = 0; t < g.nacc; ++t) A[t] = 0.0;
         return;
     }
     double cc[3];
     cell_centre(g, c, cc);
     double dx[3] = {0, 0, 0}, v[3] = {0, 0, 0};
-    A[0] = M;
+    put(0, M);
     if (F.f[1])
         for (int a = 0; a < 3; ++a) {
             dx[a] = F.f[1][3 * c + a] - cc[a];
-            A[g.ox + a] = M * dx[a];
+            put(g.ox + a, M * dx[a]);
         }
     if (F.f[2])
         for (int a = 0; a < 3; ++a) {
             v[a] = F.f[2][3 * c + a];
-            A[g.ov + a] = M * v[a];
+            put(g.ov + a, M * v[a]);
         }
     if (F.f[3]) {
         int s = 0;
         for (int a = 0; a < 3; ++a)
-            for (int b = a; b < 3; ++b, ++s) A[g.oxx + s] = M * (F.f[3][9 * c + 3 * a + b] + dx[a] * dx[b]);
+            for (int b = a; b < 3; ++b, ++s) put(g.oxx + s, M * (F.f[3][9 * c + 3 * a + b] + dx[a] * dx[b]));
     }
     if (F.f[4])
         for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) A[g.oxv + 3 * a + b] = M * (F.f[4][9 * c + 3 * a + b] + dx[a] * v[b]);
+            for (int b = 0; b < 3; ++b) put(g.oxv + 3 * a + b, M * (F.f[4][9 * c + 3 * a + b] + dx[a] * v[b]));
     if (F.f[5]) {
         int s = 0;
         for (int a = 0; a < 3; ++a)
-            for (int b = a; b < 3; ++b, ++s) A[g.ovv + s] = M * (F.f[5][9 * c + 3 * a + b] + v[a] * v[b]);
+            for (int b = a; b < 3; ++b, ++s) put(g.ovv + s, M * (F.f[5][9 * c + 3 * a + b] + v[a] * v[b]));
     }
 }
+
+PVX_HD void cell_to_raw(const GridInfo& g, const FieldPtrs& F, double* acc, int64_t c) { cell_raw<false>(g, F, acc, c); }
 
 PVX_HD void cell_finalize(const GridInfo& g, const double* acc, const FieldPtrs& F, int64_t c) {
     const double* A = acc + c * g.nacc;
@@ -106,6 +113,17 @@ PVX_HD void cell_finalize(const GridInfo& g, const double* acc, const FieldPtrs&
                 F.f[5][9 * c + 3 * b + a] = t;
             }
     }
+}
+
+/* The merge done at the end instead of the start: acc holds only the new
+ * deposits; the cell's present contents are added to them as raw moments
+ * and the sum finalized.  The same sums as cell_to_raw before the deposit
+ * and cell_finalize after it, in another order.  A cell that received no
+ * mass keeps its contents bit for bit. */
+PVX_HD void cell_merge_finalize(const GridInfo& g, double* acc, const FieldPtrs& F, int64_t c) {
+    if (acc[c * g.nacc] == 0.0) return;
+    cell_raw<true>(g, F, acc, c);
+    cell_finalize(g, acc, F, c);
 }
 
 }  // namespace pvx

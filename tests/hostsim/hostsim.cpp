@@ -384,3 +384,29 @@ extern "C" int64_t hs_annihilation(int vpe, int64_t nelem, const double* pos, co
     *out_pairs = npairs;
     return status ? -(int64_t)status : nfrag;
 }
+
+/* the grid-side conversions on the host: fields -> raw moments (acc), and
+ * the end-of-call merge of new raw moments (acc) into the fields */
+extern "C" void hs_cells_to_raw(const int* n, const double* win6, const double* d, double** fields, int xvfix,
+                                double* acc) {
+    GridInfo g;
+    MeshBox mb;
+    const double box6[6] = {0, 0, 0, 0, 0, 0};
+    fill_grid(g, mb, n, win6, d, fields, xvfix, 0, box6);
+    FieldPtrs F;
+    for (int f = 0; f < 6; ++f) F.f[f] = fields[f];
+    const int64_t nvox = (int64_t)n[0] * n[1] * n[2];
+    for (int64_t c = 0; c < nvox; ++c) cell_to_raw(g, F, acc, c);
+}
+
+extern "C" void hs_cells_merge_finalize(const int* n, const double* win6, const double* d, double** fields,
+                                        int xvfix, double* acc) {
+    GridInfo g;
+    MeshBox mb;
+    const double box6[6] = {0, 0, 0, 0, 0, 0};
+    fill_grid(g, mb, n, win6, d, fields, xvfix, 0, box6);
+    FieldPtrs F;
+    for (int f = 0; f < 6; ++f) F.f[f] = fields[f];
+    const int64_t nvox = (int64_t)n[0] * n[1] * n[2];
+    for (int64_t c = 0; c < nvox; ++c) cell_merge_finalize(g, acc, F, c);
+}

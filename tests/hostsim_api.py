@@ -89,3 +89,27 @@ def annihilation(P, reftol=1.0, maxlvl=0, out=None):
                            C.c_int(int(P.periodic)), box, n, win, d, vp(out["m"]), C.c_double(reftol),
                            C.c_int(maxlvl), C.byref(pairs))
     return out, nf, pairs.value
+
+
+def _grid_args(P, out):
+    order = ["m", "x", "v", "xx", "xv", "vv"]
+    fptr = (C.c_void_p * 6)(*[out[f].ctypes.data if f in out else None for f in order])
+    return (C.c_int * 3)(*P.n), (C.c_double * 6)(*P.bounds.reshape(-1)), (C.c_double * 3)(*P.d), fptr
+
+
+def cells_to_raw(P, fields, xvfix=False):
+    """the additive raw moments (nvox, nacc) of a grid's fields"""
+    n, win, d, fptr = _grid_args(P, fields)
+    nacc = 1 + 3 * ("x" in fields) + 3 * ("v" in fields) + 6 * ("xx" in fields) + 9 * ("xv" in fields) + \
+        6 * ("vv" in fields)
+    acc = np.zeros((int(np.prod(P.n)), nacc))
+    lib().hs_cells_to_raw(n, win, d, fptr, C.c_int(int(xvfix)), C.c_void_p(acc.ctypes.data))
+    return acc
+
+
+def cells_merge_finalize(P, fields, acc, xvfix=False):
+    """merge new raw moments acc into the fields in place (the end-of-call merge)"""
+    n, win, d, fptr = _grid_args(P, fields)
+    acc = np.ascontiguousarray(acc, dtype=np.float64).copy()
+    lib().hs_cells_merge_finalize(n, win, d, fptr, C.c_int(int(xvfix)), C.c_void_p(acc.ctypes.data))
+    return fields

@@ -100,6 +100,39 @@ def test_merge_two_calls(PSI, oracle_mod):
     check_close(grid, ref)
 
 
+@pytest.mark.parametrize("kernel", ["plain", "fast"])
+def test_merge_large_host_fields(PSI, oracle_mod, kernel):
+    """Host fields of 16 MB and more travel to the device beside the kernels
+    and are merged at the end: two calls onto a 48^3 grid with all six
+    fields (29 MB) give the oracle's merged result, and cells the second
+    mesh does not reach keep the first call's values bit for bit (as the
+    reference, which only updates cells a fragment lands in)."""
+    from helpers import TOL32
+    from pypsi_amd.synth import zeldovich_block
+    print(f"case: merge 48^3 all fields, host arrays, kernel={kernel}", flush=True)
+    G = 48
+    pos1, vel1 = zeldovich_block(24, 1.0, seed=1)
+    pos2, vel2 = zeldovich_block(12, 0.5, seed=2)
+    pos2 = np.ascontiguousarray(0.3 + 0.25 * pos2)  # an open mesh inside [0.3, 0.55]^3
+    grid = PSI.Grid(n=(G,) * 3, bounds=BOX, fields=ALL)
+    PSI.voxels(grid=grid, mesh=PSI.Mesh(loader="block", pos=pos1, vel=vel1, periodic_box=BOX), kernel=kernel)
+    first = {k: np.array(v, copy=True) for k, v in grid.fields.items()}
+    PSI.voxels(grid=grid, mesh=PSI.Mesh(loader="block", pos=pos2, vel=vel2), kernel=kernel)
+    P1 = oracle_mod.Problem.block(pos1, vel1, periodic_box=BOX, n=(G,) * 3, bounds=BOX, fields=ALL)
+    P2 = oracle_mod.Problem.block(pos2, vel2, periodic_box=None, n=(G,) * 3, bounds=BOX, fields=ALL)
+    ref, _ = oracle_mod.voxels(P1)
+    ref, _ = oracle_mod.voxels(P2, out=ref)
+    check_close(grid, ref, TOL if kernel == "plain" else TOL32)
+    # cells two cells or more away from the second mesh's box
+    lo, hi = pos2.reshape(-1, 3).min(0) - 2 / G, pos2.reshape(-1, 3).max(0) + 2 / G
+    c = (np.arange(G) + 0.5) / G
+    near = [(c >= lo[a]) & (c <= hi[a]) for a in range(3)]
+    far = ~(near[0][:, None, None] & near[1][None, :, None] & near[2][None, None, :])
+    assert far.any() and (~far).any()
+    for k, v in grid.fields.items():
+        np.testing.assert_array_equal(v[far], first[k][far], err_msg=k)
+
+
 def test_device_pointer_path(oracle_mod):
     """The C ABI with device-resident torch tensors (the benchmark path)."""
     import torch
