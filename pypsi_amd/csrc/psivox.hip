@@ -2,14 +2,16 @@
  * psivox.hip -- MI355X (gfx950) voxelizer behind PSI.voxels() and its C ABI
  * (include/psivox.h).
  *
- * Device pipeline for one call (all on one HIP stream):
- *   1. k_to_raw     grid fields (reference layout) -> additive raw moments
+ * Device pipeline for one call (on one HIP stream):
+ *   1. a zeroed accumulator of additive raw moments per cell
  *   2. k_expand     element -> wrapped, refined tets (count/scan/emit for
  *                   refine_max_lvl > 0, fixed 6-slot output otherwise)
  *   3. k_vox_plain  one thread per tet: FP64 clip + sweep + atomic deposit
- *      k_vox_fast   binned FP32 voxel-local path (psivox_fast.h)
- *   4. k_finalize   raw moments -> mass, means, covariances in place
- * Fields travel host<->device only when the caller passes host pointers.
+ *      k_vox_cols   FP32 column tasks, FP64 deposit (psivox_fast.h)
+ *   4. k_merge_finalize  the fields' present contents (reference layout)
+ *                   plus the new raw moments -> mass, means, covariances
+ * Fields travel host<->device only when the caller passes host pointers;
+ * large ones go up on a second stream while steps 1-3 run.
  */
 #include <hip/hip_runtime.h>
 
@@ -76,22 +78,6 @@ struct AtomicSink {
 /* ------------------------------------------------------------------ */
 /* kernels                                                             */
 /* ------------------------------------------------------------------ */
-
-/* existing grid contents -> raw moments (merge into what is there, as the
- * reference's in-place online update does) */
-__global__ void k_to_raw(GridInfo g, FieldPtrs f, double* __restrict__ acc, int64_t nvox) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nvox) return;
-    cell_to_raw(g, f, acc, c);
-}
-
-/* raw moments -> reference layout; cells that received nothing keep their
- * previous contents */
-__global__ void k_finalize(GridInfo g, const double* __restrict__ acc, FieldPtrs f, int64_t nvox) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nvox) return;
-    cell_finalize(g, acc, f, c);
-}
 
 /* the end-of-call merge: new deposits in acc plus the fields' contents */
 __global__ void k_merge_finalize(GridInfo g, double* __restrict__ acc, FieldPtrs f, int64_t nvox) {
@@ -671,6 +657,7 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
         }
         int nmax = ncell;
         for (int o = FBLOCK / 2; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+        PlaneCarry pc{false, false, {0.f, 0.f, 0.f}};
         /* cell steps in lockstep: every lane takes part in each step's
          * deposit, which issues the wavefront's fragments slot by slot so
          * that neighbouring lanes add into the same cell's consecutive
@@ -686,7 +673,7 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
                 bool hl, hh;
                 double lc, hc;
                 cell_planes(S, k, hl, lc, hh, hc);
-                slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O);
+                slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O, &pc);
                 has = Mm.m0 > 0.0f;
             }
             const uint64_t mask = __ballot(has);
@@ -790,9 +777,12 @@ struct RowSink {
 };
 
 /* one thread per task: its share of its tet's residual, deposited into the
- * task's largest fragment.  Each lane stages its deposit (nacc values) in
- * LDS; the wavefront then issues the staged rows slot by slot, so that
- * neighbouring lanes add into the same cell's consecutive slots. */
+ * task's largest fragment.  The lanes that deposit stage their rows (nacc
+ * values) in LDS in rank order; the wavefront then issues the staged rows
+ * slot by slot, so that neighbouring lanes add into the same cell's
+ * consecutive slots, and the issue loop covers only the rows staged (one
+ * task per tet deposits when the residual goes whole to its largest
+ * fragment). */
 template <int ORDER, bool BATCH>
 __global__ __launch_bounds__(256) void k_resid_dep(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
                                                    int64_t ntask, int64_t base, GridInfo g0, MeshBox mb0,
@@ -807,46 +797,54 @@ __global__ __launch_bounds__(256) void k_resid_dep(const double* __restrict__ te
     double* target = nullptr;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t kb = c < ntask ? out.key[c] : 0ull;
+    bool will = false;
+    uint64_t w = 0;
+    int64_t t = 0;
+    double weight = 0.0;
     if (kb != 0ull) {
-        const uint64_t w = tasks[c];
-        const int64_t t = (int64_t)(w & 0xffffffffull);
+        w = tasks[c];
+        t = (int64_t)(w & 0xffffffffull);
         const int64_t whole = tetb[t];
-        const double weight = whole >= 0 ? tetf[t] : tetf[t] * out.sum[c];
-        if ((whole < 0 || whole == c + base) && weight != 0.0) {
-            BestFrag B;
-            B.kl = (int)(kb & 0xffffffffull);
-            const uint32_t mbits = (uint32_t)(kb >> 32);
-            memcpy(&B.M.m0, &mbits, 4);
-            if (ORDER > 0) {
-                const float4* m = (const float4*)(out.mom + 16 * c);
-                const float4 a = m[0], b = m[1], cc = m[2], d = m[3];
-                B.M.s1[0] = a.y;
-                B.M.s1[1] = a.z;
-                B.M.s1[2] = a.w;
-                B.M.s2[0] = b.x;
-                B.M.s2[1] = b.y;
-                B.M.s2[2] = b.z;
-                B.M.s2[3] = b.w;
-                B.M.s2[4] = cc.x;
-                B.M.s2[5] = cc.y;
-                B.O[0] = cc.z;
-                B.O[1] = cc.w;
-                B.O[2] = d.x;
-            }
-            const double* r = tets + (int64_t)TREC * t;
-            with_mesh<BATCH>(r, g0, mb0, acc0, table, [&](const GridInfo& g, const MeshBox& mb, double* acc, int) {
-                RowSink rs{rows + lane * na, -1};
-                task_residual<ORDER>(rs, g, mb, r, w, B, weight);
-                if (rs.base >= 0) target = acc + rs.base;
-            });
-        }
+        weight = whole >= 0 ? tetf[t] : tetf[t] * out.sum[c];
+        will = (whole < 0 || whole == c + base) && weight != 0.0;
     }
-    dst[lane] = target;
+    const uint64_t active = __ballot(will);
+    const int nact = __popcll(active);
+    const int rank = __popcll(active & ((1ull << lane) - 1ull));
+    if (will) {
+        BestFrag B;
+        B.kl = (int)(kb & 0xffffffffull);
+        const uint32_t mbits = (uint32_t)(kb >> 32);
+        memcpy(&B.M.m0, &mbits, 4);
+        if (ORDER > 0) {
+            const float4* m = (const float4*)(out.mom + 16 * c);
+            const float4 a = m[0], b = m[1], cc = m[2], d = m[3];
+            B.M.s1[0] = a.y;
+            B.M.s1[1] = a.z;
+            B.M.s1[2] = a.w;
+            B.M.s2[0] = b.x;
+            B.M.s2[1] = b.y;
+            B.M.s2[2] = b.z;
+            B.M.s2[3] = b.w;
+            B.M.s2[4] = cc.x;
+            B.M.s2[5] = cc.y;
+            B.O[0] = cc.z;
+            B.O[1] = cc.w;
+            B.O[2] = d.x;
+        }
+        const double* r = tets + (int64_t)TREC * t;
+        with_mesh<BATCH>(r, g0, mb0, acc0, table, [&](const GridInfo& g, const MeshBox& mb, double* acc, int) {
+            RowSink rs{rows + rank * na, -1};
+            task_residual<ORDER>(rs, g, mb, r, w, B, weight);
+            if (rs.base >= 0) target = acc + rs.base;
+        });
+        dst[rank] = target;
+    }
     __syncthreads();
-    /* value q = lane + FBLOCK s of the wavefront's rows: row q / na, slot q % na */
+    /* value q = lane + FBLOCK s of the wavefront's staged rows: row q / na, slot q % na */
     const int fstep = FBLOCK / na, jstep = FBLOCK - fstep * na;
     int f = lane / na, j = lane - (lane / na) * na;
-    for (int q = lane; q < FBLOCK * na; q += FBLOCK) {
+    for (int q = lane; q < nact * na; q += FBLOCK) {
         double* d = dst[f];
         if (d) unsafeAtomicAdd(d + j, rows[q]);
         f += fstep;
@@ -1609,8 +1607,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     if ((rc = mesh_to_device(W, mesh, dev, s, md))) return rc;
 
     /* grid fields on the device.  Large host fields go up beside the
-     * kernels (SideUpload): the deposit starts from a zeroed accumulator
-     * and the fields' contents are merged in at the end. */
+     * kernels (SideUpload). */
     size_t field_bytes = 0;
     for (int f = 0; f < 6; ++f)
         if (grid->fields[f]) field_bytes += (size_t)nvox * FIELD_WIDTH[f] * sizeof(double);
@@ -1640,26 +1637,20 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     HIPCHK(hipMemsetAsync(counters, 0, 64, s));
 
     SideUpload side;
-    /* the accumulator's starting state: the fields' contents, or zero when
-     * they are merged at the end */
+    /* The deposit starts from a zeroed accumulator and the fields' present
+     * contents are merged in at the end (k_merge_finalize): cells that
+     * receive no mass keep their contents bit for bit, and large host
+     * fields can travel while the kernels run. */
     auto start_acc = [&]() -> int {
-        if (side_up) {
-            HIPCHK(hipMemsetAsync(acc, 0, (size_t)nvox * g.nacc * sizeof(double), s));
-        } else {
-            k_to_raw<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, fptrs, acc, nvox);
-            HIPCHK(hipGetLastError());
-        }
+        HIPCHK(hipMemsetAsync(acc, 0, (size_t)nvox * g.nacc * sizeof(double), s));
         return 0;
     };
-    /* raw moments -> the fields (after the side upload, when there is one) */
     auto finish_acc = [&]() -> int {
         if (side_up) {
             int r = side.finish(W, s);
             if (r) return r;
-            k_merge_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, acc, fptrs, nvox);
-        } else {
-            k_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, acc, fptrs, nvox);
         }
+        k_merge_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, acc, fptrs, nvox);
         HIPCHK(hipGetLastError());
         return 0;
     };
@@ -2080,8 +2071,7 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                 }
             }
             for (int f = 0; f < 6; ++f) fp[q].f[f] = fdev[q][f];
-            k_to_raw<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, fp[q], bacc + acc_off[q], nvox);
-            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemsetAsync(bacc + acc_off[q], 0, (size_t)nvox * g.nacc * sizeof(double), s));
             if (m.nelem > 0) {
                 launch_expand_fixed(md[q], g, MB[i], 0, m.nelem, tets + TREC * T[q], (double)q, s);
                 HIPCHK(hipGetLastError());
@@ -2159,7 +2149,7 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                 const int i = i0 + q;
                 const GridInfo& g = G[i];
                 const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
-                k_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, bacc + acc_off[q], fp[q], nvox);
+                k_merge_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, bacc + acc_off[q], fp[q], nvox);
                 HIPCHK(hipGetLastError());
                 for (int f = 0; f < 6; ++f)
                     if (grids[i].fields[f] && !grids[i].on_device &&

@@ -473,14 +473,26 @@ struct SlabCell {
     uint64_t inside; /* the same for fan triangles wholly inside the slab */
 };
 
+/* The point plane_point_le found on a cell's upper plane, kept for the
+ * next cell of the column, whose lower plane is the same plane (the same
+ * FP32 offset and the same vertex sides, so the search would find the same
+ * point): each interior plane is searched once. */
+struct PlaneCarry {
+    bool valid; /* the last cell had an upper plane */
+    bool found; /* and an edge crossed it at p */
+    float p[3];
+};
+
 /*
  * The slab lo < x_ax <= hi of P (either side may be open): its planes, the
  * origin and cap anchor on them, and the mask of fan triangles reaching
  * into it (z_max > lo and z_min <= hi: a triangle lying on the upper plane
- * belongs to the piece, one on the lower plane not).
+ * belongs to the piece, one on the lower plane not).  pc (optional) brings
+ * the lower plane's point from the column's previous cell and takes the
+ * upper plane's for the next.
  */
 PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi, double hi_c,
-                       SlabCell& S) {
+                       SlabCell& S, PlaneCarry* pc = nullptr) {
     const double x0d = pick3(G.x0d, ax);
     S.has_lo = has_lo;
     S.has_hi = has_hi;
@@ -504,11 +516,25 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
     const int v0 = P.at(0);
     float O[3] = {P.qa(v0, 0), P.qa(v0, 1), P.qa(v0, 2)};
     float Gh[3] = {O[0], O[1], O[2]};
+    bool fh = false; /* a point on the upper plane was found */
     if (has_lo) {
-        plane_point_le(P, zd0, zd1, zd2, S.lo, ~above, O);
-        if (has_hi && !plane_point_le(P, zd0, zd1, zd2, S.hi, below, Gh)) Gh[0] = O[0], Gh[1] = O[1], Gh[2] = O[2];
+        if (pc && pc->valid) {
+            if (pc->found) O[0] = pc->p[0], O[1] = pc->p[1], O[2] = pc->p[2];
+        } else {
+            plane_point_le(P, zd0, zd1, zd2, S.lo, ~above, O);
+        }
+        if (has_hi) {
+            fh = plane_point_le(P, zd0, zd1, zd2, S.hi, below, Gh);
+            if (pc) pc->p[0] = Gh[0], pc->p[1] = Gh[1], pc->p[2] = Gh[2];
+            if (!fh) Gh[0] = O[0], Gh[1] = O[1], Gh[2] = O[2];
+        }
     } else if (has_hi) {
-        plane_point_le(P, zd0, zd1, zd2, S.hi, below, O);
+        fh = plane_point_le(P, zd0, zd1, zd2, S.hi, below, O);
+        if (pc) pc->p[0] = O[0], pc->p[1] = O[1], pc->p[2] = O[2];
+    }
+    if (pc) {
+        pc->valid = has_hi;
+        pc->found = fh;
     }
     for (int i = 0; i < 3; ++i) {
         S.O[i] = O[i];
@@ -619,10 +645,10 @@ PVX_HD void inside_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
 /* moments of P's piece in the slab, one lane doing every fan triangle */
 template <int ORDER>
 PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi,
-                           double hi_c, Mom& M, float* O) {
+                           double hi_c, Mom& M, float* O, PlaneCarry* pc = nullptr) {
     M.zero();
     SlabCell S;
-    slab_setup(P, G, ax, has_lo, lo_c, has_hi, hi_c, S);
+    slab_setup(P, G, ax, has_lo, lo_c, has_hi, hi_c, S, pc);
     O[0] = S.O[0];
     O[1] = S.O[1];
     O[2] = S.O[2];
@@ -1318,6 +1344,7 @@ PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const 
     if (ORDER > 0) tet_vel(rec, V);
     V.mass = rec[24];
     pin(V.mass);
+    PlaneCarry pc{false, false, {0.f, 0.f, 0.f}};
 #pragma unroll 1
     for (int k = S.k0; k < S.k1; ++k) {
         bool hl, hh;
@@ -1325,7 +1352,7 @@ PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const 
         cell_planes(S, k, hl, lc, hh, hc);
         Mom Mm;
         float O[3];
-        slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O);
+        slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O, &pc);
         cell_finish<ORDER>(sink, g, S, V, k, Mm, O, T);
     }
     return true;

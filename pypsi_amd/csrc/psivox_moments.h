@@ -121,8 +121,11 @@ PVX_HD void cell_finalize(const GridInfo& g, const double* acc, const FieldPtrs&
  * and cell_finalize after it, in another order.  A cell that received no
  * mass keeps its contents bit for bit. */
 PVX_HD void cell_merge_finalize(const GridInfo& g, double* acc, const FieldPtrs& F, int64_t c) {
-    if (acc[c * g.nacc] == 0.0) return;
-    cell_raw<true>(g, F, acc, c);
+    /* both loads first: one memory round trip decides the common cases
+     * (nothing new; new mass into an empty cell) */
+    const double a0 = acc[c * g.nacc], m_old = F.f[0][c];
+    if (a0 == 0.0) return;
+    if (m_old != 0.0) cell_raw<true>(g, F, acc, c);
     cell_finalize(g, acc, F, c);
 }
 

@@ -58,7 +58,7 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
     std::vector<double> acc((size_t)nvox * g.nacc, 0.0);
     FieldPtrs F;
     for (int f = 0; f < 6; ++f) F.f[f] = fields[f];
-    for (int64_t c = 0; c < nvox; ++c) cell_to_raw(g, F, acc.data(), c);
+    /* the deposit starts from zero; the fields' contents are merged at the end, as on the device */
     double dxmin = d[0] < d[1] ? d[0] : d[1];
     dxmin = dxmin < d[2] ? dxmin : d[2];
     double tol2 = (reftol * dxmin) * (reftol * dxmin);
@@ -133,7 +133,7 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
         }
         ntets += nt;
     }
-    for (int64_t c = 0; c < nvox; ++c) cell_finalize(g, acc.data(), F, c);
+    for (int64_t c = 0; c < nvox; ++c) cell_merge_finalize(g, acc.data(), F, c);
     delete RS;
     delete[] pool;
     out_stats[0] = nfrag;
@@ -169,7 +169,7 @@ extern "C" int hs_voxels_plain(int vpe, int64_t npart, int64_t nelem, const doub
     std::vector<double> acc((size_t)nvox * g.nacc, 0.0);
     FieldPtrs F;
     for (int f = 0; f < 6; ++f) F.f[f] = fields[f];
-    for (int64_t c = 0; c < nvox; ++c) cell_to_raw(g, F, acc.data(), c);
+    /* the deposit starts from zero; the fields' contents are merged at the end, as on the device */
     double dxmin = d[0] < d[1] ? d[0] : d[1];
     dxmin = dxmin < d[2] ? dxmin : d[2];
     double tol2 = (reftol * dxmin) * (reftol * dxmin);
@@ -190,7 +190,7 @@ extern "C" int hs_voxels_plain(int vpe, int64_t npart, int64_t nelem, const doub
         }
         ntets += nt;
     }
-    for (int64_t c = 0; c < nvox; ++c) cell_finalize(g, acc.data(), F, c);
+    for (int64_t c = 0; c < nvox; ++c) cell_merge_finalize(g, acc.data(), F, c);
     delete RS;
     delete W;
     out_stats[0] = nfrag;
