@@ -975,7 +975,7 @@ PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float
                 const float c = s == 0 ? xl : (s == 1 ? xh : (s == 2 ? yl : yh));
                 const float du = X[u][ax] - c, dv = X[v][ax] - c;
                 if ((du < 0.f) == (dv < 0.f)) continue;
-                const float t = du / (du - dv);
+                const float t = du * frcp(du - dv);
                 const float px = ax == 0 ? c : X[u][0] + t * (X[v][0] - X[u][0]);
                 const float py = ax == 1 ? c : X[u][1] + t * (X[v][1] - X[u][1]);
                 take(px, py, X[u][2] + t * (X[v][2] - X[u][2]));
@@ -988,7 +988,7 @@ PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float
                 c02 = D[0][1] * D[1][2] - D[0][2] * D[1][1];
     const float det = D[0][0] * c00 + D[1][0] * c01 + D[2][0] * c02;
     if (det != 0.f) {
-        const float id = 1.0f / det;
+        const float id = frcp(det);
         float inv[3][3];
         inv[0][0] = c00 * id;
         inv[0][1] = c01 * id;
@@ -1000,6 +1000,9 @@ PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float
         inv[2][1] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) * id;
         inv[2][2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
         const float b1 = inv[0][2], b2 = inv[1][2], b3 = inv[2][2], b0 = -(b1 + b2 + b3);
+        /* reciprocals of the four rates, shared by the four vertical edges */
+        const float rb[4] = {b0 != 0.f ? frcp(b0) : 0.f, b1 != 0.f ? frcp(b1) : 0.f, b2 != 0.f ? frcp(b2) : 0.f,
+                             b3 != 0.f ? frcp(b3) : 0.f};
         for (int q = 0; q < 4; ++q) {
             const float rx = ((q & 1) ? xh : xl) - X[0][0], ry = ((q & 2) ? yh : yl) - X[0][1], rz = -X[0][2];
             const float a1 = inv[0][0] * rx + inv[0][1] * ry + b1 * rz, a2 = inv[1][0] * rx + inv[1][1] * ry + b2 * rz,
@@ -1008,9 +1011,9 @@ PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float
             float zl = -3.0e38f, zh = 3.0e38f;
             for (int i = 0; i < 4; ++i) {
                 if (bv[i] > 0.f)
-                    zl = fmaxf(zl, -av[i] / bv[i]);
+                    zl = fmaxf(zl, -av[i] * rb[i]);
                 else if (bv[i] < 0.f)
-                    zh = fminf(zh, -av[i] / bv[i]);
+                    zh = fminf(zh, -av[i] * rb[i]);
                 else if (av[i] < 0.f)
                     zl = 3.0e38f;
             }
@@ -1030,7 +1033,8 @@ PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float
 /* Number of z cells of a column task (copy C, column (il, jl) relative to
  * the copy's index box) and (nvert) the column polyhedron's vertex count,
  * from column_zext: the sort key that groups tasks of equal work into the
- * same wavefront. */
+ * same wavefront.  Only the processing order depends on it, so column_zext
+ * divides with the hardware reciprocal. */
 PVX_HD int column_cells(const CopySetup& C, int il, int jl, int* nvert = nullptr) {
     float zmin, zmax;
     if (!column_zext(C.X, (float)il, (float)(il + 1), (float)jl, (float)(jl + 1), zmin, zmax, nvert)) return 0;
