@@ -8,8 +8,8 @@
  *                   refine_max_lvl > 0, fixed 6-slot output otherwise)
  *   3. k_vox_plain  one thread per tet: FP64 clip + sweep + atomic deposit
  *      k_vox_cols   FP32 column tasks, FP64 deposit (psivox_fast.h)
- *   4. k_merge_finalize  the fields' present contents (reference layout)
- *                   plus the new raw moments -> mass, means, covariances
+ *   4. k_merge_raw, k_finalize  the fields' present contents (reference
+ *                   layout) plus the new raw moments -> mass, means, covariances
  * Fields travel host<->device only when the caller passes host pointers;
  * large ones go up on a second stream while steps 1-3 run.
  */
@@ -79,11 +79,27 @@ struct AtomicSink {
 /* kernels                                                             */
 /* ------------------------------------------------------------------ */
 
-/* the end-of-call merge: new deposits in acc plus the fields' contents */
-__global__ void k_merge_finalize(GridInfo g, double* __restrict__ acc, FieldPtrs f, int64_t nvox) {
+/* The end-of-call merge (cell_merge_finalize) in two passes: the fields'
+ * present contents join the new deposits of a cell as raw moments, then
+ * the sums are finalized.  Two kernels, so that the second reads acc as
+ * read-only memory and issues all of a cell's loads at once (one kernel
+ * that may write acc took twice as long). */
+__global__ void k_merge_raw(GridInfo g, double* __restrict__ acc, FieldPtrs f, int64_t nvox) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nvox) return;
-    cell_merge_finalize(g, acc, f, c);
+    if (acc[c * g.nacc] != 0.0 && f.f[0][c] != 0.0) cell_raw<true>(g, f, acc, c);
+}
+
+__global__ void k_finalize(GridInfo g, const double* __restrict__ acc, FieldPtrs f, int64_t nvox) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nvox) return;
+    cell_finalize(g, acc, f, c);
+}
+
+static void launch_merge_finalize(const GridInfo& g, double* acc, const FieldPtrs& f, int64_t nvox, hipStream_t s) {
+    const unsigned nb = (unsigned)((nvox + 255) / 256);
+    k_merge_raw<<<nb, 256, 0, s>>>(g, acc, f, nvox);
+    k_finalize<<<nb, 256, 0, s>>>(g, acc, f, nvox);
 }
 
 struct MeshDev {
@@ -776,18 +792,19 @@ struct RowSink {
     }
 };
 
-/* one thread per task: its share of its tet's residual, deposited into the
- * task's largest fragment.  The lanes that deposit stage their rows (nacc
- * values) in LDS in rank order; the wavefront then issues the staged rows
- * slot by slot, so that neighbouring lanes add into the same cell's
- * consecutive slots, and the issue loop covers only the rows staged (one
- * task per tet deposits when the residual goes whole to its largest
- * fragment). */
-template <int ORDER, bool BATCH>
+/* The residual deposits, staged.  WHOLE: one thread per tet of [t0, t1)
+ * whose residual goes whole to its largest fragment (tetb >= 0), for that
+ * fragment's task; otherwise one thread per task of a tet whose residual
+ * is spread, for the task's share.  The lanes that deposit stage their
+ * rows (nacc values) in LDS in rank order; the wavefront then issues the
+ * staged rows slot by slot, so that neighbouring lanes add into the same
+ * cell's consecutive slots, and the issue loop covers only the rows
+ * staged. */
+template <int ORDER, bool BATCH, bool WHOLE>
 __global__ __launch_bounds__(256) void k_resid_dep(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
-                                                   int64_t ntask, int64_t base, GridInfo g0, MeshBox mb0,
-                                                   double* __restrict__ acc0, const BatchGeo* table, TaskOut out,
-                                                   const double* __restrict__ tetf,
+                                                   int64_t ntask, int64_t base, int64_t t0, int64_t t1, GridInfo g0,
+                                                   MeshBox mb0, double* __restrict__ acc0, const BatchGeo* table,
+                                                   TaskOut out, const double* __restrict__ tetf,
                                                    const int64_t* __restrict__ tetb) {
     extern __shared__ double rstage[]; /* per wavefront: FBLOCK rows of nacc values, FBLOCK targets */
     const int na = g0.nacc;
@@ -795,18 +812,32 @@ __global__ __launch_bounds__(256) void k_resid_dep(const double* __restrict__ te
     double* rows = rstage + (size_t)wv * FBLOCK * (na + 1);
     double** dst = (double**)(rows + FBLOCK * na);
     double* target = nullptr;
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t kb = c < ntask ? out.key[c] : 0ull;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool will = false;
-    uint64_t w = 0;
-    int64_t t = 0;
+    int64_t c = 0, t = 0;
+    uint64_t kb = 0, w = 0;
     double weight = 0.0;
-    if (kb != 0ull) {
-        w = tasks[c];
-        t = (int64_t)(w & 0xffffffffull);
-        const int64_t whole = tetb[t];
-        weight = whole >= 0 ? tetf[t] : tetf[t] * out.sum[c];
-        will = (whole < 0 || whole == c + base) && weight != 0.0;
+    if (WHOLE) {
+        t = t0 + gid;
+        const int64_t cb = t < t1 ? tetb[t] : -1;
+        if (cb >= 0) {
+            c = cb - base;
+            kb = out.key[c];
+            w = tasks[c];
+            weight = tetf[t];
+            will = kb != 0ull && weight != 0.0;
+        }
+    } else {
+        c = gid;
+        kb = c < ntask ? out.key[c] : 0ull;
+        if (kb != 0ull) {
+            w = tasks[c];
+            t = (int64_t)(w & 0xffffffffull);
+            if (tetb[t] < 0) {
+                weight = tetf[t] * out.sum[c];
+                will = weight != 0.0;
+            }
+        }
     }
     const uint64_t active = __ballot(will);
     const int nact = __popcll(active);
@@ -912,17 +943,23 @@ static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask,
 }
 
 template <int ORDER>
-static void launch_resid_dep_o(const double* tets, const uint64_t* tasks, int64_t ntask, int64_t base,
-                               const PassGeo& G, TaskOut out, const double* tetf, const int64_t* tetb,
+static void launch_resid_dep_o(const double* tets, const uint64_t* tasks, int64_t ntask, int64_t base, int64_t t0,
+                               int64_t t1, const PassGeo& G, TaskOut out, const double* tetf, const int64_t* tetb,
                                hipStream_t s) {
-    const unsigned nb = (unsigned)((ntask + 255) / 256);
+    const unsigned nbt = (unsigned)((t1 - t0 + 255) / 256), nb = (unsigned)((ntask + 255) / 256);
     const size_t sb = (size_t)(256 / FBLOCK) * stage_bytes(G.g, FBLOCK);
-    if (G.table)
-        k_resid_dep<ORDER, true><<<nb, 256, sb, s>>>(tets, tasks, ntask, base, G.g, G.mb, G.acc, G.table, out, tetf,
-                                                     tetb);
-    else
-        k_resid_dep<ORDER, false><<<nb, 256, sb, s>>>(tets, tasks, ntask, base, G.g, G.mb, G.acc, nullptr, out, tetf,
-                                                      tetb);
+    if (nbt == 0u || nb == 0u) return;
+    if (G.table) {
+        k_resid_dep<ORDER, true, true><<<nbt, 256, sb, s>>>(tets, tasks, ntask, base, t0, t1, G.g, G.mb, G.acc,
+                                                            G.table, out, tetf, tetb);
+        k_resid_dep<ORDER, true, false><<<nb, 256, sb, s>>>(tets, tasks, ntask, base, t0, t1, G.g, G.mb, G.acc,
+                                                            G.table, out, tetf, tetb);
+    } else {
+        k_resid_dep<ORDER, false, true><<<nbt, 256, sb, s>>>(tets, tasks, ntask, base, t0, t1, G.g, G.mb, G.acc,
+                                                             nullptr, out, tetf, tetb);
+        k_resid_dep<ORDER, false, false><<<nb, 256, sb, s>>>(tets, tasks, ntask, base, t0, t1, G.g, G.mb, G.acc,
+                                                             nullptr, out, tetf, tetb);
+    }
 }
 
 /* the residual pass over the tets [t0, t1) and their ntask tasks; tetf,
@@ -933,11 +970,11 @@ static int launch_residual(const double* tets, int64_t t0, int64_t t1, const Pas
     k_resid_factor<<<(unsigned)((t1 - t0 + 255) / 256), 256, 0, s>>>(tets, t0, t1, colstart, base, out.sum, out.key,
                                                                      tetf, tetb);
     if (G.g.order == 0)
-        launch_resid_dep_o<0>(tets, tasks, ntask, base, G, out, tetf, tetb, s);
+        launch_resid_dep_o<0>(tets, tasks, ntask, base, t0, t1, G, out, tetf, tetb, s);
     else if (G.g.order == 1)
-        launch_resid_dep_o<1>(tets, tasks, ntask, base, G, out, tetf, tetb, s);
+        launch_resid_dep_o<1>(tets, tasks, ntask, base, t0, t1, G, out, tetf, tetb, s);
     else
-        launch_resid_dep_o<2>(tets, tasks, ntask, base, G, out, tetf, tetb, s);
+        launch_resid_dep_o<2>(tets, tasks, ntask, base, t0, t1, G, out, tetf, tetb, s);
     return (int)hipGetLastError();
 }
 
@@ -1197,7 +1234,7 @@ int upload(DevBuf& b, const void* src, size_t bytes, int dev, hipStream_t s, con
 
 /* A host-to-device upload of the grid's fields run by a thread of its own
  * on the workspace's side stream, so that the PCIe transfer overlaps the
- * deposit (which starts from a zeroed accumulator; k_merge_finalize adds
+ * deposit (which starts from a zeroed accumulator; the merge at the end adds
  * the fields once W.side_done has passed).  The destructor joins, so every
  * return path of the caller waits for the thread. */
 struct SideUpload {
@@ -1638,7 +1675,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
 
     SideUpload side;
     /* The deposit starts from a zeroed accumulator and the fields' present
-     * contents are merged in at the end (k_merge_finalize): cells that
+     * contents are merged in at the end (launch_merge_finalize): cells that
      * receive no mass keep their contents bit for bit, and large host
      * fields can travel while the kernels run. */
     auto start_acc = [&]() -> int {
@@ -1650,7 +1687,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
             int r = side.finish(W, s);
             if (r) return r;
         }
-        k_merge_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, acc, fptrs, nvox);
+        launch_merge_finalize(g, acc, fptrs, nvox, s);
         HIPCHK(hipGetLastError());
         return 0;
     };
@@ -2149,7 +2186,7 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                 const int i = i0 + q;
                 const GridInfo& g = G[i];
                 const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
-                k_merge_finalize<<<grid_blocks(nvox, 256), 256, 0, s>>>(g, bacc + acc_off[q], fp[q], nvox);
+                launch_merge_finalize(g, bacc + acc_off[q], fp[q], nvox, s);
                 HIPCHK(hipGetLastError());
                 for (int f = 0; f < 6; ++f)
                     if (grids[i].fields[f] && !grids[i].on_device &&
