@@ -1338,7 +1338,7 @@ PVX_HD void cell_finish(Sink& sink, const GridInfo& g, const ColumnState& S, con
  */
 template <int ORDER, class Sink>
 PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task, LPoly& P,
-                        TaskTally& T) {
+                        TaskTally& T, bool carry = true) {
     T.reset();
     ColumnState S;
     const int st = column_begin(g, mb, rec, task, P, S);
@@ -1356,7 +1356,7 @@ PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const 
         cell_planes(S, k, hl, lc, hh, hc);
         Mom Mm;
         float O[3];
-        slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O, &pc);
+        slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O, carry ? &pc : nullptr);
         cell_finish<ORDER>(sink, g, S, V, k, Mm, O, T);
     }
     return true;

@@ -46,6 +46,11 @@ static void fill_grid(GridInfo& g, MeshBox& mb, const int* n, const double* win6
     }
 }
 
+/* whether the column path carries each interior plane's point to the next
+ * cell (as the device does) or searches every plane afresh */
+static bool g_carry = true;
+extern "C" void hs_set_plane_carry(int on) { g_carry = on != 0; }
+
 /* the FP32 fast path on the host: kernel == 2 */
 extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const double* vel, const double* mass,
                               const int32_t* conn, int periodic, const double* box6, const int* n, const double* win6,
@@ -97,11 +102,11 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
                 bool ok;
                 TaskTally T;
                 if (g.order == 0)
-                    ok = column_task<0>(sink, g, mb, r, tasks[c], P, T);
+                    ok = column_task<0>(sink, g, mb, r, tasks[c], P, T, g_carry);
                 else if (g.order == 1)
-                    ok = column_task<1>(sink, g, mb, r, tasks[c], P, T);
+                    ok = column_task<1>(sink, g, mb, r, tasks[c], P, T, g_carry);
                 else
-                    ok = column_task<2>(sink, g, mb, r, tasks[c], P, T);
+                    ok = column_task<2>(sink, g, mb, r, tasks[c], P, T, g_carry);
                 if (!ok) fails++;
                 nfrag += T.nfrag;
                 tsum[c] = T.sum_m0;
