@@ -502,6 +502,25 @@ static int Mesh_init(Mesh* self, PyObject* args, PyObject* kwds) {
         }
         self->elemtype = (int)vpe;
         self->nelem = PyArray_DIMS(ca)[0];
+        {
+            /* the reference reads pos[conn] unchecked (src/cpsi.c:85-98); here
+             * an entry outside [0, npart) is an error, not a read past the
+             * array (the library checks device meshes the same way) */
+            PyArrayObject* cc = PyArray_GETCONTIGUOUS(ca);
+            if (!cc) return -1;
+            const int32_t* cd = (const int32_t*)PyArray_DATA(cc);
+            const npy_intp nc = PyArray_SIZE(cc);
+            npy_intp bad = -1;
+            for (npy_intp i = 0; i < nc && bad < 0; ++i)
+                if (cd[i] < 0 || (int64_t)cd[i] >= self->npart) bad = i;
+            const long long bv = bad >= 0 ? (long long)cd[bad] : 0;
+            Py_DECREF(cc);
+            if (bad >= 0) {
+                PyErr_Format(PyExc_ValueError, "connectivity entry %lld (element %lld) = %lld is outside [0, %lld)",
+                             (long long)bad, (long long)(bad / vpe), bv, (long long)self->npart);
+                return -1;
+            }
+        }
         Py_INCREF(connar);
         Py_XSETREF(self->connectivity, connar);
         if (velar == NULL) {

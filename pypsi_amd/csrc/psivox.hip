@@ -110,6 +110,26 @@ struct MeshDev {
     int vpe;
 };
 
+/* p[0, n) = v: clears flags and counters with a kernel (a memset node
+ * recorded into a captured graph is not used) */
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+/* *flag |= 1 when an entry of conn[0, n) lies outside [0, npart): every
+ * kernel reads particle arrays through conn, so a bad entry must stop the
+ * call before any of them runs */
+__global__ __launch_bounds__(256) void k_conn_check(const int32_t* __restrict__ conn, int64_t n, int64_t npart,
+                                                    unsigned* __restrict__ flag) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = conn[i];
+        bad = bad || v < 0 || v >= npart;
+    }
+    if (__any(bad) && (threadIdx.x & (warpSize - 1)) == 0) atomicOr(flag, 1u);
+}
+
 /* refine_max_lvl == 0: fixed slots (6 per element, 1 for simplices);
  * elements outside the window leave invalid records */
 /* tag: written to every record's slot 27 (the mesh's index in a batched
@@ -1077,7 +1097,7 @@ struct DevBuf {
 };
 
 struct Workspace {
-    DevBuf pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
+    DevBuf connchk, pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
         colsum, colkey, colmom, perm, cut, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt, bidx, bgat,
         an_box, an_valid, an_key, an_key2, an_id, an_id2, an_pcnt, an_poff, an_pairs, an_cntA, an_offA, an_cntB,
         an_offB, an_ta, an_tb, an_tcnt, an_toff, an_tasks;
@@ -1288,7 +1308,28 @@ struct SideUpload {
 
 inline unsigned grid_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-/* the mesh arrays as device pointers (uploaded when they are on the host) */
+/* PSIVOX_E_ARG unless every entry of the device connectivity conn (nelem x
+ * vpe) lies in [0, npart); waits for the check */
+int check_conn(Workspace& W, const int32_t* conn, int64_t nelem, int vpe, int64_t npart, int dev, hipStream_t s) {
+    if (nelem <= 0) return 0;
+    int rc = W.connchk.get(sizeof(unsigned), dev);
+    if (rc) return rc;
+    unsigned* flag = (unsigned*)W.connchk.p;
+    k_fill_u32<<<1, 64, 0, s>>>(flag, 1, 0u);
+    const int64_t n = nelem * vpe;
+    k_conn_check<<<std::min(grid_blocks(n, 256), 4096u), 256, 0, s>>>(conn, n, npart, flag);
+    HIPCHK(hipGetLastError());
+    unsigned h = 0;
+    HIPCHK(hipMemcpyAsync(&h, flag, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (h)
+        return set_err(PSIVOX_E_ARG, "connectivity entries must lie in [0, npart) with npart = %lld",
+                       (long long)npart);
+    return 0;
+}
+
+/* the mesh arrays as device pointers (uploaded when they are on the host),
+ * connectivity checked against npart */
 int mesh_to_device(Workspace& W, const psivox_mesh* mesh, int dev, hipStream_t s, MeshDev& md) {
     int rc;
     md = MeshDev{};
@@ -1299,7 +1340,7 @@ int mesh_to_device(Workspace& W, const psivox_mesh* mesh, int dev, hipStream_t s
         md.vel = mesh->vel;
         md.mass = mesh->mass;
         md.conn = mesh->conn;
-        return 0;
+        return check_conn(W, md.conn, mesh->nelem, md.vpe, mesh->npart, dev, s);
     }
     const void* p;
     if ((rc = upload(W.pos, mesh->pos, (size_t)mesh->npart * 3 * sizeof(double), dev, s, &p))) return rc;
@@ -1312,7 +1353,7 @@ int mesh_to_device(Workspace& W, const psivox_mesh* mesh, int dev, hipStream_t s
     md.mass = (const double*)p;
     if ((rc = upload(W.conn, mesh->conn, (size_t)mesh->nelem * md.vpe * sizeof(int32_t), dev, s, &p))) return rc;
     md.conn = (const int32_t*)p;
-    return 0;
+    return check_conn(W, md.conn, mesh->nelem, md.vpe, mesh->npart, dev, s);
 }
 
 /*
@@ -1875,6 +1916,10 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
     GridInfo g;
     for (int a = 0; a < 3; ++a) {
         if (grid->n[a] <= 0) return set_err(PSIVOX_E_ARG, "grid n[%d] = %d must be positive", a, grid->n[a]);
+        /* adjoint tasks carry the plane index in 17 bits, planes 0..n */
+        if (grid->n[a] >= ADJ_MAXPLANE)
+            return set_err(PSIVOX_E_ARG, "the mass adjoint takes at most %d cells per axis (n[%d] = %d)",
+                           ADJ_MAXPLANE - 1, a, grid->n[a]);
         if (!(grid->d[a] > 0.0)) return set_err(PSIVOX_E_ARG, "grid d[%d] must be positive", a);
         g.n[a] = grid->n[a];
         g.w0[a] = grid->window[0][a];
@@ -2092,6 +2137,7 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                 md[q].conn = (const int32_t*)mp;
                 mp += al(m.nelem * m.elemtype * 4);
             }
+            if ((rc = check_conn(W, md[q].conn, m.nelem, m.elemtype, m.npart, dev, s))) return rc;
             const GridInfo& g = G[i];
             const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
             double* fbase = (double*)W.bfield.p + fld_off[q];
@@ -2331,6 +2377,7 @@ int index_build(const psivox_mesh* mesh, const double window[2][3], hipStream_t 
             return rc;
         X->md.conn = (const int32_t*)p;
     }
+    if ((rc = check_conn(g_ws, X->md.conn, mesh->nelem, mesh->elemtype, mesh->npart, dev, s))) return rc;
     GridInfo g;
     memset(&g, 0, sizeof g);
     MeshBox mb;

@@ -85,3 +85,14 @@ def test_module_argument_errors():
         PSI.VDF(m, (0.5, 0.5))
     with pytest.raises(TypeError):
         PSI.RStarTree(mesh=3)
+
+
+def test_mesh_array_connectivity_range():
+    """Connectivity entries outside [0, npart) are refused when the mesh is
+    made (the reference reads pos[conn] unchecked, src/cpsi.c:85-98)."""
+    from pypsi_amd import PSI
+    pos = np.random.default_rng(0).uniform(size=(5, 3))
+    PSI.Mesh(loader="array", pos=pos, connectivity=np.array([[0, 1, 2, 4]], dtype=np.int32))
+    for bad in ([[0, 1, 2, 5]], [[-1, 1, 2, 3]], [[0, 1, 2, 3], [4, 3, 2, 7]]):
+        with pytest.raises(ValueError, match="outside"):
+            PSI.Mesh(loader="array", pos=pos, connectivity=np.array(bad, dtype=np.int32))
