@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PSIVOX_ABI_VERSION 1
+#define PSIVOX_ABI_VERSION 2
 
 /* element types: vertices per element (include/mesh.h:26-28) */
 #define PSIVOX_SIMPLEX 4
@@ -49,9 +49,10 @@ extern "C" {
 #define PSIVOX_MODE_ANNIHILATION 1
 
 /* kernels */
-#define PSIVOX_KERNEL_AUTO 0  /* the fastest available (PSIVOX_KERNEL_FAST) */
+#define PSIVOX_KERNEL_AUTO 0  /* the column kernel: FP32 for most tets, FP64 for tets spanning more than
+                                 4 cells along an axis (their smallest fragments need it) */
 #define PSIVOX_KERNEL_PLAIN 1 /* one thread per tet, FP64 clip */
-#define PSIVOX_KERNEL_FAST 2  /* one lane per (tet, column) task, FP32 cell-local clip, FP64 deposit */
+#define PSIVOX_KERNEL_FAST 2  /* one lane per (tet, column) task, FP32 cell-local clip for every tet, FP64 deposit */
 
 /* error codes */
 #define PSIVOX_OK 0
@@ -121,8 +122,9 @@ typedef struct {
                            column kernel (task setup, ordering and residual kernels not
                            included); a batched call reports its whole column pass */
     double total_ms;    /* wall time of the whole call */
-    int32_t kernel;     /* kernel that ran */
-    int32_t reserved;
+    int32_t kernel;     /* kernel that ran (PSIVOX_KERNEL_FAST for the column kernel in either precision) */
+    int32_t reserved;   /* 1: a tet the column path cannot take sent the call to the FP64 kernel */
+    int64_t nwide;      /* tets the automatic choice gave to the FP64 column kernel */
 } psivox_stats;
 
 /* One-mesh voxelization, PSI.voxels(grid, mesh, mode, refine_tolerance,

@@ -444,13 +444,6 @@ __global__ __launch_bounds__(64) void k_targets(double* __restrict__ tets, int64
     r[26] = window_target(r, g, mb, pool);
 }
 
-/* dst[i] = src[idx[i]] */
-__global__ void k_gather(const int64_t* __restrict__ src, const int64_t* __restrict__ idx, int n,
-                         int64_t* __restrict__ dst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[idx[i]];
-}
-
 /* One entry per mesh of a batched call: its grid, box and accumulator.
  * Tet records carry their entry's index in slot 27. */
 struct BatchGeo {
@@ -480,6 +473,9 @@ __device__ __forceinline__ void with_mesh(const double* r, const GridInfo& g0, c
     }
 }
 
+/* per-mesh counters of a batched call: fragments, copies, wide tets */
+constexpr int MCNT = 4;
+
 /* per-mesh counter add, aggregated over the wave when its lanes share a
  * mesh; every lane of the wave calls it (id < 0: nothing) */
 __device__ void mesh_count(unsigned long long* mcnt, int id, int64_t v, int slot) {
@@ -487,27 +483,38 @@ __device__ void mesh_count(unsigned long long* mcnt, int id, int64_t v, int slot
     if (__all(id == id0)) {
         unsigned long long sum = (unsigned long long)v;
         for (int o = warpSize / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-        if ((threadIdx.x & (warpSize - 1)) == 0 && id0 >= 0 && sum) atomicAdd(mcnt + 2 * id0 + slot, sum);
+        if ((threadIdx.x & (warpSize - 1)) == 0 && id0 >= 0 && sum) atomicAdd(mcnt + MCNT * id0 + slot, sum);
     } else if (id >= 0 && v) {
-        atomicAdd(mcnt + 2 * id + slot, (unsigned long long)v);
+        atomicAdd(mcnt + MCNT * id + slot, (unsigned long long)v);
     }
 }
 
-/* column tasks per tet record (0 for invalid records); a tet the column
- * path cannot take flags the FP64 fallback */
-template <bool BATCH>
+/* Column passes.  PASS_ALL: every tet in FP32 (kernel FAST).  Under the
+ * automatic choice, PASS_NARROW takes the tets that are not tet_wide in
+ * FP32 and counts the wide ones; PASS_WIDE takes only those, in FP64. */
+enum { PASS_ALL = 0, PASS_NARROW = 1, PASS_WIDE = 2 };
+
+/* column tasks per tet record for one pass (0 for invalid records and for
+ * tets of the other pass); a tet the column path cannot take flags the
+ * FP64 fallback.  counters[2] gets the wide tets PASS_NARROW leaves out. */
+template <bool BATCH, int PASS>
 __global__ __launch_bounds__(256) void k_col_count(const double* __restrict__ tets, int64_t ntet, GridInfo g0,
                                                    MeshBox mb0, const BatchGeo* table, int64_t* __restrict__ cnt,
                                                    unsigned* status, unsigned long long* counters,
                                                    unsigned long long* mcnt) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int nc = 0, id = -1;
+    int nc = 0, id = -1, nwide = 0;
     if (t < ntet) {
         const double* r = tets + (int64_t)TREC * t;
         int64_t n = 0;
         if (r[25] != 0.0) {
             with_mesh<BATCH>(r, g0, mb0, nullptr, table, [&](const GridInfo& g, const MeshBox& mb, double*, int m) {
-                n = tet_columns(r, t, g, mb, nullptr, &nc);
+                const bool wide = PASS != PASS_ALL && tet_wide(r, g);
+                if (PASS == PASS_NARROW && wide)
+                    nwide = 1;
+                else if (PASS != PASS_WIDE || wide)
+                    n = PASS == PASS_WIDE ? tet_columns<double>(r, t, g, mb, nullptr, &nc)
+                                          : tet_columns<float>(r, t, g, mb, nullptr, &nc);
                 id = m;
             });
             if (n < 0) {
@@ -518,11 +525,16 @@ __global__ __launch_bounds__(256) void k_col_count(const double* __restrict__ te
         cnt[t] = n;
     }
     block_count(0, nc, counters);
-    if (BATCH) mesh_count(mcnt, id, nc, 1);
+    if (PASS == PASS_NARROW) block_count(nwide, 0, counters + 2);
+    if (BATCH) {
+        mesh_count(mcnt, id, nc, 1);
+        if (PASS == PASS_NARROW) mesh_count(mcnt, id, nwide, 2);
+    }
 }
 
-/* write the column tasks of tets [t0, t1) at colstart[t] - base */
-template <bool BATCH>
+/* write the column tasks of tets [t0, t1) at colstart[t] - base (the tets
+ * with tasks in this pass; WIDE: enumerated for the FP64 kernel) */
+template <bool BATCH, bool WIDE>
 __global__ __launch_bounds__(256) void k_col_emit(const double* __restrict__ tets, int64_t t0, int64_t t1, GridInfo g0,
                                                   MeshBox mb0, const BatchGeo* table,
                                                   const int64_t* __restrict__ colstart, int64_t base,
@@ -532,7 +544,10 @@ __global__ __launch_bounds__(256) void k_col_emit(const double* __restrict__ tet
     const double* r = tets + (int64_t)TREC * t;
     if (r[25] == 0.0 || colstart[t + 1] == colstart[t]) return;
     with_mesh<BATCH>(r, g0, mb0, nullptr, table, [&](const GridInfo& g, const MeshBox& mb, double*, int) {
-        tet_columns(r, t, g, mb, tasks + (colstart[t] - base), nullptr);
+        if (WIDE)
+            tet_columns<double>(r, t, g, mb, tasks + (colstart[t] - base), nullptr);
+        else
+            tet_columns<float>(r, t, g, mb, tasks + (colstart[t] - base), nullptr);
     });
 }
 
@@ -557,9 +572,10 @@ static_assert(ORDW % 256 == 0, "whole threads per window");
 /* perm[w0 .. w0 + ORDW) of each window: the window's task indices, largest
  * class first (order within a class is arbitrary).  Also clears the column
  * kernel's wavefront counter.  256 threads: one class each in the scan. */
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
-                                                   int64_t ntask, GridInfo g, MeshBox mb, uint32_t* __restrict__ perm,
-                                                   unsigned* __restrict__ next) {
+                                                   int64_t ntask, GridInfo g0, MeshBox mb0, const BatchGeo* table,
+                                                   uint32_t* __restrict__ perm, unsigned* __restrict__ next) {
     __shared__ int cnt[ORDK];
     const int64_t w0 = (int64_t)blockIdx.x * ORDW;
     const int nw = (int)(ntask - w0 < ORDW ? ntask - w0 : ORDW);
@@ -578,13 +594,15 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
         int h, il, jl;
         unpack_task(w, t, h, il, jl);
         const double* r = tets + (int64_t)TREC * t;
-        double lo[3], hi[3];
-        aabb((const double (*)[3])r, 4, lo, hi);
-        uint32_t shp;
-        ghost_shifts_packed(lo, hi, g, mb, shp);
-        CopySetup C;
         int nc = 0, nv = 0;
-        if (copy_setup(r, lo, hi, shp, h, g, mb, C)) nc = column_cells(C, il, jl, &nv);
+        with_mesh<BATCH>(r, g0, mb0, nullptr, table, [&](const GridInfo& g, const MeshBox& mb, double*, int) {
+            double lo[3], hi[3];
+            aabb((const double (*)[3])r, 4, lo, hi);
+            uint32_t shp;
+            ghost_shifts_packed(lo, hi, g, mb, shp);
+            CopySetup<float> C;
+            if (copy_setup(r, lo, hi, shp, h, g, mb, C)) nc = column_cells(C, il, jl, &nv);
+        });
         nc = nc < 15 ? nc : 15;
         nv = nv < 15 ? nv : 15;
         key[q] = ORDK - 1 - (16 * nc + nv);
@@ -613,14 +631,6 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
     }
 }
 
-/* One lane per column task: FP32 column clip in this lane's LDS slice,
- * cell moments, FP64 deposit.  The grid is as many workgroups as the chip
- * holds at once; each takes wavefronts of tasks (perm order) from the
- * counter until it runs past the end.  The hardware places workgroups on
- * XCDs and CUs in a fixed rotation, and the XCDs do not run at quite the
- * same speed: with a fixed mapping the slowest XCD would set the time.
- * One mesh per launch: its geometry stays in kernel arguments (a batched
- * call launches it once per mesh over that mesh's contiguous task range). */
 /* A fragment's deposit staged in LDS: add(i, v) puts v at row[i - base]
  * (deposit_f adds each of the cell's nacc slots exactly once). */
 struct StageSink {
@@ -641,25 +651,158 @@ constexpr int SFRAG = FBLOCK / 2;
  * kernel built for them keeps no position data through its cell loop */
 constexpr int FMASK_NOX = ~(F_X | F_XX | F_XV);
 
-template <int ORDER, int FMASK>
+/* *p read as wave-uniform values (each 32-bit word through readfirstlane),
+ * so the compiler keeps it in scalar registers; p must be the same for
+ * every lane */
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    T v;
+    const uint32_t* s = (const uint32_t*)p;
+    uint32_t* d = (uint32_t*)&v;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = __builtin_amdgcn_readfirstlane(s[i]);
+    return v;
+}
+
+/*
+ * The work of one wavefront on the tasks of its lanes with act set, all of
+ * one mesh (geometry g, mb, accumulator acc): build each lane's column in
+ * its LDS slice, step the cells in lockstep, stage each step's fragments
+ * in LDS and issue their FP64 atomics slot by slot, and write each task's
+ * tally (fragment-mass sum, largest fragment) for the residual pass.
+ * Every lane of the wavefront calls it.
+ */
+template <int ORDER, int FMASK, class R>
+__device__ __forceinline__ int64_t wave_columns(bool act, int64_t p, uint64_t w, const double* r, const GridInfo& g,
+                                                const MeshBox& mb, double* __restrict__ acc, LPoly<R>& P,
+                                                double* stage, TaskOut out, unsigned* status) {
+    const int na = g.nacc;
+    int64_t* sgi = (int64_t*)(stage + SFRAG * na);
+    const int lane = threadIdx.x;
+    /* a lane's first staged value in a round of the transposed deposit, as
+     * (fragment, slot), and the step to its next */
+    const int f0 = lane / na, j0 = lane - f0 * na, fstep = FBLOCK / na, jstep = FBLOCK - fstep * na;
+    /* the task's tally: fragments, FP64 mass sum and its largest fragment
+     * (moments written to out.mom as they improve) */
+    int tfrag = 0;
+    double tsum = 0.0;
+    float bm0 = 0.0f;
+    int bkl = -1;
+    ColumnState<R> S;
+    TetVel<R> V;
+    int ncell = 0;
+    if (act) {
+        const int st = column_begin(g, mb, r, w, P, S);
+        if (st < 0) atomicOr(status, ST_FALLBACK);
+        if (st > 0) {
+            ncell = S.k1 - S.k0;
+            if (ORDER > 0) tet_vel(r, V);
+            V.mass = r[24];
+            pin(V.mass);
+        }
+    }
+    int nmax = ncell;
+    for (int o = FBLOCK / 2; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+    PlaneCarry<R> pc{false, false, {R(0), R(0), R(0)}};
+    /* cell steps in lockstep: every lane takes part in each step's
+     * deposit, which issues the wavefront's fragments slot by slot so
+     * that neighbouring lanes add into the same cell's consecutive
+     * slots (the memory-side atomic units then take several lanes per
+     * request; one lane per cell would give one request per lane) */
+    for (int it = 0; it < nmax; ++it) {
+        Mom<R> Mm;
+        R O[3];
+        int k = 0;
+        bool has = false;
+        if (it < ncell) {
+            k = S.k0 + it;
+            bool hl, hh;
+            double lc, hc;
+            cell_planes(S, k, hl, lc, hh, hc);
+            slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O, &pc);
+            has = Mm.m0 > R(0);
+        }
+        const uint64_t mask = __ballot(has);
+        if (mask == 0ull) continue;
+        const int nf = __popcll(mask);
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        /* the step's fragments in groups of SFRAG: stage, then issue */
+        for (int r0 = 0; r0 < nf; r0 += SFRAG) {
+            if (has && rank >= r0 && rank < r0 + SFRAG) {
+                const int64_t gi = ((int64_t)g.n[1] * g.n[2] * S.i + (int64_t)g.n[2] * S.j + k) * na;
+                StageSink ss{stage + (rank - r0) * na, gi};
+                deposit_f<ORDER, FMASK>(ss, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0, (const R (*)[3])V.DV,
+                                        V.mass, (double)Mm.m0);
+                sgi[rank - r0] = gi;
+            }
+            __syncthreads();
+            const int np = (nf - r0 < SFRAG ? nf - r0 : SFRAG) * na;
+            int f = f0, j = j0;
+            for (int q = lane; q < np; q += FBLOCK) {
+                unsafeAtomicAdd(acc + sgi[f] + j, stage[q]);
+                f += fstep;
+                j += jstep;
+                if (j >= na) {
+                    j -= na;
+                    ++f;
+                }
+            }
+            __syncthreads(); /* the stage is read before it is written again */
+        }
+        if (has) {
+            tfrag++;
+            tsum += (double)Mm.m0;
+            if ((float)Mm.m0 > bm0) {
+                bm0 = (float)Mm.m0;
+                bkl = k - S.C.G.ib0[2];
+                if (ORDER > 0) {
+                    float4* m = (float4*)(out.mom + 16 * p);
+                    m[0] = make_float4((float)Mm.m0, (float)Mm.s1[0], (float)Mm.s1[1], (float)Mm.s1[2]);
+                    m[1] = make_float4((float)Mm.s2[0], (float)Mm.s2[1], (float)Mm.s2[2], (float)Mm.s2[3]);
+                    m[2] = make_float4((float)Mm.s2[4], (float)Mm.s2[5], (float)O[0], (float)O[1]);
+                    m[3] = make_float4((float)O[2], 0.f, 0.f, 0.f);
+                }
+            }
+        }
+    }
+    if (act) {
+        out.sum[p] = tsum;
+        out.key[p] = bkl >= 0 ? best_key(bm0, bkl) : 0ull;
+    }
+    return tfrag;
+}
+
+/*
+ * One lane per column task: column clip in this lane's LDS slice (R =
+ * float, or double for the wide tets of the automatic choice), cell
+ * moments, FP64 deposit.  The grid is as many workgroups as the chip holds
+ * at once; each takes wavefronts of tasks (perm order) from the counter
+ * until it runs past the end.  The hardware places workgroups on XCDs and
+ * CUs in a fixed rotation, and the XCDs do not run at quite the same speed:
+ * with a fixed mapping the slowest XCD would set the time.
+ * BATCH: the tasks of many meshes in one launch.  A task's mesh is its tet
+ * record's slot 27; a wavefront works through the meshes among its lanes
+ * one at a time (tasks are in tet order and tets in mesh order, so only a
+ * wavefront at a boundary between meshes has more than one), with that
+ * mesh's table entry in scalar registers.
+ */
+template <int ORDER, int FMASK, class R, bool BATCH>
 __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
                                                         const uint64_t* __restrict__ tasks,
                                                         const uint32_t* __restrict__ perm, unsigned* __restrict__ next,
-                                                        int64_t ntask, GridInfo g, MeshBox mb,
-                                                        double* __restrict__ acc, TaskOut out, unsigned* status,
-                                                        unsigned long long* counters) {
-    __shared__ __align__(16) uint32_t sv[NVF * VSTRIDE];
+                                                        int64_t ntask, GridInfo g0, MeshBox mb0,
+                                                        double* __restrict__ acc0,
+                                                        const BatchGeo* __restrict__ table, TaskOut out,
+                                                        unsigned* status, unsigned long long* counters,
+                                                        unsigned long long* mcnt) {
+    __shared__ __align__(32) uint32_t sv[VRec<R>::WORDS];
     __shared__ unsigned swave;
     /* the deposit stage: up to SFRAG fragments of one cell step, in rank
      * order, nacc values each, then their accumulator offsets */
     extern __shared__ double stage[];
-    const int na = g.nacc;
-    int64_t* sgi = (int64_t*)(stage + SFRAG * na);
     const int lane = threadIdx.x;
-    LPoly P{(uint32_t*)__builtin_assume_aligned(&sv[4 * lane], 16), 0};
-    /* a lane's first staged value in a round of the transposed deposit, as
-     * (fragment, slot), and the step to its next */
-    const int f0 = lane / na, j0 = lane - f0 * na, fstep = FBLOCK / na, jstep = FBLOCK - fstep * na;
+    LPoly<R> P{(uint32_t*)__builtin_assume_aligned(&sv[VRec<R>::W * lane], 16), 0, 0, 0};
     int64_t nfrag = 0;
     for (;;) {
         if (lane == 0) swave = atomicAdd(next, 1u);
@@ -668,97 +811,32 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
         __syncthreads(); /* everyone has read swave before lane 0 writes it again */
         if (c0 >= ntask) break;
         const int64_t c = c0 + lane;
-        /* the task's tally: fragments, FP64 mass sum and its largest
-         * fragment (moments written to out.mom as they improve) */
-        int tfrag = 0;
-        double tsum = 0.0;
-        float bm0 = 0.0f;
-        int bkl = -1;
-        ColumnState S;
-        TetVel V;
         int64_t p = 0;
-        int ncell = 0;
+        uint64_t w = 0;
+        const double* r = tets;
+        int my = -1;
         if (c < ntask) {
             p = perm[c];
-            const uint64_t w = tasks[p];
-            const double* r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
-            const int st = column_begin(g, mb, r, w, P, S);
-            if (st < 0) atomicOr(status, ST_FALLBACK);
-            if (st > 0) {
-                ncell = S.k1 - S.k0;
-                if (ORDER > 0) tet_vel(r, V);
-                V.mass = r[24];
-                pin(V.mass);
-            }
+            w = tasks[p];
+            r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
+            my = BATCH ? (int)r[27] : 0;
         }
-        int nmax = ncell;
-        for (int o = FBLOCK / 2; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
-        PlaneCarry pc{false, false, {0.f, 0.f, 0.f}};
-        /* cell steps in lockstep: every lane takes part in each step's
-         * deposit, which issues the wavefront's fragments slot by slot so
-         * that neighbouring lanes add into the same cell's consecutive
-         * slots (the memory-side atomic units then take several lanes per
-         * request; one lane per cell would give one request per lane) */
-        for (int it = 0; it < nmax; ++it) {
-            Mom Mm;
-            float O[3];
-            int k = 0;
-            bool has = false;
-            if (it < ncell) {
-                k = S.k0 + it;
-                bool hl, hh;
-                double lc, hc;
-                cell_planes(S, k, hl, lc, hh, hc);
-                slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O, &pc);
-                has = Mm.m0 > 0.0f;
-            }
-            const uint64_t mask = __ballot(has);
-            if (mask == 0ull) continue;
-            const int nf = __popcll(mask);
-            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-            /* the step's fragments in groups of SFRAG: stage, then issue */
-            for (int r0 = 0; r0 < nf; r0 += SFRAG) {
-                if (has && rank >= r0 && rank < r0 + SFRAG) {
-                    const int64_t gi = ((int64_t)g.n[1] * g.n[2] * S.i + (int64_t)g.n[2] * S.j + k) * na;
-                    StageSink ss{stage + (rank - r0) * na, gi};
-                    deposit_f<ORDER, FMASK>(ss, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0,
-                                            (const float (*)[3])V.DV, V.mass, (double)Mm.m0);
-                    sgi[rank - r0] = gi;
-                }
-                __syncthreads();
-                const int np = (nf - r0 < SFRAG ? nf - r0 : SFRAG) * na;
-                int f = f0, j = j0;
-                for (int q = lane; q < np; q += FBLOCK) {
-                    unsafeAtomicAdd(acc + sgi[f] + j, stage[q]);
-                    f += fstep;
-                    j += jstep;
-                    if (j >= na) {
-                        j -= na;
-                        ++f;
-                    }
-                }
-                __syncthreads(); /* the stage is read before it is written again */
-            }
-            if (has) {
-                tfrag++;
-                tsum += (double)Mm.m0;
-                if (Mm.m0 > bm0) {
-                    bm0 = Mm.m0;
-                    bkl = k - S.C.G.ib0[2];
-                    if (ORDER > 0) {
-                        float4* m = (float4*)(out.mom + 16 * p);
-                        m[0] = make_float4(Mm.m0, Mm.s1[0], Mm.s1[1], Mm.s1[2]);
-                        m[1] = make_float4(Mm.s2[0], Mm.s2[1], Mm.s2[2], Mm.s2[3]);
-                        m[2] = make_float4(Mm.s2[4], Mm.s2[5], O[0], O[1]);
-                        m[3] = make_float4(O[2], 0.f, 0.f, 0.f);
-                    }
-                }
-            }
+        if (!BATCH) {
+            nfrag += wave_columns<ORDER, FMASK, R>(my >= 0, p, w, r, g0, mb0, acc0, P, stage, out, status);
+            continue;
         }
-        if (c < ntask) {
-            nfrag += tfrag;
-            out.sum[p] = tsum;
-            out.key[p] = bkl >= 0 ? best_key(bm0, bkl) : 0ull;
+        uint64_t pending = __ballot(my >= 0);
+        while (pending) {
+            const int id = __builtin_amdgcn_readlane(my, (int)__builtin_ctzll(pending));
+            const bool act = my == id;
+            pending &= ~__ballot(act);
+            const GridInfo g = load_uniform(&table[id].g);
+            const MeshBox mb = load_uniform(&table[id].mb);
+            double* acc = load_uniform(&table[id].acc);
+            /* inactive lanes count no fragments */
+            const int64_t nf = wave_columns<ORDER, FMASK, R>(act, p, w, r, g, mb, acc, P, stage, out, status);
+            mesh_count(mcnt, id, nf, 0);
+            nfrag += nf;
         }
     }
     block_count(nfrag, 0, counters);
@@ -917,7 +995,7 @@ struct PassGeo {
 };
 
 typedef void (*KColsFn)(const double*, const uint64_t*, const uint32_t*, unsigned*, int64_t, GridInfo, MeshBox,
-                        double*, TaskOut, unsigned*, unsigned long long*);
+                        double*, const BatchGeo*, TaskOut, unsigned*, unsigned long long*, unsigned long long*);
 
 /* workgroups of a column kernel the current device holds at once, with sb
  * bytes of dynamic LDS */
@@ -936,29 +1014,46 @@ static int64_t resident_blocks(KColsFn kern, size_t sb) {
     return (int64_t)ncu * (per > 0 ? per : 8);
 }
 
-/* the column kernel built for a grid's moment order and field set */
-static KColsFn cols_kernel(const GridInfo& g) {
+/* the column kernel built for a grid's moment order and field set, in FP32
+ * or (wide tets) FP64, for one mesh or a batch */
+template <class R, bool BATCH>
+static KColsFn cols_kernel_t(const GridInfo& g) {
     const bool nox = !(g.flags & (F_X | F_XX | F_XV));
-    if (g.order == 0) return k_vox_cols<0, -1>;
-    if (g.order == 1) return nox ? k_vox_cols<1, FMASK_NOX> : k_vox_cols<1, -1>;
-    return nox ? k_vox_cols<2, FMASK_NOX> : k_vox_cols<2, -1>;
+    if (g.order == 0) return k_vox_cols<0, -1, R, BATCH>;
+    /* the one-mesh FP32 kernel, the hot one, is also built without the
+     * position-moment code */
+    if constexpr (sizeof(R) == 4 && !BATCH) {
+        if (nox) return g.order == 1 ? k_vox_cols<1, FMASK_NOX, R, BATCH> : k_vox_cols<2, FMASK_NOX, R, BATCH>;
+    }
+    (void)nox;
+    return g.order == 1 ? k_vox_cols<1, -1, R, BATCH> : k_vox_cols<2, -1, R, BATCH>;
 }
 
-/* the column kernel over tasks [0, ntask) of one mesh (perm: ntask
- * entries of scratch for the task order; next: one counter); mark, when
- * given, is recorded between the ordering and the column kernel */
-static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask, const GridInfo& g, const MeshBox& mb,
-                       double* acc, TaskOut out, uint32_t* perm, unsigned* next, unsigned* status,
-                       unsigned long long* counters, hipStream_t s, hipEvent_t mark = nullptr) {
-    k_col_order<<<(unsigned)((ntask + ORDW - 1) / ORDW), 256, 0, s>>>(tets, tasks, ntask, g, mb, perm, next);
+static KColsFn cols_kernel(const GridInfo& g, bool fp64, bool batch) {
+    if (fp64) return batch ? cols_kernel_t<double, true>(g) : cols_kernel_t<double, false>(g);
+    return batch ? cols_kernel_t<float, true>(g) : cols_kernel_t<float, false>(g);
+}
+
+/* the column kernel over tasks [0, ntask) of the pass geometry G (one mesh,
+ * or a batch's table), FP64 for wide tets (perm: ntask entries of scratch
+ * for the task order; next: one counter); mark, when given, is recorded
+ * between the ordering and the column kernel */
+static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask, const PassGeo& G, bool fp64,
+                       TaskOut out, uint32_t* perm, unsigned* next, unsigned* status, unsigned long long* counters,
+                       hipStream_t s, hipEvent_t mark = nullptr) {
+    const unsigned nord = (unsigned)((ntask + ORDW - 1) / ORDW);
+    if (G.table)
+        k_col_order<true><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, G.table, perm, next);
+    else
+        k_col_order<false><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, nullptr, perm, next);
     if (mark) (void)hipEventRecord(mark, s);
     const int64_t nwave = (ntask + FBLOCK - 1) / FBLOCK;
-    const size_t sb = stage_bytes(g, SFRAG);
-    const KColsFn kern = cols_kernel(g);
+    const size_t sb = stage_bytes(G.g, SFRAG);
+    const KColsFn kern = cols_kernel(G.g, fp64, G.table != nullptr);
     const int64_t hold = resident_blocks(kern, sb);
     const unsigned nb = (unsigned)(nwave < hold ? nwave : hold);
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(FBLOCK), sb, s, tets, tasks, perm, next, ntask, g, mb, acc, out, status,
-                       counters);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(FBLOCK), sb, s, tets, tasks, perm, next, ntask, G.g, G.mb, G.acc, G.table,
+                       out, status, counters, G.mcnt);
     return (int)hipGetLastError();
 }
 
@@ -1098,7 +1193,7 @@ struct DevBuf {
 
 struct Workspace {
     DevBuf connchk, pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
-        colsum, colkey, colmom, perm, cut, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt, bidx, bgat,
+        colsum, colkey, colmom, perm, cut, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt,
         an_box, an_valid, an_key, an_key2, an_id, an_id2, an_pcnt, an_poff, an_pairs, an_cntA, an_offA, an_cntB,
         an_offB, an_ta, an_tb, an_tcnt, an_toff, an_tasks;
     hipStream_t stream = nullptr;
@@ -1356,12 +1451,6 @@ int mesh_to_device(Workspace& W, const psivox_mesh* mesh, int dev, hipStream_t s
     return check_conn(W, md.conn, mesh->nelem, md.vpe, mesh->npart, dev, s);
 }
 
-/*
- * Column tasks of tet records [0, nt): count, scan, and for each chunk of
- * tets whose tasks fit the task buffer emit them and call
- * fn(a0, a1, colstart, base, ntask_chunk).  Returns PSIVOX_E_* or 0; *fell
- * is set (and nothing run) when a tet cannot take the column path.
- */
 /* The end a1 of the task chunk that starts at tet a0: the largest a1 <= nt
  * with colstart[a1] - colstart[a0] <= cap (a single tet has fewer than
  * TASK_MAXPERTET < cap tasks, so a1 > a0).  out = {a1, colstart[a0],
@@ -1388,22 +1477,53 @@ __global__ void k_chunk_cut(const int64_t* __restrict__ colstart, int64_t nt, in
  * task-order entries */
 static unsigned* perm_counter(const DevBuf& b, int64_t nsub) { return (unsigned*)b.p + nsub; }
 
+/* k_col_count for a pass, one mesh or a batch */
+template <bool BATCH>
+static void launch_col_count(int pass, const double* tets, int64_t nt, const PassGeo& G, int64_t* cnt, unsigned* status,
+                             unsigned long long* counters, hipStream_t s) {
+    const unsigned nb = grid_blocks(nt, 256);
+    if (pass == PASS_NARROW)
+        k_col_count<BATCH, PASS_NARROW><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
+    else if (pass == PASS_WIDE)
+        k_col_count<BATCH, PASS_WIDE><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
+    else
+        k_col_count<BATCH, PASS_ALL><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
+}
+
+template <bool BATCH>
+static void launch_col_emit(bool wide, const double* tets, int64_t t0, int64_t t1, const PassGeo& G,
+                            const int64_t* colstart, int64_t base, uint64_t* tasks, hipStream_t s) {
+    const unsigned nb = grid_blocks(t1 - t0, 256);
+    if (wide)
+        k_col_emit<BATCH, true><<<nb, 256, 0, s>>>(tets, t0, t1, G.g, G.mb, G.table, colstart, base, tasks);
+    else
+        k_col_emit<BATCH, false><<<nb, 256, 0, s>>>(tets, t0, t1, G.g, G.mb, G.table, colstart, base, tasks);
+}
+
+/*
+ * Column tasks of tet records [0, nt) for one pass (PASS_*): count, scan,
+ * and for each chunk of tets whose tasks fit the task buffer emit them and
+ * call fn(a0, a1, colstart, base, tasks, ntask_chunk).  Returns PSIVOX_E_*
+ * or 0; *fell is set (and nothing run) when a tet cannot take the column
+ * path.  nwide (PASS_NARROW): the wide tets left for PASS_WIDE.
+ */
 template <class Fn>
-int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo& G, unsigned* status,
-                    unsigned long long* counters, hipStream_t s, int dev, bool* fell, Fn fn) {
+int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo& G, int pass, unsigned* status,
+                    unsigned long long* counters, hipStream_t s, int dev, bool* fell, int64_t* nwide, Fn fn) {
     int rc;
     *fell = false;
+    if (nwide) *nwide = 0;
     if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
     if ((rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
     int64_t* colcnt = (int64_t*)W.colcnt.p;
     int64_t* colstart = (int64_t*)W.colstart.p;
-    HIPCHK(hipMemsetAsync(colcnt + nt, 0, sizeof(int64_t), s));
+    /* colcnt[nt] = 0 and the wide-tet counter counters[2] = 0 */
+    k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)(colcnt + nt), 2, 0u);
+    k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)(counters + 2), 2, 0u);
     if (G.table)
-        k_col_count<true><<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, colcnt, status, counters,
-                                                               G.mcnt);
+        launch_col_count<true>(pass, tets, nt, G, colcnt, status, counters, s);
     else
-        k_col_count<false><<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, G.g, G.mb, nullptr, colcnt, status,
-                                                                counters, nullptr);
+        launch_col_count<false>(pass, tets, nt, G, colcnt, status, counters, s);
     HIPCHK(hipGetLastError());
     size_t tb = 0;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, colcnt, colstart, nt + 1, s));
@@ -1411,9 +1531,12 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp.p, tb, colcnt, colstart, nt + 1, s));
     int64_t ntask = 0;
     unsigned hs = 0;
+    unsigned long long hw = 0;
     HIPCHK(hipMemcpyAsync(&ntask, colstart + nt, sizeof ntask, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&hs, status, sizeof hs, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hw, counters + 2, sizeof hw, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (nwide) *nwide = (int64_t)hw;
     if (hs & ST_FALLBACK) {
         *fell = true;
         return 0;
@@ -1442,11 +1565,9 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
         }
         if (nsub == 0) continue;
         if (G.table)
-            k_col_emit<true><<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, G.g, G.mb, G.table, colstart,
-                                                                       base, tasks);
+            launch_col_emit<true>(pass == PASS_WIDE, tets, a0, a1, G, colstart, base, tasks, s);
         else
-            k_col_emit<false><<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, G.g, G.mb, nullptr, colstart,
-                                                                        base, tasks);
+            launch_col_emit<false>(pass == PASS_WIDE, tets, a0, a1, G, colstart, base, tasks, s);
         HIPCHK(hipGetLastError());
         if ((rc = fn(a0, a1, (const int64_t*)colstart, base, (const uint64_t*)tasks, nsub))) return rc;
     }
@@ -1649,6 +1770,8 @@ int run_annihilation(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, c
         st->nfrag = (int64_t)hcnt[0];
         st->kernel_ms = kms;
         st->kernel = PSIVOX_KERNEL_PLAIN;
+        st->reserved = 0;
+        st->nwide = 0;
         st->total_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     }
@@ -1668,6 +1791,8 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     int kernel = o->kernel == PSIVOX_KERNEL_AUTO ? PSIVOX_KERNEL_FAST : o->kernel;
     if (kernel != PSIVOX_KERNEL_PLAIN && kernel != PSIVOX_KERNEL_FAST)
         return set_err(PSIVOX_E_ARG, "invalid kernel %d", o->kernel);
+    /* the automatic choice sends wide tets through the FP64 column kernel */
+    const bool route = o->kernel == PSIVOX_KERNEL_AUTO;
 
     MeshBox mb;
     mb.periodic = mesh->periodic ? 1 : 0;
@@ -1743,7 +1868,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
 
     int64_t batch = o->batch_elements > 0 ? o->batch_elements : (int64_t)1 << 22;
     const int64_t ne = mesh->nelem;
-    int64_t ntets_total = 0;
+    int64_t ntets_total = 0, nwide_total = 0;
     double kms = 0.0;
     int fell_back = 0;
 
@@ -1806,39 +1931,48 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 k_vox_plain<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, acc, status, counters);
                 HIPCHK(hipGetLastError());
             } else {
-                /* column tasks, then the per-tet residual pass */
-                bool fell = false;
+                /* column tasks, then the per-tet residual pass; under the
+                 * automatic choice the wide tets take a second pass in FP64
+                 * (a tet that cannot take the column path stops the passes:
+                 * the status check below restarts with the FP64 kernel) */
                 const PassGeo G{g, mb, acc, nullptr, nullptr};
-                rc = for_task_chunks(
-                    W, tets, nt, G, status, counters, s, dev, &fell,
-                    [&](int64_t a0, int64_t a1, const int64_t* colstart, int64_t base, const uint64_t* tasks,
-                        int64_t nsub) -> int {
-                        int rc2;
-                        if ((rc2 = W.colsum.get((size_t)nsub * sizeof(double), dev))) return rc2;
-                        if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
-                        if (g.order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev))) return rc2;
-                        if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
-                        TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
-                        rc2 = launch_cols(tets, tasks, nsub, g, mb, acc, out, (uint32_t*)W.perm.p,
-                                          perm_counter(W.perm, nsub), status, counters, s, W.ev0);
-                        if (rc2)
-                            return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
-                                           hipGetErrorString((hipError_t)rc2));
-                        HIPCHK(hipEventRecord(W.ev1, s));
-                        HIPCHK(hipEventSynchronize(W.ev1));
-                        float ms = 0.f;
-                        HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
-                        kms += ms;
-                        if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
-                        rc2 = launch_residual(tets, a0, a1, G, colstart, base, tasks, nsub, out,
-                                              (double*)W.colcnt.p, (int64_t*)W.tetb.p, s);
-                        if (rc2)
-                            return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
-                                           hipGetErrorString((hipError_t)rc2));
-                        return 0;
-                    });
-                if (rc) return rc;
-                (void)fell; /* the status check below restarts with the FP64 kernel */
+                int64_t nwide = 0;
+                for (int pass = route ? PASS_NARROW : PASS_ALL;; pass = PASS_WIDE) {
+                    bool fell = false;
+                    rc = for_task_chunks(
+                        W, tets, nt, G, pass, status, counters, s, dev, &fell,
+                        pass == PASS_NARROW ? &nwide : nullptr,
+                        [&](int64_t a0, int64_t a1, const int64_t* colstart, int64_t base, const uint64_t* tasks,
+                            int64_t nsub) -> int {
+                            int rc2;
+                            if ((rc2 = W.colsum.get((size_t)nsub * sizeof(double), dev))) return rc2;
+                            if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
+                            if (g.order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev)))
+                                return rc2;
+                            if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
+                            TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
+                            rc2 = launch_cols(tets, tasks, nsub, G, pass == PASS_WIDE, out, (uint32_t*)W.perm.p,
+                                              perm_counter(W.perm, nsub), status, counters, s, W.ev0);
+                            if (rc2)
+                                return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
+                                               hipGetErrorString((hipError_t)rc2));
+                            HIPCHK(hipEventRecord(W.ev1, s));
+                            HIPCHK(hipEventSynchronize(W.ev1));
+                            float ms = 0.f;
+                            HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+                            kms += ms;
+                            if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
+                            rc2 = launch_residual(tets, a0, a1, G, colstart, base, tasks, nsub, out,
+                                                  (double*)W.colcnt.p, (int64_t*)W.tetb.p, s);
+                            if (rc2)
+                                return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
+                                               hipGetErrorString((hipError_t)rc2));
+                            return 0;
+                        });
+                    if (rc) return rc;
+                    if (fell || pass != PASS_NARROW || nwide == 0) break;
+                }
+                nwide_total += nwide;
             }
             if (kernel == PSIVOX_KERNEL_PLAIN) {
                 HIPCHK(hipEventRecord(W.ev1, s));
@@ -1862,6 +1996,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                     e0 = 0;
                     kms = 0.0;
                     ntets_total = 0;
+                    nwide_total = 0;
                     continue;
                 }
             }
@@ -1901,6 +2036,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
         st->total_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
         st->nelem_used = -1;
+        st->nwide = nwide_total;
     }
     return 0;
 }
@@ -2021,9 +2157,9 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
  * Many (mesh, grid) pairs in one device pass.  Meshes are grouped up to the
  * tet-buffer capacity; per group: raw conversion and expansion per mesh
  * (records tagged with the mesh's index), one count / scan / emit over all
- * its tets, the column kernel launched per mesh on that mesh's contiguous
- * task range (its geometry in kernel arguments), one residual pass, and
- * finalize per mesh.  The pass needs one field set for the whole batch,
+ * its tets, one ordering and one column kernel over all its tasks (each
+ * wavefront reads its meshes' geometry from a table), one residual pass,
+ * and finalize per mesh.  The pass needs one field set for the whole batch,
  * refinement level 0 and the column kernel; other batches, and a group in
  * which a tet needs the FP64 kernel, run mesh by mesh.
  */
@@ -2032,6 +2168,7 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
     auto t_start = std::chrono::steady_clock::now();
     int rc;
     const int kernel = o->kernel == PSIVOX_KERNEL_AUTO ? PSIVOX_KERNEL_FAST : o->kernel;
+    const bool route = o->kernel == PSIVOX_KERNEL_AUTO;
     std::vector<GridInfo> G(nmesh);
     std::vector<MeshBox> MB(nmesh);
     bool same = kernel == PSIVOX_KERNEL_FAST && o->mode == PSIVOX_MODE_DENSITY;
@@ -2099,13 +2236,13 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
         if ((rc = W.bfield.get((size_t)fld_off[nm] * sizeof(double), dev))) return rc;
         if ((rc = W.bacc.get((size_t)acc_off[nm] * sizeof(double), dev))) return rc;
         if ((rc = W.tets.get((size_t)nt * TREC * sizeof(double), dev))) return rc;
-        if ((rc = W.bmcnt.get((size_t)(2 * nm + 8) * sizeof(unsigned long long), dev))) return rc;
+        if ((rc = W.bmcnt.get((size_t)(MCNT * nm + 8) * sizeof(unsigned long long), dev))) return rc;
         if ((rc = W.counters.get(64, dev))) return rc;
         unsigned long long* counters = (unsigned long long*)W.counters.p;
         unsigned* status = (unsigned*)(counters + 4);
         unsigned long long* mcnt = (unsigned long long*)W.bmcnt.p;
         HIPCHK(hipMemsetAsync(counters, 0, 64, s));
-        HIPCHK(hipMemsetAsync(mcnt, 0, (size_t)(2 * nm) * sizeof(unsigned long long), s));
+        HIPCHK(hipMemsetAsync(mcnt, 0, (size_t)(MCNT * nm) * sizeof(unsigned long long), s));
         double* tets = (double*)W.tets.p;
         double* bacc = (double*)W.bacc.p;
         std::vector<FieldPtrs> fp(nm);
@@ -2169,55 +2306,45 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
         for (int q = 0; q < nm; ++q) hb[q] = BatchGeo{G[i0 + q], MB[i0 + q], bacc + acc_off[q]};
         if ((rc = W.btable.get(nm * sizeof(BatchGeo), dev))) return rc;
         HIPCHK(hipMemcpyAsync(W.btable.p, hb.data(), nm * sizeof(BatchGeo), hipMemcpyHostToDevice, s));
-        if ((rc = W.bidx.get((nm + 1) * sizeof(int64_t), dev))) return rc;
-        if ((rc = W.bgat.get((nm + 1) * sizeof(int64_t), dev))) return rc;
         const PassGeo PG{G[i0], MB[i0], nullptr, (const BatchGeo*)W.btable.p, mcnt};
         double kms = 0.0;
         bool fell = false;
-        rc = for_task_chunks(
-            W, tets, nt, PG, status, counters, s, dev, &fell,
-            [&](int64_t a0, int64_t a1, const int64_t* colstart, int64_t base, const uint64_t* tasks,
-                int64_t nsub) -> int {
-                int rc2;
-                /* task ranges of the meshes in this chunk of tets */
-                std::vector<int64_t> bnd(nm + 1), hv(nm + 1);
-                for (int q = 0; q <= nm; ++q) bnd[q] = std::min(std::max(T[q], a0), a1);
-                HIPCHK(hipMemcpyAsync(W.bidx.p, bnd.data(), (nm + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-                k_gather<<<grid_blocks(nm + 1, 64), 64, 0, s>>>(colstart, (const int64_t*)W.bidx.p, nm + 1,
-                                                                (int64_t*)W.bgat.p);
-                HIPCHK(hipGetLastError());
-                HIPCHK(hipMemcpyAsync(hv.data(), W.bgat.p, (nm + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-                HIPCHK(hipStreamSynchronize(s));
-                if ((rc2 = W.colsum.get((size_t)nsub * sizeof(double), dev))) return rc2;
-                if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
-                if (G[i0].order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev))) return rc2;
-                if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
-                TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
-                HIPCHK(hipEventRecord(W.ev0, s));
-                for (int q = 0; q < nm; ++q) {
-                    const int64_t c0 = hv[q] - base, c1 = hv[q + 1] - base;
-                    if (c1 <= c0) continue;
-                    TaskOut oq{out.sum + c0, out.key + c0, out.mom ? out.mom + 16 * c0 : nullptr};
-                    rc2 = launch_cols(tets, tasks + c0, c1 - c0, G[i0 + q], MB[i0 + q], bacc + acc_off[q], oq,
-                                      (uint32_t*)W.perm.p + c0, perm_counter(W.perm, nsub), status, mcnt + 2 * q, s);
+        int64_t nwide = 0;
+        /* one ordering and one column kernel over the tasks of all the
+         * group's meshes per chunk (and pass) */
+        for (int pass = route ? PASS_NARROW : PASS_ALL;; pass = PASS_WIDE) {
+            rc = for_task_chunks(
+                W, tets, nt, PG, pass, status, counters, s, dev, &fell, pass == PASS_NARROW ? &nwide : nullptr,
+                [&](int64_t a0, int64_t a1, const int64_t* colstart, int64_t base, const uint64_t* tasks,
+                    int64_t nsub) -> int {
+                    int rc2;
+                    if ((rc2 = W.colsum.get((size_t)nsub * sizeof(double), dev))) return rc2;
+                    if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
+                    if (G[i0].order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev)))
+                        return rc2;
+                    if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
+                    TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
+                    rc2 = launch_cols(tets, tasks, nsub, PG, pass == PASS_WIDE, out, (uint32_t*)W.perm.p,
+                                      perm_counter(W.perm, nsub), status, counters, s, W.ev0);
                     if (rc2)
                         return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
                                        hipGetErrorString((hipError_t)rc2));
-                }
-                HIPCHK(hipEventRecord(W.ev1, s));
-                if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
-                rc2 = launch_residual(tets, a0, a1, PG, colstart, base, tasks, nsub, out, (double*)W.colcnt.p,
-                                      (int64_t*)W.tetb.p, s);
-                if (rc2)
-                    return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
-                                   hipGetErrorString((hipError_t)rc2));
-                HIPCHK(hipEventSynchronize(W.ev1));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
-                kms += ms;
-                return 0;
-            });
-        if (rc) return rc;
+                    HIPCHK(hipEventRecord(W.ev1, s));
+                    if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
+                    rc2 = launch_residual(tets, a0, a1, PG, colstart, base, tasks, nsub, out, (double*)W.colcnt.p,
+                                          (int64_t*)W.tetb.p, s);
+                    if (rc2)
+                        return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
+                                       hipGetErrorString((hipError_t)rc2));
+                    HIPCHK(hipEventSynchronize(W.ev1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+                    kms += ms;
+                    return 0;
+                });
+            if (rc) return rc;
+            if (fell || pass != PASS_NARROW || nwide == 0) break;
+        }
         unsigned hs = 0;
         HIPCHK(hipMemcpyAsync(&hs, status, sizeof hs, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -2227,7 +2354,7 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
             for (int i = i0; i < i1; ++i)
                 if ((rc = one(i))) return rc;
         } else {
-            std::vector<unsigned long long> hm(2 * nm);
+            std::vector<unsigned long long> hm(MCNT * nm);
             for (int q = 0; q < nm; ++q) {
                 const int i = i0 + q;
                 const GridInfo& g = G[i];
@@ -2240,15 +2367,16 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                                        s)))
                         return rc;
             }
-            HIPCHK(hipMemcpyAsync(hm.data(), mcnt, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hm.data(), mcnt, MCNT * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (stats) {
                 const double wall =
                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
                 for (int q = 0; q < nm; ++q) {
                     psivox_stats& st = stats[i0 + q];
-                    st.nfrag = (int64_t)hm[2 * q];
-                    st.nghosts = (int64_t)hm[2 * q + 1];
+                    st.nfrag = (int64_t)hm[MCNT * q];
+                    st.nghosts = (int64_t)hm[MCNT * q + 1];
+                    st.nwide = (int64_t)hm[MCNT * q + 2];
                     st.ntets = T[q + 1] - T[q];
                     st.kernel_ms = kms;
                     st.total_ms = wall;

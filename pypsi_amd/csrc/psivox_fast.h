@@ -1,11 +1,11 @@
 /*
- * psivox_fast.h -- the fast voxelization path: FP32 clipping in cell-local
+ * psivox_fast.h -- the column voxelization path: clipping in cell-local
  * coordinates, FP64 deposit, one lane per column task (a tet's periodic copy
  * restricted to one (x, y) column of its index box).
  *
  * A task clips the tet by the column's four planes into a polyhedron held in
  * the lane's slice of LDS (vertex graph with 3 neighbours per vertex as in
- * the reference, include/geometry.h:118-129, but storing only the FP32 mass
+ * the reference, include/geometry.h:118-129, but storing only the mass
  * coordinate q: positions are x(q) = x0 + DX q in grid units relative to the
  * copy's index box).  Each cell of the column is then integrated without
  * building its fragment: the column's faces are fanned once as the
@@ -14,14 +14,21 @@
  * weighted terms (no branches on the geometry, so the lanes of a wavefront
  * stay together).
  *
+ * Everything geometric is templated on the real type R.  R = float is the
+ * fast path; R = double is the same method in FP64, used for the few tets
+ * wide enough (tet_wide) that FP32 cell-local coordinates would lose the
+ * accuracy of their smallest fragments: an FP32 cut point carries an error
+ * of about FP32 epsilon times the tet's extent, so a sliver fragment of a
+ * tet spanning many cells gets a large relative error.
+ *
  * Differences from the reference arithmetic (the method is the same):
- *   - cuts and moments in FP32 on cell-local coordinates, plane offsets
- *     formed in FP64; a vertex within 1e-5 cells of a plane is classified
- *     with an FP64 evaluation of its signed distance;
+ *   - cuts and moments in R on cell-local coordinates, plane offsets
+ *     formed in FP64; in FP32 a vertex within 1e-5 cells of a plane is
+ *     classified with an FP64 evaluation of its signed distance;
  *   - each column is clipped from the tet rather than from its neighbour
- *     slab, so shared faces of neighbouring columns agree only to FP32
+ *     slab, so shared faces of neighbouring columns agree only to R
  *     rounding; the per-tet mass residual (FP64 exact target minus FP64
- *     sum of the FP32 fragment masses) goes to its largest fragment when
+ *     sum of the fragment masses) goes to its largest fragment when
  *     small, otherwise it is spread over the largest fragment of each of
  *     its columns; total mass is conserved to FP64 rounding;
  *   - deposits are the same additive FP64 raw moments as the FP64 kernel.
@@ -40,11 +47,26 @@ namespace pvx {
 
 constexpr int NVF = 16;   /* vertex slots of the column polyhedron (tet and 4 planes: <= 12) */
 constexpr int FBLOCK = 64; /* lanes per workgroup (one wavefront) */
-/* words between a lane's consecutive vertex slots: a vertex is 4 words
- * (q0, q1, q2 as FP32 bit patterns, then its link word) and the 64 lanes'
- * copies of one slot are consecutive, so one 16-byte read fetches a vertex
- * and 8 consecutive lanes cover all 32 banks */
-constexpr int VSTRIDE = 4 * FBLOCK;
+
+/* Words of one vertex record in LDS: q0, q1, q2 (as R), then the link word
+ * (padded to 8 words for R = double).  The 64 lanes' copies of one slot are
+ * consecutive, so one wide read fetches a vertex and neighbouring lanes
+ * cover all 32 banks. */
+template <class R>
+struct VRec {
+    static constexpr int W = sizeof(R) == 4 ? 4 : 8;       /* words per vertex */
+    static constexpr int LINK = sizeof(R) == 4 ? 3 : 6;    /* word of the link */
+    static constexpr int STRIDE = W * FBLOCK;              /* words between a lane's slots */
+    static constexpr int WORDS = NVF * STRIDE;             /* words of a wavefront's polyhedra */
+};
+/* FP32 layout: words between a lane's consecutive vertex slots */
+constexpr int VSTRIDE = VRec<float>::STRIDE;
+
+/* Tets whose index box spans more than WIDE_CELLS cells along an axis go
+ * through the FP64 column path under the automatic kernel choice: measured
+ * on the host build, FP32 second moments stay within 1e-3 of the FP64
+ * oracle on every heavy cell for tets up to about this extent. */
+constexpr double WIDE_CELLS = 4.0;
 
 /* Link word layout: neighbour k in bits [5k, 5k+5); the fan apex of the
  * face to the left of the directed edge (v -> neighbour k) in bits
@@ -66,9 +88,10 @@ PVX_HD uint32_t f2u(float f) {
     return u;
 }
 
-/* 1 / x in FP32 with the hardware reciprocal on the GPU (about 1 ulp, where
- * a correctly rounded division costs some ten instructions; the FP32 path's
- * tolerance is far coarser), an exact division on the host */
+/* 1 / x: in FP32 the hardware reciprocal on the GPU (about 1 ulp, where a
+ * correctly rounded division costs some ten instructions; the FP32 path's
+ * tolerance is far coarser), an exact division on the host; in FP64 the
+ * division */
 PVX_HD float frcp(float x) {
 #ifdef __HIP_DEVICE_COMPILE__
     return __builtin_amdgcn_rcpf(x);
@@ -76,6 +99,19 @@ PVX_HD float frcp(float x) {
     return 1.0f / x;
 #endif
 }
+PVX_HD double frcp(double x) { return 1.0 / x; }
+
+/* min, max, |x|, floor, ceil in the argument's precision */
+PVX_HD float rmin(float a, float b) { return fminf(a, b); }
+PVX_HD double rmin(double a, double b) { return fmin(a, b); }
+PVX_HD float rmax(float a, float b) { return fmaxf(a, b); }
+PVX_HD double rmax(double a, double b) { return fmax(a, b); }
+PVX_HD float rabs(float a) { return fabsf(a); }
+PVX_HD double rabs(double a) { return fabs(a); }
+PVX_HD float rfloor(float a) { return floorf(a); }
+PVX_HD double rfloor(double a) { return floor(a); }
+PVX_HD float rceil(float a) { return ceilf(a); }
+PVX_HD double rceil(double a) { return ceil(a); }
 
 /* Keeps a value in a register from here on.  Values derived from the tet
  * record are otherwise re-read from memory where the compiler likes (the
@@ -97,35 +133,49 @@ PVX_HD void pin(double& x) {
 #endif
 }
 
-/* A lane's polyhedron: vertex v at w[v * VSTRIDE .. + 4) */
+/* A lane's polyhedron: vertex v at w[v * STRIDE .. + W) */
+template <class R>
 struct LPoly {
+    using L = VRec<R>;
     uint32_t* w;
     uint32_t live; /* bit v: slot v holds a vertex */
     int nv;        /* vertices */
     uint64_t ord;  /* their slots in order, 4 bits each (rank r in bits [4r, 4r+4)) */
     PVX_HD int at(int r) const { return (int)((ord >> (4 * r)) & 15u); }
-    PVX_HD float qa(int v, int a) const { return u2f(w[v * VSTRIDE + a]); }
-    PVX_HD void setq(int v, float a, float b, float c) {
-        w[v * VSTRIDE] = f2u(a);
-        w[v * VSTRIDE + 1] = f2u(b);
-        w[v * VSTRIDE + 2] = f2u(c);
+    PVX_HD R qa(int v, int a) const {
+        if (sizeof(R) == 4) return (R)u2f(w[v * L::STRIDE + a]);
+        R x;
+        memcpy(&x, &w[v * L::STRIDE + 2 * a], sizeof(R));
+        return x;
     }
-    PVX_HD int nbr(int v, int k) const { return nbf(w[v * VSTRIDE + 3], k); }
-    PVX_HD uint32_t nbw(int v) const { return w[v * VSTRIDE + 3]; }
-    PVX_HD void setnbw(int v, uint32_t x) { w[v * VSTRIDE + 3] = x; }
+    PVX_HD void setq(int v, R a, R b, R c) {
+        if (sizeof(R) == 4) {
+            w[v * L::STRIDE] = f2u((float)a);
+            w[v * L::STRIDE + 1] = f2u((float)b);
+            w[v * L::STRIDE + 2] = f2u((float)c);
+        } else {
+            memcpy(&w[v * L::STRIDE], &a, sizeof(R));
+            memcpy(&w[v * L::STRIDE + 2], &b, sizeof(R));
+            memcpy(&w[v * L::STRIDE + 4], &c, sizeof(R));
+        }
+    }
+    PVX_HD int nbr(int v, int k) const { return nbf(w[v * L::STRIDE + L::LINK], k); }
+    PVX_HD uint32_t nbw(int v) const { return w[v * L::STRIDE + L::LINK]; }
+    PVX_HD void setnbw(int v, uint32_t x) { w[v * L::STRIDE + L::LINK] = x; }
     PVX_HD void setnbr(int v, int k, int val) {
-        uint32_t x = w[v * VSTRIDE + 3];
+        uint32_t x = w[v * L::STRIDE + L::LINK];
         x = (x & ~(31u << (5 * k))) | ((uint32_t)val << (5 * k));
-        w[v * VSTRIDE + 3] = x;
+        w[v * L::STRIDE + L::LINK] = x;
     }
 };
 
 /* The copy being processed: geometry in grid units relative to its index
- * box corner in FP32; the FP64 tie-break re-reads the tet record. */
+ * box corner in R; the FP64 tie-break re-reads the tet record. */
+template <class R>
 struct CopyGeo {
-    float x0[3];
+    R x0[3];
     double x0d[3]; /* x0 in FP64: plane offsets c - x0 are formed in FP64 */
-    float DX[3][3]; /* DX[a][b] = x_{b+1,a} - x_{0,a} */
+    R DX[3][3];    /* DX[a][b] = x_{b+1,a} - x_{0,a} */
     const double* rec;
     double shift[3];
     int ib0[3];
@@ -133,16 +183,18 @@ struct CopyGeo {
 
 /* one axis of the copy's affine map, x_ax(q) = o + d0 q0 + d1 q1 + d2 q2,
  * plus what the FP64 tie-break needs */
+template <class R>
 struct AxisMap {
-    float o, d0, d1, d2;
+    R o, d0, d1, d2;
     double od;
     int ax;
     double shift, w0, dinv;
     int ib0;
 };
 
-PVX_HD AxisMap axis_map(const CopyGeo& G, const GridInfo& g, int ax) {
-    AxisMap m;
+template <class R>
+PVX_HD AxisMap<R> axis_map(const CopyGeo<R>& G, const GridInfo& g, int ax) {
+    AxisMap<R> m;
     m.ax = ax;
     m.o = pick3(G.x0, ax);
     m.od = pick3(G.x0d, ax);
@@ -158,7 +210,8 @@ PVX_HD AxisMap axis_map(const CopyGeo& G, const GridInfo& g, int ax) {
 
 /* FP64 c - x_ax(q) from the record: positions shifted by the copy's
  * periodic offset, in grid units relative to the index box corner */
-PVX_HD double sdist64(const double* r, const AxisMap& m, double c, float q0, float q1, float q2) {
+template <class R>
+PVX_HD double sdist64(const double* r, const AxisMap<R>& m, double c, R q0, R q1, R q2) {
     const double ra = m.ax == 0 ? r[0] : (m.ax == 1 ? r[1] : r[2]);
     const double rb = m.ax == 0 ? r[3] : (m.ax == 1 ? r[4] : r[5]);
     const double rc = m.ax == 0 ? r[6] : (m.ax == 1 ? r[7] : r[8]);
@@ -170,33 +223,30 @@ PVX_HD double sdist64(const double* r, const AxisMap& m, double c, float q0, flo
 /* signed distance c - x_ax(q) in cells as off - DX_ax q with off = c - x0
  * formed in FP64 (the plane crosses the tet, so |off| is at most the tet's
  * extent and its rounding scales with the tet, not with the cell index);
- * re-evaluated in FP64 from the record within 1e-5 cells of the plane */
-PVX_HD float sdist(const double* rec, const AxisMap& m, double c, float off, float q0, float q1, float q2) {
-    float s = off - (m.d0 * q0 + m.d1 * q1 + m.d2 * q2);
-    if (fabsf(s) < 1e-5f) {
+ * in FP32 re-evaluated in FP64 from the record within 1e-5 cells of the
+ * plane */
+template <class R>
+PVX_HD R sdist(const double* rec, const AxisMap<R>& m, double c, R off, R q0, R q1, R q2) {
+    R s = off - (m.d0 * q0 + m.d1 * q1 + m.d2 * q2);
+    if (sizeof(R) == 4 && rabs(s) < R(1e-5)) {
         double sd = sdist64(rec, m, c, q0, q1, q2);
-        s = (sd < 0.0) ? fminf(-1e-30f, (float)sd) : fmaxf(0.0f, (float)sd);
+        s = (sd < 0.0) ? rmin(R(-1e-30), (R)sd) : rmax(R(0), (R)sd);
     }
     return s;
 }
 
 /*
- * Clip P in place to one side of the plane x_ax = c.  keep_upper: keep
- * x > c (drop x <= c); otherwise keep x <= c.  New vertices go into free
- * slots (the graph is the reference's, src/geometry.c:391-501) and are
- * linked by the face walk; the slots of dropped vertices are freed, nothing
- * is renumbered.  Only registers and the lane's LDS slice are used.  Returns
- * false on slot overflow or a broken graph.
- */
-/* The graph part of a clip: drop the vertices in gone (a non-empty proper
+ * The graph part of a clip: drop the vertices in gone (a non-empty proper
  * subset of P's), put a new vertex into a free slot on each edge from a
  * survivor vc to a dropped vn (place(vx, vc, vn) sets its q), link the new
  * vertices around their faces, and keep the order: survivors in their order,
  * then the new vertices as they are made (the order a compacted graph has;
- * the faces' fan apexes and the moment sums follow it).  False on slot
- * overflow or a broken graph. */
-template <class Place>
-PVX_HD bool clip_graph(LPoly& P, uint32_t gone, Place&& place) {
+ * the faces' fan apexes and the moment sums follow it).  The graph is the
+ * reference's (src/geometry.c:391-501); nothing is renumbered, the slots of
+ * dropped vertices are freed.  False on slot overflow or a broken graph.
+ */
+template <class R, class Place>
+PVX_HD bool clip_graph(LPoly<R>& P, uint32_t gone, Place&& place) {
     const uint32_t live = P.live;
     uint64_t nord = 0;
     int nk = 0;
@@ -261,23 +311,28 @@ PVX_HD bool clip_graph(LPoly& P, uint32_t gone, Place&& place) {
 
 /* the point of the edge vc -> vn on the plane, from the two vertices'
  * signed distances sc, sn (of opposite signs) */
-PVX_HD void cut_point(LPoly& P, int vx, int vc, int vn, float sc, float sn) {
-    const float c0 = P.qa(vc, 0), c1 = P.qa(vc, 1), c2 = P.qa(vc, 2);
-    const float n0 = P.qa(vn, 0), n1 = P.qa(vn, 1), n2 = P.qa(vn, 2);
-    const float inv = frcp(sc - sn);
+template <class R>
+PVX_HD void cut_point(LPoly<R>& P, int vx, int vc, int vn, R sc, R sn) {
+    const R c0 = P.qa(vc, 0), c1 = P.qa(vc, 1), c2 = P.qa(vc, 2);
+    const R n0 = P.qa(vn, 0), n1 = P.qa(vn, 1), n2 = P.qa(vn, 2);
+    const R inv = frcp(sc - sn);
     P.setq(vx, inv * (n0 * sc - c0 * sn), inv * (n1 * sc - c1 * sn), inv * (n2 * sc - c2 * sn));
 }
 
-PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int ax, bool keep_upper) {
+/* Clip P in place to one side of the plane x_ax = c.  keep_upper: keep
+ * x > c (drop x <= c); otherwise keep x <= c.  Only registers and the
+ * lane's LDS slice are used.  False on slot overflow or a broken graph. */
+template <class R>
+PVX_HD bool clip_f(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g, double c, int ax, bool keep_upper) {
     const uint32_t live = P.live;
     if (live == 0u) return true;
-    const AxisMap m = axis_map(G, g, ax);
-    const float off = (float)(c - m.od);
+    const AxisMap<R> m = axis_map(G, g, ax);
+    const R off = (R)(c - m.od);
     uint32_t gone = 0;
     for (uint32_t lm = live; lm; lm &= lm - 1u) {
         const int v = __builtin_ctz(lm);
-        const float t = sdist(G.rec, m, c, off, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
-        const bool drop = keep_upper ? (t >= 0.0f) : (t < 0.0f);
+        const R t = sdist(G.rec, m, c, off, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
+        const bool drop = keep_upper ? (t >= R(0)) : (t < R(0));
         if (drop) gone |= 1u << v;
     }
     if (gone == 0u) return true;
@@ -287,8 +342,8 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int 
         return true;
     }
     return clip_graph(P, gone, [&](int vx, int vc, int vn) {
-        const float sc = sdist(G.rec, m, c, off, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
-        const float sn = sdist(G.rec, m, c, off, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
+        const R sc = sdist(G.rec, m, c, off, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
+        const R sn = sdist(G.rec, m, c, off, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
         cut_point(P, vx, vc, vn, sc, sn);
     });
 }
@@ -296,21 +351,22 @@ PVX_HD bool clip_f(LPoly& P, const CopyGeo& G, const GridInfo& g, double c, int 
 /* q-moments accumulator about an origin: m0 = 6 vol, m1 = sum d S / 4,
  * M2 = sum d (AA^T + BB^T + CC^T + SS^T) / 20 with S = A+B+C, over signed
  * tets (origin, A, B, C); same integrals as src/geometry.c:725-740 */
+template <class R>
 struct Mom {
-    float m0, s1[3], s2[6];
+    R m0, s1[3], s2[6];
     PVX_HD void zero() {
-        m0 = 0.f;
-        for (int i = 0; i < 3; ++i) s1[i] = 0.f;
-        for (int i = 0; i < 6; ++i) s2[i] = 0.f;
+        m0 = R(0);
+        for (int i = 0; i < 3; ++i) s1[i] = R(0);
+        for (int i = 0; i < 6; ++i) s2[i] = R(0);
     }
 };
 
-template <int ORDER>
-PVX_HD void tri_acc(Mom& M, const float* A, const float* B, const float* C) {
-    float d = A[0] * (B[1] * C[2] - B[2] * C[1]) - A[1] * (B[0] * C[2] - B[2] * C[0]) + A[2] * (B[0] * C[1] - B[1] * C[0]);
+template <int ORDER, class R>
+PVX_HD void tri_acc(Mom<R>& M, const R* A, const R* B, const R* C) {
+    R d = A[0] * (B[1] * C[2] - B[2] * C[1]) - A[1] * (B[0] * C[2] - B[2] * C[0]) + A[2] * (B[0] * C[1] - B[1] * C[0]);
     M.m0 += d;
     if (ORDER > 0) {
-        float S0 = A[0] + B[0] + C[0], S1 = A[1] + B[1] + C[1], S2 = A[2] + B[2] + C[2];
+        R S0 = A[0] + B[0] + C[0], S1 = A[1] + B[1] + C[1], S2 = A[2] + B[2] + C[2];
         M.s1[0] += d * S0;
         M.s1[1] += d * S1;
         M.s1[2] += d * S2;
@@ -331,7 +387,8 @@ PVX_HD void tri_acc(Mom& M, const float* A, const float* B, const float* C) {
  * walk's first vertex, so a face (v0, v1, ..., vm) is fanned into triangles
  * (v0, vi, vi+1) exactly as the reference's reduce fans it.
  */
-PVX_HD void face_apexes(LPoly& P) {
+template <class R>
+PVX_HD void face_apexes(LPoly<R>& P) {
     uint64_t mark = 0;
     for (int r = 0; r < P.nv; ++r) {
         const int vs = P.at(r);
@@ -362,13 +419,13 @@ PVX_HD void face_apexes(LPoly& P) {
 
 /* tri_acc with the signed volume scaled by w (0 or 1: a term that is
  * absent contributes nothing without a branch) */
-template <int ORDER>
-PVX_HD void tri_acc_w(Mom& M, const float* A, const float* B, const float* C, float w) {
-    float d = A[0] * (B[1] * C[2] - B[2] * C[1]) - A[1] * (B[0] * C[2] - B[2] * C[0]) + A[2] * (B[0] * C[1] - B[1] * C[0]);
+template <int ORDER, class R>
+PVX_HD void tri_acc_w(Mom<R>& M, const R* A, const R* B, const R* C, R w) {
+    R d = A[0] * (B[1] * C[2] - B[2] * C[1]) - A[1] * (B[0] * C[2] - B[2] * C[0]) + A[2] * (B[0] * C[1] - B[1] * C[0]);
     d *= w;
     M.m0 += d;
     if (ORDER > 0) {
-        const float S0 = A[0] + B[0] + C[0], S1 = A[1] + B[1] + C[1], S2 = A[2] + B[2] + C[2];
+        const R S0 = A[0] + B[0] + C[0], S1 = A[1] + B[1] + C[1], S2 = A[2] + B[2] + C[2];
         M.s1[0] += d * S0;
         M.s1[1] += d * S1;
         M.s1[2] += d * S2;
@@ -383,12 +440,12 @@ PVX_HD void tri_acc_w(Mom& M, const float* A, const float* B, const float* C, fl
     }
 }
 
-
 /* A point of P on the plane z = p (z = zd . q): on the first edge, in rank
  * order then neighbour order, from a vertex at or below p to one above it;
  * le has bit v set when z(v) <= p, so only that edge's vertices are read.
  * False when no edge crosses (P lies on one side). */
-PVX_HD bool plane_point_le(const LPoly& P, float zd0, float zd1, float zd2, float p, uint32_t le, float* out) {
+template <class R>
+PVX_HD bool plane_point_le(const LPoly<R>& P, R zd0, R zd1, R zd2, R p, uint32_t le, R* out) {
     for (int r = 0; r < P.nv; ++r) {
         const int v = P.at(r);
         if (!((le >> v) & 1u)) continue;
@@ -396,11 +453,11 @@ PVX_HD bool plane_point_le(const LPoly& P, float zd0, float zd1, float zd2, floa
         for (int k = 0; k < 3; ++k) {
             const int b = nbf(w, k);
             if ((le >> b) & 1u) continue;
-            const float q0 = P.qa(v, 0), q1 = P.qa(v, 1), q2 = P.qa(v, 2);
-            const float zv = zd0 * q0 + zd1 * q1 + zd2 * q2;
-            const float r0 = P.qa(b, 0), r1 = P.qa(b, 1), r2 = P.qa(b, 2);
-            const float zb = zd0 * r0 + zd1 * r1 + zd2 * r2;
-            const float t = (p - zv) * frcp(zb - zv);
+            const R q0 = P.qa(v, 0), q1 = P.qa(v, 1), q2 = P.qa(v, 2);
+            const R zv = zd0 * q0 + zd1 * q1 + zd2 * q2;
+            const R r0 = P.qa(b, 0), r1 = P.qa(b, 1), r2 = P.qa(b, 2);
+            const R zb = zd0 * r0 + zd1 * r1 + zd2 * r2;
+            const R t = (p - zv) * frcp(zb - zv);
             out[0] = q0 + t * (r0 - q0);
             out[1] = q1 + t * (r1 - q1);
             out[2] = q2 + t * (r2 - q2);
@@ -414,19 +471,19 @@ PVX_HD bool plane_point_le(const LPoly& P, float zd0, float zd1, float zd2, floa
  * on the plane z = p; inside(z) is the kept side.  X: where the boundary
  * leaves the kept side, N: where it re-enters; valid when both exist.
  * Invalid segments come back as (c, c). */
-template <bool UPPER_KEPT_IS_LOWER>
-PVX_HD bool tri_segment(const float* qc, const float* qa, const float* qb, float zc, float za, float zb, float p,
-                        float rca, float rab, float rbc, float* X, float* N) {
+template <bool UPPER_KEPT_IS_LOWER, class R>
+PVX_HD bool tri_segment(const R* qc, const R* qa, const R* qb, R zc, R za, R zb, R p, R rca, R rab, R rbc, R* X,
+                        R* N) {
     /* kept: z > p (lower plane of a slab) or z <= p (upper plane) */
     const bool ic = UPPER_KEPT_IS_LOWER ? (zc <= p) : (zc > p);
     const bool ia = UPPER_KEPT_IS_LOWER ? (za <= p) : (za > p);
     const bool ib = UPPER_KEPT_IS_LOWER ? (zb <= p) : (zb > p);
     const bool valid = (ic != ia) || (ia != ib);
     /* crossing parameters of the three edges (only used where they cross) */
-    const float tca = (za != zc) ? (p - zc) * rca : 0.f;
-    const float tab = (zb != za) ? (p - za) * rab : 0.f;
-    const float tbc = (zc != zb) ? (p - zb) * rbc : 0.f;
-    float Pca[3], Pab[3], Pbc[3];
+    const R tca = (za != zc) ? (p - zc) * rca : R(0);
+    const R tab = (zb != za) ? (p - za) * rab : R(0);
+    const R tbc = (zc != zb) ? (p - zb) * rbc : R(0);
+    R Pca[3], Pab[3], Pbc[3];
     for (int i = 0; i < 3; ++i) {
         Pca[i] = qc[i] + tca * (qa[i] - qc[i]);
         Pab[i] = qa[i] + tab * (qb[i] - qa[i]);
@@ -435,8 +492,8 @@ PVX_HD bool tri_segment(const float* qc, const float* qa, const float* qb, float
     const bool xca = ic && !ia, xab = ia && !ib;
     const bool nca = !ic && ia, nab = !ia && ib;
     for (int i = 0; i < 3; ++i) {
-        const float x = xca ? Pca[i] : (xab ? Pab[i] : Pbc[i]);
-        const float n = nca ? Pca[i] : (nab ? Pab[i] : Pbc[i]);
+        const R x = xca ? Pca[i] : (xab ? Pab[i] : Pbc[i]);
+        const R n = nca ? Pca[i] : (nab ? Pab[i] : Pbc[i]);
         X[i] = valid ? x : qc[i];
         N[i] = valid ? n : qc[i];
     }
@@ -445,16 +502,17 @@ PVX_HD bool tri_segment(const float* qc, const float* qa, const float* qb, float
 
 /* the piece of the edge u -> v inside lo < z <= hi as parameters t0 < t1
  * (valid) of u + t (v - u); inv = 1 / (zv - zu) (read only when zv != zu) */
-PVX_HD bool edge_piece(float zu, float zv, float inv, float lo, float hi, float& t0, float& t1) {
-    const float dz = zv - zu;
-    if (dz != 0.f) {
-        const float tl = (lo - zu) * inv, th = (hi - zu) * inv;
-        t0 = fmaxf(0.f, fminf(tl, th));
-        t1 = fminf(1.f, fmaxf(tl, th));
+template <class R>
+PVX_HD bool edge_piece(R zu, R zv, R inv, R lo, R hi, R& t0, R& t1) {
+    const R dz = zv - zu;
+    if (dz != R(0)) {
+        const R tl = (lo - zu) * inv, th = (hi - zu) * inv;
+        t0 = rmax(R(0), rmin(tl, th));
+        t1 = rmin(R(1), rmax(tl, th));
     } else {
         const bool in = zu > lo && zu <= hi;
-        t0 = in ? 0.f : 1.f;
-        t1 = in ? 1.f : 0.f;
+        t0 = in ? R(0) : R(1);
+        t1 = in ? R(1) : R(0);
     }
     return t0 < t1;
 }
@@ -463,10 +521,11 @@ PVX_HD bool edge_piece(float zu, float zv, float inv, float lo, float hi, float&
  * 0 along the sweep axis, the moment origin O (on the lower plane, on the
  * upper one when there is no lower plane), the upper cap's anchor
  * relative to O, and the fan triangles reaching into the slab. */
+template <class R>
 struct SlabCell {
-    float lo, hi; /* -3e38 / +3e38 when the side is open */
-    float O[3], gh[3];
-    float zd[3]; /* x_ax(q) - x_ax(q = 0) = zd . q */
+    R lo, hi; /* -3e38 / +3e38 when the side is open */
+    R O[3], gh[3];
+    R zd[3]; /* x_ax(q) - x_ax(q = 0) = zd . q */
     bool has_lo, has_hi;
     uint64_t mask;   /* bit 3r + k: the fan triangle of directed edge a -> k, a the vertex of rank r,
                         reaching into the slab and cut by one of its planes */
@@ -475,12 +534,13 @@ struct SlabCell {
 
 /* The point plane_point_le found on a cell's upper plane, kept for the
  * next cell of the column, whose lower plane is the same plane (the same
- * FP32 offset and the same vertex sides, so the search would find the same
+ * offset and the same vertex sides, so the search would find the same
  * point): each interior plane is searched once. */
+template <class R>
 struct PlaneCarry {
     bool valid; /* the last cell had an upper plane */
     bool found; /* and an edge crossed it at p */
-    float p[3];
+    R p[3];
 };
 
 /*
@@ -491,16 +551,17 @@ struct PlaneCarry {
  * the lower plane's point from the column's previous cell and takes the
  * upper plane's for the next.
  */
-PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi, double hi_c,
-                       SlabCell& S, PlaneCarry* pc = nullptr) {
+template <class R>
+PVX_HD void slab_setup(const LPoly<R>& P, const CopyGeo<R>& G, int ax, bool has_lo, double lo_c, bool has_hi,
+                       double hi_c, SlabCell<R>& S, PlaneCarry<R>* pc = nullptr) {
     const double x0d = pick3(G.x0d, ax);
     S.has_lo = has_lo;
     S.has_hi = has_hi;
-    S.lo = has_lo ? (float)(lo_c - x0d) : -3.0e38f;
-    S.hi = has_hi ? (float)(hi_c - x0d) : 3.0e38f;
-    const float zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
-                zd1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]),
-                zd2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
+    S.lo = has_lo ? (R)(lo_c - x0d) : R(-3.0e38);
+    S.hi = has_hi ? (R)(hi_c - x0d) : R(3.0e38);
+    const R zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
+            zd1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]),
+            zd2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
     S.zd[0] = zd0;
     S.zd[1] = zd1;
     S.zd[2] = zd2;
@@ -509,13 +570,13 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
     uint32_t above = 0u, below = 0u;
     for (uint32_t lm = P.live; lm; lm &= lm - 1u) {
         const int v = __builtin_ctz(lm);
-        const float z = zd0 * P.qa(v, 0) + zd1 * P.qa(v, 1) + zd2 * P.qa(v, 2);
+        const R z = zd0 * P.qa(v, 0) + zd1 * P.qa(v, 1) + zd2 * P.qa(v, 2);
         above |= (z > S.lo ? 1u : 0u) << v;
         below |= (z <= S.hi ? 1u : 0u) << v;
     }
     const int v0 = P.at(0);
-    float O[3] = {P.qa(v0, 0), P.qa(v0, 1), P.qa(v0, 2)};
-    float Gh[3] = {O[0], O[1], O[2]};
+    R O[3] = {P.qa(v0, 0), P.qa(v0, 1), P.qa(v0, 2)};
+    R Gh[3] = {O[0], O[1], O[2]};
     bool fh = false; /* a point on the upper plane was found */
     if (has_lo) {
         if (pc && pc->valid) {
@@ -575,54 +636,54 @@ PVX_HD void slab_setup(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, do
  * same segments; the cap on the lower plane adds nothing (O lies on it).
  * Five fixed terms, each weighted 0 or 1.
  */
-template <int ORDER>
-PVX_HD void pair_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
+template <int ORDER, class R>
+PVX_HD void pair_terms(const LPoly<R>& P, int e, const SlabCell<R>& S, Mom<R>& M) {
     const int ra = e / 3, k = e - 3 * ra, a = P.at(ra);
-    const float* O = S.O;
-    const float zd0 = S.zd[0], zd1 = S.zd[1], zd2 = S.zd[2];
-    const float lo = S.lo, hi = S.hi;
+    const R* O = S.O;
+    const R zd0 = S.zd[0], zd1 = S.zd[1], zd2 = S.zd[2];
+    const R lo = S.lo, hi = S.hi;
     const uint32_t wa = P.nbw(a);
-    const float pa0 = P.qa(a, 0), pa1 = P.qa(a, 1), pa2 = P.qa(a, 2);
-    const float qa[3] = {pa0 - O[0], pa1 - O[1], pa2 - O[2]};
-    const float za = zd0 * pa0 + zd1 * pa1 + zd2 * pa2;
+    const R pa0 = P.qa(a, 0), pa1 = P.qa(a, 1), pa2 = P.qa(a, 2);
+    const R qa[3] = {pa0 - O[0], pa1 - O[1], pa2 - O[2]};
+    const R za = zd0 * pa0 + zd1 * pa1 + zd2 * pa2;
     const int b = nbf(wa, k), c = apf(wa, k);
-    const float pb0 = P.qa(b, 0), pb1 = P.qa(b, 1), pb2 = P.qa(b, 2);
-    const float pc0 = P.qa(c, 0), pc1 = P.qa(c, 1), pc2 = P.qa(c, 2);
-    const float qb[3] = {pb0 - O[0], pb1 - O[1], pb2 - O[2]};
-    const float qc[3] = {pc0 - O[0], pc1 - O[1], pc2 - O[2]};
-    const float zb = zd0 * pb0 + zd1 * pb1 + zd2 * pb2;
-    const float zc = zd0 * pc0 + zd1 * pc1 + zd2 * pc2;
+    const R pb0 = P.qa(b, 0), pb1 = P.qa(b, 1), pb2 = P.qa(b, 2);
+    const R pc0 = P.qa(c, 0), pc1 = P.qa(c, 1), pc2 = P.qa(c, 2);
+    const R qb[3] = {pb0 - O[0], pb1 - O[1], pb2 - O[2]};
+    const R qc[3] = {pc0 - O[0], pc1 - O[1], pc2 - O[2]};
+    const R zb = zd0 * pb0 + zd1 * pb1 + zd2 * pb2;
+    const R zc = zd0 * pc0 + zd1 * pc1 + zd2 * pc2;
     /* the edges' reciprocal z-extents, shared by the edge pieces and the
      * plane segments */
-    const float rca = za != zc ? frcp(za - zc) : 0.f, rab = zb != za ? frcp(zb - za) : 0.f,
-                rbc = zc != zb ? frcp(zc - zb) : 0.f;
+    const R rca = za != zc ? frcp(za - zc) : R(0), rab = zb != za ? frcp(zb - za) : R(0),
+            rbc = zc != zb ? frcp(zc - zb) : R(0);
     /* the edge pieces inside the slab, in traversal order */
-    float s0, s1, u0, u1, v0, v1;
+    R s0, s1, u0, u1, v0, v1;
     const bool eca = edge_piece(zc, za, rca, lo, hi, s0, s1);
     const bool eab = edge_piece(za, zb, rab, lo, hi, u0, u1);
     const bool ebc = edge_piece(zb, zc, rbc, lo, hi, v0, v1);
     (void)u1;
-    float A1[3], B1[3], A3[3], B3[3], Gp[3];
+    R A1[3], B1[3], A3[3], B3[3], Gp[3];
     for (int i = 0; i < 3; ++i) {
         A1[i] = qc[i] + s0 * (qa[i] - qc[i]);
         B1[i] = qc[i] + s1 * (qa[i] - qc[i]);
-        const float A2 = qa[i] + u0 * (qb[i] - qa[i]);
+        const R A2 = qa[i] + u0 * (qb[i] - qa[i]);
         A3[i] = qb[i] + v0 * (qc[i] - qb[i]);
         B3[i] = qb[i] + v1 * (qc[i] - qb[i]);
         Gp[i] = eab ? A2 : (eca ? A1[i] : (ebc ? A3[i] : qc[i]));
     }
-    tri_acc_w<ORDER>(M, Gp, B1, A1, eca ? 1.f : 0.f);
-    tri_acc_w<ORDER>(M, Gp, B3, A3, ebc ? 1.f : 0.f);
+    tri_acc_w<ORDER>(M, Gp, B1, A1, eca ? R(1) : R(0));
+    tri_acc_w<ORDER>(M, Gp, B3, A3, ebc ? R(1) : R(0));
     if (S.has_lo) {
-        float X[3], N[3];
+        R X[3], N[3];
         const bool v = tri_segment<false>(qc, qa, qb, zc, za, zb, lo, rca, rab, rbc, X, N);
-        tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
+        tri_acc_w<ORDER>(M, Gp, N, X, v ? R(1) : R(0));
     }
     if (S.has_hi) {
-        float X[3], N[3];
+        R X[3], N[3];
         const bool v = tri_segment<true>(qc, qa, qb, zc, za, zb, hi, rca, rab, rbc, X, N);
-        tri_acc_w<ORDER>(M, Gp, N, X, v ? 1.f : 0.f);
-        if (S.has_lo) tri_acc_w<ORDER>(M, S.gh, X, N, v ? 1.f : 0.f);
+        tri_acc_w<ORDER>(M, Gp, N, X, v ? R(1) : R(0));
+        if (S.has_lo) tri_acc_w<ORDER>(M, S.gh, X, N, v ? R(1) : R(0));
     }
 }
 
@@ -630,24 +691,24 @@ PVX_HD void pair_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
  * slab: the cone over T from O, the one term of pair_terms that does not
  * vanish there (its edge pieces are the whole edges, it has no segment on
  * either plane). */
-template <int ORDER>
-PVX_HD void inside_terms(const LPoly& P, int e, const SlabCell& S, Mom& M) {
+template <int ORDER, class R>
+PVX_HD void inside_terms(const LPoly<R>& P, int e, const SlabCell<R>& S, Mom<R>& M) {
     const int ra = e / 3, k = e - 3 * ra, a = P.at(ra);
     const uint32_t wa = P.nbw(a);
     const int b = nbf(wa, k), c = apf(wa, k);
-    const float* O = S.O;
-    const float qa[3] = {P.qa(a, 0) - O[0], P.qa(a, 1) - O[1], P.qa(a, 2) - O[2]};
-    const float qb[3] = {P.qa(b, 0) - O[0], P.qa(b, 1) - O[1], P.qa(b, 2) - O[2]};
-    const float qc[3] = {P.qa(c, 0) - O[0], P.qa(c, 1) - O[1], P.qa(c, 2) - O[2]};
+    const R* O = S.O;
+    const R qa[3] = {P.qa(a, 0) - O[0], P.qa(a, 1) - O[1], P.qa(a, 2) - O[2]};
+    const R qb[3] = {P.qa(b, 0) - O[0], P.qa(b, 1) - O[1], P.qa(b, 2) - O[2]};
+    const R qc[3] = {P.qa(c, 0) - O[0], P.qa(c, 1) - O[1], P.qa(c, 2) - O[2]};
     tri_acc<ORDER>(M, qa, qc, qb);
 }
 
 /* moments of P's piece in the slab, one lane doing every fan triangle */
-template <int ORDER>
-PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo, double lo_c, bool has_hi,
-                           double hi_c, Mom& M, float* O, PlaneCarry* pc = nullptr) {
+template <int ORDER, class R>
+PVX_HD void slab_moments_u(const LPoly<R>& P, const CopyGeo<R>& G, int ax, bool has_lo, double lo_c, bool has_hi,
+                           double hi_c, Mom<R>& M, R* O, PlaneCarry<R>* pc = nullptr) {
     M.zero();
-    SlabCell S;
+    SlabCell<R> S;
     slab_setup(P, G, ax, has_lo, lo_c, has_hi, hi_c, S, pc);
     O[0] = S.O[0];
     O[1] = S.O[1];
@@ -668,19 +729,20 @@ PVX_HD void slab_moments_u(const LPoly& P, const CopyGeo& G, int ax, bool has_lo
 
 /* index range of the x_ax-extent of a polyhedron inside [plo, phi), with
  * the zero-extent rule of the FP64 path (one cell) */
-PVX_HD void lpoly_range(const LPoly& P, const CopyGeo& G, int ax, int ib0, int plo, int phi, int n, int& lo,
+template <class R>
+PVX_HD void lpoly_range(const LPoly<R>& P, const CopyGeo<R>& G, int ax, int ib0, int plo, int phi, int n, int& lo,
                         int& hi) {
-    float zmin = 3.0e38f, zmax = -3.0e38f;
+    R zmin = R(3.0e38), zmax = R(-3.0e38);
     for (uint32_t lm = P.live; lm; lm &= lm - 1u) {
         const int v = __builtin_ctz(lm);
-        float z = G.x0[ax] + G.DX[ax][0] * P.qa(v, 0) + G.DX[ax][1] * P.qa(v, 1) + G.DX[ax][2] * P.qa(v, 2);
-        zmin = fminf(zmin, z);
-        zmax = fmaxf(zmax, z);
+        R z = G.x0[ax] + G.DX[ax][0] * P.qa(v, 0) + G.DX[ax][1] * P.qa(v, 1) + G.DX[ax][2] * P.qa(v, 2);
+        zmin = rmin(zmin, z);
+        zmax = rmax(zmax, z);
     }
     /* local coordinates are relative to the copy's index box corner ib0 */
-    float fl = floorf(zmin), fh = ceilf(zmax);
-    lo = (fl < (float)(plo - ib0)) ? plo : ib0 + (int)fl;
-    hi = (fh > (float)(phi - ib0)) ? phi : ib0 + (int)fh;
+    R fl = rfloor(zmin), fh = rceil(zmax);
+    lo = (fl < (R)(plo - ib0)) ? plo : ib0 + (int)fl;
+    hi = (fh > (R)(phi - ib0)) ? phi : ib0 + (int)fh;
     if (lo < plo) lo = plo;
     if (hi > phi) hi = phi;
     if (hi <= lo) hi = lo + 1;
@@ -689,53 +751,53 @@ PVX_HD void lpoly_range(const LPoly& P, const CopyGeo& G, int ax, int ib0, int p
 
 /* y-extent of the tet restricted to the x-slab lo < X <= hi (local units):
  * tet vertices inside plus edge crossings of the two planes */
-PVX_HD void slab_yrange(const float (*X)[3], bool has_lo, float lo, bool has_hi, float hi, float& ymin, float& ymax) {
-    ymin = 3.0e38f;
-    ymax = -3.0e38f;
+template <class R>
+PVX_HD void slab_yrange(const R (*X)[3], bool has_lo, R lo, bool has_hi, R hi, R& ymin, R& ymax) {
+    ymin = R(3.0e38);
+    ymax = R(-3.0e38);
     /* the closed slab: a vertex on a plane bounds the piece next to it */
     for (int v = 0; v < 4; ++v) {
         bool in = (!has_lo || X[v][0] >= lo) && (!has_hi || X[v][0] <= hi);
         if (in) {
-            ymin = fminf(ymin, X[v][1]);
-            ymax = fmaxf(ymax, X[v][1]);
+            ymin = rmin(ymin, X[v][1]);
+            ymax = rmax(ymax, X[v][1]);
         }
     }
     for (int a = 0; a < 4; ++a)
         for (int b = a + 1; b < 4; ++b) {
-            float xa = X[a][0], xb = X[b][0];
+            R xa = X[a][0], xb = X[b][0];
             for (int pl = 0; pl < 2; ++pl) {
                 if (pl == 0 && !has_lo) continue;
                 if (pl == 1 && !has_hi) continue;
-                float c = pl == 0 ? lo : hi;
-                if ((xa - c) * (xb - c) < 0.0f) {
-                    float t = (c - xa) / (xb - xa);
-                    float y = X[a][1] + t * (X[b][1] - X[a][1]);
-                    ymin = fminf(ymin, y);
-                    ymax = fmaxf(ymax, y);
+                R c = pl == 0 ? lo : hi;
+                if ((xa - c) * (xb - c) < R(0)) {
+                    R t = (c - xa) / (xb - xa);
+                    R y = X[a][1] + t * (X[b][1] - X[a][1]);
+                    ymin = rmin(ymin, y);
+                    ymax = rmax(ymax, y);
                 }
             }
         }
 }
 
-/* per-fragment deposit from FP32 q-moments about origin O (FP64 sums);
- * FMASK: the field flags a caller can have (the others compile away) */
-template <int ORDER, int FMASK = -1, class Sink>
-PVX_HD void deposit_f(Sink& sink, const GridInfo& g, const CopyGeo& G, const int* ib0, int i, int j, int k,
-                      const Mom& Mm, const float* O, const double* v0, const float (*DV)[3], double mass,
-                      double wm0) {
+/* per-fragment deposit from q-moments about origin O (FP64 sums); FMASK:
+ * the field flags a caller can have (the others compile away) */
+template <int ORDER, int FMASK = -1, class R, class Sink>
+PVX_HD void deposit_f(Sink& sink, const GridInfo& g, const CopyGeo<R>& G, const int* ib0, int i, int j, int k,
+                      const Mom<R>& Mm, const R* O, const double* v0, const R (*DV)[3], double mass, double wm0) {
     /* weight: mass times the fragment's mass fraction wm0 (its m0, or m0
      * plus the tet's residual for the fragment that absorbs it) */
     const double w = mass * wm0;
     const int64_t gi = ((int64_t)g.n[1] * g.n[2] * i + (int64_t)g.n[2] * j + k) * g.nacc;
     sink.add(gi, w);
     if (ORDER == 0) return;
-    const float inv = frcp(Mm.m0);
+    const R inv = frcp(Mm.m0);
     /* mean q (origin shifted back) and q covariance about the mean */
-    const float mu[3] = {0.25f * Mm.s1[0] * inv, 0.25f * Mm.s1[1] * inv, 0.25f * Mm.s1[2] * inv};
-    const float qb[3] = {O[0] + mu[0], O[1] + mu[1], O[2] + mu[2]};
-    float cov[3][3];
+    const R mu[3] = {R(0.25) * Mm.s1[0] * inv, R(0.25) * Mm.s1[1] * inv, R(0.25) * Mm.s1[2] * inv};
+    const R qb[3] = {O[0] + mu[0], O[1] + mu[1], O[2] + mu[2]};
+    R cov[3][3];
     if (ORDER > 1) {
-        const float c20 = 0.05f * inv;
+        const R c20 = R(0.05) * inv;
         cov[0][0] = c20 * Mm.s2[0] - mu[0] * mu[0];
         cov[0][1] = cov[1][0] = c20 * Mm.s2[1] - mu[0] * mu[1];
         cov[0][2] = cov[2][0] = c20 * Mm.s2[2] - mu[0] * mu[2];
@@ -746,13 +808,13 @@ PVX_HD void deposit_f(Sink& sink, const GridInfo& g, const CopyGeo& G, const int
     const int fl = g.flags & FMASK;
     const int idx[3] = {i, j, k};
     double dx[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
-    float b_[3] = {0, 0, 0}; /* DV qbar */
+    R b_[3] = {R(0), R(0), R(0)}; /* DV qbar */
     if (fl & (F_X | F_XX | F_XV))
         for (int a = 0; a < 3; ++a) {
-            /* centroid minus cell centre: the FP32 offset from the cell centre
-             * is accurate to FP32 of a cell size */
-            const float dq = G.DX[a][0] * qb[0] + G.DX[a][1] * qb[1] + G.DX[a][2] * qb[2];
-            const float cl = (float)(G.x0d[a] - ((double)(idx[a] - ib0[a]) + 0.5)) + dq;
+            /* centroid minus cell centre: the offset from the cell centre is
+             * accurate to R of a cell size */
+            const R dq = G.DX[a][0] * qb[0] + G.DX[a][1] * qb[1] + G.DX[a][2] * qb[2];
+            const R cl = (R)(G.x0d[a] - ((double)(idx[a] - ib0[a]) + 0.5)) + dq;
             dx[a] = (double)cl * g.d[a];
         }
     if (fl & (F_V | F_VV | F_XV))
@@ -766,34 +828,34 @@ PVX_HD void deposit_f(Sink& sink, const GridInfo& g, const CopyGeo& G, const int
         for (int a = 0; a < 3; ++a) sink.add(gi + g.ov + a, w * cv[a]);
     if (ORDER < 2) return;
     /* covariances of x and v inside the fragment: J cov J^T */
-    float DXa[3][3];
+    R DXa[3][3];
     if (fl & (F_XX | F_XV))
         for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) DXa[a][b] = G.DX[a][b] * (float)g.d[a];
+            for (int b = 0; b < 3; ++b) DXa[a][b] = G.DX[a][b] * (R)g.d[a];
     if (fl & F_XX) {
-        float T[3][3];
+        R T[3][3];
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) T[a][b] = DXa[a][0] * cov[0][b] + DXa[a][1] * cov[1][b] + DXa[a][2] * cov[2][b];
         int s = 0;
         for (int a = 0; a < 3; ++a)
             for (int b = a; b < 3; ++b, ++s) {
-                float sxx = T[a][0] * DXa[b][0] + T[a][1] * DXa[b][1] + T[a][2] * DXa[b][2];
+                R sxx = T[a][0] * DXa[b][0] + T[a][1] * DXa[b][1] + T[a][2] * DXa[b][2];
                 sink.add(gi + g.oxx + s, w * ((double)sxx + dx[a] * dx[b]));
             }
     }
     if (fl & F_VV) {
-        float T[3][3];
+        R T[3][3];
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) T[a][b] = DV[a][0] * cov[0][b] + DV[a][1] * cov[1][b] + DV[a][2] * cov[2][b];
         int s = 0;
         for (int a = 0; a < 3; ++a)
             for (int b = a; b < 3; ++b, ++s) {
-                float svv = T[a][0] * DV[b][0] + T[a][1] * DV[b][1] + T[a][2] * DV[b][2];
+                R svv = T[a][0] * DV[b][0] + T[a][1] * DV[b][1] + T[a][2] * DV[b][2];
                 sink.add(gi + g.ovv + s, w * ((double)svv + cv[a] * cv[b]));
             }
     }
     if (fl & F_XV) {
-        float T[3][3];
+        R T[3][3];
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) T[a][b] = DXa[a][0] * cov[0][b] + DXa[a][1] * cov[1][b] + DXa[a][2] * cov[2][b];
         const bool fix = (fl & F_XVFIX) != 0;
@@ -826,20 +888,22 @@ struct FastCounters {
 /* column tasks                                                        */
 /* ------------------------------------------------------------------ */
 
-/* A copy of a tet prepared for column work: its FP32 cell-local geometry,
- * index box, window-plane mask and window planes in local units. */
+/* A copy of a tet prepared for column work: its cell-local geometry, index
+ * box, window-plane mask and window planes in local units. */
+template <class R>
 struct CopySetup {
-    CopyGeo G;
+    CopyGeo<R> G;
     int ib1[3];
     unsigned wmask; /* window planes crossed: bit 2a lower, 2a+1 upper */
     double wlo[3], whi[3]; /* window planes, local units */
-    float X[4][3];
+    R X[4][3];
 };
 
 /* copy h of the tet record (shift pattern from ghost_shifts_packed);
  * returns false when the copy's index box starts beyond the grid */
+template <class R>
 PVX_HD bool copy_setup(const double* rec, const double* lo, const double* hi, uint32_t shp, int h, const GridInfo& g,
-                       const MeshBox& mb, CopySetup& C) {
+                       const MeshBox& mb, CopySetup<R>& C) {
     C.G.rec = rec;
     C.wmask = 0;
     for (int a = 0; a < 3; ++a) {
@@ -861,11 +925,10 @@ PVX_HD bool copy_setup(const double* rec, const double* lo, const double* hi, ui
     if (C.G.ib0[0] >= g.n[0] || C.G.ib0[1] >= g.n[1] || C.G.ib0[2] >= g.n[2]) return false;
     for (int a = 0; a < 3; ++a) {
         const double inv = g.dinv[a];
-        for (int v = 0; v < 4; ++v)
-            C.X[v][a] = (float)((rec[3 * v + a] + C.G.shift[a] - g.w0[a]) * inv - C.G.ib0[a]);
+        for (int v = 0; v < 4; ++v) C.X[v][a] = (R)((rec[3 * v + a] + C.G.shift[a] - g.w0[a]) * inv - C.G.ib0[a]);
         C.G.x0[a] = C.X[0][a];
         C.G.x0d[a] = (rec[a] + C.G.shift[a] - g.w0[a]) * inv - C.G.ib0[a];
-        for (int b = 0; b < 3; ++b) C.G.DX[a][b] = (float)((rec[3 * (b + 1) + a] - rec[a]) * inv);
+        for (int b = 0; b < 3; ++b) C.G.DX[a][b] = (R)((rec[3 * (b + 1) + a] - rec[a]) * inv);
         C.wlo[a] = (double)(-C.G.ib0[a]);
         C.whi[a] = g.span[a] - C.G.ib0[a];
     }
@@ -873,16 +936,17 @@ PVX_HD bool copy_setup(const double* rec, const double* lo, const double* hi, ui
 }
 
 /* y-column range of slab i of a prepared copy */
-PVX_HD bool slab_columns(const CopySetup& C, int i, const GridInfo& g, int& j0, int& j1) {
+template <class R>
+PVX_HD bool slab_columns(const CopySetup<R>& C, int i, const GridInfo& g, int& j0, int& j1) {
     const int il = i - C.G.ib0[0];
     const bool xlo = i > C.G.ib0[0], xhi = i < C.ib1[0] - 1;
     const bool sxlo = xlo || (C.wmask & 1u), sxhi = xhi || (C.wmask & 2u);
-    const float cxlo = xlo ? (float)il : (float)C.wlo[0], cxhi = xhi ? (float)(il + 1) : (float)C.whi[0];
-    float ymin, ymax;
+    const R cxlo = xlo ? (R)il : (R)C.wlo[0], cxhi = xhi ? (R)(il + 1) : (R)C.whi[0];
+    R ymin, ymax;
     slab_yrange(C.X, sxlo, cxlo, sxhi, cxhi, ymin, ymax);
     if (ymin > ymax) return false;
-    j0 = C.G.ib0[1] + (int)floorf(ymin);
-    j1 = C.G.ib0[1] + (int)ceilf(ymax);
+    j0 = C.G.ib0[1] + (int)rfloor(ymin);
+    j1 = C.G.ib0[1] + (int)rceil(ymax);
     if (j0 < C.G.ib0[1]) j0 = C.G.ib0[1];
     if (j1 > C.ib1[1]) j1 = C.ib1[1];
     if (j1 <= j0) j1 = j0 + 1;
@@ -908,6 +972,16 @@ PVX_HD bool finite_rec(const double* rec) {
     return f;
 }
 
+/* whether a tet record spans more than WIDE_CELLS cells along an axis (its
+ * columns then run in FP64 under the automatic kernel choice) */
+PVX_HD bool tet_wide(const double* rec, const GridInfo& g) {
+    double lo[3], hi[3];
+    aabb((const double (*)[3])rec, 4, lo, hi);
+    bool w = false;
+    for (int a = 0; a < 3; ++a) w = w || (hi[a] - lo[a]) * g.dinv[a] > WIDE_CELLS;
+    return w;
+}
+
 /* limits of the task encoding: column offsets within a copy's index box,
  * cells along z, and tasks per tet (the largest-fragment key) */
 constexpr int TASK_MAXIJ = 1 << 14;
@@ -917,7 +991,9 @@ constexpr int64_t TASK_MAXPERTET = 1 << 22;
 /* Enumerate the column tasks of one tet record; writes them to out (when
  * given) and returns the count, or -1 when the tet cannot take the column
  * path (non-finite vertices, or an index box beyond the task encoding): the
- * caller then runs the FP64 kernel.  ncopies gets the number of copies. */
+ * caller then runs the FP64 kernel.  ncopies gets the number of copies.
+ * R: the precision of the column kernel that will take the tasks. */
+template <class R>
 PVX_HD int64_t tet_columns(const double* rec, int64_t t, const GridInfo& g, const MeshBox& mb, uint64_t* out,
                            int* ncopies) {
     if (ncopies) *ncopies = 0;
@@ -929,7 +1005,7 @@ PVX_HD int64_t tet_columns(const double* rec, int64_t t, const GridInfo& g, cons
     if (ncopies) *ncopies = ng;
     int64_t n = 0;
     for (int h = 0; h < ng; ++h) {
-        CopySetup C;
+        CopySetup<R> C;
         if (!copy_setup(rec, lo, hi, shp, h, g, mb, C)) continue;
         if (C.ib1[0] - C.G.ib0[0] >= TASK_MAXIJ || C.ib1[1] - C.G.ib0[1] >= TASK_MAXIJ ||
             C.ib1[2] - C.G.ib0[2] >= TASK_MAXK)
@@ -954,7 +1030,8 @@ PVX_HD int64_t tet_columns(const double* rec, int64_t t, const GridInfo& g, cons
  * the column's vertical edges where they pass through the tet (the interval
  * on which all four barycentric coordinates stay non-negative).  False when
  * the intersection is empty.  nvert (optional) gets the number of those
- * vertices (a point on several planes counts more than once).
+ * vertices (a point on several planes counts more than once).  FP32: only
+ * the processing order depends on it.
  */
 PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float yh, float& zmin, float& zmax,
                         int* nvert = nullptr) {
@@ -1035,7 +1112,7 @@ PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float
  * from column_zext: the sort key that groups tasks of equal work into the
  * same wavefront.  Only the processing order depends on it, so column_zext
  * divides with the hardware reciprocal. */
-PVX_HD int column_cells(const CopySetup& C, int il, int jl, int* nvert = nullptr) {
+PVX_HD int column_cells(const CopySetup<float>& C, int il, int jl, int* nvert = nullptr) {
     float zmin, zmax;
     if (!column_zext(C.X, (float)il, (float)(il + 1), (float)jl, (float)(jl + 1), zmin, zmax, nvert)) return 0;
     const int nk = C.ib1[2] - C.G.ib0[2];
@@ -1066,14 +1143,15 @@ struct SlabStep {
     uint8_t pad[15];
 };
 
-PVX_HD void tet_start(LPoly& P) {
+template <class R>
+PVX_HD void tet_start(LPoly<R>& P) {
     P.live = 0xfu;
     P.nv = 4;
     P.ord = 0x3210ull;
-    P.setq(0, 0.f, 0.f, 0.f);
-    P.setq(1, 1.f, 0.f, 0.f);
-    P.setq(2, 0.f, 1.f, 0.f);
-    P.setq(3, 0.f, 0.f, 1.f);
+    P.setq(0, R(0), R(0), R(0));
+    P.setq(1, R(1), R(0), R(0));
+    P.setq(2, R(0), R(1), R(0));
+    P.setq(3, R(0), R(0), R(1));
     P.setnbw(0, pack3(1, 3, 2));
     P.setnbw(1, pack3(2, 3, 0));
     P.setnbw(2, pack3(0, 3, 1));
@@ -1082,8 +1160,8 @@ PVX_HD void tet_start(LPoly& P) {
 
 /* one table entry, from the graph steps alone (no coordinates) */
 inline bool make_slab_step(int pattern, SlabStep& E) {
-    static uint32_t mem[NVF * VSTRIDE];
-    LPoly P{mem, 0};
+    static uint32_t mem[VRec<float>::WORDS];
+    LPoly<float> P{mem, 0};
     tet_start(P);
     uint8_t src[NVF] = {0};
     for (int v = 0; v < 4; ++v) src[v] = (uint8_t)(v << 2);
@@ -1141,19 +1219,20 @@ PVX_HD const SlabStep& slab_step(int p) {
 /* P = the tet clipped by the upper x plane chi (when has_hi) and then the
  * lower one clo (when has_lo), from the table.  False when the table does
  * not apply (a cut of the upper plane on the dropped side of the lower one:
- * a window slab thinner than FP32 resolves); P then holds the tet. */
-PVX_HD bool slab_from_table(LPoly& P, const CopyGeo& G, const GridInfo& g, bool has_hi, double chi, bool has_lo,
+ * a window slab thinner than the precision resolves); P then holds the tet. */
+template <class R>
+PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g, bool has_hi, double chi, bool has_lo,
                             double clo) {
     tet_start(P);
-    const AxisMap m = axis_map(G, g, 0);
-    const float offh = (float)(chi - m.od), offl = (float)(clo - m.od);
-    float th[4], tl[4];
+    const AxisMap<R> m = axis_map(G, g, 0);
+    const R offh = (R)(chi - m.od), offl = (R)(clo - m.od);
+    R th[4], tl[4];
     int pat = 0;
     for (int v = 3; v >= 0; --v) {
-        const float q0 = v == 1 ? 1.f : 0.f, q1 = v == 2 ? 1.f : 0.f, q2 = v == 3 ? 1.f : 0.f;
-        th[v] = has_hi ? sdist(G.rec, m, chi, offh, q0, q1, q2) : 1.f;
-        tl[v] = has_lo ? sdist(G.rec, m, clo, offl, q0, q1, q2) : -1.f;
-        pat = 3 * pat + (th[v] < 0.0f ? 2 : (tl[v] >= 0.0f ? 0 : 1));
+        const R q0 = v == 1 ? R(1) : R(0), q1 = v == 2 ? R(1) : R(0), q2 = v == 3 ? R(1) : R(0);
+        th[v] = has_hi ? sdist(G.rec, m, chi, offh, q0, q1, q2) : R(1);
+        tl[v] = has_lo ? sdist(G.rec, m, clo, offl, q0, q1, q2) : R(-1);
+        pat = 3 * pat + (th[v] < R(0) ? 2 : (tl[v] >= R(0) ? 0 : 1));
     }
     const SlabStep& E = slab_step(pat);
     if (E.nv == 0) {
@@ -1167,10 +1246,10 @@ PVX_HD bool slab_from_table(LPoly& P, const CopyGeo& G, const GridInfo& g, bool 
     for (int r = 0; r < 8; ++r) {
         if (r >= nv || (E.src[r] >> 6) != 1) continue;
         const int vc = (E.src[r] >> 2) & 15, vn = E.src[r] & 3, vx = E.slot[r];
-        const float sc = vc == 0 ? th[0] : (vc == 1 ? th[1] : (vc == 2 ? th[2] : th[3]));
-        const float sn = vn == 0 ? th[0] : (vn == 1 ? th[1] : (vn == 2 ? th[2] : th[3]));
+        const R sc = vc == 0 ? th[0] : (vc == 1 ? th[1] : (vc == 2 ? th[2] : th[3]));
+        const R sn = vn == 0 ? th[0] : (vn == 1 ? th[1] : (vn == 2 ? th[2] : th[3]));
         cut_point(P, vx, vc, vn, sc, sn);
-        if (has_lo && sdist(G.rec, m, clo, offl, P.qa(vx, 0), P.qa(vx, 1), P.qa(vx, 2)) >= 0.0f) ok = false;
+        if (has_lo && sdist(G.rec, m, clo, offl, P.qa(vx, 0), P.qa(vx, 1), P.qa(vx, 2)) >= R(0)) ok = false;
     }
     if (!ok) {
         tet_start(P);
@@ -1180,8 +1259,8 @@ PVX_HD bool slab_from_table(LPoly& P, const CopyGeo& G, const GridInfo& g, bool 
     for (int r = 0; r < 8; ++r) {
         if (r >= nv || (E.src[r] >> 6) != 2) continue;
         const int vc = (E.src[r] >> 2) & 15, vn = E.src[r] & 3, vx = E.slot[r];
-        const float sc = sdist(G.rec, m, clo, offl, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
-        const float sn = vn == 0 ? tl[0] : (vn == 1 ? tl[1] : (vn == 2 ? tl[2] : tl[3]));
+        const R sc = sdist(G.rec, m, clo, offl, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
+        const R sn = vn == 0 ? tl[0] : (vn == 1 ? tl[1] : (vn == 2 ? tl[2] : tl[3]));
         cut_point(P, vx, vc, vn, sc, sn);
     }
     uint32_t live = 0;
@@ -1203,8 +1282,9 @@ PVX_HD bool slab_from_table(LPoly& P, const CopyGeo& G, const GridInfo& g, bool 
  * clipped by the column's four planes -- interior grid planes or, on the
  * copy's first/last slab or column, the window planes it crosses
  * (src/geometry.c:816-843).  False on a vertex-budget overflow. */
-PVX_HD bool build_column(LPoly& P, const CopySetup& C, const GridInfo& g, int i, int j, int j0, int j1) {
-    const CopyGeo& G = C.G;
+template <class R>
+PVX_HD bool build_column(LPoly<R>& P, const CopySetup<R>& C, const GridInfo& g, int i, int j, int j0, int j1) {
+    const CopyGeo<R>& G = C.G;
     const int il = i - G.ib0[0], jl = j - G.ib0[1];
     const bool xlo = i > G.ib0[0], xhi = i < C.ib1[0] - 1, ylo = j > j0, yhi = j < j1 - 1;
     /* the x-slab: from the table, or (when it does not apply) by the two
@@ -1229,16 +1309,18 @@ PVX_HD bool build_column(LPoly& P, const CopySetup& C, const GridInfo& g, int i,
 
 /* a column task after its column is built: the copy, the column's cell
  * (i, j) and its z-cell range [k0, k1) */
+template <class R>
 struct ColumnState {
-    CopySetup C;
+    CopySetup<R> C;
     int i, j, k0, k1;
 };
 
 /* the planes of cell k of a built column: interior planes between the
  * column's cells, window planes where the copy crosses them, open ends
  * elsewhere (src/geometry.c:816-843) */
-PVX_HD void cell_planes(const ColumnState& S, int k, bool& has_lo, double& lo_c, bool& has_hi, double& hi_c) {
-    const CopySetup& C = S.C;
+template <class R>
+PVX_HD void cell_planes(const ColumnState<R>& S, int k, bool& has_lo, double& lo_c, bool& has_hi, double& hi_c) {
+    const CopySetup<R>& C = S.C;
     const int kl = k - C.G.ib0[2];
     has_lo = k > S.k0 || (k == C.G.ib0[2] && (C.wmask & 16u));
     has_hi = k < S.k1 - 1 || (k == C.ib1[2] - 1 && (C.wmask & 32u));
@@ -1248,8 +1330,9 @@ PVX_HD void cell_planes(const ColumnState& S, int k, bool& has_lo, double& lo_c,
 
 /* Build the column of a task: 1 with cells [k0, k1), 0 when the column is
  * empty, -1 on a vertex-budget overflow (the FP64 kernel takes over). */
-PVX_HD int column_begin(const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task, LPoly& P,
-                        ColumnState& S) {
+template <class R>
+PVX_HD int column_begin(const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task, LPoly<R>& P,
+                        ColumnState<R>& S) {
     int64_t t;
     int h, il, jl;
     unpack_task(task, t, h, il, jl);
@@ -1270,23 +1353,25 @@ PVX_HD int column_begin(const GridInfo& g, const MeshBox& mb, const double* rec,
 }
 
 /* per-tet mass and velocity data of a record */
+template <class R>
 struct TetVel {
     double u0[3];
-    float DV[3][3];
+    R DV[3][3];
     double mass;
 };
 
-PVX_HD void tet_vel(const double* rec, TetVel& V) {
+template <class R>
+PVX_HD void tet_vel(const double* rec, TetVel<R>& V) {
     for (int a = 0; a < 3; ++a) {
         V.u0[a] = rec[12 + a];
-        for (int b = 0; b < 3; ++b) V.DV[a][b] = (float)(rec[12 + 3 * (b + 1) + a] - rec[12 + a]);
+        for (int b = 0; b < 3; ++b) V.DV[a][b] = (R)(rec[12 + 3 * (b + 1) + a] - rec[12 + a]);
     }
 }
 
-
-/* the largest fragment of a column task: its moments, origin and cell */
+/* the largest fragment of a column task: its moments, origin and cell (in
+ * FP32: the residual that is deposited with them is small) */
 struct BestFrag {
-    Mom M;
+    Mom<float> M;
     float O[3];
     int kl; /* z cell relative to the copy's index box; -1: none */
 };
@@ -1314,19 +1399,21 @@ struct TaskTally {
 };
 
 /* deposit cell k of a task from its slab moments Mm about O */
-template <int ORDER, class Sink>
-PVX_HD void cell_finish(Sink& sink, const GridInfo& g, const ColumnState& S, const TetVel& V, int k, const Mom& Mm,
-                        const float* O, TaskTally& T) {
-    if (!(Mm.m0 > 0.0f)) return;
-    deposit_f<ORDER>(sink, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0, (const float (*)[3])V.DV, V.mass,
+template <int ORDER, class R, class Sink>
+PVX_HD void cell_finish(Sink& sink, const GridInfo& g, const ColumnState<R>& S, const TetVel<R>& V, int k,
+                        const Mom<R>& Mm, const R* O, TaskTally& T) {
+    if (!(Mm.m0 > R(0))) return;
+    deposit_f<ORDER>(sink, g, S.C.G, S.C.G.ib0, S.i, S.j, k, Mm, O, V.u0, (const R (*)[3])V.DV, V.mass,
                      (double)Mm.m0);
     T.nfrag++;
     T.sum_m0 += (double)Mm.m0;
-    if (Mm.m0 > T.B.M.m0) {
-        T.B.M = Mm;
-        T.B.O[0] = O[0];
-        T.B.O[1] = O[1];
-        T.B.O[2] = O[2];
+    if ((float)Mm.m0 > T.B.M.m0) {
+        T.B.M.m0 = (float)Mm.m0;
+        for (int i = 0; i < 3; ++i) T.B.M.s1[i] = (float)Mm.s1[i];
+        for (int i = 0; i < 6; ++i) T.B.M.s2[i] = (float)Mm.s2[i];
+        T.B.O[0] = (float)O[0];
+        T.B.O[1] = (float)O[1];
+        T.B.O[2] = (float)O[2];
         T.B.kl = k - S.C.G.ib0[2];
     }
 }
@@ -1336,26 +1423,26 @@ PVX_HD void cell_finish(Sink& sink, const GridInfo& g, const ColumnState& S, con
  * deposit its cells.  Returns false on overflow (the caller falls back to
  * the FP64 kernel).  T gets the task's tally for the residual pass.
  */
-template <int ORDER, class Sink>
-PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task, LPoly& P,
-                        TaskTally& T, bool carry = true) {
+template <int ORDER, class R, class Sink>
+PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task,
+                        LPoly<R>& P, TaskTally& T, bool carry = true) {
     T.reset();
-    ColumnState S;
+    ColumnState<R> S;
     const int st = column_begin(g, mb, rec, task, P, S);
     if (st < 0) return false;
     if (st == 0) return true;
-    TetVel V;
+    TetVel<R> V;
     if (ORDER > 0) tet_vel(rec, V);
     V.mass = rec[24];
     pin(V.mass);
-    PlaneCarry pc{false, false, {0.f, 0.f, 0.f}};
+    PlaneCarry<R> pc{false, false, {R(0), R(0), R(0)}};
 #pragma unroll 1
     for (int k = S.k0; k < S.k1; ++k) {
         bool hl, hh;
         double lc, hc;
         cell_planes(S, k, hl, lc, hh, hc);
-        Mom Mm;
-        float O[3];
+        Mom<R> Mm;
+        R O[3];
         slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O, carry ? &pc : nullptr);
         cell_finish<ORDER>(sink, g, S, V, k, Mm, O, T);
     }
@@ -1364,12 +1451,12 @@ PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const 
 
 /*
  * The residual of one tet, r = (exact in-window mass fraction, record slot
- * 26) - (FP64 sum of its FP32 fragment masses), is spread over its column
- * tasks: task c's largest fragment B takes r * sum_c / sum (sum_c: the
- * task's fragment-mass sum), deposited with that fragment's own means and
+ * 26) - (FP64 sum of its fragment masses), is spread over its column tasks:
+ * task c's largest fragment B takes r * sum_c / sum (sum_c: the task's
+ * fragment-mass sum), deposited with that fragment's own means and
  * covariances so every moment stays consistent.  Spread this way each
- * correction is about the size of the fragments' own FP32 error (a single
- * receiving fragment would take the whole tet's).  This deposits one
+ * correction is about the size of the fragments' own rounding error (a
+ * single receiving fragment would take the whole tet's).  This deposits one
  * task's share: weight = r * sum_c / sum (or r for the one receiver of a
  * tet with a small residual, RESID_WHOLE).
  */
@@ -1384,9 +1471,9 @@ PVX_HD void task_residual(Sink& sink, const GridInfo& g, const MeshBox& mb, cons
     aabb((const double (*)[3])rec, 4, lo, hi);
     uint32_t shp;
     ghost_shifts_packed(lo, hi, g, mb, shp);
-    CopySetup C;
+    CopySetup<float> C;
     if (!copy_setup(rec, lo, hi, shp, h, g, mb, C)) return;
-    TetVel V;
+    TetVel<float> V;
     if (ORDER > 0) tet_vel(rec, V);
     deposit_f<ORDER>(sink, g, C.G, C.G.ib0, C.G.ib0[0] + il, C.G.ib0[1] + jl, C.G.ib0[2] + B.kl, B.M, B.O, V.u0,
                      (const float (*)[3])V.DV, rec[24], weight);

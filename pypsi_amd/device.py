@@ -41,7 +41,7 @@ class Opts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("nelem_used", C.c_int64), ("ntets", C.c_int64), ("nghosts", C.c_int64), ("nfrag", C.c_int64),
                 ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("kernel", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("nwide", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

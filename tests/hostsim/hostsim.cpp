@@ -51,6 +51,74 @@ static void fill_grid(GridInfo& g, MeshBox& mb, const int* n, const double* win6
 static bool g_carry = true;
 extern "C" void hs_set_plane_carry(int on) { g_carry = on != 0; }
 
+/* 1: wide tets (tet_wide) take the FP64 column path, as the device's
+ * automatic kernel choice does; 0: every tet in FP32 (kernel 'fast') */
+static int g_route = 0;
+extern "C" void hs_set_route_wide(int on) { g_route = on; }
+
+/* the column tasks of one tet record in precision R, deposited through
+ * sink, with the per-tet residual; returns false on a task failure */
+template <class R>
+static bool tet_by_columns(HostSink& sink, const GridInfo& g, const MeshBox& mb, const double* r, int64_t t,
+                           LPoly<R>& P, std::vector<uint64_t>& tasks, int64_t& nfrag, int64_t& ncopies,
+                           int64_t& ntasks, int64_t& fails) {
+    int nc = 0;
+    const int64_t ncol = tet_columns<R>(r, t, g, mb, nullptr, &nc);
+    ncopies += nc;
+    if (ncol < 0) {
+        fails++;
+        return false;
+    }
+    tasks.resize((size_t)ncol);
+    tet_columns<R>(r, t, g, mb, tasks.data(), nullptr);
+    ntasks += ncol;
+    std::vector<double> tsum((size_t)ncol);
+    std::vector<BestFrag> tbest((size_t)ncol);
+    double sum = 0.0;
+    bool all = true;
+    for (int64_t c = 0; c < ncol; ++c) {
+        bool ok;
+        TaskTally T;
+        if (g.order == 0)
+            ok = column_task<0>(sink, g, mb, r, tasks[c], P, T, g_carry);
+        else if (g.order == 1)
+            ok = column_task<1>(sink, g, mb, r, tasks[c], P, T, g_carry);
+        else
+            ok = column_task<2>(sink, g, mb, r, tasks[c], P, T, g_carry);
+        if (!ok) {
+            fails++;
+            all = false;
+        }
+        nfrag += T.nfrag;
+        tsum[c] = T.sum_m0;
+        tbest[c] = T.B;
+        sum += tsum[c];
+    }
+    /* the largest fragment (first task on ties, as the GPU's key max) */
+    int64_t cb = -1;
+    uint64_t kb = 0;
+    for (int64_t c = 0; c < ncol; ++c) {
+        const uint64_t k = tbest[c].kl >= 0 ? best_key(tbest[c].M.m0, tbest[c].kl) : 0ull;
+        if (k > kb) {
+            kb = k;
+            cb = c;
+        }
+    }
+    const double resid = r[26] - sum;
+    const bool whole = cb >= 0 && fabs(resid) <= RESID_WHOLE * (double)tbest[cb].M.m0;
+    const double f = residual_factor(r, sum);
+    for (int64_t c = 0; c < ncol; ++c) {
+        const double wgt = whole ? (c == cb ? resid : 0.0) : f * tsum[c];
+        if (g.order == 0)
+            task_residual<0>(sink, g, mb, r, tasks[c], tbest[c], wgt);
+        else if (g.order == 1)
+            task_residual<1>(sink, g, mb, r, tasks[c], tbest[c], wgt);
+        else
+            task_residual<2>(sink, g, mb, r, tasks[c], tbest[c], wgt);
+    }
+    return all;
+}
+
 /* the FP32 fast path on the host: kernel == 2 */
 extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const double* vel, const double* mass,
                               const int32_t* conn, int periodic, const double* box6, const int* n, const double* win6,
@@ -70,8 +138,10 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
     if (vpe == 4) maxlvl = 0;
     RefineScratch* RS = new RefineScratch;
     Poly64* pool = new Poly64[2];
-    std::vector<uint32_t> lv(NVF * VSTRIDE);
-    LPoly P{&lv[0], 0};
+    std::vector<uint32_t> lv(VRec<float>::WORDS);
+    LPoly<float> P{&lv[0], 0, 0, 0};
+    std::vector<double> lv64((size_t)VRec<double>::WORDS / 2);
+    LPoly<double> P64{(uint32_t*)lv64.data(), 0, 0, 0};
     HostSink sink{acc.data()};
     std::vector<double> tets;
     std::vector<uint64_t> tasks;
@@ -85,56 +155,10 @@ extern "C" int hs_voxels_fast(int vpe, int64_t nelem, const double* pos, const d
             double* r = &tets[(size_t)t * TREC];
             if (r[25] == 0.0) continue;
             if (!(mb.periodic && mb.full_window)) r[26] = window_target(r, g, mb, pool);
-            int nc = 0;
-            const int64_t ncol = tet_columns(r, t, g, mb, nullptr, &nc);
-            ncopies += nc;
-            if (ncol < 0) {
-                fails++;
-                continue;
-            }
-            tasks.resize((size_t)ncol);
-            tet_columns(r, t, g, mb, tasks.data(), nullptr);
-            ntasks += ncol;
-            std::vector<double> tsum((size_t)ncol);
-            std::vector<BestFrag> tbest((size_t)ncol);
-            double sum = 0.0;
-            for (int64_t c = 0; c < ncol; ++c) {
-                bool ok;
-                TaskTally T;
-                if (g.order == 0)
-                    ok = column_task<0>(sink, g, mb, r, tasks[c], P, T, g_carry);
-                else if (g.order == 1)
-                    ok = column_task<1>(sink, g, mb, r, tasks[c], P, T, g_carry);
-                else
-                    ok = column_task<2>(sink, g, mb, r, tasks[c], P, T, g_carry);
-                if (!ok) fails++;
-                nfrag += T.nfrag;
-                tsum[c] = T.sum_m0;
-                tbest[c] = T.B;
-                sum += tsum[c];
-            }
-            /* the largest fragment (first task on ties, as the GPU's key max) */
-            int64_t cb = -1;
-            uint64_t kb = 0;
-            for (int64_t c = 0; c < ncol; ++c) {
-                const uint64_t k = tbest[c].kl >= 0 ? best_key(tbest[c].M.m0, tbest[c].kl) : 0ull;
-                if (k > kb) {
-                    kb = k;
-                    cb = c;
-                }
-            }
-            const double resid = r[26] - sum;
-            const bool whole = cb >= 0 && fabs(resid) <= RESID_WHOLE * (double)tbest[cb].M.m0;
-            const double f = residual_factor(r, sum);
-            for (int64_t c = 0; c < ncol; ++c) {
-                const double wgt = whole ? (c == cb ? resid : 0.0) : f * tsum[c];
-                if (g.order == 0)
-                    task_residual<0>(sink, g, mb, r, tasks[c], tbest[c], wgt);
-                else if (g.order == 1)
-                    task_residual<1>(sink, g, mb, r, tasks[c], tbest[c], wgt);
-                else
-                    task_residual<2>(sink, g, mb, r, tasks[c], tbest[c], wgt);
-            }
+            if (g_route && tet_wide(r, g))
+                tet_by_columns<double>(sink, g, mb, r, t, P64, tasks, nfrag, ncopies, ntasks, fails);
+            else
+                tet_by_columns<float>(sink, g, mb, r, t, P, tasks, nfrag, ncopies, ntasks, fails);
         }
         ntets += nt;
     }

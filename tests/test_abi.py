@@ -40,7 +40,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_argument_errors():
     from pypsi_amd import device as D
     L = D.lib()
-    assert L.psivox_abi_version() == 1
+    assert L.psivox_abi_version() == 2
     # NULL descriptors are rejected before any device work
     assert L.psivox_voxels(None, None, None, None) == -1
     assert b"required" in L.psivox_last_error()
@@ -58,7 +58,7 @@ def test_struct_layouts_match_header():
     from pypsi_amd import device as D
     assert C.sizeof(D.Mesh) == 4 + 4 + 8 + 8 + 4 * 8 + 48 + 4 + 4
     assert C.sizeof(D.Grid) == 12 + 4 + 48 + 24 + 48
-    assert C.sizeof(D.Stats) == 4 * 8 + 2 * 8 + 4 + 4
+    assert C.sizeof(D.Stats) == 4 * 8 + 2 * 8 + 4 + 4 + 8
 
 
 def test_module_argument_errors():

@@ -3,8 +3,13 @@ oracle: lattice sizes 6..12, displacements of 0.2..3 lattice cells, grids of
 up to 3 cells per element edge with unequal axes, random field sets,
 periodic and open meshes (open ones unwrapped), full grids and random
 windows, refinement levels 0..2 (tolerance 0.2 cells).  The FP64 kernel is
-held to the FP64 bar and the FP32 column kernel to its FP32 bar (both in
-tests/helpers.py and test_gpu_parity.py)."""
+held to the FP64 bar, the FP32 column kernel and the automatic choice to the
+FP32 bar (both in tests/helpers.py and test_gpu_parity.py).
+
+Then the regime that needs the automatic choice: elements spanning 8 cells
+per edge, and an open mesh made of wrapped periodic positions (elements
+spanning the whole grid).  kernel='auto' (PSI.voxels' default) sends those
+tets to the FP64 column kernel and holds the FP32 bar there."""
 import numpy as np
 import pytest
 
@@ -63,9 +68,42 @@ def test_sweep(PSI, oracle_mod, seed):
     kw = dict(loader="block", pos=pos, vel=vel)
     if periodic:
         kw["periodic_box"] = BOX
-    for kernel, tol in (("plain", TOL), ("fast", TOL32)):
+    for kernel, tol in (("plain", TOL), ("fast", TOL32), ("auto", TOL32)):
         grid = PSI.Grid(n=gn, bounds=bounds, fields=fields)
         PSI.voxels(grid=grid, mesh=PSI.Mesh(**kw), refine_tolerance=0.2, refine_max_lvl=lvl, kernel=kernel)
         errs = assert_close(grid.fields, ref, tol, kernel)
         print(kernel, {k: "%.1e/%.1e" % v for k, v in errs.items()}, flush=True)
         assert abs(1 - grid.fields["m"].sum() / ref["m"].sum()) < 1e-12
+
+
+@pytest.mark.parametrize("n,gn,rms,periodic", [(6, 48, 1.5, True), (8, 64, 1.0, True), (8, 16, 2.0, False),
+                                               (10, 40, 3.0, False)])
+def test_auto_wide_tets(PSI, oracle_mod, n, gn, rms, periodic):
+    import torch
+    from pypsi_amd import device as D
+    from pypsi_amd.synth import zeldovich_block
+    fields = ["x", "v", "xx", "xv", "vv"]
+    print(f"case: wide tets {n}^3 rms{rms} -> {gn}^3 periodic={periodic} (open: wrapped positions)", flush=True)
+    pos, vel = zeldovich_block(n, rms, seed=n)
+    P = oracle_mod.Problem.block(pos, vel, periodic_box=BOX if periodic else None, n=(gn,) * 3, bounds=BOX,
+                                 fields=fields)
+    ref, st = oracle_mod.voxels(P)
+    # the drop-in call with its default kernel
+    grid = PSI.Grid(n=(gn,) * 3, bounds=BOX, fields=fields)
+    kw = dict(loader="block", pos=pos, vel=vel)
+    if periodic:
+        kw["periodic_box"] = BOX
+    PSI.voxels(grid=grid, mesh=PSI.Mesh(**kw))
+    errs = assert_close(grid.fields, ref, TOL32, "auto")
+    print("auto", {k: "%.1e/%.1e" % v for k, v in errs.items()}, flush=True)
+    assert abs(1 - grid.fields["m"].sum() / ref["m"].sum()) < 1e-12
+    # the same through the C ABI: the wide tets went to the FP64 column kernel
+    dev = torch.device("cuda:0")
+    tf = {f: torch.zeros(v.shape, dtype=torch.float64, device=dev) for f, v in ref.items()}
+    tp, tv = torch.from_numpy(P.pos).to(dev), torch.from_numpy(P.vel).to(dev)
+    mesh = D.make_mesh(tp, tv, torch.from_numpy(P.mass).to(dev), torch.from_numpy(P.conn).to(dev),
+                       BOX if periodic else None)
+    stats = D.voxels(mesh, D.make_grid((gn,) * 3, BOX, tf), D.make_opts(kernel="auto"))
+    assert stats["nwide"] > 0
+    assert abs(stats["nfrag"] - st["nfrag"]) <= 1e-5 * st["nfrag"]
+    assert_close({k: v.cpu().numpy() for k, v in tf.items()}, ref, TOL32, "auto (C ABI)")

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import hostsim_api as H
-from helpers import ALL, BOX, assert_close, TOL32
+from helpers import ALL, BOX, assert_close, field_errors, TOL32
 
 TOL64 = {"m": (1e-12, 1e-13), "x": (1e-10, 1e-13), "v": (1e-10, 1e-13), "xx": (1e-7, 1e-12), "xv": (1e-7, 1e-12),
          "vv": (1e-7, 1e-12)}
@@ -91,3 +91,34 @@ def test_merge_existing_contents(oracle_mod):
     got, _ = H.run(P1, "plain")
     got, _ = H.run(P2, "plain", out=got)
     assert_close(got, ref, TOL64, "merge")
+
+
+def test_wide_tets_routed_to_fp64_columns(oracle_mod):
+    """Elements spanning 8 cells per edge (6^3 lattice onto 48^3): every tet
+    is wider than the FP32 column path's 4-cell limit.  In FP32 their
+    sliver fragments break the second-moment bar (about 3e-3 here); the
+    automatic choice gives them to the FP64 column kernel, which meets the
+    FP64 bars.  An open mesh of wrapped positions (elements spanning the
+    whole grid) and a mixed mesh hold the FP32 bar the same way."""
+    L = H.lib()
+    try:
+        pos, vel = zel(6, 1.5, 0)
+        P = oracle_mod.Problem.block(pos, vel, periodic_box=BOX, n=(48,) * 3, bounds=BOX, fields=["x", "v", "xx", "vv"])
+        ref, st = oracle_mod.voxels(P)
+        L.hs_set_route_wide(0)
+        fp32, _ = H.run(P, "fast")
+        e32 = field_errors(fp32, ref)
+        assert e32["vv"][0] > TOL32["vv"][0]  # the regime the routing exists for
+        L.hs_set_route_wide(1)
+        got, hst = H.run(P, "fast")
+        assert hst[0] == st["nfrag"]
+        assert_close(got, ref, TOL64, "wide tets in FP64 columns")
+        for n, gn, rms, per in ((8, 16, 2.0, False), (10, 30, 3.0, True)):
+            pos, vel = zel(n, rms, n)
+            P = oracle_mod.Problem.block(pos, vel, periodic_box=BOX if per else None, n=(gn,) * 3, bounds=BOX,
+                                         fields=ALL)
+            ref, _ = oracle_mod.voxels(P)
+            got, _ = H.run(P, "fast")
+            assert_close(got, ref, TOL32, f"routed {n}^3 -> {gn}^3 periodic={per}")
+    finally:
+        L.hs_set_route_wide(0)
