@@ -137,6 +137,20 @@ int psivox_voxels(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts*
 int psivox_voxels_batched(int32_t nmesh, const psivox_mesh* meshes, psivox_grid* grids, const psivox_opts* opts,
                           psivox_stats* stats);
 
+/* psivox_voxels without allocating device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The mesh
+ * and the grid must be device arrays (on_device = 1), mode density,
+ * refine_max_lvl 0.  Every buffer beyond them comes from a fixed 256 MiB
+ * scratch array in device memory that each GPU holds until the process
+ * ends; the grid's raw-moment accumulator (nvox x 1/10/28 doubles for m,
+ * m+v+vv, all fields) must leave room in it for one element batch and one
+ * chunk of column tasks, else PSIVOX_E_NOMEM with a message.  Calls on one
+ * GPU share the array, so calls that overlap on one GPU race; the library
+ * serializes the calls of one process.  Runs on opts->stream (NULL: the
+ * null stream); stats->kernel_ms is 0 (no events are recorded). */
+int psivox_voxels_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* opts, psivox_stats* stats);
+
 /* Adjoint of the mass deposit.  No reference counterpart (the reference has
  * no derivatives; the forward it differentiates is psi_voxels,
  * src/cpsi.c:41, density branch :100-112).  Given dL/dm per cell (float64,

@@ -1163,6 +1163,15 @@ __global__ __launch_bounds__(FBLOCK) void k_adj(const double* __restrict__ tets,
 
 namespace {
 
+/* A fixed region of device memory handed out in order (the allocation-free
+ * entry point's scratch): while one is active on a thread, DevBuf::get
+ * carves from it instead of calling hipMalloc. */
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, off = 0;
+};
+thread_local Arena* t_arena = nullptr;
+
 /* grow-only device buffers reused across calls on this thread's device */
 struct DevBuf {
     void* p = nullptr;
@@ -1170,6 +1179,20 @@ struct DevBuf {
     int dev = -1;
     int get(size_t bytes, int device) {
         if (p && n >= bytes && dev == device) return 0;
+        if (t_arena) {
+            /* buffers are carved once per call at their largest size: a
+             * second, larger request would leave the first in use */
+            if (p) return set_err(PSIVOX_E_NOMEM, "scratch buffer of %zu bytes asked to grow to %zu", n, bytes);
+            const size_t need = (std::max<size_t>(bytes, 256) + 255) & ~(size_t)255;
+            if (t_arena->off + need > t_arena->size)
+                return set_err(PSIVOX_E_NOMEM, "the problem needs more than the %zu bytes of scratch (%zu in use, %zu more asked)",
+                               t_arena->size, t_arena->off, need);
+            p = t_arena->base + t_arena->off;
+            t_arena->off += need;
+            n = need;
+            dev = device;
+            return 0;
+        }
         if (p) {
             int cur = 0;
             (void)hipGetDevice(&cur);
@@ -1199,6 +1222,8 @@ struct Workspace {
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timing = true;                     /* kernel_ms from events (off: no events, kernel_ms = 0) */
+    int64_t task_cap = (int64_t)1 << 28;    /* column tasks per chunk */
     hipStream_t side = nullptr; /* the field upload beside the kernels */
     int side_dev = -1;
     hipEvent_t side_done = nullptr;
@@ -1544,7 +1569,7 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
     if (ntask == 0) return 0;
     /* tet sub-ranges whose tasks fit the task buffers, cut on the device
      * (colstart stays there: for a large batch it is hundreds of MB) */
-    const int64_t cap = (int64_t)1 << 28;
+    const int64_t cap = W.task_cap;
     const int64_t tcap = std::min(ntask, cap);
     if ((rc = W.tasks.get((size_t)tcap * sizeof(uint64_t), dev))) return rc;
     if ((rc = W.cut.get(4 * sizeof(int64_t), dev))) return rc;
@@ -1562,6 +1587,11 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
             a1 = hc[0];
             base = hc[1];
             nsub = hc[2] - hc[1];
+            /* a single tet with more tasks than a chunk holds (only with a
+             * small scratch) */
+            if (nsub > cap)
+                return set_err(PSIVOX_E_NOMEM, "a tet has %lld column tasks, more than the %lld a chunk holds",
+                               (long long)nsub, (long long)cap);
         }
         if (nsub == 0) continue;
         if (G.table)
@@ -1927,7 +1957,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 HIPCHK(hipGetLastError());
             }
             if (kernel == PSIVOX_KERNEL_PLAIN) {
-                HIPCHK(hipEventRecord(W.ev0, s));
+                if (W.timing) HIPCHK(hipEventRecord(W.ev0, s));
                 k_vox_plain<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, acc, status, counters);
                 HIPCHK(hipGetLastError());
             } else {
@@ -1952,15 +1982,18 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                             if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
                             TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
                             rc2 = launch_cols(tets, tasks, nsub, G, pass == PASS_WIDE, out, (uint32_t*)W.perm.p,
-                                              perm_counter(W.perm, nsub), status, counters, s, W.ev0);
+                                              perm_counter(W.perm, nsub), status, counters, s,
+                                              W.timing ? W.ev0 : nullptr);
                             if (rc2)
                                 return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
                                                hipGetErrorString((hipError_t)rc2));
-                            HIPCHK(hipEventRecord(W.ev1, s));
-                            HIPCHK(hipEventSynchronize(W.ev1));
-                            float ms = 0.f;
-                            HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
-                            kms += ms;
+                            if (W.timing) {
+                                HIPCHK(hipEventRecord(W.ev1, s));
+                                HIPCHK(hipEventSynchronize(W.ev1));
+                                float ms = 0.f;
+                                HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+                                kms += ms;
+                            }
                             if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
                             rc2 = launch_residual(tets, a0, a1, G, colstart, base, tasks, nsub, out,
                                                   (double*)W.colcnt.p, (int64_t*)W.tetb.p, s);
@@ -1974,7 +2007,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 }
                 nwide_total += nwide;
             }
-            if (kernel == PSIVOX_KERNEL_PLAIN) {
+            if (kernel == PSIVOX_KERNEL_PLAIN && W.timing) {
                 HIPCHK(hipEventRecord(W.ev1, s));
                 HIPCHK(hipEventSynchronize(W.ev1));
                 float ms = 0.f;
@@ -2395,6 +2428,42 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
     return 0;
 }
 
+/*
+ * Scratch of the allocation-free entry point (psivox_voxels_noalloc): every
+ * device buffer beyond the caller's inputs and outputs is carved from this
+ * array.  Each GPU the library runs on holds its own copy, at its own
+ * address, until the process ends.  Host code never passes the array's name
+ * to a kernel or a copy: it takes the device address once per device from
+ * hipGetSymbolAddress (scratch_address) and uses that.
+ */
+constexpr size_t NOALLOC_SCRATCH = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH];
+
+/* the scratch array's address on device dev (current), looked up once */
+int scratch_address(int dev, char** out) {
+    static std::mutex mu;
+    static void* ptr[256];
+    if (dev < 0 || dev >= 256) return set_err(PSIVOX_E_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(PSIVOX_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        ptr[dev] = p;
+    }
+    *out = (char*)ptr[dev];
+    return 0;
+}
+
+/* the thread's arena while an allocation-free call runs */
+struct ArenaScope {
+    explicit ArenaScope(Arena* a) { t_arena = a; }
+    ~ArenaScope() { t_arena = nullptr; }
+};
+
+Workspace g_ws_noalloc;
+
 /* the x-slab step table in the current device's constant memory, once per
  * device (every thread's calls see it once the first upload returned) */
 int upload_tables(int dev) {
@@ -2407,6 +2476,78 @@ int upload_tables(int dev) {
                              hipMemcpyHostToDevice));
     done[dev] = true;
     return 0;
+}
+
+/*
+ * psivox_voxels without allocating: device mesh and grid, density mode,
+ * refinement level 0.  The grid accumulator, the tet records of an element
+ * batch and a chunk of column tasks are laid out in the device's scratch
+ * array: the batch takes up to two fifths of what the accumulator leaves,
+ * the task chunk the rest.  Runs on opts->stream (NULL: the null stream),
+ * records no events (kernel_ms is 0).
+ */
+int run_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* o, psivox_stats* st) {
+    int rc = check_mesh(mesh);
+    if (rc) return rc;
+    if (!mesh->on_device || !grid->on_device)
+        return set_err(PSIVOX_E_ARG, "psivox_voxels_noalloc takes a device mesh and a device grid (on_device = 1)");
+    if (o->mode != PSIVOX_MODE_DENSITY) return set_err(PSIVOX_E_MODE, "psivox_voxels_noalloc: density mode only");
+    if (o->refine_max_lvl > 0 && mesh->elemtype != PSIVOX_SIMPLEX)
+        return set_err(PSIVOX_E_MODE, "psivox_voxels_noalloc: refinement level 0 only");
+    GridInfo g;
+    if ((rc = fill_grid_info(grid, o->xv_true_covariance, g))) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(PSIVOX_E_NODEV, "no GPU visible");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if ((rc = upload_tables(dev))) return rc;
+    char* base = nullptr;
+    if ((rc = scratch_address(dev, &base))) return rc;
+    hipStream_t s = (hipStream_t)o->stream;
+
+    Workspace& W = g_ws_noalloc;
+    W = Workspace{};
+    W.timing = false;
+    Arena A{base, NOALLOC_SCRATCH, 0};
+    ArenaScope scope(&A);
+    const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    if ((rc = W.acc.get((size_t)nvox * g.nacc * sizeof(double), dev))) return rc;
+    if ((rc = W.counters.get(64, dev)) || (rc = W.connchk.get(sizeof(unsigned), dev)) ||
+        (rc = W.cut.get(4 * sizeof(int64_t), dev)))
+        return rc;
+    /* element batch: tet records, per-tet counts, starts and residual
+     * slots in up to two fifths of the rest */
+    const int per = mesh->elemtype == PSIVOX_SIMPLEX ? 1 : 6;
+    const size_t per_tet = TREC * sizeof(double) + 3 * sizeof(int64_t);
+    const int64_t nt_room = (int64_t)((A.size - A.off) * 2 / 5 / per_tet);
+    int64_t ne_batch = std::max<int64_t>(1, nt_room / per);
+    if (mesh->nelem > 0) ne_batch = std::min<int64_t>(ne_batch, mesh->nelem);
+    const int64_t nt = ne_batch * per;
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (const int64_t*)nullptr, (int64_t*)nullptr, nt + 1, s));
+    if ((rc = W.tets.get((size_t)nt * TREC * sizeof(double), dev)) ||
+        (rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev)) ||
+        (rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev)) ||
+        (rc = W.tetb.get((size_t)nt * sizeof(int64_t), dev)) || (rc = W.scan_tmp.get(tb, dev)))
+        return rc;
+    /* a chunk of column tasks: task words, mass sums, largest-fragment
+     * keys and moments, processing order */
+    const size_t per_task = 3 * sizeof(uint64_t) + sizeof(uint32_t) + (g.order > 0 ? 16 * sizeof(float) : 0);
+    const size_t left = A.size - A.off;
+    const int64_t cap = left > 8192 ? (int64_t)((left - 8192) / per_task) : 0;
+    if (cap < 4096)
+        return set_err(PSIVOX_E_NOMEM, "psivox_voxels_noalloc: the grid's accumulator (%zu bytes) leaves too little of the "
+                                       "%zu bytes of scratch", (size_t)nvox * g.nacc * sizeof(double), A.size);
+    W.task_cap = std::min<int64_t>(cap, (int64_t)1 << 28);
+    if ((rc = W.tasks.get((size_t)W.task_cap * sizeof(uint64_t), dev)) ||
+        (rc = W.colsum.get((size_t)W.task_cap * sizeof(double), dev)) ||
+        (rc = W.colkey.get((size_t)W.task_cap * sizeof(uint64_t), dev)) ||
+        (g.order > 0 && (rc = W.colmom.get((size_t)W.task_cap * 16 * sizeof(float), dev))) ||
+        (rc = W.perm.get((size_t)W.task_cap * sizeof(uint32_t) + 256, dev)))
+        return rc;
+    psivox_opts ob = *o;
+    ob.batch_elements = ne_batch;
+    return run_one(W, mesh, grid, &ob, st, s, dev);
 }
 
 int prepare(Workspace& W, const psivox_opts* o, hipStream_t* s, int* dev) {
@@ -2622,6 +2763,15 @@ int psivox_voxels(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts*
     int rc = prepare(g_ws, &o, &s, &dev);
     if (rc) return rc;
     return run_one(g_ws, mesh, grid, &o, stats, s, dev);
+}
+
+int psivox_voxels_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* opts, psivox_stats* stats) {
+    if (!mesh || !grid) return set_err(PSIVOX_E_ARG, "mesh and grid are required");
+    psivox_opts o{};
+    o.refine_tolerance = 1.0;
+    if (opts) o = *opts;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    return run_noalloc(mesh, grid, &o, stats);
 }
 
 int psivox_voxels_batched(int32_t nmesh, const psivox_mesh* meshes, psivox_grid* grids, const psivox_opts* opts,

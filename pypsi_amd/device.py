@@ -61,6 +61,7 @@ def lib():
         P = C.c_void_p
         L.psivox_voxels.argtypes = [P, P, P, P]
         L.psivox_voxels_batched.argtypes = [C.c_int32, P, P, P, P]
+        L.psivox_voxels_noalloc.argtypes = [P, P, P, P]
         L.psivox_mass_adjoint.argtypes = [P, P, P, P, P]
         L.psivox_last_error.restype = C.c_char_p
         L.psivox_device_count.restype = C.c_int
@@ -74,7 +75,7 @@ def lib():
     return _lib
 
 
-EXPORTS = ["psivox_voxels", "psivox_voxels_batched", "psivox_mass_adjoint", "psivox_last_error",
+EXPORTS = ["psivox_voxels", "psivox_voxels_batched", "psivox_voxels_noalloc", "psivox_mass_adjoint", "psivox_last_error",
            "psivox_abi_version", "psivox_device_count", "psivox_elem_index_build", "psivox_elem_index_free",
            "psivox_elem_index_query", "psivox_sample_vdf"]
 
