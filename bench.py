@@ -253,16 +253,39 @@ def vdf_leg(pos, vel, npts=400):
            "tree_build_ms": build_ms, "us_per_point": us, "points_per_s": 1e6 / us,
            "streams_per_point": streams / npts,
            "what": "PSI.VDF(mesh, x, tree) from host: box scan, candidate refinement and barycentric tests on the GPU"}
+    # against the reference's own path: its R*-tree built once, then
+    # psi_sample_vdf through it (as its PSI.VDF(mesh, x, tree=t)); on the
+    # 64^3 mesh of the same construction (the reference's tree over 128^3
+    # takes some 40 s to build), the GPU on the same mesh and points
     try:
         import oracle as O
-        P = O.Problem.block(pos.reshape(n, n, n, 3), vel.reshape(n, n, n, 3), periodic_box=box, n=(8,) * 3)
-        W = O.tree_window(P)
-        t0 = time.perf_counter()
-        for x in pts[:5]:
-            O.sample_vdf(P, x, window=W)
-        out["cpu_oracle_us_per_point"] = 1e6 * (time.perf_counter() - t0) / 5
+        from pypsi_amd.synth import zeldovich_block
+        if O.have_ref():
+            p64, v64 = zeldovich_block(64, 2.0, 0)
+            P = O.Problem.block(p64, v64, periodic_box=box, n=(8,) * 3)
+            m64 = PSI.Mesh(loader="block", pos=p64, vel=v64, periodic_box=box)
+            t0 = time.perf_counter()
+            t64 = PSI.RStarTree(m64)
+            gbuild = 1e3 * (time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            for x in pts[:200]:
+                PSI.VDF(m64, tuple(x), tree=t64)
+            gus = 1e6 * (time.perf_counter() - t0) / 200
+            t0 = time.perf_counter()
+            T = O.RefTree(P, O.tree_window(P))
+            rbuild = 1e3 * (time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            for x in pts[:200]:
+                T.sample_vdf(x)
+            rus = 1e6 * (time.perf_counter() - t0) / 200
+            T.close()
+            out["vs_reference_64^3"] = {"gpu_tree_build_ms": gbuild, "gpu_us_per_point": gus,
+                                         "cpu_reference_tree_build_ms": rbuild, "cpu_reference_us_per_point": rus,
+                                         "cpu_reference_cores": 1,
+                                         "what": "200 points, refine level 0; the reference's R*-tree and "
+                                                 "psi_sample_vdf single-threaded"}
     except Exception as e:  # report, never hide
-        out["cpu_oracle_error"] = repr(e)
+        out["cpu_reference_error"] = repr(e)
     return out
 
 
@@ -373,7 +396,7 @@ def build_refined(wname):
     return pos.reshape(-1, 3), vel.reshape(-1, 3), np.ones(conn.shape[0]), conn, gn, fields, lvl, tol
 
 
-def refined_leg(wname, steps=2, warmup=1, kernel="fast"):
+def refined_leg(wname, steps=2, warmup=1, kernel="auto", cpu_sample_elements=0):
     """The refined workload on the GPU: on-device level-2 refinement then
     the column kernel, fields m, v, vv on a 512^3 FP64 grid."""
     import torch
@@ -410,12 +433,21 @@ def refined_leg(wname, steps=2, warmup=1, kernel="fast"):
     rec = load_counts().get(wname, {})
     nfrag = rec.get("nfrag")
     src = rec.get("source", "oracle")
+    cpu = None
+    if cpu_sample_elements:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        e1 = cpu_sample_elements
+        P = O.Problem(pos, vel, mass[:e1], conn[:e1], periodic_box=box, n=(gn,) * 3, bounds=box, fields=fields)
+        cpu = cpu_sample([P], reftol=tol, maxlvl=lvl,
+                         what=f"first {e1} of {conn.shape[0]} elements, refinement level {lvl} (tolerance {tol} cells), "
+                              f"onto the full {gn}^3 grid, fields m,{','.join(fields)}; single-threaded")
     if nfrag is None:
         nfrag, src = int(st["nfrag"]), "gpu (count not cached)"
     return {"workload": wname, "value": nfrag / (ms * 1e-3), "unit": "fragments/s", "ms_per_step": ms,
             "kernel_ms_avg": sum(kms) / steps, "tets": int(st["ntets"]), "nfrag": nfrag, "nfrag_source": src,
             "gpu_nfrag": int(st["nfrag"]), "mass_rel_err": mass_err, "kernel": ["auto", "plain", "fast"][st["kernel"]],
-            "steps": steps}
+            "wide_tets": int(st["nwide"]), "steps": steps, "cpu_baseline": cpu}
 
 
 def count_fragments(wname):
@@ -432,7 +464,7 @@ def count_fragments(wname):
     print(json.dumps(data[wname]), flush=True)
 
 
-def batched_leg(steps=3, warmup=1):
+def batched_leg(steps=3, warmup=1, cpu=True):
     """64 meshes (seeds 0..63): lattice n log-uniform in 16..96, rms uniform
     in 0.2..3.0 cells, each onto its own grid 2^round(log2(8n/3)) clipped to
     32..256, mass only.  One psivox_voxels_batched call per step, beside a
@@ -444,7 +476,7 @@ def batched_leg(steps=3, warmup=1):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
     dev = torch.device("cuda:0")
-    meshes, grids, fields = [], [], []
+    meshes, grids, fields, host = [], [], [], []
     ntet = 0
     for n, rms, gn, seed in uneven_batch_specs(64):
         pos, _ = zeldovich_block(n, rms, seed)
@@ -455,6 +487,7 @@ def batched_leg(steps=3, warmup=1):
                 for kk in range(2):
                     conn[..., 4 * ii + 2 * jj + kk] = n * n * ((i + ii) % n) + n * ((j + jj) % n) + ((k + kk) % n)
         conn = conn.reshape(-1, 8)
+        host.append((pos, conn))
         tp = torch.from_numpy(pos.reshape(-1, 3)).to(dev)
         tc = torch.from_numpy(conn).to(dev)
         tm = torch.ones(conn.shape[0], dtype=torch.float64, device=dev)
@@ -485,6 +518,17 @@ def batched_leg(steps=3, warmup=1):
         res[label] = 1e3 * (time.perf_counter() - t0) / steps
         res[label + "_gpu_nfrag"] = int(sum(st["nfrag"] for st in sts))
     mass_err = max(abs(float(m.sum().item()) / float(meshes[q].nelem) - 1.0) for q, m in enumerate(fields))
+    # CPU: the reference's C core on the first 1% of every mesh's elements
+    import oracle as O
+    probs = []
+    for (n, rms, gn, seed), (pos, conn) in zip(uneven_batch_specs(64), host):
+        e1 = max(1, conn.shape[0] // 100)
+        probs.append(O.Problem(pos.reshape(-1, 3), None, None, conn[:e1], periodic_box=box, n=(gn,) * 3, bounds=box))
+    if cpu:
+        cpu = cpu_sample(probs, what="the first 1% of the elements of each of the 64 meshes, each onto its own grid, "
+                                     "mass; single-threaded, -O3 x86-64-v3")
+    else:
+        cpu = None
     nfrag = load_counts().get(UNEVEN, {}).get("nfrag")
     src = "oracle"
     if nfrag is None:
@@ -492,7 +536,120 @@ def batched_leg(steps=3, warmup=1):
     return {"workload": UNEVEN, "meshes": 64, "tets": ntet, "nfrag": nfrag, "nfrag_source": src,
             "value": nfrag / (res["batched"] * 1e-3), "unit": "fragments/s", "ms_per_step": res["batched"],
             "loop_ms_per_step": res["loop"], "loop_value": nfrag / (res["loop"] * 1e-3),
-            "gpu_nfrag": res["batched_gpu_nfrag"], "max_mass_rel_err": mass_err, "steps": steps}
+            "gpu_nfrag": res["batched_gpu_nfrag"], "max_mass_rel_err": mass_err, "steps": steps,
+            "cpu_baseline": cpu}
+
+
+def parity_leg(pos, vel, mass, conn, gn, fields, nsub=1 << 19):
+    """Per-voxel errors at the headline configuration: the first nsub
+    elements of the mesh (lattice x-slabs 0..31 of 128: 3.1M tets) onto the
+    full grid, every field, each GPU kernel against the FP64 oracle (the
+    errors of tests/helpers.py: relative on cells holding >= 1e-3 of the
+    peak mass, mass-weighted absolute over all cells)."""
+    import torch
+    from pypsi_amd import device as D
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+    from helpers import field_errors
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    P = O.Problem(pos, vel, mass[:nsub], conn[:nsub], periodic_box=box, n=(gn,) * 3, bounds=box, fields=fields)
+    t0 = time.perf_counter()
+    ref, ost = O.voxels(P, threads=-1)
+    oracle_s = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    mesh = D.make_mesh(torch.from_numpy(pos).to(dev), torch.from_numpy(vel).to(dev),
+                       torch.from_numpy(mass[:nsub]).to(dev), torch.from_numpy(conn[:nsub]).to(dev), periodic_box=box)
+    out = {"sample": f"first {nsub} of {conn.shape[0]} elements ({6 * nsub} tets) onto the full {gn}^3 grid",
+           "oracle_nfrag": ost["nfrag"], "oracle_s": oracle_s}
+    for kernel in ("auto", "plain"):
+        tf = {"m": torch.zeros((gn,) * 3, dtype=torch.float64, device=dev)}
+        for f in fields:
+            tf[f] = torch.zeros((gn,) * 3 + ((3,) if len(f) == 1 else (3, 3)), dtype=torch.float64, device=dev)
+        st = D.voxels(mesh, D.make_grid((gn,) * 3, box, tf), D.make_opts(kernel=kernel))
+        got = {k: v.cpu().numpy() for k, v in tf.items()}
+        errs = field_errors(got, ref)
+        out[kernel] = {"errors": {k: {"rel": v[0], "wabs": v[1]} for k, v in errs.items()},
+                       "mass_rel_err": abs(1.0 - float(got["m"].sum()) / float(ref["m"].sum())),
+                       "gpu_nfrag": int(st["nfrag"]), "wide_tets": int(st["nwide"])}
+        del tf, got
+    return out
+
+
+C1 = "zeldovich-32^3-rms0.3-to-64^3-m"
+
+
+def c1_leg(steps=20, warmup=3):
+    """The reference's own CPU-runnable case: 32^3 particles (rms 0.3 cells)
+    onto 64^3, mass.  The GPU kernels from device-resident inputs, beside the
+    reference's C core on the whole mesh single-threaded and the oracle on
+    all host cores."""
+    import torch
+    from pypsi_amd import device as D
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    pos, vel, mass, conn, gn, fields = build_inputs(C1)
+    nfrag = load_counts().get(C1, {}).get("nfrag")
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    dev = torch.device("cuda:0")
+    t = [torch.from_numpy(a).to(dev) for a in (pos, vel, mass, conn)]
+    mesh = D.make_mesh(*t, periodic_box=box)
+    m = torch.zeros((gn,) * 3, dtype=torch.float64, device=dev)
+    grid = D.make_grid((gn,) * 3, box, {"m": m})
+    out = {"workload": C1, "nfrag": nfrag}
+    for kernel in ("auto", "plain"):
+        opts = D.make_opts(kernel=kernel, stream=torch.cuda.current_stream().cuda_stream)
+        for _ in range(warmup):
+            m.zero_()
+            st = D.voxels(mesh, grid, opts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.zero_()
+            st = D.voxels(mesh, grid, opts)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        n = nfrag or int(st["nfrag"])
+        out[kernel] = {"value": n / (ms * 1e-3), "unit": "fragments/s", "ms_per_step": ms, "gpu_nfrag": int(st["nfrag"])}
+    P = O.Problem(pos, vel, mass, conn, periodic_box=box, n=(gn,) * 3, bounds=box)
+    if O.have_ref():
+        t0 = time.perf_counter()
+        O.ref_voxels(P)
+        dt = time.perf_counter() - t0
+        out["cpu_reference"] = {"value": (nfrag or 0) / dt, "unit": "fragments/s", "cores": 1, "kind": "reference",
+                                "s": dt, "sample": "the whole mesh"}
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    O.voxels(P, threads=nth)
+    dt = time.perf_counter() - t0
+    out["cpu_oracle_all_cores"] = {"value": (nfrag or 0) / dt, "unit": "fragments/s", "cores": nth, "kind": "port",
+                                   "s": dt, "sample": "the whole mesh"}
+    return out
+
+
+def cpu_sample(probs, reftol=1.0, maxlvl=0, what=""):
+    """The reference's C core single-threaded over a list of oracle Problems
+    (a bounded sample of a workload), fragments counted by the oracle on
+    all host cores: {value fragments/s, ...}."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    kind = "reference" if O.have_ref() else "port"
+    dt, nfrag = 0.0, 0
+    for P in probs:
+        out = P.zero_fields()
+        t0 = time.perf_counter()
+        if kind == "reference":
+            O.ref_voxels(P, out=out, reftol=reftol, maxlvl=maxlvl)
+        else:
+            O.voxels(P, out=out, reftol=reftol, maxlvl=maxlvl)
+        dt += time.perf_counter() - t0
+        del out
+        Pm = O.Problem(P.pos, P.vel, P.mass, P.conn, periodic_box=P.box if P.periodic else None, n=P.n,
+                       bounds=P.bounds)
+        _, st = O.voxels(Pm, reftol=reftol, maxlvl=maxlvl, threads=-1)
+        nfrag += st["nfrag"]
+    return {"value": nfrag / dt, "unit": "fragments/s", "cores": 1, "kind": kind, "sample": what,
+            "sample_fragments": nfrag, "s": dt}
 
 
 def fwd_adjoint_leg(steps=5, warmup=2):
@@ -549,6 +706,7 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "plain", "fast"])
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true", help="skip the per-voxel comparison with the oracle")
     ap.add_argument("--count-fragments", choices=sorted(REFINED),
                     help="count a refined workload's fragments with the FP64 kernel into profiles/ and exit")
     ap.add_argument("--no-extra", action="store_true",
@@ -659,15 +817,25 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "flops_per_fragment": FLOPS_PER_FRAG[fk]},
     }
+    if not args.no_parity:
+        log("[bench] per-voxel errors against the oracle on a sample of the same workload")
+        try:
+            par = parity_leg(pos, vel, mass, conn, gn, fields)
+            out["parity"] = par
+            out["config"]["max_rel_err"] = {f: e["rel"] for f, e in par["auto"]["errors"].items()}
+        except Exception as e:  # report, never hide
+            out["parity"] = {"error": repr(e)}
     if not args.no_extra:
+        log("[bench] the reference's CPU-runnable case, 32^3 -> 64^3")
+        out["c1"] = c1_leg()
         log("[bench] plain FP64 kernel on the same workload")
         out["plain"] = plain_leg(mesh, grid, tf, stream, nfrag)
         log("[bench] forward + adjoint, 64^3 -> 128^3")
         out["fwd_adjoint"] = fwd_adjoint_leg()
         log("[bench] batched uneven meshes")
-        out["batched_uneven"] = batched_leg()
+        out["batched_uneven"] = batched_leg(cpu=not args.no_cpu_baseline)
         log("[bench] refined clumps, level 2, 512^3")
-        out["refined"] = refined_leg(next(iter(REFINED)))
+        out["refined"] = refined_leg(next(iter(REFINED)), cpu_sample_elements=0 if args.no_cpu_baseline else 9000)
         log("[bench] end to end from host arrays")
         out["end_to_end"] = end_to_end_leg(pos, vel, mass, conn, gn, fields, nfrag)
         log("[bench] gadget2 snapshot, Example 2 settings")

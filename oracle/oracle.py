@@ -475,3 +475,44 @@ def ref_tree_query(prob: Problem, window, q):
         out.append(e.value)
     L.psi_rtree_destroy(tree)
     return np.array(out, dtype=np.int32)
+
+
+class RefTree:
+    """The reference's R*-tree (src/rtree.c:48-73) built once over a mesh,
+    then psi_sample_vdf (src/cpsi.c:177-275) answered through it, as the
+    reference's PSI.VDF(mesh, x, tree=t) does (src/PSI.c:1022-1058): used
+    to time the reference's own query path."""
+
+    def __init__(self, prob: Problem, window):
+        L = reflib()
+        P = C.c_void_p
+        L.psi_rtree_from_mesh.argtypes = [P, P, P]
+        L.psi_rtree_destroy.argtypes = [P]
+        L.psi_sample_vdf.argtypes = [_RVec, P, P, P, P, P, C.c_double, C.c_int]
+        L.psi_sample_vdf.restype = None
+        self._L = L
+        self._prob = prob
+        self._m = _rmesh(prob)
+        self._tree = (C.c_char * 256)()  # psi_rtree, include/rtree.h:53-60 (32 bytes)
+        W = (_RVec * 2)(_RVec(*window[0]), _RVec(*window[1]))
+        with _silence_stdout():
+            L.psi_rtree_from_mesh(self._tree, C.byref(self._m), W)
+        libc = C.CDLL(None)
+        libc.free.argtypes = [P]
+        self._free = libc.free
+
+    def sample_vdf(self, x, reftol=1.0, maxlvl=0):
+        rp, vp = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+        n = C.c_int(0)
+        self._L.psi_sample_vdf(_RVec(*[float(v) for v in x]), C.byref(self._m), self._tree, C.byref(rp), C.byref(vp),
+                               C.byref(n), reftol, maxlvl)
+        k = n.value
+        rho = np.array([rp[i] for i in range(k)])
+        self._free(C.cast(rp, C.c_void_p))
+        self._free(C.cast(vp, C.c_void_p))
+        return rho
+
+    def close(self):
+        if self._tree is not None:
+            self._L.psi_rtree_destroy(self._tree)
+            self._tree = None

@@ -551,6 +551,32 @@ __global__ __launch_bounds__(256) void k_col_emit(const double* __restrict__ tet
     });
 }
 
+/* Copies the records of the wide tets (tet_wide) among [0, ntet) to wide[]
+ * and counts them in *n, so that the FP64 column pass touches only them
+ * (order across wavefronts arbitrary, within a wavefront kept). */
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_wide_gather(const double* __restrict__ tets, int64_t ntet, GridInfo g0,
+                                                     MeshBox mb0, const BatchGeo* table, double* __restrict__ wide,
+                                                     unsigned long long* n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double* r = tets + (int64_t)TREC * (t < ntet ? t : 0);
+    bool w = false;
+    if (t < ntet && r[25] != 0.0)
+        with_mesh<BATCH>(r, g0, mb0, nullptr, table,
+                         [&](const GridInfo& g, const MeshBox&, double*, int) { w = tet_wide(r, g); });
+    const uint64_t m = __ballot(w);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & (warpSize - 1);
+    const int first = __ffsll((long long)m) - 1;
+    long long base = 0;
+    if (lane == first) base = (long long)atomicAdd(n, (unsigned long long)__popcll(m));
+    base = __shfl(base, first);
+    if (w) {
+        double* o = wide + (int64_t)TREC * (base + __popcll(m & ((1ull << lane) - 1ull)));
+        for (int q = 0; q < TREC; ++q) o[q] = r[q];
+    }
+}
+
 /* per-task outputs for the residual pass: FP64 fragment-mass sum, the
  * largest fragment's key and (ORDER > 0) its moments and origin */
 struct TaskOut {
@@ -1222,6 +1248,8 @@ struct Workspace {
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf wtets;                           /* the wide tets' records for the FP64 column pass */
+    bool compact_wide = true;               /* the FP64 pass on a copy of the wide tets' records */
     bool timing = true;                     /* kernel_ms from events (off: no events, kernel_ms = 0) */
     int64_t task_cap = (int64_t)1 << 28;    /* column tasks per chunk */
     hipStream_t side = nullptr; /* the field upload beside the kernels */
@@ -1604,6 +1632,29 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
     return 0;
 }
 
+/* The tet records of a column pass: for PASS_WIDE (when W compacts them)
+ * a copy of the nwide wide tets among tets[0, nt), in *pt, *pn; otherwise
+ * tets and nt themselves.  counters[3] counts the copies. */
+int pass_tets(Workspace& W, int pass, const double* tets, int64_t nt, const PassGeo& G, int64_t nwide,
+              unsigned long long* counters, hipStream_t s, int dev, const double** pt, int64_t* pn) {
+    *pt = tets;
+    *pn = nt;
+    if (pass != PASS_WIDE || !W.compact_wide) return 0;
+    int rc = W.wtets.get((size_t)nwide * TREC * sizeof(double), dev);
+    if (rc) return rc;
+    k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)(counters + 3), 2, 0u);
+    if (G.table)
+        k_wide_gather<true><<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, (double*)W.wtets.p,
+                                                                 counters + 3);
+    else
+        k_wide_gather<false><<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, G.g, G.mb, nullptr, (double*)W.wtets.p,
+                                                                  counters + 3);
+    HIPCHK(hipGetLastError());
+    *pt = (const double*)W.wtets.p;
+    *pn = nwide;
+    return 0;
+}
+
 /* One (mesh, grid) voxelization on stream s with workspace W. */
 /* exclusive sum of cnt[0..n) into off[0..n] (cnt[n] is set to 0 first);
  * returns off[n] on the host */
@@ -1969,8 +2020,11 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 int64_t nwide = 0;
                 for (int pass = route ? PASS_NARROW : PASS_ALL;; pass = PASS_WIDE) {
                     bool fell = false;
+                    const double* ptets;
+                    int64_t pnt;
+                    if ((rc = pass_tets(W, pass, tets, nt, G, nwide, counters, s, dev, &ptets, &pnt))) return rc;
                     rc = for_task_chunks(
-                        W, tets, nt, G, pass, status, counters, s, dev, &fell,
+                        W, ptets, pnt, G, pass, status, counters, s, dev, &fell,
                         pass == PASS_NARROW ? &nwide : nullptr,
                         [&](int64_t a0, int64_t a1, const int64_t* colstart, int64_t base, const uint64_t* tasks,
                             int64_t nsub) -> int {
@@ -1981,7 +2035,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                                 return rc2;
                             if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
                             TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
-                            rc2 = launch_cols(tets, tasks, nsub, G, pass == PASS_WIDE, out, (uint32_t*)W.perm.p,
+                            rc2 = launch_cols(ptets, tasks, nsub, G, pass == PASS_WIDE, out, (uint32_t*)W.perm.p,
                                               perm_counter(W.perm, nsub), status, counters, s,
                                               W.timing ? W.ev0 : nullptr);
                             if (rc2)
@@ -1995,7 +2049,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                                 kms += ms;
                             }
                             if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
-                            rc2 = launch_residual(tets, a0, a1, G, colstart, base, tasks, nsub, out,
+                            rc2 = launch_residual(ptets, a0, a1, G, colstart, base, tasks, nsub, out,
                                                   (double*)W.colcnt.p, (int64_t*)W.tetb.p, s);
                             if (rc2)
                                 return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
@@ -2346,8 +2400,11 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
         /* one ordering and one column kernel over the tasks of all the
          * group's meshes per chunk (and pass) */
         for (int pass = route ? PASS_NARROW : PASS_ALL;; pass = PASS_WIDE) {
+            const double* ptets;
+            int64_t pnt;
+            if ((rc = pass_tets(W, pass, tets, nt, PG, nwide, counters, s, dev, &ptets, &pnt))) return rc;
             rc = for_task_chunks(
-                W, tets, nt, PG, pass, status, counters, s, dev, &fell, pass == PASS_NARROW ? &nwide : nullptr,
+                W, ptets, pnt, PG, pass, status, counters, s, dev, &fell, pass == PASS_NARROW ? &nwide : nullptr,
                 [&](int64_t a0, int64_t a1, const int64_t* colstart, int64_t base, const uint64_t* tasks,
                     int64_t nsub) -> int {
                     int rc2;
@@ -2357,14 +2414,14 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                         return rc2;
                     if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
                     TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
-                    rc2 = launch_cols(tets, tasks, nsub, PG, pass == PASS_WIDE, out, (uint32_t*)W.perm.p,
+                    rc2 = launch_cols(ptets, tasks, nsub, PG, pass == PASS_WIDE, out, (uint32_t*)W.perm.p,
                                       perm_counter(W.perm, nsub), status, counters, s, W.ev0);
                     if (rc2)
                         return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
                                        hipGetErrorString((hipError_t)rc2));
                     HIPCHK(hipEventRecord(W.ev1, s));
                     if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
-                    rc2 = launch_residual(tets, a0, a1, PG, colstart, base, tasks, nsub, out, (double*)W.colcnt.p,
+                    rc2 = launch_residual(ptets, a0, a1, PG, colstart, base, tasks, nsub, out, (double*)W.colcnt.p,
                                           (int64_t*)W.tetb.p, s);
                     if (rc2)
                         return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
@@ -2508,6 +2565,7 @@ int run_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* o
     Workspace& W = g_ws_noalloc;
     W = Workspace{};
     W.timing = false;
+    W.compact_wide = false; /* the FP64 pass runs in place: its scratch is laid out before the call */
     Arena A{base, NOALLOC_SCRATCH, 0};
     ArenaScope scope(&A);
     const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
