@@ -665,7 +665,7 @@ def fwd_adjoint_leg(steps=5, warmup=2):
     tconn = torch.from_numpy(conn).to(dev)
     tmass = torch.from_numpy(mass).to(dev)
     stream = torch.cuda.current_stream().cuda_stream
-    opts = D.make_opts(kernel="fast", stream=stream)
+    opts = D.make_opts(kernel="auto", stream=stream)
     V = (1.0 / gn) ** 3
     m = torch.zeros((gn,) * 3, dtype=torch.float64, device=dev)
     grid = D.make_grid((gn,) * 3, box, {"m": m})
@@ -693,9 +693,43 @@ def fwd_adjoint_leg(steps=5, warmup=2):
             tf.append(t1 - t0)
             ta.append(t2 - t1)
     f, a = 1e3 * sum(tf) / steps, 1e3 * sum(ta) / steps
-    return {"workload": "zeldovich-64^3-rms1.0-to-128^3-m, loss sum (rho - rho_t)^2, rho_t seed 1",
-            "fwd_ms": f, "adjoint_ms": a, "ratio_fwd_plus_adj_over_fwd": (f + a) / f, "steps": steps,
-            "adjoint_dtype": "f64"}
+    out = {"workload": "zeldovich-64^3-rms1.0-to-128^3-m, loss sum (rho - rho_t)^2, rho_t seed 1",
+           "forward_kernel": "auto (FP32 clip; the default of voxel_density)",
+           "fwd_ms": f, "adjoint_ms": a, "ratio_fwd_plus_adj_over_fwd": (f + a) / f, "steps": steps,
+           "adjoint_dtype": "f64"}
+    # the fields m, v, vv and their adjoint with respect to positions and
+    # velocities (voxel_fields: FP64 column forward)
+    vel = torch.from_numpy(build_inputs("zeldovich-64^3-rms1.0-to-128^3-m")[1]).to(dev)
+    fl = {"m": m, "v": torch.zeros((gn,) * 3 + (3,), dtype=torch.float64, device=dev),
+          "vv": torch.zeros((gn,) * 3 + (3, 3), dtype=torch.float64, device=dev)}
+    fgrid = D.make_grid((gn,) * 3, box, fl)
+    fmesh = D.make_mesh(tpos, vel, tmass, tconn, box)
+    o64 = D.make_opts(kernel="fast64", stream=stream)
+    gr = {k: torch.randn(t.shape, dtype=torch.float64, device=dev, generator=torch.Generator(device=dev).manual_seed(7))
+          for k, t in fl.items()}
+    gv = torch.zeros_like(vel)
+    tf, ta = [], []
+    for it in range(warmup + steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in fl.values():
+            t.zero_()
+        D.voxels(fmesh, fgrid, o64)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g.zero_()
+        gv.zero_()
+        D.adjoint(fmesh, fgrid, gr, g, gv, opts)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if it >= warmup:
+            tf.append(t1 - t0)
+            ta.append(t2 - t1)
+    f, a = 1e3 * sum(tf) / steps, 1e3 * sum(ta) / steps
+    out["fields_m_v_vv"] = {"forward_kernel": "fast64 (FP64 columns; the default of voxel_fields)", "fwd_ms": f,
+                            "adjoint_ms": a, "ratio_fwd_plus_adj_over_fwd": (f + a) / f,
+                            "what": "dL/dpos and dL/dvel of a loss on m, v, vv"}
+    return out
 
 
 def main():

@@ -53,6 +53,7 @@ extern "C" {
                                  4 cells along an axis (their smallest fragments need it) */
 #define PSIVOX_KERNEL_PLAIN 1 /* one thread per tet, FP64 clip */
 #define PSIVOX_KERNEL_FAST 2  /* one lane per (tet, column) task, FP32 cell-local clip for every tet, FP64 deposit */
+#define PSIVOX_KERNEL_FAST64 3 /* the column kernel in FP64 for every tet */
 
 /* error codes */
 #define PSIVOX_OK 0
@@ -161,6 +162,23 @@ int psivox_voxels_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psiv
  * Errors: PSIVOX_E_MODE for refine_max_lvl > 0 on 8/27-node elements. */
 int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,
                         const psivox_opts* opts);
+
+/* Adjoint of the fields m, v and vv.  No reference counterpart (the
+ * forward it differentiates is psi_voxels, src/cpsi.c:41, density branch
+ * :100-112, with the deposit of src/geometry.c:106-186).  grid->fields
+ * holds the forward's m, v and vv (the fields psivox_voxels wrote into
+ * zeroed arrays; v and vv may be NULL when their gradients are).
+ * dl_dfields[6] points to dL/d(m, x, v, xx, xv, vv) in the fields' layouts
+ * (NULL: zero; x, xx and xv must be NULL).  ADDS dL/dpos and dL/dvel
+ * (npart x 3 float64 each; either may be NULL) for refine_max_lvl 0 (or
+ * simplex meshes).  Device pointers follow grid->on_device (fields and
+ * their gradients) and mesh->on_device (dl_dpos, dl_dvel).  Position
+ * gradients: the plane-section integrals of the mass adjoint with the
+ * velocity fields' cubic integrand; velocity gradients: the fragments'
+ * q-moments against each cell's weights.  Errors: PSIVOX_E_MODE for
+ * gradients of x, xx, xv or refinement. */
+int psivox_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* const* dl_dfields, double* dl_dpos,
+                   double* dl_dvel, const psivox_opts* opts);
 
 /* ---- point queries: RStarTree and PSI.VDF ---- */
 

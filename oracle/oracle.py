@@ -64,6 +64,8 @@ def lib():
         L.orc_voxelize_one_tet.argtypes = [P, P, P, P, C.c_double, C.c_int, P]
         L.orc_mass_adjoint.argtypes = [P, P, P, P]
         L.orc_fd_mass_loss.argtypes = [P, P, P, P, C.c_int, C.c_int64, C.c_int, C.c_double, P]
+        L.orc_adjoint.argtypes = [P, P, P, P, P]
+        L.orc_fd_fields_loss.argtypes = [P, P, P, P, P, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_double, P]
         L.orc_annihilation.argtypes = [P, P, C.c_double, C.c_int, P]
         L.orc_elem_query.argtypes = [P, P, P, P, C.c_int64]
         L.orc_elem_query.restype = C.c_int64
@@ -219,6 +221,87 @@ def fd_mass_loss_grad(prob: Problem, r, particles, axes, h=1e-6, csr=None):
                                 int(a), float(h), C.byref(val))
         if rc != 0:
             raise RuntimeError("oracle finite difference failed")
+        out[i] = val.value
+    return out
+
+
+def adjoint_weights(fields, grads):
+    """Per-cell weights dL/d(M, Sv, Svv), (nvox, 13), from the forward's
+    fields {'m', 'v', 'vv'} and the gradients {'m', 'v', 'vv'} of a loss
+    with respect to them (m = M, v = Sv / M, vv = Svv / M - v v^T; a cell
+    without mass keeps dL/dM = dL/dm only)."""
+    m = fields["m"].reshape(-1)
+    n = m.size
+    W = np.zeros((n, 13))
+    gm = grads.get("m")
+    if gm is not None:
+        W[:, 0] = gm.reshape(-1)
+    ok = m != 0
+    iM = np.where(ok, 1.0 / np.where(ok, m, 1.0), 0.0)
+    v = fields["v"].reshape(n, 3) if "v" in fields else np.zeros((n, 3))
+    gv = grads.get("v")
+    if gv is not None:
+        gv = gv.reshape(n, 3)
+        W[:, 1:4] += gv * iM[:, None]
+        W[:, 0] -= np.sum(gv * v, axis=1) * iM
+    gvv = grads.get("vv")
+    if gvv is not None:
+        G = gvv.reshape(n, 3, 3)
+        vv = fields["vv"].reshape(n, 3, 3)
+        W[:, 4:] = (G * iM[:, None, None]).reshape(n, 9)
+        W[:, 1:4] -= np.einsum("nab,nb->na", G + np.swapaxes(G, 1, 2), v) * iM[:, None]
+        W[:, 0] += np.einsum("nab,nab->n", G, v[:, :, None] * v[:, None, :] - vv) * iM
+    W[~ok, 1:] = 0.0
+    return W
+
+
+def adjoint(prob: Problem, W):
+    """(dL/dpos, dL/dvel) of the fields m, v, vv from the cells' weights
+    W (nvox, 13), FP64, refine level 0."""
+    L = lib()
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    dpos = np.zeros((prob.pos.shape[0], 3))
+    dvel = np.zeros((prob.pos.shape[0], 3))
+    m, g = prob._omesh(), prob._ogrid(prob.zero_fields())
+    if L.orc_adjoint(C.byref(m), C.byref(g), W.ctypes.data, dpos.ctypes.data, dvel.ctypes.data) != 0:
+        raise RuntimeError("oracle adjoint failed")
+    return dpos, dvel
+
+
+def raw_sums(fields):
+    """(nvox, 10) additive sums M, Sv[3], Svv (xx xy xz yy yz zz) of the
+    finalized fields m, v, vv."""
+    m = fields["m"].reshape(-1)
+    n = m.size
+    v = fields["v"].reshape(n, 3)
+    vv = fields["vv"].reshape(n, 3, 3)
+    S2 = (vv + v[:, :, None] * v[:, None, :]) * m[:, None, None]
+    out = np.empty((n, 10))
+    out[:, 0] = m
+    out[:, 1:4] = v * m[:, None]
+    out[:, 4:] = S2[:, [0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]]
+    return out
+
+
+def fd_fields_loss_grad(prob: Problem, base, G, particles, axes, which, h=1e-6, csr=None):
+    """Central differences of sum_C G_m m + G_v . v + G_vv : vv (G: (nvox,
+    13) coefficients; base: raw_sums of the unperturbed fields) for the
+    position (which 0) or velocity (which 1) components (particles[i],
+    axes[i])."""
+    L = lib()
+    start, elems = csr if csr is not None else particle_elements(prob)
+    base = np.ascontiguousarray(base, dtype=np.float64)
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    m, g = prob._omesh(), prob._ogrid(prob.zero_fields())
+    out = np.empty(len(particles))
+    val = C.c_double()
+    for i, (p, a) in enumerate(zip(particles, axes)):
+        el = np.ascontiguousarray(elems[start[p]:start[p + 1]])
+        rc = L.orc_fd_fields_loss(C.byref(m), C.byref(g), base.ctypes.data, G.ctypes.data, el.ctypes.data, len(el),
+                                  C.c_int64(int(p)), C.c_int(int(a)), C.c_int(int(which)), C.c_double(h),
+                                  C.byref(val))
+        if rc:
+            raise RuntimeError("oracle finite differences failed")
         out[i] = val.value
     return out
 

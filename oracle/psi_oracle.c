@@ -1090,6 +1090,42 @@ int64_t orc_internal_tet_fragments(const orc_mesh* mesh, const orc_grid* g, cons
     return ctx.n;
 }
 
+/* (voxel index, q-moments m[10] of the fragment: m0, first and second
+ * moments as psi_reduce, src/geometry.c:692-767) of the fragments of one
+ * tet copy x with bounding box rb; returns the count (may exceed cap) */
+typedef struct {
+    int64_t cap, n;
+    int64_t* gi;
+    double* m10;
+} o_mom_ctx;
+
+static void o_mom_fn(void* vctx, int64_t gi, const int* ijk, const double* m, const opoly* P) {
+    (void)ijk;
+    (void)P;
+    o_mom_ctx* c = (o_mom_ctx*)vctx;
+    if (c->n < c->cap) {
+        c->gi[c->n] = gi;
+        memcpy(c->m10 + 10 * c->n, m, 10 * sizeof(double));
+    }
+    c->n++;
+}
+
+int64_t orc_internal_tet_moments(const orc_mesh* mesh, const orc_grid* g, const double (*x)[3], const double (*rb)[3],
+                                 int64_t* gi, double* m10, int64_t cap) {
+    (void)mesh;
+    orc_stats st;
+    memset(&st, 0, sizeof st);
+    oscratch* S = (oscratch*)malloc(sizeof(oscratch));
+    if (!S) return -1;
+    o_scratch_init(S);
+    o_mom_ctx ctx = {cap, 0, gi, m10};
+    const double u[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    o_voxelize_tet(S, (orc_grid*)g, x, u, 1.0, rb, &st, 0, 2, o_mom_fn, &ctx);
+    o_scratch_free(S);
+    free(S);
+    return ctx.n;
+}
+
 /* ------------------------------------------------------------------ */
 /* point queries: element boxes (src/rtree.c:48-123) and the           */
 /* multistream sample (src/cpsi.c:177-275)                             */

@@ -65,6 +65,13 @@ int orc_annihilation(const orc_mesh* mesh, orc_grid* g, double reftol, int maxlv
 /* mass adjoint (psi_adjoint.c), refine_max_lvl = 0: dpos = dL/dpos
  * (npart x 3, overwritten) from w = dL/dm per cell */
 int orc_mass_adjoint(const orc_mesh* mesh, const orc_grid* g, const double* w, double* dpos);
+/* adjoint of the fields m, v, vv: dpos, dvel (npart x 3, overwritten, either NULL) from the
+ * cells' weights W (13 per cell: dL/dM, dL/dSv[3], dL/dSvv[3][3]); refine level 0; serial */
+int orc_adjoint(const orc_mesh* mesh, const orc_grid* g, const double* W, double* dpos, double* dvel);
+/* central difference of sum_C G_m m + G_v . v + G_vv : vv for a position (which 0) or velocity
+ * (which 1) component of particle p; base: raw sums per cell (M, Sv[3], Svv sym 6) */
+int orc_fd_fields_loss(orc_mesh* mesh, const orc_grid* g, const double* base, const double* G, const int64_t* elems,
+                       int nel, int64_t p, int a, int which, double h, double* out);
 /* central difference (L(x+h) - L(x-h)) / 2h of L = sum (m/V - rho_t)^2 for
  * particle p, axis a; r = m/V - rho_t of the unperturbed mesh; elems are
  * the elements using p */

@@ -9,7 +9,7 @@
  * happens on the GPU through libpsivox (include/psivox.h); this file only
  * validates arguments, owns the NumPy arrays and moves pointers.
  *
- * Differences: voxels() also takes kernel='auto'|'plain'|'fast' and
+ * Differences: voxels() also takes kernel='auto'|'plain'|'fast'|'fast64' and
  * xv_true_covariance=False; non-contiguous input arrays are read correctly
  * (the reference reads their raw buffer); the gadget2/gevolution loaders
  * reject malformed particle ids and non-cubic particle counts;
@@ -756,6 +756,8 @@ static PyObject* PSI_voxels(PyObject* self, PyObject* args, PyObject* kwds) {
         kernel = PSIVOX_KERNEL_PLAIN;
     else if (strcmp(kernstr, "fast") == 0)
         kernel = PSIVOX_KERNEL_FAST;
+    else if (strcmp(kernstr, "fast64") == 0)
+        kernel = PSIVOX_KERNEL_FAST64;
     else {
         PyErr_Format(PyExc_ValueError, "Invalid kernel for PSI.voxels(): %s", kernstr);
         return NULL;

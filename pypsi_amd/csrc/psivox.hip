@@ -491,8 +491,12 @@ __device__ void mesh_count(unsigned long long* mcnt, int id, int64_t v, int slot
 
 /* Column passes.  PASS_ALL: every tet in FP32 (kernel FAST).  Under the
  * automatic choice, PASS_NARROW takes the tets that are not tet_wide in
- * FP32 and counts the wide ones; PASS_WIDE takes only those, in FP64. */
-enum { PASS_ALL = 0, PASS_NARROW = 1, PASS_WIDE = 2 };
+ * FP32 and counts the wide ones; PASS_WIDE takes only those, in FP64.
+ * PASS_ALL64: every tet in FP64 (kernel FAST64, the velocity adjoint). */
+enum { PASS_ALL = 0, PASS_NARROW = 1, PASS_WIDE = 2, PASS_ALL64 = 3 };
+
+/* whether a pass's columns are built in FP64 */
+PVX_HD bool pass_fp64(int pass) { return pass == PASS_WIDE || pass == PASS_ALL64; }
 
 /* column tasks per tet record for one pass (0 for invalid records and for
  * tets of the other pass); a tet the column path cannot take flags the
@@ -509,12 +513,12 @@ __global__ __launch_bounds__(256) void k_col_count(const double* __restrict__ te
         int64_t n = 0;
         if (r[25] != 0.0) {
             with_mesh<BATCH>(r, g0, mb0, nullptr, table, [&](const GridInfo& g, const MeshBox& mb, double*, int m) {
-                const bool wide = PASS != PASS_ALL && tet_wide(r, g);
+                const bool wide = (PASS == PASS_NARROW || PASS == PASS_WIDE) && tet_wide(r, g);
                 if (PASS == PASS_NARROW && wide)
                     nwide = 1;
                 else if (PASS != PASS_WIDE || wide)
-                    n = PASS == PASS_WIDE ? tet_columns<double>(r, t, g, mb, nullptr, &nc)
-                                          : tet_columns<float>(r, t, g, mb, nullptr, &nc);
+                    n = pass_fp64(PASS) ? tet_columns<double>(r, t, g, mb, nullptr, &nc)
+                                        : tet_columns<float>(r, t, g, mb, nullptr, &nc);
                 id = m;
             });
             if (n < 0) {
@@ -1151,15 +1155,53 @@ __global__ __launch_bounds__(256) void k_adj_emit(const double* __restrict__ tet
 }
 
 struct AdjArgs {
-    const double* w; /* dL/dm */
+    const double* w; /* per-cell weights: dL/dm, or ADJ_NW per cell */
     MeshDev md;
     int64_t e0;      /* first element of the batch */
     int per;         /* tets per element */
     double* dpos;    /* dL/dpos, npart x 3 */
+    double* dvel;    /* dL/dvel, npart x 3 */
 };
 
+/* Per cell: the adjoint weights dL/d(M, Sv, Svv) (ADJ_NW values) from the
+ * forward's fields m, v, vv and the upstream gradients gm, gv, gvv (any may
+ * be null: zero).  With m = M, v = Sv / M, vv = Svv / M - v v^T:
+ *   W_Svv = Gvv / M,  W_Sv = Gv / M - (Gvv + Gvv^T) v / M,
+ *   W_M = Gm - Gv . v / M + Gvv : (v v^T - vv) / M;
+ * a cell without mass has only W_M = Gm (its means are not defined). */
+__global__ void k_adj_weights(int64_t nvox, const double* __restrict__ m, const double* __restrict__ v,
+                              const double* __restrict__ vv, const double* __restrict__ gm,
+                              const double* __restrict__ gv, const double* __restrict__ gvv, double* __restrict__ W) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nvox) return;
+    double w[ADJ_NW];
+    for (int q = 0; q < ADJ_NW; ++q) w[q] = 0.0;
+    w[0] = gm ? gm[c] : 0.0;
+    const double M = m[c];
+    if (M != 0.0) {
+        const double iM = 1.0 / M;
+        const double V[3] = {v[3 * c], v[3 * c + 1], v[3 * c + 2]};
+        if (gv)
+            for (int a = 0; a < 3; ++a) {
+                w[1 + a] += gv[3 * c + a] * iM;
+                w[0] -= gv[3 * c + a] * V[a] * iM;
+            }
+        if (gvv)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double G = gvv[9 * c + 3 * a + b];
+                    w[4 + 3 * a + b] = G * iM;
+                    w[1 + a] -= G * V[b] * iM;
+                    w[1 + b] -= G * V[a] * iM;
+                    w[0] += G * (V[a] * V[b] - vv[9 * c + 3 * a + b]) * iM;
+                }
+    }
+    for (int q = 0; q < ADJ_NW; ++q) W[ADJ_NW * c + q] = w[q];
+}
+
 /* one lane per (tet, copy, axis, plane): FP64 section integrals scattered
- * to the tet's particles */
+ * to the tet's particles; FULL: weights of the velocity fields too */
+template <bool FULL>
 __global__ __launch_bounds__(FBLOCK) void k_adj(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
                                                 int64_t ntask, GridInfo g, MeshBox mb, AdjArgs A) {
     __shared__ double ru[8][FBLOCK];
@@ -1173,13 +1215,41 @@ __global__ __launch_bounds__(FBLOCK) void k_adj(const double* __restrict__ tets,
     unpack_adj(w, t, h, ax, p);
     double o[4];
     Poly2<FBLOCK> R{&ru[0][lane], &rv[0][lane]};
-    adj_plane_task<FBLOCK>(g, mb, tets + (int64_t)TREC * t, w, A.w, R, o);
+    adj_plane_task<FBLOCK, FULL>(g, mb, tets + (int64_t)TREC * t, w, A.w, R, o);
     const int64_t e = A.e0 + t / A.per;
     const int tl = (int)(t % A.per);
     for (int v = 0; v < 4; ++v) {
         if (o[v] == 0.0) continue;
         const int64_t id = A.md.conn[e * A.md.vpe + tet_node(A.md.vpe, tl, v)];
         unsafeAtomicAdd(A.dpos + 3 * id + ax, o[v]);
+    }
+}
+
+/* one lane per column task: the velocity part of the adjoint (FP64 column
+ * moments: FP32 ones left 8e-5 of the finite differences' gradient on
+ * small components) scattered to the tet's particles */
+__global__ __launch_bounds__(FBLOCK) void k_adj_vel(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
+                                                    int64_t ntask, GridInfo g, MeshBox mb, AdjArgs A,
+                                                    unsigned* status) {
+    __shared__ __align__(32) uint32_t sv[VRec<double>::WORDS];
+    const int lane = threadIdx.x;
+    const int64_t c = (int64_t)blockIdx.x * FBLOCK + lane;
+    if (c >= ntask) return;
+    LPoly<double> P{(uint32_t*)__builtin_assume_aligned(&sv[VRec<double>::W * lane], 16), 0, 0, 0};
+    const uint64_t w = tasks[c];
+    const int64_t t = (int64_t)(w & 0xffffffffull);
+    double o[4][3];
+    for (int i = 0; i < 4; ++i) o[i][0] = o[i][1] = o[i][2] = 0.0;
+    if (!adj_vel_task<double>(g, mb, tets + (int64_t)TREC * t, w, A.w, P, o)) {
+        atomicOr(status, ST_OVERFLOW);
+        return;
+    }
+    const int64_t e = A.e0 + t / A.per;
+    const int tl = (int)(t % A.per);
+    for (int v = 0; v < 4; ++v) {
+        const int64_t id = A.md.conn[e * A.md.vpe + tet_node(A.md.vpe, tl, v)];
+        for (int b = 0; b < 3; ++b)
+            if (o[v][b] != 0.0) unsafeAtomicAdd(A.dvel + 3 * id + b, o[v][b]);
     }
 }
 
@@ -1244,7 +1314,7 @@ struct Workspace {
     DevBuf connchk, pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
         colsum, colkey, colmom, perm, cut, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt,
         an_box, an_valid, an_key, an_key2, an_id, an_id2, an_pcnt, an_poff, an_pairs, an_cntA, an_offA, an_cntB,
-        an_offB, an_ta, an_tb, an_tcnt, an_toff, an_tasks;
+        an_offB, an_ta, an_tb, an_tcnt, an_toff, an_tasks, adj_in[7], adj_v;
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1539,6 +1609,8 @@ static void launch_col_count(int pass, const double* tets, int64_t nt, const Pas
         k_col_count<BATCH, PASS_NARROW><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
     else if (pass == PASS_WIDE)
         k_col_count<BATCH, PASS_WIDE><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
+    else if (pass == PASS_ALL64)
+        k_col_count<BATCH, PASS_ALL64><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
     else
         k_col_count<BATCH, PASS_ALL><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
 }
@@ -1623,9 +1695,9 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
         }
         if (nsub == 0) continue;
         if (G.table)
-            launch_col_emit<true>(pass == PASS_WIDE, tets, a0, a1, G, colstart, base, tasks, s);
+            launch_col_emit<true>(pass_fp64(pass), tets, a0, a1, G, colstart, base, tasks, s);
         else
-            launch_col_emit<false>(pass == PASS_WIDE, tets, a0, a1, G, colstart, base, tasks, s);
+            launch_col_emit<false>(pass_fp64(pass), tets, a0, a1, G, colstart, base, tasks, s);
         HIPCHK(hipGetLastError());
         if ((rc = fn(a0, a1, (const int64_t*)colstart, base, (const uint64_t*)tasks, nsub))) return rc;
     }
@@ -1870,10 +1942,12 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     if (o->mode == PSIVOX_MODE_ANNIHILATION) return run_annihilation(W, mesh, grid, g, o, st, s, dev);
     if (o->mode != PSIVOX_MODE_DENSITY) return set_err(PSIVOX_E_ARG, "invalid mode %d", o->mode);
     int kernel = o->kernel == PSIVOX_KERNEL_AUTO ? PSIVOX_KERNEL_FAST : o->kernel;
-    if (kernel != PSIVOX_KERNEL_PLAIN && kernel != PSIVOX_KERNEL_FAST)
+    if (kernel != PSIVOX_KERNEL_PLAIN && kernel != PSIVOX_KERNEL_FAST && kernel != PSIVOX_KERNEL_FAST64)
         return set_err(PSIVOX_E_ARG, "invalid kernel %d", o->kernel);
-    /* the automatic choice sends wide tets through the FP64 column kernel */
+    /* the automatic choice sends wide tets through the FP64 column kernel;
+     * FAST64 sends every tet there */
     const bool route = o->kernel == PSIVOX_KERNEL_AUTO;
+    const int first_pass = route ? PASS_NARROW : (kernel == PSIVOX_KERNEL_FAST64 ? PASS_ALL64 : PASS_ALL);
 
     MeshBox mb;
     mb.periodic = mesh->periodic ? 1 : 0;
@@ -2003,7 +2077,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 launch_expand_fixed(md, g, mb, e0, e1 - e0, tets, 0.0, s);
             }
             HIPCHK(hipGetLastError());
-            if (kernel == PSIVOX_KERNEL_FAST && !(mb.periodic && mb.full_window)) {
+            if (kernel != PSIVOX_KERNEL_PLAIN && !(mb.periodic && mb.full_window)) {
                 k_targets<<<grid_blocks(nt, 64), 64, 0, s>>>(tets, nt, g, mb);
                 HIPCHK(hipGetLastError());
             }
@@ -2018,7 +2092,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                  * the status check below restarts with the FP64 kernel) */
                 const PassGeo G{g, mb, acc, nullptr, nullptr};
                 int64_t nwide = 0;
-                for (int pass = route ? PASS_NARROW : PASS_ALL;; pass = PASS_WIDE) {
+                for (int pass = first_pass;; pass = PASS_WIDE) {
                     bool fell = false;
                     const double* ptets;
                     int64_t pnt;
@@ -2035,7 +2109,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                                 return rc2;
                             if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
                             TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
-                            rc2 = launch_cols(ptets, tasks, nsub, G, pass == PASS_WIDE, out, (uint32_t*)W.perm.p,
+                            rc2 = launch_cols(ptets, tasks, nsub, G, pass_fp64(pass), out, (uint32_t*)W.perm.p,
                                               perm_counter(W.perm, nsub), status, counters, s,
                                               W.timing ? W.ev0 : nullptr);
                             if (rc2)
@@ -2069,7 +2143,7 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                 kms += ms;
             }
             ntets_total += nt;
-            if (kernel == PSIVOX_KERNEL_FAST) {
+            if (kernel != PSIVOX_KERNEL_PLAIN) {
                 unsigned hs = 0;
                 HIPCHK(hipMemcpyAsync(&hs, status, sizeof hs, hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
@@ -2128,14 +2202,25 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
     return 0;
 }
 
-/* dL/dpos += adjoint of the mass deposit (refine level 0) */
-int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,
-                const psivox_opts* o, hipStream_t s, int dev) {
+/*
+ * The adjoint of the fields m, v, vv of psivox_voxels (refine level 0):
+ * dl_dpos += dL/dpos, dl_dvel += dL/dvel for the upstream gradients
+ * dl_df[PSIVOX_M, V, VV] (null: zero; x, xx, xv must be null).  With only
+ * dL/dm the weights are dL/dm itself and the velocities get nothing; with
+ * dL/dv or dL/dvv grid->fields must hold the forward's m, v (and vv).
+ */
+int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, const double* const* dl_df,
+                double* dl_dpos, double* dl_dvel, const psivox_opts* o, hipStream_t s, int dev) {
     int rc = check_mesh(mesh);
     if (rc) return rc;
-    if (!dl_dm || !dl_dpos) return set_err(PSIVOX_E_ARG, "dl_dm and dl_dpos are required");
+    if (dl_df[PSIVOX_X] || dl_df[PSIVOX_XX] || dl_df[PSIVOX_XV])
+        return set_err(PSIVOX_E_MODE, "the adjoint takes gradients of m, v and vv only");
+    const bool full = dl_df[PSIVOX_V] || dl_df[PSIVOX_VV];
+    if (!dl_df[PSIVOX_M] && !full) return set_err(PSIVOX_E_ARG, "no field gradient given");
     if (o->refine_max_lvl > 0 && mesh->elemtype != 4)
-        return set_err(PSIVOX_E_MODE, "the mass adjoint supports refine_max_lvl = 0 only");
+        return set_err(PSIVOX_E_MODE, "the adjoint supports refine_max_lvl = 0 only");
+    if (full && (!grid->fields[PSIVOX_M] || !grid->fields[PSIVOX_V] || (dl_df[PSIVOX_VV] && !grid->fields[PSIVOX_VV])))
+        return set_err(PSIVOX_E_ARG, "the adjoint of v and vv needs the forward's fields m, v (and vv) in grid->fields");
     GridInfo g;
     for (int a = 0; a < 3; ++a) {
         if (grid->n[a] <= 0) return set_err(PSIVOX_E_ARG, "grid n[%d] = %d must be positive", a, grid->n[a]);
@@ -2164,18 +2249,46 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
     const int per = vpe == 4 ? 1 : 6;
     MeshDev md{};
     if ((rc = mesh_to_device(W, mesh, dev, s, md))) return rc;
-    const double* w = dl_dm;
-    if (!grid->on_device) {
+    /* the cells' weights on the device (adj_in: forward m, v, vv, then the
+     * gradients of m, v, vv, then dL/dvel) */
+    auto dev_field = [&](int slot, const double* src, int width, const double** out) -> int {
+        *out = src;
+        if (!src || grid->on_device) return 0;
         const void* p;
-        if ((rc = upload(W.adj_w, dl_dm, (size_t)nvox * sizeof(double), dev, s, &p))) return rc;
-        w = (const double*)p;
+        int r = upload(W.adj_in[slot], src, (size_t)nvox * width * sizeof(double), dev, s, &p);
+        if (r) return r;
+        *out = (const double*)p;
+        return 0;
+    };
+    const double* w = nullptr;
+    if (!full) {
+        if ((rc = dev_field(3, dl_df[PSIVOX_M], 1, &w))) return rc;
+    } else {
+        const double *fm, *fv, *fvv = nullptr, *gm = nullptr, *gv = nullptr, *gvv = nullptr;
+        if ((rc = dev_field(0, grid->fields[PSIVOX_M], 1, &fm))) return rc;
+        if ((rc = dev_field(1, grid->fields[PSIVOX_V], 3, &fv))) return rc;
+        if (dl_df[PSIVOX_VV] && (rc = dev_field(2, grid->fields[PSIVOX_VV], 9, &fvv))) return rc;
+        if ((rc = dev_field(3, dl_df[PSIVOX_M], 1, &gm))) return rc;
+        if ((rc = dev_field(4, dl_df[PSIVOX_V], 3, &gv))) return rc;
+        if ((rc = dev_field(5, dl_df[PSIVOX_VV], 9, &gvv))) return rc;
+        if ((rc = W.adj_v.get((size_t)nvox * ADJ_NW * sizeof(double), dev))) return rc;
+        k_adj_weights<<<grid_blocks(nvox, 256), 256, 0, s>>>(nvox, fm, fv, fvv, gm, gv, gvv, (double*)W.adj_v.p);
+        HIPCHK(hipGetLastError());
+        w = (const double*)W.adj_v.p;
     }
-    double* dpos = dl_dpos;
+    /* the gradients on the device */
     const size_t gbytes = (size_t)mesh->npart * 3 * sizeof(double);
-    if (!mesh->on_device && mesh->npart > 0) {
+    double* dpos = dl_dpos;
+    double* dvel = full ? dl_dvel : nullptr;
+    if (dpos && !mesh->on_device && mesh->npart > 0) {
         const void* p;
         if ((rc = upload(W.adj_g, dl_dpos, gbytes, dev, s, &p))) return rc;
         dpos = (double*)p;
+    }
+    if (dvel && !mesh->on_device && mesh->npart > 0) {
+        const void* p;
+        if ((rc = upload(W.adj_in[6], dl_dvel, gbytes, dev, s, &p))) return rc;
+        dvel = (double*)p;
     }
     if ((rc = W.counters.get(64, dev))) return rc;
     unsigned long long* counters = (unsigned long long*)W.counters.p;
@@ -2183,60 +2296,81 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
     HIPCHK(hipMemsetAsync(counters, 0, 64, s));
     const int64_t ne = mesh->nelem;
     const int64_t batch = o->batch_elements > 0 ? o->batch_elements : (int64_t)1 << 22;
+    AdjArgs A;
+    A.w = w;
+    A.md = md;
+    A.per = per;
+    A.dpos = dpos;
+    A.dvel = dvel;
     for (int64_t e0 = 0; e0 < ne; e0 += batch) {
         const int64_t e1 = std::min(ne, e0 + batch);
         const int64_t nt = (e1 - e0) * per;
+        A.e0 = e0;
         if ((rc = W.tets.get((size_t)nt * TREC * sizeof(double), dev))) return rc;
         double* tets = (double*)W.tets.p;
         launch_expand_fixed(md, g, mb, e0, e1 - e0, tets, 0.0, s);
         HIPCHK(hipGetLastError());
-        /* tasks: count, scan, emit (chunks of tets whose tasks fit) */
-        if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
-        if ((rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
-        int64_t* cnt = (int64_t*)W.colcnt.p;
-        int64_t* start = (int64_t*)W.colstart.p;
-        HIPCHK(hipMemsetAsync(cnt + nt, 0, sizeof(int64_t), s));
-        k_adj_count<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, cnt, status);
-        HIPCHK(hipGetLastError());
-        size_t tb = 0;
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, start, nt + 1, s));
-        if ((rc = W.scan_tmp.get(tb, dev))) return rc;
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp.p, tb, cnt, start, nt + 1, s));
-        std::vector<int64_t> hs((size_t)nt + 1);
-        HIPCHK(hipMemcpyAsync(hs.data(), start, (nt + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        const int64_t cap = (int64_t)1 << 28;
-        if ((rc = W.tasks.get((size_t)std::min(hs[nt], cap) * sizeof(uint64_t) + 8, dev))) return rc;
-        uint64_t* tasks = (uint64_t*)W.tasks.p;
-        AdjArgs A;
-        A.w = w;
-        A.md = md;
-        A.e0 = e0;
-        A.per = per;
-        A.dpos = dpos;
-        for (int64_t a0 = 0; a0 < nt;) {
-            int64_t a1 = std::upper_bound(hs.begin() + a0 + 1, hs.end(), hs[a0] + cap) - hs.begin() - 1;
-            if (a1 <= a0) a1 = a0 + 1;
-            const int64_t nsub = hs[a1] - hs[a0];
-            if (nsub > 0) {
-                k_adj_emit<<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, g, mb, start, hs[a0], tasks);
-                HIPCHK(hipGetLastError());
-                k_adj<<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, tasks, nsub, g, mb, A);
-                HIPCHK(hipGetLastError());
+        if (dpos) {
+            /* plane tasks: count, scan, emit (chunks of tets whose tasks fit) */
+            if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
+            if ((rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
+            int64_t* cnt = (int64_t*)W.colcnt.p;
+            int64_t* start = (int64_t*)W.colstart.p;
+            HIPCHK(hipMemsetAsync(cnt + nt, 0, sizeof(int64_t), s));
+            k_adj_count<<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, g, mb, cnt, status);
+            HIPCHK(hipGetLastError());
+            size_t tb = 0;
+            HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, start, nt + 1, s));
+            if ((rc = W.scan_tmp.get(tb, dev))) return rc;
+            HIPCHK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp.p, tb, cnt, start, nt + 1, s));
+            std::vector<int64_t> hs((size_t)nt + 1);
+            HIPCHK(hipMemcpyAsync(hs.data(), start, (nt + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            const int64_t cap = (int64_t)1 << 28;
+            if ((rc = W.tasks.get((size_t)std::min(hs[nt], cap) * sizeof(uint64_t) + 8, dev))) return rc;
+            uint64_t* tasks = (uint64_t*)W.tasks.p;
+            for (int64_t a0 = 0; a0 < nt;) {
+                int64_t a1 = std::upper_bound(hs.begin() + a0 + 1, hs.end(), hs[a0] + cap) - hs.begin() - 1;
+                if (a1 <= a0) a1 = a0 + 1;
+                const int64_t nsub = hs[a1] - hs[a0];
+                if (nsub > 0) {
+                    k_adj_emit<<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, g, mb, start, hs[a0], tasks);
+                    HIPCHK(hipGetLastError());
+                    if (full)
+                        k_adj<true><<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, tasks, nsub, g, mb, A);
+                    else
+                        k_adj<false><<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, tasks, nsub, g, mb, A);
+                    HIPCHK(hipGetLastError());
+                }
+                a0 = a1;
             }
-            a0 = a1;
+        }
+        if (dvel) {
+            /* column tasks of the batch, the velocity part per task */
+            bool fell = false;
+            const PassGeo G{g, mb, nullptr, nullptr, nullptr};
+            rc = for_task_chunks(W, tets, nt, G, PASS_ALL64, status, counters, s, dev, &fell, nullptr,
+                                 [&](int64_t, int64_t, const int64_t*, int64_t, const uint64_t* tasks, int64_t nsub) -> int {
+                                     k_adj_vel<<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, tasks, nsub, g, mb, A,
+                                                                                            status);
+                                     HIPCHK(hipGetLastError());
+                                     return 0;
+                                 });
+            if (rc) return rc;
+            if (fell) return set_err(PSIVOX_E_OVERFLOW, "adjoint: a tet cannot take the column path");
         }
         if (o->progress && o->progress(o->progress_ctx, e1, ne)) {
             (void)hipStreamSynchronize(s);
             return set_err(PSIVOX_E_CANCELLED, "cancelled after %lld of %lld elements", (long long)e1, (long long)ne);
         }
     }
-    if (!mesh->on_device && mesh->npart > 0)
-        HIPCHK(hipMemcpyAsync(dl_dpos, dpos, gbytes, hipMemcpyDeviceToHost, s));
+    if (dpos && dpos != dl_dpos) HIPCHK(hipMemcpyAsync(dl_dpos, dpos, gbytes, hipMemcpyDeviceToHost, s));
+    if (dvel && dvel != dl_dvel) HIPCHK(hipMemcpyAsync(dl_dvel, dvel, gbytes, hipMemcpyDeviceToHost, s));
     unsigned hst = 0;
     HIPCHK(hipMemcpyAsync(&hst, status, sizeof hst, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (hst & ST_BADGEOM) return set_err(PSIVOX_E_OVERFLOW, "mass adjoint: a tet has non-finite vertices");
+    if (hst & ST_BADGEOM) return set_err(PSIVOX_E_OVERFLOW, "adjoint: a tet has non-finite vertices");
+    if (hst & ST_OVERFLOW) return set_err(PSIVOX_E_OVERFLOW, "adjoint: a column exceeded its vertex budget");
     return 0;
 }
 
@@ -2850,6 +2984,7 @@ int psivox_voxels_batched(int32_t nmesh, const psivox_mesh* meshes, psivox_grid*
 int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,
                         const psivox_opts* opts) {
     if (!mesh || !grid) return set_err(PSIVOX_E_ARG, "mesh and grid are required");
+    if (!dl_dm || !dl_dpos) return set_err(PSIVOX_E_ARG, "dl_dm and dl_dpos are required");
     psivox_opts o{};
     o.refine_tolerance = 1.0;
     if (opts) o = *opts;
@@ -2858,7 +2993,23 @@ int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const 
     int dev;
     int rc = prepare(g_ws, &o, &s, &dev);
     if (rc) return rc;
-    return run_adjoint(g_ws, mesh, grid, dl_dm, dl_dpos, &o, s, dev);
+    const double* df[6] = {dl_dm, nullptr, nullptr, nullptr, nullptr, nullptr};
+    return run_adjoint(g_ws, mesh, grid, df, dl_dpos, nullptr, &o, s, dev);
+}
+
+int psivox_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* const* dl_dfields, double* dl_dpos,
+                   double* dl_dvel, const psivox_opts* opts) {
+    if (!mesh || !grid || !dl_dfields) return set_err(PSIVOX_E_ARG, "mesh, grid and dl_dfields are required");
+    if (!dl_dpos && !dl_dvel) return set_err(PSIVOX_E_ARG, "dl_dpos or dl_dvel is required");
+    psivox_opts o{};
+    o.refine_tolerance = 1.0;
+    if (opts) o = *opts;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    hipStream_t s;
+    int dev;
+    int rc = prepare(g_ws, &o, &s, &dev);
+    if (rc) return rc;
+    return run_adjoint(g_ws, mesh, grid, dl_dfields, dl_dpos, dl_dvel, &o, s, dev);
 }
 
 int psivox_elem_index_build(const psivox_mesh* mesh, const double window[2][3], void* stream,

@@ -30,6 +30,7 @@
 #ifndef PSIVOX_ADJOINT_H
 #define PSIVOX_ADJOINT_H
 
+#include "psivox_fast.h"
 #include "psivox_geom.h"
 #include "psivox_refine.h"
 
@@ -176,11 +177,49 @@ PVX_HD void slab_edge(double pu, double pv, double qu, double qv, int k, double 
 }
 
 /*
- * One adjoint task.  w: dL/dm (grid layout n0 x n1 x n2); R: a lane's row
- * polygon buffer.  out[v] gets dL/dx_{v, axis} of this tet for the task's
- * plane.  Tets of zero volume contribute nothing.
+ * Per-cell weights of the adjoint: dL/d(raw sums) of the cell, w[ADJ_NW * c
+ * ...]: M (the mass), then with velocity fields Sv (3) and Svv (3 x 3),
+ * the mass-weighted sums of v and v v^T that the fields are finalized from
+ * (m = M, v = Sv / M, vv = Svv / M - v v^T).  A tet's deposit into cell C
+ * is (mass / vol) times the integral over T n C of
+ *   g_C(x) = W_M + W_Sv . v(x) + v(x)^T W_Svv v(x),
+ * v(x) the tet's linear velocity interpolation.
  */
-template <int FB>
+constexpr int ADJ_NW = 13;
+
+/* g_C at a point with velocity v (W: the cell's ADJ_NW weights) */
+PVX_HD double adj_g(const double* W, const double* v) {
+    double s = W[0] + W[1] * v[0] + W[2] * v[1] + W[3] * v[2];
+    for (int a = 0; a < 3; ++a) s += v[a] * (W[4 + 3 * a] * v[0] + W[5 + 3 * a] * v[1] + W[6 + 3 * a] * v[2]);
+    return s;
+}
+
+/* The point X (record frame) of a section piece: barycentric coordinates l
+ * in the tet and the velocity there */
+PVX_HD void adj_point(const TetInv64& T, const double* rec, const double* X, double* l, double* v) {
+    double q[3];
+    for (int r = 0; r < 3; ++r)
+        q[r] = T.B[r][0] * (X[0] - T.x0[0]) + T.B[r][1] * (X[1] - T.x0[1]) + T.B[r][2] * (X[2] - T.x0[2]);
+    l[0] = 1.0 - q[0] - q[1] - q[2];
+    l[1] = q[0];
+    l[2] = q[1];
+    l[3] = q[2];
+    for (int a = 0; a < 3; ++a)
+        v[a] = l[0] * rec[12 + a] + l[1] * rec[15 + a] + l[2] * rec[18 + a] + l[3] * rec[21 + a];
+}
+
+/*
+ * One adjoint task.  w: the cells' weights (grid layout n0 x n1 x n2; FULL:
+ * ADJ_NW per cell, else dL/dm alone); R: a lane's row polygon buffer.
+ * out[v] gets dL/dx_{v, axis} of this tet for the task's plane: the flux
+ * of lambda_v (g_above - g_below) through each cell's piece of the section.
+ * Without velocity fields g is constant per cell and the piece's centroid
+ * gives the integral; with them the integrand is cubic and each piece is
+ * fanned into triangles integrated with a rule exact to degree 3 (centroid
+ * -27/48, points 3/5 of the way to each corner 25/48 each).  Tets of zero
+ * volume contribute nothing.
+ */
+template <int FB, bool FULL = false>
 PVX_HD void adj_plane_task(const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task,
                            const double* __restrict__ w, Poly2<FB> R, double* out) {
     int64_t t;
@@ -277,6 +316,52 @@ PVX_HD void adj_plane_task(const GridInfo& g, const MeshBox& mb, const double* r
 #pragma unroll 1
         for (int k = k0; k <= k1; ++k) {
             const int64_t gi = (int64_t)j * Sb + (int64_t)k * Sc;
+            if (FULL) {
+                /* the weights of the cells above and below the plane (none
+                 * outside the grid) */
+                const double* Wa = p < na ? w + ADJ_NW * (gi + (int64_t)p * Sa) : nullptr;
+                const double* Wb = p > 0 ? w + ADJ_NW * (gi + (int64_t)(p - 1) * Sa) : nullptr;
+                /* fan the piece from its first point; each triangle by the
+                 * degree-3 rule */
+                double P0u = 0, P0v = 0, Pu = 0, Pv = 0;
+                int np = 0;
+                auto tri = [&](double au, double av, double bu, double bv, double cu, double cv) {
+                    const double ar = 0.5 * fabs((bu - au) * (cv - av) - (cu - au) * (bv - av)) * db * dc;
+                    if (ar == 0.0) return;
+                    const double Qu[4] = {(au + bu + cu) / 3.0, 0.6 * au + 0.2 * (bu + cu), 0.6 * bu + 0.2 * (au + cu),
+                                          0.6 * cu + 0.2 * (au + bu)};
+                    const double Qv[4] = {(av + bv + cv) / 3.0, 0.6 * av + 0.2 * (bv + cv), 0.6 * bv + 0.2 * (av + cv),
+                                          0.6 * cv + 0.2 * (av + bv)};
+                    for (int qp = 0; qp < 4; ++qp) {
+                        const double wt = ar * (qp == 0 ? -27.0 / 48.0 : 25.0 / 48.0);
+                        const double xb = w0b + db * ((double)j + Qu[qp]) - sb, xc = w0c + dc * ((double)k + Qv[qp]) - sc;
+                        const double X[3] = {a == 0 ? xp : (b == 0 ? xb : xc), a == 1 ? xp : (b == 1 ? xb : xc),
+                                             a == 2 ? xp : (b == 2 ? xb : xc)};
+                        double l[4], v[3];
+                        adj_point(T, rec, X, l, v);
+                        const double f = wt * ((Wa ? adj_g(Wa, v) : 0.0) - (Wb ? adj_g(Wb, v) : 0.0));
+                        for (int r = 0; r < 4; ++r) acc[r] += f * l[r];
+                    }
+                };
+                auto emitf = [&](double uu, double vv) {
+                    vv -= (double)k;
+                    if (np == 0) {
+                        P0u = uu;
+                        P0v = vv;
+                    } else if (np >= 2) {
+                        tri(P0u, P0v, Pu, Pv, uu, vv);
+                    }
+                    Pu = uu;
+                    Pv = vv;
+                    np++;
+                };
+                for (int q = 0; q < m; ++q) {
+                    const int r = q + 1 == m ? 0 : q + 1;
+                    slab_edge(R.u[q * FB], R.v[q * FB], R.u[r * FB], R.v[r * FB], 1, (double)k, (double)(k + 1),
+                              emitf);
+                }
+                continue;
+            }
             const double wa = p < na ? w[gi + (int64_t)p * Sa] : 0.0;
             const double wb = p > 0 ? w[gi + (int64_t)(p - 1) * Sa] : 0.0;
             const double J = wa - wb;
@@ -307,6 +392,79 @@ PVX_HD void adj_plane_task(const GridInfo& g, const MeshBox& mb, const double* r
     }
     const double s = rec[24] / T.vol;
     for (int v = 0; v < 4; ++v) out[v] = acc[v] * s;
+}
+
+/*
+ * The velocity part of the adjoint for one column task (copy, column) of a
+ * tet record: out[i][b] += dL/du_{i,b}, u_i the velocity of tet vertex i.
+ * The fields Sv and Svv of cell C are sums over fragments of mass m0 E[v]
+ * and mass m0 E[v v^T] (E: the mean over the fragment in mass coordinates,
+ * v = sum_i lambda_i u_i), so
+ *   dL/du_{i,b} = mass m0 (W_Sv,b E[lambda_i]
+ *                          + sum_c (W_Svv,bc + W_Svv,cb) E[lambda_i v_c])
+ * with E[lambda_i v_c] from the fragment's first and second q-moments.  The
+ * moments come from the column path in precision R.  W: ADJ_NW weights per
+ * cell.  False when the column overflows its vertex budget.
+ */
+template <class R>
+PVX_HD bool adj_vel_task(const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task,
+                         const double* __restrict__ W, LPoly<R>& P, double (*out)[3]) {
+    ColumnState<R> S;
+    const int st = column_begin(g, mb, rec, task, P, S);
+    if (st < 0) return false;
+    if (st == 0) return true;
+    double u0[3], DV[3][3];
+    for (int a = 0; a < 3; ++a) {
+        u0[a] = rec[12 + a];
+        for (int k = 0; k < 3; ++k) DV[a][k] = rec[12 + 3 * (k + 1) + a] - rec[12 + a];
+    }
+    const double mass = rec[24];
+    PlaneCarry<R> pc{false, false, {R(0), R(0), R(0)}};
+#pragma unroll 1
+    for (int k = S.k0; k < S.k1; ++k) {
+        bool hl, hh;
+        double lc, hc;
+        cell_planes(S, k, hl, lc, hh, hc);
+        Mom<R> M;
+        R O[3];
+        slab_moments_u<2>(P, S.C.G, 2, hl, lc, hh, hc, M, O, &pc);
+        if (!(M.m0 > R(0))) continue;
+        const double* Wc = W + ADJ_NW * (((int64_t)g.n[1] * S.i + S.j) * g.n[2] + k);
+        /* fragment mean q and E[q q^T] (about q = 0) */
+        const double m0 = (double)M.m0, inv = 1.0 / m0;
+        double qb[3], E2[3][3];
+        for (int a = 0; a < 3; ++a) qb[a] = (double)O[a] + 0.25 * (double)M.s1[a] * inv;
+        const int SI[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) {
+                /* second moment about O, shifted to q = 0 */
+                const double mu_a = qb[a] - (double)O[a], mu_c = qb[c] - (double)O[c];
+                const double cov = 0.05 * (double)M.s2[SI[a][c]] * inv - mu_a * mu_c;
+                E2[a][c] = cov + qb[a] * qb[c];
+            }
+        /* E[lambda_i], E[lambda_i q_c] with lambda = (1 - sum q, q0, q1, q2) */
+        double El[4], Elq[4][3];
+        El[0] = 1.0 - qb[0] - qb[1] - qb[2];
+        for (int c = 0; c < 3; ++c) {
+            El[c + 1] = qb[c];
+            Elq[0][c] = qb[c] - E2[0][c] - E2[1][c] - E2[2][c];
+            for (int l = 0; l < 3; ++l) Elq[l + 1][c] = E2[l][c];
+        }
+        /* symmetric part of W_Svv, doubled */
+        double Ws[3][3];
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) Ws[b][c] = Wc[4 + 3 * b + c] + Wc[4 + 3 * c + b];
+        const double f = mass * m0;
+        for (int i = 0; i < 4; ++i) {
+            /* E[lambda_i v_c] */
+            double Ev[3];
+            for (int c = 0; c < 3; ++c)
+                Ev[c] = u0[c] * El[i] + DV[c][0] * Elq[i][0] + DV[c][1] * Elq[i][1] + DV[c][2] * Elq[i][2];
+            for (int b = 0; b < 3; ++b)
+                out[i][b] += f * (Wc[1 + b] * El[i] + Ws[b][0] * Ev[0] + Ws[b][1] * Ev[1] + Ws[b][2] * Ev[2]);
+        }
+    }
+    return true;
 }
 
 /* element-local node of vertex v of tet t at refinement level 0

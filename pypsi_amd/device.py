@@ -13,7 +13,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libpsivox.so")
 
-KERNELS = {"auto": 0, "plain": 1, "fast": 2}
+KERNELS = {"auto": 0, "plain": 1, "fast": 2, "fast64": 3}
 FIELD_ORDER = ["m", "x", "v", "xx", "xv", "vv"]
 FIELD_WIDTH = {"m": 1, "x": 3, "v": 3, "xx": 9, "xv": 9, "vv": 9}
 
@@ -63,6 +63,7 @@ def lib():
         L.psivox_voxels_batched.argtypes = [C.c_int32, P, P, P, P]
         L.psivox_voxels_noalloc.argtypes = [P, P, P, P]
         L.psivox_mass_adjoint.argtypes = [P, P, P, P, P]
+        L.psivox_adjoint.argtypes = [P, P, P, P, P, P]
         L.psivox_last_error.restype = C.c_char_p
         L.psivox_device_count.restype = C.c_int
         L.psivox_abi_version.restype = C.c_int
@@ -75,7 +76,7 @@ def lib():
     return _lib
 
 
-EXPORTS = ["psivox_voxels", "psivox_voxels_batched", "psivox_voxels_noalloc", "psivox_mass_adjoint", "psivox_last_error",
+EXPORTS = ["psivox_voxels", "psivox_voxels_batched", "psivox_voxels_noalloc", "psivox_mass_adjoint", "psivox_adjoint", "psivox_last_error",
            "psivox_abi_version", "psivox_device_count", "psivox_elem_index_build", "psivox_elem_index_free",
            "psivox_elem_index_query", "psivox_sample_vdf"]
 
@@ -157,6 +158,18 @@ def mass_adjoint(mesh: Mesh, grid: Grid, dl_dm, dl_dpos, opts: Opts) -> None:
     float64, and (npart, 3) float64)."""
     check(lib().psivox_mass_adjoint(C.byref(mesh), C.byref(grid), C.c_void_p(dl_dm.data_ptr()),
                                     C.c_void_p(dl_dpos.data_ptr()), C.byref(opts)))
+
+
+def adjoint(mesh: Mesh, grid: Grid, dl_dfields: dict, dl_dpos=None, dl_dvel=None, opts: Opts = None) -> None:
+    """dl_dpos += dL/dpos, dl_dvel += dL/dvel for the gradients dl_dfields
+    {'m', 'v', 'vv': device tensors in the fields' layouts}; grid holds the
+    forward's fields (psivox_adjoint)."""
+    ptrs = (C.c_void_p * 6)(*[dl_dfields[f].data_ptr() if f in dl_dfields and dl_dfields[f] is not None else None
+                              for f in FIELD_ORDER])
+    check(lib().psivox_adjoint(C.byref(mesh), C.byref(grid), ptrs,
+                               C.c_void_p(dl_dpos.data_ptr()) if dl_dpos is not None else None,
+                               C.c_void_p(dl_dvel.data_ptr()) if dl_dvel is not None else None,
+                               C.byref(opts if opts is not None else make_opts())))
 
 
 class ElemIndex:

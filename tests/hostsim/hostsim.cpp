@@ -280,6 +280,68 @@ extern "C" int hs_mass_adjoint(int vpe, int64_t npart, int64_t nelem, const doub
     return 0;
 }
 
+/* the adjoint of the fields m, v, vv with the device's task functions: W
+ * holds ADJ_NW weights per cell; plane tasks for dpos, column tasks (FP64
+ * moments) for dvel */
+extern "C" int hs_adjoint(int vpe, int64_t npart, int64_t nelem, const double* pos, const double* vel,
+                          const double* mass, const int32_t* conn, int periodic, const double* box6, const int* n,
+                          const double* win6, const double* d, const double* W, double* dpos, double* dvel) {
+    double* fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    GridInfo g;
+    MeshBox mb;
+    fill_grid(g, mb, n, win6, d, fields, 0, periodic, box6);
+    for (int64_t q = 0; q < 3 * npart; ++q) dpos[q] = dvel[q] = 0.0;
+    RefineScratch* RS = new RefineScratch;
+    std::vector<double> ru(8), rv(8);
+    Poly2<1> R{ru.data(), rv.data()};
+    std::vector<double> lv((size_t)VRec<double>::WORDS / 2);
+    LPoly<double> P{(uint32_t*)lv.data(), 0, 0, 0};
+    std::vector<double> tets;
+    std::vector<uint64_t> tasks;
+    int fails = 0;
+    const int per = vpe == 4 ? 1 : 6;
+    for (int64_t e = 0; e < nelem; ++e) {
+        int64_t nt = expand_element(pos, vel, mass, conn, vpe, e, g, mb, 0, 0.0, *RS, nullptr);
+        if (nt == 0) continue;
+        tets.resize((size_t)nt * TREC);
+        expand_element(pos, vel, mass, conn, vpe, e, g, mb, 0, 0.0, *RS, tets.data());
+        for (int64_t t = 0; t < nt; ++t) {
+            const double* r = &tets[(size_t)t * TREC];
+            if (r[25] == 0.0) continue;
+            int64_t ids[4];
+            for (int v = 0; v < 4; ++v) ids[v] = conn[e * vpe + tet_node(vpe, (int)(t % per), v)];
+            const int64_t nk = adj_tasks(r, t, g, mb, nullptr);
+            if (nk < 0) {
+                fails++;
+                continue;
+            }
+            tasks.resize((size_t)nk);
+            adj_tasks(r, t, g, mb, tasks.data());
+            for (int64_t c = 0; c < nk; ++c) {
+                double o4[4];
+                adj_plane_task<1, true>(g, mb, r, tasks[c], W, R, o4);
+                const int ax = (int)((tasks[c] >> 35) & 3u);
+                for (int v = 0; v < 4; ++v) dpos[3 * ids[v] + ax] += o4[v];
+            }
+            const int64_t ncol = tet_columns<double>(r, t, g, mb, nullptr, nullptr);
+            if (ncol < 0) {
+                fails++;
+                continue;
+            }
+            tasks.resize((size_t)ncol);
+            tet_columns<double>(r, t, g, mb, tasks.data(), nullptr);
+            for (int64_t c = 0; c < ncol; ++c) {
+                double o[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                if (!adj_vel_task<double>(g, mb, r, tasks[c], W, P, o)) fails++;
+                for (int v = 0; v < 4; ++v)
+                    for (int b = 0; b < 3; ++b) dvel[3 * ids[v] + b] += o[v][b];
+            }
+        }
+    }
+    delete RS;
+    return fails;
+}
+
 /* the device query path on the host: element boxes as k_elem_boxes, the
  * box test of k_box_meets, candidates expanded as k_vdf_count/emit and
  * sampled as k_vdf_sample; returns the stream count, writes up to cap */

@@ -113,3 +113,20 @@ def cells_merge_finalize(P, fields, acc, xvfix=False):
     acc = np.ascontiguousarray(acc, dtype=np.float64).copy()
     lib().hs_cells_merge_finalize(n, win, d, fptr, C.c_int(int(xvfix)), C.c_void_p(acc.ctypes.data))
     return fields
+
+
+def adjoint(P, W):
+    """(dpos, dvel) of the device adjoint of m, v, vv run on the host, for
+    the cells' weights W (nvox, 13)."""
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    dpos = np.zeros((P.pos.shape[0], 3))
+    dvel = np.zeros((P.pos.shape[0], 3))
+    n = (C.c_int * 3)(*P.n)
+    win = (C.c_double * 6)(*P.bounds.reshape(-1))
+    d = (C.c_double * 3)(*P.d)
+    box = (C.c_double * 6)(*P.box.reshape(-1))
+    vp = lambda a: C.c_void_p(a.ctypes.data)
+    fails = lib().hs_adjoint(C.c_int(P.elemtype), C.c_int64(P.pos.shape[0]), C.c_int64(P.conn.shape[0]), vp(P.pos),
+                             vp(P.vel), vp(P.mass), vp(P.conn), C.c_int(int(P.periodic)), box, n, win, d, vp(W),
+                             vp(dpos), vp(dvel))
+    return dpos, dvel, fails

@@ -1,6 +1,6 @@
-"""GPU mass adjoint (psivox_mass_adjoint through the C ABI, and the
-torch.autograd wrapper) against the FP64 oracle adjoint and FP64 central
-finite differences.
+"""GPU adjoints (psivox_mass_adjoint and psivox_adjoint through the C ABI,
+and the torch.autograd wrappers) against the FP64 oracle adjoint and FP64
+central finite differences.
 
 Tolerances: the adjoint kernel (FP64 throughout), fed the same dL/dm as the
 oracle, within 1e-9 of the largest gradient component; the 64^3 -> 128^3 workload (rms 1.0 cells, loss
@@ -8,8 +8,8 @@ sum (rho - rho_t)^2 with rho_t from seed 1) on 1000 seeded (vertex, axis)
 pairs: |g - g_fd| / max(|g_fd|, 1e-3 rms(g_fd)) <= 1e-4 against FP64
 central differences with h = 1e-6 box units, for the adjoint fed the
 oracle's FP64 dL/dm and for the end-to-end autograd path with the FP64
-forward (1e-3 with the FP32-clip forward, whose cell masses carry FP32
-error into dL/dm).
+forward, and the FP32-clip forward (its cell masses carry FP32 error into
+dL/dm: 7.2e-5 measured).
 """
 import numpy as np
 import pytest
@@ -131,12 +131,14 @@ def test_c4_adjoint_vs_fd_and_oracle(oracle_mod):
     assert e_or.max() <= 1e-6
 
 
-@pytest.mark.parametrize("kernel,tol", [("plain", 1e-4), ("fast", 1e-3)])
+@pytest.mark.parametrize("kernel,tol", [("plain", 1e-4), ("fast64", 1e-4), ("auto", 1e-4), ("fast", 1e-4)])
 def test_c4_autograd_vs_fd(oracle_mod, kernel, tol):
     """The torch.autograd path end to end (GPU forward, loss in torch,
-    backward through the adjoint kernel).  With the FP64 forward the bar is
-    the adjoint's, 1e-4; with the FP32-clip forward dL/dm itself carries the
-    forward's FP32 error (cell masses within 1e-5 relative), bar 1e-3."""
+    backward through the adjoint kernel), 1e-4 for every forward.  With an
+    FP64 forward (the plain kernel, or the column kernel in FP64) the error
+    is the finite differences' own (4.0e-5 measured); the FP32-clip forward
+    (voxel_density's default, 'auto') carries its cell masses' FP32 error
+    into dL/dm: 7.2e-5 measured."""
     import torch
     from pypsi_amd.autograd import voxel_density
     print("case: c4 autograd", kernel, flush=True)
@@ -158,3 +160,87 @@ def test_c4_autograd_vs_fd(oracle_mod, kernel, tol):
     rho32 = voxel_density(p32, conn, (128,) * 3, BOX, periodic_box=BOX, kernel=kernel)
     ((rho32 - trho_t) ** 2).sum().backward()
     assert p32.grad.dtype == torch.float32 and torch.isfinite(p32.grad).all()
+
+
+# ---------------------------------------------------------------------------
+# the adjoint of the fields m, v, vv (positions and velocities)
+# ---------------------------------------------------------------------------
+
+def _fields_problem(oracle_mod, n, gn, rms, seed, periodic=True):
+    from pypsi_amd.synth import zeldovich_block
+    pos, vel = zeldovich_block(n, rms, seed)
+    P = oracle_mod.Problem.block(pos, vel, periodic_box=BOX if periodic else None, n=(gn,) * 3, bounds=BOX,
+                                 fields=["v", "vv"])
+    ref, _ = oracle_mod.voxels(P, threads=-1)
+    return P, ref
+
+
+def _rand_grads(ref, seed):
+    rng = np.random.default_rng(seed)
+    return {k: rng.standard_normal(ref[k].shape) for k in ("m", "v", "vv")}
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_fields_adjoint_vs_oracle(oracle_mod, periodic):
+    """psivox_adjoint through the C ABI against the oracle's FP64 adjoint
+    fed the same forward fields and gradients: within 1e-9 of the largest
+    component (positions and velocities)."""
+    import torch
+    from pypsi_amd import device as D
+    print("case: fields adjoint vs oracle, periodic", periodic, flush=True)
+    P, ref = _fields_problem(oracle_mod, 7, 13, 1.5, 2, periodic)
+    grads = _rand_grads(ref, 9)
+    dpos_o, dvel_o = oracle_mod.adjoint(P, oracle_mod.adjoint_weights(ref, grads))
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in ref.items()}
+    tg = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in grads.items()}
+    pos, vel = torch.from_numpy(P.pos).to(dev), torch.from_numpy(P.vel).to(dev)
+    mesh = D.make_mesh(pos, vel, torch.from_numpy(P.mass).to(dev), torch.from_numpy(P.conn).to(dev),
+                       BOX if periodic else None)
+    grid = D.make_grid(P.n, BOX, t)
+    dpos, dvel = torch.zeros_like(pos), torch.zeros_like(vel)
+    D.adjoint(mesh, grid, tg, dpos, dvel, D.make_opts(stream=torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    ep = np.abs(dpos.cpu().numpy() - dpos_o).max() / np.abs(dpos_o).max()
+    ev = np.abs(dvel.cpu().numpy() - dvel_o).max() / np.abs(dvel_o).max()
+    print("gpu fields adjoint vs oracle: positions %.2e, velocities %.2e" % (ep, ev), flush=True)
+    assert ep <= 1e-9 and ev <= 1e-9
+
+
+@pytest.mark.parametrize("kernel,tol", [("fast64", 1e-4), ("plain", 1e-4), ("auto", 2e-4)])
+def test_fields_autograd_vs_fd(oracle_mod, kernel, tol):
+    """voxel_fields end to end: GPU forward of m, v, vv (kernel; its default
+    is the FP64 column kernel 'fast64', measured 1.5e-7; the FP32-clip
+    forward's cell fields carry their FP32 error into the weights, measured
+    8.2e-5 on velocities, bar 2e-4), the loss
+    sum_C G_m m + G_v . v + G_vv : vv in torch with seeded random
+    coefficients, backward through psivox_adjoint; against FP64 central
+    differences of the same loss (oracle forward, h = 1e-6) on 300 seeded
+    position and 300 velocity components, |g - fd| / max(|fd|, 1e-3 rms fd)."""
+    import torch
+    from pypsi_amd.autograd import voxel_fields
+    from oracle import block_connectivity
+    print("case: fields autograd vs fd, forward", kernel, flush=True)
+    P, ref = _fields_problem(oracle_mod, 12, 24, 1.0, 3)
+    grads = _rand_grads(ref, 4)
+    dev = torch.device("cuda:0")
+    pos = torch.from_numpy(P.pos).to(dev).requires_grad_(True)
+    vel = torch.from_numpy(P.vel).to(dev).requires_grad_(True)
+    conn = torch.from_numpy(P.conn).to(dev)
+    m, v, vv = voxel_fields(pos, vel, conn, P.n, BOX, periodic_box=BOX, kernel=kernel)
+    tg = {k: torch.from_numpy(g).to(dev) for k, g in grads.items()}
+    loss = (tg["m"] * m).sum() + (tg["v"] * v).sum() + (tg["vv"] * vv).sum()
+    loss.backward()
+    rng = np.random.default_rng(77)
+    parts = rng.choice(P.pos.shape[0], size=300, replace=False)
+    axes = rng.integers(0, 3, size=300)
+    base = oracle_mod.raw_sums(ref)
+    coef = np.concatenate([grads["m"].reshape(-1, 1), grads["v"].reshape(-1, 3), grads["vv"].reshape(-1, 9)], axis=1)
+    csr = oracle_mod.particle_elements(P)
+    for which, t in ((0, pos), (1, vel)):
+        g = t.grad.cpu().numpy()[parts, axes]
+        fd = oracle_mod.fd_fields_loss_grad(P, base, coef, parts, axes, which, h=1e-6, csr=csr)
+        e = _rel_err(g, fd)
+        print("fields autograd (%s forward) vs fd, %s: max %.2e" % (kernel, ["positions", "velocities"][which], e.max()),
+              flush=True)
+        assert e.max() <= tol
