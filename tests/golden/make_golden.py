@@ -7,7 +7,8 @@ Runs only where the reference checkout is present (it builds oracle/_ref from
 the reference sources with oracle/Makefile).  Each fixture holds its inputs
 (mesh arrays, grid, tolerance, level) and the reference's outputs, bit for bit.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py snapshot   # snap010_example2 only (~90 s)
 """
 import os
 import sys
@@ -74,6 +75,7 @@ def main():
     P = O.Problem.block(pos, vel, n=(9, 7, 8), bounds=((0.15, 0.2, 0.1), (0.75, 0.8, 0.7)), fields=["v", "vv"])
     save("g4_window_open", P, 1.0, 0, O.ref_voxels(P))
     modes()
+    snapshot_example2()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
@@ -114,5 +116,35 @@ def modes():
                         count=np.array([len(h) for h in hits]), elems=np.concatenate(hits).astype(np.int32))
 
 
+SNAPSHOT = "/root/reference/tutorial/snapshot_010"
+SUB = (slice(56, 72), slice(56, 72), slice(56, 72))
+
+
+def snapshot_example2():
+    """The notebook's Example 2 (tutorial/Tutorial.ipynb:193-204): the
+    tutorial's Gadget-2 snapshot_010, loaded by the reference's own loader,
+    onto 128^3 over its wrap box with fields v and vv, refinement level 2 at
+    tolerance 0.1.  The fixture holds the loaded arrays (positions and
+    velocities are float32 in the file, so float32 here holds them exactly),
+    the reference's total mass and per-field sums over the whole grid (plain
+    and mass-weighted), and every field on the central 16^3 block."""
+    pos, vel, mass, conn, box = O.ref_load_snapshot(SNAPSHOT, "gadget2")[:5]
+    assert np.all(mass == mass[0])
+    P = O.Problem(pos, vel, mass, conn, n=(128,) * 3, bounds=box, periodic_box=box, fields=["v", "vv"])
+    out = O.ref_voxels(P, reftol=0.1, maxlvl=2)
+    np.savez_compressed(os.path.join(HERE, "snap010_example2.npz"), pos=pos.astype(np.float32),
+                        vel=vel.astype(np.float32), mass=np.array(mass[0]), conn=conn, box=box,
+                        n=np.array(P.n), reftol=np.array(0.1), maxlvl=np.array(2),
+                        sub_lo=np.array([s.start for s in SUB]), sum_m=np.array(out["m"].sum()),
+                        sum_v=out["v"].sum(axis=(0, 1, 2)), sum_vv=out["vv"].sum(axis=(0, 1, 2)),
+                        sum_mv=(out["m"][..., None] * out["v"]).sum(axis=(0, 1, 2)),
+                        sum_mvv=(out["m"][..., None, None] * out["vv"]).sum(axis=(0, 1, 2)),
+                        sub_m=out["m"][SUB], sub_v=out["v"][SUB], sub_vv=out["vv"][SUB])
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["snapshot"]:
+        O.build(ref=True)
+        snapshot_example2()
+    else:
+        main()

@@ -78,3 +78,10 @@ def assert_close(a, b, tol, what=""):
         assert rel <= rt and wabs <= wt, f"{what} field {f}: rel {rel:.3e} (tol {rt:.0e}), " \
                                          f"wabs {wabs:.3e} (tol {wt:.0e}); all: {errs}"
     return errs
+
+# The FP64 kernel against the FP64 oracle (same measures): masses within
+# 1e-12 relative, means within 1e-10, covariances within 1e-7 of their
+# diagonal scale on heavy cells; mass-weighted errors within 1e-13 (1e-12 for
+# the covariances) of the peak.
+TOL64 = {"m": (1e-12, 1e-13), "x": (1e-10, 1e-13), "v": (1e-10, 1e-13), "xx": (1e-7, 1e-12), "xv": (1e-7, 1e-12),
+         "vv": (1e-7, 1e-12)}
