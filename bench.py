@@ -653,9 +653,10 @@ def cpu_sample(probs, reftol=1.0, maxlvl=0, what=""):
 
 
 def fwd_adjoint_leg(steps=5, warmup=2):
-    """64^3 particles, rms 1.0 cells -> 128^3, mass: forward (FP32-clip
-    kernel) and the mass adjoint of L = sum (rho - rho_t)^2, rho_t from the
-    same construction with seed 1.  Reports t(fwd + adj) / t(fwd)."""
+    """64^3 particles, rms 1.0 cells -> 128^3, mass: forward (FP64-clip
+    column kernel, voxel_density's default) and the mass adjoint of
+    L = sum (rho - rho_t)^2, rho_t from the same construction with seed 1.
+    Reports t(fwd + adj) / t(fwd), and the same with the FP32-clip forward."""
     import torch
     from pypsi_amd import device as D
     pos, vel, mass, conn, gn, _ = build_inputs("zeldovich-64^3-rms1.0-to-128^3-m")
@@ -665,7 +666,8 @@ def fwd_adjoint_leg(steps=5, warmup=2):
     tconn = torch.from_numpy(conn).to(dev)
     tmass = torch.from_numpy(mass).to(dev)
     stream = torch.cuda.current_stream().cuda_stream
-    opts = D.make_opts(kernel="auto", stream=stream)
+    opts = D.make_opts(kernel="fast64", stream=stream)
+    o32 = D.make_opts(kernel="auto", stream=stream)
     V = (1.0 / gn) ** 3
     m = torch.zeros((gn,) * 3, dtype=torch.float64, device=dev)
     grid = D.make_grid((gn,) * 3, box, {"m": m})
@@ -693,10 +695,22 @@ def fwd_adjoint_leg(steps=5, warmup=2):
             tf.append(t1 - t0)
             ta.append(t2 - t1)
     f, a = 1e3 * sum(tf) / steps, 1e3 * sum(ta) / steps
+    t32 = []
+    for it in range(warmup + steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.zero_()
+        D.voxels(mesh, grid, o32)
+        torch.cuda.synchronize()
+        if it >= warmup:
+            t32.append(time.perf_counter() - t0)
+    f32 = 1e3 * sum(t32) / steps
     out = {"workload": "zeldovich-64^3-rms1.0-to-128^3-m, loss sum (rho - rho_t)^2, rho_t seed 1",
-           "forward_kernel": "auto (FP32 clip; the default of voxel_density)",
+           "forward_kernel": "fast64 (FP64 clip; the default of voxel_density, within 1e-4 of FP64 FD)",
            "fwd_ms": f, "adjoint_ms": a, "ratio_fwd_plus_adj_over_fwd": (f + a) / f, "steps": steps,
-           "adjoint_dtype": "f64"}
+           "adjoint_dtype": "f64",
+           "fp32_forward": {"kernel": "auto", "fwd_ms": f32, "ratio_fwd_plus_adj_over_fwd": (f32 + a) / f32,
+                            "gradient_vs_fd": "within 3e-4 (FP32 cell masses)"}}
     # the fields m, v, vv and their adjoint with respect to positions and
     # velocities (voxel_fields: FP64 column forward)
     vel = torch.from_numpy(build_inputs("zeldovich-64^3-rms1.0-to-128^3-m")[1]).to(dev)

@@ -58,12 +58,17 @@ class VoxelMass(torch.autograd.Function):
         return dpos.to(dtype), None, None, None, None, None, None
 
 
-def voxel_mass(pos, conn, n, bounds, mass=None, periodic_box=None, kernel="auto"):
-    """Grid mass field (float64, shape n) of PSI.voxels, differentiable in pos."""
+def voxel_mass(pos, conn, n, bounds, mass=None, periodic_box=None, kernel="fast64"):
+    """Grid mass field (float64, shape n) of PSI.voxels, differentiable in pos.
+    The default forward clips in FP64 (kernel 'fast64'), so dL/dm carries no
+    FP32 error: gradients agree with FP64 finite differences to 1e-4.  The
+    FP32-clip forwards ('auto', 'fast') are faster; their cell masses' FP32
+    error (about 1e-6 relative) reaches the gradient wherever the loss's
+    dL/dm is small beside its neighbours'."""
     return VoxelMass.apply(pos, conn, mass, tuple(int(v) for v in n), bounds, periodic_box, kernel)
 
 
-def voxel_density(pos, conn, n, bounds, mass=None, periodic_box=None, kernel="auto"):
+def voxel_density(pos, conn, n, bounds, mass=None, periodic_box=None, kernel="fast64"):
     """Mass per unit volume (m / cell volume)."""
     import math
     vol = math.prod((float(bounds[1][a]) - float(bounds[0][a])) / int(n[a]) for a in range(3))

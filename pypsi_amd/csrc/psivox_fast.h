@@ -70,12 +70,16 @@ constexpr double WIDE_CELLS = 4.0;
 
 /* Link word layout: neighbour k in bits [5k, 5k+5); the fan apex of the
  * face to the left of the directed edge (v -> neighbour k) in bits
- * [15+5k, 20+5k) (filled once per column by face_apexes).  31 marks an
- * unset link. */
+ * [15+5k, 20+5k): one vertex of that face, the same for all its directed
+ * edges (set by tet_start, kept by every clip).  31 marks an unset link. */
 constexpr uint32_t NBUNSET = 31u;
 
 PVX_HD int nbf(uint32_t w, int k) { return (int)((w >> (5 * k)) & 31u); }
 PVX_HD int apf(uint32_t w, int k) { return (int)((w >> (15 + 5 * k)) & 31u); }
+PVX_HD uint32_t with_nb(uint32_t w, int k, int v) { return (w & ~(31u << (5 * k))) | ((uint32_t)v << (5 * k)); }
+PVX_HD uint32_t with_apex(uint32_t w, int k, int v) {
+    return (w & ~(31u << (15 + 5 * k))) | ((uint32_t)v << (15 + 5 * k));
+}
 PVX_HD uint32_t pack3(int a, int b, int c) { return (uint32_t)a | ((uint32_t)b << 5) | ((uint32_t)c << 10); }
 PVX_HD float u2f(uint32_t u) {
     float f;
@@ -240,10 +244,15 @@ PVX_HD R sdist(const double* rec, const AxisMap<R>& m, double c, R off, R q0, R 
  * subset of P's), put a new vertex into a free slot on each edge from a
  * survivor vc to a dropped vn (place(vx, vc, vn) sets its q), link the new
  * vertices around their faces, and keep the order: survivors in their order,
- * then the new vertices as they are made (the order a compacted graph has;
- * the faces' fan apexes and the moment sums follow it).  The graph is the
- * reference's (src/geometry.c:391-501); nothing is renumbered, the slots of
- * dropped vertices are freed.  False on slot overflow or a broken graph.
+ * then the new vertices as they are made (the order a compacted graph has).
+ * The graph is the reference's (src/geometry.c:391-501); nothing is
+ * renumbered, the slots of dropped vertices are freed.
+ * Fan apexes: the faces the clip leaves alone keep theirs; each cut face is
+ * walked once from a new vertex (the walk that links it to the next new
+ * vertex around the face) and fanned from that new vertex; the new face on
+ * the plane is fanned from the first new vertex.  Any vertex of a convex
+ * face fans it, so the moments are those of the reference's fans up to
+ * rounding.  False on slot overflow or a broken graph.
  */
 template <class R, class Place>
 PVX_HD bool clip_graph(LPoly<R>& P, uint32_t gone, Place&& place) {
@@ -279,7 +288,10 @@ PVX_HD bool clip_graph(LPoly<R>& P, uint32_t gone, Place&& place) {
         }
     }
     const uint32_t after = (live & ~gone) | fresh;
-    /* link each new vertex to the next one around its face */
+    /* new vertices take the lowest free slots in turn: the first one made */
+    const int cap = fresh ? __builtin_ctz(fresh) : 0;
+    /* link each new vertex vs to the next one around the face to the left
+     * of vs -> vc, and fan that face from vs */
     for (uint32_t fm = fresh; fm; fm &= fm - 1u) {
         const int vs = __builtin_ctz(fm);
         int vcur = vs, vnext = P.nbr(vs, 0);
@@ -298,10 +310,14 @@ PVX_HD bool clip_graph(LPoly<R>& P, uint32_t gone, Place&& place) {
             }
             vcur = vnext;
             if ((fresh >> vcur) & 1u) break;
-            vnext = nbf(w, (k + 1) % 3);
+            const int kn = k == 2 ? 0 : k + 1;
+            P.setnbw(vcur, with_apex(w, kn, vs));
+            vnext = nbf(w, kn);
         }
-        P.setnbr(vs, 2, vcur);
-        P.setnbr(vcur, 1, vs);
+        /* vs -> vc lies on this face, vs -> vcur on the new one */
+        P.setnbw(vs, with_apex(with_apex(with_nb(P.nbw(vs), 2, vcur), 0, vs), 2, cap));
+        /* vcur -> vs closes this face */
+        P.setnbw(vcur, with_apex(with_nb(P.nbw(vcur), 1, vs), 1, vs));
     }
     P.live = after;
     P.nv = nt;
@@ -377,42 +393,6 @@ PVX_HD void tri_acc(Mom<R>& M, const R* A, const R* B, const R* C) {
             M.s2[3] += d * (A[1] * A[1] + B[1] * B[1] + C[1] * C[1] + S1 * S1);
             M.s2[4] += d * (A[1] * A[2] + B[1] * B[2] + C[1] * C[2] + S1 * S2);
             M.s2[5] += d * (A[2] * A[2] + B[2] * B[2] + C[2] * C[2] + S2 * S2);
-        }
-    }
-}
-
-/*
- * Fan apexes of P's faces: every face is walked once in the reference's
- * order (src/geometry.c:708-746) and each of its directed edges records the
- * walk's first vertex, so a face (v0, v1, ..., vm) is fanned into triangles
- * (v0, vi, vi+1) exactly as the reference's reduce fans it.
- */
-template <class R>
-PVX_HD void face_apexes(LPoly<R>& P) {
-    uint64_t mark = 0;
-    for (int r = 0; r < P.nv; ++r) {
-        const int vs = P.at(r);
-        for (int ks = 0; ks < 3; ++ks) {
-            if (mark & (1ull << (3 * vs + ks))) continue;
-            int a = vs, ka = ks, guard = 0;
-            /* a's link word as it is in LDS: read once per step, for the
-             * next vertex, and carried (nothing writes it in between) */
-            uint32_t wa = P.nbw(a);
-            do {
-                mark |= 1ull << (3 * a + ka);
-                /* every directed edge lies on one face: its apex field is
-                 * written exactly once, so stale bits need no clearing */
-                const int sh = 15 + 5 * ka;
-                wa = (wa & ~(31u << sh)) | ((uint32_t)vs << sh);
-                P.setnbw(a, wa);
-                const int b = nbf(wa, ka);
-                const uint32_t wb = P.nbw(b);
-                int kb = 0;
-                while (kb < 2 && nbf(wb, kb) != a) kb++;
-                a = b;
-                ka = (kb + 1) % 3;
-                wa = wb;
-            } while (!(a == vs && ka == ks) && ++guard <= 3 * NVF);
         }
     }
 }
@@ -1152,10 +1132,11 @@ PVX_HD void tet_start(LPoly<R>& P) {
     P.setq(1, R(1), R(0), R(0));
     P.setq(2, R(0), R(1), R(0));
     P.setq(3, R(0), R(0), R(1));
+    /* neighbours, and the fan apex of each face: its first vertex */
     P.setnbw(0, pack3(1, 3, 2));
-    P.setnbw(1, pack3(2, 3, 0));
-    P.setnbw(2, pack3(0, 3, 1));
-    P.setnbw(3, pack3(1, 2, 0));
+    P.setnbw(1, with_apex(pack3(2, 3, 0), 1, 1));
+    P.setnbw(2, with_apex(pack3(0, 3, 1), 2, 1));
+    P.setnbw(3, with_apex(pack3(1, 2, 0), 1, 1));
 }
 
 /* one table entry, from the graph steps alone (no coordinates) */
@@ -1188,7 +1169,7 @@ inline bool make_slab_step(int pattern, SlabStep& E) {
         const int sl = P.at(r);
         E.slot[r] = (uint8_t)sl;
         E.src[r] = src[sl];
-        E.link[r] = P.nbw(sl) & 0x7fffu;
+        E.link[r] = P.nbw(sl);
     }
     return true;
 }
@@ -1347,7 +1328,6 @@ PVX_HD int column_begin(const GridInfo& g, const MeshBox& mb, const double* rec,
     if (!slab_columns(S.C, S.i, g, j0, j1)) return 0;
     if (!build_column(P, S.C, g, S.i, S.j, j0, j1)) return -1;
     if (P.live == 0u) return 0;
-    face_apexes(P);
     lpoly_range(P, S.C.G, 2, S.C.G.ib0[2], S.C.G.ib0[2], S.C.ib1[2], g.n[2], S.k0, S.k1);
     return S.k1 > S.k0 ? 1 : 0;
 }
