@@ -97,6 +97,12 @@ struct TetInv64 {
 };
 
 PVX_HD bool tet_inv64(const double* rec, TetInv64& T) {
+    /* no fused multiply-adds here: a tet with a repeated vertex has two equal
+     * columns of D, whose cofactor products cancel exactly only when each
+     * product is rounded alike; a fused a*b - c*d leaves the rounding error
+     * of c*d, a determinant of order 1e-20 instead of 0, and the tet would
+     * not be dropped as the oracle drops it */
+#pragma clang fp contract(off)
     double D[3][3];
     for (int a = 0; a < 3; ++a) {
         T.x0[a] = rec[a];
