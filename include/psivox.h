@@ -156,10 +156,13 @@ int psivox_voxels_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psiv
  * no derivatives; the forward it differentiates is psi_voxels,
  * src/cpsi.c:41, density branch :100-112).  Given dL/dm per cell (float64,
  * shaped like grid m), ADDS dL/dpos (npart x 3 float64) of the mass field m
- * that psivox_voxels produces on the same mesh and grid into dl_dpos
- * (refine_max_lvl 0, or simplex meshes).  dl_dm is a device pointer when
- * grid->on_device, dl_dpos when mesh->on_device; grid->fields is not used.
- * Errors: PSIVOX_E_MODE for refine_max_lvl > 0 on 8/27-node elements. */
+ * that psivox_voxels produces on the same mesh and grid (with the same
+ * refine_tolerance and refine_max_lvl) into dl_dpos.  On refined meshes a
+ * tet's vertices are the element's trilinear or triquadratic map at fixed
+ * parameter points, so their gradients reach the element's nodes through
+ * that map; the refinement decisions are piecewise constant and carry no
+ * gradient.  dl_dm is a device pointer when grid->on_device, dl_dpos when
+ * mesh->on_device; grid->fields is not used. */
 int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* dl_dm, double* dl_dpos,
                         const psivox_opts* opts);
 
@@ -170,13 +173,13 @@ int psivox_mass_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const 
  * zeroed arrays; v and vv may be NULL when their gradients are).
  * dl_dfields[6] points to dL/d(m, x, v, xx, xv, vv) in the fields' layouts
  * (NULL: zero; x, xx and xv must be NULL).  ADDS dL/dpos and dL/dvel
- * (npart x 3 float64 each; either may be NULL) for refine_max_lvl 0 (or
- * simplex meshes).  Device pointers follow grid->on_device (fields and
+ * (npart x 3 float64 each; either may be NULL), on refined meshes as
+ * psivox_mass_adjoint.  Device pointers follow grid->on_device (fields and
  * their gradients) and mesh->on_device (dl_dpos, dl_dvel).  Position
  * gradients: the plane-section integrals of the mass adjoint with the
  * velocity fields' cubic integrand; velocity gradients: the fragments'
  * q-moments against each cell's weights.  Errors: PSIVOX_E_MODE for
- * gradients of x, xx, xv or refinement. */
+ * gradients of x, xx or xv. */
 int psivox_adjoint(const psivox_mesh* mesh, const psivox_grid* grid, const double* const* dl_dfields, double* dl_dpos,
                    double* dl_dvel, const psivox_opts* opts);
 

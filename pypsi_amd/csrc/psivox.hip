@@ -221,12 +221,13 @@ __global__ void k_expand_count(MeshDev m, GridInfo g, MeshBox mb, int64_t ne, in
 }
 
 __global__ void k_expand_emit(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, int64_t e1, int maxlvl, double tol2,
-                              const int64_t* __restrict__ offsets, int64_t base, double* __restrict__ tets) {
+                              const int64_t* __restrict__ offsets, int64_t base, double* __restrict__ tets,
+                              uint64_t* __restrict__ prm = nullptr) {
     int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= e1) return;
     RefineScratch S;
     expand_element(m.pos, m.vel, m.mass, m.conn, m.vpe, e, g, mb, maxlvl, tol2, S,
-                   tets + (int64_t)TREC * (offsets[e] - base));
+                   tets + (int64_t)TREC * (offsets[e] - base), prm ? prm + PREC * (offsets[e] - base) : nullptr);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1161,7 +1162,33 @@ struct AdjArgs {
     int per;         /* tets per element */
     double* dpos;    /* dL/dpos, npart x 3 */
     double* dvel;    /* dL/dvel, npart x 3 */
+    const uint64_t* prm; /* refined meshes: the tets' parameter records (else null) */
 };
+
+/* add val to component b of particle-indexed out for vertex v of tet t:
+ * the vertex is a node of the element (level 0: the fixed 6-tet split), or
+ * (refined) the element's map at the tet's parameter point, whose weights
+ * spread the value over the element's nodes */
+__device__ __forceinline__ void adj_scatter(const AdjArgs& A, int64_t t, int v, int b, double val, double* out) {
+    const MeshDev& md = A.md;
+    if (!A.prm) {
+        const int64_t e = A.e0 + t / A.per;
+        const int64_t id = md.conn[e * md.vpe + tet_node(md.vpe, (int)(t % A.per), v)];
+        unsafeAtomicAdd(out + 3 * id + b, val);
+        return;
+    }
+    const uint64_t* r = A.prm + PREC * t;
+    const int64_t e = (int64_t)r[0];
+    if (md.vpe == 4) {
+        unsafeAtomicAdd(out + 3 * (int64_t)md.conn[4 * e + v] + b, val);
+        return;
+    }
+    double xi[3], wn[27];
+    unpack_param(r, v, xi);
+    const int nn = node_weights(md.vpe, xi, wn);
+    for (int n = 0; n < nn; ++n)
+        if (wn[n] != 0.0) unsafeAtomicAdd(out + 3 * (int64_t)md.conn[e * md.vpe + n] + b, val * wn[n]);
+}
 
 /* Per cell: the adjoint weights dL/d(M, Sv, Svv) (ADJ_NW values) from the
  * forward's fields m, v, vv and the upstream gradients gm, gv, gvv (any may
@@ -1216,13 +1243,8 @@ __global__ __launch_bounds__(FBLOCK) void k_adj(const double* __restrict__ tets,
     double o[4];
     Poly2<FBLOCK> R{&ru[0][lane], &rv[0][lane]};
     adj_plane_task<FBLOCK, FULL>(g, mb, tets + (int64_t)TREC * t, w, A.w, R, o);
-    const int64_t e = A.e0 + t / A.per;
-    const int tl = (int)(t % A.per);
-    for (int v = 0; v < 4; ++v) {
-        if (o[v] == 0.0) continue;
-        const int64_t id = A.md.conn[e * A.md.vpe + tet_node(A.md.vpe, tl, v)];
-        unsafeAtomicAdd(A.dpos + 3 * id + ax, o[v]);
-    }
+    for (int v = 0; v < 4; ++v)
+        if (o[v] != 0.0) adj_scatter(A, t, v, ax, o[v], A.dpos);
 }
 
 /* one lane per column task: the velocity part of the adjoint (FP64 column
@@ -1244,13 +1266,9 @@ __global__ __launch_bounds__(FBLOCK) void k_adj_vel(const double* __restrict__ t
         atomicOr(status, ST_OVERFLOW);
         return;
     }
-    const int64_t e = A.e0 + t / A.per;
-    const int tl = (int)(t % A.per);
-    for (int v = 0; v < 4; ++v) {
-        const int64_t id = A.md.conn[e * A.md.vpe + tet_node(A.md.vpe, tl, v)];
+    for (int v = 0; v < 4; ++v)
         for (int b = 0; b < 3; ++b)
-            if (o[v][b] != 0.0) unsafeAtomicAdd(A.dvel + 3 * id + b, o[v][b]);
-    }
+            if (o[v][b] != 0.0) adj_scatter(A, t, v, b, o[v][b], A.dvel);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1314,7 +1332,7 @@ struct Workspace {
     DevBuf connchk, pos, vel, mass, conn, fields[6], acc, tets, counts, offs, scan_tmp, counters, colcnt, colstart, tasks,
         colsum, colkey, colmom, perm, cut, tetb, ptets, adj_w, adj_g, bmesh, bfield, bacc, btable, bmcnt,
         an_box, an_valid, an_key, an_key2, an_id, an_id2, an_pcnt, an_poff, an_pairs, an_cntA, an_offA, an_cntB,
-        an_offB, an_ta, an_tb, an_tcnt, an_toff, an_tasks, adj_in[7], adj_v;
+        an_offB, an_ta, an_tb, an_tcnt, an_toff, an_tasks, adj_in[7], adj_v, adj_prm;
     hipStream_t stream = nullptr;
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -2217,8 +2235,6 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
         return set_err(PSIVOX_E_MODE, "the adjoint takes gradients of m, v and vv only");
     const bool full = dl_df[PSIVOX_V] || dl_df[PSIVOX_VV];
     if (!dl_df[PSIVOX_M] && !full) return set_err(PSIVOX_E_ARG, "no field gradient given");
-    if (o->refine_max_lvl > 0 && mesh->elemtype != 4)
-        return set_err(PSIVOX_E_MODE, "the adjoint supports refine_max_lvl = 0 only");
     if (full && (!grid->fields[PSIVOX_M] || !grid->fields[PSIVOX_V] || (dl_df[PSIVOX_VV] && !grid->fields[PSIVOX_VV])))
         return set_err(PSIVOX_E_ARG, "the adjoint of v and vv needs the forward's fields m, v (and vv) in grid->fields");
     GridInfo g;
@@ -2302,14 +2318,67 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
     A.per = per;
     A.dpos = dpos;
     A.dvel = dvel;
-    for (int64_t e0 = 0; e0 < ne; e0 += batch) {
-        const int64_t e1 = std::min(ne, e0 + batch);
-        const int64_t nt = (e1 - e0) * per;
-        A.e0 = e0;
-        if ((rc = W.tets.get((size_t)nt * TREC * sizeof(double), dev))) return rc;
-        double* tets = (double*)W.tets.p;
-        launch_expand_fixed(md, g, mb, e0, e1 - e0, tets, 0.0, s);
+    A.prm = nullptr;
+    /* refinement: the forward's expansion (counts, offsets), each tet with
+     * its parameter record, so that the gradients of its vertices reach the
+     * element's nodes; the refinement decisions are piecewise constant in
+     * the positions and carry no gradient */
+    const int maxlvl = vpe == 4 ? 0 : std::min(o->refine_max_lvl, (int)MAXLVL);
+    const double dxmin = std::min(g.d[0], std::min(g.d[1], g.d[2]));
+    const double tol2 = (o->refine_tolerance * dxmin) * (o->refine_tolerance * dxmin);
+    std::vector<int64_t> hoffs;
+    if (maxlvl > 0 && ne > 0) {
+        if ((rc = W.counts.get((size_t)(ne + 1) * sizeof(int64_t), dev))) return rc;
+        if ((rc = W.offs.get((size_t)(ne + 1) * sizeof(int64_t), dev))) return rc;
+        int64_t* counts = (int64_t*)W.counts.p;
+        HIPCHK(hipMemsetAsync(counts + ne, 0, sizeof(int64_t), s));
+        k_expand_count<<<grid_blocks(ne, 128), 128, 0, s>>>(md, g, mb, ne, maxlvl, tol2, counts);
         HIPCHK(hipGetLastError());
+        size_t tmp_bytes = 0;
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, (int64_t*)W.offs.p, ne + 1, s));
+        if ((rc = W.scan_tmp.get(tmp_bytes, dev))) return rc;
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp.p, tmp_bytes, counts, (int64_t*)W.offs.p, ne + 1, s));
+        hoffs.resize(ne + 1);
+        HIPCHK(hipMemcpyAsync(hoffs.data(), W.offs.p, (ne + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    const int64_t max_tets_batch = (int64_t)6 << 22;
+    for (int64_t e0 = 0; e0 < ne;) {
+        int64_t e1, nt;
+        if (maxlvl > 0) {
+            /* largest e1 with offs[e1] - offs[e0] <= max_tets_batch (at least one element) */
+            e1 = e0 + 1;
+            int64_t lo = e0 + 1, hi = ne;
+            while (lo <= hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if (hoffs[mid] - hoffs[e0] <= max_tets_batch) {
+                    e1 = mid;
+                    lo = mid + 1;
+                } else
+                    hi = mid - 1;
+            }
+            nt = hoffs[e1] - hoffs[e0];
+        } else {
+            e1 = std::min(ne, e0 + batch);
+            nt = (e1 - e0) * per;
+        }
+        A.e0 = e0;
+        if ((rc = W.tets.get((size_t)std::max<int64_t>(nt, 1) * TREC * sizeof(double), dev))) return rc;
+        double* tets = (double*)W.tets.p;
+        if (maxlvl > 0) {
+            if ((rc = W.adj_prm.get((size_t)std::max<int64_t>(nt, 1) * PREC * sizeof(uint64_t), dev))) return rc;
+            A.prm = (const uint64_t*)W.adj_prm.p;
+            k_expand_emit<<<grid_blocks(e1 - e0, 128), 128, 0, s>>>(md, g, mb, e0, e1, maxlvl, tol2,
+                                                                    (const int64_t*)W.offs.p, hoffs[e0], tets,
+                                                                    (uint64_t*)W.adj_prm.p);
+        } else {
+            launch_expand_fixed(md, g, mb, e0, e1 - e0, tets, 0.0, s);
+        }
+        HIPCHK(hipGetLastError());
+        if (nt == 0) {
+            e0 = e1;
+            continue;
+        }
         if (dpos) {
             /* plane tasks: count, scan, emit (chunks of tets whose tasks fit) */
             if ((rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
@@ -2363,6 +2432,7 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
             (void)hipStreamSynchronize(s);
             return set_err(PSIVOX_E_CANCELLED, "cancelled after %lld of %lld elements", (long long)e1, (long long)ne);
         }
+        e0 = e1;
     }
     if (dpos && dpos != dl_dpos) HIPCHK(hipMemcpyAsync(dl_dpos, dpos, gbytes, hipMemcpyDeviceToHost, s));
     if (dvel && dvel != dl_dvel) HIPCHK(hipMemcpyAsync(dl_dvel, dvel, gbytes, hipMemcpyDeviceToHost, s));

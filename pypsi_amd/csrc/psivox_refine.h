@@ -230,16 +230,77 @@ struct RefineScratch {
     double u[MAXLVL + 1][27][3];
     double tmp[125][3];
     int child[MAXLVL + 1];
+    int org[MAXLVL + 1][3]; /* each level's node corner, in units of 1/PARAM_UNIT of the element */
 };
+
+/*
+ * Where a refined tet's vertices sit in its element: every lattice point of
+ * the refinement is the element's own map (trilinear over its 8 corners,
+ * triquadratic over its 27 nodes) at a dyadic parameter point, since the
+ * midpoints of lin27 and the quarter-point weights of quad125 are that map
+ * at those points.  A tet's parameter record is 3 words: the element index,
+ * then the 4 vertices' parameters (x | y << 8 | z << 16, each in units of
+ * 1/PARAM_UNIT, two vertices per word).  The adjoint maps a vertex's
+ * gradient onto the element's nodes with the map's weights there.
+ */
+constexpr int PARAM_UNIT = 2 << MAXLVL; /* 128: the lattice of the deepest level */
+constexpr int PREC = 3;
+
+PVX_HD uint32_t pack_param(int x, int y, int z) { return (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16); }
+
+/* the parameter record of tet t of the 6-tet split of a 27-point lattice
+ * with corner o and point spacing st (lattice point l = 9i + 3j + k) */
+PVX_HD void emit6_params(uint64_t* prm, int64_t e, const int* o, int st) {
+    for (int t = 0; t < 6; ++t) {
+        uint32_t xi[4];
+        for (int v = 0; v < 4; ++v) {
+            const int l = tet27(t, v);
+            xi[v] = pack_param(o[0] + st * (l / 9), o[1] + st * ((l / 3) % 3), o[2] + st * (l % 3));
+        }
+        prm[PREC * t] = (uint64_t)e;
+        prm[PREC * t + 1] = (uint64_t)xi[0] | ((uint64_t)xi[1] << 32);
+        prm[PREC * t + 2] = (uint64_t)xi[2] | ((uint64_t)xi[3] << 32);
+    }
+}
+
+/* vertex v's parameters of a parameter record */
+PVX_HD void unpack_param(const uint64_t* prm, int v, double* xi) {
+    const uint32_t w = (uint32_t)(prm[1 + (v >> 1)] >> (32 * (v & 1)));
+    xi[0] = (double)(w & 255u) / PARAM_UNIT;
+    xi[1] = (double)((w >> 8) & 255u) / PARAM_UNIT;
+    xi[2] = (double)((w >> 16) & 255u) / PARAM_UNIT;
+}
+
+/* the element map's weights at xi: trilinear over the corners 4i+2j+k
+ * (vpe 8) or triquadratic over the nodes 9i+3j+k at 0, 1/2, 1 (vpe 27) */
+PVX_HD int node_weights(int vpe, const double* xi, double* w) {
+    if (vpe == 8) {
+        for (int n = 0; n < 8; ++n) {
+            const int i = n >> 2, j = (n >> 1) & 1, k = n & 1;
+            w[n] = (i ? xi[0] : 1.0 - xi[0]) * (j ? xi[1] : 1.0 - xi[1]) * (k ? xi[2] : 1.0 - xi[2]);
+        }
+        return 8;
+    }
+    double L[3][3];
+    for (int a = 0; a < 3; ++a) {
+        const double t = xi[a];
+        L[a][0] = 2.0 * (t - 0.5) * (t - 1.0);
+        L[a][1] = 4.0 * t * (1.0 - t);
+        L[a][2] = 2.0 * t * (t - 0.5);
+    }
+    for (int n = 0; n < 27; ++n) w[n] = L[0][n / 9] * L[1][(n / 3) % 3] * L[2][n % 3];
+    return 27;
+}
 
 /*
  * Expand element e into tets.  If out is null only counts.  Returns the tet
  * count (0 if the element misses the window).  Corners in the mesh's
- * connectivity order; vpe in {4, 8, 27}.
+ * connectivity order; vpe in {4, 8, 27}.  prm (optional, with out): each
+ * tet's parameter record (element index and its vertices' parameters).
  */
 PVX_HD int64_t expand_element(const double* pos, const double* vel, const double* mass, const int32_t* conn,
                               int vpe, int64_t e, const GridInfo& g, const MeshBox& mb, int maxlvl, double tol2,
-                              RefineScratch& S, double* out) {
+                              RefineScratch& S, double* out, uint64_t* prm = nullptr) {
     double P[27][3], U[27][3];
     for (int v = 0; v < vpe; ++v) {
         int64_t id = conn[e * vpe + v];
@@ -261,6 +322,11 @@ PVX_HD int64_t expand_element(const double* pos, const double* vel, const double
             out[25] = 1.0;
             out[26] = 1.0;
             out[27] = 0.0;
+            if (prm) {
+                /* a tet element is its own tet: vertex v is node v */
+                prm[0] = (uint64_t)e;
+                prm[1] = prm[2] = 0;
+            }
         }
         return 1;
     }
@@ -282,6 +348,7 @@ PVX_HD int64_t expand_element(const double* pos, const double* vel, const double
     int64_t count = 0;
     int d = 0;
     double dm = m;
+    S.org[0][0] = S.org[0][1] = S.org[0][2] = 0;
     /* visit node d */
     for (;;) {
         bool refine = d < maxlvl && needs_refine((const double (*)[3])S.p[d], quad, tol2);
@@ -294,15 +361,24 @@ PVX_HD int64_t expand_element(const double* pos, const double* vel, const double
             S.child[d] = 0;
         } else {
             if (out) {
+                /* a node of level d spans PARAM_UNIT >> d; its lattice points
+                 * are half that apart */
+                const int half = (PARAM_UNIT / 2) >> d;
                 if (last) {
                     for (int c = 0; c < 8; ++c) {
                         double c8p[8][3], c8u[8][3];
                         child_corners((const double (*)[3])S.p[d], (const double (*)[3])S.u[d], c, c8p, c8u);
                         emit6_corners(out + TREC * (count + 6 * c), (const double (*)[3])c8p,
                                       (const double (*)[3])c8u, dm * 0.125);
+                        if (prm) {
+                            const int oc[3] = {S.org[d][0] + half * ((c >> 2) & 1), S.org[d][1] + half * ((c >> 1) & 1),
+                                               S.org[d][2] + half * (c & 1)};
+                            emit6_params(prm + PREC * (count + 6 * c), e, oc, half / 2);
+                        }
                     }
                 } else {
                     emit6(out + TREC * count, (const double (*)[3])S.p[d], (const double (*)[3])S.u[d], dm);
+                    if (prm) emit6_params(prm + PREC * count, e, S.org[d], half);
                 }
             }
             count += last ? 48 : 6;
@@ -317,6 +393,12 @@ PVX_HD int64_t expand_element(const double* pos, const double* vel, const double
         /* descend into child S.child[d] */
         child_node((const double (*)[3])S.p[d], (const double (*)[3])S.u[d], S.child[d], quad, S.p[d + 1],
                    S.u[d + 1], S.tmp, !counting);
+        {
+            const int half = (PARAM_UNIT / 2) >> d, c = S.child[d];
+            S.org[d + 1][0] = S.org[d][0] + half * ((c >> 2) & 1);
+            S.org[d + 1][1] = S.org[d][1] + half * ((c >> 1) & 1);
+            S.org[d + 1][2] = S.org[d][2] + half * (c & 1);
+        }
         S.child[d]++;
         d++;
         dm *= 0.125;

@@ -231,6 +231,96 @@ extern "C" int hs_voxels_plain(int vpe, int64_t npart, int64_t nelem, const doub
 
 /* the mass adjoint of the GPU kernels on the host (refine level 0): dpos
  * (npart x 3) overwritten with dL/dpos for w = dL/dm */
+/* every refined tet's parameter record checked against its vertices: the
+ * element's map (wrapped nodes for positions) at each vertex's parameters
+ * reproduces the vertex.  out: max position error, max velocity error, tet
+ * count */
+extern "C" int hs_expand_params(int vpe, int64_t nelem, const double* pos, const double* vel, const double* mass,
+                                const int32_t* conn, int periodic, const double* box6, const int* n,
+                                const double* win6, const double* d, double reftol, int maxlvl, double* out) {
+    double* fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    GridInfo g;
+    MeshBox mb;
+    fill_grid(g, mb, n, win6, d, fields, 0, periodic, box6);
+    const double dxmin = std::fmin(g.d[0], std::fmin(g.d[1], g.d[2]));
+    const double tol2 = (reftol * dxmin) * (reftol * dxmin);
+    RefineScratch* RS = new RefineScratch;
+    std::vector<double> tets;
+    std::vector<uint64_t> prm;
+    double ep = 0.0, eu = 0.0, ntot = 0.0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        const int64_t nt = expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, nullptr);
+        if (nt == 0) continue;
+        tets.resize((size_t)nt * TREC);
+        prm.resize((size_t)nt * PREC);
+        expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, tets.data(), prm.data());
+        double P[27][3];
+        for (int v = 0; v < vpe; ++v)
+            for (int a = 0; a < 3; ++a) P[v][a] = pos[3 * (int64_t)conn[e * vpe + v] + a];
+        wrap_and_test(P, vpe, g, mb);
+        for (int64_t t = 0; t < nt; ++t) {
+            const uint64_t* r = &prm[(size_t)t * PREC];
+            if ((int64_t)r[0] != e) return -1;
+            for (int v = 0; v < 4; ++v) {
+                double w[27];
+                int nn = 0;
+                if (vpe == 4) {
+                    for (int q = 0; q < 4; ++q) w[q] = q == v ? 1.0 : 0.0;
+                    nn = 4;
+                } else {
+                    double xi[3];
+                    unpack_param(r, v, xi);
+                    nn = node_weights(vpe, xi, w);
+                }
+                for (int a = 0; a < 3; ++a) {
+                    double x = 0.0, u = 0.0;
+                    for (int q = 0; q < nn; ++q) {
+                        x += w[q] * P[q][a];
+                        u += w[q] * vel[3 * (int64_t)conn[e * vpe + q] + a];
+                    }
+                    ep = std::fmax(ep, std::fabs(x - tets[(size_t)t * TREC + 3 * v + a]));
+                    eu = std::fmax(eu, std::fabs(u - tets[(size_t)t * TREC + 12 + 3 * v + a]));
+                }
+            }
+        }
+        ntot += (double)nt;
+    }
+    delete RS;
+    out[0] = ep;
+    out[1] = eu;
+    out[2] = ntot;
+    return 0;
+}
+
+/* the refined expansion of elements [0, nelem): tet records (TREC doubles
+ * each) and parameter records (PREC words each) into caller buffers of cap
+ * tets; returns the tet count (or -1 when cap is too small) */
+extern "C" int64_t hs_expand_records(int vpe, int64_t nelem, const double* pos, const double* vel, const double* mass,
+                                     const int32_t* conn, int periodic, const double* box6, const int* n,
+                                     const double* win6, const double* d, double reftol, int maxlvl, int64_t cap,
+                                     double* tets, uint64_t* prm) {
+    double* fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    GridInfo g;
+    MeshBox mb;
+    fill_grid(g, mb, n, win6, d, fields, 0, periodic, box6);
+    const double dxmin = std::fmin(g.d[0], std::fmin(g.d[1], g.d[2]));
+    const double tol2 = (reftol * dxmin) * (reftol * dxmin);
+    RefineScratch* RS = new RefineScratch;
+    int64_t tot = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        const int64_t nt = expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, nullptr);
+        if (tot + nt > cap) {
+            delete RS;
+            return -1;
+        }
+        if (nt) expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, tets + TREC * tot,
+                               prm + PREC * tot);
+        tot += nt;
+    }
+    delete RS;
+    return tot;
+}
+
 extern "C" int hs_mass_adjoint(int vpe, int64_t npart, int64_t nelem, const double* pos, const double* mass,
                                const int32_t* conn, int periodic, const double* box6, const int* n, const double* win6,
                                const double* d, const double* w, double* dpos, int64_t* out_stats) {

@@ -130,3 +130,40 @@ def adjoint(P, W):
                              vp(P.vel), vp(P.mass), vp(P.conn), C.c_int(int(P.periodic)), box, n, win, d, vp(W),
                              vp(dpos), vp(dvel))
     return dpos, dvel, fails
+
+
+def expand_params(P, reftol=1.0, maxlvl=0):
+    """Check every refined tet's parameter record against its vertices (the
+    element's map at the vertex parameters): (max position error, max
+    velocity error, tet count)."""
+    out = np.zeros(3)
+    n = (C.c_int * 3)(*P.n)
+    win = (C.c_double * 6)(*P.bounds.reshape(-1))
+    d = (C.c_double * 3)(*P.d)
+    box = (C.c_double * 6)(*P.box.reshape(-1))
+    vp = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    rc = lib().hs_expand_params(C.c_int(P.elemtype), C.c_int64(P.conn.shape[0]), vp(P.pos), vp(P.vel), vp(P.mass),
+                                vp(P.conn), C.c_int(int(P.periodic)), box, n, win, d, C.c_double(reftol),
+                                C.c_int(maxlvl), vp(out))
+    if rc != 0:
+        raise RuntimeError("parameter record names the wrong element")
+    return out[0], out[1], int(out[2])
+
+
+def expand_records(P, reftol=1.0, maxlvl=0, cap=1 << 20):
+    """The refined expansion: (tets (nt, 28), params (nt, 3) uint64)."""
+    tets = np.zeros((cap, 28))
+    prm = np.zeros((cap, 3), np.uint64)
+    n = (C.c_int * 3)(*P.n)
+    win = (C.c_double * 6)(*P.bounds.reshape(-1))
+    d = (C.c_double * 3)(*P.d)
+    box = (C.c_double * 6)(*P.box.reshape(-1))
+    vp = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    f = lib().hs_expand_records
+    f.restype = C.c_int64
+    nt = f(C.c_int(P.elemtype), C.c_int64(P.conn.shape[0]), vp(P.pos), vp(P.vel), vp(P.mass), vp(P.conn),
+           C.c_int(int(P.periodic)), box, n, win, d, C.c_double(reftol), C.c_int(maxlvl), C.c_int64(cap), vp(tets),
+           vp(prm))
+    if nt < 0:
+        raise RuntimeError("expansion larger than cap")
+    return tets[:nt], prm[:nt]

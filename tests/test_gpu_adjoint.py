@@ -14,7 +14,7 @@ dL/dm: 7.2e-5 measured).
 import numpy as np
 import pytest
 
-from adjoint_cases import BOX, cases, loss_weights, sample
+from adjoint_cases import BOX, cases, loss_weights, refined_fd, refined_problem, sample
 
 pytestmark = pytest.mark.gpu
 
@@ -93,19 +93,75 @@ def test_adjoint_accumulates(oracle_mod):
     assert np.abs(out.cpu().numpy() - 1.0 - 2.0 * go).max() <= 1e-9 * np.abs(go).max()
 
 
-def test_adjoint_rejects_refinement(oracle_mod):
+@pytest.mark.parametrize("kind", ["trilinear", "triquadratic"])
+def test_refined_adjoint_vs_fd(oracle_mod, kind):
+    """The adjoint on refined meshes: every refined tet's vertex is the
+    element's map (trilinear or triquadratic) at a dyadic parameter point,
+    so a vertex's gradient reaches the element's nodes with the map's weights
+    there; the refinement decisions are piecewise constant and carry none.
+    The mass adjoint (loss sum_C w_C m_C) and the fields adjoint (loss
+    sum_C G_m m + G_v . v + G_vv : vv, v and vv weighted on cells holding at
+    least 1e-3 of the peak mass) through the C ABI, against FP64 central
+    differences of the oracle's refined forward (h = 1e-6) on 60 position and
+    60 velocity components: within 1e-6 (mass) and 1e-5 (fields) of the
+    largest difference."""
     import torch
     from pypsi_amd import device as D
-    label, P = cases(oracle_mod)[0]
+    O = oracle_mod
+    print("case: refined adjoint vs fd,", kind, flush=True)
+    P, reftol, lvl = refined_problem(O, kind)
+    ref, st = O.voxels(P, reftol=reftol, maxlvl=lvl)
+    per = 6 if P.conn.shape[1] != 4 else 1
+    assert st["ntets"] > per * P.conn.shape[0], "the case must refine"
+    rng = np.random.default_rng(11)
+    npart = P.pos.shape[0]
+    comps = [(int(p), int(a)) for p, a in zip(rng.integers(0, npart, 60), rng.integers(0, 3, 60))]
     dev = torch.device("cuda:0")
-    pos = torch.from_numpy(P.pos).to(dev)
-    mass = torch.from_numpy(P.mass).to(dev)
-    conn = torch.from_numpy(P.conn).to(dev)
-    mesh = D.make_mesh(pos, None, mass, conn, P.box)
-    grid = D.make_grid(P.n, P.bounds, {})
-    tw = torch.zeros(P.n, dtype=torch.float64, device=dev)
-    with pytest.raises(RuntimeError, match="refine_max_lvl"):
-        D.mass_adjoint(mesh, grid, tw, torch.zeros_like(pos), D.make_opts(refine_max_lvl=1))
+    tpos, tvel = torch.from_numpy(P.pos).to(dev), torch.from_numpy(P.vel).to(dev)
+    mesh = D.make_mesh(tpos, tvel, torch.from_numpy(P.mass).to(dev), torch.from_numpy(P.conn).to(dev),
+                       BOX if P.periodic else None)
+    opts = D.make_opts(kernel="plain", refine_tolerance=reftol, refine_max_lvl=lvl,
+                       stream=torch.cuda.current_stream().cuda_stream)
+    # mass: L = sum w m
+    w = rng.standard_normal(P.n)
+    g = torch.zeros_like(tpos)
+    D.mass_adjoint(mesh, D.make_grid(P.n, BOX, {}), torch.from_numpy(w).to(dev), g, opts)
+    torch.cuda.synchronize()
+    fd = refined_fd(O, P, lambda f: float((w * f["m"]).sum()), 0, comps, reftol, lvl)
+    gm = g.cpu().numpy()[tuple(np.array(comps).T)]
+    em = np.abs(gm - fd).max() / np.abs(fd).max()
+    print("mass adjoint vs fd: %.2e (max |fd| %.3e)" % (em, np.abs(fd).max()), flush=True)
+    assert em <= 1e-6
+    # the same through torch.autograd (voxel_mass with the refinement settings)
+    from pypsi_amd.autograd import voxel_mass
+    pa = tpos.clone().requires_grad_(True)
+    mm = voxel_mass(pa, torch.from_numpy(P.conn).to(dev), P.n, BOX, mass=torch.from_numpy(P.mass).to(dev),
+                    periodic_box=BOX if P.periodic else None, kernel="plain", refine_tolerance=reftol,
+                    refine_max_lvl=lvl)
+    (mm * torch.from_numpy(w).to(dev)).sum().backward()
+    assert torch.allclose(pa.grad, g, rtol=1e-12, atol=1e-12 * float(g.abs().max()))
+    # fields: the forward on the GPU (FP64), then the adjoint of m, v, vv
+    fl = {k: torch.zeros(v.shape, dtype=torch.float64, device=dev) for k, v in ref.items()}
+    grid = D.make_grid(P.n, BOX, fl)
+    D.voxels(mesh, grid, opts)
+    heavy = ref["m"] >= 1e-3 * ref["m"].max()
+    G = {k: rng.standard_normal(ref[k].shape) for k in ("m", "v", "vv")}
+    G["v"][~heavy] = 0.0
+    G["vv"][~heavy] = 0.0
+    tg = {k: torch.from_numpy(v).to(dev) for k, v in G.items()}
+    dpos, dvel = torch.zeros_like(tpos), torch.zeros_like(tvel)
+    D.adjoint(mesh, grid, tg, dpos, dvel, opts)
+    torch.cuda.synchronize()
+
+    def loss(f):
+        return float(sum((G[k] * f[k]).sum() for k in G))
+
+    for which, gg, name in ((0, dpos, "positions"), (1, dvel, "velocities")):
+        fd = refined_fd(O, P, loss, which, comps, reftol, lvl)
+        gv = gg.cpu().numpy()[tuple(np.array(comps).T)]
+        e = np.abs(gv - fd).max() / np.abs(fd).max()
+        print("fields adjoint vs fd, %s: %.2e (max |fd| %.3e)" % (name, e, np.abs(fd).max()), flush=True)
+        assert e <= 1e-5
 
 
 def _c4(oracle_mod):
@@ -231,12 +287,13 @@ def test_fields_adjoint_vs_oracle(oracle_mod, periodic):
     assert ep <= 1e-9 and ev <= 1e-9
 
 
-@pytest.mark.parametrize("kernel,tol", [("fast64", 1e-4), ("plain", 1e-4), ("auto", 2e-4)])
+@pytest.mark.parametrize("kernel,tol", [("fast64", 1e-4), ("plain", 1e-4), ("auto", 3e-4)])
 def test_fields_autograd_vs_fd(oracle_mod, kernel, tol):
     """voxel_fields end to end: GPU forward of m, v, vv (kernel; its default
-    is the FP64 column kernel 'fast64', measured 1.5e-7; the FP32-clip
+    is the FP64 column kernel 'fast64', measured 1.6e-7; the FP32-clip
     forward's cell fields carry their FP32 error into the weights, measured
-    8.2e-5 on velocities, bar 2e-4), the loss
+    8e-5 to 1.8e-4 on velocities as the clips' rounding pattern changes,
+    bar 3e-4), the loss
     sum_C G_m m + G_v . v + G_vv : vv in torch with seeded random
     coefficients, backward through psivox_adjoint; against FP64 central
     differences of the same loss (oracle forward, h = 1e-6) on 300 seeded
