@@ -343,12 +343,14 @@ def annihilation_leg(steps=3):
     return out
 
 
-def plain_leg(mesh, grid, tf, stream, nfrag, steps=2, warmup=1):
-    """The FP64 one-thread-per-tet kernel on the same inputs (the plain GPU
-    method the FP32-clip column kernel is measured against)."""
+def plain_leg(mesh, grid, tf, stream, nfrag, steps=2, warmup=1, kernel="plain"):
+    """An FP64 kernel on the same inputs: 'plain', one thread per tet with the
+    reference's cut sequence (the plain GPU method: its four polyhedra live
+    in per-lane scratch memory), or 'fast64', the column kernel with every
+    tet in FP64 (the column method at the reference's precision)."""
     import torch
     from pypsi_amd import device as D
-    opts = D.make_opts(kernel="plain", stream=stream)
+    opts = D.make_opts(kernel=kernel, stream=stream)
 
     def step():
         for t in tf.values():
@@ -884,6 +886,10 @@ def main():
         out["c1"] = c1_leg()
         log("[bench] plain FP64 kernel on the same workload")
         out["plain"] = plain_leg(mesh, grid, tf, stream, nfrag)
+        log("[bench] FP64 column kernel on the same workload")
+        out["fp64_columns"] = plain_leg(mesh, grid, tf, stream, nfrag, steps=3, kernel="fast64")
+        out["fp64_columns"]["what"] = ("the column kernel with every tet clipped in FP64 (kernel 'fast64'): "
+                                       "the FP32 path's speed-up at equal method, value / this value")
         log("[bench] forward + adjoint, 64^3 -> 128^3")
         out["fwd_adjoint"] = fwd_adjoint_leg()
         log("[bench] batched uneven meshes")

@@ -598,22 +598,20 @@ struct TaskOut {
  * wavefronts with more nearly equal lanes: on the Zel'dovich 128^3 ->
  * 256^3 workload the column pass takes 56.7 ms with windows of 1024 tasks,
  * 53.0 ms with 4096, 51.7 ms with 16384 and 51.4 ms with 65536 (whose
- * ordering costs more than that saves).  A batch keeps 1024: a wavefront
- * then spans fewer meshes, each of which it takes in turn. */
-template <bool BATCH>
-constexpr int ordw() { return BATCH ? 1024 : 16384; }
+ * ordering costs more than that saves); the 64-mesh batch, 530 -> 488 ms
+ * (a wavefront spans two meshes only where a window crosses a mesh
+ * boundary). */
+constexpr int ORDW = 16384;
 constexpr int ORDK = 256;
-static_assert(ordw<false>() % 256 == 0 && ordw<true>() % 256 == 0, "whole threads per window");
+static_assert(ORDW % 256 == 0, "whole threads per window");
 
-/* perm[w0 .. w0 + ORDW) of each window (ORDW = ordw<BATCH>()): the
- * window's task indices, largest class first (order within a class is
- * arbitrary).  Also clears the column kernel's wavefront counter.  256
+/* perm[w0 .. w0 + ORDW) of each window: the window's task indices, largest
+ * class first (order within a class is arbitrary).  Also clears the column kernel's wavefront counter.  256
  * threads: one class each in the scan. */
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
                                                    int64_t ntask, GridInfo g0, MeshBox mb0, const BatchGeo* table,
                                                    uint32_t* __restrict__ perm, unsigned* __restrict__ next) {
-    constexpr int ORDW = ordw<BATCH>();
     __shared__ int cnt[ORDK];
     const int64_t w0 = (int64_t)blockIdx.x * ORDW;
     const int nw = (int)(ntask - w0 < ORDW ? ntask - w0 : ORDW);
@@ -1079,12 +1077,11 @@ static KColsFn cols_kernel(const GridInfo& g, bool fp64, bool batch) {
 static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask, const PassGeo& G, bool fp64,
                        TaskOut out, uint32_t* perm, unsigned* next, unsigned* status, unsigned long long* counters,
                        hipStream_t s, hipEvent_t mark = nullptr) {
+    const unsigned nord = (unsigned)((ntask + ORDW - 1) / ORDW);
     if (G.table)
-        k_col_order<true><<<(unsigned)((ntask + ordw<true>() - 1) / ordw<true>()), 256, 0, s>>>(
-            tets, tasks, ntask, G.g, G.mb, G.table, perm, next);
+        k_col_order<true><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, G.table, perm, next);
     else
-        k_col_order<false><<<(unsigned)((ntask + ordw<false>() - 1) / ordw<false>()), 256, 0, s>>>(
-            tets, tasks, ntask, G.g, G.mb, nullptr, perm, next);
+        k_col_order<false><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, nullptr, perm, next);
     if (mark) (void)hipEventRecord(mark, s);
     const int64_t nwave = (ntask + FBLOCK - 1) / FBLOCK;
     const size_t sb = stage_bytes(G.g, SFRAG);
