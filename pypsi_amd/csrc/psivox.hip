@@ -2740,6 +2740,7 @@ int upload_tables(int dev) {
     if (done[dev]) return 0;
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(slab_steps), host_slab_steps(), sizeof(SlabStep) * 81, 0,
                              hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(y_steps), host_y_steps(), sizeof(YStep) * YSTEPS, 0, hipMemcpyHostToDevice));
     done[dev] = true;
     return 0;
 }

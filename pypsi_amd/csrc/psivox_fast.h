@@ -1203,7 +1203,7 @@ PVX_HD const SlabStep& slab_step(int p) {
  * a window slab thinner than the precision resolves); P then holds the tet. */
 template <class R>
 PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g, bool has_hi, double chi, bool has_lo,
-                            double clo) {
+                            double clo, int* patp = nullptr) {
     tet_start(P);
     const AxisMap<R> m = axis_map(G, g, 0);
     const R offh = (R)(chi - m.od), offl = (R)(clo - m.od);
@@ -1215,6 +1215,7 @@ PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g,
         tl[v] = has_lo ? sdist(G.rec, m, clo, offl, q0, q1, q2) : R(-1);
         pat = 3 * pat + (th[v] < R(0) ? 2 : (tl[v] >= R(0) ? 0 : 1));
     }
+    if (patp) *patp = pat;
     const SlabStep& E = slab_step(pat);
     if (E.nv == 0) {
         P.live = 0u;
@@ -1259,6 +1260,138 @@ PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g,
     return true;
 }
 
+/*
+ * One y plane clipped off an x-slab polyhedron, tabulated.  A clip's graph
+ * work depends only on the graph and the set of vertices it drops; the
+ * x-slab polyhedron of table pattern pat is entry pat's graph (its slots,
+ * link words and order), so the result of dropping a set of its vertices
+ * (a bit mask over their ranks: 256 sets) is made once by running
+ * clip_graph on that graph: per output vertex (rank order) its slot, link
+ * word (fan apexes included) and source (a vertex of the slab polyhedron,
+ * or the cut of the edge from slot vc to slot vn).  The lane classifies the
+ * slab polyhedron's vertices as clip_f does, looks the set up and evaluates
+ * the cut points with clip_f's own formula, so the result is clip_f's, bit
+ * for bit.  nv = 0 marks a set with no entry: clip_f then runs.
+ */
+constexpr int YNV = 12;
+constexpr int YSTEPS = 81 * 256;
+constexpr uint8_t YKEEP = 0xffu;
+
+struct YStep {
+    uint32_t link[YNV];
+    uint8_t slot[YNV];
+    uint8_t src[YNV]; /* YKEEP: a vertex of the slab polyhedron; else vc << 4 | vn */
+    uint8_t nv;
+    uint8_t pad[7];
+};
+
+/* the entry of dropping the ranks in goner from pattern pat's polyhedron,
+ * from the graph steps alone */
+inline bool make_y_step(int pat, uint32_t goner, YStep& Y) {
+    Y = YStep{};
+    const SlabStep& E = host_slab_steps()[pat];
+    const int nv = E.nv;
+    if (nv == 0 || goner == 0u || (goner >> nv) != 0u || goner == (1u << nv) - 1u) return false;
+    static uint32_t mem[VRec<float>::WORDS];
+    LPoly<float> P{mem, 0};
+    P.live = 0u;
+    P.ord = 0ull;
+    uint32_t gone = 0u;
+    for (int r = 0; r < nv; ++r) {
+        const int sl = E.slot[r];
+        P.setnbw(sl, E.link[r]);
+        P.live |= 1u << sl;
+        P.ord |= (uint64_t)sl << (4 * r);
+        if ((goner >> r) & 1u) gone |= 1u << sl;
+    }
+    P.nv = nv;
+    uint8_t src[NVF];
+    for (int v = 0; v < NVF; ++v) src[v] = YKEEP;
+    if (!clip_graph(P, gone, [&](int vx, int vc, int vn) { src[vx] = (uint8_t)((vc << 4) | vn); })) return false;
+    if (P.nv > YNV) return false;
+    Y.nv = (uint8_t)P.nv;
+    for (int r = 0; r < P.nv; ++r) {
+        const int sl = P.at(r);
+        Y.slot[r] = (uint8_t)sl;
+        Y.link[r] = P.nbw(sl);
+        Y.src[r] = src[sl];
+    }
+    return true;
+}
+
+/* the 81 x 256 entries, built once on the host */
+inline const YStep* host_y_steps() {
+    static YStep* T = nullptr;
+    if (!T) {
+        YStep* t = new YStep[YSTEPS];
+        for (int p = 0; p < 81; ++p)
+            for (int m = 0; m < 256; ++m) make_y_step(p, (uint32_t)m, t[p * 256 + m]);
+        T = t;
+    }
+    return T;
+}
+
+#ifdef __HIPCC__
+__device__ YStep y_steps[YSTEPS];
+#endif
+
+PVX_HD const YStep& y_step(int i) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return y_steps[i];
+#else
+    return host_y_steps()[i];
+#endif
+}
+
+/* Clip P, the x-slab polyhedron of pattern pat as slab_from_table left it,
+ * by the plane y = c (keep_upper as clip_f) through the y table.  False
+ * when the table has no entry for the dropped set (P unchanged: run
+ * clip_f). */
+template <class R>
+PVX_HD bool clip_y_table(LPoly<R>& P, int pat, const CopyGeo<R>& G, const GridInfo& g, double c, bool keep_upper) {
+    if (P.live == 0u) return true;
+    const AxisMap<R> m = axis_map(G, g, 1);
+    const R off = (R)(c - m.od);
+    const int nv = P.nv;
+    uint32_t goner = 0u;
+    for (int r = 0; r < 8; ++r) {
+        if (r >= nv) continue;
+        const int v = P.at(r);
+        const R t = sdist(G.rec, m, c, off, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
+        const bool drop = keep_upper ? (t >= R(0)) : (t < R(0));
+        goner |= (drop ? 1u : 0u) << r;
+    }
+    if (goner == 0u) return true;
+    if (goner == (1u << nv) - 1u) {
+        P.live = 0u;
+        P.nv = 0;
+        return true;
+    }
+    const YStep& Y = y_step(pat * 256 + (int)goner);
+    const int ny = Y.nv;
+    if (ny == 0) return false;
+    for (int r = 0; r < YNV; ++r) {
+        if (r >= ny || Y.src[r] == YKEEP) continue;
+        const int vx = Y.slot[r], vc = Y.src[r] >> 4, vn = Y.src[r] & 15;
+        const R sc = sdist(G.rec, m, c, off, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
+        const R sn = sdist(G.rec, m, c, off, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
+        cut_point(P, vx, vc, vn, sc, sn);
+    }
+    uint32_t live = 0u;
+    uint64_t ord = 0ull;
+    for (int r = 0; r < YNV; ++r) {
+        if (r >= ny) continue;
+        const int sl = Y.slot[r];
+        P.setnbw(sl, Y.link[r]);
+        live |= 1u << sl;
+        ord |= (uint64_t)sl << (4 * r);
+    }
+    P.live = live;
+    P.nv = ny;
+    P.ord = ord;
+    return true;
+}
+
 /* Build the column polyhedron of task (copy C, cell i, j) in P: the tet
  * clipped by the column's four planes -- interior grid planes or, on the
  * copy's first/last slab or column, the window planes it crosses
@@ -1273,18 +1406,25 @@ PVX_HD bool build_column(LPoly<R>& P, const CopySetup<R>& C, const GridInfo& g, 
     const bool has_hi = xhi || (C.wmask & 2u), has_lo = xlo || (C.wmask & 1u);
     const double chi = xhi ? (double)(il + 1) : C.whi[0], clo = xlo ? (double)il : C.wlo[0];
     bool ok = true;
-    if (!slab_from_table(P, G, g, has_hi, chi, has_lo, clo)) {
+    int pat = -1; /* the table pattern P still is, or -1 */
+    if (!slab_from_table(P, G, g, has_hi, chi, has_lo, clo, &pat)) {
+        pat = -1;
         if (has_hi) ok = clip_f(P, G, g, chi, 0, false);
         if (ok && has_lo) ok = clip_f(P, G, g, clo, 0, true);
     }
-    if (ok && yhi)
-        ok = clip_f(P, G, g, (double)(jl + 1), 1, false);
-    else if (ok && j == C.ib1[1] - 1 && (C.wmask & 8u))
-        ok = clip_f(P, G, g, C.whi[1], 1, false);
-    if (ok && ylo)
-        ok = clip_f(P, G, g, (double)jl, 1, true);
-    else if (ok && j == G.ib0[1] && (C.wmask & 4u))
-        ok = clip_f(P, G, g, C.wlo[1], 1, true);
+    /* the y planes: the first one that cuts the slab polyhedron through the
+     * y table, the rest by the generic clip */
+    const bool yh = yhi || (j == C.ib1[1] - 1 && (C.wmask & 8u));
+    const bool yl = ylo || (j == G.ib0[1] && (C.wmask & 4u));
+    const double cyh = yhi ? (double)(jl + 1) : C.whi[1], cyl = ylo ? (double)jl : C.wlo[1];
+    if (ok && yh) {
+        const uint32_t before = P.live;
+        if (pat < 0 || !clip_y_table(P, pat, G, g, cyh, false)) ok = clip_f(P, G, g, cyh, 1, false);
+        if (P.live != before) pat = -1;
+    }
+    if (ok && yl) {
+        if (pat < 0 || !clip_y_table(P, pat, G, g, cyl, true)) ok = clip_f(P, G, g, cyl, 1, true);
+    }
     return ok;
 }
 

@@ -321,6 +321,96 @@ extern "C" int64_t hs_expand_records(int vpe, int64_t nelem, const double* pos, 
     return tot;
 }
 
+/* every column task of the mesh built twice: by build_column (the y table
+ * where it applies) and by the generic clips alone (slab_from_table, then
+ * clip_f for each plane); the two polyhedra compared bit for bit (live
+ * slots, order, each live slot's q words and link word).  out: tasks,
+ * tasks whose first y clip went through the table, mismatches */
+template <class R>
+static void ycheck_tet(const double* r, int64_t t, const GridInfo& g, const MeshBox& mb, LPoly<R>& A, LPoly<R>& B,
+                       int64_t* out) {
+    std::vector<uint64_t> tasks(4096);
+    const int64_t k = tet_columns<R>(r, t, g, mb, tasks.data(), nullptr);
+    for (int64_t c = 0; c < k; ++c) {
+        int64_t tt;
+        int h, il, jl;
+        unpack_task(tasks[c], tt, h, il, jl);
+        double lo[3], hi[3];
+        aabb((const double (*)[3])r, 4, lo, hi);
+        uint32_t shp;
+        ghost_shifts_packed(lo, hi, g, mb, shp);
+        CopySetup<R> C;
+        if (!copy_setup(r, lo, hi, shp, h, g, mb, C)) continue;
+        const int i = C.G.ib0[0] + il, j = C.G.ib0[1] + jl;
+        int j0, j1;
+        if (!slab_columns(C, i, g, j0, j1)) continue;
+        out[0]++;
+        const bool okA = build_column(A, C, g, i, j, j0, j1);
+        /* the generic sequence */
+        const CopyGeo<R>& G = C.G;
+        const bool xlo = i > G.ib0[0], xhi = i < C.ib1[0] - 1, ylo = j > j0, yhi = j < j1 - 1;
+        const bool has_hi = xhi || (C.wmask & 2u), has_lo = xlo || (C.wmask & 1u);
+        const double chi = xhi ? (double)(il + 1) : C.whi[0], clo = xlo ? (double)il : C.wlo[0];
+        bool okB = true;
+        const bool table = slab_from_table(B, G, g, has_hi, chi, has_lo, clo);
+        if (!table) {
+            if (has_hi) okB = clip_f(B, G, g, chi, 0, false);
+            if (okB && has_lo) okB = clip_f(B, G, g, clo, 0, true);
+        }
+        const uint32_t slab_live = B.live;
+        if (okB && yhi)
+            okB = clip_f(B, G, g, (double)(jl + 1), 1, false);
+        else if (okB && j == C.ib1[1] - 1 && (C.wmask & 8u))
+            okB = clip_f(B, G, g, C.whi[1], 1, false);
+        if (okB && ylo)
+            okB = clip_f(B, G, g, (double)jl, 1, true);
+        else if (okB && j == G.ib0[1] && (C.wmask & 4u))
+            okB = clip_f(B, G, g, C.wlo[1], 1, true);
+        if (table && B.live != slab_live) out[1]++;
+        bool same = okA == okB && A.live == B.live && A.nv == B.nv && A.ord == B.ord;
+        for (uint32_t lm = A.live; same && lm; lm &= lm - 1u) {
+            const int v = __builtin_ctz(lm);
+            for (int w = 0; w < VRec<R>::W; ++w)
+                if (w == VRec<R>::LINK || w < (sizeof(R) == 4 ? 3 : 6))
+                    same = same && A.w[v * VRec<R>::STRIDE + w] == B.w[v * VRec<R>::STRIDE + w];
+        }
+        if (!same) out[2]++;
+    }
+}
+
+extern "C" int hs_ytable_check(int vpe, int64_t nelem, const double* pos, const double* vel, const double* mass,
+                               const int32_t* conn, int periodic, const double* box6, const int* n, const double* win6,
+                               const double* d, double reftol, int maxlvl, int fp64, int64_t* out) {
+    double* fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    GridInfo g;
+    MeshBox mb;
+    fill_grid(g, mb, n, win6, d, fields, 0, periodic, box6);
+    const double dxmin = std::fmin(g.d[0], std::fmin(g.d[1], g.d[2]));
+    const double tol2 = (reftol * dxmin) * (reftol * dxmin);
+    RefineScratch* RS = new RefineScratch;
+    std::vector<uint32_t> la(VRec<double>::WORDS), lb(VRec<double>::WORDS);
+    LPoly<float> A32{la.data(), 0, 0, 0}, B32{lb.data(), 0, 0, 0};
+    LPoly<double> A64{la.data(), 0, 0, 0}, B64{lb.data(), 0, 0, 0};
+    std::vector<double> tets;
+    out[0] = out[1] = out[2] = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        const int64_t nt = expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, nullptr);
+        if (nt == 0) continue;
+        tets.resize((size_t)nt * TREC);
+        expand_element(pos, vel, mass, conn, vpe, e, g, mb, maxlvl, tol2, *RS, tets.data());
+        for (int64_t t = 0; t < nt; ++t) {
+            const double* r = &tets[(size_t)t * TREC];
+            if (r[25] == 0.0) continue;
+            if (fp64)
+                ycheck_tet<double>(r, t, g, mb, A64, B64, out);
+            else
+                ycheck_tet<float>(r, t, g, mb, A32, B32, out);
+        }
+    }
+    delete RS;
+    return 0;
+}
+
 extern "C" int hs_mass_adjoint(int vpe, int64_t npart, int64_t nelem, const double* pos, const double* mass,
                                const int32_t* conn, int periodic, const double* box6, const int* n, const double* win6,
                                const double* d, const double* w, double* dpos, int64_t* out_stats) {

@@ -122,3 +122,29 @@ def test_wide_tets_routed_to_fp64_columns(oracle_mod):
             assert_close(got, ref, TOL32, f"routed {n}^3 -> {gn}^3 periodic={per}")
     finally:
         L.hs_set_route_wide(0)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_y_table_matches_generic_clips(oracle_mod, fp64):
+    """The column's first y clip through the table of the x-slab polyhedra
+    (build_column) gives the generic clips' polyhedron bit for bit (live
+    slots, order, coordinates and link words) on every column task of a
+    periodic block (levels 0 and 2) and of periodic and open meshes in a
+    zoom window, in FP32 and FP64."""
+    import hostsim_api as H
+    from pypsi_amd.synth import zeldovich_block
+    O = oracle_mod
+    cases = []
+    for n, rms, gn, lvl in ((16, 2.0, 32, 0), (12, 1.0, 24, 2), (8, 3.0, 40, 0)):
+        pos, vel = zeldovich_block(n, rms, seed=n)
+        cases.append((O.Problem.block(pos, vel, periodic_box=BOX, n=(gn,) * 3, bounds=BOX, fields=["v"]), lvl))
+    win = ((0.13, 0.07, 0.21), (0.81, 0.93, 0.77))
+    pos, vel = zeldovich_block(10, 2.0, seed=4)
+    cases.append((O.Problem.block(pos, vel, periodic_box=BOX, n=(23, 17, 29), bounds=win, fields=["v"]), 1))
+    g = (np.stack(np.meshgrid(*(np.arange(10),) * 3, indexing="ij"), -1) + 0.5) / 10
+    cases.append((O.Problem.block(np.ascontiguousarray(g + vel), vel, n=(23, 17, 29), bounds=win, fields=["v"]), 1))
+    for P, lvl in cases:
+        ntask, ntable, bad = H.ytable_check(P, 0.2, lvl, fp64)
+        print(P.n, lvl, "tasks", ntask, "y-clipped", ntable, "mismatches", bad)
+        assert ntask > 0 and ntable > 0.5 * ntask
+        assert bad == 0
