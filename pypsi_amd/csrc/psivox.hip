@@ -2730,8 +2730,8 @@ struct ArenaScope {
 
 Workspace g_ws_noalloc;
 
-/* the x-slab step table in the current device's constant memory, once per
- * device (every thread's calls see it once the first upload returned) */
+/* the x-slab and y-slab step tables in the current device's memory, once
+ * per device (every thread's calls see them once the first upload returned) */
 int upload_tables(int dev) {
     static std::mutex mu;
     static bool done[256];
@@ -2740,7 +2740,10 @@ int upload_tables(int dev) {
     if (done[dev]) return 0;
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(slab_steps), host_slab_steps(), sizeof(SlabStep) * 81, 0,
                              hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(y_steps), host_y_steps(), sizeof(YStep) * YSTEPS, 0, hipMemcpyHostToDevice));
+    if (host_y2_base().base[81] != Y2STEPS) return set_err(PSIVOX_E_HIP, "y-slab table size mismatch");
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(y2_base), &host_y2_base(), sizeof(Y2Base), 0, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(y2_steps), host_y2_steps(), sizeof(Y2Step) * Y2STEPS, 0,
+                             hipMemcpyHostToDevice));
     done[dev] = true;
     return 0;
 }

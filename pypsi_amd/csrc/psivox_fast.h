@@ -667,6 +667,70 @@ PVX_HD void pair_terms(const LPoly<R>& P, int e, const SlabCell<R>& S, Mom<R>& M
     }
 }
 
+/*
+ * pair_terms for a slab with one plane p (the first or last cell of a
+ * column): S.O lies on p, so the kept piece K of T = (c, a, b) has no cap
+ * term, and fanned from its exit point X (on p) every boundary piece
+ * through X adds nothing either.  With v1 the first kept vertex after the
+ * entry point N (on p) and w1 the next vertex: K = (N, v1, X) or, when w1
+ * is kept too, (N, v1, w1, X): the cones over (X, v1, N) and (X, w1, v1),
+ * the second weighted 0 or 1.  Two fixed terms; the same integrals as
+ * pair_terms (the reference's fan moments, src/geometry.c:692-767).
+ */
+template <int ORDER, class R>
+PVX_HD void pair_terms_1(const LPoly<R>& P, int e, const SlabCell<R>& S, Mom<R>& M) {
+    const int ra = e / 3, k = e - 3 * ra, a = P.at(ra);
+    const R* O = S.O;
+    const R zd0 = S.zd[0], zd1 = S.zd[1], zd2 = S.zd[2];
+    const uint32_t wa = P.nbw(a);
+    const int b = nbf(wa, k), c = apf(wa, k);
+    /* the triangle in traversal order c -> a -> b, relative to O */
+    R q[3][3], z[3];
+    const int vs[3] = {c, a, b};
+    for (int s = 0; s < 3; ++s) {
+        const int v = s == 0 ? vs[0] : (s == 1 ? vs[1] : vs[2]);
+        const R p0 = P.qa(v, 0), p1 = P.qa(v, 1), p2 = P.qa(v, 2);
+        z[s] = zd0 * p0 + zd1 * p1 + zd2 * p2;
+        q[s][0] = p0 - O[0];
+        q[s][1] = p1 - O[1];
+        q[s][2] = p2 - O[2];
+    }
+    /* kept side: z <= hi (the slab's plane is its upper one) or z > lo */
+    const bool upper = S.has_hi;
+    const R p = upper ? S.hi : S.lo;
+    bool kp[3];
+    for (int s = 0; s < 3; ++s) kp[s] = upper ? (z[s] <= p) : (z[s] > p);
+    /* v1: the kept vertex whose predecessor is not kept; w1, w2 follow it */
+    const int i1 = (kp[0] && !kp[2]) ? 0 : ((kp[1] && !kp[0]) ? 1 : 2);
+    const int i2 = i1 == 2 ? 0 : i1 + 1, i3 = i2 == 2 ? 0 : i2 + 1;
+    const bool two = i2 == 0 ? kp[0] : (i2 == 1 ? kp[1] : kp[2]);
+    R V1[3], W1[3], W2[3];
+    R zv1, zw1, zw2;
+#define PVX_PICK3(dst, zdst, idx)                                                              \
+    do {                                                                                       \
+        for (int t = 0; t < 3; ++t) dst[t] = (idx) == 0 ? q[0][t] : ((idx) == 1 ? q[1][t] : q[2][t]); \
+        zdst = (idx) == 0 ? z[0] : ((idx) == 1 ? z[1] : z[2]);                                 \
+    } while (0)
+    PVX_PICK3(V1, zv1, i1);
+    PVX_PICK3(W1, zw1, i2);
+    PVX_PICK3(W2, zw2, i3);
+#undef PVX_PICK3
+    /* N on the entry edge w2 -> v1; X on the exit edge (two ? w1 -> w2 : v1 -> w1) */
+    R N[3], X[3];
+    const R tn = (p - zw2) * frcp(zv1 - zw2);
+    R U[3], D[3];
+    for (int t = 0; t < 3; ++t) {
+        N[t] = W2[t] + tn * (V1[t] - W2[t]);
+        U[t] = two ? W1[t] : V1[t];
+        D[t] = two ? W2[t] : W1[t];
+    }
+    const R zu = two ? zw1 : zv1, zdn = two ? zw2 : zw1;
+    const R tx = (p - zu) * frcp(zdn - zu);
+    for (int t = 0; t < 3; ++t) X[t] = U[t] + tx * (D[t] - U[t]);
+    tri_acc_w<ORDER>(M, X, V1, N, R(1));
+    tri_acc_w<ORDER>(M, X, W1, V1, two ? R(1) : R(0));
+}
+
 /* The moments (about S.O) of a fan triangle T = (c, a, b) wholly inside the
  * slab: the cone over T from O, the one term of pair_terms that does not
  * vanish there (its edge pieces are the whole edges, it has no segment on
@@ -700,10 +764,18 @@ PVX_HD void slab_moments_u(const LPoly<R>& P, const CopyGeo<R>& G, int ax, bool 
         inside_terms<ORDER>(P, e, S, M);
     }
     uint64_t mask = S.mask;
-    while (mask) {
-        const int e = __builtin_ctzll(mask);
-        mask &= mask - 1;
-        pair_terms<ORDER>(P, e, S, M);
+    if (S.has_lo && S.has_hi) {
+        while (mask) {
+            const int e = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            pair_terms<ORDER>(P, e, S, M);
+        }
+    } else {
+        while (mask) {
+            const int e = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            pair_terms_1<ORDER>(P, e, S, M);
+        }
     }
 }
 
@@ -1261,127 +1333,194 @@ PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g,
 }
 
 /*
- * One y plane clipped off an x-slab polyhedron, tabulated.  A clip's graph
- * work depends only on the graph and the set of vertices it drops; the
- * x-slab polyhedron of table pattern pat is entry pat's graph (its slots,
- * link words and order), so the result of dropping a set of its vertices
- * (a bit mask over their ranks: 256 sets) is made once by running
- * clip_graph on that graph: per output vertex (rank order) its slot, link
- * word (fan apexes included) and source (a vertex of the slab polyhedron,
- * or the cut of the edge from slot vc to slot vn).  The lane classifies the
- * slab polyhedron's vertices as clip_f does, looks the set up and evaluates
- * the cut points with clip_f's own formula, so the result is clip_f's, bit
- * for bit.  nv = 0 marks a set with no entry: clip_f then runs.
+ * The y-slab step of a column, tabulated like the x-slab: clipping the
+ * x-slab polyhedron of table pattern pat by the column's upper y plane and
+ * then its lower one gives a graph that depends only on where each of its
+ * vertices lies (state 0 dropped by the lower plane, 1 kept, 2 dropped by
+ * the upper plane; state = sum of state_r 3^r over the vertex ranks r).
+ * Each entry is the result of clip_graph on entry pat's graph for those
+ * states: per output vertex (rank order) its slot, its link word (fan
+ * apexes included) and its source (kind 0: the slab polyhedron's vertex in
+ * that slot; kind 1: cut of the upper plane on the edge from slot vc to
+ * slot vn; kind 2: cut of the lower plane on the edge from slot vc to slot
+ * vn).  The lane evaluates the cuts with clip_f's own formula in clip_f's
+ * sequence, so the result is the two generic clips', bit for bit.  Entries
+ * are one 128-byte line each; nv = Y2NONE marks a state set that no convex
+ * polyhedron produces (the generic clips then run).
  */
-constexpr int YNV = 12;
-constexpr int YSTEPS = 81 * 256;
-constexpr uint8_t YKEEP = 0xffu;
+constexpr int Y2NV = 14;
+constexpr uint8_t Y2NONE = 0xffu;
 
-struct YStep {
-    uint32_t link[YNV];
-    uint8_t slot[YNV];
-    uint8_t src[YNV]; /* YKEEP: a vertex of the slab polyhedron; else vc << 4 | vn */
+struct Y2Step {
+    uint32_t link[Y2NV];
+    uint16_t src[Y2NV]; /* kind << 12 | slot << 8 | vc << 4 | vn */
     uint8_t nv;
-    uint8_t pad[7];
+    uint8_t pad[128 - 4 * Y2NV - 2 * Y2NV - 1];
+};
+static_assert(sizeof(Y2Step) == 128, "one line per entry");
+
+/* the entry index of pattern pat's first state, and the total */
+struct Y2Base {
+    int base[82];
 };
 
-/* the entry of dropping the ranks in goner from pattern pat's polyhedron,
- * from the graph steps alone */
-inline bool make_y_step(int pat, uint32_t goner, YStep& Y) {
-    Y = YStep{};
+inline const Y2Base& host_y2_base() {
+    static Y2Base B;
+    static bool made = false;
+    if (!made) {
+        const SlabStep* T = host_slab_steps();
+        int o = 0;
+        for (int p = 0; p < 81; ++p) {
+            B.base[p] = o;
+            int n = 1;
+            for (int r = 0; r < T[p].nv; ++r) n *= 3;
+            o += T[p].nv ? n : 0;
+        }
+        B.base[81] = o;
+        made = true;
+    }
+    return B;
+}
+
+/* entries: sum over the 81 patterns of 3^nv */
+constexpr int Y2STEPS = 296703;
+
+/* the entry of pattern pat for the vertex states st, from the graph steps alone */
+inline void make_y2_step(int pat, int st, Y2Step& Y) {
+    Y = Y2Step{};
+    Y.nv = Y2NONE;
     const SlabStep& E = host_slab_steps()[pat];
     const int nv = E.nv;
-    if (nv == 0 || goner == 0u || (goner >> nv) != 0u || goner == (1u << nv) - 1u) return false;
     static uint32_t mem[VRec<float>::WORDS];
-    LPoly<float> P{mem, 0};
-    P.live = 0u;
-    P.ord = 0ull;
+    LPoly<float> P{mem, 0u, 0, 0ull};
+    int sr[8];
+    for (int r = 0, x = st; r < nv; ++r, x /= 3) sr[r] = x % 3;
+    uint16_t src[NVF];
     uint32_t gone = 0u;
     for (int r = 0; r < nv; ++r) {
         const int sl = E.slot[r];
         P.setnbw(sl, E.link[r]);
         P.live |= 1u << sl;
         P.ord |= (uint64_t)sl << (4 * r);
-        if ((goner >> r) & 1u) gone |= 1u << sl;
+        src[sl] = (uint16_t)(sl << 8);
+        if (sr[r] == 2) gone |= 1u << sl;
     }
     P.nv = nv;
-    uint8_t src[NVF];
-    for (int v = 0; v < NVF; ++v) src[v] = YKEEP;
-    if (!clip_graph(P, gone, [&](int vx, int vc, int vn) { src[vx] = (uint8_t)((vc << 4) | vn); })) return false;
-    if (P.nv > YNV) return false;
+    if (gone == P.live) {
+        Y.nv = 0;
+        return;
+    }
+    if (gone && !clip_graph(P, gone, [&](int vx, int vc, int vn) {
+            src[vx] = (uint16_t)((1 << 12) | (vx << 8) | (vc << 4) | vn);
+        }))
+        return;
+    gone = 0u;
+    for (int r = 0; r < nv; ++r)
+        if (sr[r] == 0 && ((P.live >> E.slot[r]) & 1u)) gone |= 1u << E.slot[r];
+    if (gone == P.live) {
+        Y.nv = 0;
+        return;
+    }
+    if (gone && !clip_graph(P, gone, [&](int vx, int vc, int vn) {
+            src[vx] = (uint16_t)((2 << 12) | (vx << 8) | (vc << 4) | vn);
+        }))
+        return;
+    if (P.nv > Y2NV) return;
     Y.nv = (uint8_t)P.nv;
     for (int r = 0; r < P.nv; ++r) {
         const int sl = P.at(r);
-        Y.slot[r] = (uint8_t)sl;
         Y.link[r] = P.nbw(sl);
         Y.src[r] = src[sl];
     }
-    return true;
 }
 
-/* the 81 x 256 entries, built once on the host */
-inline const YStep* host_y_steps() {
-    static YStep* T = nullptr;
+/* the Y2STEPS entries, built once on the host */
+inline const Y2Step* host_y2_steps() {
+    static Y2Step* T = nullptr;
     if (!T) {
-        YStep* t = new YStep[YSTEPS];
+        const Y2Base& B = host_y2_base();
+        Y2Step* t = new Y2Step[Y2STEPS];
         for (int p = 0; p < 81; ++p)
-            for (int m = 0; m < 256; ++m) make_y_step(p, (uint32_t)m, t[p * 256 + m]);
+            for (int q = 0; q < B.base[p + 1] - B.base[p]; ++q) make_y2_step(p, q, t[B.base[p] + q]);
         T = t;
     }
     return T;
 }
 
 #ifdef __HIPCC__
-__device__ YStep y_steps[YSTEPS];
+__device__ Y2Step y2_steps[Y2STEPS];
+__constant__ Y2Base y2_base;
 #endif
 
-PVX_HD const YStep& y_step(int i) {
+PVX_HD const Y2Step& y2_step(int i) {
 #ifdef __HIP_DEVICE_COMPILE__
-    return y_steps[i];
+    return y2_steps[i];
 #else
-    return host_y_steps()[i];
+    return host_y2_steps()[i];
+#endif
+}
+
+PVX_HD int y2_first(int pat) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return y2_base.base[pat];
+#else
+    return host_y2_base().base[pat];
 #endif
 }
 
 /* Clip P, the x-slab polyhedron of pattern pat as slab_from_table left it,
- * by the plane y = c (keep_upper as clip_f) through the y table.  False
- * when the table has no entry for the dropped set (P unchanged: run
- * clip_f). */
+ * by the upper y plane chi (when has_hi) and then the lower one clo (when
+ * has_lo) through the table.  False when the table does not apply (no
+ * entry, or a cut of the upper plane on the dropped side of the lower
+ * one); P's graph is then unchanged and the generic clips run. */
 template <class R>
-PVX_HD bool clip_y_table(LPoly<R>& P, int pat, const CopyGeo<R>& G, const GridInfo& g, double c, bool keep_upper) {
+PVX_HD bool yslab_from_table(LPoly<R>& P, int pat, const CopyGeo<R>& G, const GridInfo& g, bool has_hi, double chi,
+                             bool has_lo, double clo) {
     if (P.live == 0u) return true;
     const AxisMap<R> m = axis_map(G, g, 1);
-    const R off = (R)(c - m.od);
+    const R offh = (R)(chi - m.od), offl = (R)(clo - m.od);
     const int nv = P.nv;
-    uint32_t goner = 0u;
-    for (int r = 0; r < 8; ++r) {
+    int st = 0;
+    for (int r = 7; r >= 0; --r) {
         if (r >= nv) continue;
         const int v = P.at(r);
-        const R t = sdist(G.rec, m, c, off, P.qa(v, 0), P.qa(v, 1), P.qa(v, 2));
-        const bool drop = keep_upper ? (t >= R(0)) : (t < R(0));
-        goner |= (drop ? 1u : 0u) << r;
+        const R q0 = P.qa(v, 0), q1 = P.qa(v, 1), q2 = P.qa(v, 2);
+        const R th = has_hi ? sdist(G.rec, m, chi, offh, q0, q1, q2) : R(1);
+        const R tl = has_lo ? sdist(G.rec, m, clo, offl, q0, q1, q2) : R(-1);
+        st = 3 * st + (th < R(0) ? 2 : (tl >= R(0) ? 0 : 1));
     }
-    if (goner == 0u) return true;
-    if (goner == (1u << nv) - 1u) {
+    const Y2Step& Y = y2_step(y2_first(pat) + st);
+    const int ny = Y.nv;
+    if (ny == (int)Y2NONE) return false;
+    if (ny == 0) {
         P.live = 0u;
         P.nv = 0;
         return true;
     }
-    const YStep& Y = y_step(pat * 256 + (int)goner);
-    const int ny = Y.nv;
-    if (ny == 0) return false;
-    for (int r = 0; r < YNV; ++r) {
-        if (r >= ny || Y.src[r] == YKEEP) continue;
-        const int vx = Y.slot[r], vc = Y.src[r] >> 4, vn = Y.src[r] & 15;
-        const R sc = sdist(G.rec, m, c, off, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
-        const R sn = sdist(G.rec, m, c, off, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
+    /* upper cuts, as the generic clip makes them (into free slots) */
+    bool ok = true;
+    for (int r = 0; r < Y2NV; ++r) {
+        if (r >= ny || (Y.src[r] >> 12) != 1) continue;
+        const int vx = (Y.src[r] >> 8) & 15, vc = (Y.src[r] >> 4) & 15, vn = Y.src[r] & 15;
+        const R sc = sdist(G.rec, m, chi, offh, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
+        const R sn = sdist(G.rec, m, chi, offh, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
+        cut_point(P, vx, vc, vn, sc, sn);
+        if (has_lo && sdist(G.rec, m, clo, offl, P.qa(vx, 0), P.qa(vx, 1), P.qa(vx, 2)) >= R(0)) ok = false;
+    }
+    if (!ok) return false;
+    /* lower cuts, from a survivor to a vertex the lower plane drops */
+    for (int r = 0; r < Y2NV; ++r) {
+        if (r >= ny || (Y.src[r] >> 12) != 2) continue;
+        const int vx = (Y.src[r] >> 8) & 15, vc = (Y.src[r] >> 4) & 15, vn = Y.src[r] & 15;
+        const R sc = sdist(G.rec, m, clo, offl, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
+        const R sn = sdist(G.rec, m, clo, offl, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
         cut_point(P, vx, vc, vn, sc, sn);
     }
     uint32_t live = 0u;
     uint64_t ord = 0ull;
-    for (int r = 0; r < YNV; ++r) {
+    for (int r = 0; r < Y2NV; ++r) {
         if (r >= ny) continue;
-        const int sl = Y.slot[r];
+        const int sl = (Y.src[r] >> 8) & 15;
         P.setnbw(sl, Y.link[r]);
         live |= 1u << sl;
         ord |= (uint64_t)sl << (4 * r);
@@ -1412,18 +1551,14 @@ PVX_HD bool build_column(LPoly<R>& P, const CopySetup<R>& C, const GridInfo& g, 
         if (has_hi) ok = clip_f(P, G, g, chi, 0, false);
         if (ok && has_lo) ok = clip_f(P, G, g, clo, 0, true);
     }
-    /* the y planes: the first one that cuts the slab polyhedron through the
-     * y table, the rest by the generic clip */
+    /* the y planes: both through the y-slab table, or (when it does not
+     * apply) by the two generic clips */
     const bool yh = yhi || (j == C.ib1[1] - 1 && (C.wmask & 8u));
     const bool yl = ylo || (j == G.ib0[1] && (C.wmask & 4u));
     const double cyh = yhi ? (double)(jl + 1) : C.whi[1], cyl = ylo ? (double)jl : C.wlo[1];
-    if (ok && yh) {
-        const uint32_t before = P.live;
-        if (pat < 0 || !clip_y_table(P, pat, G, g, cyh, false)) ok = clip_f(P, G, g, cyh, 1, false);
-        if (P.live != before) pat = -1;
-    }
-    if (ok && yl) {
-        if (pat < 0 || !clip_y_table(P, pat, G, g, cyl, true)) ok = clip_f(P, G, g, cyl, 1, true);
+    if (ok && (yh || yl) && (pat < 0 || !yslab_from_table(P, pat, G, g, yh, cyh, yl, cyl))) {
+        if (yh) ok = clip_f(P, G, g, cyh, 1, false);
+        if (ok && yl) ok = clip_f(P, G, g, cyl, 1, true);
     }
     return ok;
 }

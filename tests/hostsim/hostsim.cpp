@@ -321,11 +321,11 @@ extern "C" int64_t hs_expand_records(int vpe, int64_t nelem, const double* pos, 
     return tot;
 }
 
-/* every column task of the mesh built twice: by build_column (the y table
+/* every column task of the mesh built twice: by build_column (the y-slab table
  * where it applies) and by the generic clips alone (slab_from_table, then
  * clip_f for each plane); the two polyhedra compared bit for bit (live
  * slots, order, each live slot's q words and link word).  out: tasks,
- * tasks whose first y clip went through the table, mismatches */
+ * tasks whose y planes cut the slab polyhedron, mismatches */
 template <class R>
 static void ycheck_tet(const double* r, int64_t t, const GridInfo& g, const MeshBox& mb, LPoly<R>& A, LPoly<R>& B,
                        int64_t* out) {

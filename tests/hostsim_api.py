@@ -170,8 +170,9 @@ def expand_records(P, reftol=1.0, maxlvl=0, cap=1 << 20):
 
 
 def ytable_check(P, reftol=1.0, maxlvl=0, fp64=False):
-    """Every column task built with the y table and with the generic clips
-    alone: (tasks, tasks whose first y clip took the table, mismatches)."""
+    """Every column task built with the y-slab table and with the generic
+    clips alone: (tasks, tasks whose y planes cut the slab polyhedron,
+    mismatches)."""
     out = np.zeros(3, np.int64)
     n = (C.c_int * 3)(*P.n)
     win = (C.c_double * 6)(*P.bounds.reshape(-1))

@@ -126,10 +126,10 @@ def test_wide_tets_routed_to_fp64_columns(oracle_mod):
 
 @pytest.mark.parametrize("fp64", [False, True])
 def test_y_table_matches_generic_clips(oracle_mod, fp64):
-    """The column's first y clip through the table of the x-slab polyhedra
-    (build_column) gives the generic clips' polyhedron bit for bit (live
-    slots, order, coordinates and link words) on every column task of a
-    periodic block (levels 0 and 2) and of periodic and open meshes in a
+    """The column's two y clips through the y-slab table of the x-slab
+    polyhedra (build_column) give the generic clips' polyhedron bit for bit
+    (live slots, order, coordinates and link words) on every column task of
+    a periodic block (levels 0 and 2) and of periodic and open meshes in a
     zoom window, in FP32 and FP64."""
     import hostsim_api as H
     from pypsi_amd.synth import zeldovich_block
