@@ -2740,8 +2740,7 @@ int upload_tables(int dev) {
     if (done[dev]) return 0;
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(slab_steps), host_slab_steps(), sizeof(SlabStep) * 81, 0,
                              hipMemcpyHostToDevice));
-    if (host_y2_base().base[81] != Y2STEPS) return set_err(PSIVOX_E_HIP, "y-slab table size mismatch");
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(y2_base), &host_y2_base(), sizeof(Y2Base), 0, hipMemcpyHostToDevice));
+    if (host_y2_count() != Y2STEPS) return set_err(PSIVOX_E_HIP, "y-slab table size mismatch");
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(y2_steps), host_y2_steps(), sizeof(Y2Step) * Y2STEPS, 0,
                              hipMemcpyHostToDevice));
     done[dev] = true;

@@ -1187,13 +1187,34 @@ PVX_HD int column_cells(const CopySetup<float>& C, int il, int jl, int* nvert = 
  * formula in the generic clip's sequence, so the result is the same
  * polyhedron.
  */
-struct SlabStep {
+struct alignas(16) SlabStep {
     uint32_t link[8];
     uint8_t slot[8];
     uint8_t src[8]; /* kind in bits 6-7, vc in bits 2-5, vn in bits 0-1 */
     uint8_t nv;
-    uint8_t pad[15];
+    uint8_t pad[3];
+    int32_t y2base; /* the y-slab table's first entry for this pattern */
+    uint8_t pad2[8];
 };
+static_assert(sizeof(SlabStep) == 64, "four 16-byte words per entry");
+
+/* A table entry copied into registers with 16-byte loads: the lane then
+ * decodes it without further memory round trips (a field read straight
+ * from the table is one dependent load each). */
+template <class T>
+PVX_HD T load_entry(const T& src) {
+    static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "whole aligned 16-byte words");
+#ifdef __HIP_DEVICE_COMPILE__
+    T v;
+    const uint4* p = reinterpret_cast<const uint4*>(&src);
+    uint4* d = reinterpret_cast<uint4*>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 16); ++i) d[i] = p[i];
+    return v;
+#else
+    return src;
+#endif
+}
 
 template <class R>
 PVX_HD void tet_start(LPoly<R>& P) {
@@ -1246,12 +1267,20 @@ inline bool make_slab_step(int pattern, SlabStep& E) {
     return true;
 }
 
-/* the 81 entries, built once on the host */
+/* the 81 entries, built once on the host, each with the index of its first
+ * y-slab table entry (the patterns' 3^nv state sets one after another) */
 inline const SlabStep* host_slab_steps() {
     static SlabStep T[81];
     static bool made = false;
     if (!made) {
-        for (int p = 0; p < 81; ++p) make_slab_step(p, T[p]);
+        int o = 0;
+        for (int p = 0; p < 81; ++p) {
+            make_slab_step(p, T[p]);
+            T[p].y2base = o;
+            int n = 1;
+            for (int r = 0; r < T[p].nv; ++r) n *= 3;
+            o += T[p].nv ? n : 0;
+        }
         made = true;
     }
     return T;
@@ -1275,7 +1304,7 @@ PVX_HD const SlabStep& slab_step(int p) {
  * a window slab thinner than the precision resolves); P then holds the tet. */
 template <class R>
 PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g, bool has_hi, double chi, bool has_lo,
-                            double clo, int* patp = nullptr) {
+                            double clo, int* y2p = nullptr) {
     tet_start(P);
     const AxisMap<R> m = axis_map(G, g, 0);
     const R offh = (R)(chi - m.od), offl = (R)(clo - m.od);
@@ -1287,8 +1316,8 @@ PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g,
         tl[v] = has_lo ? sdist(G.rec, m, clo, offl, q0, q1, q2) : R(-1);
         pat = 3 * pat + (th[v] < R(0) ? 2 : (tl[v] >= R(0) ? 0 : 1));
     }
-    if (patp) *patp = pat;
-    const SlabStep& E = slab_step(pat);
+    const SlabStep E = load_entry(slab_step(pat));
+    if (y2p) *y2p = E.nv ? E.y2base : -1;
     if (E.nv == 0) {
         P.live = 0u;
         P.nv = 0;
@@ -1351,36 +1380,13 @@ PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g,
 constexpr int Y2NV = 14;
 constexpr uint8_t Y2NONE = 0xffu;
 
-struct Y2Step {
+struct alignas(16) Y2Step {
     uint32_t link[Y2NV];
     uint16_t src[Y2NV]; /* kind << 12 | slot << 8 | vc << 4 | vn */
     uint8_t nv;
     uint8_t pad[128 - 4 * Y2NV - 2 * Y2NV - 1];
 };
 static_assert(sizeof(Y2Step) == 128, "one line per entry");
-
-/* the entry index of pattern pat's first state, and the total */
-struct Y2Base {
-    int base[82];
-};
-
-inline const Y2Base& host_y2_base() {
-    static Y2Base B;
-    static bool made = false;
-    if (!made) {
-        const SlabStep* T = host_slab_steps();
-        int o = 0;
-        for (int p = 0; p < 81; ++p) {
-            B.base[p] = o;
-            int n = 1;
-            for (int r = 0; r < T[p].nv; ++r) n *= 3;
-            o += T[p].nv ? n : 0;
-        }
-        B.base[81] = o;
-        made = true;
-    }
-    return B;
-}
 
 /* entries: sum over the 81 patterns of 3^nv */
 constexpr int Y2STEPS = 296703;
@@ -1434,14 +1440,25 @@ inline void make_y2_step(int pat, int st, Y2Step& Y) {
     }
 }
 
+/* the number of y-slab entries the x-slab table asks for */
+inline int host_y2_count() {
+    const SlabStep& E = host_slab_steps()[80];
+    int n = 1;
+    for (int r = 0; r < E.nv; ++r) n *= 3;
+    return E.y2base + (E.nv ? n : 0);
+}
+
 /* the Y2STEPS entries, built once on the host */
 inline const Y2Step* host_y2_steps() {
     static Y2Step* T = nullptr;
     if (!T) {
-        const Y2Base& B = host_y2_base();
+        const SlabStep* X = host_slab_steps();
         Y2Step* t = new Y2Step[Y2STEPS];
-        for (int p = 0; p < 81; ++p)
-            for (int q = 0; q < B.base[p + 1] - B.base[p]; ++q) make_y2_step(p, q, t[B.base[p] + q]);
+        for (int p = 0; p < 81; ++p) {
+            int n = 1;
+            for (int r = 0; r < X[p].nv; ++r) n *= 3;
+            for (int q = 0; X[p].nv && q < n && X[p].y2base + q < Y2STEPS; ++q) make_y2_step(p, q, t[X[p].y2base + q]);
+        }
         T = t;
     }
     return T;
@@ -1449,7 +1466,6 @@ inline const Y2Step* host_y2_steps() {
 
 #ifdef __HIPCC__
 __device__ Y2Step y2_steps[Y2STEPS];
-__constant__ Y2Base y2_base;
 #endif
 
 PVX_HD const Y2Step& y2_step(int i) {
@@ -1460,22 +1476,14 @@ PVX_HD const Y2Step& y2_step(int i) {
 #endif
 }
 
-PVX_HD int y2_first(int pat) {
-#ifdef __HIP_DEVICE_COMPILE__
-    return y2_base.base[pat];
-#else
-    return host_y2_base().base[pat];
-#endif
-}
-
-/* Clip P, the x-slab polyhedron of pattern pat as slab_from_table left it,
- * by the upper y plane chi (when has_hi) and then the lower one clo (when
- * has_lo) through the table.  False when the table does not apply (no
+/* Clip P, the x-slab polyhedron as slab_from_table left it (y2first: its
+ * pattern's first y-slab entry), by the upper y plane chi (when has_hi) and
+ * then the lower one clo (when has_lo) through the table.  False when the table does not apply (no
  * entry, or a cut of the upper plane on the dropped side of the lower
  * one); P's graph is then unchanged and the generic clips run. */
 template <class R>
-PVX_HD bool yslab_from_table(LPoly<R>& P, int pat, const CopyGeo<R>& G, const GridInfo& g, bool has_hi, double chi,
-                             bool has_lo, double clo) {
+PVX_HD bool yslab_from_table(LPoly<R>& P, int y2first, const CopyGeo<R>& G, const GridInfo& g, bool has_hi,
+                             double chi, bool has_lo, double clo) {
     if (P.live == 0u) return true;
     const AxisMap<R> m = axis_map(G, g, 1);
     const R offh = (R)(chi - m.od), offl = (R)(clo - m.od);
@@ -1489,7 +1497,7 @@ PVX_HD bool yslab_from_table(LPoly<R>& P, int pat, const CopyGeo<R>& G, const Gr
         const R tl = has_lo ? sdist(G.rec, m, clo, offl, q0, q1, q2) : R(-1);
         st = 3 * st + (th < R(0) ? 2 : (tl >= R(0) ? 0 : 1));
     }
-    const Y2Step& Y = y2_step(y2_first(pat) + st);
+    const Y2Step Y = load_entry(y2_step(y2first + st));
     const int ny = Y.nv;
     if (ny == (int)Y2NONE) return false;
     if (ny == 0) {
@@ -1545,9 +1553,9 @@ PVX_HD bool build_column(LPoly<R>& P, const CopySetup<R>& C, const GridInfo& g, 
     const bool has_hi = xhi || (C.wmask & 2u), has_lo = xlo || (C.wmask & 1u);
     const double chi = xhi ? (double)(il + 1) : C.whi[0], clo = xlo ? (double)il : C.wlo[0];
     bool ok = true;
-    int pat = -1; /* the table pattern P still is, or -1 */
-    if (!slab_from_table(P, G, g, has_hi, chi, has_lo, clo, &pat)) {
-        pat = -1;
+    int y2first = -1; /* the y-slab table's entries for P's x-slab pattern, or -1 */
+    if (!slab_from_table(P, G, g, has_hi, chi, has_lo, clo, &y2first)) {
+        y2first = -1;
         if (has_hi) ok = clip_f(P, G, g, chi, 0, false);
         if (ok && has_lo) ok = clip_f(P, G, g, clo, 0, true);
     }
@@ -1556,7 +1564,7 @@ PVX_HD bool build_column(LPoly<R>& P, const CopySetup<R>& C, const GridInfo& g, 
     const bool yh = yhi || (j == C.ib1[1] - 1 && (C.wmask & 8u));
     const bool yl = ylo || (j == G.ib0[1] && (C.wmask & 4u));
     const double cyh = yhi ? (double)(jl + 1) : C.whi[1], cyl = ylo ? (double)jl : C.wlo[1];
-    if (ok && (yh || yl) && (pat < 0 || !yslab_from_table(P, pat, G, g, yh, cyh, yl, cyl))) {
+    if (ok && (yh || yl) && (y2first < 0 || !yslab_from_table(P, y2first, G, g, yh, cyh, yl, cyl))) {
         if (yh) ok = clip_f(P, G, g, cyh, 1, false);
         if (ok && yl) ok = clip_f(P, G, g, cyl, 1, true);
     }
