@@ -606,12 +606,15 @@ constexpr int ORDK = 256;
 static_assert(ORDW % 256 == 0, "whole threads per window");
 
 /* perm[w0 .. w0 + ORDW) of each window: the window's task indices, largest
- * class first (order within a class is arbitrary).  Also clears the column kernel's wavefront counter.  256
+ * class first (order within a class is arbitrary), and sorted[] the task
+ * words in the same order (the column kernel then reads both coalesced and
+ * independently).  Also clears the column kernel's wavefront counter.  256
  * threads: one class each in the scan. */
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
                                                    int64_t ntask, GridInfo g0, MeshBox mb0, const BatchGeo* table,
-                                                   uint32_t* __restrict__ perm, unsigned* __restrict__ next) {
+                                                   uint32_t* __restrict__ perm, uint64_t* __restrict__ sorted,
+                                                   unsigned* __restrict__ next) {
     __shared__ int cnt[ORDK];
     const int64_t w0 = (int64_t)blockIdx.x * ORDW;
     const int nw = (int)(ntask - w0 < ORDW ? ntask - w0 : ORDW);
@@ -664,6 +667,7 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
         if (key[q] < 0) continue;
         const int pos = atomicAdd(&cnt[key[q]], 1);
         perm[w0 + pos] = (uint32_t)(w0 + tid + 256 * q);
+        sorted[w0 + pos] = tasks[w0 + tid + 256 * q];
     }
 }
 
@@ -825,7 +829,7 @@ __device__ __forceinline__ int64_t wave_columns(bool act, int64_t p, uint64_t w,
  */
 template <int ORDER, int FMASK, class R, bool BATCH>
 __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
-                                                        const uint64_t* __restrict__ tasks,
+                                                        const uint64_t* __restrict__ sorted,
                                                         const uint32_t* __restrict__ perm, unsigned* __restrict__ next,
                                                         int64_t ntask, GridInfo g0, MeshBox mb0,
                                                         double* __restrict__ acc0,
@@ -840,12 +844,17 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
     const int lane = threadIdx.x;
     LPoly<R> P{(uint32_t*)__builtin_assume_aligned(&sv[VRec<R>::W * lane], 16), 0, 0, 0};
     int64_t nfrag = 0;
+    /* each wavefront of tasks is claimed one round ahead, so the counter's
+     * round trip overlaps the work on the one before */
+    unsigned claim = 0u;
+    if (lane == 0) claim = atomicAdd(next, 1u);
     for (;;) {
-        if (lane == 0) swave = atomicAdd(next, 1u);
+        if (lane == 0) swave = claim;
         __syncthreads();
         const int64_t c0 = (int64_t)swave * FBLOCK;
         __syncthreads(); /* everyone has read swave before lane 0 writes it again */
         if (c0 >= ntask) break;
+        if (lane == 0) claim = atomicAdd(next, 1u);
         const int64_t c = c0 + lane;
         int64_t p = 0;
         uint64_t w = 0;
@@ -853,7 +862,7 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
         int my = -1;
         if (c < ntask) {
             p = perm[c];
-            w = tasks[p];
+            w = sorted[c];
             r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
             my = BATCH ? (int)r[27] : 0;
         }
@@ -1070,25 +1079,37 @@ static KColsFn cols_kernel(const GridInfo& g, bool fp64, bool batch) {
     return batch ? cols_kernel_t<float, true>(g) : cols_kernel_t<float, false>(g);
 }
 
+/* The ordering scratch of ntask tasks: perm (4 bytes a task) and the
+ * column kernel's counter, then the sorted task words (8 bytes a task). */
+static size_t order_bytes(int64_t ntask) { return (((size_t)ntask * 4 + 256 + 255) & ~(size_t)255) + (size_t)ntask * 8; }
+static uint32_t* order_perm(void* b) { return (uint32_t*)b; }
+static unsigned* perm_counter(void* b, int64_t nsub) { return (unsigned*)b + nsub; }
+static uint64_t* order_sorted(void* b, int64_t nsub) {
+    return (uint64_t*)((char*)b + (((size_t)nsub * 4 + 256 + 255) & ~(size_t)255));
+}
+
 /* the column kernel over tasks [0, ntask) of the pass geometry G (one mesh,
- * or a batch's table), FP64 for wide tets (perm: ntask entries of scratch
- * for the task order; next: one counter); mark, when given, is recorded
- * between the ordering and the column kernel */
+ * or a batch's table), FP64 for wide tets (ord: order_bytes(ntask) of
+ * scratch for the task order and the counter); mark, when given, is
+ * recorded between the ordering and the column kernel */
 static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask, const PassGeo& G, bool fp64,
-                       TaskOut out, uint32_t* perm, unsigned* next, unsigned* status, unsigned long long* counters,
+                       TaskOut out, void* ord, unsigned* status, unsigned long long* counters,
                        hipStream_t s, hipEvent_t mark = nullptr) {
+    uint32_t* perm = order_perm(ord);
+    uint64_t* sorted = order_sorted(ord, ntask);
+    unsigned* next = perm_counter(ord, ntask);
     const unsigned nord = (unsigned)((ntask + ORDW - 1) / ORDW);
     if (G.table)
-        k_col_order<true><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, G.table, perm, next);
+        k_col_order<true><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, G.table, perm, sorted, next);
     else
-        k_col_order<false><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, nullptr, perm, next);
+        k_col_order<false><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, nullptr, perm, sorted, next);
     if (mark) (void)hipEventRecord(mark, s);
     const int64_t nwave = (ntask + FBLOCK - 1) / FBLOCK;
     const size_t sb = stage_bytes(G.g, SFRAG);
     const KColsFn kern = cols_kernel(G.g, fp64, G.table != nullptr);
     const int64_t hold = resident_blocks(kern, sb);
     const unsigned nb = (unsigned)(nwave < hold ? nwave : hold);
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(FBLOCK), sb, s, tets, tasks, perm, next, ntask, G.g, G.mb, G.acc, G.table,
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(FBLOCK), sb, s, tets, sorted, perm, next, ntask, G.g, G.mb, G.acc, G.table,
                        out, status, counters, G.mcnt);
     return (int)hipGetLastError();
 }
@@ -1621,8 +1642,6 @@ __global__ void k_chunk_cut(const int64_t* __restrict__ colstart, int64_t nt, in
 
 /* the column kernel's wavefront counter: the word after a chunk's nsub
  * task-order entries */
-static unsigned* perm_counter(const DevBuf& b, int64_t nsub) { return (unsigned*)b.p + nsub; }
-
 /* k_col_count for a pass, one mesh or a batch */
 template <bool BATCH>
 static void launch_col_count(int pass, const double* tets, int64_t nt, const PassGeo& G, int64_t* cnt, unsigned* status,
@@ -2130,10 +2149,9 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                             if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
                             if (g.order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev)))
                                 return rc2;
-                            if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
+                            if ((rc2 = W.perm.get(order_bytes(nsub), dev))) return rc2;
                             TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
-                            rc2 = launch_cols(ptets, tasks, nsub, G, pass_fp64(pass), out, (uint32_t*)W.perm.p,
-                                              perm_counter(W.perm, nsub), status, counters, s,
+                            rc2 = launch_cols(ptets, tasks, nsub, G, pass_fp64(pass), out, W.perm.p, status, counters, s,
                                               W.timing ? W.ev0 : nullptr);
                             if (rc2)
                                 return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
@@ -2621,10 +2639,10 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                     if ((rc2 = W.colkey.get((size_t)nsub * sizeof(uint64_t), dev))) return rc2;
                     if (G[i0].order > 0 && (rc2 = W.colmom.get((size_t)nsub * 16 * sizeof(float), dev)))
                         return rc2;
-                    if ((rc2 = W.perm.get((size_t)nsub * sizeof(uint32_t) + 256, dev))) return rc2;
+                    if ((rc2 = W.perm.get(order_bytes(nsub), dev))) return rc2;
                     TaskOut out{(double*)W.colsum.p, (uint64_t*)W.colkey.p, (float*)W.colmom.p};
-                    rc2 = launch_cols(ptets, tasks, nsub, PG, pass == PASS_WIDE, out, (uint32_t*)W.perm.p,
-                                      perm_counter(W.perm, nsub), status, counters, s, W.ev0);
+                    rc2 = launch_cols(ptets, tasks, nsub, PG, pass == PASS_WIDE, out, W.perm.p, status, counters, s,
+                                      W.ev0);
                     if (rc2)
                         return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
                                        hipGetErrorString((hipError_t)rc2));
@@ -2802,7 +2820,7 @@ int run_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* o
         return rc;
     /* a chunk of column tasks: task words, mass sums, largest-fragment
      * keys and moments, processing order */
-    const size_t per_task = 3 * sizeof(uint64_t) + sizeof(uint32_t) + (g.order > 0 ? 16 * sizeof(float) : 0);
+    const size_t per_task = 4 * sizeof(uint64_t) + sizeof(uint32_t) + (g.order > 0 ? 16 * sizeof(float) : 0);
     const size_t left = A.size - A.off;
     const int64_t cap = left > 8192 ? (int64_t)((left - 8192) / per_task) : 0;
     if (cap < 4096)
@@ -2813,7 +2831,7 @@ int run_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* o
         (rc = W.colsum.get((size_t)W.task_cap * sizeof(double), dev)) ||
         (rc = W.colkey.get((size_t)W.task_cap * sizeof(uint64_t), dev)) ||
         (g.order > 0 && (rc = W.colmom.get((size_t)W.task_cap * 16 * sizeof(float), dev))) ||
-        (rc = W.perm.get((size_t)W.task_cap * sizeof(uint32_t) + 256, dev)))
+        (rc = W.perm.get(order_bytes(W.task_cap), dev)))
         return rc;
     psivox_opts ob = *o;
     ob.batch_elements = ne_batch;

@@ -1205,11 +1205,18 @@ template <class T>
 PVX_HD T load_entry(const T& src) {
     static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "whole aligned 16-byte words");
 #ifdef __HIP_DEVICE_COMPILE__
-    T v;
+    /* every word is loaded here, all loads in flight together: the empty
+     * asm statements need the words in registers, so the compiler can
+     * neither split a load by field nor defer one past a branch */
+    constexpr int NW = (int)(sizeof(T) / 16);
     const uint4* p = reinterpret_cast<const uint4*>(&src);
-    uint4* d = reinterpret_cast<uint4*>(&v);
+    uint4 w[NW];
 #pragma unroll
-    for (int i = 0; i < (int)(sizeof(T) / 16); ++i) d[i] = p[i];
+    for (int i = 0; i < NW; ++i) w[i] = p[i];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) __asm__ volatile("" : "+v"(w[i].x), "+v"(w[i].y), "+v"(w[i].z), "+v"(w[i].w));
+    T v;
+    memcpy(&v, w, sizeof(T));
     return v;
 #else
     return src;
