@@ -616,18 +616,14 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
                                                    uint32_t* __restrict__ perm, uint64_t* __restrict__ sorted,
                                                    unsigned* __restrict__ next) {
     __shared__ int cnt[ORDK];
+    __shared__ uint8_t skey[ORDW]; /* each task's class, kept here between the two passes */
     const int64_t w0 = (int64_t)blockIdx.x * ORDW;
     const int nw = (int)(ntask - w0 < ORDW ? ntask - w0 : ORDW);
     const int tid = (int)threadIdx.x;
     if (blockIdx.x == 0 && tid == 0) *next = 0u;
     cnt[tid] = 0;
     __syncthreads();
-    int key[ORDW / 256];
-#pragma unroll
-    for (int q = 0; q < ORDW / 256; ++q) {
-        const int e = tid + 256 * q;
-        key[q] = -1;
-        if (e >= nw) continue;
+    for (int e = tid; e < nw; e += 256) {
         const uint64_t w = tasks[w0 + e];
         int64_t t;
         int h, il, jl;
@@ -644,8 +640,9 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
         });
         nc = nc < 15 ? nc : 15;
         nv = nv < 15 ? nv : 15;
-        key[q] = ORDK - 1 - (16 * nc + nv);
-        atomicAdd(&cnt[key[q]], 1);
+        const int key = ORDK - 1 - (16 * nc + nv);
+        skey[e] = (uint8_t)key;
+        atomicAdd(&cnt[key], 1);
     }
     __syncthreads();
     /* exclusive scan of the 256 class counts (Hillis-Steele) */
@@ -662,12 +659,10 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
     __syncthreads();
     cnt[tid] = ex;
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < ORDW / 256; ++q) {
-        if (key[q] < 0) continue;
-        const int pos = atomicAdd(&cnt[key[q]], 1);
-        perm[w0 + pos] = (uint32_t)(w0 + tid + 256 * q);
-        sorted[w0 + pos] = tasks[w0 + tid + 256 * q];
+    for (int e = tid; e < nw; e += 256) {
+        const int pos = atomicAdd(&cnt[skey[e]], 1);
+        perm[w0 + pos] = (uint32_t)(w0 + e);
+        sorted[w0 + pos] = tasks[w0 + e];
     }
 }
 
@@ -690,20 +685,6 @@ constexpr int SFRAG = FBLOCK / 2;
 /* field flags of grids without position moments (x, xx, xv): the column
  * kernel built for them keeps no position data through its cell loop */
 constexpr int FMASK_NOX = ~(F_X | F_XX | F_XV);
-
-/* *p read as wave-uniform values (each 32-bit word through readfirstlane),
- * so the compiler keeps it in scalar registers; p must be the same for
- * every lane */
-template <class T>
-__device__ __forceinline__ T load_uniform(const T* p) {
-    static_assert(sizeof(T) % 4 == 0, "whole words");
-    T v;
-    const uint32_t* s = (const uint32_t*)p;
-    uint32_t* d = (uint32_t*)&v;
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = __builtin_amdgcn_readfirstlane(s[i]);
-    return v;
-}
 
 /*
  * The work of one wavefront on the tasks of its lanes with act set, all of
@@ -875,11 +856,12 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
             const int id = __builtin_amdgcn_readlane(my, (int)__builtin_ctzll(pending));
             const bool act = my == id;
             pending &= ~__ballot(act);
-            const GridInfo g = load_uniform(&table[id].g);
-            const MeshBox mb = load_uniform(&table[id].mb);
-            double* acc = load_uniform(&table[id].acc);
+            /* the entry in place at a wave-uniform address (scalar loads); a
+             * local copy would live in scratch, as some of its fields are
+             * indexed at run time */
+            const BatchGeo* e = table + id;
             /* inactive lanes count no fragments */
-            const int64_t nf = wave_columns<ORDER, FMASK, R>(act, p, w, r, g, mb, acc, P, stage, out, status);
+            const int64_t nf = wave_columns<ORDER, FMASK, R>(act, p, w, r, e->g, e->mb, e->acc, P, stage, out, status);
             mesh_count(mcnt, id, nf, 0);
             nfrag += nf;
         }
