@@ -1024,9 +1024,11 @@ struct PassGeo {
 typedef void (*KColsFn)(const double*, const uint64_t*, const uint32_t*, unsigned*, int64_t, GridInfo, MeshBox,
                         double*, const BatchGeo*, TaskOut, unsigned*, unsigned long long*, unsigned long long*);
 
-/* workgroups of a column kernel the current device holds at once, with sb
+/* workgroups of a persistent one-wavefront kernel (the column kernel, the
+ * adjoint's velocity tasks) the current device holds at once, with sb
  * bytes of dynamic LDS */
-static int64_t resident_blocks(KColsFn kern, size_t sb) {
+template <class K>
+static int64_t resident_blocks(K kern, size_t sb) {
     static thread_local int cached_dev = -1;
     static thread_local int ncu = 0;
     int dev = 0;
@@ -1257,26 +1259,40 @@ __global__ __launch_bounds__(FBLOCK) void k_adj(const double* __restrict__ tets,
 
 /* one lane per column task: the velocity part of the adjoint (FP64 column
  * moments: FP32 ones left 8e-5 of the finite differences' gradient on
- * small components) scattered to the tet's particles */
-__global__ __launch_bounds__(FBLOCK) void k_adj_vel(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
-                                                    int64_t ntask, GridInfo g, MeshBox mb, AdjArgs A,
-                                                    unsigned* status) {
+ * small components) scattered to the tet's particles.  The tasks come in
+ * the column kernel's cost order (sorted: the task words in that order),
+ * and the grid is persistent: each workgroup takes the next wavefront of
+ * tasks from the counter until it runs past the end. */
+__global__ __launch_bounds__(FBLOCK) void k_adj_vel(const double* __restrict__ tets, const uint64_t* __restrict__ sorted,
+                                                    unsigned* __restrict__ next, int64_t ntask, GridInfo g, MeshBox mb,
+                                                    AdjArgs A, unsigned* status) {
     __shared__ __align__(32) uint32_t sv[VRec<double>::WORDS];
+    __shared__ unsigned swave;
     const int lane = threadIdx.x;
-    const int64_t c = (int64_t)blockIdx.x * FBLOCK + lane;
-    if (c >= ntask) return;
     LPoly<double> P{(uint32_t*)__builtin_assume_aligned(&sv[VRec<double>::W * lane], 16), 0, 0, 0};
-    const uint64_t w = tasks[c];
-    const int64_t t = (int64_t)(w & 0xffffffffull);
-    double o[4][3];
-    for (int i = 0; i < 4; ++i) o[i][0] = o[i][1] = o[i][2] = 0.0;
-    if (!adj_vel_task<double>(g, mb, tets + (int64_t)TREC * t, w, A.w, P, o)) {
-        atomicOr(status, ST_OVERFLOW);
-        return;
+    unsigned claim = 0u;
+    if (lane == 0) claim = atomicAdd(next, 1u);
+    for (;;) {
+        if (lane == 0) swave = claim;
+        __syncthreads();
+        const int64_t c0 = (int64_t)swave * FBLOCK;
+        __syncthreads();
+        if (c0 >= ntask) break;
+        if (lane == 0) claim = atomicAdd(next, 1u);
+        const int64_t c = c0 + lane;
+        if (c >= ntask) continue;
+        const uint64_t w = sorted[c];
+        const int64_t t = (int64_t)(w & 0xffffffffull);
+        double o[4][3];
+        for (int i = 0; i < 4; ++i) o[i][0] = o[i][1] = o[i][2] = 0.0;
+        if (!adj_vel_task<double>(g, mb, tets + (int64_t)TREC * t, w, A.w, P, o)) {
+            atomicOr(status, ST_OVERFLOW);
+            continue;
+        }
+        for (int v = 0; v < 4; ++v)
+            for (int b = 0; b < 3; ++b)
+                if (o[v][b] != 0.0) adj_scatter(A, t, v, b, o[v][b], A.dvel);
     }
-    for (int v = 0; v < 4; ++v)
-        for (int b = 0; b < 3; ++b)
-            if (o[v][b] != 0.0) adj_scatter(A, t, v, b, o[v][b], A.dvel);
 }
 
 /* ------------------------------------------------------------------ */
@@ -2425,8 +2441,17 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
             const PassGeo G{g, mb, nullptr, nullptr, nullptr};
             rc = for_task_chunks(W, tets, nt, G, PASS_ALL64, status, counters, s, dev, &fell, nullptr,
                                  [&](int64_t, int64_t, const int64_t*, int64_t, const uint64_t* tasks, int64_t nsub) -> int {
-                                     k_adj_vel<<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, tasks, nsub, g, mb, A,
-                                                                                            status);
+                                     /* the forward's cost order and a persistent grid */
+                                     int rc2;
+                                     if ((rc2 = W.perm.get(order_bytes(nsub), dev))) return rc2;
+                                     uint64_t* sorted = order_sorted(W.perm.p, nsub);
+                                     unsigned* next = perm_counter(W.perm.p, nsub);
+                                     k_col_order<false><<<(unsigned)((nsub + ORDW - 1) / ORDW), 256, 0, s>>>(
+                                         tets, tasks, nsub, g, mb, nullptr, order_perm(W.perm.p), sorted, next);
+                                     const int64_t hold = resident_blocks(k_adj_vel, 0);
+                                     const int64_t nwave = (nsub + FBLOCK - 1) / FBLOCK;
+                                     k_adj_vel<<<(unsigned)(nwave < hold ? nwave : hold), FBLOCK, 0, s>>>(
+                                         tets, sorted, next, nsub, g, mb, A, status);
                                      HIPCHK(hipGetLastError());
                                      return 0;
                                  });

@@ -162,6 +162,53 @@ struct Shoelace {
     }
 };
 
+/* The moments int u^p v^q du dv (p + q <= 3) of a polygon from its
+ * vertices in order, by Green's theorem edge by edge (each edge (u0, v0) ->
+ * (u1, v1) contributes cr = u0 v1 - u1 v0 times a symmetric polynomial of
+ * its ends), about the polygon's first vertex (u, v measured from it: a
+ * small polygon far from the coordinates' origin would lose its area to
+ * cancellation between the cross products).  m[0] M00, m[1] M10, m[2] M01,
+ * m[3] M20, m[4] M11, m[5] M02, m[6] M30, m[7] M21, m[8] M12, m[9] M03. */
+struct PolyMom3 {
+    double s[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double fu = 0.0, fv = 0.0, pu = 0.0, pv = 0.0;
+    int n = 0;
+    PVX_HD void edge(double u0, double v0, double u1, double v1) {
+        const double cr = u0 * v1 - u1 * v0;
+        const double uu = u0 * u0 + u0 * u1 + u1 * u1, vv = v0 * v0 + v0 * v1 + v1 * v1;
+        s[0] += cr;
+        s[1] += cr * (u0 + u1);
+        s[2] += cr * (v0 + v1);
+        s[3] += cr * uu;
+        s[4] += cr * (2.0 * u0 * v0 + u0 * v1 + u1 * v0 + 2.0 * u1 * v1);
+        s[5] += cr * vv;
+        s[6] += cr * (u0 + u1) * (u0 * u0 + u1 * u1);
+        s[7] += cr * (3.0 * u0 * u0 * v0 + 2.0 * u0 * u1 * v0 + u1 * u1 * v0 + u0 * u0 * v1 + 2.0 * u0 * u1 * v1 +
+                      3.0 * u1 * u1 * v1);
+        s[8] += cr * (3.0 * v0 * v0 * u0 + 2.0 * v0 * v1 * u0 + v1 * v1 * u0 + v0 * v0 * u1 + 2.0 * v0 * v1 * u1 +
+                      3.0 * v1 * v1 * u1);
+        s[9] += cr * (v0 + v1) * (v0 * v0 + v1 * v1);
+    }
+    PVX_HD void emit(double u, double v) {
+        if (n == 0) {
+            fu = u;
+            fv = v;
+        } else {
+            edge(pu, pv, u - fu, v - fv);
+        }
+        pu = u - fu;
+        pv = v - fv;
+        n++;
+    }
+    /* close the polygon and scale the sums to the moments (about the first
+     * vertex, fu, fv) */
+    PVX_HD void close(double* m) {
+        if (n > 2) edge(pu, pv, 0.0, 0.0);
+        const double f[10] = {1.0 / 2, 1.0 / 6, 1.0 / 6, 1.0 / 12, 1.0 / 24, 1.0 / 12, 1.0 / 20, 1.0 / 60, 1.0 / 60, 1.0 / 20};
+        for (int i = 0; i < 10; ++i) m[i] = n > 2 ? s[i] * f[i] : 0.0;
+    }
+};
+
 /* points of the edge P -> Q inside lo <= x_k <= hi, in order: P if inside,
  * then the crossings of the two lines met along the open edge */
 template <class Emit>
@@ -220,10 +267,10 @@ PVX_HD void adj_point(const TetInv64& T, const double* rec, const double* X, dou
  * out[v] gets dL/dx_{v, axis} of this tet for the task's plane: the flux
  * of lambda_v (g_above - g_below) through each cell's piece of the section.
  * Without velocity fields g is constant per cell and the piece's centroid
- * gives the integral; with them the integrand is cubic and each piece is
- * fanned into triangles integrated with a rule exact to degree 3 (centroid
- * -27/48, points 3/5 of the way to each corner 25/48 each).  Tets of zero
- * volume contribute nothing.
+ * gives the integral; with them the integrand lambda_v (g_above - g_below)
+ * is a cubic polynomial on the plane (lambda and the velocity are affine
+ * there), integrated exactly from the piece's polygon moments up to third
+ * order.  Tets of zero volume contribute nothing.
  */
 template <int FB, bool FULL = false>
 PVX_HD void adj_plane_task(const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task,
@@ -323,49 +370,83 @@ PVX_HD void adj_plane_task(const GridInfo& g, const MeshBox& mb, const double* r
         for (int k = k0; k <= k1; ++k) {
             const int64_t gi = (int64_t)j * Sb + (int64_t)k * Sc;
             if (FULL) {
-                /* the weights of the cells above and below the plane (none
+                /* G = g_above - g_below, the cells' weights differenced (none
                  * outside the grid) */
                 const double* Wa = p < na ? w + ADJ_NW * (gi + (int64_t)p * Sa) : nullptr;
                 const double* Wb = p > 0 ? w + ADJ_NW * (gi + (int64_t)(p - 1) * Sa) : nullptr;
-                /* fan the piece from its first point; each triangle by the
-                 * degree-3 rule */
-                double P0u = 0, P0v = 0, Pu = 0, Pv = 0;
-                int np = 0;
-                auto tri = [&](double au, double av, double bu, double bv, double cu, double cv) {
-                    const double ar = 0.5 * fabs((bu - au) * (cv - av) - (cu - au) * (bv - av)) * db * dc;
-                    if (ar == 0.0) return;
-                    const double Qu[4] = {(au + bu + cu) / 3.0, 0.6 * au + 0.2 * (bu + cu), 0.6 * bu + 0.2 * (au + cu),
-                                          0.6 * cu + 0.2 * (au + bu)};
-                    const double Qv[4] = {(av + bv + cv) / 3.0, 0.6 * av + 0.2 * (bv + cv), 0.6 * bv + 0.2 * (av + cv),
-                                          0.6 * cv + 0.2 * (av + bv)};
-                    for (int qp = 0; qp < 4; ++qp) {
-                        const double wt = ar * (qp == 0 ? -27.0 / 48.0 : 25.0 / 48.0);
-                        const double xb = w0b + db * ((double)j + Qu[qp]) - sb, xc = w0c + dc * ((double)k + Qv[qp]) - sc;
-                        const double X[3] = {a == 0 ? xp : (b == 0 ? xb : xc), a == 1 ? xp : (b == 1 ? xb : xc),
-                                             a == 2 ? xp : (b == 2 ? xb : xc)};
-                        double l[4], v[3];
-                        adj_point(T, rec, X, l, v);
-                        const double f = wt * ((Wa ? adj_g(Wa, v) : 0.0) - (Wb ? adj_g(Wb, v) : 0.0));
-                        for (int r = 0; r < 4; ++r) acc[r] += f * l[r];
-                    }
-                };
-                auto emitf = [&](double uu, double vv) {
-                    vv -= (double)k;
-                    if (np == 0) {
-                        P0u = uu;
-                        P0v = vv;
-                    } else if (np >= 2) {
-                        tri(P0u, P0v, Pu, Pv, uu, vv);
-                    }
-                    Pu = uu;
-                    Pv = vv;
-                    np++;
-                };
+                double Wd[ADJ_NW];
+                bool any = false;
+                for (int i = 0; i < ADJ_NW; ++i) {
+                    Wd[i] = (Wa ? Wa[i] : 0.0) - (Wb ? Wb[i] : 0.0);
+                    any = any || Wd[i] != 0.0;
+                }
+                if (!any) continue;
+                /* the piece's moments in cell-local (u, v) */
+                PolyMom3 PM;
+                auto emitm = [&](double uu, double vv) { PM.emit(uu, vv - (double)k); };
                 for (int q = 0; q < m; ++q) {
                     const int r = q + 1 == m ? 0 : q + 1;
                     slab_edge(R.u[q * FB], R.v[q * FB], R.u[r * FB], R.v[r * FB], 1, (double)k, (double)(k + 1),
-                              emitf);
+                              emitm);
                 }
+                double M[10];
+                PM.close(M);
+                if (M[0] == 0.0) continue;
+                if (M[0] < 0.0) /* the piece's orientation: moments of the area itself */
+                    for (int i = 0; i < 10; ++i) M[i] = -M[i];
+                /* lambda and the velocity as affine functions of (u, v) on the
+                 * plane: value at the piece's first vertex (the moments'
+                 * origin), rate along u and v */
+                const double xb = w0b + db * ((double)j + PM.fu) - sb, xc = w0c + dc * ((double)k + PM.fv) - sc;
+                const double X[3] = {a == 0 ? xp : (b == 0 ? xb : xc), a == 1 ? xp : (b == 1 ? xb : xc),
+                                     a == 2 ? xp : (b == 2 ? xb : xc)};
+                double l0[4], lu[4], lv[4];
+                for (int r = 0; r < 3; ++r) {
+                    l0[r + 1] = T.B[r][0] * (X[0] - T.x0[0]) + T.B[r][1] * (X[1] - T.x0[1]) +
+                                T.B[r][2] * (X[2] - T.x0[2]);
+                    lu[r + 1] = pick3(T.B[r], b) * db;
+                    lv[r + 1] = pick3(T.B[r], c) * dc;
+                }
+                l0[0] = 1.0 - l0[1] - l0[2] - l0[3];
+                lu[0] = -(lu[1] + lu[2] + lu[3]);
+                lv[0] = -(lv[1] + lv[2] + lv[3]);
+                double v0[3], vu[3], vw[3];
+                for (int e = 0; e < 3; ++e) {
+                    v0[e] = vu[e] = vw[e] = 0.0;
+                    for (int i = 0; i < 4; ++i) {
+                        const double ui = rec[12 + 3 * i + e];
+                        v0[e] += l0[i] * ui;
+                        vu[e] += lu[i] * ui;
+                        vw[e] += lv[i] * ui;
+                    }
+                }
+                /* G(u, v) = c00 + c10 u + c01 v + c20 u^2 + c11 u v + c02 v^2
+                 * with vel = v0 + vu u + vw v: G = Wd0 + Wd1 . vel + vel^T Wd2 vel */
+                double S0[3], Su[3], Sw[3]; /* Wd2 times v0, vu, vw, and its transpose times them */
+                double T0[3], Tu[3];
+                for (int e = 0; e < 3; ++e) {
+                    S0[e] = Wd[4 + 3 * e] * v0[0] + Wd[5 + 3 * e] * v0[1] + Wd[6 + 3 * e] * v0[2];
+                    Su[e] = Wd[4 + 3 * e] * vu[0] + Wd[5 + 3 * e] * vu[1] + Wd[6 + 3 * e] * vu[2];
+                    Sw[e] = Wd[4 + 3 * e] * vw[0] + Wd[5 + 3 * e] * vw[1] + Wd[6 + 3 * e] * vw[2];
+                    T0[e] = Wd[4 + e] * v0[0] + Wd[7 + e] * v0[1] + Wd[10 + e] * v0[2];
+                    Tu[e] = Wd[4 + e] * vu[0] + Wd[7 + e] * vu[1] + Wd[10 + e] * vu[2];
+                }
+                const double c00 = Wd[0] + Wd[1] * v0[0] + Wd[2] * v0[1] + Wd[3] * v0[2] +
+                                   v0[0] * S0[0] + v0[1] * S0[1] + v0[2] * S0[2];
+                const double c10 = Wd[1] * vu[0] + Wd[2] * vu[1] + Wd[3] * vu[2] + vu[0] * S0[0] + vu[1] * S0[1] +
+                                   vu[2] * S0[2] + vu[0] * T0[0] + vu[1] * T0[1] + vu[2] * T0[2];
+                const double c01 = Wd[1] * vw[0] + Wd[2] * vw[1] + Wd[3] * vw[2] + vw[0] * S0[0] + vw[1] * S0[1] +
+                                   vw[2] * S0[2] + vw[0] * T0[0] + vw[1] * T0[1] + vw[2] * T0[2];
+                const double c20 = vu[0] * Su[0] + vu[1] * Su[1] + vu[2] * Su[2];
+                const double c11 = vw[0] * Su[0] + vw[1] * Su[1] + vw[2] * Su[2] + vw[0] * Tu[0] + vw[1] * Tu[1] +
+                                   vw[2] * Tu[2];
+                const double c02 = vw[0] * Sw[0] + vw[1] * Sw[1] + vw[2] * Sw[2];
+                /* int G, int u G, int v G over the piece */
+                const double I0 = c00 * M[0] + c10 * M[1] + c01 * M[2] + c20 * M[3] + c11 * M[4] + c02 * M[5];
+                const double Iu = c00 * M[1] + c10 * M[3] + c01 * M[4] + c20 * M[6] + c11 * M[7] + c02 * M[8];
+                const double Iw = c00 * M[2] + c10 * M[4] + c01 * M[5] + c20 * M[7] + c11 * M[8] + c02 * M[9];
+                const double area = db * dc;
+                for (int r = 0; r < 4; ++r) acc[r] += area * (l0[r] * I0 + lu[r] * Iu + lv[r] * Iw);
                 continue;
             }
             const double wa = p < na ? w[gi + (int64_t)p * Sa] : 0.0;
