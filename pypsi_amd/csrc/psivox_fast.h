@@ -1180,21 +1180,20 @@ PVX_HD int column_cells(const CopySetup<float>& C, int il, int jl, int* nvert = 
  * each vertex lies: state 0 dropped by the lower plane, 1 kept, 2 dropped by
  * the upper plane; 81 patterns, pattern = sum of state_v 3^v.  Each entry is
  * the result of clip_graph on the tet for that pattern: per output vertex
- * (rank order) its slot, its link word, and its source (kind 0: tet vertex
- * vc; kind 1: cut of the upper plane on the edge from tet vertex vc to tet
- * vertex vn; kind 2: cut of the lower plane on the edge from slot vc to tet
- * vertex vn).  The device evaluates the cuts with the generic clip's own
- * formula in the generic clip's sequence, so the result is the same
- * polyhedron.
+ * (rank order) its slot and its link word, and the new vertices in the order
+ * they are made (kind 1: cut of the upper plane on the edge from tet vertex
+ * vc to tet vertex vn; kind 2: cut of the lower plane on the edge from slot
+ * vc to tet vertex vn; each into slot vx).  The device evaluates the cuts
+ * with the generic clip's own formula in the generic clip's sequence, so
+ * the result is the same polyhedron.
  */
 struct alignas(16) SlabStep {
     uint32_t link[8];
     uint8_t slot[8];
-    uint8_t src[8]; /* kind in bits 6-7, vc in bits 2-5, vn in bits 0-1 */
-    uint8_t nv;
-    uint8_t pad[3];
+    uint16_t cut[8]; /* kind << 12 | vx << 8 | vc << 4 | vn */
+    uint8_t nv, ncut;
+    uint8_t pad[2];
     int32_t y2base; /* the y-slab table's first entry for this pattern */
-    uint8_t pad2[8];
 };
 static_assert(sizeof(SlabStep) == 64, "four 16-byte words per entry");
 
@@ -1223,6 +1222,12 @@ PVX_HD T load_entry(const T& src) {
 #endif
 }
 
+/* component i of a 4-vector without dynamic indexing */
+template <class R>
+PVX_HD R pick4(const R* v, int i) {
+    return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
+}
+
 template <class R>
 PVX_HD void tet_start(LPoly<R>& P) {
     P.live = 0xfu;
@@ -1244,8 +1249,8 @@ inline bool make_slab_step(int pattern, SlabStep& E) {
     static uint32_t mem[VRec<float>::WORDS];
     LPoly<float> P{mem, 0};
     tet_start(P);
-    uint8_t src[NVF] = {0};
-    for (int v = 0; v < 4; ++v) src[v] = (uint8_t)(v << 2);
+    uint16_t cut[2 * NVF];
+    int nc = 0;
     int st[4];
     for (int v = 0, x = pattern; v < 4; ++v, x /= 3) st[v] = x % 3;
     E = SlabStep{};
@@ -1253,24 +1258,25 @@ inline bool make_slab_step(int pattern, SlabStep& E) {
     for (int v = 0; v < 4; ++v) gone |= (st[v] == 2 ? 1u : 0u) << v;
     if (gone == P.live) return true;
     if (gone && !clip_graph(P, gone, [&](int vx, int vc, int vn) {
-            src[vx] = (uint8_t)(64 | (vc << 2) | vn);
+            if (nc < 2 * NVF) cut[nc++] = (uint16_t)((1 << 12) | (vx << 8) | (vc << 4) | vn);
         }))
         return false;
     gone = 0;
     for (int v = 0; v < 4; ++v) gone |= (st[v] == 0 && ((P.live >> v) & 1u) ? 1u : 0u) << v;
     if (gone == P.live) return true;
     if (gone && !clip_graph(P, gone, [&](int vx, int vc, int vn) {
-            src[vx] = (uint8_t)(128 | (vc << 2) | vn);
+            if (nc < 2 * NVF) cut[nc++] = (uint16_t)((2 << 12) | (vx << 8) | (vc << 4) | vn);
         }))
         return false;
-    if (P.nv > 8) return false;
+    if (P.nv > 8 || nc > 8) return false;
     E.nv = (uint8_t)P.nv;
+    E.ncut = (uint8_t)nc;
     for (int r = 0; r < P.nv; ++r) {
         const int sl = P.at(r);
         E.slot[r] = (uint8_t)sl;
-        E.src[r] = src[sl];
         E.link[r] = P.nbw(sl);
     }
+    for (int i = 0; i < nc; ++i) E.cut[i] = cut[i];
     return true;
 }
 
@@ -1330,28 +1336,27 @@ PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g,
         P.nv = 0;
         return true;
     }
-    const int nv = E.nv;
-    /* upper cuts (between tet vertices), as the generic clip makes them */
+    const int nv = E.nv, nc = E.ncut;
+    /* the cut points in the order the generic clips make them (upper cuts,
+     * between tet vertices, then lower cuts, from a survivor to a tet
+     * vertex); a lane writes no lower cut once one of its upper cuts lies
+     * on the dropped side of the lower plane (the table does not apply) */
     bool ok = true;
-    for (int r = 0; r < 8; ++r) {
-        if (r >= nv || (E.src[r] >> 6) != 1) continue;
-        const int vc = (E.src[r] >> 2) & 15, vn = E.src[r] & 3, vx = E.slot[r];
-        const R sc = vc == 0 ? th[0] : (vc == 1 ? th[1] : (vc == 2 ? th[2] : th[3]));
-        const R sn = vn == 0 ? th[0] : (vn == 1 ? th[1] : (vn == 2 ? th[2] : th[3]));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (i >= nc) continue;
+        const int cw = E.cut[i];
+        const bool up = (cw >> 12) == 1;
+        if (!up && !ok) continue;
+        const int vx = (cw >> 8) & 15, vc = (cw >> 4) & 15, vn = cw & 3;
+        const R sn = up ? pick4(th, vn) : pick4(tl, vn);
+        const R sc = up ? pick4(th, vc) : sdist(G.rec, m, clo, offl, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
         cut_point(P, vx, vc, vn, sc, sn);
-        if (has_lo && sdist(G.rec, m, clo, offl, P.qa(vx, 0), P.qa(vx, 1), P.qa(vx, 2)) >= R(0)) ok = false;
+        if (up && has_lo && sdist(G.rec, m, clo, offl, P.qa(vx, 0), P.qa(vx, 1), P.qa(vx, 2)) >= R(0)) ok = false;
     }
     if (!ok) {
         tet_start(P);
         return false;
-    }
-    /* lower cuts (from a survivor to a tet vertex) */
-    for (int r = 0; r < 8; ++r) {
-        if (r >= nv || (E.src[r] >> 6) != 2) continue;
-        const int vc = (E.src[r] >> 2) & 15, vn = E.src[r] & 3, vx = E.slot[r];
-        const R sc = sdist(G.rec, m, clo, offl, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
-        const R sn = vn == 0 ? tl[0] : (vn == 1 ? tl[1] : (vn == 2 ? tl[2] : tl[3]));
-        cut_point(P, vx, vc, vn, sc, sn);
     }
     uint32_t live = 0;
     uint64_t ord = 0;
@@ -1375,11 +1380,11 @@ PVX_HD bool slab_from_table(LPoly<R>& P, const CopyGeo<R>& G, const GridInfo& g,
  * vertices lies (state 0 dropped by the lower plane, 1 kept, 2 dropped by
  * the upper plane; state = sum of state_r 3^r over the vertex ranks r).
  * Each entry is the result of clip_graph on entry pat's graph for those
- * states: per output vertex (rank order) its slot, its link word (fan
- * apexes included) and its source (kind 0: the slab polyhedron's vertex in
- * that slot; kind 1: cut of the upper plane on the edge from slot vc to
- * slot vn; kind 2: cut of the lower plane on the edge from slot vc to slot
- * vn).  The lane evaluates the cuts with clip_f's own formula in clip_f's
+ * states: per output vertex (rank order) its slot and its link word (fan
+ * apexes included), and the new vertices in the order they are made (kind
+ * 1: cut of the upper plane on the edge from slot vc to slot vn; kind 2:
+ * cut of the lower plane on the edge from slot vc to slot vn; each into
+ * slot vx).  The lane evaluates the cuts with clip_f's own formula in clip_f's
  * sequence, so the result is the two generic clips', bit for bit.  Entries
  * are one 128-byte line each; nv = Y2NONE marks a state set that no convex
  * polyhedron produces (the generic clips then run).
@@ -1389,9 +1394,11 @@ constexpr uint8_t Y2NONE = 0xffu;
 
 struct alignas(16) Y2Step {
     uint32_t link[Y2NV];
-    uint16_t src[Y2NV]; /* kind << 12 | slot << 8 | vc << 4 | vn */
-    uint8_t nv;
-    uint8_t pad[128 - 4 * Y2NV - 2 * Y2NV - 1];
+    uint8_t slot[Y2NV];
+    uint16_t cut[Y2NV]; /* the new vertices in the order they are made (the upper plane's first):
+                           kind << 12 | slot << 8 | vc << 4 | vn */
+    uint8_t nv, ncut;
+    uint8_t pad[128 - 4 * Y2NV - Y2NV - 2 * Y2NV - 2];
 };
 static_assert(sizeof(Y2Step) == 128, "one line per entry");
 
@@ -1408,14 +1415,14 @@ inline void make_y2_step(int pat, int st, Y2Step& Y) {
     LPoly<float> P{mem, 0u, 0, 0ull};
     int sr[8];
     for (int r = 0, x = st; r < nv; ++r, x /= 3) sr[r] = x % 3;
-    uint16_t src[NVF];
+    uint16_t cut[2 * NVF];
+    int nc = 0;
     uint32_t gone = 0u;
     for (int r = 0; r < nv; ++r) {
         const int sl = E.slot[r];
         P.setnbw(sl, E.link[r]);
         P.live |= 1u << sl;
         P.ord |= (uint64_t)sl << (4 * r);
-        src[sl] = (uint16_t)(sl << 8);
         if (sr[r] == 2) gone |= 1u << sl;
     }
     P.nv = nv;
@@ -1424,7 +1431,7 @@ inline void make_y2_step(int pat, int st, Y2Step& Y) {
         return;
     }
     if (gone && !clip_graph(P, gone, [&](int vx, int vc, int vn) {
-            src[vx] = (uint16_t)((1 << 12) | (vx << 8) | (vc << 4) | vn);
+            if (nc < 2 * NVF) cut[nc++] = (uint16_t)((1 << 12) | (vx << 8) | (vc << 4) | vn);
         }))
         return;
     gone = 0u;
@@ -1435,16 +1442,22 @@ inline void make_y2_step(int pat, int st, Y2Step& Y) {
         return;
     }
     if (gone && !clip_graph(P, gone, [&](int vx, int vc, int vn) {
-            src[vx] = (uint16_t)((2 << 12) | (vx << 8) | (vc << 4) | vn);
+            if (nc < 2 * NVF) cut[nc++] = (uint16_t)((2 << 12) | (vx << 8) | (vc << 4) | vn);
         }))
         return;
-    if (P.nv > Y2NV) return;
+    if (P.nv > Y2NV || nc > Y2NV) return;
+    /* an upper cut whose vertex the lower clip drops again has no place in
+     * the list (the device takes the upper cuts as kept): no entry */
+    for (int i = 0; i < nc; ++i)
+        if ((cut[i] >> 12) == 1 && !((P.live >> ((cut[i] >> 8) & 15)) & 1u)) return;
     Y.nv = (uint8_t)P.nv;
+    Y.ncut = (uint8_t)nc;
     for (int r = 0; r < P.nv; ++r) {
         const int sl = P.at(r);
         Y.link[r] = P.nbw(sl);
-        Y.src[r] = src[sl];
+        Y.slot[r] = (uint8_t)sl;
     }
+    for (int i = 0; i < nc; ++i) Y.cut[i] = cut[i];
 }
 
 /* the number of y-slab entries the x-slab table asks for */
@@ -1512,30 +1525,34 @@ PVX_HD bool yslab_from_table(LPoly<R>& P, int y2first, const CopyGeo<R>& G, cons
         P.nv = 0;
         return true;
     }
-    /* upper cuts, as the generic clip makes them (into free slots) */
+    /* the cut points in the order the generic clips make them (each lane
+     * walks its own list: upper cuts into free slots, then lower cuts), with
+     * the generic clips' formula; a lane writes no lower cut once one of its
+     * upper cuts lies on the dropped side of the lower plane (a slab thinner
+     * than the precision resolves: its polyhedron is left for the generic
+     * clips) */
+    const int nc = Y.ncut;
     bool ok = true;
-    for (int r = 0; r < Y2NV; ++r) {
-        if (r >= ny || (Y.src[r] >> 12) != 1) continue;
-        const int vx = (Y.src[r] >> 8) & 15, vc = (Y.src[r] >> 4) & 15, vn = Y.src[r] & 15;
-        const R sc = sdist(G.rec, m, chi, offh, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
-        const R sn = sdist(G.rec, m, chi, offh, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
+#pragma unroll
+    for (int i = 0; i < Y2NV; ++i) {
+        if (i >= nc) continue;
+        const int cw = Y.cut[i];
+        const bool up = (cw >> 12) == 1;
+        if (!up && !ok) continue;
+        const int vx = (cw >> 8) & 15, vc = (cw >> 4) & 15, vn = cw & 15;
+        const double cp = up ? chi : clo;
+        const R off = up ? offh : offl;
+        const R sc = sdist(G.rec, m, cp, off, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
+        const R sn = sdist(G.rec, m, cp, off, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
         cut_point(P, vx, vc, vn, sc, sn);
-        if (has_lo && sdist(G.rec, m, clo, offl, P.qa(vx, 0), P.qa(vx, 1), P.qa(vx, 2)) >= R(0)) ok = false;
+        if (up && has_lo && sdist(G.rec, m, clo, offl, P.qa(vx, 0), P.qa(vx, 1), P.qa(vx, 2)) >= R(0)) ok = false;
     }
     if (!ok) return false;
-    /* lower cuts, from a survivor to a vertex the lower plane drops */
-    for (int r = 0; r < Y2NV; ++r) {
-        if (r >= ny || (Y.src[r] >> 12) != 2) continue;
-        const int vx = (Y.src[r] >> 8) & 15, vc = (Y.src[r] >> 4) & 15, vn = Y.src[r] & 15;
-        const R sc = sdist(G.rec, m, clo, offl, P.qa(vc, 0), P.qa(vc, 1), P.qa(vc, 2));
-        const R sn = sdist(G.rec, m, clo, offl, P.qa(vn, 0), P.qa(vn, 1), P.qa(vn, 2));
-        cut_point(P, vx, vc, vn, sc, sn);
-    }
     uint32_t live = 0u;
     uint64_t ord = 0ull;
     for (int r = 0; r < Y2NV; ++r) {
         if (r >= ny) continue;
-        const int sl = (Y.src[r] >> 8) & 15;
+        const int sl = Y.slot[r];
         P.setnbw(sl, Y.link[r]);
         live |= 1u << sl;
         ord |= (uint64_t)sl << (4 * r);
