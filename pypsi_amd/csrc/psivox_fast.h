@@ -589,16 +589,21 @@ PVX_HD void slab_setup(const LPoly<R>& P, const CopyGeo<R>& G, int ax, bool has_
     for (int ra = 0; ra < P.nv; ++ra) {
         const int a = P.at(ra);
         const uint32_t wa = P.nbw(a);
+        /* the rank's three directed edges as a 3-bit group each, without
+         * branches (an edge whose apex is one of its ends has no triangle) */
+        uint32_t gi = 0u, gm = 0u;
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             const int b = nbf(wa, k), c = apf(wa, k);
-            if (c == a || c == b) continue;
             const uint32_t tri = (1u << a) | (1u << b) | (1u << c);
-            const uint64_t bit = 1ull << (3 * ra + k);
-            if ((both & tri) == tri)
-                inside |= bit;
-            else if ((above & tri) && (below & tri))
-                mask |= bit;
+            const bool valid = c != a && c != b;
+            const bool in = valid && (both & tri) == tri;
+            const bool cut = valid && !in && (above & tri) != 0u && (below & tri) != 0u;
+            gi |= (in ? 1u : 0u) << k;
+            gm |= (cut ? 1u : 0u) << k;
         }
+        inside |= (uint64_t)gi << (3 * ra);
+        mask |= (uint64_t)gm << (3 * ra);
     }
     S.mask = mask;
     S.inside = inside;
