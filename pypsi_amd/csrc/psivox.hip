@@ -318,8 +318,11 @@ __device__ void block_count(int64_t a, int64_t b, unsigned long long* out) {
     }
 }
 
-/* one thread per tet record, FP64 throughout */
-__global__ __launch_bounds__(256) void k_vox_plain(const double* __restrict__ tets, int64_t ntet, GridInfo g,
+/* one thread per tet record, FP64 throughout.  The polyhedra live in
+ * scratch, so the kernel waits on memory: capped at 128 VGPRs it holds 4
+ * wavefronts per SIMD instead of 2 (2.43 -> 1.70 s on the 128^3 -> 256^3
+ * m, v, vv workload; 8 wavefronts spill too much, 3.18 s) */
+__global__ __launch_bounds__(256, 4) void k_vox_plain(const double* __restrict__ tets, int64_t ntet, GridInfo g,
                                                    MeshBox mb, double* __restrict__ acc, unsigned* status,
                                                    unsigned long long* counters) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
