@@ -654,7 +654,37 @@ def cpu_sample(probs, reftol=1.0, maxlvl=0, what=""):
             "sample_fragments": nfrag, "s": dt}
 
 
-def fwd_adjoint_leg(steps=5, warmup=2):
+def fwd_adjoint_cpu(pos, mass, conn, gn, w, gpu_fwd_ms, gpu_adj_ms, frac=8):
+    """The same forward and mass adjoint on the host: the reference's own C
+    core single-threaded for the forward and the oracle's FP64 adjoint
+    single-threaded, each on the first 1/frac of the elements (a bounded
+    sample), scaled to the whole mesh by the element count."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    box = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    e1 = max(1, conn.shape[0] // frac)
+    P = O.Problem(pos.reshape(-1, 3), None, np.ascontiguousarray(mass[:e1]), np.ascontiguousarray(conn[:e1]),
+                  periodic_box=box, n=(gn,) * 3, bounds=box)
+    scale = conn.shape[0] / e1
+    t0 = time.perf_counter()
+    kind = "reference"
+    if O.have_ref():
+        O.ref_voxels(P)
+    else:
+        O.voxels(P)
+        kind = "port"
+    t1 = time.perf_counter()
+    O.mass_adjoint(P, w)
+    t2 = time.perf_counter()
+    fwd, adj = 1e3 * (t1 - t0) * scale, 1e3 * (t2 - t1) * scale
+    return {"fwd_ms": fwd, "fwd_kind": kind, "adjoint_ms": adj, "adjoint_kind": "port (the oracle's FP64 adjoint)",
+            "ratio_fwd_plus_adj_over_fwd": (fwd + adj) / fwd, "cores": 1,
+            "sample": f"first {e1} of {conn.shape[0]} elements, scaled by the element count",
+            "gpu_speedup_fwd_plus_adj": (fwd + adj) / (gpu_fwd_ms + gpu_adj_ms)}
+
+
+def fwd_adjoint_leg(steps=5, warmup=2, cpu=True):
     """64^3 particles, rms 1.0 cells -> 128^3, mass: forward (FP64-clip
     column kernel, voxel_density's default) and the mass adjoint of
     L = sum (rho - rho_t)^2, rho_t from the same construction with seed 1.
@@ -713,6 +743,8 @@ def fwd_adjoint_leg(steps=5, warmup=2):
            "adjoint_dtype": "f64",
            "fp32_forward": {"kernel": "auto", "fwd_ms": f32, "ratio_fwd_plus_adj_over_fwd": (f32 + a) / f32,
                             "gradient_vs_fd": "within 3e-4 (FP32 cell masses)"}}
+    if cpu:
+        out["cpu"] = fwd_adjoint_cpu(pos, mass, conn, gn, w.cpu().numpy(), f, a)
     # the fields m, v, vv and their adjoint with respect to positions and
     # velocities (voxel_fields: FP64 column forward)
     vel = torch.from_numpy(build_inputs("zeldovich-64^3-rms1.0-to-128^3-m")[1]).to(dev)
@@ -891,7 +923,7 @@ def main():
         out["fp64_columns"]["what"] = ("the column kernel with every tet clipped in FP64 (kernel 'fast64'): "
                                        "the FP32 path's speed-up at equal method, value / this value")
         log("[bench] forward + adjoint, 64^3 -> 128^3")
-        out["fwd_adjoint"] = fwd_adjoint_leg()
+        out["fwd_adjoint"] = fwd_adjoint_leg(cpu=not args.no_cpu_baseline)
         log("[bench] batched uneven meshes")
         out["batched_uneven"] = batched_leg(cpu=not args.no_cpu_baseline)
         log("[bench] refined clumps, level 2, 512^3")
