@@ -521,15 +521,64 @@ struct PlaneCarry {
     bool valid; /* the last cell had an upper plane */
     bool found; /* and an edge crossed it at p */
     R p[3];
+    /* that plane's sides: the live vertices at or below it, and the fan
+     * triangles (bit 3r + k, as in SlabCell) wholly at or below it and
+     * crossing it; all: every fan triangle of the column */
+    uint32_t below;
+    uint64_t bm, xm, all;
 };
+
+/* One pass over P's fan triangles against the vertex set le (the vertices
+ * at or below a plane): all fan triangles, those wholly in le, and those
+ * with vertices on both sides.  Bit 3r + k: the triangle of the directed
+ * edge from the vertex of rank r to its neighbour k (an edge whose fan apex
+ * is one of its ends has no triangle). */
+template <class R>
+PVX_HD void tri_sides(const LPoly<R>& P, uint32_t le, uint64_t& all, uint64_t& bm, uint64_t& xm) {
+    all = bm = xm = 0ull;
+    for (int ra = 0; ra < P.nv; ++ra) {
+        const int a = P.at(ra);
+        const uint32_t wa = P.nbw(a);
+        /* the rank's three directed edges as a 3-bit group each, without branches */
+        uint32_t ga = 0u, gb = 0u, gx = 0u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int b = nbf(wa, k), c = apf(wa, k);
+            const uint32_t tri = (1u << a) | (1u << b) | (1u << c);
+            const bool valid = c != a && c != b;
+            const uint32_t in = le & tri;
+            ga |= (valid ? 1u : 0u) << k;
+            gb |= (valid && in == tri ? 1u : 0u) << k;
+            gx |= (valid && in != 0u && in != tri ? 1u : 0u) << k;
+        }
+        all |= (uint64_t)ga << (3 * ra);
+        bm |= (uint64_t)gb << (3 * ra);
+        xm |= (uint64_t)gx << (3 * ra);
+    }
+}
+
+/* the live vertices of P at or below the plane z = p (z = zd . q) */
+template <class R>
+PVX_HD uint32_t side_le(const LPoly<R>& P, R zd0, R zd1, R zd2, R p) {
+    uint32_t le = 0u;
+    for (uint32_t lm = P.live; lm; lm &= lm - 1u) {
+        const int v = __builtin_ctz(lm);
+        const R z = zd0 * P.qa(v, 0) + zd1 * P.qa(v, 1) + zd2 * P.qa(v, 2);
+        le |= (z <= p ? 1u : 0u) << v;
+    }
+    return le;
+}
 
 /*
  * The slab lo < x_ax <= hi of P (either side may be open): its planes, the
- * origin and cap anchor on them, and the mask of fan triangles reaching
+ * origin and cap anchor on them, and the masks of fan triangles reaching
  * into it (z_max > lo and z_min <= hi: a triangle lying on the upper plane
- * belongs to the piece, one on the lower plane not).  pc (optional) brings
- * the lower plane's point from the column's previous cell and takes the
- * upper plane's for the next.
+ * belongs to the piece, one on the lower plane not): cut by one of its
+ * planes (crossing lo or hi) or wholly inside (wholly above lo and wholly
+ * at or below hi).  pc (optional) brings the lower plane's point and sides
+ * from the column's previous cell, whose upper plane it is, and takes the
+ * upper plane's for the next: each plane's vertex sides and triangle pass
+ * are computed once, and the column's last cell needs neither.
  */
 template <class R>
 PVX_HD void slab_setup(const LPoly<R>& P, const CopyGeo<R>& G, int ax, bool has_lo, double lo_c, bool has_hi,
@@ -545,21 +594,43 @@ PVX_HD void slab_setup(const LPoly<R>& P, const CopyGeo<R>& G, int ax, bool has_
     S.zd[0] = zd0;
     S.zd[1] = zd1;
     S.zd[2] = zd2;
-    /* each vertex's side of the two planes, once: bit v of above is
-     * z(v) > lo, of below z(v) <= hi */
-    uint32_t above = 0u, below = 0u;
-    for (uint32_t lm = P.live; lm; lm &= lm - 1u) {
-        const int v = __builtin_ctz(lm);
-        const R z = zd0 * P.qa(v, 0) + zd1 * P.qa(v, 1) + zd2 * P.qa(v, 2);
-        above |= (z > S.lo ? 1u : 0u) << v;
-        below |= (z <= S.hi ? 1u : 0u) << v;
+    const bool carried = has_lo && pc && pc->valid;
+    /* the upper plane's sides (all of P below an open upper side) */
+    const uint32_t below = has_hi ? side_le(P, zd0, zd1, zd2, S.hi) : P.live;
+    uint64_t all, bhi, xhi;
+    if (has_hi || !carried) {
+        tri_sides(P, below, all, bhi, xhi);
+    } else {
+        all = pc->all;
+        bhi = all;
+        xhi = 0ull;
     }
+    /* the lower plane's: carried, computed, or none */
+    uint32_t above;
+    uint64_t alo, xlo;
+    if (carried) {
+        above = P.live & ~pc->below;
+        xlo = pc->xm;
+        alo = all & ~(pc->bm | pc->xm);
+    } else if (has_lo) {
+        const uint32_t le = side_le(P, zd0, zd1, zd2, S.lo);
+        uint64_t a2, blo;
+        tri_sides(P, le, a2, blo, xlo);
+        above = P.live & ~le;
+        alo = all & ~(blo | xlo);
+    } else {
+        above = P.live;
+        xlo = 0ull;
+        alo = all;
+    }
+    S.mask = xlo | xhi;
+    S.inside = alo & bhi;
     const int v0 = P.at(0);
     R O[3] = {P.qa(v0, 0), P.qa(v0, 1), P.qa(v0, 2)};
     R Gh[3] = {O[0], O[1], O[2]};
     bool fh = false; /* a point on the upper plane was found */
     if (has_lo) {
-        if (pc && pc->valid) {
+        if (carried) {
             if (pc->found) O[0] = pc->p[0], O[1] = pc->p[1], O[2] = pc->p[2];
         } else {
             plane_point_le(P, zd0, zd1, zd2, S.lo, ~above, O);
@@ -576,37 +647,15 @@ PVX_HD void slab_setup(const LPoly<R>& P, const CopyGeo<R>& G, int ax, bool has_
     if (pc) {
         pc->valid = has_hi;
         pc->found = fh;
+        pc->below = below;
+        pc->bm = bhi;
+        pc->xm = xhi;
+        pc->all = all;
     }
     for (int i = 0; i < 3; ++i) {
         S.O[i] = O[i];
         S.gh[i] = Gh[i] - O[i];
     }
-    /* a fan triangle reaches into the slab when its highest vertex is above
-     * lo and its lowest at or below hi: one of its vertices each; it lies
-     * wholly inside when all three are */
-    const uint32_t both = above & below;
-    uint64_t mask = 0, inside = 0;
-    for (int ra = 0; ra < P.nv; ++ra) {
-        const int a = P.at(ra);
-        const uint32_t wa = P.nbw(a);
-        /* the rank's three directed edges as a 3-bit group each, without
-         * branches (an edge whose apex is one of its ends has no triangle) */
-        uint32_t gi = 0u, gm = 0u;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int b = nbf(wa, k), c = apf(wa, k);
-            const uint32_t tri = (1u << a) | (1u << b) | (1u << c);
-            const bool valid = c != a && c != b;
-            const bool in = valid && (both & tri) == tri;
-            const bool cut = valid && !in && (above & tri) != 0u && (below & tri) != 0u;
-            gi |= (in ? 1u : 0u) << k;
-            gm |= (cut ? 1u : 0u) << k;
-        }
-        inside |= (uint64_t)gi << (3 * ra);
-        mask |= (uint64_t)gm << (3 * ra);
-    }
-    S.mask = mask;
-    S.inside = inside;
 }
 
 /*
