@@ -1060,17 +1060,23 @@ PVX_HD bool slab_columns(const CopySetup<R>& C, int i, const GridInfo& g, int& j
     return j1 > j0;
 }
 
-/* column task: tet index, copy, and the column's cell (i, j) relative to
- * the copy's index box; 32 + 3 + 14 + 14 bits */
-PVX_HD uint64_t pack_task(int64_t t, int h, int il, int jl) {
-    return (uint64_t)t | ((uint64_t)h << 32) | ((uint64_t)il << 35) | ((uint64_t)jl << 49);
+/* column task: tet index, copy, the column's cell (i, j) relative to the
+ * copy's index box, and whether the column is the first and the last of its
+ * x-slab's columns (its y planes are then the copy's window planes or open;
+ * the emit pass knows this, the column kernel need not recompute the
+ * slab's y range); 32 + 3 + 14 + 13 + 1 + 1 bits */
+PVX_HD uint64_t pack_task(int64_t t, int h, int il, int jl, bool first, bool last) {
+    return (uint64_t)t | ((uint64_t)h << 32) | ((uint64_t)il << 35) | ((uint64_t)jl << 49) |
+           ((uint64_t)(first ? 1 : 0) << 62) | ((uint64_t)(last ? 1 : 0) << 63);
 }
 PVX_HD void unpack_task(uint64_t w, int64_t& t, int& h, int& il, int& jl) {
     t = (int64_t)(w & 0xffffffffull);
     h = (int)((w >> 32) & 7u);
     il = (int)((w >> 35) & 0x3fffu);
-    jl = (int)((w >> 49) & 0x3fffu);
+    jl = (int)((w >> 49) & 0x1fffu);
 }
+PVX_HD bool task_first(uint64_t w) { return ((w >> 62) & 1u) != 0u; }
+PVX_HD bool task_last(uint64_t w) { return ((w >> 63) & 1u) != 0u; }
 
 PVX_HD bool finite_rec(const double* rec) {
     bool f = true;
@@ -1090,7 +1096,8 @@ PVX_HD bool tet_wide(const double* rec, const GridInfo& g) {
 
 /* limits of the task encoding: column offsets within a copy's index box,
  * cells along z, and tasks per tet (the largest-fragment key) */
-constexpr int TASK_MAXIJ = 1 << 14;
+constexpr int TASK_MAXI = 1 << 14;
+constexpr int TASK_MAXJ = 1 << 13;
 constexpr int TASK_MAXK = 1 << 10;
 constexpr int64_t TASK_MAXPERTET = 1 << 22;
 
@@ -1113,14 +1120,14 @@ PVX_HD int64_t tet_columns(const double* rec, int64_t t, const GridInfo& g, cons
     for (int h = 0; h < ng; ++h) {
         CopySetup<R> C;
         if (!copy_setup(rec, lo, hi, shp, h, g, mb, C)) continue;
-        if (C.ib1[0] - C.G.ib0[0] >= TASK_MAXIJ || C.ib1[1] - C.G.ib0[1] >= TASK_MAXIJ ||
+        if (C.ib1[0] - C.G.ib0[0] >= TASK_MAXI || C.ib1[1] - C.G.ib0[1] >= TASK_MAXJ ||
             C.ib1[2] - C.G.ib0[2] >= TASK_MAXK)
             return -1;
         for (int i = C.G.ib0[0]; i < C.ib1[0]; ++i) {
             int j0, j1;
             if (!slab_columns(C, i, g, j0, j1)) continue;
             for (int j = j0; j < j1; ++j) {
-                if (out) out[n] = pack_task(t, h, i - C.G.ib0[0], j - C.G.ib0[1]);
+                if (out) out[n] = pack_task(t, h, i - C.G.ib0[0], j - C.G.ib0[1], j == j0, j == j1 - 1);
                 n++;
             }
         }
@@ -1620,12 +1627,13 @@ PVX_HD bool yslab_from_table(LPoly<R>& P, int y2first, const CopyGeo<R>& G, cons
 /* Build the column polyhedron of task (copy C, cell i, j) in P: the tet
  * clipped by the column's four planes -- interior grid planes or, on the
  * copy's first/last slab or column, the window planes it crosses
- * (src/geometry.c:816-843).  False on a vertex-budget overflow. */
+ * (src/geometry.c:816-843).  ylo, yhi: the column has a slab column below
+ * and above it (interior y planes).  False on a vertex-budget overflow. */
 template <class R>
-PVX_HD bool build_column(LPoly<R>& P, const CopySetup<R>& C, const GridInfo& g, int i, int j, int j0, int j1) {
+PVX_HD bool build_column(LPoly<R>& P, const CopySetup<R>& C, const GridInfo& g, int i, int j, bool ylo, bool yhi) {
     const CopyGeo<R>& G = C.G;
     const int il = i - G.ib0[0], jl = j - G.ib0[1];
-    const bool xlo = i > G.ib0[0], xhi = i < C.ib1[0] - 1, ylo = j > j0, yhi = j < j1 - 1;
+    const bool xlo = i > G.ib0[0], xhi = i < C.ib1[0] - 1;
     /* the x-slab: from the table, or (when it does not apply) by the two
      * generic clips */
     const bool has_hi = xhi || (C.wmask & 2u), has_lo = xlo || (C.wmask & 1u);
@@ -1685,9 +1693,7 @@ PVX_HD int column_begin(const GridInfo& g, const MeshBox& mb, const double* rec,
     if (!copy_setup(rec, lo, hi, shp, h, g, mb, S.C)) return 0;
     S.i = S.C.G.ib0[0] + il;
     S.j = S.C.G.ib0[1] + jl;
-    int j0, j1;
-    if (!slab_columns(S.C, S.i, g, j0, j1)) return 0;
-    if (!build_column(P, S.C, g, S.i, S.j, j0, j1)) return -1;
+    if (!build_column(P, S.C, g, S.i, S.j, !task_first(task), !task_last(task))) return -1;
     if (P.live == 0u) return 0;
     lpoly_range(P, S.C.G, 2, S.C.G.ib0[2], S.C.G.ib0[2], S.C.ib1[2], g.n[2], S.k0, S.k1);
     return S.k1 > S.k0 ? 1 : 0;

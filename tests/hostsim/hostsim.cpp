@@ -345,7 +345,7 @@ static void ycheck_tet(const double* r, int64_t t, const GridInfo& g, const Mesh
         int j0, j1;
         if (!slab_columns(C, i, g, j0, j1)) continue;
         out[0]++;
-        const bool okA = build_column(A, C, g, i, j, j0, j1);
+        const bool okA = build_column(A, C, g, i, j, j > j0, j < j1 - 1);
         /* the generic sequence */
         const CopyGeo<R>& G = C.G;
         const bool xlo = i > G.ib0[0], xhi = i < C.ib1[0] - 1, ylo = j > j0, yhi = j < j1 - 1;
