@@ -134,45 +134,65 @@ __global__ __launch_bounds__(256) void k_conn_check(const int32_t* __restrict__ 
  * elements outside the window leave invalid records */
 /* tag: written to every record's slot 27 (the mesh's index in a batched
  * call, 0 otherwise) */
-__global__ __launch_bounds__(128) void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, int64_t ne, double* __restrict__ tets,
-                               double tag = 0.0) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ne) return;
-    const int per = (m.vpe == 4) ? 1 : 6;
-    const int64_t e = e0 + t;
-    double* out = tets + (int64_t)TREC * per * t;
+/* Expansion at refinement level 0 for simplex and trilinear elements: one
+ * lane per element, corners in registers.  The wavefront's records go out
+ * through LDS one tet index at a time (record s of its 64 elements, at a
+ * 1344-byte stride in the output for trilinear elements), so that
+ * neighbouring lanes store neighbouring doubles instead of records 1344
+ * bytes apart. */
+__global__ __launch_bounds__(64) void k_expand_fixed(MeshDev m, GridInfo g, MeshBox mb, int64_t e0, int64_t ne,
+                                                     double* __restrict__ tets, double tag = 0.0) {
+    __shared__ double st[64 * TREC];
+    const int lane = threadIdx.x;
+    const int64_t w0 = (int64_t)blockIdx.x * 64;
+    const int64_t t = w0 + lane;
+    const int nel = (int)(ne - w0 < 64 ? ne - w0 : 64);
+    const bool simplex = m.vpe == 4;
+    const int per = simplex ? 1 : 6;
     /* simplex and linear elements: corners only, all in registers */
     double P[8][3], U[8][3];
-    const int nv = m.vpe == 4 ? 4 : 8;
-    for (int v = 0; v < 8; ++v) {
-        const int64_t id = m.conn[e * m.vpe + (v < nv ? v : 0)];
-        for (int a = 0; a < 3; ++a) {
-            P[v][a] = m.pos[3 * id + a];
-            U[v][a] = m.vel ? m.vel[3 * id + a] : 0.0;
-        }
-    }
-    if (!(m.vpe == 4 ? wrap_and_test(P, 4, g, mb) : wrap_and_test(P, 8, g, mb))) {
-        for (int s = 0; s < per; ++s) {
-            out[TREC * s + 25] = 0.0;
-            out[TREC * s + 27] = tag;
-        }
-        return;
-    }
-    const double mass = m.mass[e];
-    if (m.vpe == 4) {
-        for (int v = 0; v < 4; ++v)
+    bool ok = false;
+    double mass = 0.0;
+    if (t < ne) {
+        const int64_t e = e0 + t;
+        const int nv = simplex ? 4 : 8;
+        for (int v = 0; v < 8; ++v) {
+            const int64_t id = m.conn[e * m.vpe + (v < nv ? v : 0)];
             for (int a = 0; a < 3; ++a) {
-                out[3 * v + a] = P[v][a];
-                out[12 + 3 * v + a] = U[v][a];
+                P[v][a] = m.pos[3 * id + a];
+                U[v][a] = m.vel ? m.vel[3 * id + a] : 0.0;
             }
-        out[24] = mass;
-        out[25] = 1.0;
-        out[26] = 1.0;
-        out[27] = tag;
-        return;
+        }
+        ok = simplex ? wrap_and_test(P, 4, g, mb) : wrap_and_test(P, 8, g, mb);
+        mass = m.mass[e];
     }
-    emit6_corners(out, (const double (*)[3])P, (const double (*)[3])U, mass);
-    for (int s = 0; s < 6; ++s) out[TREC * s + 27] = tag;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        if (s >= per) break;
+        /* this lane's record s (the 6-tet table on the corners, or the tet) */
+        double* r = st + TREC * lane;
+        if (t < ne) {
+            for (int v = 0; v < 4; ++v) {
+                const int l = tet27(s, v), c = simplex ? v : 4 * ((l / 9) / 2) + 2 * (((l / 3) % 3) / 2) + (l % 3) / 2;
+                for (int a = 0; a < 3; ++a) {
+                    r[3 * v + a] = P[c][a];
+                    r[12 + 3 * v + a] = U[c][a];
+                }
+            }
+            r[24] = simplex ? mass : mass * (1.0 / 6);
+            r[25] = ok ? 1.0 : 0.0;
+            r[26] = 1.0;
+            r[27] = tag;
+        }
+        __syncthreads();
+        /* the wavefront stores the records, neighbouring lanes neighbouring
+         * doubles (each record's 28 contiguous) */
+        for (int q = lane; q < nel * TREC; q += 64) {
+            const int l = q / TREC, wq = q - TREC * l;
+            tets[(int64_t)TREC * (per * (w0 + l) + s) + wq] = st[q];
+        }
+        __syncthreads(); /* the stage is read before the next record is written */
+    }
 }
 
 /* the same for triquadratic elements: the 27 nodes are the lattice */
@@ -208,7 +228,7 @@ static void launch_expand_fixed(const MeshDev& md, const GridInfo& g, const Mesh
     if (md.vpe == 27)
         k_expand_fixed27<<<nb, 128, 0, s>>>(md, g, mb, e0, ne, tets, tag);
     else
-        k_expand_fixed<<<nb, 128, 0, s>>>(md, g, mb, e0, ne, tets, tag);
+        k_expand_fixed<<<(unsigned)((ne + 63) / 64), 64, 0, s>>>(md, g, mb, e0, ne, tets, tag);
 }
 
 /* refine_max_lvl > 0: count pass and emit pass of the per-element DFS */
