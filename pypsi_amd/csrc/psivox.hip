@@ -333,9 +333,14 @@ __device__ void block_count(int64_t a, int64_t b, unsigned long long* out) {
             ta += sa[w];
             tb += sb[w];
         }
-        atomicAdd(out, ta);
-        atomicAdd(out + 1, tb);
+        /* every block adds to the same two words, so the adds serialize:
+         * leave out the zeros (most blocks have no wide tet, and the count
+         * pass has no fragments) */
+        if (ta) atomicAdd(out, ta);
+        if (tb) atomicAdd(out + 1, tb);
     }
+    /* thread 0 has read sa, sb before a following call writes them again */
+    __syncthreads();
 }
 
 /* one thread per tet record, FP64 throughout.  The polyhedra live in
