@@ -502,19 +502,30 @@ __device__ __forceinline__ void with_mesh(const double* r, const GridInfo& g0, c
     }
 }
 
-/* per-mesh counters of a batched call: fragments, copies, wide tets */
+/* per-mesh counters of a batched call: fragments, copies, wide tets; each
+ * in MSTRIPE stripes one 128-byte line apart (stripe: the workgroup's
+ * index), summed on the host: the wavefronts of one mesh would otherwise
+ * all add to one word, and such adds serialize */
 constexpr int MCNT = 4;
+constexpr int MSTRIPE = 16;
+constexpr int MLINE = 16; /* words per 128-byte line */
+
+PVX_HD int64_t mcnt_words(int nm) { return (int64_t)MCNT * nm * MSTRIPE * MLINE; }
+PVX_HD int64_t mcnt_word(int id, int slot, int stripe) {
+    return (((int64_t)MCNT * id + slot) * MSTRIPE + stripe) * MLINE;
+}
 
 /* per-mesh counter add, aggregated over the wave when its lanes share a
  * mesh; every lane of the wave calls it (id < 0: nothing) */
 __device__ void mesh_count(unsigned long long* mcnt, int id, int64_t v, int slot) {
     const int id0 = __builtin_amdgcn_readfirstlane(id);
+    const int stripe = (int)(blockIdx.x & (MSTRIPE - 1));
     if (__all(id == id0)) {
         unsigned long long sum = (unsigned long long)v;
         for (int o = warpSize / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-        if ((threadIdx.x & (warpSize - 1)) == 0 && id0 >= 0 && sum) atomicAdd(mcnt + MCNT * id0 + slot, sum);
+        if ((threadIdx.x & (warpSize - 1)) == 0 && id0 >= 0 && sum) atomicAdd(mcnt + mcnt_word(id0, slot, stripe), sum);
     } else if (id >= 0 && v) {
-        atomicAdd(mcnt + MCNT * id + slot, (unsigned long long)v);
+        atomicAdd(mcnt + mcnt_word(id, slot, stripe), (unsigned long long)v);
     }
 }
 
@@ -642,7 +653,7 @@ template <bool BATCH>
 __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
                                                    int64_t ntask, GridInfo g0, MeshBox mb0, const BatchGeo* table,
                                                    uint32_t* __restrict__ perm, uint64_t* __restrict__ sorted,
-                                                   unsigned* __restrict__ next) {
+                                                   uint16_t* __restrict__ mid, unsigned* __restrict__ next) {
     __shared__ int cnt[ORDK];
     __shared__ uint8_t skey[ORDW]; /* each task's class, kept here between the two passes */
     const int64_t w0 = (int64_t)blockIdx.x * ORDW;
@@ -689,8 +700,12 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
     __syncthreads();
     for (int e = tid; e < nw; e += 256) {
         const int pos = atomicAdd(&cnt[skey[e]], 1);
+        const uint64_t w = tasks[w0 + e];
         perm[w0 + pos] = (uint32_t)(w0 + e);
-        sorted[w0 + pos] = tasks[w0 + e];
+        sorted[w0 + pos] = w;
+        /* batched: the task's mesh, in processing order (the column kernel
+         * then need not read it from the scattered tet record first) */
+        if (BATCH) mid[w0 + pos] = (uint16_t)tets[(int64_t)TREC * (int64_t)(w & 0xffffffffull) + 27];
     }
 }
 
@@ -839,7 +854,8 @@ __device__ __forceinline__ int64_t wave_columns(bool act, int64_t p, uint64_t w,
 template <int ORDER, int FMASK, class R, bool BATCH>
 __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
                                                         const uint64_t* __restrict__ sorted,
-                                                        const uint32_t* __restrict__ perm, unsigned* __restrict__ next,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const uint16_t* __restrict__ mid, unsigned* __restrict__ next,
                                                         int64_t ntask, GridInfo g0, MeshBox mb0,
                                                         double* __restrict__ acc0,
                                                         const BatchGeo* __restrict__ table, TaskOut out,
@@ -873,7 +889,7 @@ __global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict
             p = perm[c];
             w = sorted[c];
             r = tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull);
-            my = BATCH ? (int)r[27] : 0;
+            my = BATCH ? (int)mid[c] : 0;
         }
         if (!BATCH) {
             nfrag += wave_columns<ORDER, FMASK, R>(my >= 0, p, w, r, g0, mb0, acc0, P, stage, out, status);
@@ -1049,8 +1065,8 @@ struct PassGeo {
     unsigned long long* mcnt;
 };
 
-typedef void (*KColsFn)(const double*, const uint64_t*, const uint32_t*, unsigned*, int64_t, GridInfo, MeshBox,
-                        double*, const BatchGeo*, TaskOut, unsigned*, unsigned long long*, unsigned long long*);
+typedef void (*KColsFn)(const double*, const uint64_t*, const uint32_t*, const uint16_t*, unsigned*, int64_t, GridInfo,
+                        MeshBox, double*, const BatchGeo*, TaskOut, unsigned*, unsigned long long*, unsigned long long*);
 
 /* workgroups of a persistent one-wavefront kernel (the column kernel, the
  * adjoint's velocity tasks) the current device holds at once, with sb
@@ -1093,12 +1109,15 @@ static KColsFn cols_kernel(const GridInfo& g, bool fp64, bool batch) {
 
 /* The ordering scratch of ntask tasks: perm (4 bytes a task) and the
  * column kernel's counter, then the sorted task words (8 bytes a task). */
-static size_t order_bytes(int64_t ntask) { return (((size_t)ntask * 4 + 256 + 255) & ~(size_t)255) + (size_t)ntask * 8; }
+static size_t order_bytes(int64_t ntask) {
+    return (((size_t)ntask * 4 + 256 + 255) & ~(size_t)255) + (size_t)ntask * 8 + (size_t)ntask * 2;
+}
 static uint32_t* order_perm(void* b) { return (uint32_t*)b; }
 static unsigned* perm_counter(void* b, int64_t nsub) { return (unsigned*)b + nsub; }
 static uint64_t* order_sorted(void* b, int64_t nsub) {
     return (uint64_t*)((char*)b + (((size_t)nsub * 4 + 256 + 255) & ~(size_t)255));
 }
+static uint16_t* order_mid(void* b, int64_t nsub) { return (uint16_t*)(order_sorted(b, nsub) + nsub); }
 
 /* the column kernel over tasks [0, ntask) of the pass geometry G (one mesh,
  * or a batch's table), FP64 for wide tets (ord: order_bytes(ntask) of
@@ -1109,20 +1128,21 @@ static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask,
                        hipStream_t s, hipEvent_t mark = nullptr) {
     uint32_t* perm = order_perm(ord);
     uint64_t* sorted = order_sorted(ord, ntask);
+    uint16_t* mid = order_mid(ord, ntask);
     unsigned* next = perm_counter(ord, ntask);
     const unsigned nord = (unsigned)((ntask + ORDW - 1) / ORDW);
     if (G.table)
-        k_col_order<true><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, G.table, perm, sorted, next);
+        k_col_order<true><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, G.table, perm, sorted, mid, next);
     else
-        k_col_order<false><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, nullptr, perm, sorted, next);
+        k_col_order<false><<<nord, 256, 0, s>>>(tets, tasks, ntask, G.g, G.mb, nullptr, perm, sorted, mid, next);
     if (mark) (void)hipEventRecord(mark, s);
     const int64_t nwave = (ntask + FBLOCK - 1) / FBLOCK;
     const size_t sb = stage_bytes(G.g, SFRAG);
     const KColsFn kern = cols_kernel(G.g, fp64, G.table != nullptr);
     const int64_t hold = resident_blocks(kern, sb);
     const unsigned nb = (unsigned)(nwave < hold ? nwave : hold);
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(FBLOCK), sb, s, tets, sorted, perm, next, ntask, G.g, G.mb, G.acc, G.table,
-                       out, status, counters, G.mcnt);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(FBLOCK), sb, s, tets, sorted, perm, mid, next, ntask, G.g, G.mb, G.acc,
+                       G.table, out, status, counters, G.mcnt);
     return (int)hipGetLastError();
 }
 
@@ -2475,7 +2495,7 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
                                      uint64_t* sorted = order_sorted(W.perm.p, nsub);
                                      unsigned* next = perm_counter(W.perm.p, nsub);
                                      k_col_order<false><<<(unsigned)((nsub + ORDW - 1) / ORDW), 256, 0, s>>>(
-                                         tets, tasks, nsub, g, mb, nullptr, order_perm(W.perm.p), sorted, next);
+                                         tets, tasks, nsub, g, mb, nullptr, order_perm(W.perm.p), sorted, nullptr, next);
                                      const int64_t hold = resident_blocks(k_adj_vel, 0);
                                      const int64_t nwave = (nsub + FBLOCK - 1) / FBLOCK;
                                      k_adj_vel<<<(unsigned)(nwave < hold ? nwave : hold), FBLOCK, 0, s>>>(
@@ -2551,7 +2571,9 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
         /* the group [i0, i1) */
         int i1 = i0;
         int64_t nt = 0;
-        while (i1 < nmesh && (i1 == i0 || nt + meshes[i1].nelem * per_of(i1) <= max_tets)) {
+        /* at most 65535 meshes a group: the ordering hands the column kernel
+         * each task's mesh as 16 bits */
+        while (i1 < nmesh && i1 - i0 < 65535 && (i1 == i0 || nt + meshes[i1].nelem * per_of(i1) <= max_tets)) {
             nt += meshes[i1].nelem * per_of(i1);
             i1++;
         }
@@ -2585,13 +2607,13 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
         if ((rc = W.bfield.get((size_t)fld_off[nm] * sizeof(double), dev))) return rc;
         if ((rc = W.bacc.get((size_t)acc_off[nm] * sizeof(double), dev))) return rc;
         if ((rc = W.tets.get((size_t)nt * TREC * sizeof(double), dev))) return rc;
-        if ((rc = W.bmcnt.get((size_t)(MCNT * nm + 8) * sizeof(unsigned long long), dev))) return rc;
+        if ((rc = W.bmcnt.get((size_t)(mcnt_words(nm) + 8) * sizeof(unsigned long long), dev))) return rc;
         if ((rc = W.counters.get(64, dev))) return rc;
         unsigned long long* counters = (unsigned long long*)W.counters.p;
         unsigned* status = (unsigned*)(counters + 4);
         unsigned long long* mcnt = (unsigned long long*)W.bmcnt.p;
         HIPCHK(hipMemsetAsync(counters, 0, 64, s));
-        HIPCHK(hipMemsetAsync(mcnt, 0, (size_t)(MCNT * nm) * sizeof(unsigned long long), s));
+        HIPCHK(hipMemsetAsync(mcnt, 0, (size_t)mcnt_words(nm) * sizeof(unsigned long long), s));
         double* tets = (double*)W.tets.p;
         double* bacc = (double*)W.bacc.p;
         std::vector<FieldPtrs> fp(nm);
@@ -2706,7 +2728,7 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
             for (int i = i0; i < i1; ++i)
                 if ((rc = one(i))) return rc;
         } else {
-            std::vector<unsigned long long> hm(MCNT * nm);
+            std::vector<unsigned long long> hm((size_t)mcnt_words(nm));
             for (int q = 0; q < nm; ++q) {
                 const int i = i0 + q;
                 const GridInfo& g = G[i];
@@ -2719,16 +2741,20 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                                        s)))
                         return rc;
             }
-            HIPCHK(hipMemcpyAsync(hm.data(), mcnt, MCNT * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hm.data(), mcnt, (size_t)mcnt_words(nm) * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (stats) {
                 const double wall =
                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
                 for (int q = 0; q < nm; ++q) {
                     psivox_stats& st = stats[i0 + q];
-                    st.nfrag = (int64_t)hm[MCNT * q];
-                    st.nghosts = (int64_t)hm[MCNT * q + 1];
-                    st.nwide = (int64_t)hm[MCNT * q + 2];
+                    int64_t sum[3] = {0, 0, 0};
+                    for (int slot = 0; slot < 3; ++slot)
+                        for (int k = 0; k < MSTRIPE; ++k) sum[slot] += (int64_t)hm[mcnt_word(q, slot, k)];
+                    st.nfrag = sum[0];
+                    st.nghosts = sum[1];
+                    st.nwide = sum[2];
                     st.ntets = T[q + 1] - T[q];
                     st.kernel_ms = kms;
                     st.total_ms = wall;
