@@ -769,6 +769,11 @@ __device__ __forceinline__ int64_t wave_columns(bool act, int64_t p, uint64_t w,
     int nmax = ncell;
     for (int o = FBLOCK / 2; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
     PlaneCarry<R> pc{false, false, {R(0), R(0), R(0)}};
+    /* a column of two cells takes both in its first step (one pass over
+     * its fan triangles); its second step deposits the upper cell, Mu */
+    const bool pair = ncell == 2 && two_cell_column(S);
+    Mom<R> Mu;
+    R O[3];
     /* cell steps in lockstep: every lane takes part in each step's
      * deposit, which issues the wavefront's fragments slot by slot so
      * that neighbouring lanes add into the same cell's consecutive
@@ -776,15 +781,21 @@ __device__ __forceinline__ int64_t wave_columns(bool act, int64_t p, uint64_t w,
      * request; one lane per cell would give one request per lane) */
     for (int it = 0; it < nmax; ++it) {
         Mom<R> Mm;
-        R O[3];
         int k = 0;
         bool has = false;
         if (it < ncell) {
             k = S.k0 + it;
-            bool hl, hh;
-            double lc, hc;
-            cell_planes(S, k, hl, lc, hh, hc);
-            slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O, &pc);
+            if (pair) {
+                if (it == 0)
+                    slab_moments_2<ORDER>(P, S.C.G, 2, first_plane(S), Mm, Mu, O);
+                else
+                    Mm = Mu;
+            } else {
+                bool hl, hh;
+                double lc, hc;
+                cell_planes(S, k, hl, lc, hh, hc);
+                slab_moments_u<ORDER>(P, S.C.G, 2, hl, lc, hh, hc, Mm, O, &pc);
+            }
             has = Mm.m0 > R(0);
         }
         const uint64_t mask = __ballot(has);
@@ -852,7 +863,7 @@ __device__ __forceinline__ int64_t wave_columns(bool act, int64_t p, uint64_t w,
  * mesh's table entry in scalar registers.
  */
 template <int ORDER, int FMASK, class R, bool BATCH>
-__global__ __launch_bounds__(FBLOCK, 2) void k_vox_cols(const double* __restrict__ tets,
+__global__ __launch_bounds__(FBLOCK, ORDER == 0 && sizeof(R) == 4 ? 3 : 2) void k_vox_cols(const double* __restrict__ tets,
                                                         const uint64_t* __restrict__ sorted,
                                                         const uint32_t* __restrict__ perm,
                                                         const uint16_t* __restrict__ mid, unsigned* __restrict__ next,
@@ -1108,7 +1119,8 @@ static KColsFn cols_kernel(const GridInfo& g, bool fp64, bool batch) {
 }
 
 /* The ordering scratch of ntask tasks: perm (4 bytes a task) and the
- * column kernel's counter, then the sorted task words (8 bytes a task). */
+ * column kernel's counter, then the sorted task words (8 bytes a task) and
+ * their mesh ids (2 bytes a task, batched calls). */
 static size_t order_bytes(int64_t ntask) {
     return (((size_t)ntask * 4 + 256 + 255) & ~(size_t)255) + (size_t)ntask * 8 + (size_t)ntask * 2;
 }
@@ -2880,8 +2892,10 @@ int run_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* o
         (rc = W.tetb.get((size_t)nt * sizeof(int64_t), dev)) || (rc = W.scan_tmp.get(tb, dev)))
         return rc;
     /* a chunk of column tasks: task words, mass sums, largest-fragment
-     * keys and moments, processing order */
-    const size_t per_task = 4 * sizeof(uint64_t) + sizeof(uint32_t) + (g.order > 0 ? 16 * sizeof(float) : 0);
+     * keys and moments, and the ordering's scratch (order_bytes: the
+     * processing order, the task words in that order, the mesh ids) */
+    const size_t per_task = 4 * sizeof(uint64_t) + sizeof(uint32_t) + sizeof(uint16_t) +
+                            (g.order > 0 ? 16 * sizeof(float) : 0);
     const size_t left = A.size - A.off;
     const int64_t cap = left > 8192 ? (int64_t)((left - 8192) / per_task) : 0;
     if (cap < 4096)

@@ -507,6 +507,7 @@ struct SlabCell {
     R O[3], gh[3];
     R zd[3]; /* x_ax(q) - x_ax(q = 0) = zd . q */
     bool has_lo, has_hi;
+    uint32_t le; /* with one plane: the vertices at or below it (by slot) */
     uint64_t mask;   /* bit 3r + k: the fan triangle of directed edge a -> k, a the vertex of rank r,
                         reaching into the slab and cut by one of its planes */
     uint64_t inside; /* the same for fan triangles wholly inside the slab */
@@ -608,16 +609,19 @@ PVX_HD void slab_setup(const LPoly<R>& P, const CopyGeo<R>& G, int ax, bool has_
     /* the lower plane's: carried, computed, or none */
     uint32_t above;
     uint64_t alo, xlo;
+    S.le = below;
     if (carried) {
         above = P.live & ~pc->below;
         xlo = pc->xm;
         alo = all & ~(pc->bm | pc->xm);
+        if (!has_hi) S.le = pc->below;
     } else if (has_lo) {
         const uint32_t le = side_le(P, zd0, zd1, zd2, S.lo);
         uint64_t a2, blo;
         tri_sides(P, le, a2, blo, xlo);
         above = P.live & ~le;
         alo = all & ~(blo | xlo);
+        if (!has_hi) S.le = le;
     } else {
         above = P.live;
         xlo = 0ull;
@@ -722,67 +726,69 @@ PVX_HD void pair_terms(const LPoly<R>& P, int e, const SlabCell<R>& S, Mom<R>& M
 }
 
 /*
+ * The crossing of a fan triangle T = (c, a, b) of P (directed edge e) with
+ * the plane z = p, whose side mask le (vertices at or below p, by slot) T
+ * crosses: the vertex s alone on its side, r1 and r2 after it in traversal
+ * order (c -> a -> b), all relative to O, and the points P1 on s -> r1 and
+ * P2 on s -> r2 where the plane cuts T, both measured from s (so the cells
+ * on the two sides of p get the same points).  Returns whether s lies at or
+ * below p.
+ */
+template <class R>
+PVX_HD bool tri_cross(const LPoly<R>& P, int e, uint32_t le, const R* O, R zd0, R zd1, R zd2, R p, R* qs, R* q1,
+                      R* q2, R* P1, R* P2) {
+    const int ra = e / 3, k = e - 3 * ra, a = P.at(ra);
+    const uint32_t wa = P.nbw(a);
+    const int b = nbf(wa, k), c = apf(wa, k);
+    const bool lc = (le >> c) & 1u, la = (le >> a) & 1u, lb = (le >> b) & 1u;
+    const int s = lc == la ? b : (lc == lb ? a : c);
+    const int r1 = lc == la ? c : (lc == lb ? b : a);
+    const int r2 = lc == la ? a : (lc == lb ? c : b);
+    const R s0 = P.qa(s, 0), s1 = P.qa(s, 1), s2 = P.qa(s, 2);
+    const R u0 = P.qa(r1, 0), u1 = P.qa(r1, 1), u2 = P.qa(r1, 2);
+    const R v0 = P.qa(r2, 0), v1 = P.qa(r2, 1), v2 = P.qa(r2, 2);
+    const R zs = zd0 * s0 + zd1 * s1 + zd2 * s2;
+    const R zu = zd0 * u0 + zd1 * u1 + zd2 * u2;
+    const R zv = zd0 * v0 + zd1 * v1 + zd2 * v2;
+    qs[0] = s0 - O[0], qs[1] = s1 - O[1], qs[2] = s2 - O[2];
+    q1[0] = u0 - O[0], q1[1] = u1 - O[1], q1[2] = u2 - O[2];
+    q2[0] = v0 - O[0], q2[1] = v1 - O[1], q2[2] = v2 - O[2];
+    /* the sides come from le; a z that rounds onto s's own z (a vertex on
+     * p, classified apart from s elsewhere) gives that vertex's own point */
+    const R t1 = zu != zs ? (p - zs) * frcp(zu - zs) : R(0), t2 = zv != zs ? (p - zs) * frcp(zv - zs) : R(0);
+    for (int i = 0; i < 3; ++i) {
+        P1[i] = qs[i] + t1 * (q1[i] - qs[i]);
+        P2[i] = qs[i] + t2 * (q2[i] - qs[i]);
+    }
+    return (le >> s) & 1u;
+}
+
+/*
  * pair_terms for a slab with one plane p (the first or last cell of a
- * column): S.O lies on p, so the kept piece K of T = (c, a, b) has no cap
- * term, and fanned from its exit point X (on p) every boundary piece
- * through X adds nothing either.  With v1 the first kept vertex after the
- * entry point N (on p) and w1 the next vertex: K = (N, v1, X) or, when w1
- * is kept too, (N, v1, w1, X): the cones over (X, v1, N) and (X, w1, v1),
- * the second weighted 0 or 1.  Two fixed terms; the same integrals as
- * pair_terms (the reference's fan moments, src/geometry.c:692-767).
+ * column): S.O lies on p, so the kept piece K of T has no cap term, and
+ * fanned from a point of K on p every boundary piece through that point
+ * adds nothing either.  With s, r1, r2, P1, P2 as in tri_cross: when s is
+ * kept, K is the triangle (P2, s, P1) (traversal order) and its cone is the
+ * term (P1, s, P2); otherwise K is the quadrilateral (P1, r1, r2, P2),
+ * fanned from P2: (P2, r2, r1) and (P2, r1, P1).  Two fixed terms, the second
+ * weighted 0 or 1; the same integrals as pair_terms (the reference's fan
+ * moments, src/geometry.c:692-767).
  */
 template <int ORDER, class R>
 PVX_HD void pair_terms_1(const LPoly<R>& P, int e, const SlabCell<R>& S, Mom<R>& M) {
-    const int ra = e / 3, k = e - 3 * ra, a = P.at(ra);
-    const R* O = S.O;
-    const R zd0 = S.zd[0], zd1 = S.zd[1], zd2 = S.zd[2];
-    const uint32_t wa = P.nbw(a);
-    const int b = nbf(wa, k), c = apf(wa, k);
-    /* the triangle in traversal order c -> a -> b, relative to O */
-    R q[3][3], z[3];
-    const int vs[3] = {c, a, b};
-    for (int s = 0; s < 3; ++s) {
-        const int v = s == 0 ? vs[0] : (s == 1 ? vs[1] : vs[2]);
-        const R p0 = P.qa(v, 0), p1 = P.qa(v, 1), p2 = P.qa(v, 2);
-        z[s] = zd0 * p0 + zd1 * p1 + zd2 * p2;
-        q[s][0] = p0 - O[0];
-        q[s][1] = p1 - O[1];
-        q[s][2] = p2 - O[2];
-    }
-    /* kept side: z <= hi (the slab's plane is its upper one) or z > lo */
-    const bool upper = S.has_hi;
+    const bool upper = S.has_hi; /* kept side: z <= p, or z > p */
     const R p = upper ? S.hi : S.lo;
-    bool kp[3];
-    for (int s = 0; s < 3; ++s) kp[s] = upper ? (z[s] <= p) : (z[s] > p);
-    /* v1: the kept vertex whose predecessor is not kept; w1, w2 follow it */
-    const int i1 = (kp[0] && !kp[2]) ? 0 : ((kp[1] && !kp[0]) ? 1 : 2);
-    const int i2 = i1 == 2 ? 0 : i1 + 1, i3 = i2 == 2 ? 0 : i2 + 1;
-    const bool two = i2 == 0 ? kp[0] : (i2 == 1 ? kp[1] : kp[2]);
-    R V1[3], W1[3], W2[3];
-    R zv1, zw1, zw2;
-#define PVX_PICK3(dst, zdst, idx)                                                              \
-    do {                                                                                       \
-        for (int t = 0; t < 3; ++t) dst[t] = (idx) == 0 ? q[0][t] : ((idx) == 1 ? q[1][t] : q[2][t]); \
-        zdst = (idx) == 0 ? z[0] : ((idx) == 1 ? z[1] : z[2]);                                 \
-    } while (0)
-    PVX_PICK3(V1, zv1, i1);
-    PVX_PICK3(W1, zw1, i2);
-    PVX_PICK3(W2, zw2, i3);
-#undef PVX_PICK3
-    /* N on the entry edge w2 -> v1; X on the exit edge (two ? w1 -> w2 : v1 -> w1) */
-    R N[3], X[3];
-    const R tn = (p - zw2) * frcp(zv1 - zw2);
-    R U[3], D[3];
-    for (int t = 0; t < 3; ++t) {
-        N[t] = W2[t] + tn * (V1[t] - W2[t]);
-        U[t] = two ? W1[t] : V1[t];
-        D[t] = two ? W2[t] : W1[t];
+    R qs[3], q1[3], q2[3], P1[3], P2[3];
+    const bool sb = tri_cross(P, e, S.le, S.O, S.zd[0], S.zd[1], S.zd[2], p, qs, q1, q2, P1, P2);
+    const bool skept = sb == upper;
+    R A[3], B[3], C[3];
+    for (int i = 0; i < 3; ++i) {
+        A[i] = skept ? P1[i] : P2[i];
+        B[i] = skept ? qs[i] : q2[i];
+        C[i] = skept ? P2[i] : q1[i];
     }
-    const R zu = two ? zw1 : zv1, zdn = two ? zw2 : zw1;
-    const R tx = (p - zu) * frcp(zdn - zu);
-    for (int t = 0; t < 3; ++t) X[t] = U[t] + tx * (D[t] - U[t]);
-    tri_acc_w<ORDER>(M, X, V1, N, R(1));
-    tri_acc_w<ORDER>(M, X, W1, V1, two ? R(1) : R(0));
+    tri_acc<ORDER>(M, A, B, C);
+    tri_acc_w<ORDER>(M, P2, q1, P1, skept ? R(0) : R(1));
 }
 
 /* The moments (about S.O) of a fan triangle T = (c, a, b) wholly inside the
@@ -830,6 +836,102 @@ PVX_HD void slab_moments_u(const LPoly<R>& P, const CopyGeo<R>& G, int ax, bool 
             mask &= mask - 1;
             pair_terms_1<ORDER>(P, e, S, M);
         }
+    }
+}
+
+/* tri_acc of the cone over (A, B, C) into Ml with weight wl and into Mu
+ * with weight 1 - wl (wl 0 or 1): the integrals once, two accumulations */
+template <int ORDER, class R>
+PVX_HD void tri_acc2(Mom<R>& Ml, Mom<R>& Mu, const R* A, const R* B, const R* C, bool lower) {
+    const R d = A[0] * (B[1] * C[2] - B[2] * C[1]) - A[1] * (B[0] * C[2] - B[2] * C[0]) +
+                A[2] * (B[0] * C[1] - B[1] * C[0]);
+    const R dl = lower ? d : R(0), du = lower ? R(0) : d;
+    Ml.m0 += dl;
+    Mu.m0 += du;
+    if (ORDER > 0) {
+        const R S0 = A[0] + B[0] + C[0], S1 = A[1] + B[1] + C[1], S2 = A[2] + B[2] + C[2];
+        const R s1[3] = {S0, S1, S2};
+        for (int i = 0; i < 3; ++i) {
+            Ml.s1[i] += dl * s1[i];
+            Mu.s1[i] += du * s1[i];
+        }
+        if (ORDER > 1) {
+            const R t[6] = {A[0] * A[0] + B[0] * B[0] + C[0] * C[0] + S0 * S0,
+                            A[0] * A[1] + B[0] * B[1] + C[0] * C[1] + S0 * S1,
+                            A[0] * A[2] + B[0] * B[2] + C[0] * C[2] + S0 * S2,
+                            A[1] * A[1] + B[1] * B[1] + C[1] * C[1] + S1 * S1,
+                            A[1] * A[2] + B[1] * B[2] + C[1] * C[2] + S1 * S2,
+                            A[2] * A[2] + B[2] * B[2] + C[2] * C[2] + S2 * S2};
+            for (int i = 0; i < 6; ++i) {
+                Ml.s2[i] += dl * t[i];
+                Mu.s2[i] += du * t[i];
+            }
+        }
+    }
+}
+
+/*
+ * The two pieces of a fan triangle T of P (directed edge e) crossing the
+ * plane z = p (side mask le), for the cell below p (Ml) and the cell above
+ * (Mu), about O on p: with s, r1, r2, P1, P2 as in tri_cross, the piece on
+ * s's side is the triangle (P2, s, P1), the other the quadrilateral (P1, r1,
+ * r2, P2): the same three cone terms pair_terms_1 takes for the two cells,
+ * from the same points, where two single-plane passes take four.
+ */
+template <int ORDER, class R>
+PVX_HD void pair_terms_2(const LPoly<R>& P, int e, uint32_t le, const R* O, R zd0, R zd1, R zd2, R p, Mom<R>& Ml,
+                         Mom<R>& Mu) {
+    R qs[3], q1[3], q2[3], P1[3], P2[3];
+    const bool sb = tri_cross(P, e, le, O, zd0, zd1, zd2, p, qs, q1, q2, P1, P2);
+    tri_acc2<ORDER>(Ml, Mu, P1, qs, P2, sb);
+    tri_acc2<ORDER>(Ml, Mu, P2, q2, q1, !sb);
+    tri_acc2<ORDER>(Ml, Mu, P2, q1, P1, !sb);
+}
+
+/*
+ * The moments of both cells of a column with exactly two cells and one
+ * plane between them (hi_c along ax; no window plane in either cell), in
+ * one pass: the cells' single-plane passes (slab_moments_u) set up the same
+ * plane twice, and each crossing triangle is cut once here instead of once
+ * per cell.  Ml, Mu: the lower and upper cell, both about O (on the plane,
+ * the point the lower cell's pass takes).
+ */
+template <int ORDER, class R>
+PVX_HD void slab_moments_2(const LPoly<R>& P, const CopyGeo<R>& G, int ax, double hi_c, Mom<R>& Ml, Mom<R>& Mu,
+                           R* O) {
+    Ml.zero();
+    Mu.zero();
+    const double x0d = pick3(G.x0d, ax);
+    const R p = (R)(hi_c - x0d);
+    const R zd0 = ax == 0 ? G.DX[0][0] : (ax == 1 ? G.DX[1][0] : G.DX[2][0]),
+            zd1 = ax == 0 ? G.DX[0][1] : (ax == 1 ? G.DX[1][1] : G.DX[2][1]),
+            zd2 = ax == 0 ? G.DX[0][2] : (ax == 1 ? G.DX[1][2] : G.DX[2][2]);
+    const uint32_t below = side_le(P, zd0, zd1, zd2, p);
+    uint64_t all, bm, xm;
+    tri_sides(P, below, all, bm, xm);
+    const int v0 = P.at(0);
+    O[0] = P.qa(v0, 0);
+    O[1] = P.qa(v0, 1);
+    O[2] = P.qa(v0, 2);
+    plane_point_le(P, zd0, zd1, zd2, p, below, O);
+    /* triangles wholly on one side: the cone over each, to its side's cell */
+    uint64_t in = all & ~xm;
+    while (in) {
+        const int e = __builtin_ctzll(in);
+        in &= in - 1;
+        const int ra = e / 3, k = e - 3 * ra, a = P.at(ra);
+        const uint32_t wa = P.nbw(a);
+        const int b = nbf(wa, k), c = apf(wa, k);
+        const R qa[3] = {P.qa(a, 0) - O[0], P.qa(a, 1) - O[1], P.qa(a, 2) - O[2]};
+        const R qb[3] = {P.qa(b, 0) - O[0], P.qa(b, 1) - O[1], P.qa(b, 2) - O[2]};
+        const R qc[3] = {P.qa(c, 0) - O[0], P.qa(c, 1) - O[1], P.qa(c, 2) - O[2]};
+        tri_acc2<ORDER>(Ml, Mu, qa, qc, qb, ((bm >> e) & 1ull) != 0ull);
+    }
+    uint64_t mask = xm;
+    while (mask) {
+        const int e = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        pair_terms_2<ORDER>(P, e, below, O, zd0, zd1, zd2, p, Ml, Mu);
     }
 }
 
@@ -1107,8 +1209,8 @@ constexpr int64_t TASK_MAXPERTET = 1 << 22;
  * caller then runs the FP64 kernel.  ncopies gets the number of copies.
  * R: the precision of the column kernel that will take the tasks. */
 template <class R>
-PVX_HD int64_t tet_columns(const double* rec, int64_t t, const GridInfo& g, const MeshBox& mb, uint64_t* out,
-                           int* ncopies) {
+PVX_HDI int64_t tet_columns(const double* rec, int64_t t, const GridInfo& g, const MeshBox& mb, uint64_t* out,
+                            int* ncopies) {
     if (ncopies) *ncopies = 0;
     if (!finite_rec(rec)) return -1;
     double lo[3], hi[3];
@@ -1678,6 +1780,22 @@ PVX_HD void cell_planes(const ColumnState<R>& S, int k, bool& has_lo, double& lo
     hi_c = k < S.k1 - 1 ? (double)(kl + 1) : C.whi[2];
 }
 
+/* a column of exactly two cells whose one plane is the one between them
+ * (no window plane in either): slab_moments_2 takes both cells at once */
+template <class R>
+PVX_HD bool two_cell_column(const ColumnState<R>& S) {
+    const CopySetup<R>& C = S.C;
+    const bool wlo = S.k0 == C.G.ib0[2] && (C.wmask & 16u);
+    const bool whi = S.k1 == C.ib1[2] && (C.wmask & 32u);
+    return S.k1 - S.k0 == 2 && !wlo && !whi;
+}
+
+/* the plane between a column's first two cells, in the copy's local units */
+template <class R>
+PVX_HD double first_plane(const ColumnState<R>& S) {
+    return (double)(S.k0 - S.C.G.ib0[2] + 1);
+}
+
 /* Build the column of a task: 1 with cells [k0, k1), 0 when the column is
  * empty, -1 on a vertex-budget overflow (the FP64 kernel takes over). */
 template <class R>
@@ -1772,7 +1890,7 @@ PVX_HD void cell_finish(Sink& sink, const GridInfo& g, const ColumnState<R>& S, 
  */
 template <int ORDER, class R, class Sink>
 PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task,
-                        LPoly<R>& P, TaskTally& T, bool carry = true) {
+                        LPoly<R>& P, TaskTally& T, bool carry = true, bool pair = true) {
     T.reset();
     ColumnState<R> S;
     const int st = column_begin(g, mb, rec, task, P, S);
@@ -1782,6 +1900,14 @@ PVX_HD bool column_task(Sink& sink, const GridInfo& g, const MeshBox& mb, const 
     if (ORDER > 0) tet_vel(rec, V);
     V.mass = rec[24];
     pin(V.mass);
+    if (pair && two_cell_column(S)) {
+        Mom<R> Ml, Mu;
+        R O[3];
+        slab_moments_2<ORDER>(P, S.C.G, 2, first_plane(S), Ml, Mu, O);
+        cell_finish<ORDER>(sink, g, S, V, S.k0, Ml, O, T);
+        cell_finish<ORDER>(sink, g, S, V, S.k0 + 1, Mu, O, T);
+        return true;
+    }
     PlaneCarry<R> pc{false, false, {R(0), R(0), R(0)}};
 #pragma unroll 1
     for (int k = S.k0; k < S.k1; ++k) {

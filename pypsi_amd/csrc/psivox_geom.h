@@ -33,6 +33,9 @@
 #include <math.h>
 #define PVX_HD inline
 #endif
+/* inlined even where a call site sits in a loop (a call keeps its frame in
+ * scratch memory) */
+#define PVX_HDI PVX_HD __attribute__((always_inline))
 
 namespace pvx {
 

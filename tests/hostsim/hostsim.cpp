@@ -51,6 +51,11 @@ static void fill_grid(GridInfo& g, MeshBox& mb, const int* n, const double* win6
 static bool g_carry = true;
 extern "C" void hs_set_plane_carry(int on) { g_carry = on != 0; }
 
+/* whether a column of two cells takes both in one pass over its fan
+ * triangles (slab_moments_2, as the device does) or one pass per cell */
+static bool g_pair = true;
+extern "C" void hs_set_pair_cells(int on) { g_pair = on != 0; }
+
 /* 1: wide tets (tet_wide) take the FP64 column path, as the device's
  * automatic kernel choice does; 0: every tet in FP32 (kernel 'fast') */
 static int g_route = 0;
@@ -80,11 +85,11 @@ static bool tet_by_columns(HostSink& sink, const GridInfo& g, const MeshBox& mb,
         bool ok;
         TaskTally T;
         if (g.order == 0)
-            ok = column_task<0>(sink, g, mb, r, tasks[c], P, T, g_carry);
+            ok = column_task<0>(sink, g, mb, r, tasks[c], P, T, g_carry, g_pair);
         else if (g.order == 1)
-            ok = column_task<1>(sink, g, mb, r, tasks[c], P, T, g_carry);
+            ok = column_task<1>(sink, g, mb, r, tasks[c], P, T, g_carry, g_pair);
         else
-            ok = column_task<2>(sink, g, mb, r, tasks[c], P, T, g_carry);
+            ok = column_task<2>(sink, g, mb, r, tasks[c], P, T, g_carry, g_pair);
         if (!ok) {
             fails++;
             all = false;
