@@ -936,15 +936,46 @@ __global__ __launch_bounds__(256) void k_resid_factor(const double* __restrict__
     const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= t1) return;
     const int64_t c0 = colstart[t] - base, c1 = colstart[t + 1] - base;
+    /* RS interleaved streams (task c0 + RS q + u in stream u), so that a tet
+     * with many tasks has RS loads of each array in flight at a time; the
+     * largest key wins, the lowest task among equal keys */
+    constexpr int RS = 8;
+    double ps[RS];
+    uint64_t pk[RS];
+    int64_t pb[RS];
+#pragma unroll
+    for (int u = 0; u < RS; ++u) {
+        ps[u] = 0.0;
+        pk[u] = 0ull;
+        pb[u] = -1;
+    }
+    for (int64_t cq = c0; cq < c1; cq += RS) {
+        double v[RS];
+        uint64_t k[RS];
+#pragma unroll
+        for (int u = 0; u < RS; ++u) {
+            const bool in = cq + u < c1;
+            v[u] = in ? tsum[cq + u] : 0.0;
+            k[u] = in ? tkey[cq + u] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < RS; ++u) {
+            ps[u] += v[u];
+            if (k[u] > pk[u]) {
+                pk[u] = k[u];
+                pb[u] = cq + u;
+            }
+        }
+    }
     double sum = 0.0;
     uint64_t kb = 0;
     int64_t cb = -1;
-    for (int64_t c = c0; c < c1; ++c) {
-        sum += tsum[c];
-        const uint64_t k = tkey[c];
-        if (k > kb) {
-            kb = k;
-            cb = c;
+#pragma unroll
+    for (int u = 0; u < RS; ++u) {
+        sum += ps[u];
+        if (pk[u] > kb || (pk[u] == kb && pk[u] != 0ull && pb[u] < cb)) {
+            kb = pk[u];
+            cb = pb[u];
         }
     }
     const double* r = tets + (int64_t)TREC * t;
