@@ -541,11 +541,15 @@ PVX_HD bool pass_fp64(int pass) { return pass == PASS_WIDE || pass == PASS_ALL64
 /* column tasks per tet record for one pass (0 for invalid records and for
  * tets of the other pass); a tet the column path cannot take flags the
  * FP64 fallback.  counters[2] gets the wide tets PASS_NARROW leaves out. */
+/* words before the wide-tet list in its buffer: its counter, alone in a
+ * 64-byte line */
+constexpr int WLIST_HEAD = 16;
+
 template <bool BATCH, int PASS>
 __global__ __launch_bounds__(256) void k_col_count(const double* __restrict__ tets, int64_t ntet, GridInfo g0,
                                                    MeshBox mb0, const BatchGeo* table, int64_t* __restrict__ cnt,
                                                    unsigned* status, unsigned long long* counters,
-                                                   unsigned long long* mcnt) {
+                                                   unsigned long long* mcnt, uint32_t* __restrict__ wlist) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int nc = 0, id = -1, nwide = 0;
     if (t < ntet) {
@@ -570,6 +574,27 @@ __global__ __launch_bounds__(256) void k_col_count(const double* __restrict__ te
     }
     block_count(0, nc, counters);
     if (PASS == PASS_NARROW) block_count(nwide, 0, counters + 2);
+    /* PASS_NARROW with a list: the wide tets' indices after its 64-byte
+     * counter (a line of its own: the tallies' line takes every block's
+     * adds), a workgroup's in thread order at a place taken with one add */
+    if (PASS == PASS_NARROW && wlist) {
+        __shared__ unsigned wsum[4];
+        __shared__ unsigned long long wbase;
+        const uint64_t m = __ballot(nwide != 0);
+        const int wv = (int)(threadIdx.x / warpSize), lane = (int)(threadIdx.x & (warpSize - 1));
+        if (lane == 0) wsum[wv] = (unsigned)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            wbase = tot ? atomicAdd((unsigned long long*)wlist, (unsigned long long)tot) : 0ull;
+        }
+        __syncthreads();
+        if (nwide) {
+            unsigned off = 0;
+            for (int k = 0; k < wv; ++k) off += wsum[k];
+            wlist[WLIST_HEAD + wbase + off + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)t;
+        }
+    }
     if (BATCH) {
         mesh_count(mcnt, id, nc, 1);
         if (PASS == PASS_NARROW) mesh_count(mcnt, id, nwide, 2);
@@ -595,30 +620,13 @@ __global__ __launch_bounds__(256) void k_col_emit(const double* __restrict__ tet
     });
 }
 
-/* Copies the records of the wide tets (tet_wide) among [0, ntet) to wide[]
- * and counts them in *n, so that the FP64 column pass touches only them
- * (order across wavefronts arbitrary, within a wavefront kept). */
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_wide_gather(const double* __restrict__ tets, int64_t ntet, GridInfo g0,
-                                                     MeshBox mb0, const BatchGeo* table, double* __restrict__ wide,
-                                                     unsigned long long* n) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const double* r = tets + (int64_t)TREC * (t < ntet ? t : 0);
-    bool w = false;
-    if (t < ntet && r[25] != 0.0)
-        with_mesh<BATCH>(r, g0, mb0, nullptr, table,
-                         [&](const GridInfo& g, const MeshBox&, double*, int) { w = tet_wide(r, g); });
-    const uint64_t m = __ballot(w);
-    if (m == 0ull) return;
-    const int lane = threadIdx.x & (warpSize - 1);
-    const int first = __ffsll((long long)m) - 1;
-    long long base = 0;
-    if (lane == first) base = (long long)atomicAdd(n, (unsigned long long)__popcll(m));
-    base = __shfl(base, first);
-    if (w) {
-        double* o = wide + (int64_t)TREC * (base + __popcll(m & ((1ull << lane) - 1ull)));
-        for (int q = 0; q < TREC; ++q) o[q] = r[q];
-    }
+/* wide[TREC i + q] = record wlist[i], slot q, for the n listed tets */
+__global__ __launch_bounds__(256) void k_wide_copy(const double* __restrict__ tets, const uint32_t* __restrict__ wlist,
+                                                   int64_t n, double* __restrict__ wide) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * TREC) return;
+    const int64_t k = i / TREC;
+    wide[i] = tets[(int64_t)TREC * wlist[k] + (i - k * TREC)];
 }
 
 /* per-task outputs for the residual pass: FP64 fragment-mass sum, the
@@ -1452,6 +1460,7 @@ struct Workspace {
     int stream_dev = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf wtets;                           /* the wide tets' records for the FP64 column pass */
+    DevBuf wlist;                           /* the wide tets' indices, from the count pass */
     bool compact_wide = true;               /* the FP64 pass on a copy of the wide tets' records */
     bool timing = true;                     /* kernel_ms from events (off: no events, kernel_ms = 0) */
     int64_t task_cap = (int64_t)1 << 28;    /* column tasks per chunk */
@@ -1734,16 +1743,20 @@ __global__ void k_chunk_cut(const int64_t* __restrict__ colstart, int64_t nt, in
 /* k_col_count for a pass, one mesh or a batch */
 template <bool BATCH>
 static void launch_col_count(int pass, const double* tets, int64_t nt, const PassGeo& G, int64_t* cnt, unsigned* status,
-                             unsigned long long* counters, hipStream_t s) {
+                             unsigned long long* counters, uint32_t* wlist, hipStream_t s) {
     const unsigned nb = grid_blocks(nt, 256);
     if (pass == PASS_NARROW)
-        k_col_count<BATCH, PASS_NARROW><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
+        k_col_count<BATCH, PASS_NARROW>
+            <<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt, wlist);
     else if (pass == PASS_WIDE)
-        k_col_count<BATCH, PASS_WIDE><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
+        k_col_count<BATCH, PASS_WIDE>
+            <<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt, nullptr);
     else if (pass == PASS_ALL64)
-        k_col_count<BATCH, PASS_ALL64><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
+        k_col_count<BATCH, PASS_ALL64>
+            <<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt, nullptr);
     else
-        k_col_count<BATCH, PASS_ALL><<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt);
+        k_col_count<BATCH, PASS_ALL>
+            <<<nb, 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, cnt, status, counters, G.mcnt, nullptr);
 }
 
 template <bool BATCH>
@@ -1773,13 +1786,20 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
     if ((rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev))) return rc;
     int64_t* colcnt = (int64_t*)W.colcnt.p;
     int64_t* colstart = (int64_t*)W.colstart.p;
-    /* colcnt[nt] = 0 and the wide-tet counter counters[2] = 0 */
+    /* colcnt[nt] = 0, the wide-tet counter counters[2] = 0 and, when the
+     * wide tets are copied for their own pass, their list's counter = 0 */
+    uint32_t* wlist = nullptr;
+    if (pass == PASS_NARROW && W.compact_wide) {
+        if ((rc = W.wlist.get((size_t)(WLIST_HEAD + nt) * sizeof(uint32_t), dev))) return rc;
+        wlist = (uint32_t*)W.wlist.p;
+        k_fill_u32<<<1, 64, 0, s>>>(wlist, 2, 0u);
+    }
     k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)(colcnt + nt), 2, 0u);
     k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)(counters + 2), 2, 0u);
     if (G.table)
-        launch_col_count<true>(pass, tets, nt, G, colcnt, status, counters, s);
+        launch_col_count<true>(pass, tets, nt, G, colcnt, status, counters, wlist, s);
     else
-        launch_col_count<false>(pass, tets, nt, G, colcnt, status, counters, s);
+        launch_col_count<false>(pass, tets, nt, G, colcnt, status, counters, wlist, s);
     HIPCHK(hipGetLastError());
     size_t tb = 0;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, colcnt, colstart, nt + 1, s));
@@ -1836,22 +1856,21 @@ int for_task_chunks(Workspace& W, const double* tets, int64_t nt, const PassGeo&
 }
 
 /* The tet records of a column pass: for PASS_WIDE (when W compacts them)
- * a copy of the nwide wide tets among tets[0, nt), in *pt, *pn; otherwise
- * tets and nt themselves.  counters[3] counts the copies. */
+ * a copy of the nwide wide tets among tets[0, nt), in *pt, *pn, from the
+ * list the PASS_NARROW count made (W.wlist); otherwise tets and nt
+ * themselves. */
 int pass_tets(Workspace& W, int pass, const double* tets, int64_t nt, const PassGeo& G, int64_t nwide,
               unsigned long long* counters, hipStream_t s, int dev, const double** pt, int64_t* pn) {
+    (void)nt;
+    (void)G;
+    (void)counters;
     *pt = tets;
     *pn = nt;
     if (pass != PASS_WIDE || !W.compact_wide) return 0;
     int rc = W.wtets.get((size_t)nwide * TREC * sizeof(double), dev);
     if (rc) return rc;
-    k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)(counters + 3), 2, 0u);
-    if (G.table)
-        k_wide_gather<true><<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, G.g, G.mb, G.table, (double*)W.wtets.p,
-                                                                 counters + 3);
-    else
-        k_wide_gather<false><<<grid_blocks(nt, 256), 256, 0, s>>>(tets, nt, G.g, G.mb, nullptr, (double*)W.wtets.p,
-                                                                  counters + 3);
+    k_wide_copy<<<grid_blocks(nwide * TREC, 256), 256, 0, s>>>(tets, (const uint32_t*)W.wlist.p + WLIST_HEAD, nwide,
+                                                               (double*)W.wtets.p);
     HIPCHK(hipGetLastError());
     *pt = (const double*)W.wtets.p;
     *pn = nwide;
