@@ -367,6 +367,15 @@ PVX_HD int clamp_index(double t, int n) {
     return (int)t;
 }
 
+/* the grid plane x_ax = w0 + d idx, rounded as the reference rounds it (a
+ * multiply, then an add: src/geometry.c:887-888), not as one fused
+ * multiply-add: every sweep of a polyhedron through its cells cuts at the
+ * same planes as the reference's */
+PVX_HD double grid_plane(const GridInfo& g, int ax, int idx) {
+#pragma clang fp contract(off)
+    return g.d[ax] * (double)idx + g.w0[ax];
+}
+
 PVX_HD void index_range(const GridInfo& g, int ax, double lo, double hi, int& ilo, int& ihi) {
     ilo = clamp_index(floor((lo - g.w0[ax]) / g.d[ax]), g.n[ax]);
     ihi = clamp_index(ceil((hi - g.w0[ax]) / g.d[ax]), g.n[ax]);

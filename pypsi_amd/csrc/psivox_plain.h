@@ -152,7 +152,7 @@ PVX_HD int64_t sweep_cells64(Poly64* P, Poly64* S, Poly64* C, Poly64* V, const G
     int64_t nfrag = 0;
     for (int i = ib0[0]; i < ib1[0]; ++i) {
         if (i < ib1[0] - 1) {
-            split64(P, S, g.d[0] * (i + 1) + g.w0[0], 0, status);
+            split64(P, S, grid_plane(g, 0, i + 1), 0, status);
         } else {
             swap_ptr(P, S);
             P->nv = 0;
@@ -162,7 +162,7 @@ PVX_HD int64_t sweep_cells64(Poly64* P, Poly64* S, Poly64* C, Poly64* V, const G
             poly_range64(S, g, 1, ib0[1], ib1[1], j0, j1);
             for (int j = j0; j < j1; ++j) {
                 if (j < j1 - 1) {
-                    split64(S, C, g.d[1] * (j + 1) + g.w0[1], 1, status);
+                    split64(S, C, grid_plane(g, 1, j + 1), 1, status);
                 } else {
                     swap_ptr(S, C);
                     S->nv = 0;
@@ -172,7 +172,7 @@ PVX_HD int64_t sweep_cells64(Poly64* P, Poly64* S, Poly64* C, Poly64* V, const G
                     poly_range64(C, g, 2, ib0[2], ib1[2], k0, k1);
                     for (int k = k0; k < k1; ++k) {
                         if (k < k1 - 1) {
-                            split64(C, V, g.d[2] * (k + 1) + g.w0[2], 2, status);
+                            split64(C, V, grid_plane(g, 2, k + 1), 2, status);
                         } else {
                             swap_ptr(C, V);
                             C->nv = 0;
@@ -234,7 +234,7 @@ PVX_HD int64_t voxelize_copy64(Sink& sink, const GridInfo& g, const double (*x)[
     int64_t nfrag = 0;
     for (int i = ib0[0]; i < ib1[0]; ++i) {
         if (i < ib1[0] - 1) {
-            split64(P, S, g.d[0] * (i + 1) + g.w0[0], 0, status);
+            split64(P, S, grid_plane(g, 0, i + 1), 0, status);
         } else {
             swap_ptr(P, S);
             P->nv = 0;
@@ -244,7 +244,7 @@ PVX_HD int64_t voxelize_copy64(Sink& sink, const GridInfo& g, const double (*x)[
             poly_range64(S, g, 1, ib0[1], ib1[1], j0, j1);
             for (int j = j0; j < j1; ++j) {
                 if (j < j1 - 1) {
-                    split64(S, C, g.d[1] * (j + 1) + g.w0[1], 1, status);
+                    split64(S, C, grid_plane(g, 1, j + 1), 1, status);
                 } else {
                     swap_ptr(S, C);
                     S->nv = 0;
@@ -254,7 +254,7 @@ PVX_HD int64_t voxelize_copy64(Sink& sink, const GridInfo& g, const double (*x)[
                     poly_range64(C, g, 2, ib0[2], ib1[2], k0, k1);
                     for (int k = k0; k < k1; ++k) {
                         if (k < k1 - 1) {
-                            split64(C, V, g.d[2] * (k + 1) + g.w0[2], 2, status);
+                            split64(C, V, grid_plane(g, 2, k + 1), 2, status);
                         } else {
                             swap_ptr(C, V);
                             C->nv = 0;
