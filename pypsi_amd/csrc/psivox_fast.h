@@ -1261,17 +1261,21 @@ PVX_HD bool column_zext(const float (*X)[3], float xl, float xh, float yl, float
     };
     for (int v = 0; v < 4; ++v) take(X[v][0], X[v][1], X[v][2]);
     for (int u = 0; u < 3; ++u)
-        for (int v = u + 1; v < 4; ++v)
+        for (int v = u + 1; v < 4; ++v) {
+            /* the edge's reciprocal extent along x and y, shared by the two
+             * planes of each (read only where the edge crosses one) */
+            const float rx = frcp(X[u][0] - X[v][0]), ry = frcp(X[u][1] - X[v][1]);
             for (int s = 0; s < 4; ++s) {
                 const int ax = s >> 1;
                 const float c = s == 0 ? xl : (s == 1 ? xh : (s == 2 ? yl : yh));
                 const float du = X[u][ax] - c, dv = X[v][ax] - c;
                 if ((du < 0.f) == (dv < 0.f)) continue;
-                const float t = du * frcp(du - dv);
+                const float t = du * (ax == 0 ? rx : ry);
                 const float px = ax == 0 ? c : X[u][0] + t * (X[v][0] - X[u][0]);
                 const float py = ax == 1 ? c : X[u][1] + t * (X[v][1] - X[u][1]);
                 take(px, py, X[u][2] + t * (X[v][2] - X[u][2]));
             }
+        }
     /* barycentrics l_1..3 = Inv (p - X0), l_0 = 1 - sum; Inv = adj(D) / det */
     float D[3][3];
     for (int a = 0; a < 3; ++a)

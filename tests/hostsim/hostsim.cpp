@@ -416,6 +416,52 @@ extern "C" int hs_ytable_check(int vpe, int64_t nelem, const double* pos, const 
     return 0;
 }
 
+/* The ordering's sort key against the built columns: for every FP32
+ * column task with a non-empty column, column_cells (the key's cell count,
+ * from column_zext without building) against the built column's k1 - k0.
+ * out = {tasks, equal, key below the built count, key above it}. */
+extern "C" int hs_zext_check(int vpe, int64_t nelem, const double* pos, const double* vel, const double* mass,
+                             const int32_t* conn, int periodic, const double* box6, const int* n, const double* win6,
+                             const double* d, int64_t* out) {
+    double* fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    GridInfo g;
+    MeshBox mb;
+    fill_grid(g, mb, n, win6, d, fields, 0, periodic, box6);
+    RefineScratch* RS = new RefineScratch;
+    std::vector<uint32_t> la(VRec<float>::WORDS);
+    LPoly<float> P{la.data(), 0, 0, 0};
+    std::vector<double> tets;
+    std::vector<uint64_t> tasks;
+    out[0] = out[1] = out[2] = out[3] = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        const int64_t nt = expand_element(pos, vel, mass, conn, vpe, e, g, mb, 0, 0.0, *RS, nullptr);
+        if (nt == 0) continue;
+        tets.resize((size_t)nt * TREC);
+        expand_element(pos, vel, mass, conn, vpe, e, g, mb, 0, 0.0, *RS, tets.data());
+        for (int64_t t = 0; t < nt; ++t) {
+            const double* r = &tets[(size_t)t * TREC];
+            if (r[25] == 0.0 || tet_wide(r, g)) continue;
+            const int64_t k = tet_columns<float>(r, t, g, mb, nullptr, nullptr);
+            if (k <= 0) continue;
+            tasks.resize((size_t)k);
+            tet_columns<float>(r, t, g, mb, tasks.data(), nullptr);
+            for (int64_t c = 0; c < k; ++c) {
+                ColumnState<float> S;
+                if (column_begin(g, mb, r, tasks[c], P, S) <= 0) continue;
+                int64_t tt;
+                int h, il, jl;
+                unpack_task(tasks[c], tt, h, il, jl);
+                const int key = column_cells(S.C, il, jl);
+                const int built = S.k1 - S.k0;
+                out[0]++;
+                out[key == built ? 1 : (key < built ? 2 : 3)]++;
+            }
+        }
+    }
+    delete RS;
+    return 0;
+}
+
 extern "C" int hs_mass_adjoint(int vpe, int64_t npart, int64_t nelem, const double* pos, const double* mass,
                                const int32_t* conn, int periodic, const double* box6, const int* n, const double* win6,
                                const double* d, const double* w, double* dpos, int64_t* out_stats) {

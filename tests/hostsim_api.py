@@ -183,3 +183,18 @@ def ytable_check(P, reftol=1.0, maxlvl=0, fp64=False):
                           vp(P.conn), C.c_int(int(P.periodic)), box, n, win, d, C.c_double(reftol), C.c_int(maxlvl),
                           C.c_int(int(fp64)), vp(out))
     return tuple(int(x) for x in out)
+
+
+def zext_check(P):
+    """The ordering's sort key (column_cells) against the built columns'
+    cell counts over every FP32 column task: (tasks, equal, key below, key
+    above)."""
+    out = np.zeros(4, np.int64)
+    n = (C.c_int * 3)(*P.n)
+    win = (C.c_double * 6)(*P.bounds.reshape(-1))
+    d = (C.c_double * 3)(*P.d)
+    box = (C.c_double * 6)(*P.box.reshape(-1))
+    vp = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    lib().hs_zext_check(C.c_int(P.elemtype), C.c_int64(P.conn.shape[0]), vp(P.pos), vp(P.vel), vp(P.mass),
+                        vp(P.conn), C.c_int(int(P.periodic)), box, n, win, d, vp(out))
+    return tuple(int(x) for x in out)

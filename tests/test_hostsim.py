@@ -148,3 +148,20 @@ def test_y_table_matches_generic_clips(oracle_mod, fp64):
         print(P.n, lvl, "tasks", ntask, "y-clipped", ntable, "mismatches", bad)
         assert ntask > 0 and ntable > 0.5 * ntask
         assert bad == 0
+
+
+def test_sort_key_cell_count(oracle_mod):
+    """The ordering's key takes a column task's cell count from column_zext
+    (the z-extremes of tet and column without building the column): on a
+    periodic Zel'dovich mesh and on an open one in a window it is never
+    below the built column's cell count and equal to it for at least 99% of
+    the tasks (a vertex on a cell plane can add one)."""
+    cases = [(16, 2.0, (32,) * 3, BOX, True), (12, 1.0, (24, 20, 28), ((0.1, 0.15, 0.2), (0.8, 0.9, 0.85)), False)]
+    for n, rms, gn, bounds, periodic in cases:
+        pos, vel = zel(n, rms, 3)
+        P = oracle_mod.Problem.block(pos, vel, periodic_box=BOX if periodic else None, n=gn, bounds=bounds,
+                                     fields=[])
+        tasks, equal, below, above = H.zext_check(P)
+        assert tasks > 10000
+        assert below == 0, (tasks, equal, below, above)
+        assert equal >= 0.99 * tasks, (tasks, equal, below, above)
