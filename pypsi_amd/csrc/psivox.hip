@@ -652,6 +652,30 @@ constexpr int ORDW = 16384;
 constexpr int ORDK = 256;
 static_assert(ORDW % 256 == 0, "whole threads per window");
 
+/* The column kernel's wavefront counters, one per XCD (an MI355X has 8,
+ * each with its own L2; the hardware places workgroup b on XCD b mod 8),
+ * 128 bytes apart.  Counter x hands out the ordering windows x, x + 8,
+ * x + 16, ... wavefront by wavefront, so the tet records of a window are
+ * read into one XCD's L2 instead of all eight; a workgroup whose own
+ * windows are done takes from the other counters in turn. */
+constexpr int NXCD = 8;
+constexpr int XSTRIDE = 32;
+constexpr int64_t XCH = ORDW / FBLOCK; /* wavefronts per window */
+
+/* the next wavefront (index into the ordered tasks, in units of FBLOCK) for
+ * a workgroup on XCD home that has exhausted the counters home .. home +
+ * cur - 1; nwaves when all are exhausted */
+__device__ int64_t claim_wave(unsigned* next, int home, int& cur, int64_t nwaves) {
+    while (cur < NXCD) {
+        const int x = home + cur < NXCD ? home + cur : home + cur - NXCD;
+        const unsigned k = atomicAdd(next + x * XSTRIDE, 1u);
+        const int64_t w = ((int64_t)(k / XCH) * NXCD + x) * XCH + (int64_t)(k % XCH);
+        if (w < nwaves) return w;
+        ++cur;
+    }
+    return nwaves;
+}
+
 /* perm[w0 .. w0 + ORDW) of each window: the window's task indices, largest
  * class first (order within a class is arbitrary), and sorted[] the task
  * words in the same order (the column kernel then reads both coalesced and
@@ -667,7 +691,7 @@ __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ te
     const int64_t w0 = (int64_t)blockIdx.x * ORDW;
     const int nw = (int)(ntask - w0 < ORDW ? ntask - w0 : ORDW);
     const int tid = (int)threadIdx.x;
-    if (blockIdx.x == 0 && tid == 0) *next = 0u;
+    if (blockIdx.x == 0 && tid < NXCD) next[tid * XSTRIDE] = 0u;
     cnt[tid] = 0;
     __syncthreads();
     for (int e = tid; e < nw; e += 256) {
@@ -890,15 +914,18 @@ __global__ __launch_bounds__(FBLOCK, ORDER == 0 && sizeof(R) == 4 ? 3 : 2) void 
     int64_t nfrag = 0;
     /* each wavefront of tasks is claimed one round ahead, so the counter's
      * round trip overlaps the work on the one before */
-    unsigned claim = 0u;
-    if (lane == 0) claim = atomicAdd(next, 1u);
+    const int home = (int)(blockIdx.x % NXCD);
+    const int64_t nwaves = (ntask + FBLOCK - 1) / FBLOCK;
+    int cur = 0;
+    int64_t claim = 0;
+    if (lane == 0) claim = claim_wave(next, home, cur, nwaves);
     for (;;) {
-        if (lane == 0) swave = claim;
+        if (lane == 0) swave = (unsigned)claim;
         __syncthreads();
         const int64_t c0 = (int64_t)swave * FBLOCK;
         __syncthreads(); /* everyone has read swave before lane 0 writes it again */
         if (c0 >= ntask) break;
-        if (lane == 0) claim = atomicAdd(next, 1u);
+        if (lane == 0) claim = claim_wave(next, home, cur, nwaves);
         const int64_t c = c0 + lane;
         int64_t p = 0;
         uint64_t w = 0;
@@ -1158,16 +1185,16 @@ static KColsFn cols_kernel(const GridInfo& g, bool fp64, bool batch) {
 }
 
 /* The ordering scratch of ntask tasks: perm (4 bytes a task) and the
- * column kernel's counter, then the sorted task words (8 bytes a task) and
+ * column kernel's counters, then the sorted task words (8 bytes a task) and
  * their mesh ids (2 bytes a task, batched calls). */
-static size_t order_bytes(int64_t ntask) {
-    return (((size_t)ntask * 4 + 256 + 255) & ~(size_t)255) + (size_t)ntask * 8 + (size_t)ntask * 2;
+static size_t order_head(int64_t ntask) {
+    return ((size_t)ntask * 4 + 128 + NXCD * XSTRIDE * 4 + 255) & ~(size_t)255;
 }
+static size_t order_bytes(int64_t ntask) { return order_head(ntask) + (size_t)ntask * 8 + (size_t)ntask * 2; }
 static uint32_t* order_perm(void* b) { return (uint32_t*)b; }
-static unsigned* perm_counter(void* b, int64_t nsub) { return (unsigned*)b + nsub; }
-static uint64_t* order_sorted(void* b, int64_t nsub) {
-    return (uint64_t*)((char*)b + (((size_t)nsub * 4 + 256 + 255) & ~(size_t)255));
-}
+/* the column kernel's NXCD wavefront counters, 128 bytes apart, after perm */
+static unsigned* perm_counter(void* b, int64_t nsub) { return (unsigned*)b + ((nsub + 31) & ~(int64_t)31); }
+static uint64_t* order_sorted(void* b, int64_t nsub) { return (uint64_t*)((char*)b + order_head(nsub)); }
 static uint16_t* order_mid(void* b, int64_t nsub) { return (uint16_t*)(order_sorted(b, nsub) + nsub); }
 
 /* the column kernel over tasks [0, ntask) of the pass geometry G (one mesh,
