@@ -891,10 +891,10 @@ PVX_HD void pair_terms_2(const LPoly<R>& P, int e, uint32_t le, const R* O, R zd
 /*
  * The moments of both cells of a column with exactly two cells and one
  * plane between them (hi_c along ax; no window plane in either cell), in
- * one pass: the cells' single-plane passes (slab_moments_u) set up the same
- * plane twice, and each crossing triangle is cut once here instead of once
- * per cell.  Ml, Mu: the lower and upper cell, both about O (on the plane,
- * the point the lower cell's pass takes).
+ * one pass over the fan triangles: cell by cell (slab_moments_u) each cell
+ * walks them and cuts each crossing triangle, here each is cut once, into
+ * both cells' pieces.  Ml, Mu: the lower and upper cell, both about O (on
+ * the plane, the point the lower cell's pass takes).
  */
 template <int ORDER, class R>
 PVX_HD void slab_moments_2(const LPoly<R>& P, const CopyGeo<R>& G, int ax, double hi_c, Mom<R>& Ml, Mom<R>& Mu,
