@@ -676,69 +676,106 @@ __device__ int64_t claim_wave(unsigned* next, int home, int& cur, int64_t nwaves
     return nwaves;
 }
 
+/* One window of WIN consecutive tasks (the workgroup's, NT >= 256
+ * threads), counting-sorted by class in LDS: key(e) gives task w0 + e's
+ * class (0 is processed first), out(e, pos) places task w0 + e at w0 + pos.
+ * Order within a class is arbitrary. */
+template <int WIN, int NT, class Key, class Out>
+__device__ void window_sort(int64_t ntask, Key&& key, Out&& out) {
+    static_assert(NT >= ORDK, "one thread per class in the scan");
+    __shared__ int cnt[ORDK];
+    __shared__ uint8_t skey[WIN]; /* each task's class, kept here between the two passes */
+    const int64_t w0 = (int64_t)blockIdx.x * WIN;
+    const int nw = (int)(ntask - w0 < WIN ? ntask - w0 : WIN);
+    const int tid = (int)threadIdx.x;
+    if (tid < ORDK) cnt[tid] = 0;
+    __syncthreads();
+    for (int e = tid; e < nw; e += NT) {
+        const int k = key(e);
+        skey[e] = (uint8_t)k;
+        atomicAdd(&cnt[k], 1);
+    }
+    __syncthreads();
+    /* exclusive scan of the 256 class counts (Hillis-Steele) */
+    int v = tid < ORDK ? cnt[tid] : 0;
+    for (int d = 1; d < ORDK; d <<= 1) {
+        __syncthreads();
+        const int u = tid < ORDK && tid >= d ? cnt[tid - d] : 0;
+        __syncthreads();
+        v += u;
+        if (tid < ORDK) cnt[tid] = v;
+    }
+    __syncthreads();
+    const int ex = tid > 0 && tid < ORDK ? cnt[tid - 1] : 0;
+    __syncthreads();
+    if (tid < ORDK) cnt[tid] = ex;
+    __syncthreads();
+    for (int e = tid; e < nw; e += NT) out(e, atomicAdd(&cnt[skey[e]], 1));
+}
+
 /* perm[w0 .. w0 + ORDW) of each window: the window's task indices, largest
- * class first (order within a class is arbitrary), and sorted[] the task
- * words in the same order (the column kernel then reads both coalesced and
- * independently).  Also clears the column kernel's wavefront counter.  256
- * threads: one class each in the scan. */
+ * class first, and sorted[] the task words in the same order (the column
+ * kernel then reads both coalesced and independently).  Also clears the
+ * column kernel's wavefront counters. */
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_col_order(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
                                                    int64_t ntask, GridInfo g0, MeshBox mb0, const BatchGeo* table,
                                                    uint32_t* __restrict__ perm, uint64_t* __restrict__ sorted,
                                                    uint16_t* __restrict__ mid, unsigned* __restrict__ next) {
-    __shared__ int cnt[ORDK];
-    __shared__ uint8_t skey[ORDW]; /* each task's class, kept here between the two passes */
     const int64_t w0 = (int64_t)blockIdx.x * ORDW;
-    const int nw = (int)(ntask - w0 < ORDW ? ntask - w0 : ORDW);
-    const int tid = (int)threadIdx.x;
-    if (blockIdx.x == 0 && tid < NXCD) next[tid * XSTRIDE] = 0u;
-    cnt[tid] = 0;
-    __syncthreads();
-    for (int e = tid; e < nw; e += 256) {
-        const uint64_t w = tasks[w0 + e];
-        int64_t t;
-        int h, il, jl;
-        unpack_task(w, t, h, il, jl);
-        const double* r = tets + (int64_t)TREC * t;
-        int nc = 0, nv = 0;
-        with_mesh<BATCH>(r, g0, mb0, nullptr, table, [&](const GridInfo& g, const MeshBox& mb, double*, int) {
-            double lo[3], hi[3];
-            aabb((const double (*)[3])r, 4, lo, hi);
-            uint32_t shp;
-            ghost_shifts_packed(lo, hi, g, mb, shp);
-            CopySetup<float> C;
-            if (copy_setup(r, lo, hi, shp, h, g, mb, C)) nc = column_cells(C, il, jl, &nv);
+    if (blockIdx.x == 0 && threadIdx.x < NXCD) next[threadIdx.x * XSTRIDE] = 0u;
+    window_sort<ORDW, 256>(
+        ntask,
+        [&](int e) {
+            const uint64_t w = tasks[w0 + e];
+            int64_t t;
+            int h, il, jl;
+            unpack_task(w, t, h, il, jl);
+            const double* r = tets + (int64_t)TREC * t;
+            int nc = 0, nv = 0;
+            with_mesh<BATCH>(r, g0, mb0, nullptr, table, [&](const GridInfo& g, const MeshBox& mb, double*, int) {
+                double lo[3], hi[3];
+                aabb((const double (*)[3])r, 4, lo, hi);
+                uint32_t shp;
+                ghost_shifts_packed(lo, hi, g, mb, shp);
+                CopySetup<float> C;
+                if (copy_setup(r, lo, hi, shp, h, g, mb, C)) nc = column_cells(C, il, jl, &nv);
+            });
+            nc = nc < 15 ? nc : 15;
+            nv = nv < 15 ? nv : 15;
+            return ORDK - 1 - (16 * nc + nv);
+        },
+        [&](int e, int pos) {
+            const uint64_t w = tasks[w0 + e];
+            perm[w0 + pos] = (uint32_t)(w0 + e);
+            sorted[w0 + pos] = w;
+            /* batched: the task's mesh, in processing order (the column
+             * kernel then need not read it from the scattered tet record
+             * first) */
+            if (BATCH) mid[w0 + pos] = (uint16_t)tets[(int64_t)TREC * (int64_t)(w & 0xffffffffull) + 27];
         });
-        nc = nc < 15 ? nc : 15;
-        nv = nv < 15 ? nv : 15;
-        const int key = ORDK - 1 - (16 * nc + nv);
-        skey[e] = (uint8_t)key;
-        atomicAdd(&cnt[key], 1);
-    }
-    __syncthreads();
-    /* exclusive scan of the 256 class counts (Hillis-Steele) */
-    int v = cnt[tid];
-    for (int d = 1; d < ORDK; d <<= 1) {
-        __syncthreads();
-        const int u = tid >= d ? cnt[tid - d] : 0;
-        __syncthreads();
-        v += u;
-        cnt[tid] = v;
-    }
-    __syncthreads();
-    const int ex = tid > 0 ? cnt[tid - 1] : 0;
-    __syncthreads();
-    cnt[tid] = ex;
-    __syncthreads();
-    for (int e = tid; e < nw; e += 256) {
-        const int pos = atomicAdd(&cnt[skey[e]], 1);
-        const uint64_t w = tasks[w0 + e];
-        perm[w0 + pos] = (uint32_t)(w0 + e);
-        sorted[w0 + pos] = w;
-        /* batched: the task's mesh, in processing order (the column kernel
-         * then need not read it from the scattered tet record first) */
-        if (BATCH) mid[w0 + pos] = (uint16_t)tets[(int64_t)TREC * (int64_t)(w & 0xffffffffull) + 27];
-    }
+}
+
+/* The adjoint's plane tasks of each window sorted by the cells their
+ * section spans, most first (adj_task_cells): a lane's work grows with
+ * that count and a wavefront runs as long as its longest lane (in emit
+ * order the lanes did 37% of the longest lane's work on the Zel'dovich
+ * 128^3 -> 256^3 workload, 98% sorted; host model of the lanes).  1024
+ * threads a window: a call's few million tasks make only hundreds of
+ * windows (windows of 2048 tasks made the plane-task kernel itself 1.8x
+ * slower). */
+__global__ __launch_bounds__(1024) void k_adj_order(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
+                                                    int64_t ntask, GridInfo g, MeshBox mb,
+                                                    uint64_t* __restrict__ sorted) {
+    const int64_t w0 = (int64_t)blockIdx.x * ORDW;
+    window_sort<ORDW, 1024>(
+        ntask,
+        [&](int e) {
+            const uint64_t w = tasks[w0 + e];
+            const int n = adj_task_cells(g, mb, tets + (int64_t)TREC * (int64_t)(w & 0xffffffffull), w);
+            return ORDK - 1 - (n < ORDK - 1 ? n : ORDK - 1);
+        },
+        [&](int e, int pos) { sorted[w0 + pos] = tasks[w0 + e]; });
 }
 
 /* A fragment's deposit staged in LDS: add(i, v) puts v at row[i - base]
@@ -1319,11 +1356,16 @@ __device__ __forceinline__ void adj_scatter(const AdjArgs& A, int64_t t, int v, 
         unsafeAtomicAdd(out + 3 * (int64_t)md.conn[4 * e + v] + b, val);
         return;
     }
-    double xi[3], wn[27];
+    /* the node weights from their one-axis factors, node by node (an array
+     * of the 27 weights would live in scratch memory) */
+    double xi[3], F[3][3];
     unpack_param(r, v, xi);
-    const int nn = node_weights(md.vpe, xi, wn);
-    for (int n = 0; n < nn; ++n)
-        if (wn[n] != 0.0) unsafeAtomicAdd(out + 3 * (int64_t)md.conn[e * md.vpe + n] + b, val * wn[n]);
+    const int q = node_factors(md.vpe, xi, F);
+    for (int n = 0; n < q * q * q; ++n) {
+        const int i = n / (q * q), j = (n / q) % q, k = n % q;
+        const double wn = pick_factor(F[0], i) * pick_factor(F[1], j) * pick_factor(F[2], k);
+        if (wn != 0.0) unsafeAtomicAdd(out + 3 * (int64_t)md.conn[e * md.vpe + n] + b, val * wn);
+    }
 }
 
 /* Per cell: the adjoint weights dL/d(M, Sv, Svv) (ADJ_NW values) from the
@@ -1379,8 +1421,10 @@ __global__ __launch_bounds__(FBLOCK) void k_adj(const double* __restrict__ tets,
     double o[4];
     Poly2<FBLOCK> R{&ru[0][lane], &rv[0][lane]};
     adj_plane_task<FBLOCK, FULL>(g, mb, tets + (int64_t)TREC * t, w, A.w, R, o);
-    for (int v = 0; v < 4; ++v)
-        if (o[v] != 0.0) adj_scatter(A, t, v, ax, o[v], A.dpos);
+    for (int v = 0; v < 4; ++v) {
+        const double val = v == 0 ? o[0] : (v == 1 ? o[1] : (v == 2 ? o[2] : o[3]));
+        if (val != 0.0) adj_scatter(A, t, v, ax, val, A.dpos);
+    }
 }
 
 /* one lane per column task: the velocity part of the adjoint (FP64 column
@@ -1396,15 +1440,20 @@ __global__ __launch_bounds__(FBLOCK) void k_adj_vel(const double* __restrict__ t
     __shared__ unsigned swave;
     const int lane = threadIdx.x;
     LPoly<double> P{(uint32_t*)__builtin_assume_aligned(&sv[VRec<double>::W * lane], 16), 0, 0, 0};
-    unsigned claim = 0u;
-    if (lane == 0) claim = atomicAdd(next, 1u);
+    /* wavefronts claimed one round ahead from the per-XCD counters, as the
+     * column kernel does */
+    const int home = (int)(blockIdx.x % NXCD);
+    const int64_t nwaves = (ntask + FBLOCK - 1) / FBLOCK;
+    int cur = 0;
+    int64_t claim = 0;
+    if (lane == 0) claim = claim_wave(next, home, cur, nwaves);
     for (;;) {
-        if (lane == 0) swave = claim;
+        if (lane == 0) swave = (unsigned)claim;
         __syncthreads();
         const int64_t c0 = (int64_t)swave * FBLOCK;
         __syncthreads();
         if (c0 >= ntask) break;
-        if (lane == 0) claim = atomicAdd(next, 1u);
+        if (lane == 0) claim = claim_wave(next, home, cur, nwaves);
         const int64_t c = c0 + lane;
         if (c >= ntask) continue;
         const uint64_t w = sorted[c];
@@ -1415,9 +1464,15 @@ __global__ __launch_bounds__(FBLOCK) void k_adj_vel(const double* __restrict__ t
             atomicOr(status, ST_OVERFLOW);
             continue;
         }
-        for (int v = 0; v < 4; ++v)
-            for (int b = 0; b < 3; ++b)
-                if (o[v][b] != 0.0) adj_scatter(A, t, v, b, o[v][b], A.dvel);
+        /* one scatter site for the 12 values, each picked with selects (a
+         * run-time index would put o in scratch memory; unrolled sites make
+         * the kernel outgrow the instruction cache) */
+        for (int q = 0; q < 12; ++q) {
+            double val = 0.0;
+#pragma unroll
+            for (int u = 0; u < 12; ++u) val = q == u ? o[u / 3][u % 3] : val;
+            if (val != 0.0) adj_scatter(A, t, q / 3, q % 3, val, A.dvel);
+        }
     }
 }
 
@@ -1488,6 +1543,7 @@ struct Workspace {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf wtets;                           /* the wide tets' records for the FP64 column pass */
     DevBuf wlist;                           /* the wide tets' indices, from the count pass */
+    DevBuf asorted;                         /* the adjoint's plane tasks in processing order */
     bool compact_wide = true;               /* the FP64 pass on a copy of the wide tets' records */
     bool timing = true;                     /* kernel_ms from events (off: no events, kernel_ms = 0) */
     int64_t task_cap = (int64_t)1 << 28;    /* column tasks per chunk */
@@ -2555,7 +2611,9 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
             HIPCHK(hipStreamSynchronize(s));
             const int64_t cap = (int64_t)1 << 28;
             if ((rc = W.tasks.get((size_t)std::min(hs[nt], cap) * sizeof(uint64_t) + 8, dev))) return rc;
+            if ((rc = W.asorted.get((size_t)std::min(hs[nt], cap) * sizeof(uint64_t) + 8, dev))) return rc;
             uint64_t* tasks = (uint64_t*)W.tasks.p;
+            uint64_t* sorted = (uint64_t*)W.asorted.p;
             for (int64_t a0 = 0; a0 < nt;) {
                 int64_t a1 = std::upper_bound(hs.begin() + a0 + 1, hs.end(), hs[a0] + cap) - hs.begin() - 1;
                 if (a1 <= a0) a1 = a0 + 1;
@@ -2563,10 +2621,12 @@ int run_adjoint(Workspace& W, const psivox_mesh* mesh, const psivox_grid* grid, 
                 if (nsub > 0) {
                     k_adj_emit<<<grid_blocks(a1 - a0, 256), 256, 0, s>>>(tets, a0, a1, g, mb, start, hs[a0], tasks);
                     HIPCHK(hipGetLastError());
+                    k_adj_order<<<(unsigned)((nsub + ORDW - 1) / ORDW), 1024, 0, s>>>(tets, tasks, nsub, g, mb, sorted);
+                    HIPCHK(hipGetLastError());
                     if (full)
-                        k_adj<true><<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, tasks, nsub, g, mb, A);
+                        k_adj<true><<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, sorted, nsub, g, mb, A);
                     else
-                        k_adj<false><<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, tasks, nsub, g, mb, A);
+                        k_adj<false><<<grid_blocks(nsub, FBLOCK), FBLOCK, 0, s>>>(tets, sorted, nsub, g, mb, A);
                     HIPCHK(hipGetLastError());
                 }
                 a0 = a1;

@@ -91,6 +91,45 @@ PVX_HD int64_t adj_tasks(const double* rec, int64_t t, const GridInfo& g, const 
     return n;
 }
 
+/* The cells a plane task's section spans: the rows of its index box in the
+ * plane times their cells (adj_plane_task visits at most these), from the
+ * points where the tet's edges cross the plane.  Only the processing order
+ * depends on it (the sort key of the plane tasks). */
+PVX_HD int adj_task_cells(const GridInfo& g, const MeshBox& mb, const double* rec, uint64_t task) {
+    int64_t t;
+    int h, a, p;
+    unpack_adj(task, t, h, a, p);
+    double lo[3], hi[3];
+    uint32_t shp;
+    adj_copies(rec, g, mb, lo, hi, shp);
+    const int b = a == 2 ? 0 : a + 1, c = a == 0 ? 2 : a - 1;
+    const double sa = copy_shift(mb, shp, h, a), sb = copy_shift(mb, shp, h, b), sc = copy_shift(mb, shp, h, c);
+    const int na = pick3(g.n, a);
+    const double xp = (p == na ? pick3(g.w1, a) : pick3(g.w0, a) + pick3(g.d, a) * (double)p) - sa;
+    const double w0b = pick3(g.w0, b), w0c = pick3(g.w0, c), ib = pick3(g.dinv, b), ic = pick3(g.dinv, c);
+    /* FP32 from here, with the hardware reciprocal: a cell more or less
+     * only moves the task to a neighbouring class */
+    float umin = 3e38f, umax = -3e38f, vmin = 3e38f, vmax = -3e38f;
+    for (int u = 0; u < 3; ++u)
+        for (int v = u + 1; v < 4; ++v) {
+            const double* X = rec + 3 * u;
+            const double* Y = rec + 3 * v;
+            if ((X[a] < xp) == (Y[a] < xp)) continue;
+            const float f = (float)(xp - X[a]) * frcp((float)(Y[a] - X[a]));
+            const float su = (float)((X[b] + sb - w0b) * ib) + f * (float)((Y[b] - X[b]) * ib);
+            const float sv = (float)((X[c] + sc - w0c) * ic) + f * (float)((Y[c] - X[c]) * ic);
+            umin = fminf(umin, su);
+            umax = fmaxf(umax, su);
+            vmin = fminf(vmin, sv);
+            vmax = fmaxf(vmax, sv);
+        }
+    if (umin > umax) return 0;
+    const int nb = pick3(g.n, b), nc = pick3(g.n, c);
+    const int j0 = umin > 0.f ? (int)floorf(umin) : 0, j1 = umax < (float)(nb - 1) ? (int)floorf(umax) : nb - 1;
+    const int k0 = vmin > 0.f ? (int)floorf(vmin) : 0, k1 = vmax < (float)(nc - 1) ? (int)floorf(vmax) : nc - 1;
+    return j1 < j0 || k1 < k0 ? 0 : (j1 - j0 + 1) * (k1 - k0 + 1);
+}
+
 /* the tet's FP64 inverse edge matrix: lambda_{1..3} = B (x - x0) */
 struct TetInv64 {
     double x0[3], B[3][3], vol;
@@ -506,16 +545,9 @@ PVX_HD bool adj_vel_task(const GridInfo& g, const MeshBox& mb, const double* rec
         for (int k = 0; k < 3; ++k) DV[a][k] = rec[12 + 3 * (k + 1) + a] - rec[12 + a];
     }
     const double mass = rec[24];
-    PlaneCarry<R> pc{false, false, {R(0), R(0), R(0)}};
-#pragma unroll 1
-    for (int k = S.k0; k < S.k1; ++k) {
-        bool hl, hh;
-        double lc, hc;
-        cell_planes(S, k, hl, lc, hh, hc);
-        Mom<R> M;
-        R O[3];
-        slab_moments_u<2>(P, S.C.G, 2, hl, lc, hh, hc, M, O, &pc);
-        if (!(M.m0 > R(0))) continue;
+    /* one cell's fragment (moments M about O) into out */
+    auto cell = [&](int k, const Mom<R>& M, const R* O) {
+        if (!(M.m0 > R(0))) return;
         const double* Wc = W + ADJ_NW * (((int64_t)g.n[1] * S.i + S.j) * g.n[2] + k);
         /* fragment mean q and E[q q^T] (about q = 0) */
         const double m0 = (double)M.m0, inv = 1.0 / m0;
@@ -542,14 +574,42 @@ PVX_HD bool adj_vel_task(const GridInfo& g, const MeshBox& mb, const double* rec
         for (int b = 0; b < 3; ++b)
             for (int c = 0; c < 3; ++c) Ws[b][c] = Wc[4 + 3 * b + c] + Wc[4 + 3 * c + b];
         const double f = mass * m0;
+        /* unrolled: the vertex index must be a constant, or El, Elq and out
+         * are indexed at run time and live in scratch memory */
+#pragma unroll
         for (int i = 0; i < 4; ++i) {
             /* E[lambda_i v_c] */
             double Ev[3];
+#pragma unroll
             for (int c = 0; c < 3; ++c)
                 Ev[c] = u0[c] * El[i] + DV[c][0] * Elq[i][0] + DV[c][1] * Elq[i][1] + DV[c][2] * Elq[i][2];
+#pragma unroll
             for (int b = 0; b < 3; ++b)
                 out[i][b] += f * (Wc[1 + b] * El[i] + Ws[b][0] * Ev[0] + Ws[b][1] * Ev[1] + Ws[b][2] * Ev[2]);
         }
+    };
+    /* a two-cell column takes both cells in its first step (as the forward
+     * does) and keeps the upper one's moments for the second; one call site
+     * of the cell's body, so the kernel stays small */
+    const bool two = two_cell_column(S);
+    PlaneCarry<R> pc{false, false, {R(0), R(0), R(0)}};
+    Mom<R> Mu;
+    R O[3];
+#pragma unroll 1
+    for (int k = S.k0; k < S.k1; ++k) {
+        Mom<R> M;
+        if (two) {
+            if (k == S.k0)
+                slab_moments_2<2>(P, S.C.G, 2, first_plane(S), M, Mu, O);
+            else
+                M = Mu;
+        } else {
+            bool hl, hh;
+            double lc, hc;
+            cell_planes(S, k, hl, lc, hh, hc);
+            slab_moments_u<2>(P, S.C.G, 2, hl, lc, hh, hc, M, O, &pc);
+        }
+        cell(k, M, O);
     }
     return true;
 }

@@ -271,25 +271,38 @@ PVX_HD void unpack_param(const uint64_t* prm, int v, double* xi) {
     xi[2] = (double)((w >> 16) & 255u) / PARAM_UNIT;
 }
 
+/* the element map's one-axis factors at xi: node (i, j, k) weighs
+ * F[0][i] F[1][j] F[2][k], linear (vpe 8: nodes at 0, 1) or quadratic (vpe
+ * 27: nodes at 0, 1/2, 1); returns the nodes per axis */
+PVX_HD int node_factors(int vpe, const double* xi, double (*F)[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const double t = xi[a];
+        if (vpe == 8) {
+            F[a][0] = 1.0 - t;
+            F[a][1] = t;
+            F[a][2] = 0.0;
+        } else {
+            F[a][0] = 2.0 * (t - 0.5) * (t - 1.0);
+            F[a][1] = 4.0 * t * (1.0 - t);
+            F[a][2] = 2.0 * t * (t - 0.5);
+        }
+    }
+    return vpe == 8 ? 2 : 3;
+}
+
+/* factor i of one axis's F (i in 0..2), without indexing at run time */
+PVX_HD double pick_factor(const double* F, int i) { return i == 0 ? F[0] : (i == 1 ? F[1] : F[2]); }
+
 /* the element map's weights at xi: trilinear over the corners 4i+2j+k
  * (vpe 8) or triquadratic over the nodes 9i+3j+k at 0, 1/2, 1 (vpe 27) */
 PVX_HD int node_weights(int vpe, const double* xi, double* w) {
-    if (vpe == 8) {
-        for (int n = 0; n < 8; ++n) {
-            const int i = n >> 2, j = (n >> 1) & 1, k = n & 1;
-            w[n] = (i ? xi[0] : 1.0 - xi[0]) * (j ? xi[1] : 1.0 - xi[1]) * (k ? xi[2] : 1.0 - xi[2]);
-        }
-        return 8;
+    double F[3][3];
+    const int q = node_factors(vpe, xi, F);
+    for (int n = 0; n < q * q * q; ++n) {
+        const int i = n / (q * q), j = (n / q) % q, k = n % q;
+        w[n] = pick_factor(F[0], i) * pick_factor(F[1], j) * pick_factor(F[2], k);
     }
-    double L[3][3];
-    for (int a = 0; a < 3; ++a) {
-        const double t = xi[a];
-        L[a][0] = 2.0 * (t - 0.5) * (t - 1.0);
-        L[a][1] = 4.0 * t * (1.0 - t);
-        L[a][2] = 2.0 * t * (t - 0.5);
-    }
-    for (int n = 0; n < 27; ++n) w[n] = L[0][n / 9] * L[1][(n / 3) % 3] * L[2][n % 3];
-    return 27;
+    return q * q * q;
 }
 
 /*
