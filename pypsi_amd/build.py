@@ -4,6 +4,8 @@ cross-compiles gfx950 without a GPU).
     libpsivox.so   HIP kernels + C ABI (include/psivox.h), hipcc --offload-arch=gfx950
     PSI<ext>.so    CPython module "PSI" (C), linked to libpsivox.so with an
                    $ORIGIN rpath so the pair travels together.
+    atomic_rate    measurement program: FP64 atomic throughput of the
+                   deposit pattern (csrc/atomic_rate.hip)
 
 python -m pypsi_amd.build   builds both.
 """
@@ -19,6 +21,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBPSIVOX = os.path.join(HERE, "libpsivox.so")
 EXT = os.path.join(HERE, "PSI" + sysconfig.get_config_var("EXT_SUFFIX"))
+ATOMIC_RATE = os.path.join(HERE, "atomic_rate")
 ARCH = os.environ.get("PSIVOX_ARCH", "gfx950")
 
 HIP_SOURCES = ["psivox.hip"]
@@ -58,9 +61,16 @@ def build_module() -> str:
     return EXT
 
 
+def build_probe() -> str:
+    flags = [f for f in HIPCC_FLAGS if f not in ("-fPIC", "-shared")]
+    _run(["hipcc"] + flags + ["-o", ATOMIC_RATE, os.path.join(CSRC, "atomic_rate.hip")])
+    return ATOMIC_RATE
+
+
 def build_all() -> None:
     build_hip()
     build_module()
+    build_probe()
 
 
 if __name__ == "__main__":
