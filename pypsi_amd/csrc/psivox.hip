@@ -999,12 +999,14 @@ __global__ __launch_bounds__(FBLOCK, ORDER == 0 && sizeof(R) == 4 ? 3 : 2) void 
 /* one thread per tet of [t0, t1): how its residual is deposited.  A small
  * residual goes whole to the largest fragment: tetf[t] = the residual and
  * tetb[t] = that task; otherwise it is spread: tetf[t] = (target - sum) /
- * sum and tetb[t] = -1 (sums: the tasks' FP64 fragment-mass sums) */
+ * sum and tetb[t] = -1 (sums: the tasks' FP64 fragment-mass sums), and a
+ * nonzero spread lists t - t0 in slist (count in slist[0], entries from
+ * slist[WLIST_HEAD]; the count starts at 0) */
 __global__ __launch_bounds__(256) void k_resid_factor(const double* __restrict__ tets, int64_t t0, int64_t t1,
                                                       const int64_t* __restrict__ colstart, int64_t base,
                                                       const double* __restrict__ tsum,
                                                       const uint64_t* __restrict__ tkey, double* __restrict__ tetf,
-                                                      int64_t* __restrict__ tetb) {
+                                                      int64_t* __restrict__ tetb, uint32_t* __restrict__ slist) {
     const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= t1) return;
     const int64_t c0 = colstart[t] - base, c1 = colstart[t + 1] - base;
@@ -1055,13 +1057,52 @@ __global__ __launch_bounds__(256) void k_resid_factor(const double* __restrict__
     float m0b = 0.f;
     const uint32_t mbits = (uint32_t)(kb >> 32);
     memcpy(&m0b, &mbits, 4);
+    bool spread = false;
     if (cb >= 0 && fabs(resid) <= RESID_WHOLE * (double)m0b) {
         tetf[t] = resid;
         tetb[t] = cb + base;
     } else {
-        tetf[t] = residual_factor(r, sum);
+        const double f = residual_factor(r, sum);
+        tetf[t] = f;
         tetb[t] = -1;
+        spread = f != 0.0;
     }
+    /* one atomic per wavefront for its spread tets' list slots */
+    const uint64_t bal = __ballot(spread);
+    if (bal) {
+        const int lane = (int)(threadIdx.x & 63u), lead = __ffsll((unsigned long long)bal) - 1;
+        int b0 = 0;
+        if (lane == lead) b0 = (int)atomicAdd(slist, (unsigned)__popcll(bal));
+        b0 = __shfl(b0, lead);
+        if (spread) slist[WLIST_HEAD + b0 + __popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)(t - t0);
+    }
+}
+
+/* the largest fragment of task c from its key (cell offset, FP32 mass) and,
+ * for ORDER > 0, its moments and origin */
+template <int ORDER>
+__device__ BestFrag best_frag(uint64_t kb, const float* __restrict__ mom, int64_t c) {
+    BestFrag B;
+    B.kl = (int)(kb & 0xffffffffull);
+    const uint32_t mbits = (uint32_t)(kb >> 32);
+    memcpy(&B.M.m0, &mbits, 4);
+    if (ORDER > 0) {
+        const float4* m = (const float4*)(mom + 16 * c);
+        const float4 a = m[0], b = m[1], cc = m[2], d = m[3];
+        B.M.s1[0] = a.y;
+        B.M.s1[1] = a.z;
+        B.M.s1[2] = a.w;
+        B.M.s2[0] = b.x;
+        B.M.s2[1] = b.y;
+        B.M.s2[2] = b.z;
+        B.M.s2[3] = b.w;
+        B.M.s2[4] = cc.x;
+        B.M.s2[5] = cc.y;
+        B.O[0] = cc.z;
+        B.O[1] = cc.w;
+        B.O[2] = d.x;
+    }
+    return B;
 }
 
 /* A deposit staged in LDS when its base is not known beforehand: the
@@ -1075,77 +1116,42 @@ struct RowSink {
     }
 };
 
-/* The residual deposits, staged.  WHOLE: one thread per tet of [t0, t1)
- * whose residual goes whole to its largest fragment (tetb >= 0), for that
- * fragment's task; otherwise one thread per task of a tet whose residual
- * is spread, for the task's share.  The lanes that deposit stage their
- * rows (nacc values) in LDS in rank order; the wavefront then issues the
- * staged rows slot by slot, so that neighbouring lanes add into the same
- * cell's consecutive slots, and the issue loop covers only the rows
- * staged. */
-template <int ORDER, bool BATCH, bool WHOLE>
-__global__ __launch_bounds__(256) void k_resid_dep(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
-                                                   int64_t ntask, int64_t base, int64_t t0, int64_t t1, GridInfo g0,
-                                                   MeshBox mb0, double* __restrict__ acc0, const BatchGeo* table,
-                                                   TaskOut out, const double* __restrict__ tetf,
-                                                   const int64_t* __restrict__ tetb) {
+/* The whole residuals, staged: one thread per tet of [t0, t1) whose
+ * residual goes whole to its largest fragment (tetb >= 0), for that
+ * fragment's task.  The lanes that deposit stage their rows (nacc values)
+ * in LDS in rank order; the wavefront then issues the staged rows slot by
+ * slot, so that neighbouring lanes add into the same cell's consecutive
+ * slots, and the issue loop covers only the rows staged. */
+template <int ORDER, bool BATCH>
+__global__ __launch_bounds__(256) void k_resid_whole(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
+                                                     int64_t base, int64_t t0, int64_t t1, GridInfo g0, MeshBox mb0,
+                                                     double* __restrict__ acc0, const BatchGeo* table, TaskOut out,
+                                                     const double* __restrict__ tetf,
+                                                     const int64_t* __restrict__ tetb) {
     extern __shared__ double rstage[]; /* per wavefront: FBLOCK rows of nacc values, FBLOCK targets */
     const int na = g0.nacc;
     const int lane = threadIdx.x & (FBLOCK - 1), wv = threadIdx.x / FBLOCK;
     double* rows = rstage + (size_t)wv * FBLOCK * (na + 1);
     double** dst = (double**)(rows + FBLOCK * na);
     double* target = nullptr;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool will = false;
-    int64_t c = 0, t = 0;
+    int64_t c = 0;
     uint64_t kb = 0, w = 0;
     double weight = 0.0;
-    if (WHOLE) {
-        t = t0 + gid;
-        const int64_t cb = t < t1 ? tetb[t] : -1;
-        if (cb >= 0) {
-            c = cb - base;
-            kb = out.key[c];
-            w = tasks[c];
-            weight = tetf[t];
-            will = kb != 0ull && weight != 0.0;
-        }
-    } else {
-        c = gid;
-        kb = c < ntask ? out.key[c] : 0ull;
-        if (kb != 0ull) {
-            w = tasks[c];
-            t = (int64_t)(w & 0xffffffffull);
-            if (tetb[t] < 0) {
-                weight = tetf[t] * out.sum[c];
-                will = weight != 0.0;
-            }
-        }
+    const int64_t cb = t < t1 ? tetb[t] : -1;
+    if (cb >= 0) {
+        c = cb - base;
+        kb = out.key[c];
+        w = tasks[c];
+        weight = tetf[t];
+        will = kb != 0ull && weight != 0.0;
     }
     const uint64_t active = __ballot(will);
     const int nact = __popcll(active);
     const int rank = __popcll(active & ((1ull << lane) - 1ull));
     if (will) {
-        BestFrag B;
-        B.kl = (int)(kb & 0xffffffffull);
-        const uint32_t mbits = (uint32_t)(kb >> 32);
-        memcpy(&B.M.m0, &mbits, 4);
-        if (ORDER > 0) {
-            const float4* m = (const float4*)(out.mom + 16 * c);
-            const float4 a = m[0], b = m[1], cc = m[2], d = m[3];
-            B.M.s1[0] = a.y;
-            B.M.s1[1] = a.z;
-            B.M.s1[2] = a.w;
-            B.M.s2[0] = b.x;
-            B.M.s2[1] = b.y;
-            B.M.s2[2] = b.z;
-            B.M.s2[3] = b.w;
-            B.M.s2[4] = cc.x;
-            B.M.s2[5] = cc.y;
-            B.O[0] = cc.z;
-            B.O[1] = cc.w;
-            B.O[2] = d.x;
-        }
+        const BestFrag B = best_frag<ORDER>(kb, out.mom, c);
         const double* r = tets + (int64_t)TREC * t;
         with_mesh<BATCH>(r, g0, mb0, acc0, table, [&](const GridInfo& g, const MeshBox& mb, double* acc, int) {
             RowSink rs{rows + rank * na, -1};
@@ -1166,6 +1172,41 @@ __global__ __launch_bounds__(256) void k_resid_dep(const double* __restrict__ te
         if (j >= na) {
             j -= na;
             ++f;
+        }
+    }
+}
+
+/* The spread residuals: the tets k_resid_factor listed, one wavefront per
+ * listed tet (grid-stride), its lanes over the tet's tasks, each lane
+ * depositing its task's share (weight = factor x the task's mass sum) with
+ * direct atomics.  Few tets spread their residual (one in 238,000 on the
+ * Zel'dovich 128^3 -> 256^3 workload, host model of the lanes), and the
+ * list spares reading every task's key to find them. */
+template <int ORDER, bool BATCH>
+__global__ __launch_bounds__(256) void k_resid_spread(const double* __restrict__ tets, const uint64_t* __restrict__ tasks,
+                                                      int64_t base, int64_t t0, const int64_t* __restrict__ colstart,
+                                                      GridInfo g0, MeshBox mb0, double* __restrict__ acc0,
+                                                      const BatchGeo* table, TaskOut out,
+                                                      const double* __restrict__ tetf,
+                                                      const uint32_t* __restrict__ slist) {
+    const int64_t n = (int64_t)slist[0];
+    const int lane = (int)(threadIdx.x & (FBLOCK - 1));
+    const int64_t nwv = (int64_t)gridDim.x * (blockDim.x / FBLOCK);
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / FBLOCK; i < n; i += nwv) {
+        const int64_t t = t0 + (int64_t)slist[WLIST_HEAD + i];
+        const int64_t c0 = colstart[t] - base, c1 = colstart[t + 1] - base;
+        const double f = tetf[t];
+        const double* r = tets + (int64_t)TREC * t;
+        for (int64_t c = c0 + lane; c < c1; c += FBLOCK) {
+            const uint64_t kb = out.key[c];
+            const double weight = f * out.sum[c];
+            if (kb == 0ull || weight == 0.0) continue;
+            const BestFrag B = best_frag<ORDER>(kb, out.mom, c);
+            const uint64_t w = tasks[c];
+            with_mesh<BATCH>(r, g0, mb0, acc0, table, [&](const GridInfo& g, const MeshBox& mb, double* acc, int) {
+                AtomicSink as{acc};
+                task_residual<ORDER>(as, g, mb, r, w, B, weight);
+            });
         }
     }
 }
@@ -1262,38 +1303,40 @@ static int launch_cols(const double* tets, const uint64_t* tasks, int64_t ntask,
 }
 
 template <int ORDER>
-static void launch_resid_dep_o(const double* tets, const uint64_t* tasks, int64_t ntask, int64_t base, int64_t t0,
-                               int64_t t1, const PassGeo& G, TaskOut out, const double* tetf, const int64_t* tetb,
-                               hipStream_t s) {
-    const unsigned nbt = (unsigned)((t1 - t0 + 255) / 256), nb = (unsigned)((ntask + 255) / 256);
+static void launch_resid_dep_o(const double* tets, const uint64_t* tasks, int64_t base, int64_t t0, int64_t t1,
+                               const PassGeo& G, TaskOut out, const int64_t* colstart, const double* tetf,
+                               const int64_t* tetb, const uint32_t* slist, hipStream_t s) {
+    const unsigned nbt = (unsigned)((t1 - t0 + 255) / 256);
     const size_t sb = (size_t)(256 / FBLOCK) * stage_bytes(G.g, FBLOCK);
-    if (nbt == 0u || nb == 0u) return;
+    if (nbt == 0u) return;
     if (G.table) {
-        k_resid_dep<ORDER, true, true><<<nbt, 256, sb, s>>>(tets, tasks, ntask, base, t0, t1, G.g, G.mb, G.acc,
-                                                            G.table, out, tetf, tetb);
-        k_resid_dep<ORDER, true, false><<<nb, 256, sb, s>>>(tets, tasks, ntask, base, t0, t1, G.g, G.mb, G.acc,
-                                                            G.table, out, tetf, tetb);
+        k_resid_whole<ORDER, true><<<nbt, 256, sb, s>>>(tets, tasks, base, t0, t1, G.g, G.mb, G.acc, G.table, out,
+                                                         tetf, tetb);
+        k_resid_spread<ORDER, true><<<nbt, 256, 0, s>>>(tets, tasks, base, t0, colstart, G.g, G.mb, G.acc, G.table,
+                                                         out, tetf, slist);
     } else {
-        k_resid_dep<ORDER, false, true><<<nbt, 256, sb, s>>>(tets, tasks, ntask, base, t0, t1, G.g, G.mb, G.acc,
-                                                             nullptr, out, tetf, tetb);
-        k_resid_dep<ORDER, false, false><<<nb, 256, sb, s>>>(tets, tasks, ntask, base, t0, t1, G.g, G.mb, G.acc,
-                                                             nullptr, out, tetf, tetb);
+        k_resid_whole<ORDER, false><<<nbt, 256, sb, s>>>(tets, tasks, base, t0, t1, G.g, G.mb, G.acc, nullptr, out,
+                                                          tetf, tetb);
+        k_resid_spread<ORDER, false><<<nbt, 256, 0, s>>>(tets, tasks, base, t0, colstart, G.g, G.mb, G.acc, nullptr,
+                                                          out, tetf, slist);
     }
 }
 
 /* the residual pass over the tets [t0, t1) and their ntask tasks; tetf,
- * tetb: one double and one int64 per tet of the batch */
+ * tetb: one double and one int64 per tet of the batch; slist: WLIST_HEAD +
+ * t1 - t0 words (the spread tets' list) */
 static int launch_residual(const double* tets, int64_t t0, int64_t t1, const PassGeo& G, const int64_t* colstart,
-                           int64_t base, const uint64_t* tasks, int64_t ntask, TaskOut out, double* tetf,
-                           int64_t* tetb, hipStream_t s) {
+                           int64_t base, const uint64_t* tasks, TaskOut out, double* tetf, int64_t* tetb,
+                           uint32_t* slist, hipStream_t s) {
+    k_fill_u32<<<1, 64, 0, s>>>(slist, 1, 0u);
     k_resid_factor<<<(unsigned)((t1 - t0 + 255) / 256), 256, 0, s>>>(tets, t0, t1, colstart, base, out.sum, out.key,
-                                                                     tetf, tetb);
+                                                                     tetf, tetb, slist);
     if (G.g.order == 0)
-        launch_resid_dep_o<0>(tets, tasks, ntask, base, t0, t1, G, out, tetf, tetb, s);
+        launch_resid_dep_o<0>(tets, tasks, base, t0, t1, G, out, colstart, tetf, tetb, slist, s);
     else if (G.g.order == 1)
-        launch_resid_dep_o<1>(tets, tasks, ntask, base, t0, t1, G, out, tetf, tetb, s);
+        launch_resid_dep_o<1>(tets, tasks, base, t0, t1, G, out, colstart, tetf, tetb, slist, s);
     else
-        launch_resid_dep_o<2>(tets, tasks, ntask, base, t0, t1, G, out, tetf, tetb, s);
+        launch_resid_dep_o<2>(tets, tasks, base, t0, t1, G, out, colstart, tetf, tetb, slist, s);
     return (int)hipGetLastError();
 }
 
@@ -1543,6 +1586,7 @@ struct Workspace {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf wtets;                           /* the wide tets' records for the FP64 column pass */
     DevBuf wlist;                           /* the wide tets' indices, from the count pass */
+    DevBuf slist;                           /* the tets whose residual is spread, from k_resid_factor */
     DevBuf asorted;                         /* the adjoint's plane tasks in processing order */
     bool compact_wide = true;               /* the FP64 pass on a copy of the wide tets' records */
     bool timing = true;                     /* kernel_ms from events (off: no events, kernel_ms = 0) */
@@ -2354,9 +2398,11 @@ int run_one(Workspace& W, const psivox_mesh* mesh, psivox_grid* grid, const psiv
                                 HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
                                 kms += ms;
                             }
-                            if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
-                            rc2 = launch_residual(ptets, a0, a1, G, colstart, base, tasks, nsub, out,
-                                                  (double*)W.colcnt.p, (int64_t*)W.tetb.p, s);
+                            if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev)) ||
+                                (rc2 = W.slist.get((size_t)(WLIST_HEAD + nt) * sizeof(uint32_t), dev)))
+                                return rc2;
+                            rc2 = launch_residual(ptets, a0, a1, G, colstart, base, tasks, out, (double*)W.colcnt.p,
+                                                  (int64_t*)W.tetb.p, (uint32_t*)W.slist.p, s);
                             if (rc2)
                                 return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
                                                hipGetErrorString((hipError_t)rc2));
@@ -2853,9 +2899,11 @@ int run_batched(Workspace& W, int nmesh, const psivox_mesh* meshes, psivox_grid*
                         return set_err(PSIVOX_E_HIP, "column kernel launch failed: %s",
                                        hipGetErrorString((hipError_t)rc2));
                     HIPCHK(hipEventRecord(W.ev1, s));
-                    if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev))) return rc2;
-                    rc2 = launch_residual(ptets, a0, a1, PG, colstart, base, tasks, nsub, out, (double*)W.colcnt.p,
-                                          (int64_t*)W.tetb.p, s);
+                    if ((rc2 = W.tetb.get((size_t)nt * sizeof(int64_t), dev)) ||
+                        (rc2 = W.slist.get((size_t)(WLIST_HEAD + nt) * sizeof(uint32_t), dev)))
+                        return rc2;
+                    rc2 = launch_residual(ptets, a0, a1, PG, colstart, base, tasks, out, (double*)W.colcnt.p,
+                                          (int64_t*)W.tetb.p, (uint32_t*)W.slist.p, s);
                     if (rc2)
                         return set_err(PSIVOX_E_HIP, "residual kernel launch failed: %s",
                                        hipGetErrorString((hipError_t)rc2));
@@ -3013,10 +3061,10 @@ int run_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* o
     if ((rc = W.counters.get(64, dev)) || (rc = W.connchk.get(sizeof(unsigned), dev)) ||
         (rc = W.cut.get(4 * sizeof(int64_t), dev)))
         return rc;
-    /* element batch: tet records, per-tet counts, starts and residual
-     * slots in up to two fifths of the rest */
+    /* element batch: tet records, per-tet counts, starts, residual slots
+     * and the spread list in up to two fifths of the rest */
     const int per = mesh->elemtype == PSIVOX_SIMPLEX ? 1 : 6;
-    const size_t per_tet = TREC * sizeof(double) + 3 * sizeof(int64_t);
+    const size_t per_tet = TREC * sizeof(double) + 3 * sizeof(int64_t) + sizeof(uint32_t);
     const int64_t nt_room = (int64_t)((A.size - A.off) * 2 / 5 / per_tet);
     int64_t ne_batch = std::max<int64_t>(1, nt_room / per);
     if (mesh->nelem > 0) ne_batch = std::min<int64_t>(ne_batch, mesh->nelem);
@@ -3026,7 +3074,8 @@ int run_noalloc(const psivox_mesh* mesh, psivox_grid* grid, const psivox_opts* o
     if ((rc = W.tets.get((size_t)nt * TREC * sizeof(double), dev)) ||
         (rc = W.colcnt.get((size_t)(nt + 1) * sizeof(int64_t), dev)) ||
         (rc = W.colstart.get((size_t)(nt + 1) * sizeof(int64_t), dev)) ||
-        (rc = W.tetb.get((size_t)nt * sizeof(int64_t), dev)) || (rc = W.scan_tmp.get(tb, dev)))
+        (rc = W.tetb.get((size_t)nt * sizeof(int64_t), dev)) ||
+        (rc = W.slist.get((size_t)(WLIST_HEAD + nt) * sizeof(uint32_t), dev)) || (rc = W.scan_tmp.get(tb, dev)))
         return rc;
     /* a chunk of column tasks: task words, mass sums, largest-fragment
      * keys and moments, and the ordering's scratch (order_bytes: the
